@@ -1,0 +1,136 @@
+"""ctypes binding of libavbd_hip.so (the C ABI declared in include/avbd.h).
+
+The library is the only compute path: when it is missing the binding raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libavbd_hip.so")
+
+AVBD_OK = 0
+AVBD_ERR_INVALID = -1
+AVBD_ERR_DEVICE = -2
+AVBD_ERR_DESTROYED = -3
+AVBD_ERR_CAPACITY = -4
+
+FLAG_MATCH_NEAREST = 1
+FLAG_FACE_BIAS = 2
+FLAG_REUSE_CONTACTS = 4
+FLAG_START_AT_REST = 8
+FLAG_MASS_PENALTY = 16
+FLAGS_DEFAULT = 31
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+i32p = ctypes.POINTER(ctypes.c_int32)
+f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("body_count", ctypes.c_uint32),
+        ("bodies", f32p),
+        ("joint_count", ctypes.c_uint32),
+        ("joints", f32p),
+        ("joint_info", i32p),
+        ("ignore_count", ctypes.c_uint32),
+        ("ignore_pairs", u32p),
+        ("filters", u32p),
+        ("dt", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("up", ctypes.c_float * 3),
+        ("alpha", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("beta_lin", ctypes.c_float),
+        ("beta_ang", ctypes.c_float),
+        ("iterations", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("color_cap", ctypes.c_uint32),
+        ("color_rounds", ctypes.c_uint32),
+        ("pair_capacity", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in
+                ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "joints", "bodies")]
+
+
+# name -> (restype, argtypes): every entry point of include/avbd.h
+SIGNATURES = {
+    "avbd_last_error": (ctypes.c_char_p, []),
+    "avbd_version": (ctypes.c_char_p, []),
+    "avbd_device_count": (ctypes.c_int, []),
+    "avbd_world_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+    "avbd_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.c_uint32, ctypes.POINTER(Options),
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "avbd_world_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "avbd_world_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "avbd_world_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "avbd_world_body_count": (ctypes.c_int, [ctypes.c_void_p, u32p]),
+    "avbd_world_joint_count": (ctypes.c_int, [ctypes.c_void_p, u32p]),
+    "avbd_world_read_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, f32p]),
+    "avbd_world_write_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, f32p]),
+    "avbd_world_read_selected": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, ctypes.c_int, f32p]),
+    "avbd_world_read_joints": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, f32p]),
+    "avbd_world_write_joints": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, f32p]),
+    "avbd_world_read_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Counters)]),
+    "avbd_world_read_pairs": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u32p]),
+    "avbd_world_read_contacts": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, u32p]),
+    "avbd_world_read_colors": (ctypes.c_int, [ctypes.c_void_p, u32p]),
+    "avbd_world_read_adjacency": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, ctypes.c_uint32, u32p]),
+    "avbd_world_read_refs": (ctypes.c_int, [ctypes.c_void_p, f32p]),
+    "avbd_world_read_steps": (ctypes.c_int, [ctypes.c_void_p, u32p]),
+    "avbd_world_write_state": (ctypes.c_int, [ctypes.c_void_p, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
+    "avbd_world_set_coloring": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "avbd_world_adapt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),
+    "avbd_world_device_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+}
+
+_lib = None
+
+
+class AvbdError(RuntimeError):
+    """An error reported by the library (the reference throws Error with a message)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def load():
+    """The loaded library; raises when it has not been built (there is no other compute path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != AVBD_OK:
+        msg = load().avbd_last_error().decode() or f"avbd error {code}"
+        raise AvbdError(code, msg)
+    return code
+
+
+def ptr(array, ctype):
+    """Pointer to a contiguous numpy array (None for None)."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,376 @@
+// extern "C" entry points of libavbd_hip.so (declared in include/avbd.h) over the host engine.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/avbd.h"
+#include "engine.h"
+
+using namespace avbd;
+
+struct avbd_world {
+  std::unique_ptr<Engine> engine;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+template <class F> int guarded(F &&f) {
+  try {
+    g_error.clear();
+    return f();
+  } catch (const std::invalid_argument &e) {
+    return fail(AVBD_ERR_INVALID, e.what());
+  } catch (const std::out_of_range &e) {
+    return fail(AVBD_ERR_INVALID, e.what());
+  } catch (const std::length_error &e) {
+    return fail(AVBD_ERR_CAPACITY, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(AVBD_ERR_DEVICE, "out of memory");
+  } catch (const std::exception &e) {
+    return fail(AVBD_ERR_DEVICE, e.what());
+  }
+}
+
+HostWorld toHost(const AvbdSceneDesc &s) {
+  if (s.body_count && !s.bodies) throw std::invalid_argument("bodies is NULL");
+  if (s.joint_count && (!s.joints || !s.joint_info)) throw std::invalid_argument("joints or joint_info is NULL");
+  if (s.ignore_count && !s.ignore_pairs) throw std::invalid_argument("ignore_pairs is NULL");
+  if (!(s.dt > 0.f)) throw std::invalid_argument("timeStep must be positive");
+  if (s.iterations == 0) throw std::invalid_argument("iterations must be a positive integer");
+  float upLen = std::sqrt(s.up[0] * s.up[0] + s.up[1] * s.up[1] + s.up[2] * s.up[2]);
+  if (std::fabs(upLen - 1.f) > 1e-5f) throw std::invalid_argument("up must be a unit vector");
+  HostWorld w;
+  w.bodies.resize(s.body_count);
+  if (s.body_count) std::memcpy(w.bodies.data(), s.bodies, size_t(s.body_count) * sizeof(Body));
+  w.joints.resize(s.joint_count);
+  w.info.resize(s.joint_count);
+  if (s.joint_count) {
+    std::memcpy(w.joints.data(), s.joints, size_t(s.joint_count) * sizeof(Joint));
+    std::memcpy(w.info.data(), s.joint_info, size_t(s.joint_count) * sizeof(int4));
+  }
+  for (uint32_t k = 0; k < s.ignore_count; k++) w.ignorePairs.emplace_back(s.ignore_pairs[2 * k], s.ignore_pairs[2 * k + 1]);
+  if (s.filters) {
+    w.filters.resize(s.body_count);
+    for (uint32_t i = 0; i < s.body_count; i++) w.filters[i] = make_uint2(s.filters[2 * i], s.filters[2 * i + 1]);
+  }
+  WorldParams p{};
+  p.dt = s.dt;
+  p.gravity = s.gravity;
+  p.betaLin = s.beta_lin;
+  p.betaAng = s.beta_ang;
+  p.gamma = s.gamma;
+  p.alpha = s.alpha;
+  p.up[0] = s.up[0];
+  p.up[1] = s.up[1];
+  p.up[2] = s.up[2];
+  p.flags = s.flags;
+  p.step = 0;
+  w.params = p;
+  w.iterations = s.iterations;
+  return w;
+}
+
+EngineOptions toOptions(const AvbdOptions *o) {
+  EngineOptions e;
+  if (o) {
+    if (o->color_cap) e.colorCap = o->color_cap;
+    if (o->color_rounds) e.colorRounds = o->color_rounds;
+    e.pairCapacity = o->pair_capacity;
+    e.useGraph = o->use_graph != 0;
+    e.device = o->device;
+  }
+  if (e.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");
+  return e;
+}
+
+#define CHECK_WORLD(w)                                                              \
+  if (!(w) || !(w)->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed")
+
+}  // namespace
+
+extern "C" {
+
+const char *avbd_last_error(void) { return g_error.c_str(); }
+const char *avbd_version(void) { return "avbd-mi355x 0.1.0 (gfx950)"; }
+
+int avbd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int avbd_batch_create(const AvbdSceneDesc *scenes, uint32_t n, const AvbdOptions *options, avbd_world **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    *out = nullptr;
+    if (!scenes || n == 0) throw std::invalid_argument("a batch needs at least one scene");
+    int devices = avbd_device_count();
+    if (devices <= 0) return fail(AVBD_ERR_DEVICE, "no HIP device is available");
+    std::vector<HostWorld> worlds;
+    worlds.reserve(n);
+    for (uint32_t k = 0; k < n; k++) worlds.push_back(toHost(scenes[k]));
+    auto w = std::make_unique<avbd_world>();
+    w->engine = std::make_unique<Engine>(worlds, toOptions(options));
+    *out = w.release();
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_create(const AvbdSceneDesc *scene, const AvbdOptions *options, avbd_world **out) {
+  return avbd_batch_create(scene, 1, options, out);
+}
+
+int avbd_world_destroy(avbd_world *world) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine.reset();
+    delete world;
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_step(avbd_world *world, uint32_t n) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->step(n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_synchronize(avbd_world *world) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->synchronize();
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_body_count(const avbd_world *world, uint32_t *out) {
+  if (!world || !world->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed");
+  if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
+  *out = world->engine->bodyCount();
+  return AVBD_OK;
+}
+
+int avbd_world_joint_count(const avbd_world *world, uint32_t *out) {
+  if (!world || !world->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed");
+  if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
+  *out = world->engine->jointCount();
+  return AVBD_OK;
+}
+
+int avbd_world_read_bodies(avbd_world *world, uint32_t first, uint32_t count, float *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out && count) throw std::invalid_argument("out is NULL");
+    world->engine->readBodies(first, count, reinterpret_cast<Body *>(out));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_write_bodies(avbd_world *world, uint32_t first, uint32_t count, const float *in) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!in && count) throw std::invalid_argument("in is NULL");
+    world->engine->writeBodies(first, count, reinterpret_cast<const Body *>(in));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_selected(avbd_world *world, const uint32_t *indices, uint32_t n, int poses_only, float *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (n && (!indices || !out)) throw std::invalid_argument("indices or out is NULL");
+    Engine &e = *world->engine;
+    for (uint32_t k = 0; k < n; k++)
+      if (indices[k] >= e.bodyCount()) throw std::out_of_range("body index out of range");
+    std::vector<Body> all(e.bodyCount());
+    e.readBodies(0, e.bodyCount(), all.data());
+    for (uint32_t k = 0; k < n; k++) {
+      const Body &b = all[indices[k]];
+      if (poses_only) {
+        std::memcpy(out + 8 * size_t(k), &b.pos, 16);
+        std::memcpy(out + 8 * size_t(k) + 4, &b.rot, 16);
+      } else {
+        std::memcpy(out + 40 * size_t(k), &b, sizeof(Body));
+      }
+    }
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_joints(avbd_world *world, uint32_t first, uint32_t count, float *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out && count) throw std::invalid_argument("out is NULL");
+    world->engine->readJoints(first, count, reinterpret_cast<Joint *>(out));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_write_joints(avbd_world *world, uint32_t first, uint32_t count, const float *in) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!in && count) throw std::invalid_argument("in is NULL");
+    world->engine->writeJoints(first, count, reinterpret_cast<const Joint *>(in));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_counters(avbd_world *world, AvbdCounters *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    Counters c = world->engine->readCounters();
+    out->pairs = c.pairs;
+    out->contacts = c.contacts;
+    out->manifolds = c.manifolds;
+    out->overflow = ((c.overflow & 1u) ? 1u : 0u) | ((c.overflow & 2u) ? 2u : 0u) | ((c.overflow & 4u) ? 4u : 0u);
+    out->clashes = c.clashes;
+    out->colors = c.colors;
+    out->joints = world->engine->jointCount();
+    out->bodies = world->engine->bodyCount();
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_pairs(avbd_world *world, uint32_t *out, uint32_t capacity, uint32_t *count) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    std::vector<uint2> pairs = world->engine->readPairs();
+    if (count) *count = uint32_t(pairs.size());
+    if (out) {
+      uint32_t n = std::min<uint32_t>(capacity, uint32_t(pairs.size()));
+      std::memcpy(out, pairs.data(), size_t(n) * 8);
+    }
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_contacts(avbd_world *world, uint32_t *manifolds, uint32_t manifold_capacity, uint32_t *manifold_count,
+                             uint32_t *contacts, uint32_t contact_capacity, uint32_t *contact_count) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    std::vector<Manifold> m;
+    std::vector<Contact> c;
+    world->engine->readContacts(m, c);
+    if (manifold_count) *manifold_count = uint32_t(m.size());
+    if (contact_count) *contact_count = uint32_t(c.size());
+    if (manifolds) std::memcpy(manifolds, m.data(), std::min<size_t>(manifold_capacity, m.size()) * sizeof(Manifold));
+    if (contacts) std::memcpy(contacts, c.data(), std::min<size_t>(contact_capacity, c.size()) * sizeof(Contact));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_colors(avbd_world *world, uint32_t *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    std::vector<uint32_t> c = world->engine->readColors();
+    std::memcpy(out, c.data(), c.size() * 4);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_adjacency(avbd_world *world, uint32_t *start, uint32_t *list, uint32_t capacity, uint32_t *count) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    std::vector<uint32_t> s, l;
+    world->engine->readAdjacency(s, l);
+    if (count) *count = uint32_t(l.size());
+    if (start) std::memcpy(start, s.data(), s.size() * 4);
+    if (list) std::memcpy(list, l.data(), std::min<size_t>(capacity, l.size()) * 4);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_refs(avbd_world *world, float *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    std::vector<float4> r = world->engine->readRefs();
+    std::memcpy(out, r.data(), r.size() * sizeof(float4));
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_steps(avbd_world *world, uint32_t *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    std::vector<WorldParams> p = world->engine->readWorldParams();
+    for (size_t k = 0; k < p.size(); k++) out[k] = p[k].step;
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_write_state(avbd_world *world, const float *refs, const uint32_t *colors, uint32_t manifold_count,
+                           const uint32_t *manifolds, uint32_t contact_count, const uint32_t *contacts,
+                           const uint32_t *steps) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if ((manifold_count && !manifolds) || (contact_count && !contacts))
+      throw std::invalid_argument("manifolds or contacts is NULL");
+    std::vector<Manifold> m(manifold_count);
+    std::vector<Contact> c(contact_count);
+    if (manifold_count) std::memcpy(m.data(), manifolds, size_t(manifold_count) * sizeof(Manifold));
+    if (contact_count) std::memcpy(c.data(), contacts, size_t(contact_count) * sizeof(Contact));
+    uint32_t n = world->engine->bodyCount();
+    for (const Manifold &mf : m)
+      if (mf.ids.x >= n || mf.ids.y >= n || mf.ids.x <= mf.ids.y || (mf.ids.w & 15u) > 8u)
+        throw std::invalid_argument("manifold records must name bodies a > b with at most 8 contacts");
+    world->engine->writeState(reinterpret_cast<const float4 *>(refs), colors, m, c, steps);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_coloring(avbd_world *world, uint32_t color_cap, uint32_t color_rounds) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (color_cap == 0 || color_cap > 64) throw std::invalid_argument("color_cap must be in 1..64");
+    world->engine->setColorCap(color_cap);
+    if (color_rounds) world->engine->setColorRounds(color_rounds);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_adapt(avbd_world *world, int *changed) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    bool c = world->engine->adapt();
+    if (changed) *changed = c ? 1 : 0;
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_profile_step(avbd_world *world, float *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    PhaseTimes p = world->engine->profileStep();
+    const float v[10] = {p.total, p.collision, p.adjacency, p.coloring, p.warmStart, p.primal, p.dual, p.velocities,
+                         float(p.primalLaunches), float(p.dualLaunches)};
+    std::memcpy(out, v, sizeof v);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_device_bytes(const avbd_world *world, uint64_t *out) {
+  if (!world || !world->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed");
+  if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
+  *out = world->engine->deviceBytes();
+  return AVBD_OK;
+}
+
+}  // extern "C"

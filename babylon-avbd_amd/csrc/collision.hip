@@ -1,0 +1,472 @@
+// Collision phase of the AVBD 3D step on gfx950: frame bookkeeping, reference poses for contact reuse, the uniform-grid
+// broadphase with deterministic pair emission, the narrowphase with feature-keyed warm starts, and manifold compaction.
+// Semantics: reference/three-avbd/src/avbd3d/gpu/wgsl-collision.ts (beginFrame 65-76, updateRefs 32-47, grid 86-110,
+// findPairs/testPair 196-257, hash 278-313, reuseContacts 650-678, narrowphase 682-808).
+//
+// Design for MI355X: pair emission is count → scan → write (no atomic append), each body's pairs sorted by partner, so
+// the pair list, and every manifold and contact slot derived from it, is the same every run. The narrowphase writes a
+// pair's manifold and its contacts into fixed slots (contacts at 8 × pair index), and a scan compacts the manifolds.
+#include "engine_types.h"
+#include "kernels.h"
+#include "narrowphase.h"
+
+namespace avbd {
+
+// --- frame -----------------------------------------------------------------------------------------------------------
+
+__global__ void k_begin_frame(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < w.nWorlds) w.worlds[i].step += 1;
+  if (i == 0) {
+    uint32_t *c = w.counters;
+    c[C_PREV_MANIFOLDS] = c[C_MANIFOLDS];
+    c[C_PREV_CONTACTS] = c[C_CONTACTS];
+    c[C_MANIFOLDS] = 0;
+    c[C_CONTACTS] = 0;
+    c[C_PAIRS] = 0;
+    c[C_OVERFLOW] = 0;
+    c[C_CLASHES] = 0;
+    c[C_NUM_COLORS] = 0;
+    c[C_CANDIDATES] = 0;
+  }
+}
+
+__global__ void k_update_refs(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  const WorldParams &p = w.worlds[w.bodyWorld[i]];
+  if (!(p.flags & FLAG_REUSE_CONTACTS)) return;
+  f3 pos = xyz(w.bodies[i].pos);
+  float4 rot = w.bodies[i].rot;
+  float4 lp = w.refs[2 * i];
+  float4 lr = w.refs[2 * i + 1];
+  bool moved = (lr.x * lr.x + lr.y * lr.y + lr.z * lr.z + lr.w * lr.w) == 0.f ||
+               length(sub(pos, xyz(lp))) > REUSE_LIN_TOL || length(qsub(rot, lr)) > REUSE_ANG_TOL;
+  if (moved) {
+    w.refs[2 * i] = mk4(pos, 0.f);
+    w.refs[2 * i + 1] = rot;
+    w.bodies[i].inertialPos.w = bitsf(p.step);
+  }
+}
+
+// --- last step's manifolds by pair (wgsl-collision.ts:278-313) --------------------------------------------------------
+
+AVBD_D uint32_t pairHash(uint32_t a, uint32_t b) { return hash32((a * 0x9e3779b1u) ^ hash32(b)); }
+
+__global__ void k_hash_clear(DeviceWorld w) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h <= w.hashMask; h += gridDim.x * blockDim.x) w.table[h] = 0;
+}
+
+__global__ void k_hash_insert(DeviceWorld w) {
+  uint32_t count = w.counters[C_PREV_MANIFOLDS];
+  for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < count; m += gridDim.x * blockDim.x) {
+    uint4 ids = w.prevManifolds[m].ids;
+    uint32_t h = pairHash(ids.x, ids.y) & w.hashMask;
+    uint32_t value = m + 1;
+    for (uint32_t probe = 0; probe <= w.hashMask; probe++) {
+      value = atomicExch(&w.table[h], value);
+      if (value == 0) break;
+      h = (h + 1) & w.hashMask;
+    }
+  }
+}
+
+AVBD_D int hashFind(const DeviceWorld &w, uint32_t a, uint32_t b) {
+  uint32_t h = pairHash(a, b) & w.hashMask;
+  for (uint32_t probe = 0; probe <= w.hashMask; probe++) {
+    uint32_t v = __atomic_load_n(&w.table[h], __ATOMIC_RELAXED);
+    if (v == 0) return -1;
+    uint4 ids = w.prevManifolds[v - 1].ids;
+    if (ids.x == a && ids.y == b) return int(v - 1);
+    h = (h + 1) & w.hashMask;
+  }
+  return -1;
+}
+
+// --- broadphase ------------------------------------------------------------------------------------------------------
+
+AVBD_D bool isLarge(const DeviceWorld &w, uint32_t i) { return w.bodies[i].moment.w > w.worlds[w.bodyWorld[i]].maxSmall; }
+
+AVBD_D int4 cellCoords(const DeviceWorld &w, uint32_t i) {
+  float4 p = w.bodies[i].pos;
+  return make_int4(int(floorf(p.x / w.cellSize)), int(floorf(p.y / w.cellSize)), int(floorf(p.z / w.cellSize)),
+                   int(w.bodyWorld[i]));
+}
+
+AVBD_D uint32_t cellHash(const DeviceWorld &w, int4 c) {
+  return ((uint32_t(c.x) * 73856093u) ^ (uint32_t(c.y) * 19349663u) ^ (uint32_t(c.z) * 83492791u) ^
+          (uint32_t(c.w) * 2654435761u)) &
+         w.tableMask;
+}
+
+__global__ void k_grid_count(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  if (isLarge(w, i)) {
+    w.cellOf[i] = make_int4(0, 0, 0, -1);
+    return;
+  }
+  int4 c = cellCoords(w, i);
+  w.cellOf[i] = c;
+  atomicAdd(&w.gridCount[cellHash(w, c)], 1u);
+}
+
+__global__ void k_grid_scatter(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  int4 c = w.cellOf[i];
+  if (c.w < 0) return;
+  uint32_t h = cellHash(w, c);
+  uint32_t slot = w.gridCount[h] + atomicAdd(&w.gridCursor[h], 1u);
+  w.gridSorted[slot] = i;
+}
+
+/** Body i as the pair tests see it (wgsl-collision.ts:138-171). */
+struct Shape {
+  f3 ax[3];
+  f3 h;
+  bool sphere;
+};
+
+AVBD_D Shape shapeOf(const Body &b) {
+  Shape s;
+  s.h = scale(xyz(b.size), 0.5f);
+  s.sphere = b.angVel.w == SHAPE_SPHERE;
+  s.ax[0] = qrotate(b.rot, mk3(1.f, 0.f, 0.f));
+  s.ax[1] = qrotate(b.rot, mk3(0.f, 1.f, 0.f));
+  s.ax[2] = qrotate(b.rot, mk3(0.f, 0.f, 1.f));
+  return s;
+}
+AVBD_D float extent(const Shape &s, f3 n) {
+  if (s.sphere) return s.h.x;
+  return s.h.x * fabsf(dot(n, s.ax[0])) + s.h.y * fabsf(dot(n, s.ax[1])) + s.h.z * fabsf(dot(n, s.ax[2]));
+}
+AVBD_D f3 aabbHalf(const Shape &s) {
+  if (s.sphere) return mk3(s.h.x, s.h.x, s.h.x);
+  return add(add(scale(vabs(s.ax[0]), s.h.x), scale(vabs(s.ax[1]), s.h.y)), scale(vabs(s.ax[2]), s.h.z));
+}
+AVBD_D bool faceSeparated(const Shape &A, const Shape &B, f3 d) {
+  for (int k = 0; k < 3; k++) {
+    if (fabsf(dot(d, A.ax[k])) - extent(A, A.ax[k]) - extent(B, A.ax[k]) > 1e-3f) return true;
+    if (fabsf(dot(d, B.ax[k])) - extent(A, B.ax[k]) - extent(B, B.ax[k]) > 1e-3f) return true;
+  }
+  return false;
+}
+
+/** Binary search of the (hi, lo) no-collide list; false if every connecting joint has broken (wgsl-collision.ts:112). */
+AVBD_D bool ignored(const DeviceWorld &w, uint32_t hi, uint32_t lo) {
+  uint32_t first = 0, count = w.noCollideCount;
+  while (count > 0) {
+    uint32_t step = count / 2, mid = first + step;
+    uint32_t mh = w.noCollide[3 * mid], ml = w.noCollide[3 * mid + 1];
+    if (mh < hi || (mh == hi && ml < lo)) {
+      first = mid + 1;
+      count -= step + 1;
+    } else {
+      count = step;
+    }
+  }
+  for (uint32_t k = first; k < w.noCollideCount; k++) {
+    if (w.noCollide[3 * k] != hi || w.noCollide[3 * k + 1] != lo) break;
+    uint32_t j = w.noCollide[3 * k + 2];
+    if (j == 0xffffffffu) return true;
+    if (w.joints[j].penLin.w != 0.f || w.joints[j].penAng.w != 0.f) return true;
+  }
+  return false;
+}
+
+/**
+ * The pair predicate of testPair (wgsl-collision.ts:196-221), evaluated with the higher index a as the probe, so a
+ * pair's acceptance does not depend on which thread or broadphase structure found it.
+ */
+AVBD_D bool acceptPair(const DeviceWorld &w, uint32_t a, uint32_t b) {
+  if (w.bodyWorld[a] != w.bodyWorld[b]) return false;
+  const Body &A = w.bodies[a];
+  const Body &B = w.bodies[b];
+  if (!(A.size.w > 0.f) && B.size.w <= 0.f) return false;
+  uint2 fa = w.filters[a], fb = w.filters[b];
+  if ((fa.x & fb.y) == 0u || (fb.x & fa.y) == 0u) return false;
+  f3 d = sub(xyz(A.pos), xyz(B.pos));
+  float r = A.moment.w + B.moment.w;
+  if (dot(d, d) > r * r) return false;
+  Shape sa = shapeOf(A), sb = shapeOf(B);
+  f3 ha = aabbHalf(sa), hb = aabbHalf(sb);
+  if (fabsf(d.x) > ha.x + hb.x + 1e-4f || fabsf(d.y) > ha.y + hb.y + 1e-4f || fabsf(d.z) > ha.z + hb.z + 1e-4f)
+    return false;
+  if (faceSeparated(sa, sb, d)) return false;
+  if (ignored(w, a, b)) return false;
+  return true;
+}
+
+/**
+ * One body's pairs: small bodies scan the 27 cells around theirs for lower-index small bodies and test their world's
+ * large bodies; a large body tests the higher-index large bodies of its world (findPairs, wgsl-collision.ts:223-257).
+ * The count pass stores how many; the write pass stores them at the scanned offset, then sorts them by partner (each
+ * partner appears once, so the order is total and the list is the same whatever order the grid buckets hold).
+ */
+template <bool WRITE> __global__ void k_pairs(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  uint32_t base = WRITE ? w.pairCount[i] : 0;
+  uint32_t n = 0;
+  auto emit = [&](uint32_t a, uint32_t b) {
+    if (!acceptPair(w, a, b)) return;
+    if (WRITE && base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(a, b);
+    n++;
+  };
+  int4 c = w.cellOf[i];
+  bool small = c.w >= 0;
+  uint32_t wid = w.bodyWorld[i];
+  if (small) {
+    for (int oz = -1; oz <= 1; oz++)
+      for (int oy = -1; oy <= 1; oy++)
+        for (int ox = -1; ox <= 1; ox++) {
+          int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
+          uint32_t h = cellHash(w, cell);
+          uint32_t kEnd = w.gridCount[h + 1];
+          for (uint32_t k = w.gridCount[h]; k < kEnd; k++) {
+            uint32_t j = w.gridSorted[k];
+            if (j >= i) continue;
+            int4 cj = w.cellOf[j];
+            if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z || cj.w != cell.w) continue;
+            emit(i, j);
+          }
+        }
+  }
+  for (uint32_t k = w.largeStart[wid]; k < w.largeStart[wid + 1]; k++) {
+    uint32_t l = w.largeList[k];
+    if (l == i || (!small && l < i)) continue;
+    emit(umax(i, l), umin(i, l));
+  }
+  if (!WRITE) {
+    w.pairCount[i] = n;
+    return;
+  }
+  uint32_t lim = base + n < w.pairCapacity ? base + n : w.pairCapacity;
+  for (uint32_t x = base + 1; x < lim; x++) {
+    uint2 v = w.pairs[x];
+    uint32_t vo = v.x == i ? v.y : v.x;
+    uint32_t y = x;
+    while (y > base) {
+      uint2 u = w.pairs[y - 1];
+      uint32_t uo = u.x == i ? u.y : u.x;
+      if (uo <= vo) break;
+      w.pairs[y] = u;
+      y--;
+    }
+    w.pairs[y] = v;
+  }
+}
+
+__global__ void k_pairs_total(DeviceWorld w) {
+  uint32_t total = w.pairCount[w.nBodies];
+  if (total > w.pairCapacity) {
+    w.counters[C_OVERFLOW] |= 1u;
+    total = w.pairCapacity;
+  }
+  w.counters[C_PAIRS] = total;
+}
+
+// --- narrowphase -------------------------------------------------------------------------------------------------------
+
+AVBD_D float contactPenaltyMin(const DeviceWorld &w, const WorldParams &p, uint32_t a, uint32_t b) {
+  if (!(p.flags & FLAG_MASS_PENALTY)) return PENALTY_MIN;
+  float mA = w.bodies[a].size.w <= 0.f ? 3.4e38f : w.bodies[a].size.w;
+  float mB = w.bodies[b].size.w <= 0.f ? 3.4e38f : w.bodies[b].size.w;
+  return rclamp(rmin(mA, mB) / (p.dt * p.dt), PENALTY_MIN, PENALTY_MAX);
+}
+
+AVBD_D f3 clampPen(f3 pen, float gamma) {
+  return mk3(wclamp(pen.x * gamma, PENALTY_MIN, PENALTY_MAX), wclamp(pen.y * gamma, PENALTY_MIN, PENALTY_MAX),
+             wclamp(pen.z * gamma, PENALTY_MIN, PENALTY_MAX));
+}
+
+/** Keep last step's points of a pair whose bodies have not moved since they were made (wgsl-collision.ts:650-678). */
+AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t pslot, uint32_t a, uint32_t b) {
+  int pm = hashFind(w, a, b);
+  if (pm < 0) return false;
+  Manifold prev = w.prevManifolds[pm];
+  uint32_t count = prev.ids.w & 15u;
+  uint32_t generated = prev.ids.w >> 4;
+  uint32_t moved = umax(fbits(w.bodies[a].inertialPos.w), fbits(w.bodies[b].inertialPos.w));
+  if (count == 0 || generated < moved) return false;
+  uint32_t base = pslot * MAX_CONTACTS;
+  w.mfSlots[pslot] = Manifold{make_uint4(a, b, base, prev.ids.w), prev.geo};
+  f3 pA = xyz(w.bodies[a].pos), pB = xyz(w.bodies[b].pos);
+  float4 qA = w.bodies[a].rot, qB = w.bodies[b].rot;
+  Basis basis = orthonormal(xyz(prev.geo));
+  float pmin = contactPenaltyMin(w, p, a, b);
+  for (uint32_t i = 0; i < count; i++) {
+    Contact k = w.prevContacts[prev.ids.z + i];
+    f3 d = sub(add(qrotate(qA, xyz(k.rA)), pA), add(qrotate(qB, xyz(k.rB)), pB));
+    k.rB.w = dot(basis.n, d) + COLLISION_MARGIN;
+    k.pen.w = dot(basis.t1, d);
+    k.lam.w = dot(basis.t2, d);
+    f3 lam = scale(scale(xyz(k.lam), p.alpha), p.gamma);
+    f3 pen = clampPen(xyz(k.pen), p.gamma);
+    pen.x = rmax(pen.x, pmin);
+    k.lam = mk4(lam, k.lam.w);
+    k.pen = mk4(pen, k.pen.w);
+    w.contacts[base + i] = k;
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
+  uint32_t P = w.counters[C_PAIRS];
+  for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
+    uint2 pr = w.pairs[pslot];
+    uint32_t a = pr.x, b = pr.y;
+    const WorldParams &p = w.worlds[w.bodyWorld[a]];
+    if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(w, p, pslot, a, b)) continue;
+    const Body &BA = w.bodies[a];
+    const Body &BB = w.bodies[b];
+    Box A = makeBox(xyz(BA.pos), BA.rot, xyz(BA.size));
+    Box B = makeBox(xyz(BB.pos), BB.rot, xyz(BB.size));
+    Sat sat;
+    sat.n = mk3(0.f, 0.f, 0.f);
+    Found found;
+    found.count = 0;
+    bool sphereA = BA.angVel.w == SHAPE_SPHERE, sphereB = BB.angVel.w == SHAPE_SPHERE;
+    if (sphereA && sphereB) {
+      found = sphereSphere(A, B, sat);
+    } else if (sphereA || sphereB) {
+      if (sphereA) {
+        SphereBoxHit h = sphereBox(A, B, BB.rot);
+        sat.n = h.n;
+        if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+      } else {
+        SphereBoxHit h = sphereBox(B, A, BA.rot);
+        sat.n = neg(h.n);
+        if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+      }
+    } else {
+      found = collideBoxes(A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
+    }
+    uint32_t base = pslot * MAX_CONTACTS;
+    if (found.count == 0) {
+      w.mfSlots[pslot].ids = make_uint4(a, b, base, 0u);
+      continue;
+    }
+    f3 n = neg(sat.n);
+    w.mfSlots[pslot] = Manifold{make_uint4(a, b, base, uint32_t(found.count) | (p.step << 4)),
+                                mk4(n, sqrtf(BA.pos.w * BB.pos.w))};
+    float4 qA = BA.rot, qB = BB.rot;
+    Basis basis = orthonormal(n);
+    bool anySphere = sphereA || sphereB;
+    uint32_t prevFirst = 0, prevCount = 0;
+    int pm = hashFind(w, a, b);
+    if (pm >= 0) {
+      prevFirst = w.prevManifolds[pm].ids.z;
+      prevCount = w.prevManifolds[pm].ids.w & 15u;
+    }
+    bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
+    float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
+    float pmin = contactPenaltyMin(w, p, a, b);
+    for (int i = 0; i < found.count; i++) {
+      f3 rA = qrotate(qconj(qA), sub(found.xA[i], A.c));
+      f3 rB = qrotate(qconj(qB), sub(found.xB[i], B.c));
+      f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);
+      uint32_t stick = 0;
+      int j = -1;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+        if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == found.feature[i]) {
+          j = int(c);
+          break;
+        }
+      if (j < 0 && matchNearest) {
+        float bestDist = NEAREST_FRACTION * minSide;
+        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+          float dist = length(sub(xyz(w.prevContacts[c].rA), rA));
+          if (dist <= bestDist) {
+            bestDist = dist;
+            j = int(c);
+          }
+        }
+      }
+      if (j >= 0) {
+        Contact pc = w.prevContacts[j];
+        pen = xyz(pc.pen);
+        lam = xyz(pc.lam);
+        stick = fbits(pc.rA.w) & STICK_BIT;
+        if (stick != 0 && !anySphere) {
+          rA = xyz(pc.rA);
+          rB = xyz(pc.rB);
+        }
+      }
+      f3 d = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
+      float c0x = dot(basis.n, d) + COLLISION_MARGIN;
+      float c0y = dot(basis.t1, d);
+      float c0z = dot(basis.t2, d);
+      lam = scale(scale(lam, p.alpha), p.gamma);
+      pen = clampPen(pen, p.gamma);
+      pen.x = rmax(pen.x, pmin);
+      Contact k;
+      k.rA = mk4(rA, bitsf(found.feature[i] | stick));
+      k.rB = mk4(rB, c0x);
+      k.pen = mk4(pen, c0y);
+      k.lam = mk4(lam, c0z);
+      w.contacts[base + i] = k;
+    }
+  }
+}
+
+__global__ void k_manifold_flags(DeviceWorld w) {
+  uint32_t P = w.counters[C_PAIRS];
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= w.pairCapacity; p += gridDim.x * blockDim.x)
+    w.mfOffset[p] = (p < P && (w.mfSlots[p].ids.w & 15u) != 0u) ? 1u : 0u;
+}
+
+__global__ void k_manifold_compact(DeviceWorld w) {
+  uint32_t P = w.counters[C_PAIRS];
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+    uint32_t o = w.mfOffset[p];
+    if (w.mfOffset[p + 1] != o) {
+      Manifold m = w.mfSlots[p];
+      w.manifolds[o] = m;
+      atomicAdd(&w.counters[C_CONTACTS], m.ids.w & 15u);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
+}
+
+// --- launch wrappers ---------------------------------------------------------------------------------------------------
+
+static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) {
+  uint64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return uint32_t(g);
+}
+
+void launch_begin_frame(const DeviceWorld &w, hipStream_t s) {
+  k_begin_frame<<<blocks(w.nWorlds, 256), 256, 0, s>>>(w);
+}
+void launch_update_refs(const DeviceWorld &w, hipStream_t s) {
+  k_update_refs<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+}
+void launch_hash_build(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s) {
+  k_hash_clear<<<blocks(uint64_t(w.hashMask) + 1, 256, 4096), 256, 0, s>>>(w);
+  k_hash_insert<<<blocks(prevCapacity, 256, 2048), 256, 0, s>>>(w);
+}
+void launch_grid(const DeviceWorld &w, hipStream_t s) {
+  launch_fill_u32(w.gridCount, 0u, uint64_t(w.tableMask) + 2, s);
+  launch_fill_u32(w.gridCursor, 0u, uint64_t(w.tableMask) + 1, s);
+  k_grid_count<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+  launch_scan_exclusive(w.gridCount, w.tableMask + 2, w.scanScratch, s);
+  k_grid_scatter<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+}
+void launch_pairs(const DeviceWorld &w, hipStream_t s) {
+  k_pairs<false><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  k_pairs<true><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
+  k_pairs_total<<<1, 1, 0, s>>>(w);
+}
+void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
+  k_narrowphase<<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
+  k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);
+  launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s);
+  k_manifold_compact<<<blocks(w.pairCapacity, 256, 4096), 256, 0, s>>>(w);
+}
+
+}  // namespace avbd

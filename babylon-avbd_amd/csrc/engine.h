@@ -1,0 +1,124 @@
+// Host engine: owns the device memory of a batch of worlds (one world is a batch of one) and orders the step's kernels on
+// one HIP stream, optionally replaying them from a captured hipGraph (one graph per contact-buffer parity).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "engine_types.h"
+
+namespace avbd {
+
+/** One world of a batch as the host builds it: reference records plus parameters. */
+struct HostWorld {
+  std::vector<Body> bodies;
+  std::vector<Joint> joints;
+  std::vector<int4> info;                       // type, a, b (world-local; a = -1: world)
+  std::vector<std::pair<uint32_t, uint32_t>> ignorePairs;  // world-local body pairs that never collide
+  std::vector<uint2> filters;                   // empty: collide with everything
+  WorldParams params;
+  uint32_t iterations = 10;
+};
+
+struct EngineOptions {
+  uint32_t colorCap = 16;       // colours dispatched per iteration (and the colouring's cap)
+  uint32_t colorRounds = 16;    // Jones–Plassmann rounds per step (even)
+  uint32_t pairCapacity = 0;    // 0: chosen from the body count
+  bool useGraph = false;
+  int device = 0;
+};
+
+/** Milliseconds of one profiled step by phase, measured with HIP events on the engine's stream. */
+struct PhaseTimes {
+  float total, collision, adjacency, coloring, warmStart, primal, dual, velocities;
+  uint32_t primalLaunches, dualLaunches;
+};
+
+struct Counters {
+  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds;
+};
+
+class Engine {
+ public:
+  Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt);
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  void step(uint32_t n);
+  void synchronize();
+  /** One step (not from a graph) with HIP events around each phase and each primal/dual launch. */
+  PhaseTimes profileStep();
+
+  uint32_t bodyCount() const { return nBodies_; }
+  uint32_t jointCount() const { return nJoints_; }
+  uint32_t worldCount() const { return nWorlds_; }
+  uint32_t iterations() const { return iterations_; }
+  hipStream_t stream() const { return stream_; }
+  const DeviceWorld &view() const { return views_[parity_]; }
+  const std::vector<uint32_t> &worldFirstBody() const { return firstBody_; }
+  const std::vector<uint32_t> &worldFirstJoint() const { return firstJoint_; }
+
+  // Reads (synchronous; copy into caller buffers)
+  void readBodies(uint32_t first, uint32_t count, Body *out);
+  void writeBodies(uint32_t first, uint32_t count, const Body *in);
+  void readJoints(uint32_t first, uint32_t count, Joint *out);
+  void writeJoints(uint32_t first, uint32_t count, const Joint *in);
+  Counters readCounters();
+  /** The last step's pair list (a > b) in emission order. */
+  std::vector<uint2> readPairs();
+  /** The last step's manifolds (first = index into readContacts) and contact slots. */
+  void readContacts(std::vector<Manifold> &manifolds, std::vector<Contact> &contacts);
+  std::vector<uint32_t> readColors();
+  std::vector<float4> readRefs();
+  std::vector<WorldParams> readWorldParams();
+  /** Adjacency of the last step: start (bodies + 1), list. */
+  void readAdjacency(std::vector<uint32_t> &start, std::vector<uint32_t> &list);
+
+  /** Seed the persistent state: reference poses, colour state, last step's manifolds/contacts, step counters. */
+  void writeState(const float4 *refs, const uint32_t *colors, const std::vector<Manifold> &manifolds,
+                  const std::vector<Contact> &contacts, const uint32_t *steps);
+
+  void setColorCap(uint32_t cap);
+  void setColorRounds(uint32_t rounds);
+  uint32_t colorCap() const { return colorCap_; }
+  uint32_t pairCapacity() const { return pairCap_; }
+  /** Grow contact storage / colour cap from the last counters (the reference's adapt); returns true if it changed. */
+  bool adapt();
+
+  /** Device bytes held by this engine. */
+  size_t deviceBytes() const { return deviceBytes_; }
+
+ private:
+  void allocate();
+  void allocateContacts(uint32_t pairCap);
+  void encodeStep(const DeviceWorld &w);
+  void *dalloc(size_t bytes);
+  template <class T> T *dallocT(size_t count) { return static_cast<T *>(dalloc(count * sizeof(T))); }
+  void resetGraphs();
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  uint32_t nBodies_ = 0, nJoints_ = 0, nWorlds_ = 0, noCollideCount_ = 0;
+  uint32_t iterations_ = 10, colorCap_ = 16, colorRounds_ = 16;
+  uint32_t pairCap_ = 0, tableSize_ = 0, hashSize_ = 0, colorGroups_ = 0, maxDynamicBodies_ = 0;
+  bool useGraph_ = false;
+  int parity_ = 0;
+  std::vector<uint32_t> firstBody_, firstJoint_;
+  std::vector<void *> allocations_;
+  std::vector<void *> contactAllocations_;
+  size_t deviceBytes_ = 0, contactBytes_ = 0;
+
+  DeviceWorld base_{};
+  DeviceWorld views_[2]{};
+  Manifold *manifoldBuf_[2] = {nullptr, nullptr};
+  Contact *contactBuf_[2] = {nullptr, nullptr};
+  hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+};
+
+/** Largest bounding radius still sized into grid cells (reference/three-avbd/src/avbd3d/gpu/radii.ts:47-55). */
+double largestSmall(std::vector<double> radii);
+
+}  // namespace avbd

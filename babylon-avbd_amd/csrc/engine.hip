@@ -1,0 +1,572 @@
+// Host engine of the AVBD 3D step: batch assembly, HBM allocation, the per-step launch order, hipGraph replay, reads,
+// state seeding and capacity adaptation. The launch order is the reference's GpuSolver3D.step
+// (reference/three-avbd/src/avbd3d/gpu/solver.ts:1281-1417): collision → adjacency → colouring → warm starts →
+// iterations × (primal per colour, dual) → velocities; its capacity policy is GpuSolver3D.adapt (solver.ts:1484-1519).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <tuple>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace avbd {
+
+#define HIP_CHECK(x)                                                                                         \
+  do {                                                                                                       \
+    hipError_t err_ = (x);                                                                                   \
+    if (err_ != hipSuccess)                                                                                  \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(err_) + " at " #x);             \
+  } while (0)
+
+static uint32_t pow2AtLeast(uint64_t n) {
+  uint64_t p = 2;
+  while (p < n) p <<= 1;
+  return uint32_t(p);
+}
+
+double largestSmall(std::vector<double> radii) {
+  const size_t n = radii.size();
+  const double LARGE_FACTOR = 2;
+  const size_t MAX_LARGE = 64;
+  double median = 1;
+  if (n > 0) {
+    std::nth_element(radii.begin(), radii.begin() + (n >> 1), radii.end());
+    median = radii[n >> 1];
+  }
+  double maxSmall = 0;
+  for (double r : radii)
+    if (r <= LARGE_FACTOR * median) maxSmall = std::max(maxSmall, r);
+  if (n > MAX_LARGE) {
+    std::nth_element(radii.begin(), radii.begin() + (n - MAX_LARGE - 1), radii.end());
+    maxSmall = std::max(maxSmall, radii[n - MAX_LARGE - 1]);
+  }
+  return maxSmall;
+}
+
+void *Engine::dalloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  HIP_CHECK(hipMalloc(&p, bytes));
+  HIP_CHECK(hipMemsetAsync(p, 0, bytes, stream_));
+  allocations_.push_back(p);
+  deviceBytes_ += bytes;
+  return p;
+}
+
+Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
+  if (worlds.empty()) throw std::invalid_argument("a batch needs at least one world");
+  device_ = opt.device;
+  HIP_CHECK(hipSetDevice(device_));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  useGraph_ = opt.useGraph;
+  colorCap_ = std::min<uint32_t>(std::max<uint32_t>(opt.colorCap, 1), MAX_COLORS);
+  colorRounds_ = opt.colorRounds + (opt.colorRounds & 1u);
+  nWorlds_ = uint32_t(worlds.size());
+  iterations_ = worlds[0].iterations;
+
+  std::vector<Body> bodies;
+  std::vector<uint32_t> bodyWorld;
+  std::vector<Joint> joints;
+  std::vector<int4> info;
+  std::vector<uint2> filters;
+  std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> noCollide;
+  std::vector<WorldParams> params(nWorlds_);
+  std::vector<uint32_t> largeStart(nWorlds_ + 1, 0), largeList;
+  double cellSize = 1e-3;
+  for (uint32_t k = 0; k < nWorlds_; k++) {
+    const HostWorld &hw = worlds[k];
+    if (hw.iterations != iterations_) throw std::invalid_argument("every world of a batch takes the same iterations");
+    const uint32_t base = uint32_t(bodies.size());
+    const uint32_t jbase = uint32_t(joints.size());
+    const uint32_t n = uint32_t(hw.bodies.size());
+    firstBody_.push_back(base);
+    firstJoint_.push_back(jbase);
+    bodies.insert(bodies.end(), hw.bodies.begin(), hw.bodies.end());
+    bodyWorld.insert(bodyWorld.end(), n, k);
+    if (!hw.filters.empty() && hw.filters.size() != n) throw std::invalid_argument("filters: one pair per body");
+    for (uint32_t i = 0; i < n; i++) filters.push_back(hw.filters.empty() ? make_uint2(~0u, ~0u) : hw.filters[i]);
+    for (size_t j = 0; j < hw.joints.size(); j++) {
+      int4 f = hw.info[j];
+      if (f.z < 0 || uint32_t(f.z) >= n || f.y >= int(n) || f.y == f.z)
+        throw std::invalid_argument("joint endpoints must be distinct bodies of the world");
+      if (f.x != T_JOINT && f.x != T_SPRING && f.x != T_NONE) throw std::invalid_argument("unknown joint type");
+      if (f.x == T_SPRING && f.y < 0) throw std::invalid_argument("A spring requires two bodies");
+      int a = f.y < 0 ? -1 : int(base) + f.y;
+      int b = int(base) + f.z;
+      joints.push_back(hw.joints[j]);
+      info.push_back(make_int4(f.x, a, b, 0));
+      if (a >= 0) noCollide.emplace_back(uint32_t(std::max(a, b)), uint32_t(std::min(a, b)), jbase + uint32_t(j));
+    }
+    for (auto &pr : hw.ignorePairs) {
+      if (pr.first >= n || pr.second >= n || pr.first == pr.second)
+        throw std::invalid_argument("ignored pairs must be distinct bodies of the world");
+      uint32_t a = base + pr.first, b = base + pr.second;
+      noCollide.emplace_back(std::max(a, b), std::min(a, b), 0xffffffffu);
+    }
+    // Large/small classification and grid cell size (solver.ts:860-881, radii.ts), per world
+    std::vector<double> radii(n);
+    for (uint32_t i = 0; i < n; i++) radii[i] = hw.bodies[i].moment.w;
+    double maxSmall = largestSmall(radii);
+    double minLarge = INFINITY;
+    for (double r : radii)
+      if (r > maxSmall) minLarge = std::min(minLarge, r);
+    double threshold = std::isinf(minLarge) ? maxSmall * 1.5 : (maxSmall + minLarge) / 2;
+    cellSize = std::max(cellSize, std::max(2 * maxSmall * (1 + 1e-4), 1e-3));
+    WorldParams p = hw.params;
+    p.firstBody = base;
+    p.bodyCount = n;
+    p.maxSmall = float(threshold);
+    params[k] = p;
+    largeStart[k] = uint32_t(largeList.size());
+    for (uint32_t i = 0; i < n; i++)
+      if (double(hw.bodies[i].moment.w) > double(p.maxSmall)) largeList.push_back(base + i);
+  }
+  largeStart[nWorlds_] = uint32_t(largeList.size());
+  std::sort(noCollide.begin(), noCollide.end(),
+            [](auto &x, auto &y) { return std::tie(std::get<0>(x), std::get<1>(x)) < std::tie(std::get<0>(y), std::get<1>(y)); });
+  nBodies_ = uint32_t(bodies.size());
+  nJoints_ = uint32_t(joints.size());
+  noCollideCount_ = uint32_t(noCollide.size());
+  for (auto &b : bodies)
+    if (b.size.w > 0.f) maxDynamicBodies_++;
+
+  pairCap_ = opt.pairCapacity ? opt.pairCapacity : std::max<uint32_t>(4096, 8 * nBodies_);
+  tableSize_ = pow2AtLeast(2ull * std::max<uint32_t>(nBodies_, 1));
+  colorGroups_ = (std::max<uint32_t>(nBodies_, 1) + COLOR_WG - 1) / COLOR_WG;
+
+  DeviceWorld &w = base_;
+  w.nBodies = nBodies_;
+  w.nJoints = nJoints_;
+  w.nWorlds = nWorlds_;
+  w.noCollideCount = noCollideCount_;
+  w.bodies = dallocT<Body>(nBodies_);
+  w.bodyWorld = dallocT<uint32_t>(nBodies_);
+  w.worlds = dallocT<WorldParams>(nWorlds_);
+  w.refs = dallocT<float4>(2ull * nBodies_);
+  w.filters = dallocT<uint2>(nBodies_);
+  w.joints = dallocT<Joint>(nJoints_);
+  w.info = dallocT<int4>(nJoints_);
+  w.noCollide = dallocT<uint32_t>(3ull * noCollideCount_);
+  w.counters = dallocT<uint32_t>(COUNTER_WORDS);
+  w.gridCount = dallocT<uint32_t>(uint64_t(tableSize_) + 2);
+  w.gridCursor = dallocT<uint32_t>(tableSize_);
+  w.gridSorted = dallocT<uint32_t>(nBodies_);
+  w.cellOf = dallocT<int4>(nBodies_);
+  w.tableMask = tableSize_ - 1;
+  w.cellSize = float(cellSize);
+  w.largeStart = dallocT<uint32_t>(nWorlds_ + 1);
+  w.largeList = dallocT<uint32_t>(largeList.size());
+  w.pairCount = dallocT<uint32_t>(uint64_t(nBodies_) + 1);
+  w.adjStart = dallocT<uint32_t>(uint64_t(nBodies_) + 1);
+  w.adjFill = dallocT<uint32_t>(nBodies_);
+  w.colorA = dallocT<uint32_t>(nBodies_);
+  w.colorB = dallocT<uint32_t>(nBodies_);
+  w.colorRank = dallocT<uint32_t>(nBodies_);
+  w.colorHist = dallocT<uint32_t>(uint64_t(MAX_COLORS) * colorGroups_ + 1);
+  w.colorStart = dallocT<uint32_t>(MAX_COLORS + 1);
+  w.colorBodies = dallocT<uint32_t>(nBodies_);
+  w.colorGroups = colorGroups_;
+  w.colorCap = colorCap_;
+
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+  };
+  up(w.bodies, bodies.data(), bodies.size() * sizeof(Body));
+  up(w.bodyWorld, bodyWorld.data(), bodyWorld.size() * 4);
+  up(w.worlds, params.data(), params.size() * sizeof(WorldParams));
+  up(w.filters, filters.data(), filters.size() * sizeof(uint2));
+  up(w.joints, joints.data(), joints.size() * sizeof(Joint));
+  up(w.info, info.data(), info.size() * sizeof(int4));
+  std::vector<uint32_t> nc;
+  for (auto &t : noCollide) {
+    nc.push_back(std::get<0>(t));
+    nc.push_back(std::get<1>(t));
+    nc.push_back(std::get<2>(t));
+  }
+  up(w.noCollide, nc.data(), nc.size() * 4);
+  up(w.largeStart, largeStart.data(), largeStart.size() * 4);
+  up(w.largeList, largeList.data(), largeList.size() * 4);
+  std::vector<uint32_t> noColor(nBodies_, NO_COLOR);
+  up(w.colorA, noColor.data(), noColor.size() * 4);
+  allocateContacts(pairCap_);
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::allocateContacts(uint32_t pairCap) {
+  // Keep last step's manifolds and contacts (the next step's warm-start source) across a resize
+  std::vector<Manifold> keepM;
+  std::vector<Contact> keepC;
+  uint32_t counters[COUNTER_WORDS] = {0};
+  bool had = manifoldBuf_[0] != nullptr;
+  if (had) {
+    readContacts(keepM, keepC);
+    HIP_CHECK(hipMemcpy(counters, base_.counters, sizeof counters, hipMemcpyDeviceToHost));
+    for (void *p : contactAllocations_) {
+      HIP_CHECK(hipFree(p));
+    }
+    contactAllocations_.clear();
+    deviceBytes_ -= contactBytes_;
+  }
+  pairCap_ = pairCap;
+  hashSize_ = pow2AtLeast(2ull * pairCap_);
+  const size_t bytesBefore = deviceBytes_;
+  auto alloc = [&](size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    HIP_CHECK(hipMalloc(&p, bytes));
+    HIP_CHECK(hipMemsetAsync(p, 0, bytes, stream_));
+    contactAllocations_.push_back(p);
+    deviceBytes_ += bytes;
+    return p;
+  };
+  DeviceWorld &w = base_;
+  w.pairCapacity = pairCap_;
+  w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
+  w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
+  w.mfOffset = static_cast<uint32_t *>(alloc((size_t(pairCap_) + 1) * 4));
+  for (int k = 0; k < 2; k++) {
+    manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
+    contactBuf_[k] = static_cast<Contact *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(Contact)));
+  }
+  w.table = static_cast<uint32_t *>(alloc(size_t(hashSize_) * 4));
+  w.hashMask = hashSize_ - 1;
+  w.adjCapacity = 2 * (nJoints_ + pairCap_);
+  w.adjList = static_cast<uint32_t *>(alloc(size_t(w.adjCapacity) * 4));
+  uint32_t scanNeed = scan_scratch_words(std::max<uint32_t>({tableSize_ + 2, nBodies_ + 1, pairCap_ + 1,
+                                                              MAX_COLORS * colorGroups_ + 1}));
+  w.scanScratch = static_cast<uint32_t *>(alloc(size_t(scanNeed) * 4));
+  contactBytes_ = deviceBytes_ - bytesBefore;
+  for (int p = 0; p < 2; p++) {
+    views_[p] = base_;
+    views_[p].manifolds = manifoldBuf_[p];
+    views_[p].prevManifolds = manifoldBuf_[1 - p];
+    views_[p].contacts = contactBuf_[p];
+    views_[p].prevContacts = contactBuf_[1 - p];
+  }
+  resetGraphs();
+  if (had) {
+    // The next step reads them as "previous" from the buffers of views_[parity_]
+    HIP_CHECK(hipMemcpyAsync(views_[parity_].prevManifolds, keepM.data(), keepM.size() * sizeof(Manifold),
+                             hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(views_[parity_].prevContacts, keepC.data(), keepC.size() * sizeof(Contact),
+                             hipMemcpyHostToDevice, stream_));
+    counters[C_MANIFOLDS] = uint32_t(keepM.size());
+    counters[C_CONTACTS] = uint32_t(keepC.size());
+    HIP_CHECK(hipMemcpyAsync(base_.counters, counters, sizeof counters, hipMemcpyHostToDevice, stream_));
+  }
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  resetGraphs();
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (void *p : contactAllocations_) (void)hipFree(p);
+  for (void *p : allocations_) (void)hipFree(p);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::resetGraphs() {
+  for (auto &g : graphs_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void Engine::encodeStep(const DeviceWorld &w) {
+  hipStream_t s = stream_;
+  launch_begin_frame(w, s);
+  launch_update_refs(w, s);
+  launch_hash_build(w, pairCap_, s);
+  launch_grid(w, s);
+  launch_pairs(w, s);
+  launch_narrowphase(w, s);
+  launch_adjacency(w, pairCap_, s);
+  launch_coloring(w, colorRounds_, s);
+  launch_warm_start(w, s);
+  for (uint32_t it = 0; it < iterations_; it++) {
+    for (uint32_t c = 0; c < colorCap_; c++) launch_primal(w, c, maxDynamicBodies_, s);
+    launch_dual(w, pairCap_, s);
+  }
+  launch_velocities(w, s);
+}
+
+void Engine::step(uint32_t n) {
+  HIP_CHECK(hipSetDevice(device_));
+  for (uint32_t k = 0; k < n; k++) {
+    DeviceWorld &w = views_[parity_];
+    w.colorCap = colorCap_;
+    if (useGraph_) {
+      if (!graphs_[parity_]) {
+        hipGraph_t g;
+        HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        encodeStep(w);
+        HIP_CHECK(hipStreamEndCapture(stream_, &g));
+        HIP_CHECK(hipGraphInstantiate(&graphs_[parity_], g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(g));
+      }
+      HIP_CHECK(hipGraphLaunch(graphs_[parity_], stream_));
+    } else {
+      encodeStep(w);
+    }
+    parity_ ^= 1;
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+PhaseTimes Engine::profileStep() {
+  HIP_CHECK(hipSetDevice(device_));
+  const uint32_t launches = iterations_ * (colorCap_ + 1);
+  std::vector<hipEvent_t> ev(6 + 2 * launches);
+  for (auto &x : ev) HIP_CHECK(hipEventCreate(&x));
+  hipStream_t s = stream_;
+  DeviceWorld &w = views_[parity_];
+  w.colorCap = colorCap_;
+  HIP_CHECK(hipEventRecord(ev[0], s));
+  launch_begin_frame(w, s);
+  launch_update_refs(w, s);
+  launch_hash_build(w, pairCap_, s);
+  launch_grid(w, s);
+  launch_pairs(w, s);
+  launch_narrowphase(w, s);
+  HIP_CHECK(hipEventRecord(ev[1], s));
+  launch_adjacency(w, pairCap_, s);
+  HIP_CHECK(hipEventRecord(ev[2], s));
+  launch_coloring(w, colorRounds_, s);
+  HIP_CHECK(hipEventRecord(ev[3], s));
+  launch_warm_start(w, s);
+  HIP_CHECK(hipEventRecord(ev[4], s));
+  uint32_t k = 6;
+  for (uint32_t it = 0; it < iterations_; it++) {
+    for (uint32_t c = 0; c < colorCap_; c++) {
+      HIP_CHECK(hipEventRecord(ev[k++], s));
+      launch_primal(w, c, maxDynamicBodies_, s);
+      HIP_CHECK(hipEventRecord(ev[k++], s));
+    }
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+    launch_dual(w, pairCap_, s);
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+  }
+  launch_velocities(w, s);
+  HIP_CHECK(hipEventRecord(ev[5], s));
+  HIP_CHECK(hipEventSynchronize(ev[5]));
+  parity_ ^= 1;
+  auto ms = [&](int a, int b) {
+    float t = 0;
+    HIP_CHECK(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return t;
+  };
+  PhaseTimes p{};
+  p.total = ms(0, 5);
+  p.collision = ms(0, 1);
+  p.adjacency = ms(1, 2);
+  p.coloring = ms(2, 3);
+  p.warmStart = ms(3, 4);
+  k = 6;
+  for (uint32_t it = 0; it < iterations_; it++) {
+    for (uint32_t c = 0; c < colorCap_; c++, k += 2) p.primal += ms(k, k + 1);
+    p.dual += ms(k, k + 1);
+    k += 2;
+  }
+  p.velocities = ms(k - 1, 5);
+  p.primalLaunches = iterations_ * colorCap_;
+  p.dualLaunches = iterations_;
+  for (auto &x : ev) HIP_CHECK(hipEventDestroy(x));
+  return p;
+}
+
+void Engine::readBodies(uint32_t first, uint32_t count, Body *out) {
+  if (size_t(first) + count > nBodies_) throw std::out_of_range("body range");
+  HIP_CHECK(hipMemcpyAsync(out, base_.bodies + first, size_t(count) * sizeof(Body), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+void Engine::writeBodies(uint32_t first, uint32_t count, const Body *in) {
+  if (size_t(first) + count > nBodies_) throw std::out_of_range("body range");
+  HIP_CHECK(hipMemcpyAsync(base_.bodies + first, in, size_t(count) * sizeof(Body), hipMemcpyHostToDevice, stream_));
+  synchronize();
+}
+void Engine::readJoints(uint32_t first, uint32_t count, Joint *out) {
+  if (size_t(first) + count > nJoints_) throw std::out_of_range("joint range");
+  if (count)
+    HIP_CHECK(hipMemcpyAsync(out, base_.joints + first, size_t(count) * sizeof(Joint), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+void Engine::writeJoints(uint32_t first, uint32_t count, const Joint *in) {
+  if (size_t(first) + count > nJoints_) throw std::out_of_range("joint range");
+  if (count)
+    HIP_CHECK(hipMemcpyAsync(base_.joints + first, in, size_t(count) * sizeof(Joint), hipMemcpyHostToDevice, stream_));
+  synchronize();
+}
+
+Counters Engine::readCounters() {
+  uint32_t c[COUNTER_WORDS];
+  HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return Counters{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
+                  c[C_PREV_MANIFOLDS]};
+}
+
+std::vector<uint2> Engine::readPairs() {
+  Counters c = readCounters();
+  std::vector<uint2> out(c.pairs);
+  if (c.pairs)
+    HIP_CHECK(hipMemcpyAsync(out.data(), base_.pairs, size_t(c.pairs) * sizeof(uint2), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return out;
+}
+
+__global__ void k_contact_counts(const Manifold *m, uint32_t count, uint32_t *out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    out[i] = m[i].ids.w & 15u;
+}
+
+/** Dense copy of each manifold's contacts at its scanned offset; the copied manifold points there. */
+__global__ void k_gather_contacts(const Manifold *m, uint32_t count, const uint32_t *offsets, const Contact *src,
+                                  Contact *dst, Manifold *mout) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    Manifold mf = m[i];
+    uint32_t o = offsets[i], n = mf.ids.w & 15u;
+    for (uint32_t k = 0; k < n; k++) dst[o + k] = src[mf.ids.z + k];
+    mf.ids.z = o;
+    mout[i] = mf;
+  }
+}
+
+void Engine::readContacts(std::vector<Manifold> &manifolds, std::vector<Contact> &contacts) {
+  uint32_t c[COUNTER_WORDS];
+  HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  const uint32_t m = c[C_MANIFOLDS];
+  manifolds.resize(m);
+  contacts.clear();
+  if (m == 0) return;
+  // The last step wrote the buffers the next step reads as "previous"
+  const Manifold *dm = views_[parity_].prevManifolds;
+  const Contact *dc = views_[parity_].prevContacts;
+  uint32_t *offsets = nullptr, *scratch = nullptr;
+  Contact *dense = nullptr;
+  Manifold *mout = nullptr;
+  const uint32_t words = scan_scratch_words(m + 1);
+  HIP_CHECK(hipMalloc(&offsets, (size_t(m) + 1) * 4));
+  HIP_CHECK(hipMalloc(&scratch, size_t(words) * 4));
+  HIP_CHECK(hipMalloc(&dense, size_t(m) * MAX_CONTACTS * sizeof(Contact)));
+  HIP_CHECK(hipMalloc(&mout, size_t(m) * sizeof(Manifold)));
+  HIP_CHECK(hipMemsetAsync(offsets, 0, (size_t(m) + 1) * 4, stream_));
+  uint32_t g = std::min<uint32_t>((m + 255) / 256, 4096);
+  k_contact_counts<<<g, 256, 0, stream_>>>(dm, m, offsets);
+  launch_scan_exclusive(offsets, m + 1, scratch, stream_);
+  k_gather_contacts<<<g, 256, 0, stream_>>>(dm, m, offsets, dc, dense, mout);
+  uint32_t total = 0;
+  HIP_CHECK(hipMemcpyAsync(&total, offsets + m, 4, hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(manifolds.data(), mout, size_t(m) * sizeof(Manifold), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  contacts.resize(total);
+  if (total)
+    HIP_CHECK(hipMemcpyAsync(contacts.data(), dense, size_t(total) * sizeof(Contact), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  HIP_CHECK(hipFree(offsets));
+  HIP_CHECK(hipFree(scratch));
+  HIP_CHECK(hipFree(dense));
+  HIP_CHECK(hipFree(mout));
+}
+
+std::vector<uint32_t> Engine::readColors() {
+  std::vector<uint32_t> out(nBodies_);
+  HIP_CHECK(hipMemcpyAsync(out.data(), base_.colorA, size_t(nBodies_) * 4, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return out;
+}
+std::vector<float4> Engine::readRefs() {
+  std::vector<float4> out(2ull * nBodies_);
+  HIP_CHECK(hipMemcpyAsync(out.data(), base_.refs, out.size() * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return out;
+}
+std::vector<WorldParams> Engine::readWorldParams() {
+  std::vector<WorldParams> out(nWorlds_);
+  HIP_CHECK(hipMemcpyAsync(out.data(), base_.worlds, out.size() * sizeof(WorldParams), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  return out;
+}
+void Engine::readAdjacency(std::vector<uint32_t> &start, std::vector<uint32_t> &list) {
+  start.resize(uint64_t(nBodies_) + 1);
+  HIP_CHECK(hipMemcpyAsync(start.data(), base_.adjStart, start.size() * 4, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  list.resize(start.back());
+  if (!list.empty())
+    HIP_CHECK(hipMemcpyAsync(list.data(), base_.adjList, list.size() * 4, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
+void Engine::writeState(const float4 *refs, const uint32_t *colors, const std::vector<Manifold> &manifolds,
+                        const std::vector<Contact> &contacts, const uint32_t *steps) {
+  if (manifolds.size() > pairCap_ || contacts.size() > size_t(pairCap_) * MAX_CONTACTS)
+    throw std::length_error("state exceeds the contact capacity");
+  for (const Manifold &m : manifolds)
+    if (size_t(m.ids.z) + (m.ids.w & 15u) > contacts.size()) throw std::out_of_range("manifold contact range");
+  if (refs)
+    HIP_CHECK(hipMemcpyAsync(base_.refs, refs, 2ull * nBodies_ * sizeof(float4), hipMemcpyHostToDevice, stream_));
+  if (colors) HIP_CHECK(hipMemcpyAsync(base_.colorA, colors, size_t(nBodies_) * 4, hipMemcpyHostToDevice, stream_));
+  if (!manifolds.empty())
+    HIP_CHECK(hipMemcpyAsync(views_[parity_].prevManifolds, manifolds.data(), manifolds.size() * sizeof(Manifold),
+                             hipMemcpyHostToDevice, stream_));
+  if (!contacts.empty())
+    HIP_CHECK(hipMemcpyAsync(views_[parity_].prevContacts, contacts.data(), contacts.size() * sizeof(Contact),
+                             hipMemcpyHostToDevice, stream_));
+  uint32_t c[COUNTER_WORDS] = {0};
+  c[C_MANIFOLDS] = uint32_t(manifolds.size());
+  c[C_CONTACTS] = uint32_t(contacts.size());
+  HIP_CHECK(hipMemcpyAsync(base_.counters, c, sizeof c, hipMemcpyHostToDevice, stream_));
+  if (steps) {
+    std::vector<WorldParams> p = readWorldParams();
+    for (uint32_t k = 0; k < nWorlds_; k++) p[k].step = steps[k];
+    HIP_CHECK(hipMemcpyAsync(base_.worlds, p.data(), p.size() * sizeof(WorldParams), hipMemcpyHostToDevice, stream_));
+  }
+  synchronize();
+}
+
+void Engine::setColorCap(uint32_t cap) {
+  cap = std::min<uint32_t>(std::max<uint32_t>(cap, 1), MAX_COLORS);
+  if (cap != colorCap_) {
+    colorCap_ = cap;
+    base_.colorCap = cap;
+    views_[0].colorCap = views_[1].colorCap = cap;
+    resetGraphs();
+  }
+}
+void Engine::setColorRounds(uint32_t rounds) {
+  rounds += rounds & 1u;
+  if (rounds != colorRounds_) {
+    colorRounds_ = rounds;
+    resetGraphs();
+  }
+}
+
+bool Engine::adapt() {
+  Counters c = readCounters();
+  bool changed = false;
+  // Colour cap: grow when a spare runs out or the colouring clashed (solver.ts:1495-1511)
+  if (c.clashes > 0 || colorCap_ - std::min(colorCap_, c.colors) < 2) {
+    uint32_t want = std::min<uint32_t>(MAX_COLORS, std::max<uint32_t>(c.colors + 5, colorCap_ + 4));
+    if (want != colorCap_) {
+      setColorCap(want);
+      changed = true;
+    }
+    if (c.clashes > 0 && colorRounds_ < 32) setColorRounds(std::min<uint32_t>(32, colorRounds_ * 2));
+  }
+  // Storage: grow to twice the demand when full beyond 80% or overflowed (solver.ts:1512-1518)
+  bool overflow = (c.overflow & 7u) != 0;
+  if (overflow || c.pairs > 0.8 * pairCap_) {
+    uint32_t want = uint32_t(std::max<double>(2.0 * std::max(c.pairs, pairCap_), 1.25 * pairCap_));
+    allocateContacts(want);
+    changed = true;
+  }
+  return changed;
+}
+
+}  // namespace avbd

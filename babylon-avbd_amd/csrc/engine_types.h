@@ -1,0 +1,67 @@
+// The device-side view of a batch of worlds (a single world is a batch of one): pointers into HBM and sizes, passed by
+// value to every kernel. Worlds occupy contiguous body and joint ranges; per-step products (pairs, manifolds, contacts,
+// adjacency, colour buckets) are shared arrays over the whole batch, so one launch per phase steps every world.
+#pragma once
+#include "avbd_device.h"
+
+namespace avbd {
+
+struct DeviceWorld {
+  // Scene
+  Body *bodies;
+  uint32_t *bodyWorld;   // world of each body
+  WorldParams *worlds;
+  float4 *refs;          // 2 per body: reference pos, reference rot (contact reuse)
+  uint2 *filters;        // groups, collides-with
+  Joint *joints;
+  int4 *info;            // type, a (-1 = world), b, 0
+  uint32_t *noCollide;   // (hi, lo, joint slot or ~0) triples, sorted by (hi, lo)
+  uint32_t nBodies, nJoints, nWorlds, noCollideCount;
+  uint32_t *counters;    // COUNTER_WORDS
+  uint32_t *scanScratch; // block sums of the multi-level prefix scan
+
+  // Uniform-grid broadphase
+  uint32_t *gridCount;   // tableSize + 1 (counts, then bucket starts after the scan)
+  uint32_t *gridCursor;  // tableSize
+  uint32_t *gridSorted;  // nBodies
+  int4 *cellOf;          // per body: cell xyz, world (w = -1: large body)
+  uint32_t tableMask;
+  float cellSize;
+  uint32_t *largeStart;  // nWorlds + 1
+  uint32_t *largeList;
+
+  // Pairs: per-body counts (scanned into offsets, nBodies + 1), the pair list (a > b)
+  uint32_t *pairCount;
+  uint2 *pairs;
+  uint32_t pairCapacity;
+
+  // Manifolds: per pair slot (narrowphase output), compacted list (current), last step's list, scan scratch
+  Manifold *mfSlots;
+  uint32_t *mfOffset;  // pairCapacity + 1
+  Manifold *manifolds;
+  Manifold *prevManifolds;
+  Contact *contacts;      // 8 per pair slot
+  Contact *prevContacts;
+  uint32_t *table;        // last step's manifolds by pair hash (value = index + 1)
+  uint32_t hashMask;
+
+  // Adjacency: degree/start (nBodies + 1), fill cursor (nBodies), list
+  uint32_t *adjStart;
+  uint32_t *adjFill;
+  uint32_t *adjList;
+  uint32_t adjCapacity;
+
+  // Colouring: state A, state B, rank, histogram (64 × groups + 1), starts (65), bucketed bodies
+  uint32_t *colorA;
+  uint32_t *colorB;
+  uint32_t *colorRank;
+  uint32_t *colorHist;
+  uint32_t *colorStart;
+  uint32_t *colorBodies;
+  uint32_t colorGroups;
+  uint32_t colorCap;
+};
+
+constexpr int COLOR_WG = 256;
+
+}  // namespace avbd

@@ -1,0 +1,34 @@
+// Launch wrappers of the step's kernels (each .hip file launches its own kernels on the given stream).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine_types.h"
+
+namespace avbd {
+
+/** In-place exclusive prefix sum of n u32 values; scratch holds at least scan_scratch_words(n) words. */
+void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s);
+uint32_t scan_scratch_words(uint32_t n);
+/** Fill n words with a value (a kernel, so it can be captured in a graph). */
+void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s);
+
+// Collision
+void launch_begin_frame(const DeviceWorld &w, hipStream_t s);
+void launch_update_refs(const DeviceWorld &w, hipStream_t s);
+void launch_hash_build(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s);
+void launch_grid(const DeviceWorld &w, hipStream_t s);
+void launch_pairs(const DeviceWorld &w, hipStream_t s);
+void launch_narrowphase(const DeviceWorld &w, hipStream_t s);
+
+// Topology and colouring
+void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s);
+
+// Solve
+void launch_warm_start(const DeviceWorld &w, hipStream_t s);
+void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s);
+void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
+void launch_velocities(const DeviceWorld &w, hipStream_t s);
+
+}  // namespace avbd

@@ -1,0 +1,420 @@
+// The AVBD iterations on gfx950: joint and body warm starts, one primal pass per colour (each body folds its joints'
+// and contacts' rows into a 6x6 Newton system held in registers and solves it by LDLᵀ with the 1e-5 relative pivot
+// floor), the dual/penalty ramp over joints and contact pairs, and the BDF1 velocities.
+// Semantics: reference/three-avbd/src/avbd3d/gpu/wgsl-solve.ts (addRow 53-59, addJoint 90-140, evalContact 171-206,
+// warmStartJoints 210-226, warmStartBodies 259-300, primal/accumulate/finishBody 314-470, dual 474-535,
+// updateVelocities 537-549). A step whose contact storage overflowed is held (every kernel here leaves the world
+// untouched), as the reference's protectSolverStep does (src/gpu/gpuSolverKernels.js:72-103).
+#include "engine_types.h"
+#include "kernels.h"
+
+namespace avbd {
+
+AVBD_D bool held(const DeviceWorld &w) { return w.counters[C_OVERFLOW] != 0u; }
+
+// --- joints ------------------------------------------------------------------------------------------------------------
+
+AVBD_D f3 anchorA(const DeviceWorld &w, const Joint &k, int a) {
+  if (a < 0) return xyz(k.rA);
+  return add(qrotate(w.bodies[a].rot, xyz(k.rA)), xyz(w.bodies[a].pos));
+}
+AVBD_D float4 rotA(const DeviceWorld &w, int a) {
+  return a < 0 ? make_float4(0.f, 0.f, 0.f, 1.f) : w.bodies[a].rot;
+}
+AVBD_D f3 jointLinC(const DeviceWorld &w, const Joint &k, int a, int b) {
+  return sub(anchorA(w, k, a), add(qrotate(w.bodies[b].rot, xyz(k.rB)), xyz(w.bodies[b].pos)));
+}
+AVBD_D f3 jointAngC(const DeviceWorld &w, const Joint &k, int a, int b) {
+  return scale(qsub(rotA(w, a), w.bodies[b].rot), k.lamAng.w);
+}
+AVBD_D f3 jointLinRows(const DeviceWorld &w, const Joint &k, int a, int b, float alpha) {
+  return sub(jointLinC(w, k, a, b), k.penLin.w >= HARD ? scale(xyz(k.c0Lin), alpha) : mk3(0.f, 0.f, 0.f));
+}
+AVBD_D f3 jointAngRows(const DeviceWorld &w, const Joint &k, int a, int b, float alpha) {
+  return sub(jointAngC(w, k, a, b), k.penAng.w >= HARD ? scale(xyz(k.c0Ang), alpha) : mk3(0.f, 0.f, 0.f));
+}
+AVBD_D f3 penClamp(f3 pen, float gamma, float stiffness) {
+  return mk3(rmin(wclamp(pen.x * gamma, PENALTY_MIN, PENALTY_MAX), stiffness),
+             rmin(wclamp(pen.y * gamma, PENALTY_MIN, PENALTY_MAX), stiffness),
+             rmin(wclamp(pen.z * gamma, PENALTY_MIN, PENALTY_MAX), stiffness));
+}
+
+__global__ void k_warm_joints(DeviceWorld w) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= w.nJoints || held(w)) return;
+  int4 f = w.info[j];
+  if (f.x != T_JOINT) return;
+  Joint k = w.joints[j];
+  const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
+  k.c0Lin = mk4(jointLinC(w, k, f.y, f.z), 0.f);
+  k.c0Ang = mk4(jointAngC(w, k, f.y, f.z), 0.f);
+  float decay = p.alpha * p.gamma;
+  k.lamLin = mk4(scale(xyz(k.lamLin), decay), k.lamLin.w);
+  k.lamAng = mk4(scale(xyz(k.lamAng), decay), k.lamAng.w);
+  k.penLin = mk4(penClamp(xyz(k.penLin), p.gamma, k.penLin.w), k.penLin.w);
+  k.penAng = mk4(penClamp(xyz(k.penAng), p.gamma, k.penAng.w), k.penAng.w);
+  w.joints[j] = k;
+}
+
+__global__ void k_warm_bodies(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies || held(w)) return;
+  const WorldParams &p = w.worlds[w.bodyWorld[i]];
+  Body &B = w.bodies[i];
+  float4 pos = B.pos, rot = B.rot, vel = B.vel;
+  f3 angVel = xyz(B.angVel);
+  float dt = p.dt, g = p.gravity;
+  f3 up = mk3(p.up[0], p.up[1], p.up[2]);
+  bool dyn = B.size.w > 0.f;
+  f3 inertialPos = add(xyz(pos), scale(xyz(vel), dt));
+  if (dyn) inertialPos = add(inertialPos, scale(up, g * (dt * dt)));
+  B.inertialPos = mk4(inertialPos, B.inertialPos.w);
+  B.inertialRot = qadd(rot, scale(angVel, dt));
+  float wgt = 0.f;
+  if (fabsf(g) > 0.f) {
+    float x = (dot(xyz(vel), up) - vel.w) / dt * rsign(g) / fabsf(g);
+    wgt = rmin(rmax(x, 0.f), 1.f);
+  }
+  bool slow = length(xyz(vel)) + length(angVel) * B.moment.w < REST_SPEED;
+  bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
+  B.ipos = pos;
+  B.irot = rot;
+  if (dyn && !atRest) {
+    B.pos = mk4(add(add(xyz(pos), scale(xyz(vel), dt)), scale(up, g * (wgt * dt * dt))), pos.w);
+    B.rot = qadd(rot, scale(angVel, dt));
+  }
+  bool moving = vel.x != 0.f || vel.y != 0.f || vel.z != 0.f || angVel.x != 0.f || angVel.y != 0.f || angVel.z != 0.f;
+  if (!dyn && moving) {
+    B.pos = mk4(add(xyz(pos), scale(xyz(vel), dt)), pos.w);
+    B.rot = qadd(rot, scale(angVel, dt));
+  }
+}
+
+// --- primal --------------------------------------------------------------------------------------------------------
+
+/** One body's Newton system: lower triangles of the linear and angular blocks, the full cross block (row = angular). */
+struct Acc {
+  float l00, l10, l11, l20, l21, l22;
+  float a00, a10, a11, a20, a21, a22;
+  float c[9];
+  f3 rLin, rAng;
+};
+
+/** addRow (wgsl-solve.ts:53-59): lin += l lᵀ k, ang += a aᵀ k, cross += a lᵀ k, rhs += (l, a) f. */
+AVBD_D void addRow(Acc &s, f3 l, f3 a, float k, float f) {
+  s.l00 += l.x * l.x * k;
+  s.l10 += l.y * l.x * k;
+  s.l11 += l.y * l.y * k;
+  s.l20 += l.z * l.x * k;
+  s.l21 += l.z * l.y * k;
+  s.l22 += l.z * l.z * k;
+  s.a00 += a.x * a.x * k;
+  s.a10 += a.y * a.x * k;
+  s.a11 += a.y * a.y * k;
+  s.a20 += a.z * a.x * k;
+  s.a21 += a.z * a.y * k;
+  s.a22 += a.z * a.z * k;
+  s.c[0] += a.x * l.x * k;
+  s.c[1] += a.x * l.y * k;
+  s.c[2] += a.x * l.z * k;
+  s.c[3] += a.y * l.x * k;
+  s.c[4] += a.y * l.y * k;
+  s.c[5] += a.y * l.z * k;
+  s.c[6] += a.z * l.x * k;
+  s.c[7] += a.z * l.y * k;
+  s.c[8] += a.z * l.z * k;
+  s.rLin = add(s.rLin, scale(l, f));
+  s.rAng = add(s.rAng, scale(a, f));
+}
+
+AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, uint32_t i) {
+  int4 f = w.info[j];
+  int a = f.y, b = f.z;
+  const Joint k = w.joints[j];
+  bool isA = int(i) == a;
+  float sg = isA ? 1.f : -1.f;
+  if (f.x == T_SPRING) {
+    f3 rAW = qrotate(w.bodies[a].rot, xyz(k.rA));
+    f3 rBW = qrotate(w.bodies[b].rot, xyz(k.rB));
+    f3 d = sub(add(rAW, xyz(w.bodies[a].pos)), add(rBW, xyz(w.bodies[b].pos)));
+    float len = length(d);
+    if (len <= 1.0e-6f) return;
+    f3 n = divs(d, len);
+    float stiffness = k.penLin.w;
+    f3 r = isA ? rAW : rBW;
+    addRow(acc, scale(n, sg), scale(cross(r, n), sg), stiffness, stiffness * (len - k.rA.w));
+    return;
+  }
+  f3 penLin = xyz(k.penLin), penAng = xyz(k.penAng);
+  if (dot(penLin, penLin) > 0.f) {
+    f3 F = add(mul3(penLin, jointLinRows(w, k, a, b, alpha)), xyz(k.lamLin));
+    f3 r = isA ? qrotate(w.bodies[a].rot, xyz(k.rA)) : neg(qrotate(w.bodies[b].rot, xyz(k.rB)));
+    f3 v = neg(r);
+    addRow(acc, mk3(sg, 0.f, 0.f), mk3(0.f, -v.z, v.y), penLin.x, F.x);
+    addRow(acc, mk3(0.f, sg, 0.f), mk3(v.z, 0.f, -v.x), penLin.y, F.y);
+    addRow(acc, mk3(0.f, 0.f, sg), mk3(-v.y, v.x, 0.f), penLin.z, F.z);
+    float rf = dot(r, F);
+    acc.a00 += length(sub(scale(r, F.x), mk3(rf, 0.f, 0.f)));
+    acc.a11 += length(sub(scale(r, F.y), mk3(0.f, rf, 0.f)));
+    acc.a22 += length(sub(scale(r, F.z), mk3(0.f, 0.f, rf)));
+  }
+  if (dot(penAng, penAng) > 0.f) {
+    f3 F = add(mul3(penAng, jointAngRows(w, k, a, b, alpha)), xyz(k.lamAng));
+    float s = sg * k.lamAng.w;
+    addRow(acc, mk3(0.f, 0.f, 0.f), mk3(s, 0.f, 0.f), penAng.x, F.x);
+    addRow(acc, mk3(0.f, 0.f, 0.f), mk3(0.f, s, 0.f), penAng.y, F.y);
+    addRow(acc, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, s), penAng.z, F.z);
+  }
+}
+
+/** What a contact needs of one body: rotation, x- rotation, displacement since x- (wgsl-solve.ts:145-157). */
+struct PairBody {
+  float4 rot, irot;
+  f3 dLin, dAng;
+};
+AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
+  const Body &B = w.bodies[i];
+  PairBody pb;
+  pb.rot = B.rot;
+  pb.irot = B.irot;
+  pb.dLin = sub(xyz(B.pos), xyz(B.ipos));
+  pb.dAng = qsub(pb.rot, pb.irot);
+  return pb;
+}
+
+struct Eval {
+  f3 rAW, rBW, C, F;
+  float frictionScale, bounds;
+};
+
+AVBD_D float rowC(float c0, float keep, f3 l, const Eval &e, const PairBody &A, const PairBody &B) {
+  return c0 * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
+}
+
+/** evalContact (wgsl-solve.ts:171-206). */
+AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, const PairBody &A, const PairBody &B,
+                        float alpha) {
+  uint32_t key = fbits(k.rA.w);
+  bool sphere = (key & ~STICK_BIT) == SPHERE_FEATURE;
+  Eval e;
+  e.rAW = qrotate(sphere ? A.irot : A.rot, xyz(k.rA));
+  e.rBW = qrotate(sphere ? B.irot : B.rot, xyz(k.rB));
+  float keep = 1.f - alpha;
+  e.C = mk3(rowC(k.rB.w, keep, basis.n, e, A, B), rowC(k.pen.w, keep, basis.t1, e, A, B),
+            rowC(k.lam.w, keep, basis.t2, e, A, B));
+  f3 F = add(mul3(xyz(k.pen), e.C), xyz(k.lam));
+  F.x = rmin(F.x, 0.f);
+  e.bounds = fabsf(F.x) * friction;
+  e.frictionScale = sqrtf(F.y * F.y + F.z * F.z);
+  if (e.frictionScale > e.bounds && e.frictionScale > 0.f) {
+    float s = e.bounds / e.frictionScale;
+    F.y = F.y * s;
+    F.z = F.z * s;
+  }
+  e.F = F;
+  return e;
+}
+
+AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, float alpha, float dt, Acc &acc) {
+  const Body &B = w.bodies[i];
+  float dt2 = dt * dt;
+  float m = B.size.w / dt2;
+  f3 I = divs(xyz(B.moment), dt2);
+  acc.l00 = m; acc.l10 = 0.f; acc.l11 = m; acc.l20 = 0.f; acc.l21 = 0.f; acc.l22 = m;
+  acc.a00 = I.x; acc.a10 = 0.f; acc.a11 = I.y; acc.a20 = 0.f; acc.a21 = 0.f; acc.a22 = I.z;
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc.c[q] = 0.f;
+  acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
+  acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
+  const uint32_t J = w.nJoints;
+  for (uint32_t e = w.adjStart[i]; e < w.adjStart[i + 1]; e++) {
+    uint32_t id = w.adjList[e];
+    if (id < J) {
+      addJoint(w, acc, id, alpha, i);
+      continue;
+    }
+    Manifold mf = w.manifolds[id - J];
+    bool isA = i == mf.ids.x;
+    PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
+    Basis basis = orthonormal(xyz(mf.geo));
+    float sg = isA ? 1.f : -1.f;
+    f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
+    uint32_t cEnd = mf.ids.z + (mf.ids.w & 15u);
+    for (uint32_t c = mf.ids.z; c < cEnd; c++) {
+      Contact k = w.contacts[c];
+      Eval ev = evalContact(k, basis, mf.geo.w, PA, PB, alpha);
+      f3 r = isA ? ev.rAW : ev.rBW;
+      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+    }
+  }
+}
+
+/** LDLᵀ of the 6x6 system with each pivot floored at PIVOT_FLOOR of its diagonal, then x -= A⁻¹ rhs (wgsl-solve.ts:416-470). */
+AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
+  const float A11 = s.l00;
+  const float A21 = s.l10, A22 = s.l11;
+  const float A31 = s.l20, A32 = s.l21, A33 = s.l22;
+  const float A41 = s.c[0], A42 = s.c[1], A43 = s.c[2], A44 = s.a00;
+  const float A51 = s.c[3], A52 = s.c[4], A53 = s.c[5], A54 = s.a10, A55 = s.a11;
+  const float A61 = s.c[6], A62 = s.c[7], A63 = s.c[8], A64 = s.a20, A65 = s.a21, A66 = s.a22;
+  const float L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
+  const float D1 = A11;
+  const float D2 = rmax(A22 - L21 * L21 * D1, A22 * PIVOT_FLOOR);
+  const float L32 = (A32 - L21 * L31 * D1) / D2;
+  const float L42 = (A42 - L21 * L41 * D1) / D2;
+  const float L52 = (A52 - L21 * L51 * D1) / D2;
+  const float L62 = (A62 - L21 * L61 * D1) / D2;
+  const float D3 = rmax(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33 * PIVOT_FLOOR);
+  const float L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
+  const float L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
+  const float L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+  const float D4 = rmax(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44 * PIVOT_FLOOR);
+  const float L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
+  const float L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+  const float D5 = rmax(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55 * PIVOT_FLOOR);
+  const float L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+  const float D6 = rmax(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5),
+                        A66 * PIVOT_FLOOR);
+  const float y1 = s.rLin.x;
+  const float y2 = s.rLin.y - L21 * y1;
+  const float y3 = s.rLin.z - L31 * y1 - L32 * y2;
+  const float y4 = s.rAng.x - L41 * y1 - L42 * y2 - L43 * y3;
+  const float y5 = s.rAng.y - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+  const float y6 = s.rAng.z - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+  const float z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
+  const float w3 = z6;
+  const float w2 = z5 - L65 * w3;
+  const float w1 = z4 - L54 * w2 - L64 * w3;
+  const float v3 = z3 - L43 * w1 - L53 * w2 - L63 * w3;
+  const float v2 = z2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
+  const float v1 = z1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
+  Body &B = w.bodies[i];
+  float4 pos = B.pos;
+  B.pos = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
+  B.rot = qadd(B.rot, mk3(-w1, -w2, -w3));
+}
+
+__global__ void __launch_bounds__(64) k_primal(DeviceWorld w, uint32_t color) {
+  if (held(w)) return;
+  uint32_t start = w.colorStart[color];
+  uint32_t count = w.colorStart[color + 1] - start;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+    uint32_t i = w.colorBodies[start + k];
+    const WorldParams &p = w.worlds[w.bodyWorld[i]];
+    Acc acc;
+    accumulate(w, i, p.alpha, p.dt, acc);
+    finishBody(w, i, acc);
+  }
+}
+
+// --- dual --------------------------------------------------------------------------------------------------------------
+
+AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
+  int4 f = w.info[j];
+  if (f.x != T_JOINT) return;
+  int a = f.y, b = f.z;
+  const WorldParams &p = w.worlds[w.bodyWorld[b]];
+  float alpha = p.alpha;
+  Joint k = w.joints[j];
+  f3 penLin = xyz(k.penLin), penAng = xyz(k.penAng);
+  if (dot(penLin, penLin) > 0.f) {
+    f3 C = jointLinRows(w, k, a, b, alpha);
+    if (k.penLin.w >= HARD) k.lamLin = mk4(add(mul3(penLin, C), xyz(k.lamLin)), k.lamLin.w);
+    float cap = rmin(k.penLin.w, PENALTY_MAX);
+    f3 q = add(penLin, scale(vabs(C), p.betaLin));
+    k.penLin = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penLin.w);
+  }
+  if (dot(penAng, penAng) > 0.f) {
+    f3 C = jointAngRows(w, k, a, b, alpha);
+    if (k.penAng.w >= HARD) k.lamAng = mk4(add(mul3(penAng, C), xyz(k.lamAng)), k.lamAng.w);
+    float cap = rmin(k.penAng.w, PENALTY_MAX);
+    f3 q = add(penAng, scale(vabs(C), p.betaAng));
+    k.penAng = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penAng.w);
+  }
+  float frac = k.lamLin.w;
+  float limit = fabsf(frac);
+  f3 ll = xyz(k.lamLin), la = xyz(k.lamAng);
+  bool linear = frac < 0.f && dot(ll, ll) > limit * limit;
+  if (limit < BIG && (dot(la, la) > limit * limit || linear)) {
+    k.penLin = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.penAng = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.lamLin = make_float4(0.f, 0.f, 0.f, frac);
+    k.lamAng = make_float4(0.f, 0.f, 0.f, k.lamAng.w);
+  }
+  w.joints[j] = k;
+}
+
+AVBD_D void dualManifold(const DeviceWorld &w, uint32_t m) {
+  Manifold mf = w.manifolds[m];
+  const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
+  PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
+  Basis basis = orthonormal(xyz(mf.geo));
+  uint32_t end = mf.ids.z + (mf.ids.w & 15u);
+  for (uint32_t c = mf.ids.z; c < end; c++) {
+    Contact k = w.contacts[c];
+    Eval e = evalContact(k, basis, mf.geo.w, PA, PB, p.alpha);
+    f3 pen = xyz(k.pen);
+    if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
+    uint32_t key = fbits(k.rA.w);
+    if (e.frictionScale <= e.bounds) {
+      pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
+      pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
+      bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
+      key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+      w.contacts[c].rA.w = bitsf(key);
+    }
+    w.contacts[c].pen = mk4(pen, k.pen.w);
+    w.contacts[c].lam = mk4(e.F, k.lam.w);
+  }
+}
+
+__global__ void k_dual(DeviceWorld w) {
+  if (held(w)) return;
+  uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+    if (id < w.nJoints) dualJoint(w, id);
+    else dualManifold(w, id - w.nJoints);
+  }
+}
+
+__global__ void k_velocities(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies || held(w)) return;
+  const WorldParams &p = w.worlds[w.bodyWorld[i]];
+  Body &B = w.bodies[i];
+  f3 up = mk3(p.up[0], p.up[1], p.up[2]);
+  float prevUp = dot(xyz(B.vel), up);
+  if (B.size.w > 0.f) {
+    B.vel = mk4(divs(sub(xyz(B.pos), xyz(B.ipos)), p.dt), prevUp);
+    B.angVel = mk4(divs(qsub(B.rot, B.irot), p.dt), B.angVel.w);
+  } else {
+    B.vel.w = prevUp;
+  }
+}
+
+// --- launch wrappers ---------------------------------------------------------------------------------------------------
+
+static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) {
+  uint64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return uint32_t(g);
+}
+
+void launch_warm_start(const DeviceWorld &w, hipStream_t s) {
+  if (w.nJoints) k_warm_joints<<<blocks(w.nJoints, 256), 256, 0, s>>>(w);
+  k_warm_bodies<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+}
+void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s) {
+  k_primal<<<blocks(maxColorBodies, 64, 4096), 64, 0, s>>>(w, color);
+}
+void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
+  k_dual<<<blocks(uint64_t(w.nJoints) + manifoldCapacity, 64, 8192), 64, 0, s>>>(w);
+}
+void launch_velocities(const DeviceWorld &w, hipStream_t s) {
+  k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+}
+
+}  // namespace avbd

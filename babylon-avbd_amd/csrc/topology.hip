@@ -1,0 +1,253 @@
+// Per-step topology on gfx950: per-body constraint adjacency (degree → scan → fill → per-body sort into a fixed order, so
+// the primal sums a body's rows in the same order every run) and incremental Jones–Plassmann colouring with lowbias32
+// priorities, colour cap and clash counting, then stable per-colour buckets.
+// Semantics: reference/three-avbd/src/avbd2d/gpu/wgsl-topology.ts:61-322 (with the 3D accessors of
+// reference/three-avbd/src/avbd3d/gpu/layout.ts:245-269). Priorities use a body's index within its world, so a world
+// colours the same way alone and inside a batch.
+#include "engine_types.h"
+#include "kernels.h"
+
+namespace avbd {
+
+AVBD_D bool dynamicBody(const DeviceWorld &w, int i) { return i >= 0 && w.bodies[i].size.w > 0.f; }
+
+AVBD_D bool jointActive(const DeviceWorld &w, uint32_t j) {
+  return w.info[j].x != T_NONE && (w.joints[j].penLin.w != 0.f || w.joints[j].penAng.w != 0.f);
+}
+
+AVBD_D int2 endpoints(const DeviceWorld &w, uint32_t id) {
+  if (id < w.nJoints) {
+    int4 f = w.info[id];
+    return make_int2(f.y, f.z);
+  }
+  uint4 ids = w.manifolds[id - w.nJoints].ids;
+  return make_int2(int(ids.x), int(ids.y));
+}
+
+// --- adjacency ---------------------------------------------------------------------------------------------------------
+
+__global__ void k_adj_degree(DeviceWorld w) {
+  uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+    if (id < w.nJoints && !jointActive(w, id)) continue;
+    int2 e = endpoints(w, id);
+    if (dynamicBody(w, e.x)) atomicAdd(&w.adjStart[e.x], 1u);
+    if (dynamicBody(w, e.y)) atomicAdd(&w.adjStart[e.y], 1u);
+  }
+}
+
+__global__ void k_adj_fill(DeviceWorld w) {
+  uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+    if (id < w.nJoints && !jointActive(w, id)) continue;
+    int2 e = endpoints(w, id);
+    int ends[2] = {e.x, e.y};
+    for (int s = 0; s < 2; s++) {
+      int x = ends[s];
+      if (!dynamicBody(w, x)) continue;
+      uint32_t slot = w.adjStart[x] + atomicAdd(&w.adjFill[x], 1u);
+      if (slot < w.adjCapacity) w.adjList[slot] = id;
+    }
+  }
+}
+
+/** Joints first by id, then contact pairs by the other body (wgsl-topology.ts:114-123; one manifold per pair in 3D). */
+AVBD_D uint64_t adjKey(const DeviceWorld &w, uint32_t b, uint32_t id) {
+  if (id < w.nJoints) return uint64_t(id);
+  int2 e = endpoints(w, id);
+  int other = e.x == int(b) ? e.y : e.x;
+  return uint64_t(uint32_t(other + 2)) << 32;
+}
+
+__global__ void k_adj_sort(DeviceWorld w) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= w.nBodies) return;
+  uint32_t lo = w.adjStart[b], hi = w.adjStart[b + 1];
+  if (hi > w.adjCapacity) hi = w.adjCapacity;
+  for (uint32_t i = lo + 1; i < hi; i++) {
+    uint32_t id = w.adjList[i];
+    uint64_t key = adjKey(w, b, id);
+    uint32_t j = i;
+    while (j > lo) {
+      uint32_t prev = w.adjList[j - 1];
+      if (!(key < adjKey(w, b, prev))) break;
+      w.adjList[j] = prev;
+      j--;
+    }
+    w.adjList[j] = id;
+  }
+}
+
+// --- colouring ---------------------------------------------------------------------------------------------------------
+
+AVBD_D uint32_t priorityOf(const DeviceWorld &w, uint32_t i) {
+  return hash32(i - w.worlds[w.bodyWorld[i]].firstBody + 0x9e3779b9u);
+}
+/** i beats j: higher priority, ties by index (within a world, local and global index order agree). */
+AVBD_D bool beats(const DeviceWorld &w, uint32_t i, uint32_t j) {
+  uint32_t pi = priorityOf(w, i), pj = priorityOf(w, j);
+  return pi > pj || (pi == pj && i > j);
+}
+
+AVBD_D int neighbour(const DeviceWorld &w, uint32_t b, uint32_t e) {
+  int2 ends = endpoints(w, w.adjList[e]);
+  int other = ends.x == int(b) ? ends.y : ends.x;
+  return dynamicBody(w, other) ? other : -1;
+}
+
+AVBD_D uint32_t smallestFree(const DeviceWorld &w, uint32_t b, const uint32_t *src, bool skipPending) {
+  uint32_t lo = 0, hi = 0;
+  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1]; e++) {
+    int j = neighbour(w, b, e);
+    if (j < 0) continue;
+    uint32_t s = src[j];
+    if (skipPending && (s & PENDING)) continue;
+    uint32_t col = s & 0xffu;
+    if (col == NO_COLOR) continue;
+    if (col < 32) lo |= 1u << col;
+    else hi |= 1u << (col - 32);
+  }
+  if (~lo != 0u) return uint32_t(__builtin_ctz(~lo));
+  if (~hi != 0u) return 32u + uint32_t(__builtin_ctz(~hi));
+  return MAX_COLORS - 1;
+}
+
+// 1. Compaction: a body whose priority beats all neighbours may drop to a smaller free colour (A → B)
+__global__ void k_color_compact(DeviceWorld w) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= w.nBodies) return;
+  if (!dynamicBody(w, int(b))) {
+    w.colorB[b] = NO_COLOR;
+    return;
+  }
+  uint32_t col = w.colorA[b] & 0xffu;
+  if (col != NO_COLOR) {
+    bool isMax = true;
+    for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
+      int j = neighbour(w, b, e);
+      if (j >= 0 && !beats(w, b, uint32_t(j))) isMax = false;
+    }
+    if (isMax) col = umin(col, smallestFree(w, b, w.colorA, false));
+  }
+  w.colorB[b] = col;
+}
+
+// 2. Pending: no colour, or the same colour as a higher-priority neighbour (B → A)
+__global__ void k_color_mark(DeviceWorld w) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= w.nBodies) return;
+  uint32_t col = w.colorB[b] & 0xffu;
+  if (!dynamicBody(w, int(b))) {
+    w.colorA[b] = NO_COLOR;
+    return;
+  }
+  bool pending = col == NO_COLOR;
+  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && !pending; e++) {
+    int j = neighbour(w, b, e);
+    if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(w, uint32_t(j), b)) pending = true;
+  }
+  w.colorA[b] = col | (pending ? PENDING : 0u);
+}
+
+// 3. Jones–Plassmann round: a pending body beating all pending neighbours takes the smallest colour its settled
+// neighbours leave free; none left below the cap: the last colour, still pending (a counted clash)
+__global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= w.nBodies) return;
+  uint32_t s = src[b];
+  if (!(s & PENDING)) {
+    dst[b] = s;
+    return;
+  }
+  bool isMax = true;
+  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
+    int j = neighbour(w, b, e);
+    if (j >= 0 && (src[j] & PENDING) && !beats(w, b, uint32_t(j))) isMax = false;
+  }
+  if (isMax) {
+    uint32_t fr = smallestFree(w, b, src, true);
+    dst[b] = fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
+  } else {
+    dst[b] = s;
+  }
+}
+
+// 4. Per-chunk histogram and each body's rank within its chunk's colour (keeps index order inside a bucket)
+__global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
+  __shared__ uint32_t hist[MAX_COLORS];
+  __shared__ uint32_t chunkColors[COLOR_WG];
+  uint32_t lid = threadIdx.x;
+  if (lid < MAX_COLORS) hist[lid] = 0;
+  uint32_t b = blockIdx.x * COLOR_WG + lid;
+  uint32_t col = NO_COLOR;
+  if (b < w.nBodies && dynamicBody(w, int(b))) {
+    uint32_t s = w.colorA[b];
+    col = s & 0xffu;
+    if (s & PENDING) atomicAdd(&w.counters[C_CLASHES], 1u);
+    if (col == NO_COLOR) col = w.colorCap - 1;
+    w.colorA[b] = col;
+    atomicMax(&w.counters[C_NUM_COLORS], col + 1);
+  }
+  chunkColors[lid] = col;
+  __syncthreads();
+  if (col != NO_COLOR) {
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < lid; j++) rank += chunkColors[j] == col ? 1u : 0u;
+    w.colorRank[b] = rank;
+    atomicAdd(&hist[col], 1u);
+  }
+  __syncthreads();
+  if (lid < MAX_COLORS) w.colorHist[lid * w.colorGroups + blockIdx.x] = hist[lid];
+  if (blockIdx.x == 0 && lid == 0) w.colorHist[MAX_COLORS * w.colorGroups] = 0;
+}
+
+// 5. Colour c starts at its first chunk's offset
+__global__ void k_color_starts(DeviceWorld w) {
+  uint32_t c = threadIdx.x;
+  if (c < MAX_COLORS) w.colorStart[c] = w.colorHist[c * w.colorGroups];
+  if (c == 0) w.colorStart[MAX_COLORS] = w.colorHist[MAX_COLORS * w.colorGroups];
+}
+
+// 6. Scatter: chunk offset for the body's colour plus its rank
+__global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
+  uint32_t b = blockIdx.x * COLOR_WG + threadIdx.x;
+  if (b >= w.nBodies || !dynamicBody(w, int(b))) return;
+  uint32_t col = w.colorA[b] & 0xffu;
+  uint32_t slot = w.colorHist[col * w.colorGroups + blockIdx.x] + w.colorRank[b];
+  w.colorBodies[slot] = b;
+}
+
+// --- launch wrappers ---------------------------------------------------------------------------------------------------
+
+static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) {
+  uint64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return uint32_t(g);
+}
+
+void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
+  launch_fill_u32(w.adjStart, 0u, uint64_t(w.nBodies) + 1, s);
+  launch_fill_u32(w.adjFill, 0u, w.nBodies, s);
+  uint64_t items = uint64_t(w.nJoints) + manifoldCapacity;
+  k_adj_degree<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
+  launch_scan_exclusive(w.adjStart, w.nBodies + 1, w.scanScratch, s);
+  k_adj_fill<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
+  k_adj_sort<<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
+}
+
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s) {
+  uint32_t g = blocks(w.nBodies, 256);
+  k_color_compact<<<g, 256, 0, s>>>(w);
+  k_color_mark<<<g, 256, 0, s>>>(w);
+  for (uint32_t r = 0; r < rounds; r++) {
+    if (r % 2 == 0) k_color_round<<<g, 256, 0, s>>>(w, w.colorA, w.colorB);
+    else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA);
+  }
+  k_color_count<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+  launch_scan_exclusive(w.colorHist, MAX_COLORS * w.colorGroups + 1, w.scanScratch, s);
+  k_color_starts<<<1, 64, 0, s>>>(w);
+  k_color_scatter<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+}
+
+}  // namespace avbd

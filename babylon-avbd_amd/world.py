@@ -1,0 +1,338 @@
+"""Worlds and pipelines: the reference's createSolverPipeline (src/native/pipeline.js:14-93) with a "hip" backend whose
+worlds have the shape of its WebGPU worlds (src/native/solver3D.js:104-368, src/native/scenes.d.ts:317-454):
+step / advance / reads / adapt / destroy. A batch world steps many independent scenes in one launch per phase.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native
+from ._native import check, ptr
+from .scene import AvbdScene3D
+
+
+def _scene_desc(scene, keep):
+    """A SceneDesc for a built AvbdScene3D; `keep` holds the arrays alive for the call."""
+    bodies, joints, info, ignore, filters = scene.packed()
+    keep.extend([bodies, joints, info, ignore, filters])
+    d = _native.SceneDesc()
+    d.body_count = len(bodies)
+    d.bodies = ptr(bodies, ctypes.c_float)
+    d.joint_count = len(joints)
+    d.joints = ptr(joints, ctypes.c_float) if len(joints) else None
+    d.joint_info = ptr(info, ctypes.c_int32) if len(joints) else None
+    d.ignore_count = len(ignore)
+    d.ignore_pairs = ptr(ignore, ctypes.c_uint32) if len(ignore) else None
+    d.filters = ptr(filters, ctypes.c_uint32) if filters is not None else None
+    p = scene.params
+    d.dt = p["dt"]
+    d.gravity = p["gravity"]
+    d.up[:] = scene.up
+    d.alpha = p["alpha"]
+    d.gamma = p["gamma"]
+    d.beta_lin = p["betaLin"]
+    d.beta_ang = p["betaAng"]
+    d.iterations = p["iterations"]
+    d.flags = p["flags"]
+    return d
+
+
+def _options(options):
+    o = _native.Options()
+    o.color_cap = int(options.get("colorCap", 0))
+    o.color_rounds = int(options.get("colorRounds", 0))
+    cap = options.get("capacity", {}) or {}
+    o.pair_capacity = int(cap.get("pairs", 0))
+    o.use_graph = 1 if options.get("graph", False) else 0
+    o.device = int(options.get("device", 0))
+    return o
+
+
+class HipWorld:
+    """A world (or a batch of worlds) on the GPU. Methods mirror AvbdGpuSolver3D (scenes.d.ts:317-454)."""
+
+    backend = "hip"
+
+    def __init__(self, scenes, options=None):
+        options = options or {}
+        lib = _native.load()
+        scenes = list(scenes)
+        if not scenes:
+            raise ValueError("An AVBD scene builder is required")
+        for s in scenes:
+            if not isinstance(s, AvbdScene3D):
+                raise TypeError("An AVBD scene builder is required")
+            s.assertEditable()
+        keep = []
+        descs = (_native.SceneDesc * len(scenes))(*[_scene_desc(s, keep) for s in scenes])
+        handle = ctypes.c_void_p()
+        check(lib.avbd_batch_create(descs, len(scenes), ctypes.byref(_options(options)), ctypes.byref(handle)))
+        self._lib = lib
+        self._h = handle
+        self.destroyed = False
+        self.scenes = scenes
+        for s in scenes:
+            s.created = True
+        self.world_first_body = np.cumsum([0] + [s.body_count for s in scenes])[:-1]
+        self.world_first_joint = np.cumsum([0] + [len(s._joints) for s in scenes])[:-1]
+        p = scenes[0].params
+        self.params = dict(p, up=list(scenes[0].up))
+        self.timeAccumulator = 0.0
+        n = ctypes.c_uint32()
+        check(lib.avbd_world_body_count(self._h, ctypes.byref(n)))
+        self.bodyCount = n.value
+        check(lib.avbd_world_joint_count(self._h, ctypes.byref(n)))
+        self.jointCount = n.value
+        self.worldCount = len(scenes)
+
+    # --- lifetime ---------------------------------------------------------------------------------------------------
+
+    def _alive(self):
+        if self.destroyed:
+            raise RuntimeError("The solver has been destroyed")
+
+    def destroy(self):
+        if self.destroyed:
+            return
+        check(self._lib.avbd_world_destroy(self._h))
+        self.destroyed = True
+
+    def __del__(self):
+        try:
+            if not self.destroyed and _native._lib is not None:
+                self._lib.avbd_world_destroy(self._h)
+                self.destroyed = True
+        except Exception:
+            pass
+
+    # --- stepping ---------------------------------------------------------------------------------------------------
+
+    def step(self, n=1):
+        """Submit n steps without waiting (solver3D.js:350-359)."""
+        self._alive()
+        check(self._lib.avbd_world_step(self._h, int(n)))
+
+    def advance(self, elapsed, maxSubSteps=6, maxFrameTime=0.05):
+        """Fixed-step accumulator; returns the steps submitted (runtime.js:255-272)."""
+        self._alive()
+        if not math.isfinite(elapsed) or elapsed < 0:
+            raise ValueError("elapsed must be finite nonnegative seconds")
+        if int(maxSubSteps) != maxSubSteps or maxSubSteps < 1 or not math.isfinite(maxFrameTime) or maxFrameTime <= 0:
+            raise ValueError("maxSubSteps must be a positive integer and maxFrameTime positive")
+        dt = self.params["dt"]
+        self.timeAccumulator = min(maxSubSteps * dt, self.timeAccumulator + min(maxFrameTime, elapsed))
+        count = 0
+        while self.timeAccumulator + 1e-12 >= dt and count < maxSubSteps:
+            self.step()
+            self.timeAccumulator = max(0.0, self.timeAccumulator - dt)
+            count += 1
+        return count
+
+    def synchronize(self):
+        self._alive()
+        check(self._lib.avbd_world_synchronize(self._h))
+
+    # --- reads ----------------------------------------------------------------------------------------------------
+
+    def readBodies(self):
+        """All bodies, 40 f32 each, in slot order (solver.ts:1421-1423)."""
+        self._alive()
+        out = np.zeros((self.bodyCount, 40), np.float32)
+        check(self._lib.avbd_world_read_bodies(self._h, 0, self.bodyCount, ptr(out, ctypes.c_float)))
+        return out
+
+    def writeBodies(self, first, records):
+        self._alive()
+        records = np.ascontiguousarray(records, np.float32).reshape(-1, 40)
+        check(self._lib.avbd_world_write_bodies(self._h, int(first), len(records), ptr(records, ctypes.c_float)))
+
+    def readSelectedBodies(self, indices, posesOnly=False):
+        """readSelectedBodies (solver3D.js:223-230): 40 or (posesOnly) 8 f32 per body."""
+        self._alive()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        out = np.zeros((len(idx), 8 if posesOnly else 40), np.float32)
+        check(self._lib.avbd_world_read_selected(self._h, ptr(idx, ctypes.c_uint32), len(idx), 1 if posesOnly else 0,
+                                                 ptr(out, ctypes.c_float)))
+        return out
+
+    def readJoints(self):
+        self._alive()
+        out = np.zeros((self.jointCount, 32), np.float32)
+        if self.jointCount:
+            check(self._lib.avbd_world_read_joints(self._h, 0, self.jointCount, ptr(out, ctypes.c_float)))
+        return out
+
+    def writeJoints(self, first, records):
+        self._alive()
+        records = np.ascontiguousarray(records, np.float32).reshape(-1, 32)
+        check(self._lib.avbd_world_write_joints(self._h, int(first), len(records), ptr(records, ctypes.c_float)))
+
+    def readCounters(self):
+        """{pairs, contacts, manifolds, joints, overflow, clashes, colors, sleeping} (world.js:672-692)."""
+        self._alive()
+        c = _native.Counters()
+        check(self._lib.avbd_world_read_counters(self._h, ctypes.byref(c)))
+        return dict(pairs=c.pairs, contacts=c.contacts, manifolds=c.manifolds, joints=c.joints, overflow=c.overflow,
+                    clashes=c.clashes, colors=c.colors, sleeping=0)
+
+    def readPairs(self):
+        """The last step's broadphase pairs (a > b) as an (n, 2) u32 array, in emission order."""
+        self._alive()
+        n = ctypes.c_uint32()
+        check(self._lib.avbd_world_read_pairs(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 2), np.uint32)
+        check(self._lib.avbd_world_read_pairs(self._h, ptr(out, ctypes.c_uint32), n.value, ctypes.byref(n)))
+        return out
+
+    def readContactRecords(self):
+        """The last step's manifolds (m, 8) u32 and contacts (c, 16) u32, contacts packed densely per manifold."""
+        self._alive()
+        m = ctypes.c_uint32()
+        c = ctypes.c_uint32()
+        check(self._lib.avbd_world_read_contacts(self._h, None, 0, ctypes.byref(m), None, 0, ctypes.byref(c)))
+        mf = np.zeros((m.value, 8), np.uint32)
+        ct = np.zeros((c.value, 16), np.uint32)
+        check(self._lib.avbd_world_read_contacts(self._h, ptr(mf, ctypes.c_uint32), m.value, ctypes.byref(m),
+                                                 ptr(ct, ctypes.c_uint32), c.value, ctypes.byref(c)))
+        return mf, ct
+
+    def readContactList(self):
+        """readContactList (solver.ts:1435-1469): one dict per contact point."""
+        mf, ct = self.readContactRecords()
+        out = []
+        for row in mf:
+            a, b, first, cw = (int(x) for x in row[:4])
+            normal = row[4:7].view(np.float32).tolist()
+            for c in ct[first:first + (cw & 15)]:
+                key = int(c[3])
+                f = c.view(np.float32)
+                out.append(dict(a=a, b=b, feature=key & 0x7FFFFFFF, stick=bool(key & 0x80000000),
+                                rA=f[0:3].tolist(), rB=f[4:7].tolist(), pen=f[8:11].tolist(), lam=f[12:15].tolist(),
+                                normal=normal))
+        return out
+
+    def readColors(self):
+        self._alive()
+        out = np.zeros(self.bodyCount, np.uint32)
+        check(self._lib.avbd_world_read_colors(self._h, ptr(out, ctypes.c_uint32)))
+        return out
+
+    def readAdjacency(self):
+        self._alive()
+        n = ctypes.c_uint32()
+        start = np.zeros(self.bodyCount + 1, np.uint32)
+        check(self._lib.avbd_world_read_adjacency(self._h, ptr(start, ctypes.c_uint32), None, 0, ctypes.byref(n)))
+        lst = np.zeros(n.value, np.uint32)
+        check(self._lib.avbd_world_read_adjacency(self._h, ptr(start, ctypes.c_uint32), ptr(lst, ctypes.c_uint32),
+                                                  n.value, ctypes.byref(n)))
+        return start, lst
+
+    def readRefs(self):
+        self._alive()
+        out = np.zeros((self.bodyCount, 8), np.float32)
+        check(self._lib.avbd_world_read_refs(self._h, ptr(out, ctypes.c_float)))
+        return out
+
+    def readSteps(self):
+        self._alive()
+        out = np.zeros(self.worldCount, np.uint32)
+        check(self._lib.avbd_world_read_steps(self._h, ptr(out, ctypes.c_uint32)))
+        return out
+
+    def readState(self):
+        """Everything the next step depends on: bodies, joints, refs, colours, last contacts, step counters."""
+        mf, ct = self.readContactRecords()
+        return dict(bodies=self.readBodies(), joints=self.readJoints(), refs=self.readRefs(), colors=self.readColors(),
+                    manifolds=mf, contacts=ct, steps=self.readSteps())
+
+    def writeState(self, state):
+        """seedFrom (solver.ts:905-948): bodies, joints and the persistent state of readState()."""
+        self._alive()
+        self.writeBodies(0, state["bodies"])
+        if self.jointCount:
+            self.writeJoints(0, state["joints"])
+        refs = np.ascontiguousarray(state["refs"], np.float32)
+        colors = np.ascontiguousarray(state["colors"], np.uint32)
+        mf = np.ascontiguousarray(state["manifolds"], np.uint32)
+        ct = np.ascontiguousarray(state["contacts"], np.uint32)
+        steps = np.ascontiguousarray(state["steps"], np.uint32)
+        check(self._lib.avbd_world_write_state(self._h, ptr(refs, ctypes.c_float), ptr(colors, ctypes.c_uint32),
+                                               len(mf), ptr(mf, ctypes.c_uint32) if len(mf) else None, len(ct),
+                                               ptr(ct, ctypes.c_uint32) if len(ct) else None,
+                                               ptr(steps, ctypes.c_uint32)))
+
+    # --- policy ---------------------------------------------------------------------------------------------------
+
+    def setColoring(self, colorCap, colorRounds=0):
+        self._alive()
+        check(self._lib.avbd_world_set_coloring(self._h, int(colorCap), int(colorRounds)))
+
+    def adapt(self, counters=None):
+        """Grow storage and the colour cap from the last step's counters (solver.ts:1484-1519)."""
+        self._alive()
+        changed = ctypes.c_int()
+        check(self._lib.avbd_world_adapt(self._h, ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def profileStep(self):
+        """One step with HIP events per phase and per primal/dual launch (ms)."""
+        self._alive()
+        out = np.zeros(10, np.float32)
+        check(self._lib.avbd_world_profile_step(self._h, ptr(out, ctypes.c_float)))
+        keys = ("total", "collision", "adjacency", "coloring", "warmStart", "primal", "dual", "velocities",
+                "primalLaunches", "dualLaunches")
+        return {k: float(v) for k, v in zip(keys, out)}
+
+    def deviceBytes(self):
+        v = ctypes.c_uint64()
+        check(self._lib.avbd_world_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def gpuIndex(self, i):
+        """Slot of the scene builder's body i (bodies keep builder order)."""
+        return int(i)
+
+    bodyIndex = gpuIndex
+
+
+def createSolverPipeline(backend="hip", device=0):
+    """createSolverPipeline({backend}) (pipeline.js:14-93) with the "hip" backend."""
+    if backend != "hip":
+        raise ValueError(f"Unsupported solver backend: {backend}")
+    return SolverPipeline(device)
+
+
+class SolverPipeline:
+    def __init__(self, device=0):
+        self.backend = "hip"
+        self.device = int(device)
+        self.worlds = set()
+        self.destroyed = False
+
+    def createSolver(self, scene, options=None):
+        if self.destroyed:
+            raise RuntimeError("The solver pipeline has been destroyed")
+        opts = dict(options or {})
+        opts.setdefault("device", self.device)
+        w = HipWorld([scene], opts)
+        self.worlds.add(w)
+        return w
+
+    def createBatch(self, scenes, options=None):
+        """Independent worlds stepped together, one launch per phase."""
+        if self.destroyed:
+            raise RuntimeError("The solver pipeline has been destroyed")
+        opts = dict(options or {})
+        opts.setdefault("device", self.device)
+        w = HipWorld(scenes, opts)
+        self.worlds.add(w)
+        return w
+
+    def destroy(self):
+        if self.destroyed:
+            return
+        for w in list(self.worlds):
+            w.destroy()
+        self.worlds.clear()
+        self.destroyed = True

@@ -1,0 +1,171 @@
+/*
+ * avbd.h — C ABI of the MI355X-native AVBD 3D rigid-body step (libavbd_hip.so).
+ *
+ * The entry points are what a host binding (Node N-API addon, ctypes, cgo, JNI) needs to stand in for the reference's
+ * WebGPU world (`createSolverPipeline({backend})` → `scene.createSolver` → `NativeGpuSolver3D`, src/native/pipeline.js:14-93,
+ * src/native/solver3D.js:104-368, reference/three-avbd/src/avbd3d/gpu/solver.ts:348-1565). Records keep the reference's
+ * layouts (reference/three-avbd/src/avbd3d/gpu/layout.ts:8-55):
+ *   Body     40 f32  pos.xyz friction | rot xyzw | initialPos | initialRot | size.xyz mass | moment.xyz radius |
+ *                    inertialPos.xyz stamp(u32 bits) | inertialRot | vel.xyz prevUpVel | angVel.xyz shape
+ *   Joint    32 f32  penLin.xyz stiffLin | penAng.xyz stiffAng | lamLin.xyz fracture | lamAng.xyz torqueArm |
+ *                    C0Lin | C0Ang | rA.xyz rest | rB   (+ info i32×4: type 1 joint / 2 spring, bodyA (-1 world), bodyB, 0)
+ *   Manifold 8 u32   a, b (a > b), first contact, count | generation step << 4 | normal.xyz (f32), friction (f32)
+ *   Contact  16 u32  rA.xyz key | rB.xyz C0n | pen.xyz C0t1 | lam.xyz C0t2   (f32 except the key)
+ * Unbounded stiffness or fracture is stored as 3e38 (the reference's BIG; layout.ts header comment).
+ *
+ * Ownership and threading follow the reference: a world owns its device memory; reads copy into caller buffers; a
+ * handle is used by one caller at a time; avbd_world_step() enqueues on the world's HIP stream and returns without
+ * waiting (reads synchronise). Errors: every call returns AVBD_OK or a negative code and records a message readable
+ * with avbd_last_error() (thread-local), which a binding turns into `throw Error(msg)` as the reference does.
+ * Capacity failures are not errors: they are counters plus a held step, as in the reference.
+ */
+#ifndef AVBD_H
+#define AVBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_OK 0
+#define AVBD_ERR_INVALID (-1)
+#define AVBD_ERR_DEVICE (-2)
+#define AVBD_ERR_DESTROYED (-3)
+#define AVBD_ERR_CAPACITY (-4)
+
+#define AVBD_BODY_FLOATS 40
+#define AVBD_JOINT_FLOATS 32
+#define AVBD_MANIFOLD_WORDS 8
+#define AVBD_CONTACT_WORDS 16
+
+/* Solver flags (layout.ts:74-92); AVBD_FLAGS_DEFAULT = all on, as gpuParams3D() (solver.ts:113-125). */
+#define AVBD_FLAG_MATCH_NEAREST 1u
+#define AVBD_FLAG_FACE_BIAS 2u
+#define AVBD_FLAG_REUSE_CONTACTS 4u
+#define AVBD_FLAG_START_AT_REST 8u
+#define AVBD_FLAG_MASS_PENALTY 16u
+#define AVBD_FLAGS_DEFAULT 31u
+
+typedef struct avbd_world avbd_world;
+
+/**
+ * One world: the packed scene the reference's builder hands its GPU solver (AvbdScene3D → GpuSolver3D ctor + pack +
+ * writeJoint + uploadStatics, src/native/scenes.js:652-702; solver.ts:491-580, 794-903).
+ */
+typedef struct AvbdSceneDesc {
+  uint32_t body_count;
+  const float *bodies;          /* body_count × 40 f32 */
+  uint32_t joint_count;
+  const float *joints;          /* joint_count × 32 f32 */
+  const int32_t *joint_info;    /* joint_count × 4 i32 (type, bodyA or -1, bodyB, 0) */
+  uint32_t ignore_count;
+  const uint32_t *ignore_pairs; /* ignore_count × 2 (IgnoreCollision, forces.ts:300-306) */
+  const uint32_t *filters;      /* body_count × 2 (groups, collides-with; setFilters) or NULL: all */
+  float dt, gravity;            /* gravity: acceleration along up (negative: down) */
+  float up[3];                  /* unit */
+  float alpha, gamma, beta_lin, beta_ang;
+  uint32_t iterations;
+  uint32_t flags;               /* AVBD_FLAG_* */
+} AvbdSceneDesc;
+
+/** Solver options (GpuSolverOptions, solver.ts:127-151, plus this library's). Zero-initialised = defaults. */
+typedef struct AvbdOptions {
+  uint32_t color_cap;       /* colours per iteration and colouring cap (0: 16, max 64) */
+  uint32_t color_rounds;    /* Jones–Plassmann rounds per step, even (0: 16) */
+  uint32_t pair_capacity;   /* initial pair/manifold capacity (0: 8 × bodies) */
+  uint32_t use_graph;       /* replay each step from a captured hipGraph */
+  int32_t device;           /* HIP device ordinal */
+} AvbdOptions;
+
+/** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */
+typedef struct AvbdCounters {
+  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, joints, bodies;
+} AvbdCounters;
+
+/** Thread-local message of the last failed call ("" if none). */
+const char *avbd_last_error(void);
+/** Library version string. */
+const char *avbd_version(void);
+/** Number of HIP devices visible (0 without a GPU). */
+int avbd_device_count(void);
+
+/** Create a world (replaces `new NativeGpuSolver3D(device, ref, options)`, solver3D.js:104-109). */
+int avbd_world_create(const AvbdSceneDesc *scene, const AvbdOptions *options, avbd_world **out);
+/**
+ * Create a batch: n independent worlds stepped together, one launch per phase (the reference steps each world of a
+ * pipeline by its own submission, pipeline.js:49-79). Bodies and joints of world k follow those of worlds 0..k-1 in
+ * every read and write. All worlds take the same iteration count.
+ */
+int avbd_batch_create(const AvbdSceneDesc *scenes, uint32_t n, const AvbdOptions *options, avbd_world **out);
+/** Release a world (world.destroy(), solver3D.js). */
+int avbd_world_destroy(avbd_world *world);
+
+/** Enqueue n steps (world.step(), solver3D.js:350-359; GpuSolver3D.step, solver.ts:1281-1417). */
+int avbd_world_step(avbd_world *world, uint32_t n);
+/** Wait for enqueued steps. */
+int avbd_world_synchronize(avbd_world *world);
+
+int avbd_world_body_count(const avbd_world *world, uint32_t *out);
+int avbd_world_joint_count(const avbd_world *world, uint32_t *out);
+
+/** readBodies() (solver.ts:1421-1423): first..first+count bodies, 40 f32 each. */
+int avbd_world_read_bodies(avbd_world *world, uint32_t first, uint32_t count, float *out);
+/** writeBodies (solver.ts:760-788): replace body records (poses, velocities, shapes) before the next step. */
+int avbd_world_write_bodies(avbd_world *world, uint32_t first, uint32_t count, const float *in);
+/**
+ * readSelectedBodies(indices, {posesOnly}) (solver3D.js:223-230; bodyReadback.js:3-154): 40 f32 per body, or with
+ * poses_only 8 f32 per body (pos.xyz friction, rot.xyzw).
+ */
+int avbd_world_read_selected(avbd_world *world, const uint32_t *indices, uint32_t n, int poses_only, float *out);
+/** readJoints() (solver.ts:1425-1427): 32 f32 per joint. */
+int avbd_world_read_joints(avbd_world *world, uint32_t first, uint32_t count, float *out);
+int avbd_world_write_joints(avbd_world *world, uint32_t first, uint32_t count, const float *in);
+/** readCounters() (solver.ts:1429-1432). */
+int avbd_world_read_counters(avbd_world *world, AvbdCounters *out);
+/** readPairs() (solver.ts:1471-1475): the last step's broadphase pairs (a > b), 2 u32 each, at most capacity. */
+int avbd_world_read_pairs(avbd_world *world, uint32_t *out, uint32_t capacity, uint32_t *count);
+/**
+ * readContactList() (solver.ts:1435-1469): the last step's manifolds (8 words; `first` indexes the contact list) and
+ * their contact points (16 words), densely packed. Pass NULL buffers to query the counts.
+ */
+int avbd_world_read_contacts(avbd_world *world, uint32_t *manifolds, uint32_t manifold_capacity, uint32_t *manifold_count,
+                             uint32_t *contacts, uint32_t contact_capacity, uint32_t *contact_count);
+/** Colour of each body after the last step's colouring (255: none). */
+int avbd_world_read_colors(avbd_world *world, uint32_t *out);
+/** Adjacency of the last step: start[bodies + 1] then the constraint ids (joints, then J + manifold index). */
+int avbd_world_read_adjacency(avbd_world *world, uint32_t *start, uint32_t *list, uint32_t capacity, uint32_t *count);
+/** Reference poses for contact reuse (8 f32 per body: pos.xyz 0, rot.xyzw). */
+int avbd_world_read_refs(avbd_world *world, float *out);
+/** Steps taken by each world of the batch. */
+int avbd_world_read_steps(avbd_world *world, uint32_t *out);
+
+/**
+ * seedFrom (solver.ts:905-948): make the next step start from a given persistent state — reference poses (8 f32 per
+ * body or NULL), colour state (1 u32 per body or NULL), last step's manifolds and contacts (the warm-start source, as
+ * read by avbd_world_read_contacts) and each world's step counter (or NULL).
+ */
+int avbd_world_write_state(avbd_world *world, const float *refs, const uint32_t *colors, uint32_t manifold_count,
+                           const uint32_t *manifolds, uint32_t contact_count, const uint32_t *contacts,
+                           const uint32_t *steps);
+
+/** Colour cap and Jones–Plassmann rounds (GpuSolver3D.colorCap / colorRounds). */
+int avbd_world_set_coloring(avbd_world *world, uint32_t color_cap, uint32_t color_rounds);
+/** adapt(counters) (solver.ts:1484-1519): grow storage and the colour cap from the last step's counters. */
+int avbd_world_adapt(avbd_world *world, int *changed);
+
+/**
+ * Run one step with HIP events around each phase (the reference's per-phase timestamp queries, solver.ts:1389-1416)
+ * and around every primal and dual launch. out[10] (ms): total, collision, adjacency, colouring, warm start,
+ * primal (sum over launches), dual (sum), velocities, primal launches, dual launches.
+ */
+int avbd_world_profile_step(avbd_world *world, float *out);
+
+/** Device bytes held by the world. */
+int avbd_world_device_bytes(const avbd_world *world, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVBD_H */

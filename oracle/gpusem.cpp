@@ -1,0 +1,394 @@
+// ORACLE — test infrastructure only (see math.hpp).
+//
+// C interface (for the tests, through ctypes) to the GPU-semantics oracle: build a world from the reference's packed
+// records, seed its persistent state from a state read back from the HIP library, step it in FP32 (the mirror) or
+// FP64 (with the FP32 mirror's contacts and colours, to state the FP32 error of the solve), and read everything back in
+// the same record layouts.
+#include <cstring>
+#include <memory>
+
+#include "gpusem.hpp"
+
+using namespace oracle;
+using namespace oracle::gpu;
+
+namespace {
+
+struct Handle {
+  int precision;  // 32 or 64
+  std::unique_ptr<World<float>> f;
+  std::unique_ptr<World<double>> d;
+};
+
+template <class R> void unpackBody(const float *s, Body<R> &b) {
+  b.pos = V3<R>(s[0], s[1], s[2]);
+  b.friction = s[3];
+  b.rot = Q4<R>(s[4], s[5], s[6], s[7]);
+  b.ipos = V3<R>(s[8], s[9], s[10]);
+  b.iposw = s[11];
+  b.irot = Q4<R>(s[12], s[13], s[14], s[15]);
+  b.size = V3<R>(s[16], s[17], s[18]);
+  b.mass = s[19];
+  b.moment = V3<R>(s[20], s[21], s[22]);
+  b.radius = s[23];
+  b.inertialPos = V3<R>(s[24], s[25], s[26]);
+  b.stamp = floatToBits(s[27]);
+  b.inertialRot = Q4<R>(s[28], s[29], s[30], s[31]);
+  b.vel = V3<R>(s[32], s[33], s[34]);
+  b.prevUp = s[35];
+  b.angVel = V3<R>(s[36], s[37], s[38]);
+  b.shape = s[39];
+}
+template <class R, class T> void packBody(const Body<R> &b, T *o) {
+  const R v[40] = {b.pos.x, b.pos.y, b.pos.z, b.friction, b.rot.x, b.rot.y, b.rot.z, b.rot.w,
+                   b.ipos.x, b.ipos.y, b.ipos.z, b.iposw, b.irot.x, b.irot.y, b.irot.z, b.irot.w,
+                   b.size.x, b.size.y, b.size.z, b.mass, b.moment.x, b.moment.y, b.moment.z, b.radius,
+                   b.inertialPos.x, b.inertialPos.y, b.inertialPos.z, 0, b.inertialRot.x, b.inertialRot.y,
+                   b.inertialRot.z, b.inertialRot.w, b.vel.x, b.vel.y, b.vel.z, b.prevUp,
+                   b.angVel.x, b.angVel.y, b.angVel.z, b.shape};
+  for (int k = 0; k < 40; k++) o[k] = T(v[k]);
+  o[27] = sizeof(T) == 4 ? T(bitsToFloat(b.stamp)) : T(b.stamp);
+}
+template <class R> void unpackJoint(const float *s, const int32_t *info, Joint<R> &k) {
+  k.penLin = V3<R>(s[0], s[1], s[2]); k.stiffLin = s[3];
+  k.penAng = V3<R>(s[4], s[5], s[6]); k.stiffAng = s[7];
+  k.lamLin = V3<R>(s[8], s[9], s[10]); k.frac = s[11];
+  k.lamAng = V3<R>(s[12], s[13], s[14]); k.torqueArm = s[15];
+  k.c0Lin = V3<R>(s[16], s[17], s[18]);
+  k.c0Ang = V3<R>(s[20], s[21], s[22]);
+  k.rA = V3<R>(s[24], s[25], s[26]); k.rest = s[27];
+  k.rB = V3<R>(s[28], s[29], s[30]);
+  k.type = info[0]; k.a = info[1]; k.b = info[2];
+}
+template <class R, class T> void packJoint(const Joint<R> &k, T *o) {
+  const R v[32] = {k.penLin.x, k.penLin.y, k.penLin.z, k.stiffLin, k.penAng.x, k.penAng.y, k.penAng.z, k.stiffAng,
+                   k.lamLin.x, k.lamLin.y, k.lamLin.z, k.frac, k.lamAng.x, k.lamAng.y, k.lamAng.z, k.torqueArm,
+                   k.c0Lin.x, k.c0Lin.y, k.c0Lin.z, 0, k.c0Ang.x, k.c0Ang.y, k.c0Ang.z, 0,
+                   k.rA.x, k.rA.y, k.rA.z, k.rest, k.rB.x, k.rB.y, k.rB.z, 0};
+  for (int i = 0; i < 32; i++) o[i] = T(v[i]);
+}
+template <class R> void unpackManifold(const uint32_t *w, Manifold<R> &m) {
+  m.a = w[0]; m.b = w[1]; m.first = w[2]; m.countw = w[3];
+  m.normal = V3<R>(bitsToFloat(w[4]), bitsToFloat(w[5]), bitsToFloat(w[6]));
+  m.friction = bitsToFloat(w[7]);
+}
+template <class R> void packManifold(const Manifold<R> &m, uint32_t *w) {
+  w[0] = m.a; w[1] = m.b; w[2] = m.first; w[3] = m.countw;
+  w[4] = floatToBits(float(m.normal.x)); w[5] = floatToBits(float(m.normal.y)); w[6] = floatToBits(float(m.normal.z));
+  w[7] = floatToBits(float(m.friction));
+}
+template <class R> void unpackContact(const uint32_t *w, Contact<R> &c) {
+  auto f = [&](int i) { return R(bitsToFloat(w[i])); };
+  c.rA = V3<R>(f(0), f(1), f(2)); c.key = w[3];
+  c.rB = V3<R>(f(4), f(5), f(6)); c.c0x = f(7);
+  c.pen = V3<R>(f(8), f(9), f(10)); c.c0y = f(11);
+  c.lam = V3<R>(f(12), f(13), f(14)); c.c0z = f(15);
+}
+/** 16 values: rA, key (as its integer value), rB, c0x, pen, c0y, lam, c0z. */
+template <class R, class T> void packContact(const Contact<R> &c, T *o) {
+  const R v[16] = {c.rA.x, c.rA.y, c.rA.z, 0, c.rB.x, c.rB.y, c.rB.z, c.c0x,
+                   c.pen.x, c.pen.y, c.pen.z, c.c0y, c.lam.x, c.lam.y, c.lam.z, c.c0z};
+  for (int i = 0; i < 16; i++) o[i] = T(v[i]);
+}
+
+template <class R>
+void build(World<R> &w, uint32_t nBodies, const float *bodyRecs, const uint32_t *bodyWorld, uint32_t nWorlds,
+           const float *worldParams, uint32_t nJoints, const float *jointRecs, const int32_t *info, uint32_t nNoCollide,
+           const uint32_t *noCollide, const uint32_t *filters) {
+  w.bodies.resize(nBodies);
+  w.world.assign(nBodies, 0);
+  for (uint32_t i = 0; i < nBodies; i++) {
+    unpackBody(bodyRecs + 40 * i, w.bodies[i]);
+    if (bodyWorld) w.world[i] = bodyWorld[i];
+  }
+  w.params.resize(nWorlds);
+  w.worldBase.assign(nWorlds, 0);
+  for (uint32_t k = 0; k < nWorlds; k++) {
+    const float *p = worldParams + 16 * k;
+    WorldParams &q = w.params[k];
+    q.dt = p[0]; q.gravity = p[1]; q.betaLin = p[2]; q.betaAng = p[3]; q.gamma = p[4]; q.alpha = p[5];
+    q.up[0] = p[6]; q.up[1] = p[7]; q.up[2] = p[8];
+    q.flags = floatToBits(p[9]);
+    q.step = floatToBits(p[10]);
+  }
+  for (uint32_t i = nBodies; i-- > 0;) w.worldBase[w.world[i]] = i;
+  w.joints.resize(nJoints);
+  for (uint32_t j = 0; j < nJoints; j++) unpackJoint(jointRecs + 32 * j, info + 4 * j, w.joints[j]);
+  w.noCollide.clear();
+  for (uint32_t e = 0; e < nNoCollide; e++)
+    w.noCollide.emplace_back(noCollide[3 * e], noCollide[3 * e + 1], noCollide[3 * e + 2]);
+  std::sort(w.noCollide.begin(), w.noCollide.end());
+  w.filters.assign(2 * size_t(nBodies), 0xffffffffu);
+  if (filters) std::memcpy(w.filters.data(), filters, 8 * size_t(nBodies));
+  w.refs.assign(nBodies, RefPose());
+  w.colorA.assign(nBodies, NO_COLOR);
+}
+
+template <class R>
+void setState(World<R> &w, const float *refs, const uint32_t *colors, uint32_t nManifolds, const uint32_t *manifolds,
+              uint32_t nContacts, const uint32_t *contacts, const uint32_t *steps) {
+  if (refs)
+    for (size_t i = 0; i < w.n(); i++) {
+      std::memcpy(w.refs[i].pos, refs + 8 * i, 16);
+      std::memcpy(w.refs[i].rot, refs + 8 * i + 4, 16);
+    }
+  if (colors) std::memcpy(w.colorA.data(), colors, 4 * w.n());
+  w.manifolds.resize(nManifolds);
+  for (uint32_t m = 0; m < nManifolds; m++) unpackManifold(manifolds + 8 * m, w.manifolds[m]);
+  w.contacts.resize(nContacts);
+  for (uint32_t c = 0; c < nContacts; c++) unpackContact(contacts + 16 * c, w.contacts[c]);
+  if (steps)
+    for (size_t k = 0; k < w.params.size(); k++) w.params[k].step = steps[k];
+}
+
+/** The FP64 world takes the FP32 mirror's collision and topology products (the solve's inputs). */
+void copyTopology(World<double> &d, const World<float> &f) {
+  for (size_t i = 0; i < f.n(); i++) d.bodies[i].stamp = f.bodies[i].stamp;
+  d.refs = f.refs;
+  d.params = f.params;
+  d.pairs = f.pairs;
+  d.manifolds.resize(f.manifolds.size());
+  for (size_t m = 0; m < f.manifolds.size(); m++) {
+    const Manifold<float> &s = f.manifolds[m];
+    Manifold<double> &o = d.manifolds[m];
+    o.a = s.a; o.b = s.b; o.first = s.first; o.countw = s.countw;
+    o.normal = V3<double>(s.normal.x, s.normal.y, s.normal.z);
+    o.friction = s.friction;
+  }
+  d.contacts.resize(f.contacts.size());
+  for (size_t c = 0; c < f.contacts.size(); c++) {
+    const Contact<float> &s = f.contacts[c];
+    Contact<double> &o = d.contacts[c];
+    o.rA = V3<double>(s.rA.x, s.rA.y, s.rA.z); o.key = s.key;
+    o.rB = V3<double>(s.rB.x, s.rB.y, s.rB.z); o.c0x = s.c0x;
+    o.pen = V3<double>(s.pen.x, s.pen.y, s.pen.z); o.c0y = s.c0y;
+    o.lam = V3<double>(s.lam.x, s.lam.y, s.lam.z); o.c0z = s.c0z;
+  }
+  d.adjStart = f.adjStart;
+  d.adjList = f.adjList;
+  d.colorA = f.colorA;
+  d.colorStart = f.colorStart;
+  d.colorBodies = f.colorBodies;
+  d.clashes = f.clashes;
+  d.numColors = f.numColors;
+}
+
+}  // namespace
+
+extern "C" {
+
+/**
+ * worldParams: 16 floats per world: dt, gravity, betaLin, betaAng, gamma, alpha, up xyz, flags (bits), step (bits).
+ * precision: 32 = FP32 mirror; 64 = FP64 (its step takes the collision/topology of an internal FP32 mirror).
+ */
+void *gs_create(int precision, uint32_t nBodies, const float *bodyRecs, const uint32_t *bodyWorld, uint32_t nWorlds,
+                const float *worldParams, uint32_t nJoints, const float *jointRecs, const int32_t *info,
+                uint32_t nNoCollide, const uint32_t *noCollide, const uint32_t *filters, uint32_t colorCap,
+                uint32_t colorRounds, int iterations) {
+  auto *h = new Handle();
+  h->precision = precision;
+  h->f = std::make_unique<World<float>>();
+  build(*h->f, nBodies, bodyRecs, bodyWorld, nWorlds, worldParams, nJoints, jointRecs, info, nNoCollide, noCollide,
+        filters);
+  h->f->colorCap = colorCap;
+  h->f->colorRounds = colorRounds;
+  h->f->iterations = iterations;
+  if (precision == 64) {
+    h->d = std::make_unique<World<double>>();
+    build(*h->d, nBodies, bodyRecs, bodyWorld, nWorlds, worldParams, nJoints, jointRecs, info, nNoCollide, noCollide,
+          filters);
+    h->d->colorCap = colorCap;
+    h->d->colorRounds = colorRounds;
+    h->d->iterations = iterations;
+  }
+  return h;
+}
+
+void gs_destroy(void *p) { delete static_cast<Handle *>(p); }
+
+void gs_set_brute_force(void *p, int on) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->bruteForcePairs = on != 0;
+  if (h->d) h->d->bruteForcePairs = on != 0;
+}
+
+/**
+ * Seed the persistent state: reference poses (8 floats per body), colour state, the last step's manifolds (8 words)
+ * and contacts (16 words) as the warm-start source, and each world's step counter.
+ */
+void gs_set_state(void *p, const float *refs, const uint32_t *colors, uint32_t nManifolds, const uint32_t *manifolds,
+                  uint32_t nContacts, const uint32_t *contacts, const uint32_t *steps) {
+  auto *h = static_cast<Handle *>(p);
+  setState(*h->f, refs, colors, nManifolds, manifolds, nContacts, contacts, steps);
+  if (h->d) setState(*h->d, refs, colors, nManifolds, manifolds, nContacts, contacts, steps);
+}
+
+void gs_step(void *p, int n) {
+  auto *h = static_cast<Handle *>(p);
+  for (int s = 0; s < n; s++) {
+    if (h->precision == 64) {
+      // Poses enter the FP32 mirror's collision as the FP64 world holds them, rounded to f32
+      for (size_t i = 0; i < h->d->n(); i++) {
+        float rec[40];
+        packBody(h->d->bodies[i], rec);
+        rec[27] = bitsToFloat(h->f->bodies[i].stamp);
+        unpackBody(rec, h->f->bodies[i]);
+      }
+      h->f->contacts.resize(h->d->contacts.size());
+      for (size_t c = 0; c < h->d->contacts.size(); c++) {
+        const Contact<double> &s = h->d->contacts[c];
+        Contact<float> &o = h->f->contacts[c];
+        o.rA = V3<float>(float(s.rA.x), float(s.rA.y), float(s.rA.z)); o.key = s.key;
+        o.rB = V3<float>(float(s.rB.x), float(s.rB.y), float(s.rB.z)); o.c0x = float(s.c0x);
+        o.pen = V3<float>(float(s.pen.x), float(s.pen.y), float(s.pen.z)); o.c0y = float(s.c0y);
+        o.lam = V3<float>(float(s.lam.x), float(s.lam.y), float(s.lam.z)); o.c0z = float(s.c0z);
+      }
+      for (size_t j = 0; j < h->d->joints.size(); j++) {
+        float rec[32];
+        packJoint(h->d->joints[j], rec);
+        int32_t info[4] = {h->d->joints[j].type, h->d->joints[j].a, h->d->joints[j].b, 0};
+        unpackJoint(rec, info, h->f->joints[j]);
+      }
+      h->f->collisionAndTopology();
+      copyTopology(*h->d, *h->f);
+      h->d->solve();
+    } else {
+      h->f->step();
+    }
+  }
+}
+
+/** counts: pairs, manifolds, contacts, clashes, colours used, bodies, joints. */
+void gs_counts(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  World<float> &w = *h->f;
+  out[0] = uint32_t(w.pairs.size());
+  out[1] = uint32_t(w.manifolds.size());
+  out[2] = uint32_t(w.contacts.size());
+  out[3] = w.clashes;
+  out[4] = w.numColors;
+  out[5] = uint32_t(w.n());
+  out[6] = uint32_t(w.joints.size());
+}
+
+/** Bodies as 40 values each (inertialPos.w carries the moved-step stamp as its integer value). */
+void gs_read_bodies(void *p, double *out) {
+  auto *h = static_cast<Handle *>(p);
+  if (h->d)
+    for (size_t i = 0; i < h->d->n(); i++) packBody(h->d->bodies[i], out + 40 * i);
+  else
+    for (size_t i = 0; i < h->f->n(); i++) {
+      packBody(h->f->bodies[i], out + 40 * i);
+      out[40 * i + 27] = double(h->f->bodies[i].stamp);
+    }
+}
+void gs_read_joints(void *p, double *out) {
+  auto *h = static_cast<Handle *>(p);
+  if (h->d)
+    for (size_t j = 0; j < h->d->joints.size(); j++) packJoint(h->d->joints[j], out + 32 * j);
+  else
+    for (size_t j = 0; j < h->f->joints.size(); j++) packJoint(h->f->joints[j], out + 32 * j);
+}
+void gs_read_pairs(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t k = 0; k < h->f->pairs.size(); k++) {
+    out[2 * k] = h->f->pairs[k].first;
+    out[2 * k + 1] = h->f->pairs[k].second;
+  }
+}
+/** Manifolds in the 8-word record (first = index into the contact list below). */
+void gs_read_manifolds(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t m = 0; m < h->f->manifolds.size(); m++) packManifold(h->f->manifolds[m], out + 8 * m);
+}
+/** Contacts: keys (u32 per contact) and 16 values each (key slot 3 left 0). FP64 worlds give their own values. */
+void gs_read_contacts(void *p, uint32_t *keys, double *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t c = 0; c < h->f->contacts.size(); c++) {
+    if (keys) keys[c] = h->d ? h->d->contacts[c].key : h->f->contacts[c].key;
+    if (out) {
+      if (h->d) packContact(h->d->contacts[c], out + 16 * c);
+      else packContact(h->f->contacts[c], out + 16 * c);
+    }
+  }
+}
+/** Raw contact records (16 words, FP32 mirror) — for seeding another run. */
+void gs_read_contact_words(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t c = 0; c < h->f->contacts.size(); c++) {
+    const Contact<float> &k = h->f->contacts[c];
+    const float v[16] = {k.rA.x, k.rA.y, k.rA.z, 0, k.rB.x, k.rB.y, k.rB.z, k.c0x,
+                         k.pen.x, k.pen.y, k.pen.z, k.c0y, k.lam.x, k.lam.y, k.lam.z, k.c0z};
+    for (int i = 0; i < 16; i++) out[16 * c + i] = floatToBits(v[i]);
+    out[16 * c + 3] = k.key;
+  }
+}
+void gs_read_colors(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  std::memcpy(out, h->f->colorA.data(), 4 * h->f->n());
+}
+/** Adjacency: start (bodies + 1) then the list. */
+uint32_t gs_read_adjacency(void *p, uint32_t *start, uint32_t *list) {
+  auto *h = static_cast<Handle *>(p);
+  if (start) std::memcpy(start, h->f->adjStart.data(), 4 * h->f->adjStart.size());
+  if (list) std::memcpy(list, h->f->adjList.data(), 4 * h->f->adjList.size());
+  return uint32_t(h->f->adjList.size());
+}
+void gs_read_refs(void *p, float *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t i = 0; i < h->f->n(); i++) {
+    std::memcpy(out + 8 * i, h->f->refs[i].pos, 16);
+    std::memcpy(out + 8 * i + 4, h->f->refs[i].rot, 16);
+  }
+}
+
+// --- unit-level probes (for tests of single functions against the upstream restatement) -----------------------------
+
+/**
+ * Box/box contacts in the GPU semantics (faceBias flag), for comparison with the upstream collide(): returns the
+ * contact count and writes feature keys, world points on A and B (3 + 3 doubles each) and the normal (B to A).
+ */
+int gs_probe_box_contacts(int precision, const double *poseA, const double *sizeA, const double *poseB,
+                          const double *sizeB, int faceBias, int maxPoly, uint32_t *features, double *points,
+                          double *normal) {
+  auto run = [&](auto zero) {
+    using R = decltype(zero);
+    Box<R> A = makeBox(V3<R>(R(poseA[0]), R(poseA[1]), R(poseA[2])),
+                       Q4<R>(R(poseA[3]), R(poseA[4]), R(poseA[5]), R(poseA[6])),
+                       V3<R>(R(sizeA[0]), R(sizeA[1]), R(sizeA[2])));
+    Box<R> B = makeBox(V3<R>(R(poseB[0]), R(poseB[1]), R(poseB[2])),
+                       Q4<R>(R(poseB[3]), R(poseB[4]), R(poseB[5]), R(poseB[6])),
+                       V3<R>(R(sizeB[0]), R(sizeB[1]), R(sizeB[2])));
+    Sat<R> sat;
+    Found<R> f = collideBoxes(A, B, faceBias != 0, maxPoly, sat);
+    for (int i = 0; i < f.count; i++) {
+      features[i] = f.feature[i];
+      double v[6] = {double(f.xA[i].x), double(f.xA[i].y), double(f.xA[i].z),
+                     double(f.xB[i].x), double(f.xB[i].y), double(f.xB[i].z)};
+      std::memcpy(points + 6 * i, v, sizeof v);
+    }
+    normal[0] = -double(sat.n.x);
+    normal[1] = -double(sat.n.y);
+    normal[2] = -double(sat.n.z);
+    return f.count;
+  };
+  return precision == 64 ? run(0.0) : run(0.0f);
+}
+
+/** LDLᵀ 6x6 solve (lower triangle of the symmetric 6x6 `a`, row-major 36) with an optional pivot floor. */
+void gs_probe_solve6(const double *a, const double *b, double pivotFloor, double *x) {
+  M3<double> lin, ang, crs;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      lin.m[r * 3 + c] = a[r * 6 + c];
+      ang.m[r * 3 + c] = a[(r + 3) * 6 + c + 3];
+      crs.m[r * 3 + c] = a[(r + 3) * 6 + c];
+    }
+  V3<double> xl, xa;
+  solve6(lin, ang, crs, V3<double>(b[0], b[1], b[2]), V3<double>(b[3], b[4], b[5]), xl, xa, pivotFloor);
+  x[0] = xl.x; x[1] = xl.y; x[2] = xl.z; x[3] = xa.x; x[4] = xa.y; x[5] = xa.z;
+}
+
+/** Priority (lowbias32 of index + 0x9e3779b9) used by the colouring. */
+uint32_t gs_probe_priority(uint32_t i) { return hash32(i + 0x9e3779b9u); }
+}

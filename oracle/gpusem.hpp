@@ -1,0 +1,879 @@
+// ORACLE — test infrastructure only (see math.hpp).
+//
+// Sequential CPU restatement of the reference's GPU step semantics (the parity oracle for the HIP path), templated
+// on the real type: World<float> is the FP32 mirror (bit-exact discrete decisions: pairs, feature keys, colours),
+// World<double> the FP64 run used to state the FP32 error per quantity. Follows, in this order:
+//   updateRefs            reference/three-avbd/src/avbd3d/gpu/wgsl-collision.ts:20-47
+//   pair predicate        wgsl-collision.ts:112-221 (testPair, faceSeparated, ignored; evaluated canonically with the
+//                         higher-index body as the probe, so the set does not depend on the broadphase structure)
+//   narrowphase           wgsl-collision.ts:617-808 (reuseContacts, SAT/sphere contacts, warm start by feature, then
+//                         nearest anchor; contactPenaltyMin)
+//   adjacency             reference/three-avbd/src/avbd2d/gpu/wgsl-topology.ts:61-151 (sorted per-body lists)
+//   colouring             wgsl-topology.ts:155-322 (incremental Jones–Plassmann, lowbias32 priorities, cap + clashes)
+//   warm starts           reference/three-avbd/src/avbd3d/gpu/wgsl-solve.ts:210-300
+//   primal per colour     wgsl-solve.ts:53-206, 314-470 (outer-product accumulation, LDLᵀ with the 1e-5 pivot floor)
+//   dual                  wgsl-solve.ts:474-535
+//   velocities            wgsl-solve.ts:537-549
+// Bodies of several independent worlds may share one World (batched stepping): a body carries its world id, each world
+// its own parameters and step counter, pairs never cross worlds, and colouring priorities use the body's index within
+// its world, so a world steps to the same result alone and in a batch.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "collide.hpp"
+
+namespace oracle {
+namespace gpu {
+
+constexpr uint32_t NO_COLOR = 255, PENDING = 256, MAX_COLORS = 64, STICK_BIT = 0x80000000u;
+constexpr uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2;
+constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
+                   FLAG_MASS_PENALTY = 16;
+constexpr float SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_CAPSULE = -1;
+
+/** Reference record layouts (reference/three-avbd/src/avbd3d/gpu/layout.ts:8-55, 127-169). */
+struct BodyRec { float f[40]; };
+struct JointRec { float f[32]; };
+struct ManifoldRec { uint32_t ids[4]; float geo[4]; };
+struct ContactRec { uint32_t w[16]; };
+
+/** Per-world step parameters (layout.ts:171-207, the fields the 3D kernels read). */
+struct WorldParams {
+  float dt = 1.f / 60, gravity = -9.81f, betaLin = 10000, betaAng = 100, gamma = 0.999f, alpha = 0.99f;
+  float up[3] = {0, 1, 0};
+  uint32_t flags = FLAG_MATCH_NEAREST | FLAG_FACE_BIAS | FLAG_REUSE_CONTACTS | FLAG_START_AT_REST | FLAG_MASS_PENALTY;
+  uint32_t step = 0;  // steps taken (the running step's number while it runs, from 1)
+};
+
+template <class R> struct Body {
+  V3<R> pos; R friction = 0;
+  Q4<R> rot;
+  V3<R> ipos; R iposw = 0;
+  Q4<R> irot;
+  V3<R> size; R mass = 0;
+  V3<R> moment; R radius = 0;
+  V3<R> inertialPos; uint32_t stamp = 0;
+  Q4<R> inertialRot;
+  V3<R> vel; R prevUp = 0;
+  V3<R> angVel; R shape = 0;
+};
+
+template <class R> struct Joint {
+  V3<R> penLin; R stiffLin = 0;
+  V3<R> penAng; R stiffAng = 0;
+  V3<R> lamLin; R frac = 0;
+  V3<R> lamAng; R torqueArm = 0;
+  V3<R> c0Lin, c0Ang;
+  V3<R> rA; R rest = 0;
+  V3<R> rB;
+  int32_t type = 0, a = -1, b = -1;
+};
+
+template <class R> struct Manifold {
+  uint32_t a = 0, b = 0, first = 0, countw = 0;  // count | generation step << 4
+  V3<R> normal; R friction = 0;
+  uint32_t count() const { return countw & 15u; }
+};
+
+template <class R> struct Contact {
+  V3<R> rA; uint32_t key = 0;
+  V3<R> rB; R c0x = 0;
+  V3<R> pen; R c0y = 0;
+  V3<R> lam; R c0z = 0;
+};
+
+struct RefPose { float pos[4] = {0, 0, 0, 0}; float rot[4] = {0, 0, 0, 0}; };
+
+template <class R> inline R toR(float x) { return R(x); }
+
+template <class R> class World {
+ public:
+  // Scene (in index order)
+  std::vector<Body<R>> bodies;
+  std::vector<uint32_t> world;        // per body
+  std::vector<uint32_t> worldBase;    // per world: first body index
+  std::vector<WorldParams> params;    // per world
+  std::vector<Joint<R>> joints;
+  std::vector<uint32_t> filters;      // 2 per body: groups, collides-with
+  std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> noCollide;  // (hi, lo, joint slot or ~0), sorted
+  // Persistent state
+  std::vector<RefPose> refs;
+  std::vector<uint32_t> colorA;       // colour state per body (colour | PENDING)
+  std::vector<Manifold<R>> manifolds, prevManifolds;
+  std::vector<Contact<R>> contacts, prevContacts;
+  // Per-step products
+  std::vector<std::pair<uint32_t, uint32_t>> pairs;  // (a > b), sorted
+  std::vector<uint32_t> adjStart, adjList;
+  std::vector<uint32_t> colorStart, colorBodies;
+  uint32_t clashes = 0, numColors = 0;
+  // Global schedule
+  uint32_t colorCap = 64, colorRounds = 16;
+  int iterations = 10;
+  bool bruteForcePairs = false;
+
+  size_t n() const { return bodies.size(); }
+  bool dynamicBody(int64_t i) const { return i >= 0 && bodies[size_t(i)].mass > 0; }
+  const WorldParams &P(uint32_t i) const { return params[world[i]]; }
+  uint32_t localIndex(uint32_t i) const { return i - worldBase[world[i]]; }
+
+  // --- collision -----------------------------------------------------------------------------------------------
+
+  void beginFrame() {
+    prevManifolds.swap(manifolds);
+    prevContacts.swap(contacts);
+    manifolds.clear();
+    contacts.clear();
+  }
+
+  void updateRefs() {
+    for (uint32_t i = 0; i < n(); i++) {
+      const WorldParams &p = P(i);
+      if (!(p.flags & FLAG_REUSE_CONTACTS)) continue;
+      Body<R> &b = bodies[i];
+      RefPose &last = refs[i];
+      Q4<R> lr(toR<R>(last.rot[0]), toR<R>(last.rot[1]), toR<R>(last.rot[2]), toR<R>(last.rot[3]));
+      V3<R> lp(toR<R>(last.pos[0]), toR<R>(last.pos[1]), toR<R>(last.pos[2]));
+      bool moved = (lr.x * lr.x + lr.y * lr.y + lr.z * lr.z + lr.w * lr.w) == 0 ||
+                   length(sub(b.pos, lp)) > R(0.002f) || length(qsub(b.rot, lr)) > R(0.002f);
+      if (moved) {
+        last.pos[0] = float(b.pos.x); last.pos[1] = float(b.pos.y); last.pos[2] = float(b.pos.z); last.pos[3] = 0;
+        last.rot[0] = float(b.rot.x); last.rot[1] = float(b.rot.y); last.rot[2] = float(b.rot.z); last.rot[3] = float(b.rot.w);
+        b.stamp = p.step;
+      }
+    }
+  }
+
+  struct Shape {
+    V3<R> ax[3];
+    V3<R> h;
+    bool sphere;
+  };
+  Shape shapeOf(uint32_t i) const {
+    const Body<R> &b = bodies[i];
+    Shape s;
+    s.h = scale(b.size, R(0.5));
+    s.sphere = b.shape == R(SHAPE_SPHERE);
+    s.ax[0] = rotate(b.rot, V3<R>(1, 0, 0));
+    s.ax[1] = rotate(b.rot, V3<R>(0, 1, 0));
+    s.ax[2] = rotate(b.rot, V3<R>(0, 0, 1));
+    return s;
+  }
+  static R extent(const Shape &s, V3<R> n) {
+    if (s.sphere) return s.h.x;
+    return s.h.x * std::fabs(dot(n, s.ax[0])) + s.h.y * std::fabs(dot(n, s.ax[1])) + s.h.z * std::fabs(dot(n, s.ax[2]));
+  }
+  static V3<R> aabbHalf(const Shape &s) {
+    if (s.sphere) return V3<R>(s.h.x, s.h.x, s.h.x);
+    return add(add(scale(vabs(s.ax[0]), s.h.x), scale(vabs(s.ax[1]), s.h.y)), scale(vabs(s.ax[2]), s.h.z));
+  }
+  static bool faceSeparated(const Shape &A, const Shape &B, V3<R> d) {
+    for (int k = 0; k < 3; k++) {
+      if (std::fabs(dot(d, A.ax[k])) - extent(A, A.ax[k]) - extent(B, A.ax[k]) > R(1e-3f)) return true;
+      if (std::fabs(dot(d, B.ax[k])) - extent(A, B.ax[k]) - extent(B, B.ax[k]) > R(1e-3f)) return true;
+    }
+    return false;
+  }
+  bool ignored(uint32_t hi, uint32_t lo) const {
+    auto it = std::lower_bound(noCollide.begin(), noCollide.end(), std::make_tuple(hi, lo, 0u));
+    for (; it != noCollide.end(); ++it) {
+      if (std::get<0>(*it) != hi || std::get<1>(*it) != lo) break;
+      uint32_t j = std::get<2>(*it);
+      if (j == 0xffffffffu) return true;
+      if (joints[j].stiffLin != 0 || joints[j].stiffAng != 0) return true;
+    }
+    return false;
+  }
+  /** The pair predicate, with a = the higher index as the probe (testPair, wgsl-collision.ts:196-221). */
+  bool acceptPair(uint32_t a, uint32_t b) const {
+    if (world[a] != world[b]) return false;
+    const Body<R> &A = bodies[a], &B = bodies[b];
+    if (!(A.mass > 0) && B.mass <= 0) return false;
+    uint32_t fa0 = filters[2 * a], fa1 = filters[2 * a + 1], fb0 = filters[2 * b], fb1 = filters[2 * b + 1];
+    if ((fa0 & fb1) == 0 || (fb0 & fa1) == 0) return false;
+    V3<R> d = sub(A.pos, B.pos);
+    R r = A.radius + B.radius;
+    if (dot(d, d) > r * r) return false;
+    Shape sa = shapeOf(a), sb = shapeOf(b);
+    V3<R> ha = aabbHalf(sa), hb = aabbHalf(sb);
+    R pad = R(1e-4f);
+    if (std::fabs(d.x) > ha.x + hb.x + pad || std::fabs(d.y) > ha.y + hb.y + pad || std::fabs(d.z) > ha.z + hb.z + pad)
+      return false;
+    if (faceSeparated(sa, sb, d)) return false;
+    if (ignored(a, b)) return false;
+    return true;
+  }
+
+  void findPairs() {
+    pairs.clear();
+    const uint32_t N = uint32_t(n());
+    if (bruteForcePairs || N < 2048) {
+      for (uint32_t a = 0; a < N; a++)
+        for (uint32_t b = 0; b < a; b++)
+          if (acceptPair(a, b)) pairs.push_back({a, b});
+      return;
+    }
+    // A uniform grid over (world, cell): the largest bodies (above 4x the median radius) are tested against all
+    // bodies of their world, the rest are binned in cells of twice the largest small radius.
+    std::vector<R> radii(N);
+    for (uint32_t i = 0; i < N; i++) radii[i] = bodies[i].radius;
+    std::vector<R> sorted = radii;
+    std::nth_element(sorted.begin(), sorted.begin() + N / 2, sorted.end());
+    R median = sorted[N / 2];
+    std::vector<char> large(N, 0);
+    R maxSmall = 0;
+    for (uint32_t i = 0; i < N; i++) {
+      if (radii[i] > 4 * median) large[i] = 1;
+      else maxSmall = rmax(maxSmall, radii[i]);
+    }
+    double cell = rmax(2.0 * double(maxSmall) * 1.0001, 1e-3);
+    auto key = [&](uint32_t w, int64_t x, int64_t y, int64_t z) {
+      return (uint64_t(w) << 48) ^ (uint64_t(x & 0xffff) << 32) ^ (uint64_t(y & 0xffff) << 16) ^ uint64_t(z & 0xffff);
+    };
+    auto cellOf = [&](uint32_t i, int64_t *c) {
+      c[0] = int64_t(std::floor(double(bodies[i].pos.x) / cell));
+      c[1] = int64_t(std::floor(double(bodies[i].pos.y) / cell));
+      c[2] = int64_t(std::floor(double(bodies[i].pos.z) / cell));
+    };
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    std::vector<std::vector<uint32_t>> largeOf(worldBase.size());
+    for (uint32_t i = 0; i < N; i++) {
+      if (large[i]) {
+        largeOf[world[i]].push_back(i);
+        continue;
+      }
+      int64_t c[3];
+      cellOf(i, c);
+      grid[key(world[i], c[0], c[1], c[2])].push_back(i);
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
+    for (uint32_t i = 0; i < N; i++) {
+      if (large[i]) continue;
+      int64_t c[3];
+      cellOf(i, c);
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            auto it = grid.find(key(world[i], c[0] + dx, c[1] + dy, c[2] + dz));
+            if (it == grid.end()) continue;
+            for (uint32_t j : it->second)
+              if (j < i) cand.push_back({i, j});
+          }
+    }
+    for (auto &lst : largeOf)
+      for (uint32_t l : lst)
+        for (uint32_t j = worldBase[world[l]]; j < N && world[j] == world[l]; j++)
+          if (j != l && (!large[j] || j < l)) cand.push_back({rmax(l, j), rmin(l, j)});
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    for (auto &p : cand)
+      if (acceptPair(p.first, p.second)) pairs.push_back(p);
+  }
+
+  R contactPenaltyMin(uint32_t a, uint32_t b) const {
+    const WorldParams &p = P(a);
+    if (!(p.flags & FLAG_MASS_PENALTY)) return R(1);
+    R mA = bodies[a].mass <= 0 ? R(3.4e38f) : bodies[a].mass;
+    R mB = bodies[b].mass <= 0 ? R(3.4e38f) : bodies[b].mass;
+    R dt = R(p.dt);
+    return rclamp(rmin(mA, mB) / (dt * dt), R(1), R(1e10f));
+  }
+
+  std::map<std::pair<uint32_t, uint32_t>, uint32_t> prevIndex;
+
+  bool reuseContacts(uint32_t a, uint32_t b) {
+    auto it = prevIndex.find({a, b});
+    if (it == prevIndex.end()) return false;
+    const Manifold<R> prev = prevManifolds[it->second];
+    uint32_t count = prev.count();
+    uint32_t generated = prev.countw >> 4;
+    uint32_t moved = rmax(bodies[a].stamp, bodies[b].stamp);
+    if (count == 0 || generated < moved) return false;
+    const WorldParams &p = P(a);
+    Manifold<R> m = prev;
+    m.first = uint32_t(contacts.size());
+    manifolds.push_back(m);
+    const Body<R> &A = bodies[a], &B = bodies[b];
+    M3<R> basis = orthonormal(prev.normal);
+    R alpha = R(p.alpha), gamma = R(p.gamma);
+    R pmin = contactPenaltyMin(a, b);
+    for (uint32_t i = 0; i < count; i++) {
+      Contact<R> k = prevContacts[prev.first + i];
+      V3<R> d = sub(add(rotate(A.rot, k.rA), A.pos), add(rotate(B.rot, k.rB), B.pos));
+      k.c0x = dot(row(basis, 0), d) + R(0.01f);
+      k.c0y = dot(row(basis, 1), d);
+      k.c0z = dot(row(basis, 2), d);
+      k.lam = scale(scale(k.lam, alpha), gamma);
+      k.pen = V3<R>(rclampW(k.pen.x * gamma), rclampW(k.pen.y * gamma), rclampW(k.pen.z * gamma));
+      k.pen.x = rmaxW(k.pen.x, pmin);
+      contacts.push_back(k);
+    }
+    return true;
+  }
+  // WGSL clamp(e, lo, hi) = min(max(e, lo), hi); max/min of two finite floats
+  static R rclampW(R x) { return rmin(rmax(x, R(1)), R(1e10f)); }
+  static R rmaxW(R a, R b) { return a > b ? a : b; }
+
+  void narrowphase() {
+    prevIndex.clear();
+    for (uint32_t m = 0; m < prevManifolds.size(); m++) prevIndex[{prevManifolds[m].a, prevManifolds[m].b}] = m;
+    for (auto &pr : pairs) {
+      uint32_t a = pr.first, b = pr.second;
+      const WorldParams &p = P(a);
+      if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b)) continue;
+      const Body<R> &BA = bodies[a], &BB = bodies[b];
+      Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
+      Sat<R> sat;
+      Found<R> found;
+      bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
+      if (sphereA && sphereB) {
+        found = sphereSphere(A, B, sat);
+      } else if (sphereA || sphereB) {
+        if (sphereA) {
+          SphereBoxHit<R> h = sphereBox(A, B, BB.rot);
+          sat.n = h.n;
+          if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+        } else {
+          SphereBoxHit<R> h = sphereBox(B, A, BA.rot);
+          sat.n = neg(h.n);
+          if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+        }
+      } else {
+        found = collideBoxes(A, B, (p.flags & FLAG_FACE_BIAS) != 0, 8, sat);
+      }
+      if (found.count == 0) continue;
+      Manifold<R> mf;
+      mf.a = a;
+      mf.b = b;
+      mf.first = uint32_t(contacts.size());
+      mf.countw = uint32_t(found.count) | (p.step << 4);
+      mf.normal = neg(sat.n);
+      mf.friction = std::sqrt(BA.friction * BB.friction);
+      manifolds.push_back(mf);
+      M3<R> basis = orthonormal(mf.normal);
+      bool anySphere = sphereA || sphereB;
+      uint32_t prevFirst = 0, prevCount = 0;
+      auto it = prevIndex.find({a, b});
+      if (it != prevIndex.end()) {
+        prevFirst = prevManifolds[it->second].first;
+        prevCount = prevManifolds[it->second].count();
+      }
+      bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
+      R minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * R(2);
+      R pmin = contactPenaltyMin(a, b);
+      R alpha = R(p.alpha), gamma = R(p.gamma);
+      for (int i = 0; i < found.count; i++) {
+        V3<R> rA = rotate(conj(BA.rot), sub(found.xA[i], A.c));
+        V3<R> rB = rotate(conj(BB.rot), sub(found.xB[i], B.c));
+        V3<R> pen, lam;
+        uint32_t stick = 0;
+        int64_t j = -1;
+        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+          if ((prevContacts[c].key & ~STICK_BIT) == found.feature[i]) {
+            j = c;
+            break;
+          }
+        if (j < 0 && matchNearest) {
+          R bestDist = R(0.05f) * minSide;
+          for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+            R dist = length(sub(prevContacts[c].rA, rA));
+            if (dist <= bestDist) {
+              bestDist = dist;
+              j = c;
+            }
+          }
+        }
+        if (j >= 0) {
+          const Contact<R> &pc = prevContacts[size_t(j)];
+          pen = pc.pen;
+          lam = pc.lam;
+          stick = pc.key & STICK_BIT;
+          if (stick != 0 && !anySphere) {
+            rA = pc.rA;
+            rB = pc.rB;
+          }
+        }
+        V3<R> d = sub(add(rotate(BA.rot, rA), A.c), add(rotate(BB.rot, rB), B.c));
+        Contact<R> k;
+        k.rA = rA;
+        k.rB = rB;
+        k.key = found.feature[i] | stick;
+        k.c0x = dot(row(basis, 0), d) + R(0.01f);
+        k.c0y = dot(row(basis, 1), d);
+        k.c0z = dot(row(basis, 2), d);
+        k.lam = scale(scale(lam, alpha), gamma);
+        k.pen = V3<R>(rclampW(pen.x * gamma), rclampW(pen.y * gamma), rclampW(pen.z * gamma));
+        k.pen.x = rmaxW(k.pen.x, pmin);
+        contacts.push_back(k);
+      }
+    }
+  }
+
+  // --- topology --------------------------------------------------------------------------------------------------
+
+  bool jointActive(uint32_t j) const {
+    return joints[j].type != int32_t(T_NONE) && (joints[j].stiffLin != 0 || joints[j].stiffAng != 0);
+  }
+  std::pair<int64_t, int64_t> endpoints(uint32_t id) const {
+    uint32_t J = uint32_t(joints.size());
+    if (id < J) return {joints[id].a, joints[id].b};
+    return {manifolds[id - J].a, manifolds[id - J].b};
+  }
+  void buildAdjacency() {
+    const uint32_t N = uint32_t(n()), J = uint32_t(joints.size());
+    std::vector<std::vector<uint32_t>> lists(N);
+    auto addEnds = [&](uint32_t id) {
+      auto e = endpoints(id);
+      if (dynamicBody(e.first)) lists[size_t(e.first)].push_back(id);
+      if (dynamicBody(e.second)) lists[size_t(e.second)].push_back(id);
+    };
+    for (uint32_t j = 0; j < J; j++)
+      if (jointActive(j)) addEnds(j);
+    for (uint32_t m = 0; m < manifolds.size(); m++) addEnds(J + m);
+    adjStart.assign(N + 1, 0);
+    adjList.clear();
+    for (uint32_t b = 0; b < N; b++) {
+      auto keyOf = [&](uint32_t id) -> std::pair<uint64_t, uint64_t> {
+        if (id < J) return {0, id};
+        auto e = endpoints(id);
+        int64_t other = e.first == int64_t(b) ? e.second : e.first;
+        return {uint64_t(other + 2), 0};
+      };
+      std::stable_sort(lists[b].begin(), lists[b].end(), [&](uint32_t x, uint32_t y) { return keyOf(x) < keyOf(y); });
+      adjStart[b] = uint32_t(adjList.size());
+      adjList.insert(adjList.end(), lists[b].begin(), lists[b].end());
+    }
+    adjStart[N] = uint32_t(adjList.size());
+  }
+
+  uint32_t priority(uint32_t i) const { return hash32(localIndex(i) + 0x9e3779b9u); }
+  bool beats(uint32_t i, uint32_t j) const {
+    uint32_t pi = priority(i), pj = priority(j);
+    return pi > pj || (pi == pj && localIndex(i) > localIndex(j));
+  }
+  int64_t neighbour(uint32_t b, uint32_t e) const {
+    auto ends = endpoints(adjList[e]);
+    int64_t other = ends.first == int64_t(b) ? ends.second : ends.first;
+    return dynamicBody(other) ? other : -1;
+  }
+  uint32_t smallestFree(uint32_t b, const std::vector<uint32_t> &src, bool skipPending) const {
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t e = adjStart[b]; e < adjStart[b + 1]; e++) {
+      int64_t j = neighbour(b, e);
+      if (j < 0) continue;
+      uint32_t s = src[size_t(j)];
+      if (skipPending && (s & PENDING)) continue;
+      uint32_t col = s & 0xffu;
+      if (col == NO_COLOR) continue;
+      if (col < 32) lo |= 1u << col;
+      else hi |= 1u << (col - 32);
+    }
+    if (~lo != 0) return uint32_t(__builtin_ctz(~lo));
+    if (~hi != 0) return 32u + uint32_t(__builtin_ctz(~hi));
+    return MAX_COLORS - 1;
+  }
+  void colour() {
+    const uint32_t N = uint32_t(n());
+    std::vector<uint32_t> B(N);
+    // 1. compaction (colorCompact)
+    for (uint32_t b = 0; b < N; b++) {
+      if (!dynamicBody(b)) {
+        B[b] = NO_COLOR;
+        continue;
+      }
+      uint32_t col = colorA[b] & 0xffu;
+      if (col != NO_COLOR) {
+        bool isMax = true;
+        for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && isMax; e++) {
+          int64_t j = neighbour(b, e);
+          if (j >= 0 && !beats(b, uint32_t(j))) isMax = false;
+        }
+        if (isMax) col = rmin(col, smallestFree(b, colorA, false));
+      }
+      B[b] = col;
+    }
+    // 2. pending (colorMark)
+    for (uint32_t b = 0; b < N; b++) {
+      uint32_t col = B[b] & 0xffu;
+      if (!dynamicBody(b)) {
+        colorA[b] = NO_COLOR;
+        continue;
+      }
+      bool pending = col == NO_COLOR;
+      for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && !pending; e++) {
+        int64_t j = neighbour(b, e);
+        if (j >= 0 && (B[size_t(j)] & 0xffu) == col && beats(uint32_t(j), b)) pending = true;
+      }
+      colorA[b] = col | (pending ? PENDING : 0);
+    }
+    // 3. Jones–Plassmann rounds, Jacobi between the two state buffers
+    for (uint32_t r = 0; r < colorRounds; r++) {
+      const std::vector<uint32_t> &src = (r % 2 == 0) ? colorA : B;
+      std::vector<uint32_t> &dst = (r % 2 == 0) ? B : colorA;
+      for (uint32_t b = 0; b < N; b++) {
+        uint32_t s = src[b];
+        if (!(s & PENDING)) {
+          dst[b] = s;
+          continue;
+        }
+        bool isMax = true;
+        for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && isMax; e++) {
+          int64_t j = neighbour(b, e);
+          if (j >= 0 && (src[size_t(j)] & PENDING) && !beats(b, uint32_t(j))) isMax = false;
+        }
+        if (isMax) {
+          uint32_t fr = smallestFree(b, src, true);
+          dst[b] = fr >= colorCap ? ((colorCap - 1) | PENDING) : fr;
+        } else {
+          dst[b] = s;
+        }
+      }
+    }
+    // 4-6. counts, clashes and stable buckets
+    clashes = 0;
+    numColors = 0;
+    colorStart.assign(MAX_COLORS + 1, 0);
+    for (uint32_t b = 0; b < N; b++) {
+      if (!dynamicBody(b)) continue;
+      uint32_t s = colorA[b];
+      uint32_t col = s & 0xffu;
+      if (s & PENDING) clashes++;
+      if (col == NO_COLOR) col = colorCap - 1;
+      colorA[b] = col;
+      numColors = rmax(numColors, col + 1);
+      colorStart[col + 1]++;
+    }
+    for (uint32_t c = 0; c < MAX_COLORS; c++) colorStart[c + 1] += colorStart[c];
+    colorBodies.assign(colorStart[MAX_COLORS], 0);
+    std::vector<uint32_t> cursor(colorStart.begin(), colorStart.end() - 1);
+    for (uint32_t b = 0; b < N; b++)
+      if (dynamicBody(b)) colorBodies[cursor[colorA[b]]++] = b;
+  }
+
+  // --- solve -------------------------------------------------------------------------------------------------------
+
+  V3<R> anchorA(const Joint<R> &k) const {
+    if (k.a < 0) return k.rA;
+    const Body<R> &A = bodies[size_t(k.a)];
+    return add(rotate(A.rot, k.rA), A.pos);
+  }
+  V3<R> jointLinC(const Joint<R> &k) const {
+    const Body<R> &B = bodies[size_t(k.b)];
+    return sub(anchorA(k), add(rotate(B.rot, k.rB), B.pos));
+  }
+  V3<R> jointAngC(const Joint<R> &k) const {
+    Q4<R> ra = k.a < 0 ? Q4<R>(0, 0, 0, 1) : bodies[size_t(k.a)].rot;
+    return scale(qsub(ra, bodies[size_t(k.b)].rot), k.torqueArm);
+  }
+  static bool hard(R stiffness) { return stiffness >= R(1e30f); }
+
+  void warmStartJoints() {
+    for (Joint<R> &k : joints) {
+      if (k.type != int32_t(T_JOINT)) continue;
+      const WorldParams &p = P(uint32_t(k.b));
+      k.c0Lin = jointLinC(k);
+      k.c0Ang = jointAngC(k);
+      R decay = R(p.alpha) * R(p.gamma);
+      k.lamLin = scale(k.lamLin, decay);
+      k.lamAng = scale(k.lamAng, decay);
+      R g = R(p.gamma);
+      k.penLin = V3<R>(rmin(rclampW(k.penLin.x * g), k.stiffLin), rmin(rclampW(k.penLin.y * g), k.stiffLin),
+                       rmin(rclampW(k.penLin.z * g), k.stiffLin));
+      k.penAng = V3<R>(rmin(rclampW(k.penAng.x * g), k.stiffAng), rmin(rclampW(k.penAng.y * g), k.stiffAng),
+                       rmin(rclampW(k.penAng.z * g), k.stiffAng));
+    }
+  }
+
+  void warmStartBodies() {
+    for (uint32_t i = 0; i < n(); i++) {
+      Body<R> &b = bodies[i];
+      const WorldParams &p = P(i);
+      R dt = R(p.dt), g = R(p.gravity);
+      V3<R> up(R(p.up[0]), R(p.up[1]), R(p.up[2]));
+      bool dyn = b.mass > 0;
+      V3<R> inertialPos = add(b.pos, scale(b.vel, dt));
+      if (dyn) inertialPos = add(inertialPos, scale(up, g * (dt * dt)));
+      b.inertialPos = inertialPos;
+      b.inertialRot = qadd(b.rot, scale(b.angVel, dt));
+      R w = 0;
+      if (std::fabs(g) > 0) {
+        R x = (dot(b.vel, up) - b.prevUp) / dt * rsign(g) / std::fabs(g);
+        w = rmin(rmax(x, R(0)), R(1));
+      }
+      bool slow = length(b.vel) + length(b.angVel) * b.radius < R(0.1f);
+      bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
+      b.ipos = b.pos;
+      b.iposw = b.friction;
+      b.irot = b.rot;
+      if (dyn && !atRest) {
+        b.pos = add(add(b.pos, scale(b.vel, dt)), scale(up, g * (w * dt * dt)));
+        b.rot = qadd(b.rot, scale(b.angVel, dt));
+      }
+      bool moving = b.vel.x != 0 || b.vel.y != 0 || b.vel.z != 0 || b.angVel.x != 0 || b.angVel.y != 0 || b.angVel.z != 0;
+      if (!dyn && moving) {
+        b.pos = add(b.pos, scale(b.vel, dt));
+        b.rot = qadd(b.rot, scale(b.angVel, dt));
+      }
+    }
+  }
+
+  struct Acc {
+    M3<R> lin, ang, cross;
+    V3<R> rLin, rAng;
+  };
+  static void addRow(Acc &acc, V3<R> l, V3<R> a, R k, R f) {
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        acc.lin.m[r * 3 + c] += l[r] * l[c] * k;
+        acc.ang.m[r * 3 + c] += a[r] * a[c] * k;
+        acc.cross.m[r * 3 + c] += a[r] * l[c] * k;
+      }
+    acc.rLin = add(acc.rLin, scale(l, f));
+    acc.rAng = add(acc.rAng, scale(a, f));
+  }
+
+  struct PairBody {
+    uint32_t index;
+    Q4<R> rot;
+    V3<R> dLin, dAng;
+  };
+  PairBody pairBody(uint32_t i) const {
+    const Body<R> &b = bodies[i];
+    return {i, b.rot, sub(b.pos, b.ipos), qsub(b.rot, b.irot)};
+  }
+  struct Eval {
+    V3<R> rAW, rBW, C, F;
+    R frictionScale, bounds;
+  };
+  Eval evalContact(const Contact<R> &k, const M3<R> &basis, R friction, const PairBody &A, const PairBody &B,
+                   R alpha) const {
+    Q4<R> rotA = A.rot, rotB = B.rot;
+    if ((k.key & ~STICK_BIT) == SPHERE_FEATURE) {
+      rotA = bodies[A.index].irot;
+      rotB = bodies[B.index].irot;
+    }
+    Eval e;
+    e.rAW = rotate(rotA, k.rA);
+    e.rBW = rotate(rotB, k.rB);
+    V3<R> n = row(basis, 0), t1 = row(basis, 1), t2 = row(basis, 2);
+    R keep = R(1) - alpha;
+    auto rowC = [&](R c0, V3<R> l) {
+      return c0 * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
+    };
+    e.C = V3<R>(rowC(k.c0x, n), rowC(k.c0y, t1), rowC(k.c0z, t2));
+    V3<R> F = add(mulv3(k.pen, e.C), k.lam);
+    F.x = rmin(F.x, R(0));
+    e.bounds = std::fabs(F.x) * friction;
+    e.frictionScale = std::sqrt(F.y * F.y + F.z * F.z);
+    if (e.frictionScale > e.bounds && e.frictionScale > 0) {
+      R s = e.bounds / e.frictionScale;
+      F.y = F.y * s;
+      F.z = F.z * s;
+    }
+    e.F = F;
+    return e;
+  }
+
+  V3<R> jointLinRows(const Joint<R> &k, R alpha) const {
+    V3<R> c = jointLinC(k);
+    return sub(c, hard(k.stiffLin) ? scale(k.c0Lin, alpha) : V3<R>(0, 0, 0));
+  }
+  V3<R> jointAngRows(const Joint<R> &k, R alpha) const {
+    V3<R> c = jointAngC(k);
+    return sub(c, hard(k.stiffAng) ? scale(k.c0Ang, alpha) : V3<R>(0, 0, 0));
+  }
+
+  void addJoint(Acc &acc, const Joint<R> &k, R alpha, uint32_t i) const {
+    bool isA = int64_t(i) == k.a;
+    R sg = isA ? R(1) : R(-1);
+    if (k.type == int32_t(T_SPRING)) {
+      const Body<R> &A = bodies[size_t(k.a)], &B = bodies[size_t(k.b)];
+      V3<R> rAW = rotate(A.rot, k.rA), rBW = rotate(B.rot, k.rB);
+      V3<R> d = sub(add(rAW, A.pos), add(rBW, B.pos));
+      R len = length(d);
+      if (len <= R(1.0e-6f)) return;
+      V3<R> n = divs(d, len);
+      R stiffness = k.stiffLin;
+      V3<R> r = isA ? rAW : rBW;
+      addRow(acc, scale(n, sg), scale(cross(r, n), sg), stiffness, stiffness * (len - k.rest));
+      return;
+    }
+    if (dot(k.penLin, k.penLin) > 0) {
+      V3<R> F = add(mulv3(k.penLin, jointLinRows(k, alpha)), k.lamLin);
+      V3<R> r = isA ? rotate(bodies[size_t(k.a)].rot, k.rA) : neg(rotate(bodies[size_t(k.b)].rot, k.rB));
+      V3<R> v = neg(r);
+      addRow(acc, V3<R>(sg, 0, 0), V3<R>(0, -v.z, v.y), k.penLin.x, F.x);
+      addRow(acc, V3<R>(0, sg, 0), V3<R>(v.z, 0, -v.x), k.penLin.y, F.y);
+      addRow(acc, V3<R>(0, 0, sg), V3<R>(-v.y, v.x, 0), k.penLin.z, F.z);
+      R rf = dot(r, F);
+      V3<R> g(length(sub(scale(r, F.x), V3<R>(rf, 0, 0))), length(sub(scale(r, F.y), V3<R>(0, rf, 0))),
+              length(sub(scale(r, F.z), V3<R>(0, 0, rf))));
+      acc.ang.m[0] += g.x;
+      acc.ang.m[4] += g.y;
+      acc.ang.m[8] += g.z;
+    }
+    if (dot(k.penAng, k.penAng) > 0) {
+      V3<R> F = add(mulv3(k.penAng, jointAngRows(k, alpha)), k.lamAng);
+      R s = sg * k.torqueArm;
+      addRow(acc, V3<R>(0, 0, 0), V3<R>(s, 0, 0), k.penAng.x, F.x);
+      addRow(acc, V3<R>(0, 0, 0), V3<R>(0, s, 0), k.penAng.y, F.y);
+      addRow(acc, V3<R>(0, 0, 0), V3<R>(0, 0, s), k.penAng.z, F.z);
+    }
+  }
+
+  Acc accumulate(uint32_t i, R alpha) const {
+    const Body<R> &b = bodies[i];
+    const WorldParams &p = P(i);
+    R dt = R(p.dt);
+    R dt2 = dt * dt;
+    R m = b.mass / dt2;
+    V3<R> I = divs(b.moment, dt2);
+    Acc acc;
+    acc.lin = diag(m, m, m);
+    acc.ang = diag(I.x, I.y, I.z);
+    acc.rLin = scale(sub(b.pos, b.inertialPos), m);
+    acc.rAng = mulv3(I, qsub(b.rot, b.inertialRot));
+    const uint32_t J = uint32_t(joints.size());
+    for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++) {
+      uint32_t id = adjList[e];
+      if (id < J) {
+        addJoint(acc, joints[id], alpha, i);
+        continue;
+      }
+      const Manifold<R> &mf = manifolds[id - J];
+      bool isA = i == mf.a;
+      PairBody A = pairBody(mf.a), B = pairBody(mf.b);
+      for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
+        const Contact<R> &k = contacts[c];
+        M3<R> basis = orthonormal(mf.normal);
+        Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
+        R sg = isA ? R(1) : R(-1);
+        V3<R> r = isA ? ev.rAW : ev.rBW;
+        V3<R> l0 = scale(row(basis, 0), sg), l1 = scale(row(basis, 1), sg), l2 = scale(row(basis, 2), sg);
+        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+      }
+    }
+    return acc;
+  }
+
+  void finishBody(uint32_t i, const Acc &acc) {
+    Body<R> &b = bodies[i];
+    V3<R> xl, xa;
+    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f));
+    b.pos = sub(b.pos, xl);
+    b.rot = qadd(b.rot, neg(xa));
+  }
+
+  void dualJoint(Joint<R> &k, R alpha) {
+    if (k.type != int32_t(T_JOINT)) return;
+    const WorldParams &p = P(uint32_t(k.b));
+    if (dot(k.penLin, k.penLin) > 0) {
+      V3<R> C = jointLinRows(k, alpha);
+      if (hard(k.stiffLin)) k.lamLin = add(mulv3(k.penLin, C), k.lamLin);
+      R cap = rmin(k.stiffLin, R(1e10f));
+      V3<R> q = add(k.penLin, scale(vabs(C), R(p.betaLin)));
+      k.penLin = V3<R>(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap));
+    }
+    if (dot(k.penAng, k.penAng) > 0) {
+      V3<R> C = jointAngRows(k, alpha);
+      if (hard(k.stiffAng)) k.lamAng = add(mulv3(k.penAng, C), k.lamAng);
+      R cap = rmin(k.stiffAng, R(1e10f));
+      V3<R> q = add(k.penAng, scale(vabs(C), R(p.betaAng)));
+      k.penAng = V3<R>(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap));
+    }
+    R frac = k.frac;
+    R limit = std::fabs(frac);
+    bool linear = frac < 0 && dot(k.lamLin, k.lamLin) > limit * limit;
+    if (limit < R(3e38f) && (dot(k.lamAng, k.lamAng) > limit * limit || linear)) {
+      k.penLin = V3<R>();
+      k.stiffLin = 0;
+      k.penAng = V3<R>();
+      k.stiffAng = 0;
+      k.lamLin = V3<R>();
+      k.lamAng = V3<R>();
+    }
+  }
+
+  void dualManifold(uint32_t m, R alpha) {
+    const Manifold<R> &mf = manifolds[m];
+    const WorldParams &p = P(mf.a);
+    PairBody A = pairBody(mf.a), B = pairBody(mf.b);
+    M3<R> basis = orthonormal(mf.normal);
+    for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
+      Contact<R> &k = contacts[c];
+      Eval e = evalContact(k, basis, mf.friction, A, B, alpha);
+      k.lam = e.F;
+      V3<R> pen = k.pen;
+      if (e.F.x < 0) pen.x = rmin(pen.x + R(p.betaLin) * std::fabs(e.C.x), R(1e10f));
+      if (e.frictionScale <= e.bounds) {
+        pen.y = rmin(pen.y + R(p.betaLin) * std::fabs(e.C.y), R(1e10f));
+        pen.z = rmin(pen.z + R(p.betaLin) * std::fabs(e.C.z), R(1e10f));
+        bool stick = std::sqrt(e.C.y * e.C.y + e.C.z * e.C.z) < R(0.00001f);
+        k.key = (k.key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+      }
+      k.pen = pen;
+    }
+  }
+
+  void updateVelocities() {
+    for (uint32_t i = 0; i < n(); i++) {
+      Body<R> &b = bodies[i];
+      const WorldParams &p = P(i);
+      V3<R> up(R(p.up[0]), R(p.up[1]), R(p.up[2]));
+      R dt = R(p.dt);
+      R prevUp = dot(b.vel, up);
+      if (b.mass > 0) {
+        b.vel = divs(sub(b.pos, b.ipos), dt);
+        b.prevUp = prevUp;
+        b.angVel = divs(qsub(b.rot, b.irot), dt);
+      } else {
+        b.prevUp = prevUp;
+      }
+    }
+  }
+
+  // --- the step ----------------------------------------------------------------------------------------------------
+
+  /** Collision, adjacency and colouring (everything before the warm starts). */
+  void collisionAndTopology() {
+    for (auto &p : params) p.step++;
+    beginFrame();
+    updateRefs();
+    findPairs();
+    narrowphase();
+    buildAdjacency();
+    colour();
+  }
+
+  /** Warm starts, iterations × (primal per colour, dual), velocities. */
+  void solve() {
+    warmStartJoints();
+    warmStartBodies();
+    for (int it = 0; it < iterations; it++) {
+      for (uint32_t c = 0; c < colorCap; c++)
+        for (uint32_t s = colorStart[c]; s < colorStart[c + 1]; s++) {
+          uint32_t i = colorBodies[s];
+          R alpha = R(P(i).alpha);
+          Acc acc = accumulate(i, alpha);
+          finishBody(i, acc);
+        }
+      for (Joint<R> &k : joints) dualJoint(k, R(P(uint32_t(k.b)).alpha));
+      for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, R(P(manifolds[m].a).alpha));
+    }
+    updateVelocities();
+  }
+
+  void step() {
+    collisionAndTopology();
+    solve();
+  }
+};
+
+}  // namespace gpu
+}  // namespace oracle

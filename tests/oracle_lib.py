@@ -1,0 +1,225 @@
+"""Test-side loader of the CPU oracle (oracle/liboracle.so): the FP64 upstream restatement and the GPU-semantics
+restatement (FP32 mirror / FP64). Only the tests, smoke() and bench.py's CPU baseline use it, as the checker."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+u32p = ctypes.POINTER(ctypes.c_uint32)
+i32p = ctypes.POINTER(ctypes.c_int32)
+f32p = ctypes.POINTER(ctypes.c_float)
+f64p = ctypes.POINTER(ctypes.c_double)
+vp = ctypes.c_void_p
+
+
+def _stale():
+    if not os.path.exists(ORACLE_LIB):
+        return True
+    t = os.path.getmtime(ORACLE_LIB)
+    srcs = [f for f in os.listdir(ORACLE_DIR) if f.endswith((".cpp", ".hpp")) or f == "Makefile"]
+    return any(os.path.getmtime(os.path.join(ORACLE_DIR, f)) > t for f in srcs)
+
+
+def build_oracle():
+    if _stale():
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return ORACLE_LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build_oracle()
+        L = ctypes.CDLL(ORACLE_LIB)
+        sig = {
+            "ref_create": (vp, [ctypes.c_int, ctypes.c_int]),
+            "ref_destroy": (None, [vp]),
+            "ref_step": (None, [vp, ctypes.c_int]),
+            "ref_body_count": (ctypes.c_int, [vp]),
+            "ref_force_count": (ctypes.c_int, [vp]),
+            "ref_read_bodies": (None, [vp, f64p]),
+            "ref_joint_stats": (ctypes.c_int, [vp, f64p]),
+            "ref_pick": (ctypes.c_int, [vp, f64p, f64p, f64p]),
+            "ref_add_world_joint": (None, [vp, ctypes.c_int, f64p, f64p, ctypes.c_double, ctypes.c_double]),
+            "ref_world_joint_error": (ctypes.c_double, [vp, ctypes.c_int, f64p, f64p]),
+            "ref_box_mass": (None, [f64p, ctypes.c_double, f64p]),
+            "gs_create": (vp, [ctypes.c_int, ctypes.c_uint32, f32p, u32p, ctypes.c_uint32, f32p, ctypes.c_uint32, f32p,
+                               i32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+            "gs_destroy": (None, [vp]),
+            "gs_set_brute_force": (None, [vp, ctypes.c_int]),
+            "gs_set_state": (None, [vp, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
+            "gs_step": (None, [vp, ctypes.c_int]),
+            "gs_counts": (None, [vp, u32p]),
+            "gs_read_bodies": (None, [vp, f64p]),
+            "gs_read_joints": (None, [vp, f64p]),
+            "gs_read_pairs": (None, [vp, u32p]),
+            "gs_read_manifolds": (None, [vp, u32p]),
+            "gs_read_contacts": (None, [vp, u32p, f64p]),
+            "gs_read_contact_words": (None, [vp, u32p]),
+            "gs_read_colors": (None, [vp, u32p]),
+            "gs_read_adjacency": (ctypes.c_uint32, [vp, u32p, u32p]),
+            "gs_read_refs": (None, [vp, f32p]),
+            "gs_probe_box_contacts": (ctypes.c_int, [ctypes.c_int, f64p, f64p, f64p, f64p, ctypes.c_int, ctypes.c_int,
+                                                     u32p, f64p, f64p]),
+            "gs_probe_solve6": (None, [f64p, f64p, ctypes.c_double, f64p]),
+            "gs_probe_priority": (ctypes.c_uint32, [ctypes.c_uint32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def P(a, t):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def no_collide(info, ignore):
+    """(hi, lo, slot) triples as the engine builds them: joints with two bodies, then ignored pairs (slot ~0)."""
+    out = []
+    for j, (t, a, b, _) in enumerate(np.asarray(info).reshape(-1, 4)):
+        if a >= 0:
+            out.append((max(a, b), min(a, b), j))
+    for a, b in np.asarray(ignore).reshape(-1, 2):
+        out.append((max(a, b), min(a, b), 0xFFFFFFFF))
+    return np.array(out, np.uint32).reshape(-1, 3)
+
+
+class OracleWorld:
+    """The GPU-semantics oracle over packed records (several scenes = a batch)."""
+
+    def __init__(self, scenes, precision=32, color_cap=16, color_rounds=16, brute_force=False):
+        L = lib()
+        bodies, joints, info, ignore, filters, params, world = [], [], [], [], [], [], []
+        base = 0
+        for k, s in enumerate(scenes):
+            b, j, i, g, f = s.packed()
+            bodies.append(b)
+            joints.append(j)
+            i = i.copy()
+            i[:, 1] = np.where(i[:, 1] >= 0, i[:, 1] + base, -1)
+            i[:, 2] += base
+            info.append(i)
+            ignore.append(g.astype(np.int64) + base)
+            filters.append(f if f is not None else np.full((len(b), 2), 0xFFFFFFFF, np.uint32))
+            params.append(s.world_params())
+            world.append(np.full(len(b), k, np.uint32))
+            base += len(b)
+        self.bodies0 = np.ascontiguousarray(np.concatenate(bodies), np.float32)
+        self.joints0 = np.ascontiguousarray(np.concatenate(joints).reshape(-1, 32), np.float32)
+        self.info = np.ascontiguousarray(np.concatenate(info).reshape(-1, 4), np.int32)
+        ign = np.concatenate(ignore).reshape(-1, 2)
+        self.nc = np.ascontiguousarray(no_collide(self.info, ign))
+        self.filters = np.ascontiguousarray(np.concatenate(filters), np.uint32)
+        self.params = np.ascontiguousarray(np.stack(params), np.float32)
+        self.world = np.ascontiguousarray(np.concatenate(world), np.uint32)
+        self.n = len(self.bodies0)
+        self.nj = len(self.joints0)
+        self.nw = len(scenes)
+        self.precision = precision
+        self.h = L.gs_create(precision, self.n, P(self.bodies0, ctypes.c_float), P(self.world, ctypes.c_uint32), self.nw,
+                             P(self.params, ctypes.c_float), self.nj, P(self.joints0, ctypes.c_float),
+                             P(self.info, ctypes.c_int32), len(self.nc), P(self.nc, ctypes.c_uint32),
+                             P(self.filters, ctypes.c_uint32), color_cap, color_rounds,
+                             int(scenes[0].params["iterations"]))
+        self._cap, self._rounds, self._iters = color_cap, color_rounds, int(scenes[0].params["iterations"])
+        self._brute = brute_force
+        if brute_force:
+            L.gs_set_brute_force(self.h, 1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gs_destroy(self.h)
+            self.h = None
+
+    def seed(self, state):
+        """Start from a state read back from the HIP world (bodies, joints, refs, colours, contacts, steps)."""
+        L = lib()
+        # Rebuild with the given bodies and joints, then set the persistent state
+        bodies = np.ascontiguousarray(state["bodies"], np.float32)
+        joints = np.ascontiguousarray(state["joints"], np.float32).reshape(-1, 32)
+        cc = ctypes.c_uint32
+        old = self.h
+        counts = np.zeros(7, np.uint32)
+        L.gs_counts(old, P(counts, cc))
+        self.h = L.gs_create(self.precision, self.n, P(bodies, ctypes.c_float), P(self.world, cc), self.nw,
+                             P(self.params, ctypes.c_float), self.nj, P(joints, ctypes.c_float),
+                             P(self.info, ctypes.c_int32), len(self.nc), P(self.nc, cc), P(self.filters, cc),
+                             self._cap, self._rounds, self._iters)
+        L.gs_destroy(old)
+        if self._brute:
+            L.gs_set_brute_force(self.h, 1)
+        refs = np.ascontiguousarray(state["refs"], np.float32)
+        colors = np.ascontiguousarray(state["colors"], np.uint32)
+        mf = np.ascontiguousarray(state["manifolds"], np.uint32)
+        ct = np.ascontiguousarray(state["contacts"], np.uint32)
+        steps = np.ascontiguousarray(state["steps"], np.uint32)
+        L.gs_set_state(self.h, P(refs, ctypes.c_float), P(colors, cc), len(mf), P(mf, cc), len(ct), P(ct, cc),
+                       P(steps, cc))
+
+    def step(self, n=1):
+        lib().gs_step(self.h, n)
+
+    def counts(self):
+        c = np.zeros(7, np.uint32)
+        lib().gs_counts(self.h, P(c, ctypes.c_uint32))
+        return dict(pairs=int(c[0]), manifolds=int(c[1]), contacts=int(c[2]), clashes=int(c[3]), colors=int(c[4]))
+
+    def bodies(self):
+        out = np.zeros((self.n, 40))
+        lib().gs_read_bodies(self.h, P(out, ctypes.c_double))
+        return out
+
+    def joints(self):
+        out = np.zeros((self.nj, 32))
+        if self.nj:
+            lib().gs_read_joints(self.h, P(out, ctypes.c_double))
+        return out
+
+    def pairs(self):
+        out = np.zeros((self.counts()["pairs"], 2), np.uint32)
+        lib().gs_read_pairs(self.h, P(out, ctypes.c_uint32))
+        return out
+
+    def manifolds(self):
+        out = np.zeros((self.counts()["manifolds"], 8), np.uint32)
+        lib().gs_read_manifolds(self.h, P(out, ctypes.c_uint32))
+        return out
+
+    def contacts(self):
+        n = self.counts()["contacts"]
+        keys = np.zeros(n, np.uint32)
+        vals = np.zeros((n, 16))
+        lib().gs_read_contacts(self.h, P(keys, ctypes.c_uint32), P(vals, ctypes.c_double))
+        return keys, vals
+
+    def contact_words(self):
+        out = np.zeros((self.counts()["contacts"], 16), np.uint32)
+        lib().gs_read_contact_words(self.h, P(out, ctypes.c_uint32))
+        return out
+
+    def colors(self):
+        out = np.zeros(self.n, np.uint32)
+        lib().gs_read_colors(self.h, P(out, ctypes.c_uint32))
+        return out
+
+    def adjacency(self):
+        L = lib()
+        start = np.zeros(self.n + 1, np.uint32)
+        m = L.gs_read_adjacency(self.h, P(start, ctypes.c_uint32), None)
+        lst = np.zeros(m, np.uint32)
+        L.gs_read_adjacency(self.h, P(start, ctypes.c_uint32), P(lst, ctypes.c_uint32))
+        return start, lst
+
+    def refs(self):
+        out = np.zeros((self.n, 8), np.float32)
+        lib().gs_read_refs(self.h, P(out, ctypes.c_float))
+        return out
