@@ -55,6 +55,9 @@ class Options(ctypes.Structure):
         ("pair_capacity", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
         ("device", ctypes.c_int32),
+        ("adaptive", ctypes.c_uint32),
+        ("broadphase", ctypes.c_uint32),
+        ("bvh_rebuild_interval", ctypes.c_uint32),
     ]
 
 

@@ -88,6 +88,10 @@ EngineOptions toOptions(const AvbdOptions *o) {
     if (o->color_rounds) e.colorRounds = o->color_rounds;
     e.pairCapacity = o->pair_capacity;
     e.useGraph = o->use_graph != 0;
+    e.adaptive = o->adaptive != 0;
+    if (o->broadphase > 2) throw std::invalid_argument("broadphase must be 0 (auto), 1 (grid) or 2 (hploc)");
+    e.broadphase = o->broadphase;
+    if (o->bvh_rebuild_interval) e.rebuildInterval = o->bvh_rebuild_interval;
     e.device = o->device;
   }
   if (e.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");

@@ -9,6 +9,7 @@
 #include "engine_types.h"
 #include "kernels.h"
 #include "narrowphase.h"
+#include "pairtest.h"
 
 namespace avbd {
 
@@ -119,83 +120,9 @@ __global__ void k_grid_scatter(DeviceWorld w) {
   uint32_t h = cellHash(w, c);
   uint32_t slot = w.gridCount[h] + atomicAdd(&w.gridCursor[h], 1u);
   w.gridSorted[slot] = i;
-}
-
-/** Body i as the pair tests see it (wgsl-collision.ts:138-171). */
-struct Shape {
-  f3 ax[3];
-  f3 h;
-  bool sphere;
-};
-
-AVBD_D Shape shapeOf(const Body &b) {
-  Shape s;
-  s.h = scale(xyz(b.size), 0.5f);
-  s.sphere = b.angVel.w == SHAPE_SPHERE;
-  s.ax[0] = qrotate(b.rot, mk3(1.f, 0.f, 0.f));
-  s.ax[1] = qrotate(b.rot, mk3(0.f, 1.f, 0.f));
-  s.ax[2] = qrotate(b.rot, mk3(0.f, 0.f, 1.f));
-  return s;
-}
-AVBD_D float extent(const Shape &s, f3 n) {
-  if (s.sphere) return s.h.x;
-  return s.h.x * fabsf(dot(n, s.ax[0])) + s.h.y * fabsf(dot(n, s.ax[1])) + s.h.z * fabsf(dot(n, s.ax[2]));
-}
-AVBD_D f3 aabbHalf(const Shape &s) {
-  if (s.sphere) return mk3(s.h.x, s.h.x, s.h.x);
-  return add(add(scale(vabs(s.ax[0]), s.h.x), scale(vabs(s.ax[1]), s.h.y)), scale(vabs(s.ax[2]), s.h.z));
-}
-AVBD_D bool faceSeparated(const Shape &A, const Shape &B, f3 d) {
-  for (int k = 0; k < 3; k++) {
-    if (fabsf(dot(d, A.ax[k])) - extent(A, A.ax[k]) - extent(B, A.ax[k]) > 1e-3f) return true;
-    if (fabsf(dot(d, B.ax[k])) - extent(A, B.ax[k]) - extent(B, B.ax[k]) > 1e-3f) return true;
-  }
-  return false;
-}
-
-/** Binary search of the (hi, lo) no-collide list; false if every connecting joint has broken (wgsl-collision.ts:112). */
-AVBD_D bool ignored(const DeviceWorld &w, uint32_t hi, uint32_t lo) {
-  uint32_t first = 0, count = w.noCollideCount;
-  while (count > 0) {
-    uint32_t step = count / 2, mid = first + step;
-    uint32_t mh = w.noCollide[3 * mid], ml = w.noCollide[3 * mid + 1];
-    if (mh < hi || (mh == hi && ml < lo)) {
-      first = mid + 1;
-      count -= step + 1;
-    } else {
-      count = step;
-    }
-  }
-  for (uint32_t k = first; k < w.noCollideCount; k++) {
-    if (w.noCollide[3 * k] != hi || w.noCollide[3 * k + 1] != lo) break;
-    uint32_t j = w.noCollide[3 * k + 2];
-    if (j == 0xffffffffu) return true;
-    if (w.joints[j].penLin.w != 0.f || w.joints[j].penAng.w != 0.f) return true;
-  }
-  return false;
-}
-
-/**
- * The pair predicate of testPair (wgsl-collision.ts:196-221), evaluated with the higher index a as the probe, so a
- * pair's acceptance does not depend on which thread or broadphase structure found it.
- */
-AVBD_D bool acceptPair(const DeviceWorld &w, uint32_t a, uint32_t b) {
-  if (w.bodyWorld[a] != w.bodyWorld[b]) return false;
-  const Body &A = w.bodies[a];
-  const Body &B = w.bodies[b];
-  if (!(A.size.w > 0.f) && B.size.w <= 0.f) return false;
-  uint2 fa = w.filters[a], fb = w.filters[b];
-  if ((fa.x & fb.y) == 0u || (fb.x & fa.y) == 0u) return false;
-  f3 d = sub(xyz(A.pos), xyz(B.pos));
-  float r = A.moment.w + B.moment.w;
-  if (dot(d, d) > r * r) return false;
-  Shape sa = shapeOf(A), sb = shapeOf(B);
-  f3 ha = aabbHalf(sa), hb = aabbHalf(sb);
-  if (fabsf(d.x) > ha.x + hb.x + 1e-4f || fabsf(d.y) > ha.y + hb.y + 1e-4f || fabsf(d.z) > ha.z + hb.z + 1e-4f)
-    return false;
-  if (faceSeparated(sa, sb, d)) return false;
-  if (ignored(w, a, b)) return false;
-  return true;
+  float4 p = w.bodies[i].pos;
+  w.sortedPR[slot] = make_float4(p.x, p.y, p.z, w.bodies[i].moment.w);
+  w.sortedCell[slot] = c;
 }
 
 /**
@@ -218,20 +145,41 @@ template <bool WRITE> __global__ void k_pairs(DeviceWorld w) {
   bool small = c.w >= 0;
   uint32_t wid = w.bodyWorld[i];
   if (small) {
-    for (int oz = -1; oz <= 1; oz++)
-      for (int oy = -1; oy <= 1; oy++)
-        for (int ox = -1; ox <= 1; ox++) {
-          int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
-          uint32_t h = cellHash(w, cell);
-          uint32_t kEnd = w.gridCount[h + 1];
-          for (uint32_t k = w.gridCount[h]; k < kEnd; k++) {
-            uint32_t j = w.gridSorted[k];
-            if (j >= i) continue;
-            int4 cj = w.cellOf[j];
-            if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z || cj.w != cell.w) continue;
-            emit(i, j);
-          }
-        }
+    // Half shell: the body's own cell (lower indices only) and the 13 cells lexicographically after it, so each
+    // pair of small bodies is visited once; the cheap bounding-sphere test runs on the cell-sorted copies first
+    const float4 pi = make_float4(w.bodies[i].pos.x, w.bodies[i].pos.y, w.bodies[i].pos.z, w.bodies[i].moment.w);
+    for (int o = 0; o < 14; o++) {
+      int ox, oy, oz;
+      if (o == 0) {
+        ox = oy = oz = 0;
+      } else if (o < 10) {
+        oz = 1;
+        oy = (o - 1) / 3 - 1;
+        ox = (o - 1) % 3 - 1;
+      } else if (o < 13) {
+        oz = 0;
+        oy = 1;
+        ox = o - 11;
+      } else {
+        oz = 0;
+        oy = 0;
+        ox = 1;
+      }
+      int4 cell = make_int4(c.x + ox, c.y + oy, c.z + oz, c.w);
+      uint32_t h = cellHash(w, cell);
+      uint32_t kEnd = w.gridCount[h + 1];
+      for (uint32_t k = w.gridCount[h]; k < kEnd; k++) {
+        int4 cj = w.sortedCell[k];
+        if (cj.x != cell.x || cj.y != cell.y || cj.z != cell.z || cj.w != cell.w) continue;
+        uint32_t j = w.gridSorted[k];
+        if (o == 0 && j >= i) continue;
+        float4 pj = w.sortedPR[k];
+        f3 d = j < i ? mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) : mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        float r = j < i ? pi.w + pj.w : pj.w + pi.w;
+        if (dot(d, d) > r * r) continue;
+        emit(umax(i, j), umin(i, j));
+      }
+    }
   }
   for (uint32_t k = w.largeStart[wid]; k < w.largeStart[wid + 1]; k++) {
     uint32_t l = w.largeList[k];
@@ -419,13 +367,21 @@ __global__ void k_manifold_flags(DeviceWorld w) {
 
 __global__ void k_manifold_compact(DeviceWorld w) {
   uint32_t P = w.counters[C_PAIRS];
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-    uint32_t o = w.mfOffset[p];
-    if (w.mfOffset[p + 1] != o) {
-      Manifold m = w.mfSlots[p];
-      w.manifolds[o] = m;
-      atomicAdd(&w.counters[C_CONTACTS], m.ids.w & 15u);
+  // The loop bound is uniform per block (the wavefront reduction below needs every lane)
+  for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
+    uint32_t p = base + threadIdx.x;
+    uint32_t points = 0;
+    if (p < P) {
+      uint32_t o = w.mfOffset[p];
+      if (w.mfOffset[p + 1] != o) {
+        Manifold m = w.mfSlots[p];
+        w.manifolds[o] = m;
+        points = m.ids.w & 15u;
+      }
     }
+    // One atomic per wavefront for the contact total
+    for (int d = 32; d > 0; d >>= 1) points += __shfl_xor(points, d, 64);
+    if ((threadIdx.x & 63) == 0 && points) atomicAdd(&w.counters[C_CONTACTS], points);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
 }
@@ -462,6 +418,7 @@ void launch_pairs(const DeviceWorld &w, hipStream_t s) {
   k_pairs<true><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
   k_pairs_total<<<1, 1, 0, s>>>(w);
 }
+void launch_pairs_total(const DeviceWorld &w, hipStream_t s) { k_pairs_total<<<1, 1, 0, s>>>(w); }
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
   k_narrowphase<<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
   k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "engine_types.h"
+#include "hploc.h"
 
 namespace avbd {
 
@@ -27,7 +28,10 @@ struct EngineOptions {
   uint32_t colorRounds = 16;    // Jones–Plassmann rounds per step (even)
   uint32_t pairCapacity = 0;    // 0: chosen from the body count
   bool useGraph = false;
+  bool adaptive = false;        // adapt colour cap/rounds and storage from counters read back asynchronously
   int device = 0;
+  uint32_t broadphase = 0;      // 0 auto (H-PLOC for one world of >= 10,000 bodies, else grid), 1 grid, 2 H-PLOC
+  uint32_t rebuildInterval = 64;  // H-PLOC: rebuild every this many steps, refit in between
 };
 
 /** Milliseconds of one profiled step by phase, measured with HIP events on the engine's stream. */
@@ -85,8 +89,11 @@ class Engine {
   void setColorRounds(uint32_t rounds);
   uint32_t colorCap() const { return colorCap_; }
   uint32_t pairCapacity() const { return pairCap_; }
+  bool usesBvh() const { return useBvh_; }
   /** Grow contact storage / colour cap from the last counters (the reference's adapt); returns true if it changed. */
   bool adapt();
+  /** The reference's adapt policy applied to given counters (solver.ts:1484-1519). */
+  bool adaptFrom(const Counters &c);
 
   /** Device bytes held by this engine. */
   size_t deviceBytes() const { return deviceBytes_; }
@@ -94,10 +101,13 @@ class Engine {
  private:
   void allocate();
   void allocateContacts(uint32_t pairCap);
-  void encodeStep(const DeviceWorld &w);
+  void encodeStep(const DeviceWorld &w, bool rebuild);
+  void encodeCollision(const DeviceWorld &w, bool rebuild);
+  void allocateBvh(const std::vector<uint32_t> &counts);
   void *dalloc(size_t bytes);
   template <class T> T *dallocT(size_t count) { return static_cast<T *>(dalloc(count * sizeof(T))); }
   void resetGraphs();
+  void pollCounters();
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -115,7 +125,19 @@ class Engine {
   DeviceWorld views_[2]{};
   Manifold *manifoldBuf_[2] = {nullptr, nullptr};
   Contact *contactBuf_[2] = {nullptr, nullptr};
-  hipGraphExec_t graphs_[2] = {nullptr, nullptr};
+  hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
+  bool useBvh_ = false;
+  uint32_t rebuildInterval_ = 64;
+  uint64_t stepIndex_ = 0;
+  Bvh bvh_{};
+  BvhPlan plan_;
+  // Asynchronous counter readback (pinned host copies, one per in-flight slot)
+  bool adaptive_ = false;
+  uint32_t *hostCounters_ = nullptr;
+  hipEvent_t counterEvents_[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool counterPending_[4] = {false, false, false, false};
+  uint32_t counterSlot_ = 0;
+  uint32_t shrinkVotes_ = 0;
 };
 
 /** Largest bounding radius still sized into grid cells (reference/three-avbd/src/avbd3d/gpu/radii.ts:47-55). */

@@ -61,6 +61,12 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   HIP_CHECK(hipSetDevice(device_));
   HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   useGraph_ = opt.useGraph;
+  adaptive_ = opt.adaptive;
+  if (adaptive_) {
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hostCounters_), 4 * COUNTER_WORDS * sizeof(uint32_t),
+                            hipHostMallocDefault));
+    for (auto &ev : counterEvents_) HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
   colorCap_ = std::min<uint32_t>(std::max<uint32_t>(opt.colorCap, 1), MAX_COLORS);
   colorRounds_ = opt.colorRounds + (opt.colorRounds & 1u);
   nWorlds_ = uint32_t(worlds.size());
@@ -135,6 +141,10 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   pairCap_ = opt.pairCapacity ? opt.pairCapacity : std::max<uint32_t>(4096, 8 * nBodies_);
   tableSize_ = pow2AtLeast(2ull * std::max<uint32_t>(nBodies_, 1));
   colorGroups_ = (std::max<uint32_t>(nBodies_, 1) + COLOR_WG - 1) / COLOR_WG;
+  rebuildInterval_ = std::max<uint32_t>(opt.rebuildInterval, 1);
+  if (opt.broadphase == 2) useBvh_ = true;
+  else if (opt.broadphase == 1) useBvh_ = false;
+  else useBvh_ = nWorlds_ == 1 && nBodies_ >= 10000;
 
   DeviceWorld &w = base_;
   w.nBodies = nBodies_;
@@ -154,6 +164,8 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.gridCursor = dallocT<uint32_t>(tableSize_);
   w.gridSorted = dallocT<uint32_t>(nBodies_);
   w.cellOf = dallocT<int4>(nBodies_);
+  w.sortedPR = dallocT<float4>(nBodies_);
+  w.sortedCell = dallocT<int4>(nBodies_);
   w.tableMask = tableSize_ - 1;
   w.cellSize = float(cellSize);
   w.largeStart = dallocT<uint32_t>(nWorlds_ + 1);
@@ -190,8 +202,38 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   up(w.largeList, largeList.data(), largeList.size() * 4);
   std::vector<uint32_t> noColor(nBodies_, NO_COLOR);
   up(w.colorA, noColor.data(), noColor.size() * 4);
+  if (useBvh_) {
+    std::vector<uint32_t> counts(nWorlds_);
+    for (uint32_t k = 0; k < nWorlds_; k++) counts[k] = uint32_t(worlds[k].bodies.size());
+    allocateBvh(counts);
+  }
   allocateContacts(pairCap_);
   HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
+  for (uint32_t c : counts)
+    if (c == 0) throw std::invalid_argument("H-PLOC needs at least one body per world");
+  plan_ = makeBvhPlan(counts);
+  const uint32_t n = nBodies_;
+  const uint32_t tiles = (n + 255) / 256;
+  bvh_.nodeCount = plan_.nodeCount;
+  bvh_.nodes = dallocT<BvhNode>(plan_.nodeCount);
+  bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
+  bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
+  bvh_.keys = dallocT<uint64_t>(n);
+  bvh_.keysAlt = dallocT<uint64_t>(n);
+  bvh_.vals = dallocT<uint32_t>(n);
+  bvh_.valsAlt = dallocT<uint32_t>(n);
+  bvh_.clustersA = dallocT<uint32_t>(n);
+  bvh_.clustersB = dallocT<uint32_t>(n);
+  bvh_.roots = dallocT<uint32_t>(nWorlds_);
+  bvh_.bounds = dallocT<uint32_t>(6ull * nWorlds_);
+  bvh_.sortHist = dallocT<uint32_t>(256ull * tiles + 1);
+  bvh_.groups = dallocT<BvhGroup>(std::max<size_t>(plan_.groups.size(), 1));
+  if (!plan_.groups.empty())
+    HIP_CHECK(hipMemcpyAsync(bvh_.groups, plan_.groups.data(), plan_.groups.size() * sizeof(BvhGroup),
+                             hipMemcpyHostToDevice, stream_));
 }
 
 void Engine::allocateContacts(uint32_t pairCap) {
@@ -235,7 +277,8 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.adjCapacity = 2 * (nJoints_ + pairCap_);
   w.adjList = static_cast<uint32_t *>(alloc(size_t(w.adjCapacity) * 4));
   uint32_t scanNeed = scan_scratch_words(std::max<uint32_t>({tableSize_ + 2, nBodies_ + 1, pairCap_ + 1,
-                                                              MAX_COLORS * colorGroups_ + 1}));
+                                                              MAX_COLORS * colorGroups_ + 1,
+                                                              256 * ((nBodies_ + 255) / 256) + 1}));
   w.scanScratch = static_cast<uint32_t *>(alloc(size_t(scanNeed) * 4));
   contactBytes_ = deviceBytes_ - bytesBefore;
   for (int p = 0; p < 2; p++) {
@@ -262,6 +305,9 @@ void Engine::allocateContacts(uint32_t pairCap) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   resetGraphs();
+  for (auto &ev : counterEvents_)
+    if (ev) (void)hipEventDestroy(ev);
+  if (hostCounters_) (void)hipHostFree(hostCounters_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : contactAllocations_) (void)hipFree(p);
   for (void *p : allocations_) (void)hipFree(p);
@@ -276,14 +322,26 @@ void Engine::resetGraphs() {
     }
 }
 
-void Engine::encodeStep(const DeviceWorld &w) {
+void Engine::encodeCollision(const DeviceWorld &w, bool rebuild) {
   hipStream_t s = stream_;
   launch_begin_frame(w, s);
   launch_update_refs(w, s);
   launch_hash_build(w, pairCap_, s);
-  launch_grid(w, s);
-  launch_pairs(w, s);
+  if (useBvh_) {
+    if (rebuild) launch_bvh_build(w, bvh_, plan_, s);
+    else launch_bvh_refit(w, bvh_, plan_, s);
+    launch_bvh_pairs(w, bvh_, s);
+    launch_pairs_total(w, s);
+  } else {
+    launch_grid(w, s);
+    launch_pairs(w, s);
+  }
   launch_narrowphase(w, s);
+}
+
+void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
+  hipStream_t s = stream_;
+  encodeCollision(w, rebuild);
   launch_adjacency(w, pairCap_, s);
   launch_coloring(w, colorRounds_, s);
   launch_warm_start(w, s);
@@ -294,25 +352,50 @@ void Engine::encodeStep(const DeviceWorld &w) {
   launch_velocities(w, s);
 }
 
+void Engine::pollCounters() {
+  // The oldest in-flight readback that has landed drives the policy (one or more steps late, as the reference's)
+  for (uint32_t k = 0; k < 4; k++) {
+    uint32_t slot = (counterSlot_ + k) % 4;
+    if (!counterPending_[slot]) continue;
+    if (hipEventQuery(counterEvents_[slot]) != hipSuccess) continue;
+    counterPending_[slot] = false;
+    const uint32_t *c = hostCounters_ + slot * COUNTER_WORDS;
+    Counters cc{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
+                c[C_PREV_MANIFOLDS]};
+    adaptFrom(cc);
+  }
+}
+
 void Engine::step(uint32_t n) {
   HIP_CHECK(hipSetDevice(device_));
   for (uint32_t k = 0; k < n; k++) {
+    if (adaptive_) pollCounters();
     DeviceWorld &w = views_[parity_];
     w.colorCap = colorCap_;
+    const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
     if (useGraph_) {
-      if (!graphs_[parity_]) {
+      hipGraphExec_t &exec = graphs_[parity_ * 2 + (rebuild ? 1 : 0)];
+      if (!exec) {
         hipGraph_t g;
         HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        encodeStep(w);
+        encodeStep(w, rebuild);
         HIP_CHECK(hipStreamEndCapture(stream_, &g));
-        HIP_CHECK(hipGraphInstantiate(&graphs_[parity_], g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
         HIP_CHECK(hipGraphDestroy(g));
       }
-      HIP_CHECK(hipGraphLaunch(graphs_[parity_], stream_));
+      HIP_CHECK(hipGraphLaunch(exec, stream_));
     } else {
-      encodeStep(w);
+      encodeStep(w, rebuild);
     }
+    stepIndex_++;
     parity_ ^= 1;
+    if (adaptive_ && !counterPending_[counterSlot_]) {
+      HIP_CHECK(hipMemcpyAsync(hostCounters_ + counterSlot_ * COUNTER_WORDS, base_.counters,
+                               COUNTER_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+      HIP_CHECK(hipEventRecord(counterEvents_[counterSlot_], stream_));
+      counterPending_[counterSlot_] = true;
+      counterSlot_ = (counterSlot_ + 1) % 4;
+    }
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -327,13 +410,9 @@ PhaseTimes Engine::profileStep() {
   hipStream_t s = stream_;
   DeviceWorld &w = views_[parity_];
   w.colorCap = colorCap_;
+  const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
   HIP_CHECK(hipEventRecord(ev[0], s));
-  launch_begin_frame(w, s);
-  launch_update_refs(w, s);
-  launch_hash_build(w, pairCap_, s);
-  launch_grid(w, s);
-  launch_pairs(w, s);
-  launch_narrowphase(w, s);
+  encodeCollision(w, rebuild);
   HIP_CHECK(hipEventRecord(ev[1], s));
   launch_adjacency(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2], s));
@@ -355,6 +434,7 @@ PhaseTimes Engine::profileStep() {
   launch_velocities(w, s);
   HIP_CHECK(hipEventRecord(ev[5], s));
   HIP_CHECK(hipEventSynchronize(ev[5]));
+  stepIndex_++;
   parity_ ^= 1;
   auto ms = [&](int a, int b) {
     float t = 0;
@@ -547,17 +627,35 @@ void Engine::setColorRounds(uint32_t rounds) {
   }
 }
 
-bool Engine::adapt() {
-  Counters c = readCounters();
+bool Engine::adapt() { return adaptFrom(readCounters()); }
+
+bool Engine::adaptFrom(const Counters &c) {
   bool changed = false;
-  // Colour cap: grow when a spare runs out or the colouring clashed (solver.ts:1495-1511)
-  if (c.clashes > 0 || colorCap_ - std::min(colorCap_, c.colors) < 2) {
-    uint32_t want = std::min<uint32_t>(MAX_COLORS, std::max<uint32_t>(c.colors + 5, colorCap_ + 4));
-    if (want != colorCap_) {
-      setColorCap(want);
-      changed = true;
+  // Colour cap and rounds (solver.ts:1495-1511): grow at once when a spare runs out or the colouring clashed; shrink
+  // to the colours in use + 3 after three quiet reads
+  const uint32_t SPARE = 3;
+  const uint32_t used = c.colors;
+  uint32_t cap = colorCap_, rounds = colorRounds_;
+  if (c.clashes > 0) rounds = std::min<uint32_t>(32, rounds * 2);
+  if (c.clashes > 0 || cap < used + 2) {
+    cap = std::min<uint32_t>(MAX_COLORS, std::max<uint32_t>(used + SPARE + 2, cap + 4));
+    shrinkVotes_ = 0;
+  } else if (used + SPARE + 2 < cap || rounds > 4) {
+    if (++shrinkVotes_ >= 3) {
+      cap = std::min<uint32_t>(MAX_COLORS, std::max<uint32_t>(used + SPARE, 1));
+      rounds = std::max<uint32_t>(4, rounds - 4);
+      shrinkVotes_ = 0;
     }
-    if (c.clashes > 0 && colorRounds_ < 32) setColorRounds(std::min<uint32_t>(32, colorRounds_ * 2));
+  } else {
+    shrinkVotes_ = 0;
+  }
+  if (cap != colorCap_) {
+    setColorCap(cap);
+    changed = true;
+  }
+  if (rounds != colorRounds_) {
+    setColorRounds(rounds);
+    changed = true;
   }
   // Storage: grow to twice the demand when full beyond 80% or overflowed (solver.ts:1512-1518)
   bool overflow = (c.overflow & 7u) != 0;

@@ -25,6 +25,8 @@ struct DeviceWorld {
   uint32_t *gridCursor;  // tableSize
   uint32_t *gridSorted;  // nBodies
   int4 *cellOf;          // per body: cell xyz, world (w = -1: large body)
+  float4 *sortedPR;      // cell-sorted copies of (pos.xyz, bounding radius)
+  int4 *sortedCell;      // cell-sorted copies of cellOf
   uint32_t tableMask;
   float cellSize;
   uint32_t *largeStart;  // nWorlds + 1

@@ -1,0 +1,88 @@
+// Host/device types of the H-PLOC broadphase: node buffers, sort buffers and the level plan.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "engine_types.h"
+
+namespace avbd {
+
+/** A node: lo.xyz + left child (leaf: body index), hi.xyz + right child (leaf: ~0), children as u32 bits. */
+struct BvhNode {
+  float4 lo, hi;
+};
+
+/** One treelet of a level: its input clusters [inStart, inStart + count), output slot, first node it creates. */
+struct BvhGroup {
+  uint32_t inStart, count, outIndex, nodeBase;
+};
+
+struct BvhLevel {
+  uint32_t groupOffset, groupCount;
+};
+
+/** Level plan: from the body count of each world (static), the groups of every level down to one root per world. */
+struct BvhPlan {
+  std::vector<BvhLevel> levels;
+  std::vector<BvhGroup> groups;
+  uint32_t nodeCount = 0;
+  uint32_t sortPasses = 4;
+};
+
+struct Bvh {
+  BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
+  uint32_t *parent;    // per node
+  uint32_t *escape;    // per node
+  uint64_t *keys, *keysAlt;
+  uint32_t *vals, *valsAlt;
+  uint32_t *clustersA, *clustersB;
+  uint32_t *roots;     // per world
+  uint32_t *bounds;    // per world: min xyz, max xyz (order-preserving u32)
+  uint32_t *sortHist;  // 256 × tiles + 1
+  BvhGroup *groups;    // every level's groups
+  uint32_t nodeCount;
+};
+
+/** The plan for worlds of the given body counts (bodies of world k follow those of worlds 0..k-1). */
+inline BvhPlan makeBvhPlan(const std::vector<uint32_t> &counts) {
+  BvhPlan plan;
+  uint32_t n = 0, w = uint32_t(counts.size());
+  for (uint32_t c : counts) n += c;
+  plan.nodeCount = 2 * n - w;
+  uint32_t worldBits = 0;
+  while ((1u << worldBits) < w) worldBits++;
+  plan.sortPasses = (30 + worldBits + 7) / 8;
+  std::vector<uint32_t> c(counts.begin(), counts.end());
+  uint32_t nodeBase = n;
+  auto more = [&] {
+    for (uint32_t x : c)
+      if (x > 1) return true;
+    return false;
+  };
+  while (more()) {
+    BvhLevel lev{uint32_t(plan.groups.size()), 0};
+    uint32_t inOff = 0, outOff = 0;
+    std::vector<uint32_t> next(c.size());
+    for (size_t k = 0; k < c.size(); k++) {
+      uint32_t g = (c[k] + 15) / 16;
+      for (uint32_t q = 0; q < g; q++) {
+        uint32_t count = std::min<uint32_t>(16, c[k] - 16 * q);
+        plan.groups.push_back(BvhGroup{inOff + 16 * q, count, outOff + q, nodeBase});
+        nodeBase += count - 1;
+        lev.groupCount++;
+      }
+      inOff += c[k];
+      outOff += g;
+      next[k] = g;
+    }
+    plan.levels.push_back(lev);
+    c = next;
+  }
+  return plan;
+}
+
+void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s);
+void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s);
+void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s);
+
+}  // namespace avbd

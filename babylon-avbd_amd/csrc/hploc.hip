@@ -1,0 +1,381 @@
+// H-PLOC bounding-volume hierarchy broadphase on gfx950: per-step leaf refit, Morton ordering (world id above the
+// 30-bit code, so each world's leaves stay contiguous), a stable LSD radix sort, 16-cluster treelets merged by mutual
+// nearest neighbours under the surface-area cost, level by level until one root per world, escape ropes for stackless
+// traversal, and queries that feed the same pair predicate as the grid (so the pair set does not depend on the
+// broadphase). Semantics of the reference's portable variant: src/gpu/hplocpp.wgsl:35-268, hplocSort.wgsl:11-48,
+// gpuHploc.js:24-56 (findTreePairs), 207-375; rebuild every 64 steps, refit every step (docs/HPLOC.md:14-33).
+//
+// The level plan (which clusters each treelet merges, where its nodes go) depends only on the body count of each
+// world, so the host computes it once and every level is one launch with 4 treelets per 64-lane wavefront.
+#include "engine_types.h"
+#include "hploc.h"
+#include "kernels.h"
+#include "pairtest.h"
+
+#include <utility>
+
+namespace avbd {
+
+constexpr uint32_t END = 0xffffffffu;
+constexpr uint32_t WIDTH = 16;
+
+AVBD_D f3 fmin3(f3 a, f3 b) { return mk3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+AVBD_D f3 fmax3(f3 a, f3 b) { return mk3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+
+/** World-space AABB half extents of a body (a sphere: its radius), as the pair test computes them. */
+AVBD_D f3 bodyHalf(const Body &b) {
+  f3 h = scale(xyz(b.size), 0.5f);
+  if (b.angVel.w == SHAPE_SPHERE) return mk3(h.x, h.x, h.x);
+  f3 a0 = qrotate(b.rot, mk3(1.f, 0.f, 0.f)), a1 = qrotate(b.rot, mk3(0.f, 1.f, 0.f)),
+     a2 = qrotate(b.rot, mk3(0.f, 0.f, 1.f));
+  return add(add(scale(vabs(a0), h.x), scale(vabs(a1), h.y)), scale(vabs(a2), h.z));
+}
+
+/** Leaf boxes from the current poses, padded so that float rounding never drops a pair the exact test accepts. */
+__global__ void k_bvh_leaves(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  const Body &b = w.bodies[i];
+  f3 p = xyz(b.pos);
+  f3 h = bodyHalf(b);
+  float pad = 1e-3f + 1e-6f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + h.x + h.y + h.z);
+  h = add(h, mk3(pad, pad, pad));
+  t.nodes[i] = BvhNode{mk4(sub(p, h), __uint_as_float(i)), mk4(add(p, h), __uint_as_float(END))};
+}
+
+AVBD_D uint32_t expandBits(uint32_t v) {
+  uint32_t x = v & 1023u;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  return (x | (x << 2)) & 0x09249249u;
+}
+
+/** Per-world scene bounds of the leaf centres (atomic min/max on order-preserving integer encodings). */
+AVBD_D uint32_t orderedBits(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+AVBD_D float fromOrdered(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+__global__ void k_bvh_bounds_init(Bvh t, uint32_t nWorlds) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nWorlds) return;
+  for (int c = 0; c < 3; c++) {
+    t.bounds[6 * k + c] = 0xffffffffu;  // min
+    t.bounds[6 * k + 3 + c] = 0u;       // max
+  }
+}
+
+__global__ void k_bvh_bounds(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  float4 p = w.bodies[i].pos;
+  uint32_t k = w.bodyWorld[i];
+  float c[3] = {p.x, p.y, p.z};
+  for (int a = 0; a < 3; a++) {
+    atomicMin(&t.bounds[6 * k + a], orderedBits(c[a]));
+    atomicMax(&t.bounds[6 * k + 3 + a], orderedBits(c[a]));
+  }
+}
+
+/** Key = world << 30 | 30-bit Morton code of the body's centre within its world's bounds; value = body index. */
+__global__ void k_bvh_morton(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  uint32_t k = w.bodyWorld[i];
+  float4 p = w.bodies[i].pos;
+  float c[3] = {p.x, p.y, p.z};
+  uint32_t q[3];
+  for (int a = 0; a < 3; a++) {
+    float lo = fromOrdered(t.bounds[6 * k + a]), hi = fromOrdered(t.bounds[6 * k + 3 + a]);
+    float ext = fmaxf(hi * 0.5f - lo * 0.5f, 1e-20f);
+    float u = fminf(fmaxf((c[a] * 0.5f - lo * 0.5f) / ext, 0.f), 0.999999f);
+    q[a] = uint32_t(u * 1024.f);
+  }
+  uint32_t code = (expandBits(q[0]) << 2) | (expandBits(q[1]) << 1) | expandBits(q[2]);
+  t.keys[i] = (uint64_t(k) << 30) | code;
+  t.vals[i] = i;
+}
+
+// --- stable LSD radix sort, 8-bit digits, 256 keys per tile ------------------------------------------------------
+
+constexpr uint32_t SORT_TILE = 256;
+
+__global__ void __launch_bounds__(SORT_TILE) k_sort_count(const uint64_t *keys, uint32_t n, uint32_t shift,
+                                                          uint32_t tiles, uint32_t *hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t i = blockIdx.x * SORT_TILE + threadIdx.x;
+  if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) hist[256 * tiles] = 0;
+}
+
+__global__ void __launch_bounds__(SORT_TILE) k_sort_scatter(const uint64_t *keys, const uint32_t *vals, uint32_t n,
+                                                            uint32_t shift, uint32_t tiles, const uint32_t *hist,
+                                                            uint64_t *keysOut, uint32_t *valsOut) {
+  __shared__ uint32_t digits[SORT_TILE];
+  uint32_t i = blockIdx.x * SORT_TILE + threadIdx.x;
+  uint32_t d = 256;  // past every digit: padding lanes never match
+  uint64_t key = 0;
+  if (i < n) {
+    key = keys[i];
+    d = uint32_t(key >> shift) & 255u;
+  }
+  digits[threadIdx.x] = d;
+  __syncthreads();
+  if (i >= n) return;
+  uint32_t rank = 0;
+  for (uint32_t j = 0; j < threadIdx.x; j++) rank += digits[j] == d ? 1u : 0u;
+  uint32_t dst = hist[d * tiles + blockIdx.x] + rank;
+  keysOut[dst] = key;
+  valsOut[dst] = vals[i];
+}
+
+// --- treelets ---------------------------------------------------------------------------------------------------------
+
+AVBD_D float area(f3 lo, f3 hi) {
+  f3 d = mk3(fminf(fmaxf(hi.x - lo.x, 0.f), 1e18f), fminf(fmaxf(hi.y - lo.y, 0.f), 1e18f),
+             fminf(fmaxf(hi.z - lo.z, 0.f), 1e18f));
+  return d.x * d.y + d.y * d.z + d.z * d.x;
+}
+
+/**
+ * One level: each group of up to 16 clusters (consecutive in Morton order, within one world) merges mutual nearest
+ * neighbours (smallest union surface area, ties to the lower lane) until one cluster is left. A 64-lane block holds
+ * four groups; the loop runs until every group of the block is done.
+ */
+__global__ void __launch_bounds__(64) k_bvh_level(Bvh t, const BvhGroup *groups, uint32_t nGroups,
+                                                  const uint32_t *in, uint32_t *out) {
+  __shared__ uint32_t alive[64], id[64], best[64], flag[64];
+  __shared__ float4 lo[64], hi[64];
+  __shared__ uint32_t created[4], survivors[4];
+  const uint32_t lane = threadIdx.x % WIDTH, g = threadIdx.x / WIDTH;
+  const uint32_t gi = blockIdx.x * 4 + g;
+  const bool groupLive = gi < nGroups;
+  BvhGroup grp = groupLive ? groups[gi] : BvhGroup{0, 0, 0, 0};
+  const bool live = groupLive && lane < grp.count;
+  alive[threadIdx.x] = live ? 1u : 0u;
+  if (live) {
+    uint32_t c = in[grp.inStart + lane];
+    id[threadIdx.x] = c;
+    BvhNode nd = t.nodes[c];
+    lo[threadIdx.x] = nd.lo;
+    hi[threadIdx.x] = nd.hi;
+  }
+  if (lane == 0) {
+    created[g] = 0;
+    survivors[g] = groupLive ? grp.count : 0;
+  }
+  __syncthreads();
+  const uint32_t base = g * WIDTH;
+  for (int round = 0; round < 64; round++) {
+    bool any = false;
+    for (int q = 0; q < 4; q++) any |= survivors[q] > 1;
+    if (!any) break;
+    uint32_t winner = END;
+    float cost = 3.402823e38f;
+    if (alive[threadIdx.x] && survivors[g] > 1) {
+      f3 alo = xyz(lo[threadIdx.x]), ahi = xyz(hi[threadIdx.x]);
+      for (uint32_t j = 0; j < WIDTH; j++) {
+        if (j == lane || !alive[base + j]) continue;
+        float c = area(fmin3(alo, xyz(lo[base + j])), fmax3(ahi, xyz(hi[base + j])));
+        if (c < cost || (c == cost && j < winner)) {
+          cost = c;
+          winner = j;
+        }
+      }
+    }
+    best[threadIdx.x] = winner;
+    __syncthreads();
+    flag[threadIdx.x] = (alive[threadIdx.x] && winner != END && lane < winner && best[base + winner] == lane) ? 1u : 0u;
+    __syncthreads();
+    uint32_t rank = 0, merged = 0;
+    for (uint32_t j = 0; j < WIDTH; j++) {
+      if (j < lane) rank += flag[base + j];
+      merged += flag[base + j];
+    }
+    uint32_t aID = 0, bID = 0;
+    float4 nlo = make_float4(0, 0, 0, 0), nhi = nlo;
+    if (flag[threadIdx.x]) {
+      aID = id[threadIdx.x];
+      bID = id[base + winner];
+      f3 l = fmin3(xyz(lo[threadIdx.x]), xyz(lo[base + winner]));
+      f3 h = fmax3(xyz(hi[threadIdx.x]), xyz(hi[base + winner]));
+      nlo = mk4(l, __uint_as_float(aID));
+      nhi = mk4(h, __uint_as_float(bID));
+    }
+    __syncthreads();
+    if (flag[threadIdx.x]) {
+      uint32_t nid = grp.nodeBase + created[g] + rank;
+      t.nodes[nid] = BvhNode{nlo, nhi};
+      t.parent[aID] = nid;
+      t.parent[bID] = nid;
+      id[threadIdx.x] = nid;
+      lo[threadIdx.x] = nlo;
+      hi[threadIdx.x] = nhi;
+      alive[base + winner] = 0;
+    }
+    __syncthreads();
+    if (lane == 0 && groupLive) {
+      created[g] += merged;
+      survivors[g] -= merged;
+    }
+    __syncthreads();
+  }
+  if (live && alive[threadIdx.x]) {
+    out[grp.outIndex] = id[threadIdx.x];
+    t.parent[id[threadIdx.x]] = END;  // a root until the next level links it
+  }
+}
+
+/** Refit one level's treelets: nodes in creation order (children before parents). */
+__global__ void k_bvh_refit_level(Bvh t, const BvhGroup *groups, uint32_t nGroups) {
+  uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= nGroups) return;
+  BvhGroup grp = groups[gi];
+  for (uint32_t j = 0; j + 1 < grp.count; j++) {
+    uint32_t nid = grp.nodeBase + j;
+    BvhNode n = t.nodes[nid];
+    BvhNode a = t.nodes[__float_as_uint(n.lo.w)];
+    BvhNode b = t.nodes[__float_as_uint(n.hi.w)];
+    t.nodes[nid] = BvhNode{mk4(fmin3(xyz(a.lo), xyz(b.lo)), n.lo.w), mk4(fmax3(xyz(a.hi), xyz(b.hi)), n.hi.w)};
+  }
+}
+
+/** Escape rope of each node: the right sibling of the first ancestor it descends from on the left (END at a root). */
+__global__ void k_bvh_ropes(Bvh t, uint32_t nNodes) {
+  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nNodes) return;
+  uint32_t at = id, escape = END;
+  for (int depth = 0; depth < 256; depth++) {
+    uint32_t p = t.parent[at];
+    if (p == END) break;
+    BvhNode parent = t.nodes[p];
+    if (__float_as_uint(parent.lo.w) == at) {
+      escape = __float_as_uint(parent.hi.w);
+      break;
+    }
+    at = p;
+  }
+  t.escape[id] = escape;
+}
+
+/**
+ * Pairs of dynamic body i from its world's tree: leaves overlapping its (padded) box, dynamic partners of lower index
+ * and static partners of any index (gpuHploc.js:37-55), each through the pair predicate. Count pass / write pass as
+ * the grid's, the write pass sorting the body's pairs by partner.
+ */
+template <bool WRITE> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  uint32_t base = WRITE ? w.pairCount[i] : 0;
+  uint32_t n = 0;
+  if (w.bodies[i].size.w > 0.f) {
+    BvhNode self = t.nodes[i];
+    f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
+    uint32_t at = t.roots[w.bodyWorld[i]];
+    // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
+    for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
+      BvhNode nd = t.nodes[at];
+      if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
+          nd.lo.z > qhi.z) {
+        at = t.escape[at];
+        continue;
+      }
+      uint32_t right = __float_as_uint(nd.hi.w);
+      if (right == END) {
+        uint32_t j = __float_as_uint(nd.lo.w);
+        if (j != i && (j < i || w.bodies[j].size.w <= 0.f)) {
+          uint32_t a = umax(i, j), b = umin(i, j);
+          if (acceptPair(w, a, b)) {
+            if (WRITE && base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(a, b);
+            n++;
+          }
+        }
+        at = t.escape[at];
+      } else {
+        at = __float_as_uint(nd.lo.w);
+      }
+    }
+  }
+  if (!WRITE) {
+    w.pairCount[i] = n;
+    return;
+  }
+  uint32_t lim = base + n < w.pairCapacity ? base + n : w.pairCapacity;
+  for (uint32_t x = base + 1; x < lim; x++) {
+    uint2 v = w.pairs[x];
+    uint32_t vo = v.x == i ? v.y : v.x;
+    uint32_t y = x;
+    while (y > base) {
+      uint2 u = w.pairs[y - 1];
+      uint32_t uo = u.x == i ? u.y : u.x;
+      if (uo <= vo) break;
+      w.pairs[y] = u;
+      y--;
+    }
+    w.pairs[y] = v;
+  }
+}
+
+// --- launch wrappers ---------------------------------------------------------------------------------------------------
+
+static inline uint32_t blocks(uint64_t n, uint32_t bs) {
+  uint64_t g = (n + bs - 1) / bs;
+  return uint32_t(g < 1 ? 1 : g);
+}
+
+__global__ void k_bvh_copy(uint32_t *dst, const uint32_t *src, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
+  const uint32_t n = w.nBodies;
+  launch_fill_u32(t.parent, END, plan.nodeCount, s);
+  k_bvh_leaves<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  k_bvh_bounds_init<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t, w.nWorlds);
+  k_bvh_bounds<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  k_bvh_morton<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  // Stable LSD radix sort of (key, body)
+  const uint32_t tiles = blocks(n, SORT_TILE);
+  uint64_t *k0 = t.keys, *k1 = t.keysAlt;
+  uint32_t *v0 = t.vals, *v1 = t.valsAlt;
+  for (uint32_t pass = 0; pass < plan.sortPasses; pass++) {
+    uint32_t shift = 8 * pass;
+    k_sort_count<<<tiles, SORT_TILE, 0, s>>>(k0, n, shift, tiles, t.sortHist);
+    launch_scan_exclusive(t.sortHist, 256 * tiles + 1, w.scanScratch, s);
+    k_sort_scatter<<<tiles, SORT_TILE, 0, s>>>(k0, v0, n, shift, tiles, t.sortHist, k1, v1);
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  // Leaves in Morton order are the first level's clusters; the plan's levels merge them to one root per world
+  const uint32_t *in = v0;
+  for (size_t L = 0; L < plan.levels.size(); L++) {
+    const BvhLevel &lev = plan.levels[L];
+    uint32_t *out = (L % 2 == 0) ? t.clustersA : t.clustersB;
+    k_bvh_level<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount, in, out);
+    in = out;
+  }
+  // The last level's outputs are the world roots (a one-body world's root is its leaf: its plan has a copy level)
+  k_bvh_copy<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t.roots, in, w.nWorlds);
+  k_bvh_ropes<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
+}
+
+void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
+  k_bvh_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
+  for (const BvhLevel &lev : plan.levels)
+    k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
+}
+
+void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
+  k_bvh_pairs<false><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  k_bvh_pairs<true><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+}
+
+}  // namespace avbd

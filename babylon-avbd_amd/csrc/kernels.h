@@ -19,6 +19,8 @@ void launch_update_refs(const DeviceWorld &w, hipStream_t s);
 void launch_hash_build(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s);
 void launch_grid(const DeviceWorld &w, hipStream_t s);
 void launch_pairs(const DeviceWorld &w, hipStream_t s);
+/** C_PAIRS from the scanned per-body counts (capacity-clamped, overflow bit 0). */
+void launch_pairs_total(const DeviceWorld &w, hipStream_t s);
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s);
 
 // Topology and colouring

@@ -215,19 +215,31 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
   return e;
 }
 
-AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, float alpha, float dt, Acc &acc) {
+/** Lanes per body in the primal: lane k folds adjacency entries k, k + LANES, ... (lane 0 also the inertia terms);
+ * the partial systems are then summed by a fixed tree (k += k + 4, k += k + 2, k += k + 1), so the result does not
+ * depend on scheduling. */
+constexpr uint32_t LANES = 8;
+
+AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t lane, float alpha, float dt, Acc &acc) {
   const Body &B = w.bodies[i];
-  float dt2 = dt * dt;
-  float m = B.size.w / dt2;
-  f3 I = divs(xyz(B.moment), dt2);
-  acc.l00 = m; acc.l10 = 0.f; acc.l11 = m; acc.l20 = 0.f; acc.l21 = 0.f; acc.l22 = m;
-  acc.a00 = I.x; acc.a10 = 0.f; acc.a11 = I.y; acc.a20 = 0.f; acc.a21 = 0.f; acc.a22 = I.z;
+  acc.l00 = acc.l10 = acc.l11 = acc.l20 = acc.l21 = acc.l22 = 0.f;
+  acc.a00 = acc.a10 = acc.a11 = acc.a20 = acc.a21 = acc.a22 = 0.f;
 #pragma unroll
   for (int q = 0; q < 9; q++) acc.c[q] = 0.f;
-  acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
-  acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
+  acc.rLin = mk3(0.f, 0.f, 0.f);
+  acc.rAng = mk3(0.f, 0.f, 0.f);
+  if (lane == 0) {
+    float dt2 = dt * dt;
+    float m = B.size.w / dt2;
+    f3 I = divs(xyz(B.moment), dt2);
+    acc.l00 = m; acc.l11 = m; acc.l22 = m;
+    acc.a00 = I.x; acc.a11 = I.y; acc.a22 = I.z;
+    acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
+    acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
+  }
   const uint32_t J = w.nJoints;
-  for (uint32_t e = w.adjStart[i]; e < w.adjStart[i + 1]; e++) {
+  const uint32_t end = w.adjStart[i + 1];
+  for (uint32_t e = w.adjStart[i] + lane; e < end; e += LANES) {
     uint32_t id = w.adjList[e];
     if (id < J) {
       addJoint(w, acc, id, alpha, i);
@@ -248,6 +260,17 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, float alpha, float dt, 
       addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
       addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
     }
+  }
+}
+
+/** acc += the partial system of the lane `s` above (within the body's group of LANES lanes). */
+AVBD_D void mergeFromAbove(Acc &acc, uint32_t lane, uint32_t s) {
+  float *a = reinterpret_cast<float *>(&acc);
+  constexpr int N = sizeof(Acc) / sizeof(float);
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    float o = __shfl_down(a[q], s, LANES);
+    if (lane < s) a[q] = a[q] + o;
   }
 }
 
@@ -296,16 +319,31 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
   B.rot = qadd(B.rot, mk3(-w1, -w2, -w3));
 }
 
-__global__ void __launch_bounds__(64) k_primal(DeviceWorld w, uint32_t color) {
+constexpr uint32_t PRIMAL_BLOCK = 256;
+
+__global__ void __launch_bounds__(PRIMAL_BLOCK) k_primal(DeviceWorld w, uint32_t color) {
   if (held(w)) return;
-  uint32_t start = w.colorStart[color];
-  uint32_t count = w.colorStart[color + 1] - start;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
-    uint32_t i = w.colorBodies[start + k];
-    const WorldParams &p = w.worlds[w.bodyWorld[i]];
+  const uint32_t start = w.colorStart[color];
+  const uint32_t count = w.colorStart[color + 1] - start;
+  const uint32_t lane = threadIdx.x % LANES;
+  const uint32_t perBlock = PRIMAL_BLOCK / LANES;
+  // Every lane of a block runs the same number of loop turns (the shuffles below need whole groups)
+  for (uint32_t base = blockIdx.x * perBlock; base < count; base += gridDim.x * perBlock) {
+    const uint32_t k = base + threadIdx.x / LANES;
+    const bool live = k < count;
+    uint32_t i = live ? w.colorBodies[start + k] : 0u;
     Acc acc;
-    accumulate(w, i, p.alpha, p.dt, acc);
-    finishBody(w, i, acc);
+    if (live) {
+      const WorldParams &p = w.worlds[w.bodyWorld[i]];
+      accumulate(w, i, lane, p.alpha, p.dt, acc);
+    } else {
+      float *a = reinterpret_cast<float *>(&acc);
+      for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
+    }
+    mergeFromAbove(acc, lane, 4);
+    mergeFromAbove(acc, lane, 2);
+    mergeFromAbove(acc, lane, 1);
+    if (live && lane == 0) finishBody(w, i, acc);
   }
 }
 
@@ -408,7 +446,7 @@ void launch_warm_start(const DeviceWorld &w, hipStream_t s) {
   k_warm_bodies<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
 }
 void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s) {
-  k_primal<<<blocks(maxColorBodies, 64, 4096), 64, 0, s>>>(w, color);
+  k_primal<<<blocks(uint64_t(maxColorBodies) * LANES, PRIMAL_BLOCK, 4096), PRIMAL_BLOCK, 0, s>>>(w, color);
 }
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
   k_dual<<<blocks(uint64_t(w.nJoints) + manifoldCapacity, 64, 8192), 64, 0, s>>>(w);

@@ -121,6 +121,8 @@ __global__ void k_color_compact(DeviceWorld w) {
     return;
   }
   uint32_t col = w.colorA[b] & 0xffu;
+  // A colour at or above a (lowered) cap is no colour: the body is coloured again below the cap
+  if (col >= w.colorCap) col = NO_COLOR;
   if (col != NO_COLOR) {
     bool isMax = true;
     for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {

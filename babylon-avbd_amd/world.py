@@ -46,6 +46,12 @@ def _options(options):
     o.pair_capacity = int(cap.get("pairs", 0))
     o.use_graph = 1 if options.get("graph", False) else 0
     o.device = int(options.get("device", 0))
+    o.adaptive = 1 if options.get("adaptive", False) else 0
+    bp = options.get("broadphase", "auto")
+    if bp not in ("auto", "grid", "hploc"):
+        raise ValueError(f"Unknown broadphase: {bp}")
+    o.broadphase = {"auto": 0, "grid": 1, "hploc": 2}[bp]
+    o.bvh_rebuild_interval = int((options.get("bvh") or {}).get("rebuildInterval", 0))
     return o
 
 

@@ -37,31 +37,41 @@ def primal_bytes_per_iteration(world):
         dynamic=dynamic, entries=entries, contact_reads=points, joint_entries=joints)
 
 
-def cpu_baseline(scene_fn, steps, colour_cap):
-    """The sequential CPU restatement (FP32) of the same step on the same world: `steps` steps, one thread."""
+def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
+    """The sequential CPU restatement (FP32) of the same step, started from the GPU's state at the start of the timed
+    window and run for a bounded sample (whole steps until `budget_s` seconds or `max_steps`), one thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleWorld  # the checker, timed here as the CPU baseline
 
     scene = scene_fn()
     oracle = OracleWorld([scene], precision=32, color_cap=colour_cap, color_rounds=16)
+    oracle.seed(state)
+    steps = 0
     t0 = time.perf_counter()
-    oracle.step(steps)
+    while steps < max_steps:
+        oracle.step(1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
     dt = time.perf_counter() - t0
     n = scene.body_count
     return dict(value=n * steps / dt, unit="body-steps/s", cores=1, kind="port",
-                sample=f"first {steps} steps of the same {n}-body world, sequential FP32 restatement of the step "
-                       f"(grid broadphase, colour-ordered Gauss-Seidel), {dt:.1f} s")
+                sample=f"steps {first_step + 1}-{first_step + steps} of the same {n}-body world from the GPU's state "
+                       f"(the timed window's start), sequential FP32 restatement of the step (grid broadphase, "
+                       f"colour-ordered Gauss-Seidel), {dt:.1f} s")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="large", choices=["large", "pile1000"])
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the AVBD step here is single-GPU; "
@@ -79,19 +89,22 @@ def main():
         workload = "1,000 unit boxes, 10x10x10 lattice at 1.1 m, ground"
     scene = scene_fn()
     n_bodies = scene.body_count
-    cap = 24
+    cap = 16
     world = avbd.createSolverPipeline("hip").createSolver(
-        scene, {"colorCap": cap, "colorRounds": 16, "graph": not args.no_graph})
+        scene, {"colorCap": cap, "colorRounds": 16, "graph": not args.no_graph, "adaptive": True,
+                "broadphase": args.broadphase})
     print(f"# world: {n_bodies} bodies, {world.deviceBytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
 
-    # Warm-up: the pile starts falling; let the capacity/colour policy settle (adapt reads counters)
-    for s in range(args.warmup):
-        world.step()
-        if s % 5 == 4:
-            world.adapt()
-    world.synchronize()
+    # Warm-up: the pile falls and lands; the colour/capacity policy adapts from counters read back asynchronously
+    done = 0
+    while done < args.warmup:
+        k = min(50, args.warmup - done)
+        world.step(k)
+        world.synchronize()
+        done += k
+        print(f"# warm-up {done}: {world.readCounters()}", file=sys.stderr, flush=True)
     counters = world.readCounters()
-    print(f"# after warm-up: {counters}", file=sys.stderr, flush=True)
+    state0 = None if args.no_cpu else world.readState()
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -114,7 +127,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene_fn, args.cpu_steps, cap)
+        cpu = cpu_baseline(scene_fn, state0, args.warmup, args.cpu_steps, args.cpu_budget, 64)
 
     line = {
         "metric": "body-steps/s (100k-box pile, 10 iters)",
@@ -130,7 +143,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded)",
         "config": {"workload": workload, "bodies": n_bodies, "iterations": iters, "dt": scene.params["dt"],
-                   "broadphase": "uniform grid", "colour_cap": world_colour_cap(world, cap),
+                   "broadphase": {"auto": "auto (H-PLOC BVH: one world of >= 10,000 bodies)", "grid": "uniform grid",
+                                  "hploc": "H-PLOC BVH"}[args.broadphase], "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}",
                    "counters": counters},
         "roofline": {
             "bound": "hbm",
@@ -149,10 +163,6 @@ def main():
     }
     world.destroy()
     print(json.dumps(line))
-
-
-def world_colour_cap(world, default):
-    return default
 
 
 if __name__ == "__main__":

@@ -76,6 +76,10 @@ typedef struct AvbdOptions {
   uint32_t pair_capacity;   /* initial pair/manifold capacity (0: 8 × bodies) */
   uint32_t use_graph;       /* replay each step from a captured hipGraph */
   int32_t device;           /* HIP device ordinal */
+  uint32_t adaptive;        /* adapt colour cap/rounds and storage from counters read back without waiting */
+  uint32_t broadphase;      /* 0 auto (H-PLOC for one world of >= 10,000 bodies), 1 uniform grid, 2 H-PLOC BVH
+                               (AppGpuSolver3D broadphase "auto" | "grid" | "hploc", appGpuSolver3D.js:333-402) */
+  uint32_t bvh_rebuild_interval; /* H-PLOC rebuild period in steps, refit in between (0: 64; gpuHploc.js:62-80) */
 } AvbdOptions;
 
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */

@@ -486,6 +486,7 @@ template <class R> class World {
         continue;
       }
       uint32_t col = colorA[b] & 0xffu;
+      if (col >= colorCap) col = NO_COLOR;  // a colour at or above a lowered cap is coloured again
       if (col != NO_COLOR) {
         bool isMax = true;
         for (uint32_t e = adjStart[b]; e < adjStart[b + 1] && isMax; e++) {
@@ -725,6 +726,42 @@ template <class R> class World {
     }
   }
 
+  /** Lanes per body: lane k folds adjacency entries k, k + LANES, ... (lane 0 also the inertia terms), then the
+   * partial systems are summed by the fixed tree k += k + 4, k += k + 2, k += k + 1 (the HIP primal's order). */
+  static constexpr uint32_t LANES = 8;
+
+  void addEntry(Acc &acc, uint32_t i, uint32_t id, R alpha) const {
+    const uint32_t J = uint32_t(joints.size());
+    if (id < J) {
+      addJoint(acc, joints[id], alpha, i);
+      return;
+    }
+    const Manifold<R> &mf = manifolds[id - J];
+    bool isA = i == mf.a;
+    PairBody A = pairBody(mf.a), B = pairBody(mf.b);
+    M3<R> basis = orthonormal(mf.normal);
+    R sg = isA ? R(1) : R(-1);
+    V3<R> l0 = scale(row(basis, 0), sg), l1 = scale(row(basis, 1), sg), l2 = scale(row(basis, 2), sg);
+    for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
+      const Contact<R> &k = contacts[c];
+      Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
+      V3<R> r = isA ? ev.rAW : ev.rBW;
+      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+    }
+  }
+
+  static void mergeInto(Acc &a, const Acc &o) {
+    for (int q = 0; q < 9; q++) {
+      a.lin.m[q] = a.lin.m[q] + o.lin.m[q];
+      a.ang.m[q] = a.ang.m[q] + o.ang.m[q];
+      a.cross.m[q] = a.cross.m[q] + o.cross.m[q];
+    }
+    a.rLin = add(a.rLin, o.rLin);
+    a.rAng = add(a.rAng, o.rAng);
+  }
+
   Acc accumulate(uint32_t i, R alpha) const {
     const Body<R> &b = bodies[i];
     const WorldParams &p = P(i);
@@ -732,34 +769,16 @@ template <class R> class World {
     R dt2 = dt * dt;
     R m = b.mass / dt2;
     V3<R> I = divs(b.moment, dt2);
-    Acc acc;
-    acc.lin = diag(m, m, m);
-    acc.ang = diag(I.x, I.y, I.z);
-    acc.rLin = scale(sub(b.pos, b.inertialPos), m);
-    acc.rAng = mulv3(I, qsub(b.rot, b.inertialRot));
-    const uint32_t J = uint32_t(joints.size());
-    for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++) {
-      uint32_t id = adjList[e];
-      if (id < J) {
-        addJoint(acc, joints[id], alpha, i);
-        continue;
-      }
-      const Manifold<R> &mf = manifolds[id - J];
-      bool isA = i == mf.a;
-      PairBody A = pairBody(mf.a), B = pairBody(mf.b);
-      for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
-        const Contact<R> &k = contacts[c];
-        M3<R> basis = orthonormal(mf.normal);
-        Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
-        R sg = isA ? R(1) : R(-1);
-        V3<R> r = isA ? ev.rAW : ev.rBW;
-        V3<R> l0 = scale(row(basis, 0), sg), l1 = scale(row(basis, 1), sg), l2 = scale(row(basis, 2), sg);
-        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
-      }
-    }
-    return acc;
+    Acc lanes[LANES];
+    lanes[0].lin = diag(m, m, m);
+    lanes[0].ang = diag(I.x, I.y, I.z);
+    lanes[0].rLin = scale(sub(b.pos, b.inertialPos), m);
+    lanes[0].rAng = mulv3(I, qsub(b.rot, b.inertialRot));
+    for (uint32_t lane = 0; lane < LANES; lane++)
+      for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += LANES) addEntry(lanes[lane], i, adjList[e], alpha);
+    for (uint32_t s = LANES / 2; s > 0; s >>= 1)
+      for (uint32_t lane = 0; lane < s; lane++) mergeInto(lanes[lane], lanes[lane + s]);
+    return lanes[0];
   }
 
   void finishBody(uint32_t i, const Acc &acc) {

@@ -111,3 +111,115 @@ def test_trajectory_bit_exact_small_pile(avbd):
     want = oracle.bodies().astype(np.float32)
     assert np.array_equal(got[:, 0:8], want[:, 0:8]), f"max diff {np.abs(got[:, :8] - want[:, :8]).max()}"
     world.destroy()
+
+
+# --- H-PLOC broadphase: the same pair set as the grid (gpu-hploc.js:28-68 pattern) ------------------------------------
+
+
+def _pair_sets_equal(a, b):
+    return _pair_set(a.readPairs()) == _pair_set(b.readPairs())
+
+
+def test_hploc_and_grid_give_the_same_pairs_and_trajectory(avbd):
+    scene = lambda: avbd.configs.box_pile(counts=(12, 12, 12), seed=21)  # noqa: E731
+    grid = _world(avbd, scene(), broadphase="grid")
+    tree = _world(avbd, scene(), broadphase="hploc", bvh={"rebuildInterval": 16})
+    for s in range(8):
+        grid.step(10)
+        tree.step(10)
+        assert _pair_sets_equal(grid, tree), f"pair sets differ after {10 * (s + 1)} steps"
+    assert np.array_equal(grid.readBodies(), tree.readBodies())
+    grid.destroy()
+    tree.destroy()
+
+
+def test_hploc_seeded_step_matches_the_oracle(avbd):
+    scene = avbd.configs.pile_1000
+    world = _world(avbd, scene(), broadphase="hploc", bvh={"rebuildInterval": 7})
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    for s in range(30):
+        state = world.readState()
+        oracle.seed(state)
+        world.step()
+        oracle.step()
+        compare_step(world, oracle)
+    world.destroy()
+
+
+@pytest.mark.parametrize("count", [1, 2, 15, 16, 17, 129])
+def test_hploc_small_and_ragged_counts(avbd, count):
+    """gpu-hploc.js:81-120: pair sets equal the grid's for 1..129 bodies, with a static ground and large statics."""
+    def scene():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        s.addBox([20, 1, 20], density=0, position=[0, -0.5, 0])
+        s.addBox([1, 6, 1], density=0, position=[3, 3, 0])
+        rng = np.random.default_rng(count)
+        for k in range(count - 2 if count > 2 else count):
+            s.addBox(list(rng.uniform(0.3, 1.2, 3)), position=list(rng.uniform([-2, 0.5, -2], [2, 4, 2])),
+                     rotation=list(rng.normal(size=4)))
+        return s
+    grid = _world(avbd, scene(), broadphase="grid")
+    tree = _world(avbd, scene(), broadphase="hploc", bvh={"rebuildInterval": 5})
+    for s in range(12):
+        grid.step(1)
+        tree.step(1)
+        assert _pair_sets_equal(grid, tree), f"step {s}"
+    grid.destroy()
+    tree.destroy()
+
+
+def test_hploc_teleport_outside_bounds(avbd):
+    """A body written far outside the tree's bounds between rebuilds is still found (refit covers it)."""
+    scene = lambda: avbd.configs.box_pile(counts=(6, 6, 6), seed=4)  # noqa: E731
+    grid = _world(avbd, scene(), broadphase="grid")
+    tree = _world(avbd, scene(), broadphase="hploc", bvh={"rebuildInterval": 64})
+    grid.step(5)
+    tree.step(5)
+    for w in (grid, tree):
+        b = w.readBodies()
+        b[10, 0:3] = [500.0, 3.0, 500.0]
+        b[11, 0:3] = [500.3, 3.5, 500.2]   # overlapping the teleported body
+        b[10, 32:35] = 0
+        b[11, 32:35] = 0
+        w.writeBodies(0, b)
+    for s in range(6):
+        grid.step(1)
+        tree.step(1)
+        assert _pair_sets_equal(grid, tree)
+    assert (11, 10) in _pair_set(tree.readPairs())
+    grid.destroy()
+    tree.destroy()
+
+
+# --- batches: a world steps to the same bits alone and inside a batch ------------------------------------------------
+
+
+def test_batch_worlds_match_their_solo_runs(avbd):
+    makers = [lambda: avbd.configs.pile_of(40, 101, friction=0.3),
+              lambda: avbd.configs.pile_of(9, 102, friction=0.8),
+              lambda: avbd.configs.pile_of(300, 103, friction=0.5),
+              lambda: avbd.configs.box_pile(counts=(3, 2, 4), seed=104)]
+    pipe = avbd.createSolverPipeline("hip")
+    batch = pipe.createBatch([m() for m in makers], dict(colorCap=32, colorRounds=16))
+    solos = [pipe.createSolver(m(), dict(colorCap=32, colorRounds=16)) for m in makers]
+    batch.step(60)
+    for s in solos:
+        s.step(60)
+    got = batch.readBodies()
+    for k, s in enumerate(solos):
+        first = int(batch.world_first_body[k])
+        want = s.readBodies()
+        assert np.array_equal(got[first:first + len(want)], want), f"world {k} differs from its solo run"
+    pipe.destroy()
+
+
+def test_batch_matches_the_oracle_batch(avbd):
+    makers = [lambda: avbd.configs.pile_of(30, 201), lambda: avbd.configs.pile_of(70, 202, friction=0.2)]
+    batch = avbd.createSolverPipeline("hip").createBatch([m() for m in makers], dict(colorCap=32, colorRounds=16))
+    oracle = OracleWorld([m() for m in makers], precision=32, color_cap=32, color_rounds=16)
+    for _ in range(20):
+        oracle.seed(batch.readState())
+        batch.step()
+        oracle.step()
+        compare_step(batch, oracle)
+    batch.destroy()
