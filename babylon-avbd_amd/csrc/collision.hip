@@ -131,14 +131,26 @@ __global__ void k_grid_scatter(DeviceWorld w) {
  * The count pass stores how many; the write pass stores them at the scanned offset, then sorts them by partner (each
  * partner appears once, so the order is total and the list is the same whatever order the grid buckets hold).
  */
-template <bool WRITE> __global__ void k_pairs(DeviceWorld w) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies) return;
-  uint32_t base = WRITE ? w.pairCount[i] : 0;
+template <bool STASH> __global__ void k_pairs(DeviceWorld w) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= w.nBodies) return;
+  // Threads take bodies in cell order (small bodies) then the large ones: neighbouring threads scan the same cells
+  const uint32_t nSmall = w.nBodies - w.largeCount;
+  uint32_t i = q < nSmall ? w.gridSorted[q] : w.largeList[q - nSmall];
+  uint32_t base = STASH ? 0 : w.pairCount[i];
+  uint32_t total = STASH ? 0 : w.pairCount[i + 1] - base;
+  if (!STASH && total <= PAIR_STASH) {
+    writeStashedPairs(w, i, base, total);
+    return;
+  }
   uint32_t n = 0;
   auto emit = [&](uint32_t a, uint32_t b) {
     if (!acceptPair(w, a, b)) return;
-    if (WRITE && base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(a, b);
+    if (STASH) {
+      if (n < PAIR_STASH) w.pairStash[size_t(i) * PAIR_STASH + n] = a == i ? b : a;
+    } else if (base + n < w.pairCapacity) {
+      w.pairs[base + n] = make_uint2(a, b);
+    }
     n++;
   };
   int4 c = w.cellOf[i];
@@ -186,24 +198,11 @@ template <bool WRITE> __global__ void k_pairs(DeviceWorld w) {
     if (l == i || (!small && l < i)) continue;
     emit(umax(i, l), umin(i, l));
   }
-  if (!WRITE) {
+  if (STASH) {
     w.pairCount[i] = n;
     return;
   }
-  uint32_t lim = base + n < w.pairCapacity ? base + n : w.pairCapacity;
-  for (uint32_t x = base + 1; x < lim; x++) {
-    uint2 v = w.pairs[x];
-    uint32_t vo = v.x == i ? v.y : v.x;
-    uint32_t y = x;
-    while (y > base) {
-      uint2 u = w.pairs[y - 1];
-      uint32_t uo = u.x == i ? u.y : u.x;
-      if (uo <= vo) break;
-      w.pairs[y] = u;
-      y--;
-    }
-    w.pairs[y] = v;
-  }
+  sortBodyPairs(w, i, base, n);
 }
 
 __global__ void k_pairs_total(DeviceWorld w) {
@@ -413,9 +412,9 @@ void launch_grid(const DeviceWorld &w, hipStream_t s) {
   k_grid_scatter<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
 }
 void launch_pairs(const DeviceWorld &w, hipStream_t s) {
-  k_pairs<false><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
   k_pairs<true><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  k_pairs<false><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
   k_pairs_total<<<1, 1, 0, s>>>(w);
 }
 void launch_pairs_total(const DeviceWorld &w, hipStream_t s) { k_pairs_total<<<1, 1, 0, s>>>(w); }

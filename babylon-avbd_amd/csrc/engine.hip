@@ -10,6 +10,7 @@
 
 #include "engine.h"
 #include "kernels.h"
+#include "pairtest.h"
 
 namespace avbd {
 
@@ -171,6 +172,8 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.largeStart = dallocT<uint32_t>(nWorlds_ + 1);
   w.largeList = dallocT<uint32_t>(largeList.size());
   w.pairCount = dallocT<uint32_t>(uint64_t(nBodies_) + 1);
+  w.pairStash = dallocT<uint32_t>(uint64_t(nBodies_) * PAIR_STASH);
+  w.largeCount = uint32_t(largeList.size());
   w.adjStart = dallocT<uint32_t>(uint64_t(nBodies_) + 1);
   w.adjFill = dallocT<uint32_t>(nBodies_);
   w.colorA = dallocT<uint32_t>(nBodies_);
@@ -219,6 +222,8 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   const uint32_t tiles = (n + 255) / 256;
   bvh_.nodeCount = plan_.nodeCount;
   bvh_.nodes = dallocT<BvhNode>(plan_.nodeCount);
+  bvh_.trav = dallocT<BvhNode>(plan_.nodeCount);
+  bvh_.order = dallocT<uint32_t>(n);
   bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.keys = dallocT<uint64_t>(n);

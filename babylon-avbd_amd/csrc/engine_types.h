@@ -31,9 +31,11 @@ struct DeviceWorld {
   float cellSize;
   uint32_t *largeStart;  // nWorlds + 1
   uint32_t *largeList;
+  uint32_t largeCount;
 
   // Pairs: per-body counts (scanned into offsets, nBodies + 1), the pair list (a > b)
   uint32_t *pairCount;
+  uint32_t *pairStash;   // PAIR_STASH partners per body from the counting pass
   uint2 *pairs;
   uint32_t pairCapacity;
 

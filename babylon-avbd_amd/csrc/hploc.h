@@ -31,6 +31,8 @@ struct BvhPlan {
 
 struct Bvh {
   BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
+  BvhNode *trav;       // traversal copy: lo + left/leaf tag, hi + escape rope
+  uint32_t *order;     // bodies in Morton order (the last build's)
   uint32_t *parent;    // per node
   uint32_t *escape;    // per node
   uint64_t *keys, *keysAlt;

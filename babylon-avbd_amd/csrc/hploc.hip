@@ -69,13 +69,34 @@ __global__ void k_bvh_bounds_init(Bvh t, uint32_t nWorlds) {
 
 __global__ void k_bvh_bounds(DeviceWorld w, Bvh t) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies) return;
-  float4 p = w.bodies[i].pos;
-  uint32_t k = w.bodyWorld[i];
-  float c[3] = {p.x, p.y, p.z};
-  for (int a = 0; a < 3; a++) {
-    atomicMin(&t.bounds[6 * k + a], orderedBits(c[a]));
-    atomicMax(&t.bounds[6 * k + 3 + a], orderedBits(c[a]));
+  const bool live = i < w.nBodies;
+  uint32_t k = live ? w.bodyWorld[i] : 0xffffffffu;
+  float4 p = live ? w.bodies[i].pos : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t lo[3] = {orderedBits(p.x), orderedBits(p.y), orderedBits(p.z)};
+  uint32_t hi[3] = {lo[0], lo[1], lo[2]};
+  if (!live) {
+    lo[0] = lo[1] = lo[2] = 0xffffffffu;
+    hi[0] = hi[1] = hi[2] = 0u;
+  }
+  // A wavefront whose bodies share one world reduces in registers and makes 6 atomics instead of 6 per body
+  uint32_t k0 = __shfl(k, 0, 64);
+  bool uniform = __all(k == k0 || !live);
+  if (uniform) {
+    for (int d = 32; d > 0; d >>= 1)
+      for (int a = 0; a < 3; a++) {
+        lo[a] = umin(lo[a], __shfl_xor(lo[a], d, 64));
+        hi[a] = umax(hi[a], __shfl_xor(hi[a], d, 64));
+      }
+    if ((threadIdx.x & 63) == 0 && k0 != 0xffffffffu)
+      for (int a = 0; a < 3; a++) {
+        atomicMin(&t.bounds[6 * k0 + a], lo[a]);
+        atomicMax(&t.bounds[6 * k0 + 3 + a], hi[a]);
+      }
+  } else if (live) {
+    for (int a = 0; a < 3; a++) {
+      atomicMin(&t.bounds[6 * k + a], lo[a]);
+      atomicMax(&t.bounds[6 * k + 3 + a], hi[a]);
+    }
   }
 }
 
@@ -264,62 +285,72 @@ __global__ void k_bvh_ropes(Bvh t, uint32_t nNodes) {
   t.escape[id] = escape;
 }
 
+/** Traversal copy of the tree: lo + (left child, or body index | LEAF for a leaf), hi + escape rope. */
+constexpr uint32_t LEAF = 0x80000000u;
+
+__global__ void k_bvh_trav(Bvh t, uint32_t nNodes) {
+  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nNodes) return;
+  BvhNode n = t.nodes[id];
+  uint32_t left = __float_as_uint(n.lo.w), right = __float_as_uint(n.hi.w);
+  uint32_t tag = right == END ? (left | LEAF) : left;
+  t.trav[id] = BvhNode{mk4(xyz(n.lo), __uint_as_float(tag)), mk4(xyz(n.hi), __uint_as_float(t.escape[id]))};
+}
+
 /**
- * Pairs of dynamic body i from its world's tree: leaves overlapping its (padded) box, dynamic partners of lower index
- * and static partners of any index (gpuHploc.js:37-55), each through the pair predicate. Count pass / write pass as
- * the grid's, the write pass sorting the body's pairs by partner.
+ * Pairs of one dynamic body from its world's tree: leaves overlapping its (padded) box, dynamic partners of lower
+ * index and static partners of any index (gpuHploc.js:37-55), each through the pair predicate. Threads take bodies in
+ * Morton order (neighbouring threads walk neighbouring paths). STASH pass: count, keeping the first PAIR_STASH
+ * partners; write pass: copy the stash sorted by partner, or walk again for bodies with more.
  */
-template <bool WRITE> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies) return;
-  uint32_t base = WRITE ? w.pairCount[i] : 0;
+template <bool STASH> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= w.nBodies) return;
+  uint32_t i = t.order[q];
+  uint32_t base = STASH ? 0 : w.pairCount[i];
+  uint32_t total = STASH ? 0 : w.pairCount[i + 1] - base;
+  if (!STASH && total <= PAIR_STASH) {
+    writeStashedPairs(w, i, base, total);
+    return;
+  }
   uint32_t n = 0;
   if (w.bodies[i].size.w > 0.f) {
-    BvhNode self = t.nodes[i];
+    BvhNode self = t.trav[i];
     f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
     uint32_t at = t.roots[w.bodyWorld[i]];
     // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
     for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
-      BvhNode nd = t.nodes[at];
+      BvhNode nd = t.trav[at];
+      uint32_t tag = __float_as_uint(nd.lo.w);
       if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
           nd.lo.z > qhi.z) {
-        at = t.escape[at];
+        at = __float_as_uint(nd.hi.w);
         continue;
       }
-      uint32_t right = __float_as_uint(nd.hi.w);
-      if (right == END) {
-        uint32_t j = __float_as_uint(nd.lo.w);
+      if (tag & LEAF) {
+        uint32_t j = tag & ~LEAF;
         if (j != i && (j < i || w.bodies[j].size.w <= 0.f)) {
           uint32_t a = umax(i, j), b = umin(i, j);
           if (acceptPair(w, a, b)) {
-            if (WRITE && base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(a, b);
+            if (STASH) {
+              if (n < PAIR_STASH) w.pairStash[size_t(i) * PAIR_STASH + n] = j;
+            } else if (base + n < w.pairCapacity) {
+              w.pairs[base + n] = make_uint2(a, b);
+            }
             n++;
           }
         }
-        at = t.escape[at];
+        at = __float_as_uint(nd.hi.w);
       } else {
-        at = __float_as_uint(nd.lo.w);
+        at = tag;
       }
     }
   }
-  if (!WRITE) {
+  if (STASH) {
     w.pairCount[i] = n;
     return;
   }
-  uint32_t lim = base + n < w.pairCapacity ? base + n : w.pairCapacity;
-  for (uint32_t x = base + 1; x < lim; x++) {
-    uint2 v = w.pairs[x];
-    uint32_t vo = v.x == i ? v.y : v.x;
-    uint32_t y = x;
-    while (y > base) {
-      uint2 u = w.pairs[y - 1];
-      uint32_t uo = u.x == i ? u.y : u.x;
-      if (uo <= vo) break;
-      w.pairs[y] = u;
-      y--;
-    }
-    w.pairs[y] = v;
-  }
+  sortBodyPairs(w, i, base, n);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -364,18 +395,21 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   // The last level's outputs are the world roots (a one-body world's root is its leaf: its plan has a copy level)
   k_bvh_copy<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t.roots, in, w.nWorlds);
   k_bvh_ropes<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
+  k_bvh_copy<<<blocks(n, 256), 256, 0, s>>>(t.order, v0, n);
+  k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
 }
 
 void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
   k_bvh_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
+  k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
 }
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
-  k_bvh_pairs<false><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
   k_bvh_pairs<true><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  k_bvh_pairs<false><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
 }
 
 }  // namespace avbd

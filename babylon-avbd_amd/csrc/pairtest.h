@@ -85,4 +85,41 @@ AVBD_D bool acceptPair(const DeviceWorld &w, uint32_t a, uint32_t b) {
 }
 
 
+/** Partners a body keeps from the counting pass; bodies with more are queried again in the write pass. */
+constexpr uint32_t PAIR_STASH = 16;
+
+/** Write body i's stashed partners at `base` as (a > b) pairs, sorted by partner. */
+AVBD_D void writeStashedPairs(const DeviceWorld &w, uint32_t i, uint32_t base, uint32_t n) {
+  uint32_t p[PAIR_STASH];
+  for (uint32_t x = 0; x < n; x++) {
+    uint32_t v = w.pairStash[size_t(i) * PAIR_STASH + x];
+    uint32_t y = x;
+    while (y > 0 && p[y - 1] > v) {
+      p[y] = p[y - 1];
+      y--;
+    }
+    p[y] = v;
+  }
+  for (uint32_t x = 0; x < n; x++)
+    if (base + x < w.pairCapacity) w.pairs[base + x] = make_uint2(umax(i, p[x]), umin(i, p[x]));
+}
+
+/** Sort body i's n pairs written at `base` by partner, in place (each partner appears once: a total order). */
+AVBD_D void sortBodyPairs(const DeviceWorld &w, uint32_t i, uint32_t base, uint32_t n) {
+  uint32_t lim = base + n < w.pairCapacity ? base + n : w.pairCapacity;
+  for (uint32_t x = base + 1; x < lim; x++) {
+    uint2 v = w.pairs[x];
+    uint32_t vo = v.x == i ? v.y : v.x;
+    uint32_t y = x;
+    while (y > base) {
+      uint2 u = w.pairs[y - 1];
+      uint32_t uo = u.x == i ? u.y : u.x;
+      if (uo <= vo) break;
+      w.pairs[y] = u;
+      y--;
+    }
+    w.pairs[y] = v;
+  }
+}
+
 }  // namespace avbd
