@@ -152,6 +152,15 @@ int avbd_world_step(avbd_world *world, uint32_t n) {
   });
 }
 
+int avbd_world_step_timed(avbd_world *world, uint32_t n, float *ms) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (n && !ms) throw std::invalid_argument("ms is NULL");
+    if (n) world->engine->stepTimed(n, ms);
+    return AVBD_OK;
+  });
+}
+
 int avbd_world_synchronize(avbd_world *world) {
   return guarded([&] {
     CHECK_WORLD(world);

@@ -52,6 +52,8 @@ class Engine {
   Engine &operator=(const Engine &) = delete;
 
   void step(uint32_t n);
+  /** n steps with HIP events around each on the engine's stream; ms[k] = step k's device time. Waits. */
+  void stepTimed(uint32_t n, float *ms);
   void synchronize();
   /** One step (not from a graph) with HIP events around each phase and each primal/dual launch. */
   PhaseTimes profileStep();
@@ -138,6 +140,7 @@ class Engine {
   bool counterPending_[4] = {false, false, false, false};
   uint32_t counterSlot_ = 0;
   uint32_t shrinkVotes_ = 0;
+  std::vector<hipEvent_t> timingEvents_;
 };
 
 /** Largest bounding radius still sized into grid cells (reference/three-avbd/src/avbd3d/gpu/radii.ts:47-55). */

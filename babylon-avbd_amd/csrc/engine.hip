@@ -312,6 +312,7 @@ Engine::~Engine() {
   resetGraphs();
   for (auto &ev : counterEvents_)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : timingEvents_) (void)hipEventDestroy(ev);
   if (hostCounters_) (void)hipHostFree(hostCounters_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : contactAllocations_) (void)hipFree(p);
@@ -403,6 +404,23 @@ void Engine::step(uint32_t n) {
     }
   }
   HIP_CHECK(hipGetLastError());
+}
+
+void Engine::stepTimed(uint32_t n, float *ms) {
+  HIP_CHECK(hipSetDevice(device_));
+  while (timingEvents_.size() < 2ull * n) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    timingEvents_.push_back(e);
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    HIP_CHECK(hipEventRecord(timingEvents_[2 * k], stream_));
+    step(1);
+    HIP_CHECK(hipEventRecord(timingEvents_[2 * k + 1], stream_));
+  }
+  HIP_CHECK(hipEventSynchronize(timingEvents_[2 * n - 1]));
+  for (uint32_t k = 0; k < n; k++)
+    HIP_CHECK(hipEventElapsedTime(&ms[k], timingEvents_[2 * k], timingEvents_[2 * k + 1]));
 }
 
 void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }

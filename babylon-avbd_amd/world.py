@@ -135,6 +135,13 @@ class HipWorld:
             count += 1
         return count
 
+    def stepTimed(self, n=1):
+        """Run n steps and wait; returns each step's device milliseconds (HIP events around each step)."""
+        self._alive()
+        ms = np.zeros(int(n), np.float32)
+        check(self._lib.avbd_world_step_timed(self._h, int(n), ptr(ms, ctypes.c_float)))
+        return ms
+
     def synchronize(self):
         self._alive()
         check(self._lib.avbd_world_synchronize(self._h))

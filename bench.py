@@ -61,6 +61,70 @@ def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
                        f"colour-ordered Gauss-Seidel), {dt:.1f} s")
 
 
+def batched_worlds(avbd, steps, warmup, sample):
+    """The batched-worlds workload: 4,096 box piles (body counts log-uniform in [8, 2048], friction U(0.2, 0.9)) stepped
+    together, one launch per phase from a captured hipGraph; per-step device latency from HIP events around each step.
+    Beside it, one world per launch: the first `sample` worlds of the same draw, each its own world (own stream and
+    graph), stepped round-robin over the same step window; its aggregate rate is the sample's body-steps per second."""
+    opts = {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True}
+    scenes = avbd.configs.batched_piles()
+    bodies = sum(s.body_count for s in scenes)
+    pipe = avbd.createSolverPipeline("hip")
+    batch = pipe.createBatch(scenes, opts)
+    print(f"# batch: {len(scenes)} worlds, {bodies} bodies, {batch.deviceBytes() / 2**30:.1f} GiB on device",
+          file=sys.stderr, flush=True)
+    done = 0
+    while done < warmup:
+        k = min(50, warmup - done)
+        batch.step(k)
+        batch.synchronize()
+        done += k
+        print(f"# batch warm-up {done}: {batch.readCounters()}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ms = []
+    while len(ms) < steps:
+        ms.extend(batch.stepTimed(min(50, steps - len(ms))).tolist())
+        print(f"# batch step {warmup + len(ms)}: last {ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    counters = batch.readCounters()
+    prof = batch.profileStep()
+    batch.destroy()
+    ms = np.asarray(ms)
+    agg = bodies * steps / elapsed
+
+    solos = [pipe.createSolver(s, opts) for s in avbd.configs.batched_piles(count=sample)]
+    solo_bodies = sum(w.bodyCount for w in solos)
+    for k in range(warmup):
+        for w in solos:
+            w.step(1)
+    for w in solos:
+        w.synchronize()
+    print(f"# one world per launch: {sample} worlds, {solo_bodies} bodies warmed up", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        for w in solos:
+            w.step(1)
+        if (k + 1) % 50 == 0:
+            print(f"# one world per launch: step {warmup + k + 1}", file=sys.stderr, flush=True)
+    for w in solos:
+        w.synchronize()
+    solo_elapsed = time.perf_counter() - t0
+    solo_rate = solo_bodies * steps / solo_elapsed
+    pipe.destroy()
+    return {
+        "workload": "4,096 box piles, body counts log-uniform in [8, 2048], friction U(0.2, 0.9), 10 iterations, "
+                    "one launch per phase (hipGraph)",
+        "worlds": len(scenes), "bodies": bodies, "steps_timed": f"{warmup + 1}-{warmup + steps}",
+        "body_steps_per_s": agg, "ms_per_step": elapsed * 1e3 / steps,
+        "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
+        "step_ms_max": float(ms.max()), "counters": counters, "phases_ms": prof,
+        "one_world_per_launch": {"worlds": sample, "bodies": solo_bodies, "body_steps_per_s": solo_rate,
+                                 "sample": f"worlds 0-{sample - 1} of the same draw, each its own world, "
+                                           f"steps {warmup + 1}-{warmup + steps} round-robin"},
+        "speedup_vs_one_world_per_launch": agg / solo_rate,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +136,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=280)
+    ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-sample", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the AVBD step here is single-GPU; "
@@ -129,8 +197,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(scene_fn, state0, args.warmup, args.cpu_steps, args.cpu_budget, 64)
 
+    world.destroy()
+    batched = None
+    if not args.no_batched:
+        batched = batched_worlds(avbd, args.batch_steps, args.batch_warmup, args.batch_sample)
+
     line = {
-        "metric": "body-steps/s (100k-box pile, 10 iters)",
+        "metric": "body-steps/s (100k-box pile, 10 iters); batched-world p50/p99 step latency",
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
@@ -160,8 +233,8 @@ def main():
         },
         "phases_ms": prof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
-    world.destroy()
     print(json.dumps(line))
 
 

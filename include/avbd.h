@@ -107,6 +107,11 @@ int avbd_world_destroy(avbd_world *world);
 
 /** Enqueue n steps (world.step(), solver3D.js:350-359; GpuSolver3D.step, solver.ts:1281-1417). */
 int avbd_world_step(avbd_world *world, uint32_t n);
+/**
+ * Enqueue n steps with HIP events around each (the reference's per-step timestamp queries, solver.ts:1389-1416) and
+ * wait: ms[k] = device milliseconds of step k. For step-latency percentiles of worlds and batches.
+ */
+int avbd_world_step_timed(avbd_world *world, uint32_t n, float *ms);
 /** Wait for enqueued steps. */
 int avbd_world_synchronize(avbd_world *world);
 
