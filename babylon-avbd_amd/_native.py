@@ -63,7 +63,8 @@ class Options(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in
-                ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "joints", "bodies")]
+                ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "joints", "bodies",
+                 "pivot_fallbacks")]
 
 
 # name -> (restype, argtypes): every entry point of include/avbd.h

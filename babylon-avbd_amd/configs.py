@@ -98,6 +98,58 @@ def pile_of(n, seed, friction=0.5, iterations=10):
     return scene
 
 
+def jointed_mix(n=64, spheres=2000, capsules=20000, area=40.0, seed=SEED + 2, iterations=20, capsule_grid=None):
+    """The jointed + contact mix: an n x n net of 0.2 m boxes joined to their 4-neighbours by hard ball joints at the
+    shared face centres (anchors ±0.1 m; the penalty ramp stiffens them), dropped onto `spheres` static spheres
+    (radius U(0.3, 1.0), centres uniform over an area x area square on the ground plane) and `capsules` free capsules
+    (radius 0.2, segment half-length 0.4, so 1.2 m tip to tip). dt 1/120, 20 iterations, g = -9.81.
+
+    Choices the workload text leaves open: a static ground box (area + 20 m square, 1 m thick, top at y = 0); the
+    capsules start on a jittered lattice (25 x 32 x 25 for 20,000: 1.6 m apart across the area, 1.3 m apart upwards
+    from y = 2.5 m) with uniform random orientations; the net starts flat 3 m above the highest capsule."""
+    rng = np.random.default_rng(seed)
+    scene = AvbdScene3D(gravity=-9.81, iterations=iterations, timeStep=1 / 120)
+    g = area + 20.0
+    scene.addBox([g, 1.0, g], density=0, friction=0.5, position=[0, -0.5, 0])
+    radii = rng.uniform(0.3, 1.0, spheres)
+    xz = rng.uniform(-area / 2, area / 2, (spheres, 2))
+    for r, (x, z) in zip(radii, xz):
+        scene.addSphere(float(r), density=0, friction=0.5, position=[float(x), 0.0, float(z)])
+    if capsule_grid is None:
+        side = max(1, int(round(math.sqrt(capsules / 32)))) if capsules else 1
+        capsule_grid = (side, int(math.ceil(capsules / (side * side))) if capsules else 0, side)
+    nx, ny, nz = capsule_grid
+    pitch = area / max(nx, 1)
+    placed = 0
+    top = 0.0
+    rot = _uniform_quaternions(rng, capsules)
+    jitter = rng.uniform(-0.05, 0.05, (capsules, 3))
+    for iy in range(ny):
+        for ix in range(nx):
+            for iz in range(nz):
+                if placed == capsules:
+                    break
+                p = [(ix - (nx - 1) / 2) * pitch, 2.5 + iy * 1.3, (iz - (nz - 1) / 2) * pitch]
+                p = [p[k] + float(jitter[placed, k]) for k in range(3)]
+                scene.addCapsule(0.2, 1.2, friction=0.5, position=p, rotation=list(rot[placed]))
+                top = max(top, p[1])
+                placed += 1
+    first = scene.body_count
+    y = top + 3.0
+    half = (n - 1) / 2
+    for i in range(n):
+        for j in range(n):
+            scene.addBox([0.2, 0.2, 0.2], friction=0.5, position=[(i - half) * 0.2, y, (j - half) * 0.2])
+    for i in range(n):
+        for j in range(n):
+            k = first + i * n + j
+            if i + 1 < n:
+                scene.addJoint(k, k + n, anchorA=[0.1, 0, 0], anchorB=[-0.1, 0, 0])
+            if j + 1 < n:
+                scene.addJoint(k, k + 1, anchorA=[0, 0, 0.1], anchorB=[0, 0, -0.1])
+    return scene
+
+
 def batch_sizes(count=4096, lo=8, hi=2048, seed=SEED + 3):
     """Body counts log-uniform in [lo, hi] and frictions U(0.2, 0.9) of the batched-worlds workload."""
     rng = np.random.default_rng(seed)

@@ -51,6 +51,7 @@ enum Counter : uint32_t {
   C_MANIFOLDS = 6,
   C_PREV_MANIFOLDS = 7,
   C_CANDIDATES = 8,
+  C_PIVOT64 = 9,  // bodies solved again in FP64 this step (an FP32 pivot took its floor)
   COUNTER_WORDS = 16
 };
 

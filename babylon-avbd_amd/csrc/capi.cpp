@@ -254,6 +254,7 @@ int avbd_world_read_counters(avbd_world *world, AvbdCounters *out) {
     out->colors = c.colors;
     out->joints = world->engine->jointCount();
     out->bodies = world->engine->bodyCount();
+    out->pivot_fallbacks = c.pivot64;
     return AVBD_OK;
   });
 }

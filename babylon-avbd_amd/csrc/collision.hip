@@ -29,6 +29,7 @@ __global__ void k_begin_frame(DeviceWorld w) {
     c[C_CLASHES] = 0;
     c[C_NUM_COLORS] = 0;
     c[C_CANDIDATES] = 0;
+    c[C_PIVOT64] = 0;
   }
 }
 
@@ -275,7 +276,25 @@ __global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
     Found found;
     found.count = 0;
     bool sphereA = BA.angVel.w == SHAPE_SPHERE, sphereB = BB.angVel.w == SHAPE_SPHERE;
-    if (sphereA && sphereB) {
+    bool capsuleA = BA.angVel.w == SHAPE_CAPSULE, capsuleB = BB.angVel.w == SHAPE_CAPSULE;
+    if (capsuleA || capsuleB) {
+      // The capsule is the first argument; a capsule B flips the result back (capsuleCollision.js:17-29)
+      if (capsuleA && capsuleB) {
+        found = capsuleCapsule(capsuleOf(BA), capsuleOf(BB), sat);
+      } else if (capsuleA) {
+        Capsule C = capsuleOf(BA);
+        found = sphereB ? capsuleSphere(C, B.c, B.h.x, sat) : capsuleBox(C, B, BB.rot, sat);
+      } else {
+        Capsule C = capsuleOf(BB);
+        found = sphereA ? capsuleSphere(C, A.c, A.h.x, sat) : capsuleBox(C, A, BA.rot, sat);
+        sat.n = neg(sat.n);
+        for (int c = 0; c < found.count; c++) {
+          f3 t = found.xA[c];
+          found.xA[c] = found.xB[c];
+          found.xB[c] = t;
+        }
+      }
+    } else if (sphereA && sphereB) {
       found = sphereSphere(A, B, sat);
     } else if (sphereA || sphereB) {
       if (sphereA) {
@@ -300,7 +319,7 @@ __global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
                                 mk4(n, sqrtf(BA.pos.w * BB.pos.w))};
     float4 qA = BA.rot, qB = BB.rot;
     Basis basis = orthonormal(n);
-    bool anySphere = sphereA || sphereB;
+    bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
     uint32_t prevFirst = 0, prevCount = 0;
     int pm = hashFind(w, a, b);
     if (pm >= 0) {

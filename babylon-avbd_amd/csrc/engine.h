@@ -41,7 +41,7 @@ struct PhaseTimes {
 };
 
 struct Counters {
-  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds;
+  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
 };
 
 class Engine {

@@ -367,7 +367,7 @@ void Engine::pollCounters() {
     counterPending_[slot] = false;
     const uint32_t *c = hostCounters_ + slot * COUNTER_WORDS;
     Counters cc{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                c[C_PREV_MANIFOLDS]};
+                c[C_PREV_MANIFOLDS], c[C_PIVOT64]};
     adaptFrom(cc);
   }
 }
@@ -511,7 +511,7 @@ Counters Engine::readCounters() {
   HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
   synchronize();
   return Counters{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                  c[C_PREV_MANIFOLDS]};
+                  c[C_PREV_MANIFOLDS], c[C_PIVOT64]};
 }
 
 std::vector<uint2> Engine::readPairs() {

@@ -290,4 +290,239 @@ AVBD_D SphereBoxHit sphereBox(const Box &S, const Box &X, float4 qX) {
   return out;
 }
 
+// --- capsules (src/gpu/capsuleCollision.wgsl:1-84; dispatch as capsuleCollision.js:17-29) ---------------------------
+// Shape code -1: radius = size.x / 2, segment along local Y with half-length size.y / 2 - radius. Constants are the
+// double literals rounded to float, as the CPU restatement rounds them.
+
+struct Capsule {
+  f3 a, b;
+  float r;
+};
+struct SegmentPair {
+  f3 a, b;
+};
+
+AVBD_D Capsule capsuleOf(const Body &B) {
+  float radius = B.size.x * 0.5f;
+  f3 axis = qrotate(B.rot, mk3(0.f, rmax(0.f, B.size.y * 0.5f - radius), 0.f));
+  f3 c = xyz(B.pos);
+  return Capsule{sub(c, axis), add(c, axis), radius};
+}
+
+AVBD_D f3 capSegmentPoint(f3 a, f3 b, f3 p) {
+  f3 d = sub(b, a);
+  float t = wclamp(dot(sub(p, a), d) / rmax(dot(d, d), float(1e-20)), 0.f, 1.f);
+  return add(a, scale(d, t));
+}
+
+AVBD_D SegmentPair capSegments(f3 a, f3 b, f3 c, f3 d) {
+  f3 u = sub(b, a), v = sub(d, c), w = sub(a, c);
+  float uu = dot(u, u), vv = dot(v, v), uv = dot(u, v), uw = dot(u, w), vw = dot(v, w);
+  float s = 0.f, t = 0.f;
+  if (uu <= float(1e-16)) {
+    t = wclamp(vw / rmax(vv, float(1e-20)), 0.f, 1.f);
+  } else if (vv <= float(1e-16)) {
+    s = wclamp(-uw / uu, 0.f, 1.f);
+  } else {
+    float denominator = uu * vv - uv * uv;
+    if (denominator > float(1e-8) * uu * vv) s = wclamp((uv * vw - uw * vv) / denominator, 0.f, 1.f);
+    t = (uv * s + vw) / vv;
+    if (t < 0.f) {
+      t = 0.f;
+      s = wclamp(-uw / uu, 0.f, 1.f);
+    } else if (t > 1.f) {
+      t = 1.f;
+      s = wclamp((uv - uw) / uu, 0.f, 1.f);
+    }
+  }
+  return SegmentPair{add(a, scale(u, s)), add(c, scale(v, t))};
+}
+
+AVBD_D f3 capFallback(f3 axis) {
+  if (length(axis) < float(1e-8)) return mk3(1.f, 0.f, 0.f);
+  f3 n = normalize(axis);
+  return normalize(fabsf(n.x) > fabsf(n.z) ? mk3(-n.y, n.x, 0.f) : mk3(0.f, -n.z, n.y));
+}
+
+AVBD_D Found capsuleSphere(const Capsule &A, f3 c, float r, Sat &sat) {
+  Found found;
+  found.count = 0;
+  f3 p = capSegmentPoint(A.a, A.b, c);
+  f3 delta = sub(c, p);
+  float distance = length(delta);
+  if (distance > A.r + r) return found;
+  f3 n = capFallback(sub(A.b, A.a));
+  if (distance > float(1e-8)) n = divs(delta, distance);
+  sat.n = n;
+  addFound(found, addScaled(p, n, A.r), sub(c, scale(n, r)), CAPSULE_FEATURE);
+  return found;
+}
+
+AVBD_D Found capsuleCapsule(const Capsule &A, const Capsule &B, Sat &sat) {
+  Found found;
+  found.count = 0;
+  SegmentPair points = capSegments(A.a, A.b, B.a, B.b);
+  f3 delta = sub(points.b, points.a);
+  float distance = length(delta);
+  if (distance > A.r + B.r) return found;
+  f3 n = capFallback(sub(A.b, A.a));
+  f3 crossing = cross(sub(A.b, A.a), sub(B.b, B.a));
+  if (distance > float(1e-8)) n = divs(delta, distance);
+  else if (length(crossing) > float(1e-8)) n = normalize(crossing);
+  sat.n = n;
+  f3 da = sub(A.b, A.a), db = sub(B.b, B.a);
+  if (length(da) > float(1e-8) && length(db) > float(1e-8) &&
+      length(cross(normalize(da), normalize(db))) < float(1e-4)) {
+    f3 axis = normalize(da);
+    float lo = rmax(dot(A.a, axis), rmin(dot(B.a, axis), dot(B.b, axis)));
+    float hi = rmin(dot(A.b, axis), rmax(dot(B.a, axis), dot(B.b, axis)));
+    if (hi - lo > float(1e-5)) {
+#pragma unroll
+      for (uint32_t j = 0; j < 2; j++) {
+        f3 p = add(A.a, scale(axis, (j == 1 ? hi : lo) - dot(A.a, axis)));
+        f3 q = capSegmentPoint(B.a, B.b, p);
+        addFound(found, addScaled(p, n, A.r), sub(q, scale(n, B.r)), CAPSULE_FEATURE | j);
+      }
+      return found;
+    }
+  }
+  addFound(found, addScaled(points.a, n, A.r), sub(points.b, scale(n, B.r)), CAPSULE_FEATURE);
+  return found;
+}
+
+/**
+ * Closest points of segment ab and the box [-h, h]: the segment is cut at every face-plane crossing and the piecewise
+ * quadratic distance minimised on each piece (capSegmentBox, capsuleCollision.wgsl:43-60). The breaks are kept in
+ * fixed slots (2.0 = unused) and put in order by a 19-comparator sorting network, so the array stays in registers;
+ * the ordered values are the same as an insertion sort's.
+ */
+AVBD_D SegmentPair capSegmentBox(f3 a, f3 b, f3 h) {
+  f3 d = sub(b, a);
+  float br[8];
+  br[0] = 0.f;
+  br[1] = 1.f;
+  uint32_t count = 2;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float dk = comp(d, k), ak = comp(a, k), hk = comp(h, k);
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      float v = 2.f;
+      if (fabsf(dk) > float(1e-12)) {
+        float t = ((side == 1 ? hk : -hk) - ak) / dk;
+        if (t > 0.f && t < 1.f) {
+          v = t;
+          count++;
+        }
+      }
+      br[2 + 2 * k + side] = v;
+    }
+  }
+  constexpr int NET[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
+                              {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+  for (int c = 0; c < 19; c++) {
+    float x = br[NET[c][0]], y = br[NET[c][1]];
+    br[NET[c][0]] = x < y ? x : y;
+    br[NET[c][1]] = x < y ? y : x;
+  }
+  float best = float(3.4e38), bestT = 0.5f;
+  SegmentPair result{mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+#pragma unroll
+  for (uint32_t j = 0; j < 7; j++) {
+    if (j + 1 < count) {
+      float lo = br[j], hi = br[j + 1];
+      float mid = 0.5f * (lo + hi);
+      f3 p = add(a, scale(d, mid));
+      float numerator = 0.f, denominator = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        float pk = comp(p, k), hk = comp(h, k), dk = comp(d, k);
+        if (pk < -hk || pk > hk) {
+          float edge = wclamp(pk, -hk, hk);
+          numerator += dk * (comp(a, k) - edge);
+          denominator += dk * dk;
+        }
+      }
+      float t = mid;
+      if (denominator > float(1e-20)) t = wclamp(-numerator / denominator, lo, hi);
+      f3 point = add(a, scale(d, t));
+      f3 box = mk3(wclamp(point.x, -h.x, h.x), wclamp(point.y, -h.y, h.y), wclamp(point.z, -h.z, h.z));
+      f3 gap = sub(point, box);
+      float dist = dot(gap, gap);
+      bool exactIntersection = dist == 0.f, previousIntersection = best == 0.f;
+      if ((exactIntersection && !previousIntersection) || dist < best - float(1e-12) ||
+          (exactIntersection == previousIntersection && fabsf(dist - best) <= float(1e-12) &&
+           fabsf(t - 0.5f) < fabsf(bestT - 0.5f))) {
+        best = dist;
+        bestT = t;
+        result = SegmentPair{point, box};
+      }
+    }
+  }
+  return result;
+}
+
+AVBD_D bool capBoxAxis(const Capsule &A, f3 h, f3 axis, float &best, f3 &normal) {
+  float l = length(axis);
+  if (l < float(1e-8)) return true;
+  f3 n = divs(axis, l);
+  float lo = rmin(dot(A.a, n), dot(A.b, n)) - A.r;
+  float hi = rmax(dot(A.a, n), dot(A.b, n)) + A.r;
+  float extent = dot(vabs(n), h);
+  float positive = hi + extent, negative = extent - lo;
+  if (positive < 0.f || negative < 0.f) return false;
+  float depth = rmin(positive, negative);
+  if (depth < best) {
+    best = depth;
+    normal = positive < negative ? n : neg(n);
+  }
+  return true;
+}
+
+AVBD_D Found capsuleBox(const Capsule &A, const Box &B, float4 q, Sat &sat) {
+  Found found;
+  found.count = 0;
+  float4 inv = qconj(q);
+  Capsule local{qrotate(inv, sub(A.a, B.c)), qrotate(inv, sub(A.b, B.c)), A.r};
+  SegmentPair closest = capSegmentBox(local.a, local.b, B.h);
+  f3 delta = sub(closest.b, closest.a);
+  float distance = length(delta);
+  if (distance > A.r) return found;
+  f3 nLocal = mk3(0.f, 0.f, 0.f), center = closest.a, onBox = closest.b;
+  if (distance > float(1e-8)) {
+    nLocal = divs(delta, distance);
+  } else {
+    float depth = float(3.4e38);
+    for (int k = 0; k < 3; k++) {
+      f3 axis = mk3(0.f, 0.f, 0.f);
+      setComp(axis, k, 1.f);
+      if (!capBoxAxis(local, B.h, axis, depth, nLocal)) return found;
+      if (!capBoxAxis(local, B.h, cross(sub(local.b, local.a), axis), depth, nLocal)) return found;
+    }
+    for (uint32_t k = 0; k < 8; k++) {
+      f3 vertex = mk3((k & 1u) ? B.h.x : -B.h.x, (k & 2u) ? B.h.y : -B.h.y, (k & 4u) ? B.h.z : -B.h.z);
+      if (!capBoxAxis(local, B.h, sub(vertex, capSegmentPoint(local.a, local.b, vertex)), depth, nLocal))
+        return found;
+    }
+    center = dot(local.b, nLocal) > dot(local.a, nLocal) ? local.b : local.a;
+    f3 point = addScaled(center, nLocal, local.r);
+    onBox = sub(point, scale(nLocal, depth));
+  }
+  f3 n = qrotate(q, nLocal);
+  sat.n = n;
+  Found ends;
+  ends.count = 0;
+  for (uint32_t j = 0; j < 2; j++) {
+    Box S;
+    S.c = j == 1 ? A.b : A.a;
+    S.h = mk3(A.r, A.r, A.r);
+    SphereBoxHit hit = sphereBox(S, B, q);
+    if (hit.hit && dot(hit.n, n) > float(0.9999)) addFound(ends, hit.onSphere, hit.onBox, CAPSULE_FEATURE | j);
+  }
+  if (ends.count == 2) return ends;
+  addFound(found, addScaled(add(B.c, qrotate(q, center)), n, A.r), add(B.c, qrotate(q, onBox)), CAPSULE_FEATURE);
+  return found;
+}
+
 }  // namespace avbd

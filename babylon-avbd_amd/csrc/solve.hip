@@ -195,10 +195,13 @@ AVBD_D float rowC(float c0, float keep, f3 l, const Eval &e, const PairBody &A, 
 AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, const PairBody &A, const PairBody &B,
                         float alpha) {
   uint32_t key = fbits(k.rA.w);
-  bool sphere = (key & ~STICK_BIT) == SPHERE_FEATURE;
+  // Rounded shapes (the sphere key, wgsl-solve.ts:172-181, and the capsule keys 4<<24 | j) take their lever arms at the
+  // step-start rotation: a rolling sphere or capsule would otherwise see a false separation in its normal row
+  const uint32_t feature = key & ~STICK_BIT;
+  const bool rounded = feature == SPHERE_FEATURE || (feature >> 24) == (CAPSULE_FEATURE >> 24);
   Eval e;
-  e.rAW = qrotate(sphere ? A.irot : A.rot, xyz(k.rA));
-  e.rBW = qrotate(sphere ? B.irot : B.rot, xyz(k.rB));
+  e.rAW = qrotate(rounded ? A.irot : A.rot, xyz(k.rA));
+  e.rBW = qrotate(rounded ? B.irot : B.rot, xyz(k.rB));
   float keep = 1.f - alpha;
   e.C = mk3(rowC(k.rB.w, keep, basis.n, e, A, B), rowC(k.pen.w, keep, basis.t1, e, A, B),
             rowC(k.lam.w, keep, basis.t2, e, A, B));
@@ -274,49 +277,89 @@ AVBD_D void mergeFromAbove(Acc &acc, uint32_t lane, uint32_t s) {
   }
 }
 
-/** LDLᵀ of the 6x6 system with each pivot floored at PIVOT_FLOOR of its diagonal, then x -= A⁻¹ rhs (wgsl-solve.ts:416-470). */
+/**
+ * LDLᵀ of the symmetric 6x6 system (lower triangle `A`, row-major by rows 1..6) with each pivot floored at
+ * PIVOT_FLOOR of its diagonal (wgsl-solve.ts:416-470), then x = A⁻¹ rhs. `floored` reports whether any pivot took its
+ * floor (the unfloored pivot was not above it).
+ */
+template <class T> AVBD_D void ldlt6(const T (&A)[21], const T (&b)[6], T (&x)[6], bool &floored) {
+  const T fl = T(PIVOT_FLOOR);
+  auto pivot = [&](T d, T a) {
+    T f = a * fl;
+    floored = floored || !(d > f);
+    return d > f ? d : f;
+  };
+  const T A11 = A[0];
+  const T A21 = A[1], A22 = A[2];
+  const T A31 = A[3], A32 = A[4], A33 = A[5];
+  const T A41 = A[6], A42 = A[7], A43 = A[8], A44 = A[9];
+  const T A51 = A[10], A52 = A[11], A53 = A[12], A54 = A[13], A55 = A[14];
+  const T A61 = A[15], A62 = A[16], A63 = A[17], A64 = A[18], A65 = A[19], A66 = A[20];
+  const T L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
+  const T D1 = A11;
+  const T D2 = pivot(A22 - L21 * L21 * D1, A22);
+  const T L32 = (A32 - L21 * L31 * D1) / D2;
+  const T L42 = (A42 - L21 * L41 * D1) / D2;
+  const T L52 = (A52 - L21 * L51 * D1) / D2;
+  const T L62 = (A62 - L21 * L61 * D1) / D2;
+  const T D3 = pivot(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33);
+  const T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
+  const T L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
+  const T L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+  const T D4 = pivot(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44);
+  const T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
+  const T L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+  const T D5 = pivot(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55);
+  const T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+  const T D6 = pivot(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66);
+  const T y1 = b[0];
+  const T y2 = b[1] - L21 * y1;
+  const T y3 = b[2] - L31 * y1 - L32 * y2;
+  const T y4 = b[3] - L41 * y1 - L42 * y2 - L43 * y3;
+  const T y5 = b[4] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+  const T y6 = b[5] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+  const T z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
+  x[5] = z6;
+  x[4] = z5 - L65 * x[5];
+  x[3] = z4 - L54 * x[4] - L64 * x[5];
+  x[2] = z3 - L43 * x[3] - L53 * x[4] - L63 * x[5];
+  x[1] = z2 - L32 * x[2] - L42 * x[3] - L52 * x[4] - L62 * x[5];
+  x[0] = z1 - L21 * x[1] - L31 * x[2] - L41 * x[3] - L51 * x[4] - L61 * x[5];
+}
+
+/** The FP64 solve of a system whose FP32 factorisation hit a pivot floor. */
+AVBD_D void ldlt6Double(const float (&A)[21], const float (&b)[6], float (&x)[6]) {
+  double Ad[21], bd[6], xd[6];
+#pragma unroll
+  for (int q = 0; q < 21; q++) Ad[q] = double(A[q]);
+#pragma unroll
+  for (int q = 0; q < 6; q++) bd[q] = double(b[q]);
+  bool floored = false;
+  ldlt6<double>(Ad, bd, xd, floored);
+#pragma unroll
+  for (int q = 0; q < 6; q++) x[q] = float(xd[q]);
+}
+
+/**
+ * Solve the body's 6x6 system and move it: pos -= dx, rot = qadd(rot, -dθ) (finishBody, wgsl-solve.ts:416-470). FP32
+ * first; a near-singular system (a pivot at its floor, as stiff joint chains give) is solved again in FP64 from the
+ * same FP32 entries, with the same floor.
+ */
 AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
-  const float A11 = s.l00;
-  const float A21 = s.l10, A22 = s.l11;
-  const float A31 = s.l20, A32 = s.l21, A33 = s.l22;
-  const float A41 = s.c[0], A42 = s.c[1], A43 = s.c[2], A44 = s.a00;
-  const float A51 = s.c[3], A52 = s.c[4], A53 = s.c[5], A54 = s.a10, A55 = s.a11;
-  const float A61 = s.c[6], A62 = s.c[7], A63 = s.c[8], A64 = s.a20, A65 = s.a21, A66 = s.a22;
-  const float L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
-  const float D1 = A11;
-  const float D2 = rmax(A22 - L21 * L21 * D1, A22 * PIVOT_FLOOR);
-  const float L32 = (A32 - L21 * L31 * D1) / D2;
-  const float L42 = (A42 - L21 * L41 * D1) / D2;
-  const float L52 = (A52 - L21 * L51 * D1) / D2;
-  const float L62 = (A62 - L21 * L61 * D1) / D2;
-  const float D3 = rmax(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33 * PIVOT_FLOOR);
-  const float L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
-  const float L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
-  const float L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
-  const float D4 = rmax(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44 * PIVOT_FLOOR);
-  const float L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
-  const float L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
-  const float D5 = rmax(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55 * PIVOT_FLOOR);
-  const float L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
-  const float D6 = rmax(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5),
-                        A66 * PIVOT_FLOOR);
-  const float y1 = s.rLin.x;
-  const float y2 = s.rLin.y - L21 * y1;
-  const float y3 = s.rLin.z - L31 * y1 - L32 * y2;
-  const float y4 = s.rAng.x - L41 * y1 - L42 * y2 - L43 * y3;
-  const float y5 = s.rAng.y - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
-  const float y6 = s.rAng.z - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
-  const float z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
-  const float w3 = z6;
-  const float w2 = z5 - L65 * w3;
-  const float w1 = z4 - L54 * w2 - L64 * w3;
-  const float v3 = z3 - L43 * w1 - L53 * w2 - L63 * w3;
-  const float v2 = z2 - L32 * v3 - L42 * w1 - L52 * w2 - L62 * w3;
-  const float v1 = z1 - L21 * v2 - L31 * v3 - L41 * w1 - L51 * w2 - L61 * w3;
+  const float A[21] = {s.l00, s.l10, s.l11, s.l20, s.l21, s.l22, s.c[0], s.c[1], s.c[2], s.a00, s.c[3],
+                       s.c[4], s.c[5], s.a10, s.a11, s.c[6], s.c[7], s.c[8], s.a20, s.a21, s.a22};
+  const float b[6] = {s.rLin.x, s.rLin.y, s.rLin.z, s.rAng.x, s.rAng.y, s.rAng.z};
+  float x[6];
+  bool floored = false;
+  ldlt6<float>(A, b, x, floored);
+  if (floored) {
+    ldlt6Double(A, b, x);
+    atomicAdd(&w.counters[C_PIVOT64], 1u);
+  }
   Body &B = w.bodies[i];
   float4 pos = B.pos;
-  B.pos = make_float4(pos.x - v1, pos.y - v2, pos.z - v3, pos.w);
-  B.rot = qadd(B.rot, mk3(-w1, -w2, -w3));
+  B.pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
+  B.rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;

@@ -187,7 +187,7 @@ class HipWorld:
         c = _native.Counters()
         check(self._lib.avbd_world_read_counters(self._h, ctypes.byref(c)))
         return dict(pairs=c.pairs, contacts=c.contacts, manifolds=c.manifolds, joints=c.joints, overflow=c.overflow,
-                    clashes=c.clashes, colors=c.colors, sleeping=0)
+                    clashes=c.clashes, colors=c.colors, sleeping=0, pivotFallbacks=c.pivot_fallbacks)
 
     def readPairs(self):
         """The last step's broadphase pairs (a > b) as an (n, 2) u32 array, in emission order."""

@@ -125,6 +125,35 @@ def batched_worlds(avbd, steps, warmup, sample):
     }
 
 
+def jointed_mix(avbd, steps, warmup):
+    """The jointed + contact mix: a 64 x 64 net of 0.2 m boxes on hard ball joints dropped onto 2,000 static spheres
+    and 20,000 free capsules; dt 1/120, 20 iterations, one world (H-PLOC broadphase)."""
+    scene = avbd.configs.jointed_mix()
+    n = scene.body_count
+    world = avbd.createSolverPipeline("hip").createSolver(
+        scene, {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True})
+    done = 0
+    while done < warmup:
+        k = min(50, warmup - done)
+        world.step(k)
+        world.synchronize()
+        done += k
+        print(f"# mix warm-up {done}: {world.readCounters()}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ms = []
+    while len(ms) < steps:
+        ms.extend(world.stepTimed(min(50, steps - len(ms))).tolist())
+    elapsed = time.perf_counter() - t0
+    counters = world.readCounters()
+    prof = world.profileStep()
+    world.destroy()
+    return {"workload": "64x64 jointed net (4,096 boxes, 8,064 ball joints) + 2,000 static spheres + 20,000 capsules, "
+                        "ground, dt 1/120, 20 iterations", "bodies": n, "joints": len(scene._joints),
+            "steps_timed": f"{warmup + 1}-{warmup + steps}", "body_steps_per_s": n * steps / elapsed,
+            "ms_per_step": elapsed * 1e3 / steps, "step_ms_p50": float(np.percentile(ms, 50)),
+            "step_ms_p99": float(np.percentile(ms, 99)), "counters": counters, "phases_ms": prof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +166,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--mix-steps", type=int, default=200)
+    ap.add_argument("--mix-warmup", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=280)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-sample", type=int, default=128)
@@ -198,6 +230,7 @@ def main():
         cpu = cpu_baseline(scene_fn, state0, args.warmup, args.cpu_steps, args.cpu_budget, 64)
 
     world.destroy()
+    mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
     batched = None
     if not args.no_batched:
         batched = batched_worlds(avbd, args.batch_steps, args.batch_warmup, args.batch_sample)
@@ -234,6 +267,7 @@ def main():
         "phases_ms": prof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "jointed_mix": mix,
     }
     print(json.dumps(line))
 

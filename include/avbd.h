@@ -85,6 +85,7 @@ typedef struct AvbdOptions {
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */
 typedef struct AvbdCounters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, joints, bodies;
+  uint32_t pivot_fallbacks; /* bodies whose 6x6 solve was redone in FP64 in the last step (a pivot hit its floor) */
 } AvbdCounters;
 
 /** Thread-local message of the last failed call ("" if none). */

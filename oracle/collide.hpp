@@ -317,4 +317,234 @@ template <class R> inline SphereBoxHit<R> sphereBox(const Box<R> &S, const Box<R
   return out;
 }
 
+// --- Capsules (analytic; src/gpu/capsuleCollision.wgsl:1-84, spliced into the narrowphase by
+// src/gpu/capsuleCollision.js:2-32). Shape code -1: radius = size.x / 2, segment along local Y with half-length
+// size.y / 2 - radius. Feature keys: CAPSULE_FEATURE, or CAPSULE_FEATURE | j for two-point parallel contacts. --------
+
+/** WGSL clamp(x, lo, hi) = min(max(x, lo), hi). */
+template <class R> inline R wclamp(R x, R lo, R hi) { return rmin(rmax(x, lo), hi); }
+
+template <class R> struct Capsule {
+  V3<R> a, b;
+  R r;
+};
+template <class R> struct SegmentPair {
+  V3<R> a, b;
+};
+
+/** capsuleOf (capsuleCollision.wgsl:6-9). */
+template <class R> inline Capsule<R> capsuleOf(V3<R> pos, Q4<R> rot, V3<R> size) {
+  R radius = size.x * R(0.5);
+  V3<R> axis = rotate(rot, V3<R>(0, rmax(R(0), size.y * R(0.5) - radius), 0));
+  return {sub(pos, axis), add(pos, axis), radius};
+}
+
+/** Closest point of segment ab to p (capSegmentPoint, capsuleCollision.wgsl:10). */
+template <class R> inline V3<R> capSegmentPoint(V3<R> a, V3<R> b, V3<R> p) {
+  V3<R> d = sub(b, a);
+  R t = wclamp(dot(sub(p, a), d) / rmax(dot(d, d), R(1e-20)), R(0), R(1));
+  return add(a, scale(d, t));
+}
+
+/** Closest points of segments ab and cd (capSegments, capsuleCollision.wgsl:11-18). */
+template <class R> inline SegmentPair<R> capSegments(V3<R> a, V3<R> b, V3<R> c, V3<R> d) {
+  V3<R> u = sub(b, a), v = sub(d, c), w = sub(a, c);
+  R uu = dot(u, u), vv = dot(v, v), uv = dot(u, v), uw = dot(u, w), vw = dot(v, w);
+  R s = 0, t = 0;
+  if (uu <= R(1e-16)) {
+    t = wclamp(vw / rmax(vv, R(1e-20)), R(0), R(1));
+  } else if (vv <= R(1e-16)) {
+    s = wclamp(-uw / uu, R(0), R(1));
+  } else {
+    R denominator = uu * vv - uv * uv;
+    if (denominator > R(1e-8) * uu * vv) s = wclamp((uv * vw - uw * vv) / denominator, R(0), R(1));
+    t = (uv * s + vw) / vv;
+    if (t < R(0)) {
+      t = 0;
+      s = wclamp(-uw / uu, R(0), R(1));
+    } else if (t > R(1)) {
+      t = 1;
+      s = wclamp((uv - uw) / uu, R(0), R(1));
+    }
+  }
+  return {add(a, scale(u, s)), add(c, scale(v, t))};
+}
+
+/** A normal perpendicular to the axis for coincident segments (capFallback, capsuleCollision.wgsl:19-22). */
+template <class R> inline V3<R> capFallback(V3<R> axis) {
+  if (length(axis) < R(1e-8)) return V3<R>(1, 0, 0);
+  V3<R> n = normalize(axis);
+  return normalize(std::fabs(n.x) > std::fabs(n.z) ? V3<R>(-n.y, n.x, 0) : V3<R>(0, -n.z, n.y));
+}
+
+/** Capsule A against sphere (c, r); n from A towards the sphere (capsuleSphere, capsuleCollision.wgsl:23-27). */
+template <class R> inline Found<R> capsuleSphere(const Capsule<R> &A, V3<R> c, R r, Sat<R> &sat) {
+  Found<R> found;
+  V3<R> p = capSegmentPoint(A.a, A.b, c);
+  V3<R> delta = sub(c, p);
+  R distance = length(delta);
+  if (distance > A.r + r) return found;
+  V3<R> n = capFallback(sub(A.b, A.a));
+  if (distance > R(1e-8)) n = divs(delta, distance);
+  sat.n = n;
+  addFound(found, addScaled(p, n, A.r), sub(c, scale(n, r)), CAPSULE_FEATURE);
+  return found;
+}
+
+/** Capsule against capsule; two points when the axes are parallel and overlap (capsuleCollision.wgsl:28-40). */
+template <class R> inline Found<R> capsuleCapsule(const Capsule<R> &A, const Capsule<R> &B, Sat<R> &sat) {
+  Found<R> found;
+  SegmentPair<R> points = capSegments(A.a, A.b, B.a, B.b);
+  V3<R> delta = sub(points.b, points.a);
+  R distance = length(delta);
+  if (distance > A.r + B.r) return found;
+  V3<R> n = capFallback(sub(A.b, A.a));
+  V3<R> crossing = cross(sub(A.b, A.a), sub(B.b, B.a));
+  if (distance > R(1e-8)) n = divs(delta, distance);
+  else if (length(crossing) > R(1e-8)) n = normalize(crossing);
+  sat.n = n;
+  V3<R> da = sub(A.b, A.a), db = sub(B.b, B.a);
+  if (length(da) > R(1e-8) && length(db) > R(1e-8) && length(cross(normalize(da), normalize(db))) < R(1e-4)) {
+    V3<R> axis = normalize(da);
+    R lo = rmax(dot(A.a, axis), rmin(dot(B.a, axis), dot(B.b, axis)));
+    R hi = rmin(dot(A.b, axis), rmax(dot(B.a, axis), dot(B.b, axis)));
+    if (hi - lo > R(1e-5)) {
+      for (uint32_t j = 0; j < 2; j++) {
+        V3<R> p = add(A.a, scale(axis, (j == 1 ? hi : lo) - dot(A.a, axis)));
+        V3<R> q = capSegmentPoint(B.a, B.b, p);
+        addFound(found, addScaled(p, n, A.r), sub(q, scale(n, B.r)), CAPSULE_FEATURE | j);
+      }
+      return found;
+    }
+  }
+  addFound(found, addScaled(points.a, n, A.r), sub(points.b, scale(n, B.r)), CAPSULE_FEATURE);
+  return found;
+}
+
+/**
+ * Closest points of segment ab and the box [-h, h] (capSegmentBox, capsuleCollision.wgsl:43-60): the segment is cut
+ * at every face-plane crossing and the piecewise quadratic distance is minimised on each piece; an exact intersection
+ * wins over an exterior sample, ties go to the parameter nearest the middle.
+ */
+template <class R> inline SegmentPair<R> capSegmentBox(V3<R> a, V3<R> b, V3<R> h) {
+  V3<R> d = sub(b, a);
+  R breaks[8];
+  breaks[0] = 0;
+  breaks[1] = 1;
+  uint32_t count = 2;
+  for (int k = 0; k < 3; k++) {
+    if (std::fabs(d[k]) > R(1e-12)) {
+      for (int side = 0; side < 2; side++) {
+        R t = ((side == 1 ? h[k] : -h[k]) - a[k]) / d[k];
+        if (t > R(0) && t < R(1)) breaks[count++] = t;
+      }
+    }
+  }
+  for (uint32_t i = 1; i < count; i++) {
+    R value = breaks[i];
+    uint32_t j = i;
+    while (j > 0 && breaks[j - 1] > value) {
+      breaks[j] = breaks[j - 1];
+      j--;
+    }
+    breaks[j] = value;
+  }
+  R best = R(3.4e38), bestT = R(0.5);
+  SegmentPair<R> result{V3<R>(0, 0, 0), V3<R>(0, 0, 0)};
+  for (uint32_t j = 0; j + 1 < count; j++) {
+    R lo = breaks[j], hi = breaks[j + 1];
+    R mid = R(0.5) * (lo + hi);
+    V3<R> p = add(a, scale(d, mid));
+    R numerator = 0, denominator = 0;
+    for (int k = 0; k < 3; k++) {
+      if (p[k] < -h[k] || p[k] > h[k]) {
+        R edge = wclamp(p[k], -h[k], h[k]);
+        numerator += d[k] * (a[k] - edge);
+        denominator += d[k] * d[k];
+      }
+    }
+    R t = mid;
+    if (denominator > R(1e-20)) t = wclamp(-numerator / denominator, lo, hi);
+    V3<R> point = add(a, scale(d, t));
+    V3<R> box(wclamp(point.x, -h.x, h.x), wclamp(point.y, -h.y, h.y), wclamp(point.z, -h.z, h.z));
+    V3<R> gap = sub(point, box);
+    R dist = dot(gap, gap);
+    bool exactIntersection = dist == R(0), previousIntersection = best == R(0);
+    if ((exactIntersection && !previousIntersection) || dist < best - R(1e-12) ||
+        (exactIntersection == previousIntersection && std::fabs(dist - best) <= R(1e-12) &&
+         std::fabs(t - R(0.5)) < std::fabs(bestT - R(0.5)))) {
+      best = dist;
+      bestT = t;
+      result = {point, box};
+    }
+  }
+  return result;
+}
+
+/** One SAT axis of a capsule against the box [-h, h] (capBoxAxis, capsuleCollision.wgsl:61-66). */
+template <class R> inline bool capBoxAxis(const Capsule<R> &A, V3<R> h, V3<R> axis, R &best, V3<R> &normal) {
+  R l = length(axis);
+  if (l < R(1e-8)) return true;
+  V3<R> n = divs(axis, l);
+  R lo = rmin(dot(A.a, n), dot(A.b, n)) - A.r;
+  R hi = rmax(dot(A.a, n), dot(A.b, n)) + A.r;
+  R extent = dot(vabs(n), h);
+  R positive = hi + extent, negative = extent - lo;
+  if (positive < R(0) || negative < R(0)) return false;
+  R depth = rmin(positive, negative);
+  if (depth < best) {
+    best = depth;
+    normal = positive < negative ? n : neg(n);
+  }
+  return true;
+}
+
+/**
+ * Capsule A against box B with rotation q; n from A towards B (capsuleBox, capsuleCollision.wgsl:67-84). A segment
+ * inside the box takes the shallowest of the box faces, segment × face axes and vertex axes; end caps both in
+ * parallel support give a two-point manifold.
+ */
+template <class R> inline Found<R> capsuleBox(const Capsule<R> &A, const Box<R> &B, Q4<R> q, Sat<R> &sat) {
+  Found<R> found;
+  Q4<R> inv = conj(q);
+  Capsule<R> local{rotate(inv, sub(A.a, B.c)), rotate(inv, sub(A.b, B.c)), A.r};
+  SegmentPair<R> closest = capSegmentBox(local.a, local.b, B.h);
+  V3<R> delta = sub(closest.b, closest.a);
+  R distance = length(delta);
+  if (distance > A.r) return found;
+  V3<R> nLocal(0, 0, 0), center = closest.a, onBox = closest.b;
+  if (distance > R(1e-8)) {
+    nLocal = divs(delta, distance);
+  } else {
+    R depth = R(3.4e38);
+    for (int k = 0; k < 3; k++) {
+      V3<R> axis(0, 0, 0);
+      axis[k] = 1;
+      if (!capBoxAxis(local, B.h, axis, depth, nLocal)) return found;
+      if (!capBoxAxis(local, B.h, cross(sub(local.b, local.a), axis), depth, nLocal)) return found;
+    }
+    for (uint32_t k = 0; k < 8; k++) {
+      V3<R> vertex((k & 1u) ? B.h.x : -B.h.x, (k & 2u) ? B.h.y : -B.h.y, (k & 4u) ? B.h.z : -B.h.z);
+      if (!capBoxAxis(local, B.h, sub(vertex, capSegmentPoint(local.a, local.b, vertex)), depth, nLocal))
+        return found;
+    }
+    center = dot(local.b, nLocal) > dot(local.a, nLocal) ? local.b : local.a;
+    V3<R> point = addScaled(center, nLocal, local.r);
+    onBox = sub(point, scale(nLocal, depth));
+  }
+  V3<R> n = rotate(q, nLocal);
+  sat.n = n;
+  Found<R> ends;
+  for (uint32_t j = 0; j < 2; j++) {
+    Box<R> S;
+    S.c = j == 1 ? A.b : A.a;
+    S.h = V3<R>(A.r, A.r, A.r);
+    SphereBoxHit<R> hit = sphereBox(S, B, q);
+    if (hit.hit && dot(hit.n, n) > R(0.9999)) addFound(ends, hit.onSphere, hit.onBox, CAPSULE_FEATURE | j);
+  }
+  if (ends.count == 2) return ends;
+  addFound(found, addScaled(add(B.c, rotate(q, center)), n, A.r), add(B.c, rotate(q, onBox)), CAPSULE_FEATURE);
+  return found;
+}
+
 }  // namespace oracle

@@ -269,6 +269,7 @@ void gs_counts(void *p, uint32_t *out) {
   out[4] = w.numColors;
   out[5] = uint32_t(w.n());
   out[6] = uint32_t(w.joints.size());
+  out[7] = w.pivotFallbacks;
 }
 
 /** Bodies as 40 values each (inertialPos.w carries the moved-step stamp as its integer value). */
@@ -361,6 +362,34 @@ int gs_probe_box_contacts(int precision, const double *poseA, const double *size
                        V3<R>(R(sizeB[0]), R(sizeB[1]), R(sizeB[2])));
     Sat<R> sat;
     Found<R> f = collideBoxes(A, B, faceBias != 0, maxPoly, sat);
+    for (int i = 0; i < f.count; i++) {
+      features[i] = f.feature[i];
+      double v[6] = {double(f.xA[i].x), double(f.xA[i].y), double(f.xA[i].z),
+                     double(f.xB[i].x), double(f.xB[i].y), double(f.xB[i].z)};
+      std::memcpy(points + 6 * i, v, sizeof v);
+    }
+    normal[0] = -double(sat.n.x);
+    normal[1] = -double(sat.n.y);
+    normal[2] = -double(sat.n.z);
+    return f.count;
+  };
+  return precision == 64 ? run(0.0) : run(0.0f);
+}
+
+/**
+ * Contacts of one pair of body records (40 f32 each) by shape — box/box, sphere, capsule — as the narrowphase makes
+ * them: features, world points on A and B (6 per point) and the manifold normal (from B towards A). Returns the count.
+ */
+int gs_probe_pair_contacts(int precision, const float *recA, const float *recB, int faceBias, uint32_t *features,
+                           double *points, double *normal) {
+  auto run = [&](auto zero) {
+    using R = decltype(zero);
+    Body<R> BA, BB;
+    unpackBody(recA, BA);
+    unpackBody(recB, BB);
+    Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
+    Sat<R> sat;
+    Found<R> f = shapeContacts(BA, BB, A, B, faceBias != 0, sat);
     for (int i = 0; i < f.count; i++) {
       features[i] = f.feature[i];
       double v[6] = {double(f.xA[i].x), double(f.xA[i].y), double(f.xA[i].z),

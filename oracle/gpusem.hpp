@@ -91,6 +91,58 @@ struct RefPose { float pos[4] = {0, 0, 0, 0}; float rot[4] = {0, 0, 0, 0}; };
 
 template <class R> inline R toR(float x) { return R(x); }
 
+/**
+ * Contact keys of rounded shapes: the sphere key (wgsl-solve.ts:172-181 evaluates its lever arms at the step-start
+ * rotation, the Taylor point) and the capsule keys 4<<24 | j, which take the same rule here: a capsule rolling on its
+ * side has the sphere's problem (a lever arm turned with the body puts a false separation into the normal row, and the
+ * capsule sinks while gaining speed).
+ */
+inline bool roundedKey(uint32_t key) {
+  uint32_t k = key & ~STICK_BIT;
+  return k == SPHERE_FEATURE || (k >> 24) == (CAPSULE_FEATURE >> 24);
+}
+
+/**
+ * Contact points of one pair by shape (wgsl-collision.ts:684-726 with the capsule splice of capsuleCollision.js:17-29):
+ * capsules first, then spheres, then box/box SAT. sat.n points from A towards B.
+ */
+template <class R>
+inline Found<R> shapeContacts(const Body<R> &BA, const Body<R> &BB, const Box<R> &A, const Box<R> &B, bool faceBias,
+                              Sat<R> &sat) {
+  Found<R> found;
+  bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
+  bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
+  if (capsuleA || capsuleB) {
+    // capsuleCollision.js:17-29: the capsule is the first argument; a capsule B flips the result back
+    if (capsuleA && capsuleB) {
+      found = capsuleCapsule(capsuleOf(BA.pos, BA.rot, BA.size), capsuleOf(BB.pos, BB.rot, BB.size), sat);
+    } else if (capsuleA) {
+      Capsule<R> C = capsuleOf(BA.pos, BA.rot, BA.size);
+      found = sphereB ? capsuleSphere(C, B.c, B.h.x, sat) : capsuleBox(C, B, BB.rot, sat);
+    } else {
+      Capsule<R> C = capsuleOf(BB.pos, BB.rot, BB.size);
+      found = sphereA ? capsuleSphere(C, A.c, A.h.x, sat) : capsuleBox(C, A, BA.rot, sat);
+      sat.n = neg(sat.n);
+      for (int c = 0; c < found.count; c++) std::swap(found.xA[c], found.xB[c]);
+    }
+  } else if (sphereA && sphereB) {
+    found = sphereSphere(A, B, sat);
+  } else if (sphereA || sphereB) {
+    if (sphereA) {
+      SphereBoxHit<R> h = sphereBox(A, B, BB.rot);
+      sat.n = h.n;
+      if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    } else {
+      SphereBoxHit<R> h = sphereBox(B, A, BA.rot);
+      sat.n = neg(h.n);
+      if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+    }
+  } else {
+    found = collideBoxes(A, B, faceBias, 8, sat);
+  }
+  return found;
+}
+
 template <class R> class World {
  public:
   // Scene (in index order)
@@ -111,6 +163,7 @@ template <class R> class World {
   std::vector<uint32_t> adjStart, adjList;
   std::vector<uint32_t> colorStart, colorBodies;
   uint32_t clashes = 0, numColors = 0;
+  uint32_t pivotFallbacks = 0;  // bodies solved again in FP64 in the last step
   // Global schedule
   uint32_t colorCap = 64, colorRounds = 16;
   int iterations = 10;
@@ -329,22 +382,9 @@ template <class R> class World {
       Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
       Sat<R> sat;
       Found<R> found;
+      found = shapeContacts(BA, BB, A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
       bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
-      if (sphereA && sphereB) {
-        found = sphereSphere(A, B, sat);
-      } else if (sphereA || sphereB) {
-        if (sphereA) {
-          SphereBoxHit<R> h = sphereBox(A, B, BB.rot);
-          sat.n = h.n;
-          if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
-        } else {
-          SphereBoxHit<R> h = sphereBox(B, A, BA.rot);
-          sat.n = neg(h.n);
-          if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
-        }
-      } else {
-        found = collideBoxes(A, B, (p.flags & FLAG_FACE_BIAS) != 0, 8, sat);
-      }
+      bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
       if (found.count == 0) continue;
       Manifold<R> mf;
       mf.a = a;
@@ -355,7 +395,7 @@ template <class R> class World {
       mf.friction = std::sqrt(BA.friction * BB.friction);
       manifolds.push_back(mf);
       M3<R> basis = orthonormal(mf.normal);
-      bool anySphere = sphereA || sphereB;
+      bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
       uint32_t prevFirst = 0, prevCount = 0;
       auto it = prevIndex.find({a, b});
       if (it != prevIndex.end()) {
@@ -653,7 +693,8 @@ template <class R> class World {
   Eval evalContact(const Contact<R> &k, const M3<R> &basis, R friction, const PairBody &A, const PairBody &B,
                    R alpha) const {
     Q4<R> rotA = A.rot, rotB = B.rot;
-    if ((k.key & ~STICK_BIT) == SPHERE_FEATURE) {
+    // Rounded-shape contacts (sphere, and here also capsule keys) take the lever arms at the step-start rotation
+    if (roundedKey(k.key)) {
       rotA = bodies[A.index].irot;
       rotB = bodies[B.index].irot;
     }
@@ -781,10 +822,28 @@ template <class R> class World {
     return lanes[0];
   }
 
+  /**
+   * finishBody (wgsl-solve.ts:416-470) with the FP64 pivot fallback: when an FP32 pivot takes its floor the system is
+   * solved again in FP64 from the same FP32 entries (same floor), and the result rounded to FP32.
+   */
   void finishBody(uint32_t i, const Acc &acc) {
     Body<R> &b = bodies[i];
     V3<R> xl, xa;
-    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f));
+    bool floored = false;
+    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f), &floored);
+    if (floored && sizeof(R) == sizeof(float)) {
+      auto up = [](const M3<R> &m) {
+        M3<double> d;
+        for (int q = 0; q < 9; q++) d.m[q] = double(m.m[q]);
+        return d;
+      };
+      auto upv = [](V3<R> v) { return V3<double>(double(v.x), double(v.y), double(v.z)); };
+      V3<double> dl, da;
+      solve6(up(acc.lin), up(acc.ang), up(acc.cross), upv(acc.rLin), upv(acc.rAng), dl, da, double(1e-5f));
+      xl = V3<R>(R(dl.x), R(dl.y), R(dl.z));
+      xa = V3<R>(R(da.x), R(da.y), R(da.z));
+      pivotFallbacks++;
+    }
     b.pos = sub(b.pos, xl);
     b.rot = qadd(b.rot, neg(xa));
   }
@@ -889,6 +948,7 @@ template <class R> class World {
   }
 
   void step() {
+    pivotFallbacks = 0;
     collisionAndTopology();
     solve();
   }

@@ -174,14 +174,18 @@ template <class R> inline V3<R> row(const M3<R> &a, int k) { return {a.m[k * 3],
  */
 template <class R>
 inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3<R> bLin, V3<R> bAng, V3<R> &xLin,
-                   V3<R> &xAng, R pivotFloor = 0) {
+                   V3<R> &xAng, R pivotFloor = 0, bool *floored = nullptr) {
   const R A11 = aLin.m[0];
   const R A21 = aLin.m[3], A22 = aLin.m[4];
   const R A31 = aLin.m[6], A32 = aLin.m[7], A33 = aLin.m[8];
   const R A41 = aCross.m[0], A42 = aCross.m[1], A43 = aCross.m[2], A44 = aAng.m[0];
   const R A51 = aCross.m[3], A52 = aCross.m[4], A53 = aCross.m[5], A54 = aAng.m[3], A55 = aAng.m[4];
   const R A61 = aCross.m[6], A62 = aCross.m[7], A63 = aCross.m[8], A64 = aAng.m[6], A65 = aAng.m[7], A66 = aAng.m[8];
-  auto floorp = [&](R d, R a) { return pivotFloor > 0 ? rmax(d, a * pivotFloor) : d; };
+  auto floorp = [&](R d, R a) {
+    if (!(pivotFloor > 0)) return d;
+    if (floored && !(d > a * pivotFloor)) *floored = true;
+    return rmax(d, a * pivotFloor);
+  };
 
   const R L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
   const R D1 = A11;

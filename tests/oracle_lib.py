@@ -68,6 +68,7 @@ def lib():
             "gs_probe_box_contacts": (ctypes.c_int, [ctypes.c_int, f64p, f64p, f64p, f64p, ctypes.c_int, ctypes.c_int,
                                                      u32p, f64p, f64p]),
             "gs_probe_solve6": (None, [f64p, f64p, ctypes.c_double, f64p]),
+            "gs_probe_pair_contacts": (ctypes.c_int, [ctypes.c_int, f32p, f32p, ctypes.c_int, u32p, f64p, f64p]),
             "gs_probe_priority": (ctypes.c_uint32, [ctypes.c_uint32]),
         }
         for name, (res, args) in sig.items():
@@ -148,7 +149,7 @@ class OracleWorld:
         joints = np.ascontiguousarray(state["joints"], np.float32).reshape(-1, 32)
         cc = ctypes.c_uint32
         old = self.h
-        counts = np.zeros(7, np.uint32)
+        counts = np.zeros(8, np.uint32)
         L.gs_counts(old, P(counts, cc))
         self.h = L.gs_create(self.precision, self.n, P(bodies, ctypes.c_float), P(self.world, cc), self.nw,
                              P(self.params, ctypes.c_float), self.nj, P(joints, ctypes.c_float),
@@ -169,9 +170,10 @@ class OracleWorld:
         lib().gs_step(self.h, n)
 
     def counts(self):
-        c = np.zeros(7, np.uint32)
+        c = np.zeros(8, np.uint32)
         lib().gs_counts(self.h, P(c, ctypes.c_uint32))
-        return dict(pairs=int(c[0]), manifolds=int(c[1]), contacts=int(c[2]), clashes=int(c[3]), colors=int(c[4]))
+        return dict(pairs=int(c[0]), manifolds=int(c[1]), contacts=int(c[2]), clashes=int(c[3]), colors=int(c[4]),
+                    pivotFallbacks=int(c[7]))
 
     def bodies(self):
         out = np.zeros((self.n, 40))

@@ -68,7 +68,13 @@ def compare_step(world, oracle, exact=True):
         ctc = mc[key][2].view(np.float32).astype(np.float64)
         _close(ctg[:, 12:15], ctc[:, 12:15], 1.0, f"lambda {key}")
         _close(ctg[:, 8:11], ctc[:, 8:11], 1.0, f"penalty {key}")
-    return dict(pairs=len(pg), manifolds=len(mg))
+    if world.jointCount:
+        jg, jc = world.readJoints().astype(np.float64), oracle.joints()
+        for name, cols in (("joint penalties", np.r_[0:3, 4:7]), ("joint lambda", np.r_[8:11, 12:15])):
+            _close(jg[:, cols], jc[:, cols], 1.0, name)
+    assert world.readCounters()["pivotFallbacks"] == oracle.counts()["pivotFallbacks"], "FP64 pivot fallbacks differ"
+    features = {int(f) >> 24 for f in np.asarray(ct, np.uint32).reshape(-1, 16)[:, 3] & 0x7FFFFFFF} if len(ct) else set()
+    return dict(pairs=len(pg), manifolds=len(mg), features=features)
 
 
 def _world(avbd, scene, cap=32, rounds=16, **opts):
@@ -223,3 +229,60 @@ def test_batch_matches_the_oracle_batch(avbd):
         oracle.step()
         compare_step(batch, oracle)
     batch.destroy()
+
+
+# --- capsules, spheres and joints (the jointed + contact mix at test size) -------------------------------------------
+
+
+def _small_mix(avbd):
+    return avbd.configs.jointed_mix(n=6, spheres=30, capsules=20, area=3.0, capsule_grid=(4, 1, 5))
+
+
+def test_jointed_mix_steps_stay_in_parity(avbd):
+    """Net of boxes on hard ball joints, static spheres and free capsules: every step of 240 from the GPU's state."""
+    world = _world(avbd, _small_mix(avbd))
+    oracle = OracleWorld([_small_mix(avbd)], precision=32, color_cap=32, color_rounds=16)
+    features = set()
+    for s in range(240):
+        oracle.seed(world.readState())
+        world.step()
+        oracle.step()
+        features |= compare_step(world, oracle)["features"]
+    assert {3, 4} <= features, f"capsule and sphere contacts expected, saw kinds {features}"
+    world.destroy()
+
+
+def test_jointed_mix_trajectory_bit_exact(avbd):
+    world = _world(avbd, _small_mix(avbd))
+    oracle = OracleWorld([_small_mix(avbd)], precision=32, color_cap=32, color_rounds=16)
+    world.step(300)
+    oracle.step(300)
+    assert np.array_equal(world.readBodies()[:, 0:8], oracle.bodies()[:, 0:8].astype(np.float32))
+    assert np.array_equal(world.readJoints()[:, 0:16], oracle.joints()[:, 0:16].astype(np.float32))
+    world.destroy()
+
+
+def test_pivot_fallback_to_fp64_matches_the_oracle(avbd):
+    """A chain of 8 g boxes on hard joints warm-started at penalty 1e8: FP32 pivots hit the floor and the 6x6 solves are
+    redone in FP64 on both sides, the same bodies each time."""
+    def scene():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        for i in range(6):
+            s.addBox([0.2, 0.2, 0.2], position=[0.2 * i + 0.1, 5, 0])
+        s.addJoint(None, 0, anchorA=[0, 5, 0], anchorB=[-0.1, 0, 0])
+        for i in range(5):
+            s.addJoint(i, i + 1, anchorA=[0.1, 0, 0], anchorB=[-0.1, 0, 0])
+        for rec, _ in s._joints:
+            rec[0:3] = 1e8
+        return s
+    world = _world(avbd, scene())
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    total = 0
+    for s in range(10):
+        oracle.seed(world.readState())
+        world.step()
+        oracle.step()
+        compare_step(world, oracle)
+        total += world.readCounters()["pivotFallbacks"]
+    assert total > 0
+    world.destroy()
