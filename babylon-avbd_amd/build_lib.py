@@ -58,7 +58,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"linking libavbd_hip.so failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_addon()
     return LIB
+
+
+NODE_HEADERS = "/usr/include/node"
+ADDON = os.path.join(HERE, "js", "avbd_napi.node")
+
+
+def build_addon() -> str:
+    """The Node N-API addon (js/avbd_napi.cpp) over libavbd_hip.so, when Node's headers are installed."""
+    if not os.path.exists(os.path.join(NODE_HEADERS, "node_api.h")):
+        return ""
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", f"-I{NODE_HEADERS}",
+           os.path.join(HERE, "js", "avbd_napi.cpp"), f"-L{HERE}", "-lavbd_hip", "-Wl,-rpath,$ORIGIN/..",
+           "-o", ADDON + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the N-API addon failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(ADDON + ".tmp", ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

@@ -1,0 +1,426 @@
+// Node N-API addon over the C ABI of libavbd_hip.so (include/avbd.h): the binding the reference's JavaScript world
+// API (src/native/pipeline.js, src/native/solver3D.js) calls for its "hip" backend. Each export is one C entry point;
+// a non-zero status becomes a thrown Error carrying avbd_last_error(), as the reference throws on invalid use.
+// Typed arrays cross without copies on the way in; reads return fresh typed arrays (the reference returns fresh
+// arrays from its readbacks, solver.ts:1421-1475).
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd.h"
+
+namespace {
+
+#define NAPI_OK(call)                                       \
+  do {                                                      \
+    if ((call) != napi_ok) {                                \
+      napi_throw_error(env, nullptr, "N-API call failed");  \
+      return nullptr;                                       \
+    }                                                       \
+  } while (0)
+
+napi_value throwStatus(napi_env env, int status) {
+  const char *msg = avbd_last_error();
+  std::string text = std::string("avbd: ") + (msg && *msg ? msg : "error") + " (status " + std::to_string(status) + ")";
+  napi_throw_error(env, nullptr, text.c_str());
+  return nullptr;
+}
+
+#define AVBD_OK_OR_THROW(call)                  \
+  do {                                          \
+    int st_ = (call);                           \
+    if (st_ != AVBD_OK) return throwStatus(env, st_); \
+  } while (0)
+
+bool args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok) return false;
+  if (argc < want) {
+    napi_throw_type_error(env, nullptr, "missing arguments");
+    return false;
+  }
+  return true;
+}
+
+avbd_world *worldOf(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a world handle");
+    return nullptr;
+  }
+  return static_cast<avbd_world *>(p);
+}
+
+bool u32(napi_env env, napi_value v, uint32_t *out) {
+  if (napi_get_value_uint32(env, v, out) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "expected an unsigned integer");
+    return false;
+  }
+  return true;
+}
+
+/** Data pointer and element count of a typed array of the given type (or null/undefined when allowed). */
+bool typed(napi_env env, napi_value v, napi_typedarray_type want, void **data, size_t *length, bool optional) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (optional && (t == napi_null || t == napi_undefined)) {
+    *data = nullptr;
+    *length = 0;
+    return true;
+  }
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) {
+    napi_throw_type_error(env, nullptr, "expected a typed array");
+    return false;
+  }
+  napi_typedarray_type type;
+  napi_value buffer;
+  size_t offset;
+  if (napi_get_typedarray_info(env, v, &type, length, data, &buffer, &offset) != napi_ok || type != want) {
+    napi_throw_type_error(env, nullptr, "typed array of the wrong element type");
+    return false;
+  }
+  return true;
+}
+
+napi_value newTyped(napi_env env, napi_typedarray_type type, size_t elements, size_t elementBytes, void **data) {
+  napi_value buffer, array;
+  if (napi_create_arraybuffer(env, elements * elementBytes, data, &buffer) != napi_ok) return nullptr;
+  if (napi_create_typedarray(env, type, elements, buffer, 0, &array) != napi_ok) return nullptr;
+  return array;
+}
+
+double numberProp(napi_env env, napi_value obj, const char *name, double fallback) {
+  bool has = false;
+  napi_has_named_property(env, obj, name, &has);
+  if (!has) return fallback;
+  napi_value v;
+  double d = fallback;
+  napi_get_named_property(env, obj, name, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_number) napi_get_value_double(env, v, &d);
+  return d;
+}
+
+napi_value prop(napi_env env, napi_value obj, const char *name) {
+  napi_value v = nullptr;
+  napi_get_named_property(env, obj, name, &v);
+  return v;
+}
+
+// --- exports ------------------------------------------------------------------------------------------------------
+
+napi_value Version(napi_env env, napi_callback_info) {
+  napi_value s;
+  NAPI_OK(napi_create_string_utf8(env, avbd_version(), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+napi_value DeviceCount(napi_env env, napi_callback_info) {
+  napi_value v;
+  NAPI_OK(napi_create_int32(env, avbd_device_count(), &v));
+  return v;
+}
+
+/**
+ * createWorld(scenes, options): scenes = [{bodies: Float32Array, joints: Float32Array, jointInfo: Int32Array,
+ * ignore: Uint32Array, filters: Uint32Array | null, dt, gravity, up: [3], alpha, gamma, betaLin, betaAng, iterations,
+ * flags}]; options = {colorCap, colorRounds, pairCapacity, graph, device, adaptive, broadphase (0 auto, 1 grid,
+ * 2 hploc), bvhRebuildInterval}. Returns the world handle (avbd_batch_create).
+ */
+napi_value CreateWorld(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  uint32_t n = 0;
+  NAPI_OK(napi_get_array_length(env, argv[0], &n));
+  std::vector<AvbdSceneDesc> descs(n);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value s;
+    NAPI_OK(napi_get_element(env, argv[0], k, &s));
+    AvbdSceneDesc &d = descs[k];
+    std::memset(&d, 0, sizeof d);
+    void *p;
+    size_t len;
+    if (!typed(env, prop(env, s, "bodies"), napi_float32_array, &p, &len, false)) return nullptr;
+    if (len % AVBD_BODY_FLOATS) return throwStatus(env, AVBD_ERR_INVALID);
+    d.body_count = uint32_t(len / AVBD_BODY_FLOATS);
+    d.bodies = static_cast<const float *>(p);
+    if (!typed(env, prop(env, s, "joints"), napi_float32_array, &p, &len, true)) return nullptr;
+    d.joint_count = uint32_t(len / AVBD_JOINT_FLOATS);
+    d.joints = static_cast<const float *>(p);
+    if (!typed(env, prop(env, s, "jointInfo"), napi_int32_array, &p, &len, true)) return nullptr;
+    if (len != 4ull * d.joint_count) {
+      napi_throw_range_error(env, nullptr, "jointInfo needs 4 integers per joint");
+      return nullptr;
+    }
+    d.joint_info = static_cast<const int32_t *>(p);
+    if (!typed(env, prop(env, s, "ignore"), napi_uint32_array, &p, &len, true)) return nullptr;
+    d.ignore_count = uint32_t(len / 2);
+    d.ignore_pairs = static_cast<const uint32_t *>(p);
+    if (!typed(env, prop(env, s, "filters"), napi_uint32_array, &p, &len, true)) return nullptr;
+    d.filters = static_cast<const uint32_t *>(p);
+    d.dt = float(numberProp(env, s, "dt", 1.0 / 60));
+    d.gravity = float(numberProp(env, s, "gravity", -10));
+    napi_value up = prop(env, s, "up");
+    for (uint32_t c = 0; c < 3; c++) {
+      napi_value e;
+      double v = c == 1 ? 1 : 0;
+      if (napi_get_element(env, up, c, &e) == napi_ok) napi_get_value_double(env, e, &v);
+      d.up[c] = float(v);
+    }
+    d.alpha = float(numberProp(env, s, "alpha", 0.99));
+    d.gamma = float(numberProp(env, s, "gamma", 0.999));
+    d.beta_lin = float(numberProp(env, s, "betaLin", 1e4));
+    d.beta_ang = float(numberProp(env, s, "betaAng", 100));
+    d.iterations = uint32_t(numberProp(env, s, "iterations", 10));
+    d.flags = uint32_t(numberProp(env, s, "flags", AVBD_FLAGS_DEFAULT));
+  }
+  AvbdOptions o;
+  std::memset(&o, 0, sizeof o);
+  o.color_cap = uint32_t(numberProp(env, argv[1], "colorCap", 0));
+  o.color_rounds = uint32_t(numberProp(env, argv[1], "colorRounds", 0));
+  o.pair_capacity = uint32_t(numberProp(env, argv[1], "pairCapacity", 0));
+  o.use_graph = uint32_t(numberProp(env, argv[1], "graph", 0));
+  o.device = int32_t(numberProp(env, argv[1], "device", 0));
+  o.adaptive = uint32_t(numberProp(env, argv[1], "adaptive", 0));
+  o.broadphase = uint32_t(numberProp(env, argv[1], "broadphase", 0));
+  o.bvh_rebuild_interval = uint32_t(numberProp(env, argv[1], "bvhRebuildInterval", 0));
+  avbd_world *w = nullptr;
+  AVBD_OK_OR_THROW(avbd_batch_create(descs.data(), n, &o, &w));
+  napi_value handle;
+  NAPI_OK(napi_create_external(env, w, nullptr, nullptr, &handle));
+  return handle;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_destroy(w));
+  return nullptr;
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t n;
+  if (!w || !u32(env, argv[1], &n)) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_step(w, n));
+  return nullptr;
+}
+
+napi_value StepTimed(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t n;
+  if (!w || !u32(env, argv[1], &n)) return nullptr;
+  void *data;
+  napi_value out = newTyped(env, napi_float32_array, n, 4, &data);
+  if (!out) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_step_timed(w, n, static_cast<float *>(data)));
+  return out;
+}
+
+napi_value Synchronize(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_synchronize(w));
+  return nullptr;
+}
+
+napi_value Counts(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  uint32_t bodies = 0, joints = 0;
+  AVBD_OK_OR_THROW(avbd_world_body_count(w, &bodies));
+  AVBD_OK_OR_THROW(avbd_world_joint_count(w, &joints));
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  NAPI_OK(napi_create_uint32(env, bodies, &v));
+  NAPI_OK(napi_set_named_property(env, o, "bodies", v));
+  NAPI_OK(napi_create_uint32(env, joints, &v));
+  NAPI_OK(napi_set_named_property(env, o, "joints", v));
+  return o;
+}
+
+template <int FLOATS, int (*READ)(avbd_world *, uint32_t, uint32_t, float *)>
+napi_value ReadRecords(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t first, count;
+  if (!w || !u32(env, argv[1], &first) || !u32(env, argv[2], &count)) return nullptr;
+  void *data;
+  napi_value out = newTyped(env, napi_float32_array, size_t(count) * FLOATS, 4, &data);
+  if (!out) return nullptr;
+  if (count) AVBD_OK_OR_THROW(READ(w, first, count, static_cast<float *>(data)));
+  return out;
+}
+
+template <int FLOATS, int (*WRITE)(avbd_world *, uint32_t, uint32_t, const float *)>
+napi_value WriteRecords(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t first;
+  void *data;
+  size_t len;
+  if (!w || !u32(env, argv[1], &first) || !typed(env, argv[2], napi_float32_array, &data, &len, false)) return nullptr;
+  if (len % FLOATS) return throwStatus(env, AVBD_ERR_INVALID);
+  AVBD_OK_OR_THROW(WRITE(w, first, uint32_t(len / FLOATS), static_cast<const float *>(data)));
+  return nullptr;
+}
+
+napi_value ReadSelected(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx;
+  size_t n;
+  bool posesOnly = false;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false)) return nullptr;
+  napi_get_value_bool(env, argv[2], &posesOnly);
+  void *data;
+  napi_value out = newTyped(env, napi_float32_array, n * (posesOnly ? 8 : AVBD_BODY_FLOATS), 4, &data);
+  if (!out) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_read_selected(w, static_cast<const uint32_t *>(idx), uint32_t(n), posesOnly ? 1 : 0,
+                                            static_cast<float *>(data)));
+  return out;
+}
+
+napi_value ReadCounters(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  AvbdCounters c;
+  AVBD_OK_OR_THROW(avbd_world_read_counters(w, &c));
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  const struct {
+    const char *name;
+    uint32_t value;
+  } fields[] = {{"pairs", c.pairs},       {"contacts", c.contacts}, {"manifolds", c.manifolds},
+                {"joints", c.joints},     {"overflow", c.overflow}, {"clashes", c.clashes},
+                {"colors", c.colors},     {"sleeping", 0},          {"pivotFallbacks", c.pivot_fallbacks}};
+  for (const auto &f : fields) {
+    NAPI_OK(napi_create_uint32(env, f.value, &v));
+    NAPI_OK(napi_set_named_property(env, o, f.name, v));
+  }
+  return o;
+}
+
+napi_value ReadPairs(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  uint32_t count = 0;
+  AVBD_OK_OR_THROW(avbd_world_read_pairs(w, nullptr, 0, &count));
+  void *data;
+  napi_value out = newTyped(env, napi_uint32_array, 2ull * count, 4, &data);
+  if (!out) return nullptr;
+  uint32_t got = 0;
+  if (count) AVBD_OK_OR_THROW(avbd_world_read_pairs(w, static_cast<uint32_t *>(data), count, &got));
+  return out;
+}
+
+napi_value ReadContacts(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  uint32_t m = 0, c = 0;
+  AVBD_OK_OR_THROW(avbd_world_read_contacts(w, nullptr, 0, &m, nullptr, 0, &c));
+  void *md, *cd;
+  napi_value mo = newTyped(env, napi_uint32_array, size_t(m) * AVBD_MANIFOLD_WORDS, 4, &md);
+  napi_value co = newTyped(env, napi_uint32_array, size_t(c) * AVBD_CONTACT_WORDS, 4, &cd);
+  if (!mo || !co) return nullptr;
+  uint32_t gm = 0, gc = 0;
+  AVBD_OK_OR_THROW(avbd_world_read_contacts(w, static_cast<uint32_t *>(md), m, &gm, static_cast<uint32_t *>(cd), c, &gc));
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  NAPI_OK(napi_set_named_property(env, o, "manifolds", mo));
+  NAPI_OK(napi_set_named_property(env, o, "contacts", co));
+  return o;
+}
+
+napi_value Adapt(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  int changed = 0;
+  AVBD_OK_OR_THROW(avbd_world_adapt(w, &changed));
+  napi_value v;
+  NAPI_OK(napi_get_boolean(env, changed != 0, &v));
+  return v;
+}
+
+napi_value SetColoring(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t cap, rounds;
+  if (!w || !u32(env, argv[1], &cap) || !u32(env, argv[2], &rounds)) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_set_coloring(w, cap, rounds));
+  return nullptr;
+}
+
+napi_value DeviceBytes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  uint64_t bytes = 0;
+  AVBD_OK_OR_THROW(avbd_world_device_bytes(w, &bytes));
+  napi_value v;
+  NAPI_OK(napi_create_double(env, double(bytes), &v));
+  return v;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  const napi_property_descriptor props[] = {
+      {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"deviceCount", nullptr, DeviceCount, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"createWorld", nullptr, CreateWorld, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"stepTimed", nullptr, StepTimed, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"synchronize", nullptr, Synchronize, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"counts", nullptr, Counts, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readBodies", nullptr, ReadRecords<AVBD_BODY_FLOATS, avbd_world_read_bodies>, nullptr, nullptr, nullptr,
+       napi_enumerable, nullptr},
+      {"writeBodies", nullptr, WriteRecords<AVBD_BODY_FLOATS, avbd_world_write_bodies>, nullptr, nullptr, nullptr,
+       napi_enumerable, nullptr},
+      {"readJoints", nullptr, ReadRecords<AVBD_JOINT_FLOATS, avbd_world_read_joints>, nullptr, nullptr, nullptr,
+       napi_enumerable, nullptr},
+      {"writeJoints", nullptr, WriteRecords<AVBD_JOINT_FLOATS, avbd_world_write_joints>, nullptr, nullptr, nullptr,
+       napi_enumerable, nullptr},
+      {"readSelected", nullptr, ReadSelected, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readCounters", nullptr, ReadCounters, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readPairs", nullptr, ReadPairs, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readContacts", nullptr, ReadContacts, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"adapt", nullptr, Adapt, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setColoring", nullptr, SetColoring, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"deviceBytes", nullptr, DeviceBytes, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+  };
+  if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return nullptr;
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(avbd_napi, Init)

@@ -1,0 +1,438 @@
+'use strict';
+// JavaScript world API over the N-API addon (avbd_napi.node → libavbd_hip.so): the reference's
+// createSolverPipeline({backend}) (src/native/pipeline.js:14-93) with a "hip" backend, its AvbdScene3D builder
+// (src/native/scenes.js:107-702, bodies and joints) and a world with the shape of its GPU worlds
+// (src/native/solver3D.js:104-368; scenes.d.ts:317-454): step, advance, reads returning Promises, adapt, destroy.
+// Written for Node >= 12 (CommonJS, no optional chaining). Records are packed exactly as the Python builder of this
+// package packs them, so one scene gives the same bytes from either host.
+
+const path = require('path');
+
+let addon = null;
+function native() {
+  if (!addon) addon = require(path.join(__dirname, 'avbd_napi.node'));
+  return addon;
+}
+
+const BIG = 3.0e38;
+const T_JOINT = 1;
+const T_SPRING = 2;
+const SHAPE_BOX = 0;
+const SHAPE_SPHERE = 1;
+const SHAPE_CAPSULE = -1;
+const FLAGS_DEFAULT = 31;
+
+function finite(x) {
+  if (x === Infinity) return BIG;
+  if (x === -Infinity) return -BIG;
+  return x;
+}
+
+function vector(v, n, name) {
+  if (!v || v.length !== n) throw Error(`${name} requires ${n} finite components`);
+  const a = Array.from(v, Number);
+  if (!a.every(Number.isFinite)) throw Error(`${name} requires ${n} finite components`);
+  return a;
+}
+
+function positive(v, name, zero) {
+  const x = Number(v);
+  if (!Number.isFinite(x) || (zero ? x < 0 : x <= 0))
+    throw Error(`${name} must be ${zero ? 'nonnegative' : 'positive'} and finite`);
+  return x;
+}
+
+/** ref/math.ts rotate (x, y, z, w quaternion). */
+function rotate(q, v) {
+  const ux = q[0], uy = q[1], uz = q[2], w = q[3];
+  const tx = (uy * v[2] - uz * v[1]) * 2;
+  const ty = (uz * v[0] - ux * v[2]) * 2;
+  const tz = (ux * v[1] - uy * v[0]) * 2;
+  return [v[0] + tx * w + (uy * tz - uz * ty), v[1] + ty * w + (uz * tx - ux * tz), v[2] + tz * w + (ux * ty - uy * tx)];
+}
+
+/** Box mass, principal moments and bounding radius (ref/body.ts:38-47). */
+function boxMass(s, density) {
+  const m = s[0] * s[1] * s[2] * density;
+  const moment = [((s[1] * s[1] + s[2] * s[2]) / 12) * m, ((s[0] * s[0] + s[2] * s[2]) / 12) * m,
+    ((s[0] * s[0] + s[1] * s[1]) / 12) * m];
+  const hx = s[0] * 0.5, hy = s[1] * 0.5, hz = s[2] * 0.5;
+  return [m, moment, Math.sqrt(hx * hx + hy * hy + hz * hz)];
+}
+
+/** Solid capsule along local Y of full tip-to-tip height (src/gpu/capsuleShape.js:28-38). */
+function capsuleMass(radius, height, density) {
+  const length = height - 2 * radius;
+  const r2 = radius * radius, l2 = length * length;
+  const cylinder = Math.PI * radius * radius * length * density;
+  const caps = (4 / 3) * Math.PI * (r2 * radius) * density;
+  const axial = (cylinder * r2) / 2 + (caps * 2 * r2) / 5;
+  const radial = cylinder * (l2 / 12 + r2 / 4) + caps * ((2 * r2) / 5 + l2 / 4 + (3 * length * radius) / 8);
+  return [cylinder + caps, [radial, axial, radial], height / 2];
+}
+
+/** One 40-float body record as GpuSolver3D.pack writes a freshly built body (solver.ts:794-820). */
+function bodyRecord(position, size, mass, moment, radius, friction, shape) {
+  const r = new Float32Array(40);
+  r.set(position, 0);
+  r[3] = friction;
+  r[7] = 1; // rotation (0, 0, 0, 1)
+  r[15] = 1; // initialRot
+  r.set(size, 16);
+  r[19] = mass;
+  r.set(moment, 20);
+  r[23] = radius;
+  r[31] = 1; // inertialRot
+  r[39] = shape;
+  return r;
+}
+
+/** Portable 3D scene builder (scenes.js:107-273): Y up, gravity -10, dt 1/60, 10 iterations by default. */
+class AvbdScene3D {
+  constructor(options) {
+    const o = options || {};
+    const gravity = o.gravity === undefined ? -10 : Number(o.gravity);
+    if (!Number.isFinite(gravity)) throw Error('gravity must be finite');
+    const iterations = o.iterations === undefined ? 10 : o.iterations;
+    if (!Number.isInteger(iterations) || iterations < 1) throw Error('iterations must be an integer');
+    this.params = {
+      dt: positive(o.timeStep === undefined ? 1 / 60 : o.timeStep, 'timeStep'),
+      gravity,
+      iterations,
+      alpha: o.alpha === undefined ? 0.99 : Number(o.alpha),
+      gamma: o.gamma === undefined ? 0.999 : Number(o.gamma),
+      betaLin: o.betaLin === undefined ? 1e4 : Number(o.betaLin),
+      betaAng: o.betaAng === undefined ? 100 : Number(o.betaAng),
+      flags: o.flags === undefined ? FLAGS_DEFAULT : o.flags >>> 0,
+    };
+    this.up = vector(o.up || [0, 1, 0], 3, 'up');
+    if (Math.abs(Math.hypot(this.up[0], this.up[1], this.up[2]) - 1) > 1e-5) throw Error('up must be a unit vector');
+    this.records = [];
+    this.filters = new Map();
+    this.joints = [];
+    this.ignore = [];
+    this.created = false;
+  }
+
+  get bodyCount() { return this.records.length; }
+
+  assertEditable() {
+    if (this.created) throw Error('Build this scene before createSolver; use the solver\'s edit methods afterwards');
+  }
+
+  add(rec, options) {
+    this.assertEditable();
+    const o = options || {};
+    if (o.rotation !== undefined && o.rotation !== null) {
+      const q = vector(o.rotation, 4, 'rotation');
+      const l = Math.sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+      if (!l) throw Error('rotation must be nonzero');
+      rec.set(q.map((x) => x / l), 4);
+    }
+    if (o.mass !== undefined && o.mass !== null) {
+      const mass = positive(o.mass, 'mass', true);
+      if (rec[19] === 0 && mass > 0) throw Error('A positive mass needs positive density');
+      const ratio = rec[19] > 0 ? mass / rec[19] : 0;
+      const f = Math.fround(ratio);
+      rec[19] = mass;
+      for (let k = 20; k < 23; k++) rec[k] = Math.fround(rec[k] * f);
+    }
+    if (o.velocity !== undefined && o.velocity !== null) rec.set(vector(o.velocity, 3, 'velocity'), 32);
+    if (o.angularVelocity !== undefined && o.angularVelocity !== null)
+      rec.set(vector(o.angularVelocity, 3, 'angularVelocity'), 36);
+    const group = o.group === undefined ? 0xffffffff : o.group;
+    const mask = o.collidesWith === undefined ? 0xffffffff : o.collidesWith;
+    for (const m of [group, mask])
+      if (!Number.isInteger(m) || m < 0 || m > 0xffffffff) throw Error('Collision masks must be unsigned 32-bit integers');
+    const index = this.records.length;
+    if (group !== 0xffffffff || mask !== 0xffffffff) this.filters.set(index, [group, mask]);
+    this.records.push(rec);
+    return index;
+  }
+
+  /** A centred box with full XYZ extents; zero density makes it static (scenes.js:131-144). */
+  addBox(size, options) {
+    const o = options || {};
+    const s = vector(size, 3, 'size');
+    s.forEach((v) => positive(v, 'size'));
+    const density = positive(o.density === undefined ? 1 : o.density, 'density', true);
+    const friction = positive(o.friction === undefined ? 0.6 : o.friction, 'friction', true);
+    const pos = vector(o.position || [0, 0, 0], 3, 'position');
+    const [m, moment, radius] = boxMass(s, density);
+    return this.add(bodyRecord(pos, s, m, moment, radius, friction, SHAPE_BOX), o);
+  }
+
+  /** A solid sphere (scenes.js:145-159; shapes.ts:28-36). */
+  addSphere(radius, options) {
+    const o = options || {};
+    const r = positive(radius, 'radius');
+    const density = positive(o.density === undefined ? 1 : o.density, 'density', true);
+    const friction = positive(o.friction === undefined ? 0.6 : o.friction, 'friction', true);
+    const pos = vector(o.position || [0, 0, 0], 3, 'position');
+    const m = density > 0 ? (4 / 3) * Math.PI * r * r * r * density : 0;
+    const inertia = 0.4 * m * r * r;
+    return this.add(bodyRecord(pos, [2 * r, 2 * r, 2 * r], m, [inertia, inertia, inertia], r, friction, SHAPE_SPHERE), o);
+  }
+
+  /** An analytic capsule along local Y; height is the full tip-to-tip extent (scenes.js:181-197). */
+  addCapsule(radius, height, options) {
+    const o = options || {};
+    const r = Number(radius);
+    const h = Number(height);
+    if (!(Number.isFinite(r) && r > 0 && Number.isFinite(h) && h >= 2 * r))
+      throw Error('Capsule height must be at least twice its positive radius');
+    const density = positive(o.density === undefined ? 1 : o.density, 'density', true);
+    const friction = positive(o.friction === undefined ? 0.6 : o.friction, 'friction', true);
+    const pos = vector(o.position || [0, 0, 0], 3, 'position');
+    const [m, moment, bound] = capsuleMass(r, h, density);
+    return this.add(bodyRecord(pos, [2 * r, h, 2 * r], m, moment, bound, friction, SHAPE_CAPSULE), o);
+  }
+
+  check(a, b) {
+    const n = this.records.length;
+    if (b === null || b === undefined || a === b || !(b >= 0 && b < n) || (a !== null && !(a >= 0 && a < n)))
+      throw Error('Constraint endpoints must be distinct bodies in this scene');
+  }
+
+  /** Ball joint plus optional angle lock (scenes.js:508-540; forces.ts:68-110). a = null: anchorA is a world point. */
+  addJoint(a, b, options) {
+    const o = options || {};
+    this.assertEditable();
+    this.check(a, b);
+    const linear = o.linearStiffness === undefined ? Infinity : o.linearStiffness;
+    const angular = o.angularStiffness === undefined ? 0 : o.angularStiffness;
+    const breakForce = o.breakForce === undefined ? Infinity : o.breakForce;
+    for (const v of [linear, angular, breakForce]) if (v !== Infinity) positive(v, 'joint stiffness/breakForce', true);
+    const rA = vector(o.anchorA || [0, 0, 0], 3, 'anchorA');
+    const rB = vector(o.anchorB || [0, 0, 0], 3, 'anchorB');
+    const frac = o.breakOnPull && breakForce !== Infinity ? -breakForce : breakForce;
+    const sA = a !== null ? this.records[a].subarray(16, 19) : [0, 0, 0];
+    const sB = this.records[b].subarray(16, 19);
+    const s = [sA[0] + sB[0], sA[1] + sB[1], sA[2] + sB[2]];
+    const rec = new Float32Array(32);
+    rec[3] = finite(linear);
+    rec[7] = finite(angular);
+    rec[11] = finite(frac);
+    rec[15] = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    rec.set(rA, 24);
+    rec.set(rB, 28);
+    this.joints.push([rec, [T_JOINT, a === null ? -1 : a, b, 0]]);
+    return this.joints.length - 1;
+  }
+
+  /** Spring between body-local anchors; rest -1 takes their initial separation (scenes.js:546-575). */
+  addSpring(a, b, options) {
+    const o = options || {};
+    this.assertEditable();
+    this.check(a, b);
+    if (a === null) throw Error('A spring requires two bodies');
+    const stiffness = positive(o.stiffness === undefined ? 1000 : o.stiffness, 'stiffness', true);
+    let rest = o.rest === undefined ? -1 : o.rest;
+    if (rest !== -1) positive(rest, 'rest', true);
+    const rA = vector(o.anchorA || [0, 0, 0], 3, 'anchorA');
+    const rB = vector(o.anchorB || [0, 0, 0], 3, 'anchorB');
+    if (rest < 0) {
+      const ra = this.records[a], rb = this.records[b];
+      const qa = Array.from(ra.subarray(4, 8)), qb = Array.from(rb.subarray(4, 8));
+      const pa = rotate(qa, rA).map((x, k) => x + ra[k]);
+      const pb = rotate(qb, rB).map((x, k) => x + rb[k]);
+      const d = [pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]];
+      rest = Math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    const rec = new Float32Array(32);
+    rec[3] = stiffness;
+    rec.set(rA, 24);
+    rec[27] = rest;
+    rec.set(rB, 28);
+    this.joints.push([rec, [T_SPRING, a, b, 0]]);
+    return this.joints.length - 1;
+  }
+
+  /** The pair never collides (ref/forces.ts IgnoreCollision). */
+  ignoreCollision(a, b) {
+    this.assertEditable();
+    this.check(a, b);
+    this.ignore.push([a, b]);
+  }
+
+  /** Packed records in the layouts of include/avbd.h (what avbd_world_create takes). */
+  packed() {
+    const n = this.records.length;
+    const bodies = new Float32Array(40 * n);
+    this.records.forEach((r, i) => bodies.set(r, 40 * i));
+    const joints = new Float32Array(32 * this.joints.length);
+    const jointInfo = new Int32Array(4 * this.joints.length);
+    this.joints.forEach(([r, info], j) => { joints.set(r, 32 * j); jointInfo.set(info, 4 * j); });
+    const ignore = new Uint32Array(2 * this.ignore.length);
+    this.ignore.forEach(([a, b], k) => { ignore[2 * k] = a; ignore[2 * k + 1] = b; });
+    let filters = null;
+    if (this.filters.size) {
+      filters = new Uint32Array(2 * n).fill(0xffffffff);
+      for (const [i, [g, m]] of this.filters) { filters[2 * i] = g; filters[2 * i + 1] = m; }
+    }
+    return Object.assign({ bodies, joints, jointInfo, ignore, filters, up: this.up }, this.params);
+  }
+}
+
+function nativeOptions(options) {
+  const o = options || {};
+  const bp = o.broadphase === undefined ? 'auto' : o.broadphase;
+  const code = { auto: 0, grid: 1, hploc: 2 }[bp];
+  if (code === undefined) throw Error(`Unknown broadphase: ${bp}`);
+  const capacity = o.capacity || {};
+  return {
+    colorCap: o.colorCap || 0,
+    colorRounds: o.colorRounds || 0,
+    pairCapacity: capacity.pairs || 0,
+    graph: o.graph ? 1 : 0,
+    adaptive: o.adaptive ? 1 : 0,
+    broadphase: code,
+    bvhRebuildInterval: (o.bvh && o.bvh.rebuildInterval) || 0,
+  };
+}
+
+/** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */
+class HipSolver3D {
+  constructor(scenes, options, device) {
+    const list = Array.isArray(scenes) ? scenes : [scenes];
+    if (!list.length) throw Error('An AVBD scene builder is required');
+    for (const s of list) {
+      if (!(s instanceof AvbdScene3D)) throw Error('An AVBD scene builder is required');
+      s.assertEditable();
+    }
+    const opts = Object.assign(nativeOptions(options), { device });
+    this.handle = native().createWorld(list.map((s) => s.packed()), opts);
+    for (const s of list) s.created = true;
+    const counts = native().counts(this.handle);
+    this.bodyCount = counts.bodies;
+    this.jointCount = counts.joints;
+    this.params = Object.assign({}, list[0].params, { up: list[0].up.slice() });
+    this.timeAccumulator = 0;
+    this.destroyed = false;
+  }
+
+  alive() { if (this.destroyed) throw Error('The solver has been destroyed'); }
+
+  /** Submit one step without waiting (solver3D.js:350-359). */
+  step() { this.alive(); native().step(this.handle, 1); }
+
+  /** Fixed-step accumulator; returns the steps submitted (runtime.js:255-272). */
+  advance(elapsed, options) {
+    this.alive();
+    const o = options || {};
+    const maxSubSteps = o.maxSubSteps === undefined ? 6 : o.maxSubSteps;
+    const maxFrameTime = o.maxFrameTime === undefined ? 0.05 : o.maxFrameTime;
+    if (!Number.isFinite(elapsed) || elapsed < 0) throw Error('elapsed must be finite nonnegative seconds');
+    if (!Number.isInteger(maxSubSteps) || maxSubSteps < 1 || !Number.isFinite(maxFrameTime) || maxFrameTime <= 0)
+      throw Error('maxSubSteps must be a positive integer and maxFrameTime positive');
+    const dt = this.params.dt;
+    this.timeAccumulator = Math.min(maxSubSteps * dt, this.timeAccumulator + Math.min(maxFrameTime, elapsed));
+    let count = 0;
+    while (this.timeAccumulator + 1e-12 >= dt && count < maxSubSteps) {
+      this.step();
+      this.timeAccumulator = Math.max(0, this.timeAccumulator - dt);
+      count++;
+    }
+    return count;
+  }
+
+  /** Per-step device milliseconds of n steps (waits). */
+  stepTimed(n) { this.alive(); return native().stepTimed(this.handle, n); }
+
+  /** All bodies, 40 f32 each, in slot order (solver.ts:1421-1423). */
+  readBodies() { this.alive(); return Promise.resolve(native().readBodies(this.handle, 0, this.bodyCount)); }
+
+  /** readSelectedBodies (solver3D.js:223-230): 40 or (posesOnly) 8 f32 per body. */
+  readSelectedBodies(indices, options) {
+    this.alive();
+    const idx = indices instanceof Uint32Array ? indices : Uint32Array.from(indices);
+    return Promise.resolve(native().readSelected(this.handle, idx, !!(options && options.posesOnly)));
+  }
+
+  readJoints() { this.alive(); return Promise.resolve(native().readJoints(this.handle, 0, this.jointCount)); }
+
+  /** {pairs, contacts, manifolds, joints, overflow, clashes, colors, sleeping} (world.js:672-692). */
+  readCounters() { this.alive(); return Promise.resolve(native().readCounters(this.handle)); }
+
+  /** The last step's broadphase pairs (a > b) as [a0, b0, a1, b1, ...] (solver.ts:1471-1475). */
+  readPairs() { this.alive(); return Promise.resolve(native().readPairs(this.handle)); }
+
+  /** readContactList (solver.ts:1435-1469): {a, b, feature, stick, rA, rB, pen, lam, normal} per contact point. */
+  readContactList() {
+    this.alive();
+    const { manifolds, contacts } = native().readContacts(this.handle);
+    const f32 = new Float32Array(contacts.buffer);
+    const mf32 = new Float32Array(manifolds.buffer);
+    const out = [];
+    for (let m = 0; m < manifolds.length / 8; m++) {
+      const a = manifolds[8 * m], b = manifolds[8 * m + 1], first = manifolds[8 * m + 2], count = manifolds[8 * m + 3] & 15;
+      const normal = [mf32[8 * m + 4], mf32[8 * m + 5], mf32[8 * m + 6]];
+      for (let c = first; c < first + count; c++) {
+        const w = 16 * c;
+        out.push({
+          a, b, normal,
+          feature: (contacts[w + 3] & 0x7fffffff) >>> 0,
+          stick: (contacts[w + 3] & 0x80000000) !== 0,
+          rA: [f32[w], f32[w + 1], f32[w + 2]],
+          rB: [f32[w + 4], f32[w + 5], f32[w + 6]],
+          pen: [f32[w + 8], f32[w + 9], f32[w + 10]],
+          lam: [f32[w + 12], f32[w + 13], f32[w + 14]],
+        });
+      }
+    }
+    return Promise.resolve(out);
+  }
+
+  /** adapt(counters) (solver.ts:1484-1519): grow storage and the colour cap from the last counters. */
+  adapt() { this.alive(); return native().adapt(this.handle); }
+
+  destroy() {
+    if (this.destroyed) return;
+    native().destroy(this.handle);
+    this.destroyed = true;
+  }
+}
+
+/**
+ * createSolverPipeline({backend: "hip", device}) (pipeline.js:14-93): worlds share the pipeline's device; destroy()
+ * releases its worlds. createBatch steps many scenes in one launch per phase.
+ */
+async function createSolverPipeline(options) {
+  const o = options || {};
+  const backend = o.backend === undefined ? 'hip' : o.backend;
+  if (backend !== 'hip') throw Error(`Unsupported solver backend: ${backend}`);
+  const device = o.device === undefined ? 0 : o.device;
+  const worlds = new Set();
+  let destroyed = false;
+  const track = (world) => {
+    Object.defineProperty(world, 'backend', { value: backend, configurable: true });
+    worlds.add(world);
+    const destroy = world.destroy;
+    world.destroy = function () {
+      destroy.call(this);
+      worlds.delete(this);
+    };
+    return world;
+  };
+  return {
+    backend,
+    device,
+    createSolver(scene, solverOptions) {
+      if (destroyed) throw Error('The solver pipeline has been destroyed');
+      if (!scene || typeof scene.assertEditable !== 'function') throw Error('An AVBD scene builder is required');
+      return track(new HipSolver3D([scene], solverOptions, device));
+    },
+    createBatch(scenes, solverOptions) {
+      if (destroyed) throw Error('The solver pipeline has been destroyed');
+      return track(new HipSolver3D(scenes, solverOptions, device));
+    },
+    destroy() {
+      if (destroyed) return;
+      for (const world of Array.from(worlds)) world.destroy();
+      worlds.clear();
+      destroyed = true;
+    },
+  };
+}
+
+module.exports = { createSolverPipeline, AvbdScene3D, HipSolver3D, native, FLAGS_DEFAULT };

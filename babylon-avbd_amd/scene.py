@@ -57,17 +57,18 @@ def box_mass(size, density):
     sx, sy, sz = size
     m = sx * sy * sz * density
     moment = [((sy * sy + sz * sz) / 12) * m, ((sx * sx + sz * sz) / 12) * m, ((sx * sx + sy * sy) / 12) * m]
-    return m, moment, math.sqrt((sx * 0.5) ** 2 + (sy * 0.5) ** 2 + (sz * 0.5) ** 2)
+    hx, hy, hz = sx * 0.5, sy * 0.5, sz * 0.5
+    return m, moment, math.sqrt(hx * hx + hy * hy + hz * hz)
 
 
 def capsule_mass(radius, height, density):
     """Solid capsule along local Y of full tip-to-tip height (src/gpu/capsuleShape.js:28-38)."""
     length = height - 2 * radius
+    r2, l2 = radius * radius, length * length
     cylinder = math.pi * radius * radius * length * density
-    caps = (4 / 3) * math.pi * radius ** 3 * density
-    axial = (cylinder * radius ** 2) / 2 + (caps * 2 * radius ** 2) / 5
-    radial = cylinder * (length ** 2 / 12 + radius ** 2 / 4) + caps * (
-        (2 * radius ** 2) / 5 + length ** 2 / 4 + (3 * length * radius) / 8)
+    caps = (4 / 3) * math.pi * (r2 * radius) * density
+    axial = (cylinder * r2) / 2 + (caps * 2 * r2) / 5
+    radial = cylinder * (l2 / 12 + r2 / 4) + caps * ((2 * r2) / 5 + l2 / 4 + (3 * length * radius) / 8)
     return cylinder + caps, [radial, axial, radial], height / 2
 
 
@@ -218,7 +219,8 @@ class AvbdScene3D:
         rec[:, 20] = ((sy * sy + sz * sz) / 12) * m
         rec[:, 21] = ((sx * sx + sz * sz) / 12) * m
         rec[:, 22] = ((sx * sx + sy * sy) / 12) * m
-        rec[:, 23] = np.sqrt((sx * 0.5) ** 2 + (sy * 0.5) ** 2 + (sz * 0.5) ** 2)
+        hx, hy, hz = sx * 0.5, sy * 0.5, sz * 0.5
+        rec[:, 23] = np.sqrt(hx * hx + hy * hy + hz * hz)
         rec[:, 31] = 1.0
         if velocities is not None:
             rec[:, 32:35] = velocities
@@ -284,7 +286,8 @@ class AvbdScene3D:
             ra, rb = self.record(a), self.record(b)
             pa = np.add(_rotate([float(x) for x in ra[4:8]], rA), ra[0:3].astype(np.float64))
             pb = np.add(_rotate([float(x) for x in rb[4:8]], rB), rb[0:3].astype(np.float64))
-            rest = float(np.linalg.norm(pa - pb))
+            d = pa - pb
+            rest = math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
         rec = np.zeros(32, np.float32)
         rec[3] = float(stiffness)
         rec[24:27] = rA

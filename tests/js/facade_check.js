@@ -1,0 +1,75 @@
+'use strict';
+// Drives the JavaScript world API (babylon-avbd_amd/js/index.js) for the Python tests: builds scenes from a JSON list
+// of builder calls, prints packed records as u32 bit patterns, exercises error paths, and (with a GPU) steps a world.
+const fs = require('fs');
+const path = require('path');
+const avbd = require(path.join(__dirname, '..', '..', 'babylon-avbd_amd', 'js', 'index.js'));
+
+function build(spec) {
+  const s = new avbd.AvbdScene3D(spec.params || {});
+  for (const op of spec.ops) {
+    const [kind, ...a] = op;
+    if (kind === 'box') s.addBox(a[0], a[1]);
+    else if (kind === 'sphere') s.addSphere(a[0], a[1]);
+    else if (kind === 'capsule') s.addCapsule(a[0], a[1], a[2]);
+    else if (kind === 'joint') s.addJoint(a[0], a[1], a[2]);
+    else if (kind === 'spring') s.addSpring(a[0], a[1], a[2]);
+    else if (kind === 'ignore') s.ignoreCollision(a[0], a[1]);
+    else throw Error(`unknown op ${kind}`);
+  }
+  return s;
+}
+
+const bits = (t) => (t ? Array.from(new Uint32Array(t.buffer, t.byteOffset, t.length)) : null);
+
+async function errorOf(f) {
+  try {
+    await f();
+    return null;
+  } catch (e) {
+    return String(e.message);
+  }
+}
+
+async function main() {
+  const mode = process.argv[2];
+  const specs = process.argv[3] ? JSON.parse(fs.readFileSync(process.argv[3], 'utf8')) : [];
+  if (mode === 'exports') {
+    console.log(JSON.stringify({ exports: Object.keys(avbd.native()).sort(), version: avbd.native().version() }));
+  } else if (mode === 'pack') {
+    console.log(JSON.stringify(specs.map((spec) => {
+      const p = build(spec).packed();
+      return { bodies: bits(p.bodies), joints: bits(p.joints), jointInfo: Array.from(p.jointInfo),
+        ignore: Array.from(p.ignore), filters: p.filters ? Array.from(p.filters) : null };
+    })));
+  } else if (mode === 'errors') {
+    const out = {};
+    out.backend = await errorOf(() => avbd.createSolverPipeline({ backend: 'webgpu' }));
+    out.capsule = await errorOf(() => new avbd.AvbdScene3D().addCapsule(0.5, 0.5));
+    out.endpoints = await errorOf(() => { const s = new avbd.AvbdScene3D(); s.addBox([1, 1, 1]); s.addJoint(0, 0); });
+    const pipe = await avbd.createSolverPipeline({ backend: 'hip' });
+    const s = new avbd.AvbdScene3D();
+    s.addBox([1, 1, 1]);
+    out.create = avbd.native().deviceCount() > 0 ? 'has-gpu' : await errorOf(() => pipe.createSolver(s));
+    out.editableAfterFailure = await errorOf(() => s.addBox([1, 1, 1]));
+    console.log(JSON.stringify(out));
+  } else if (mode === 'step') {
+    const pipe = await avbd.createSolverPipeline({ backend: 'hip' });
+    const steps = Number(process.argv[4]);
+    const worlds = specs.map((spec) => pipe.createSolver(build(spec), { colorCap: 32, colorRounds: 16 }));
+    const out = [];
+    for (const w of worlds) {
+      for (let k = 0; k < steps; k++) w.step();
+      const counters = await w.readCounters();
+      out.push({ bodies: bits(await w.readBodies()), counters, pairs: Array.from(await w.readPairs()),
+        poses: bits(await w.readSelectedBodies([0, w.bodyCount - 1], { posesOnly: true })),
+        contacts: (await w.readContactList()).length });
+    }
+    pipe.destroy();
+    console.log(JSON.stringify(out));
+  } else {
+    throw Error(`unknown mode ${mode}`);
+  }
+}
+
+main().catch((e) => { console.error(e.stack || String(e)); process.exit(1); });

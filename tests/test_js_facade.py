@@ -1,0 +1,126 @@
+"""The JavaScript world API (babylon-avbd_amd/js: N-API addon + createSolverPipeline facade) against the Python host.
+
+CPU: the addon loads under Node and exports one function per C entry point it binds; a scene built by the JS builder
+packs to the same bytes as the same scene built by the Python builder (bodies, joints, joint info, ignored pairs,
+filters); errors are thrown with the reference's messages, and world creation without a GPU fails loudly.
+GPU: a world stepped through the JS facade ends bit-identical to the same world stepped through ctypes.
+"""
+import json
+import math
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+CHECK = ROOT / "tests" / "js" / "facade_check.js"
+NODE = shutil.which("node")
+ADDON = ROOT / "babylon-avbd_amd" / "js" / "avbd_napi.node"
+
+pytestmark = pytest.mark.skipif(NODE is None or not ADDON.exists(), reason="node or the built N-API addon is missing")
+
+
+def run(mode, specs=None, tmp_path=None, *extra):
+    args = [NODE, str(CHECK), mode]
+    if specs is not None:
+        f = tmp_path / "specs.json"
+        f.write_text(json.dumps(specs))
+        args.append(str(f))
+    args += [str(x) for x in extra]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def build_py(avbd, spec):
+    s = avbd.AvbdScene3D(**spec.get("params", {}))
+    for kind, *a in spec["ops"]:
+        if kind == "box":
+            s.addBox(a[0], **(a[1] if len(a) > 1 else {}))
+        elif kind == "sphere":
+            s.addSphere(a[0], **(a[1] if len(a) > 1 else {}))
+        elif kind == "capsule":
+            s.addCapsule(a[0], a[1], **(a[2] if len(a) > 2 else {}))
+        elif kind == "joint":
+            s.addJoint(a[0], a[1], **(a[2] if len(a) > 2 else {}))
+        elif kind == "spring":
+            s.addSpring(a[0], a[1], **(a[2] if len(a) > 2 else {}))
+        elif kind == "ignore":
+            s.ignoreCollision(a[0], a[1])
+    return s
+
+
+def specs_mixed():
+    rng = np.random.default_rng(11)
+    ops = [["box", [20, 1, 20], {"density": 0, "friction": 0.5, "position": [0, -0.5, 0]}]]
+    for k in range(12):
+        q = rng.normal(size=4)
+        ops.append(["box", list(rng.uniform(0.2, 1.5, 3)),
+                    {"position": list(rng.uniform(-3, 3, 3)), "rotation": list(q), "density": float(rng.uniform(0.5, 3)),
+                     "friction": float(rng.uniform(0.2, 0.9)), "velocity": list(rng.normal(size=3))}])
+    ops.append(["sphere", 0.37, {"position": [1, 2, 3], "angularVelocity": [0.1, 0.2, 0.3]}])
+    ops.append(["capsule", 0.2, 1.3, {"position": [0, 4, 0], "rotation": [0.1, 0.2, 0.3, 0.9], "density": 2.5}])
+    ops.append(["box", [1, 2, 3], {"mass": 7.3, "group": 2, "collidesWith": 5}])
+    ops.append(["joint", 1, 2, {"anchorA": [0.1, 0.2, 0.3], "anchorB": [-0.1, 0, 0.2], "angularStiffness": 50.0}])
+    ops.append(["joint", None, 3, {"anchorA": [0, 5, 0], "breakForce": 40.0, "breakOnPull": True}])
+    ops.append(["spring", 4, 5, {"anchorA": [0.2, 0, 0], "stiffness": 300.0}])
+    ops.append(["spring", 5, 6, {"rest": 1.25}])
+    ops.append(["ignore", 7, 8])
+    return [{"params": {"gravity": -9.81, "iterations": 12, "timeStep": 1 / 120}, "ops": ops},
+            {"ops": [["box", [1, 1, 1], {"position": [0, 0.5 * k, 0]}] for k in range(5)]}]
+
+
+def test_addon_exports_the_bound_entry_points(tmp_path):
+    out = run("exports")
+    assert out["version"].startswith("avbd-mi355x")
+    assert {"createWorld", "destroy", "step", "readBodies", "readSelected", "readJoints", "readCounters", "readPairs",
+            "readContacts", "adapt", "writeBodies", "writeJoints", "stepTimed"} <= set(out["exports"])
+
+
+def test_js_and_python_builders_pack_the_same_bytes(avbd, tmp_path):
+    specs = specs_mixed()
+    js = run("pack", specs, tmp_path)
+    for spec, got in zip(specs, js):
+        bodies, joints, info, ignore, filters = build_py(avbd, spec).packed()
+        assert got["bodies"] == bodies.view(np.uint32).ravel().tolist()
+        assert got["joints"] == joints.view(np.uint32).ravel().tolist()
+        assert got["jointInfo"] == info.ravel().tolist()
+        assert got["ignore"] == ignore.ravel().tolist()
+        assert got["filters"] == (None if filters is None else filters.ravel().tolist())
+
+
+def test_errors_follow_the_reference(tmp_path):
+    out = run("errors")
+    assert out["backend"] == "Unsupported solver backend: webgpu"
+    assert out["capsule"] == "Capsule height must be at least twice its positive radius"
+    assert out["endpoints"] == "Constraint endpoints must be distinct bodies in this scene"
+    if out["create"] != "has-gpu":
+        assert "no HIP device" in out["create"]
+
+
+@pytest.mark.gpu
+def test_js_world_steps_like_the_python_world(avbd, tmp_path):
+    rng = np.random.default_rng(5)
+    ops = [["box", [30, 1, 30], {"density": 0, "friction": 0.5, "position": [0, -0.5, 0]}]]
+    for k in range(60):
+        ops.append(["box", list(rng.uniform(0.4, 1.2, 3)), {"position": [float(rng.uniform(-2, 2)), 1 + 1.3 * k / 4,
+                                                                     float(rng.uniform(-2, 2))],
+                                                        "rotation": list(rng.normal(size=4)), "friction": 0.5}])
+    ops.append(["capsule", 0.25, 1.2, {"position": [0, 25, 0], "rotation": [0.3, 0.1, 0.2, 0.9]}])
+    ops.append(["sphere", 0.4, {"position": [0.5, 27, 0.2]}])
+    ops.append(["joint", 61, 62, {"anchorA": [0, 0.6, 0], "anchorB": [0, -0.4, 0]}])
+    specs = [{"params": {"gravity": -9.81}, "ops": ops}]
+    steps = 90
+    js = run("step", specs, tmp_path, steps)[0]
+    world = avbd.createSolverPipeline("hip").createSolver(build_py(avbd, specs[0]), dict(colorCap=32, colorRounds=16))
+    world.step(steps)
+    want = world.readBodies()
+    assert js["bodies"] == want.view(np.uint32).ravel().tolist()
+    assert js["pairs"] == world.readPairs().ravel().tolist()
+    assert js["counters"]["pairs"] == world.readCounters()["pairs"]
+    sel = world.readSelectedBodies([0, world.bodyCount - 1], posesOnly=True)
+    assert js["poses"] == sel.view(np.uint32).ravel().tolist()
+    assert js["contacts"] == world.readCounters()["contacts"]
+    world.destroy()
