@@ -67,6 +67,28 @@ class Counters(ctypes.Structure):
                  "pivot_fallbacks")]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_active", ctypes.c_uint32),
+        ("lookahead", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("color_cap", ctypes.c_uint32),
+        ("color_rounds", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+        ("broadphase", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in
+                ("submitted", "completed", "admissions", "evictions", "restores", "steps", "body_steps")] + \
+               [(k, ctypes.c_double) for k in
+                ("wall_seconds", "step_ms_p50", "step_ms_p99", "step_ms_max", "request_ms_p50", "request_ms_p99")] + \
+               [("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
+                ("resident_worlds", ctypes.c_uint32), ("worlds", ctypes.c_uint32)]
+
+
 # name -> (restype, argtypes): every entry point of include/avbd.h
 SIGNATURES = {
     "avbd_last_error": (ctypes.c_char_p, []),
@@ -98,6 +120,17 @@ SIGNATURES = {
     "avbd_world_adapt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),
     "avbd_world_device_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "avbd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "avbd_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "avbd_server_add_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), u32p]),
+    "avbd_server_world_body_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, u32p]),
+    "avbd_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "avbd_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "avbd_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "avbd_server_read_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, f32p]),
+    "avbd_server_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    "avbd_server_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libavbd_hip.so")
-SOURCES = ["collision.hip", "hploc.hip", "topology.hip", "solve.hip", "scan.hip", "engine.hip", "capi.cpp"]
+SOURCES = ["collision.hip", "hploc.hip", "topology.hip", "solve.hip", "scan.hip", "engine.hip", "capi.cpp", "server.cpp"]
 ARCH = os.environ.get("AVBD_OFFLOAD_ARCH", "gfx950")
 
 
@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(out_dir, src.rsplit(".", 1)[0] + ".o")
         cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-Wall", "-c",
+            cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-Wall", "-c",
                    os.path.join(CSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
@@ -53,7 +53,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"linking libavbd_hip.so failed:\n{r.stdout}\n{r.stderr}")

@@ -161,3 +161,39 @@ def batch_sizes(count=4096, lo=8, hi=2048, seed=SEED + 3):
 def batched_piles(count=4096, lo=8, hi=2048, seed=SEED + 3):
     sizes, frictions = batch_sizes(count, lo, hi, seed)
     return [pile_of(int(n), seed + 1000 + k, friction=float(f)) for k, (n, f) in enumerate(zip(sizes, frictions))]
+
+
+def boxes_world(n, seed, iterations=10, friction=0.5, spacing=2.2):
+    """n boxes with the single large world's distributions (half-extents U(0.25, 1.0), uniform random orientations,
+    v ~ N(0, 0.5²) per axis) on a jittered near-cubic lattice at 2.2 m, lowest layer 2 m above a static ground."""
+    rng = np.random.default_rng(seed)
+    side = max(1, int(math.ceil(n ** (1 / 3))))
+    nx = nz = side
+    ny = int(math.ceil(n / (nx * nz)))
+    scene = AvbdScene3D(gravity=-9.81, iterations=iterations, timeStep=1 / 60)
+    g = max(40.0, 2.2 * side * spacing)
+    scene.addBox([g, 1.0, g], density=0, friction=friction, position=[0, -0.5, 0])
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    grid = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64)[:n]
+    pos = np.empty((n, 3))
+    pos[:, 0] = (grid[:, 0] - (nx - 1) / 2) * spacing
+    pos[:, 1] = 2.0 + grid[:, 1] * spacing
+    pos[:, 2] = (grid[:, 2] - (nz - 1) / 2) * spacing
+    pos += rng.uniform(-0.05, 0.05, (n, 3))
+    half = rng.uniform(0.25, 1.0, (n, 3))
+    scene.addBoxes(2 * half, pos, _uniform_quaternions(rng, n), density=1.0, friction=friction,
+                   velocities=rng.normal(0.0, 0.5, (n, 3)))
+    return scene
+
+
+def serving_pool(count=64, lo=16, hi=50000, seed=SEED + 4):
+    """World sizes of the serving workload: log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi), count))).astype(int)
+
+
+def serving_requests(count, rate=2000.0, worlds=64, max_steps=120, seed=SEED + 5):
+    """Poisson arrivals (exponential gaps at `rate` per second) of rollout requests: (arrival s, world, steps U{1..120})."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    return t, rng.integers(0, worlds, count), rng.integers(1, max_steps + 1, count)

@@ -9,11 +9,16 @@
 
 #include "../../include/avbd.h"
 #include "engine.h"
+#include "server.h"
 
 using namespace avbd;
 
 struct avbd_world {
   std::unique_ptr<Engine> engine;
+};
+
+struct avbd_server {
+  std::unique_ptr<Server> server;
 };
 
 namespace {
@@ -385,6 +390,135 @@ int avbd_world_device_bytes(const avbd_world *world, uint64_t *out) {
   if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
   *out = world->engine->deviceBytes();
   return AVBD_OK;
+}
+
+// --- serving loop ----------------------------------------------------------------------------------------------------
+
+#define CHECK_SERVER(s) \
+  if (!(s) || !(s)->server) return fail(AVBD_ERR_DESTROYED, "the server has been destroyed")
+
+int avbd_server_create(const AvbdServerOptions *options, avbd_server **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    *out = nullptr;
+    if (avbd_device_count() <= 0) return fail(AVBD_ERR_DEVICE, "no HIP device is available");
+    ServerOptions o;
+    o.engine.useGraph = true;
+    if (options) {
+      if (options->hbm_budget_bytes) o.budgetBytes = size_t(options->hbm_budget_bytes);
+      if (options->max_active) o.maxActive = options->max_active;
+      if (options->lookahead) o.lookahead = options->lookahead;
+      o.device = options->device;
+      if (options->color_cap) o.engine.colorCap = options->color_cap;
+      if (options->color_rounds) o.engine.colorRounds = options->color_rounds;
+      if (options->broadphase > 2) throw std::invalid_argument("broadphase must be 0 (auto), 1 (grid) or 2 (hploc)");
+      o.engine.broadphase = options->broadphase;
+    }
+    if (o.engine.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");
+    auto s = std::make_unique<avbd_server>();
+    s->server = std::make_unique<Server>(o);
+    *out = s.release();
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_destroy(avbd_server *server) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    server->server.reset();
+    delete server;
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_add_world(avbd_server *server, const AvbdSceneDesc *scene, uint32_t *world_id) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!scene || !world_id) throw std::invalid_argument("scene or world_id is NULL");
+    *world_id = server->server->addWorld(toHost(*scene));
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_world_body_count(avbd_server *server, uint32_t world_id, uint32_t *out) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!out) throw std::invalid_argument("out is NULL");
+    *out = server->server->bodyCount(world_id);
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_submit(avbd_server *server, uint32_t world_id, uint32_t steps, uint64_t *request_id) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!request_id) throw std::invalid_argument("request_id is NULL");
+    *request_id = server->server->submit(world_id, steps);
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_wait(avbd_server *server, uint64_t request_id, double *latency_ms) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    double ms = server->server->wait(request_id);
+    if (latency_ms) *latency_ms = ms;
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_poll(avbd_server *server, uint64_t request_id, int *done) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!done) throw std::invalid_argument("done is NULL");
+    *done = server->server->done(request_id) ? 1 : 0;
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_read_bodies(avbd_server *server, uint32_t world_id, float *out) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!out) throw std::invalid_argument("out is NULL");
+    std::vector<Body> b;
+    server->server->readBodies(world_id, b);
+    std::memcpy(out, b.data(), b.size() * sizeof(Body));
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_stats(avbd_server *server, AvbdServerStats *out) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    if (!out) throw std::invalid_argument("out is NULL");
+    ServerStats s = server->server->stats();
+    out->submitted = s.submitted;
+    out->completed = s.completed;
+    out->admissions = s.admissions;
+    out->evictions = s.evictions;
+    out->restores = s.restores;
+    out->steps = s.steps;
+    out->body_steps = s.bodySteps;
+    out->wall_seconds = s.wallSeconds;
+    out->step_ms_p50 = s.stepMsP50;
+    out->step_ms_p99 = s.stepMsP99;
+    out->step_ms_max = s.stepMsMax;
+    out->request_ms_p50 = s.requestMsP50;
+    out->request_ms_p99 = s.requestMsP99;
+    out->resident_bytes = s.residentBytes;
+    out->peak_resident_bytes = s.peakResidentBytes;
+    out->resident_worlds = s.residentWorlds;
+    out->worlds = s.worlds;
+    return AVBD_OK;
+  });
+}
+
+int avbd_server_reset_stats(avbd_server *server) {
+  return guarded([&] {
+    CHECK_SERVER(server);
+    server->server->resetStats();
+    return AVBD_OK;
+  });
 }
 
 }  // extern "C"

@@ -40,6 +40,17 @@ struct PhaseTimes {
   uint32_t primalLaunches, dualLaunches;
 };
 
+/** Everything the next step of a batch depends on, in host memory (a world evicted from HBM keeps this). */
+struct SavedState {
+  std::vector<Body> bodies;
+  std::vector<Joint> joints;
+  std::vector<float4> refs;
+  std::vector<uint32_t> colors;
+  std::vector<Manifold> manifolds;
+  std::vector<Contact> contacts;
+  std::vector<uint32_t> steps;
+};
+
 struct Counters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
 };
@@ -86,6 +97,11 @@ class Engine {
   /** Seed the persistent state: reference poses, colour state, last step's manifolds/contacts, step counters. */
   void writeState(const float4 *refs, const uint32_t *colors, const std::vector<Manifold> &manifolds,
                   const std::vector<Contact> &contacts, const uint32_t *steps);
+
+  /** Copy the persistent state to the host; restore() on an engine built from the same worlds continues the run
+   * exactly where it stopped. */
+  SavedState save();
+  void restore(const SavedState &s);
 
   void setColorCap(uint32_t cap);
   void setColorRounds(uint32_t rounds);

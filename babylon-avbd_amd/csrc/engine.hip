@@ -179,9 +179,10 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.colorA = dallocT<uint32_t>(nBodies_);
   w.colorB = dallocT<uint32_t>(nBodies_);
   w.colorRank = dallocT<uint32_t>(nBodies_);
-  w.colorHist = dallocT<uint32_t>(uint64_t(MAX_COLORS) * colorGroups_ + 1);
-  w.colorStart = dallocT<uint32_t>(MAX_COLORS + 1);
+  w.colorHist = dallocT<uint32_t>(uint64_t(COLOR_KEYS) * colorGroups_ + 1);
+  w.colorStart = dallocT<uint32_t>(COLOR_KEYS + 1);
   w.colorBodies = dallocT<uint32_t>(nBodies_);
+  w.colorSlot = dallocT<uint4>(nBodies_);
   w.colorGroups = colorGroups_;
   w.colorCap = colorCap_;
 
@@ -281,8 +282,9 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.hashMask = hashSize_ - 1;
   w.adjCapacity = 2 * (nJoints_ + pairCap_);
   w.adjList = static_cast<uint32_t *>(alloc(size_t(w.adjCapacity) * 4));
+  w.adjEntry = static_cast<uint4 *>(alloc(size_t(w.adjCapacity) * 2 * sizeof(uint4)));
   uint32_t scanNeed = scan_scratch_words(std::max<uint32_t>({tableSize_ + 2, nBodies_ + 1, pairCap_ + 1,
-                                                              MAX_COLORS * colorGroups_ + 1,
+                                                              COLOR_KEYS * colorGroups_ + 1,
                                                               256 * ((nBodies_ + 255) / 256) + 1}));
   w.scanScratch = static_cast<uint32_t *>(alloc(size_t(scanNeed) * 4));
   contactBytes_ = deviceBytes_ - bytesBefore;
@@ -631,6 +633,29 @@ void Engine::writeState(const float4 *refs, const uint32_t *colors, const std::v
     HIP_CHECK(hipMemcpyAsync(base_.worlds, p.data(), p.size() * sizeof(WorldParams), hipMemcpyHostToDevice, stream_));
   }
   synchronize();
+}
+
+SavedState Engine::save() {
+  SavedState s;
+  s.bodies.resize(nBodies_);
+  readBodies(0, nBodies_, s.bodies.data());
+  s.joints.resize(nJoints_);
+  readJoints(0, nJoints_, s.joints.data());
+  s.refs = readRefs();
+  s.colors = readColors();
+  readContacts(s.manifolds, s.contacts);
+  std::vector<WorldParams> p = readWorldParams();
+  for (const WorldParams &q : p) s.steps.push_back(q.step);
+  return s;
+}
+
+void Engine::restore(const SavedState &s) {
+  if (s.bodies.size() != nBodies_ || s.joints.size() != nJoints_ || s.refs.size() != 2ull * nBodies_ ||
+      s.colors.size() != nBodies_ || s.steps.size() != nWorlds_)
+    throw std::invalid_argument("saved state does not match this batch");
+  writeBodies(0, nBodies_, s.bodies.data());
+  writeJoints(0, nJoints_, s.joints.data());
+  writeState(s.refs.data(), s.colors.data(), s.manifolds, s.contacts, s.steps.data());
 }
 
 void Engine::setColorCap(uint32_t cap) {

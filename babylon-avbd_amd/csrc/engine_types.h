@@ -53,6 +53,7 @@ struct DeviceWorld {
   uint32_t *adjStart;
   uint32_t *adjFill;
   uint32_t *adjList;
+  uint4 *adjEntry;        // 2 per adjacency entry: what the colouring and the primal read of it (ENTRY_* layout)
   uint32_t adjCapacity;
 
   // Colouring: state A, state B, rank, histogram (64 × groups + 1), starts (65), bucketed bodies
@@ -62,10 +63,30 @@ struct DeviceWorld {
   uint32_t *colorHist;
   uint32_t *colorStart;
   uint32_t *colorBodies;
+  uint4 *colorSlot;       // per bucketed body: body, first and end adjacency entry, world
   uint32_t colorGroups;
   uint32_t colorCap;
 };
 
 constexpr int COLOR_WG = 256;
+
+/**
+ * Primal lanes: a body with at most LANES adjacency entries takes LANES lanes (one entry per lane), a heavier one
+ * HEAVY_LANES. Each colour is bucketed as (heavy bodies, light bodies): bucket key 2 × colour + (light ? 1 : 0), so
+ * colorStart[2c], colorStart[2c + 1], colorStart[2c + 2] bound colour c's heavy and light bodies.
+ */
+constexpr uint32_t LANES = 8;
+constexpr uint32_t HEAVY_LANES = 32;
+constexpr uint32_t COLOR_KEYS = 2 * MAX_COLORS;
+
+/**
+ * Adjacency entry record, written once per step after the per-body sort (2 × uint4 = 32 B):
+ *   [0].x partner body (ENTRY_NONE for a world anchor), [0].y manifold: first contact / joint: joint id,
+ *   [0].z flags: contact count (bits 0-3), ENTRY_IS_A, ENTRY_JOINT, ENTRY_DYNAMIC (partner is dynamic),
+ *   [0].w manifold friction (f32 bits); [1].xyz manifold normal (f32 bits), [1].w constraint id.
+ * Reading it replaces the chain adjacency id → manifold/joint record → partner body in the colouring and the primal.
+ */
+constexpr uint32_t ENTRY_NONE = 0xffffffffu;
+constexpr uint32_t ENTRY_IS_A = 1u << 4, ENTRY_JOINT = 1u << 5, ENTRY_DYNAMIC = 1u << 6;
 
 }  // namespace avbd

@@ -218,12 +218,17 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
   return e;
 }
 
-/** Lanes per body in the primal: lane k folds adjacency entries k, k + LANES, ... (lane 0 also the inertia terms);
- * the partial systems are then summed by a fixed tree (k += k + 4, k += k + 2, k += k + 1), so the result does not
- * depend on scheduling. */
-constexpr uint32_t LANES = 8;
-
-AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t lane, float alpha, float dt, Acc &acc) {
+/**
+ * Lane `lane` of body i's system (accumulate, wgsl-solve.ts:351-413): inertia on lane 0, then adjacency entries
+ * lane, lane + W, ... in the body's sorted order (W lanes per body: LANES, or HEAVY_LANES for a body with more than
+ * LANES entries). The partial systems are then summed by a fixed tree (k += k + W/2, ..., k += k + 1), so the result
+ * does not depend on scheduling. Entries come as records (partner, contact range, normal,
+ * friction), so a contact entry needs one load for the record and then the partner body and the contact points
+ * together; the body's own pose is read once.
+ */
+template <uint32_t W>
+AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
+                       float dt, Acc &acc) {
   const Body &B = w.bodies[i];
   acc.l00 = acc.l10 = acc.l11 = acc.l20 = acc.l21 = acc.l22 = 0.f;
   acc.a00 = acc.a10 = acc.a11 = acc.a20 = acc.a21 = acc.a22 = 0.f;
@@ -240,41 +245,49 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t lane, float al
     acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
     acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
   }
-  const uint32_t J = w.nJoints;
-  const uint32_t end = w.adjStart[i + 1];
-  for (uint32_t e = w.adjStart[i] + lane; e < end; e += LANES) {
-    uint32_t id = w.adjList[e];
-    if (id < J) {
-      addJoint(w, acc, id, alpha, i);
-      continue;
-    }
-    Manifold mf = w.manifolds[id - J];
-    bool isA = i == mf.ids.x;
-    PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
-    Basis basis = orthonormal(xyz(mf.geo));
-    float sg = isA ? 1.f : -1.f;
-    f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
-    uint32_t cEnd = mf.ids.z + (mf.ids.w & 15u);
-    for (uint32_t c = mf.ids.z; c < cEnd; c++) {
-      Contact k = w.contacts[c];
-      Eval ev = evalContact(k, basis, mf.geo.w, PA, PB, alpha);
-      f3 r = isA ? ev.rAW : ev.rBW;
-      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+  uint32_t e = first + lane;
+  if (e >= end) return;
+  const PairBody self = pairBody(w, i);
+  for (; e < end; e += W) {
+    const uint4 head = w.adjEntry[2ull * e], tail = w.adjEntry[2ull * e + 1];
+    if (head.z & ENTRY_JOINT) {
+      addJoint(w, acc, head.y, alpha, i);
+    } else {
+      const bool isA = (head.z & ENTRY_IS_A) != 0;
+      const PairBody other = pairBody(w, head.x);
+      const PairBody PA = isA ? self : other;
+      const PairBody PB = isA ? other : self;
+      Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
+      const float friction = __uint_as_float(head.w);
+      float sg = isA ? 1.f : -1.f;
+      f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
+      uint32_t cEnd = head.y + (head.z & 15u);
+      for (uint32_t c = head.y; c < cEnd; c++) {
+        Contact k = w.contacts[c];
+        Eval ev = evalContact(k, basis, friction, PA, PB, alpha);
+        f3 r = isA ? ev.rAW : ev.rBW;
+        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+      }
     }
   }
 }
 
-/** acc += the partial system of the lane `s` above (within the body's group of LANES lanes). */
-AVBD_D void mergeFromAbove(Acc &acc, uint32_t lane, uint32_t s) {
+/** acc += the partial system of the lane `s` above, within groups of W lanes. */
+template <uint32_t W> AVBD_D void mergeFromAbove(Acc &acc, uint32_t lane, uint32_t s) {
   float *a = reinterpret_cast<float *>(&acc);
   constexpr int N = sizeof(Acc) / sizeof(float);
 #pragma unroll
   for (int q = 0; q < N; q++) {
-    float o = __shfl_down(a[q], s, LANES);
+    float o = __shfl_down(a[q], s, W);
     if (lane < s) a[q] = a[q] + o;
   }
+}
+
+template <uint32_t W> AVBD_D void mergeTree(Acc &acc, uint32_t lane) {
+#pragma unroll
+  for (uint32_t s = W / 2; s > 0; s >>= 1) mergeFromAbove<W>(acc, lane, s);
 }
 
 /**
@@ -363,30 +376,47 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;
+constexpr uint32_t WAVE = 64;
 
-__global__ void __launch_bounds__(PRIMAL_BLOCK) k_primal(DeviceWorld w, uint32_t color) {
-  if (held(w)) return;
-  const uint32_t start = w.colorStart[color];
-  const uint32_t count = w.colorStart[color + 1] - start;
-  const uint32_t lane = threadIdx.x % LANES;
-  const uint32_t perBlock = PRIMAL_BLOCK / LANES;
-  // Every lane of a block runs the same number of loop turns (the shuffles below need whole groups)
-  for (uint32_t base = blockIdx.x * perBlock; base < count; base += gridDim.x * perBlock) {
-    const uint32_t k = base + threadIdx.x / LANES;
-    const bool live = k < count;
-    uint32_t i = live ? w.colorBodies[start + k] : 0u;
-    Acc acc;
-    if (live) {
-      const WorldParams &p = w.worlds[w.bodyWorld[i]];
-      accumulate(w, i, lane, p.alpha, p.dt, acc);
-    } else {
-      float *a = reinterpret_cast<float *>(&acc);
-      for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
-    }
-    mergeFromAbove(acc, lane, 4);
-    mergeFromAbove(acc, lane, 2);
-    mergeFromAbove(acc, lane, 1);
-    if (live && lane == 0) finishBody(w, i, acc);
+/** Bucket position k (of a colour half ending at `end`) solved by a group of W lanes; `lane` is the lane in the group. */
+template <uint32_t W> AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t lane) {
+  const bool live = k < end;
+  const uint4 slot = live ? w.colorSlot[k] : make_uint4(0u, 0u, 0u, 0u);
+  Acc acc;
+  if (live) {
+    const WorldParams &p = w.worlds[slot.w];
+    accumulate<W>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, acc);
+  } else {
+    float *a = reinterpret_cast<float *>(&acc);
+    for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
+  }
+  mergeTree<W>(acc, lane);
+  if (live && lane == 0) finishBody(w, slot.x, acc);
+}
+
+/**
+ * One colour's primal (primal, wgsl-solve.ts:314-345). Work goes out by whole waves: first the colour's heavy bodies,
+ * two to a wave (HEAVY_LANES lanes each), then its light bodies, eight to a wave (LANES lanes each), so a body with many
+ * contacts does not hold back the seven bodies sharing its wave.
+ */
+__global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint32_t color) {
+  // The hold flag and the colour's ranges are read together (neither waits for the other)
+  const uint32_t overflow = w.counters[C_OVERFLOW];
+  const uint32_t heavyStart = w.colorStart[2 * color];
+  const uint32_t lightStart = w.colorStart[2 * color + 1];
+  const uint32_t lightEnd = w.colorStart[2 * color + 2];
+  if (overflow != 0u) return;
+  constexpr uint32_t heavyPerWave = WAVE / HEAVY_LANES, lightPerWave = WAVE / LANES;
+  const uint32_t heavyWaves = (lightStart - heavyStart + heavyPerWave - 1) / heavyPerWave;
+  const uint32_t waves = heavyWaves + (lightEnd - lightStart + lightPerWave - 1) / lightPerWave;
+  const uint32_t lane = threadIdx.x % WAVE;
+  constexpr uint32_t wavesPerBlock = PRIMAL_BLOCK / WAVE;
+  // u is uniform across a wave, so every lane of a group takes the same turns (the shuffles need whole groups)
+  for (uint32_t u = blockIdx.x * wavesPerBlock + threadIdx.x / WAVE; u < waves; u += gridDim.x * wavesPerBlock) {
+    if (u < heavyWaves)
+      primalGroup<HEAVY_LANES>(w, heavyStart + u * heavyPerWave + lane / HEAVY_LANES, lightStart, lane % HEAVY_LANES);
+    else
+      primalGroup<LANES>(w, lightStart + (u - heavyWaves) * lightPerWave + lane / LANES, lightEnd, lane % LANES);
   }
 }
 

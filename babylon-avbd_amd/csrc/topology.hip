@@ -59,6 +59,7 @@ AVBD_D uint64_t adjKey(const DeviceWorld &w, uint32_t b, uint32_t id) {
   return uint64_t(uint32_t(other + 2)) << 32;
 }
 
+/** Sort body b's list into key order, then write the entry records the colouring and the primal read. */
 __global__ void k_adj_sort(DeviceWorld w) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= w.nBodies) return;
@@ -76,29 +77,50 @@ __global__ void k_adj_sort(DeviceWorld w) {
     }
     w.adjList[j] = id;
   }
+  for (uint32_t e = lo; e < hi; e++) {
+    uint32_t id = w.adjList[e];
+    uint4 head, tail;
+    if (id < w.nJoints) {
+      int4 f = w.info[id];
+      int other = f.y == int(b) ? f.z : f.y;
+      head = make_uint4(other < 0 ? ENTRY_NONE : uint32_t(other), id,
+                        ENTRY_JOINT | (dynamicBody(w, other) ? ENTRY_DYNAMIC : 0u), 0u);
+      tail = make_uint4(0u, 0u, 0u, id);
+    } else {
+      const Manifold mf = w.manifolds[id - w.nJoints];
+      bool isA = mf.ids.x == b;
+      uint32_t other = isA ? mf.ids.y : mf.ids.x;
+      head = make_uint4(other, mf.ids.z,
+                        (mf.ids.w & 15u) | (isA ? ENTRY_IS_A : 0u) | (dynamicBody(w, int(other)) ? ENTRY_DYNAMIC : 0u),
+                        __float_as_uint(mf.geo.w));
+      tail = make_uint4(__float_as_uint(mf.geo.x), __float_as_uint(mf.geo.y), __float_as_uint(mf.geo.z), id);
+    }
+    w.adjEntry[2ull * e] = head;
+    w.adjEntry[2ull * e + 1] = tail;
+  }
 }
 
 // --- colouring ---------------------------------------------------------------------------------------------------------
 
-AVBD_D uint32_t priorityOf(const DeviceWorld &w, uint32_t i) {
-  return hash32(i - w.worlds[w.bodyWorld[i]].firstBody + 0x9e3779b9u);
-}
-/** i beats j: higher priority, ties by index (within a world, local and global index order agree). */
-AVBD_D bool beats(const DeviceWorld &w, uint32_t i, uint32_t j) {
-  uint32_t pi = priorityOf(w, i), pj = priorityOf(w, j);
+/** Colouring priority of body i of the world whose first body is `first` (its index within its world, hashed). */
+AVBD_D uint32_t priorityOf(uint32_t i, uint32_t first) { return hash32(i - first + 0x9e3779b9u); }
+/** i beats j (same world): higher priority, ties by index (within a world, local and global index order agree). */
+AVBD_D bool beats(uint32_t i, uint32_t j, uint32_t first) {
+  uint32_t pi = priorityOf(i, first), pj = priorityOf(j, first);
   return pi > pj || (pi == pj && i > j);
 }
+AVBD_D uint32_t firstOf(const DeviceWorld &w, uint32_t b) { return w.worlds[w.bodyWorld[b]].firstBody; }
 
-AVBD_D int neighbour(const DeviceWorld &w, uint32_t b, uint32_t e) {
-  int2 ends = endpoints(w, w.adjList[e]);
-  int other = ends.x == int(b) ? ends.y : ends.x;
-  return dynamicBody(w, other) ? other : -1;
+/** The dynamic body at the other end of adjacency entry e, or -1. */
+AVBD_D int neighbour(const DeviceWorld &w, uint32_t e) {
+  uint4 head = w.adjEntry[2ull * e];
+  return (head.z & ENTRY_DYNAMIC) ? int(head.x) : -1;
 }
 
 AVBD_D uint32_t smallestFree(const DeviceWorld &w, uint32_t b, const uint32_t *src, bool skipPending) {
   uint32_t lo = 0, hi = 0;
   for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1]; e++) {
-    int j = neighbour(w, b, e);
+    int j = neighbour(w, e);
     if (j < 0) continue;
     uint32_t s = src[j];
     if (skipPending && (s & PENDING)) continue;
@@ -125,9 +147,10 @@ __global__ void k_color_compact(DeviceWorld w) {
   if (col >= w.colorCap) col = NO_COLOR;
   if (col != NO_COLOR) {
     bool isMax = true;
+    const uint32_t first = firstOf(w, b);
     for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
-      int j = neighbour(w, b, e);
-      if (j >= 0 && !beats(w, b, uint32_t(j))) isMax = false;
+      int j = neighbour(w, e);
+      if (j >= 0 && !beats(b, uint32_t(j), first)) isMax = false;
     }
     if (isMax) col = umin(col, smallestFree(w, b, w.colorA, false));
   }
@@ -144,9 +167,10 @@ __global__ void k_color_mark(DeviceWorld w) {
     return;
   }
   bool pending = col == NO_COLOR;
+  const uint32_t first = firstOf(w, b);
   for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && !pending; e++) {
-    int j = neighbour(w, b, e);
-    if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(w, uint32_t(j), b)) pending = true;
+    int j = neighbour(w, e);
+    if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(uint32_t(j), b, first)) pending = true;
   }
   w.colorA[b] = col | (pending ? PENDING : 0u);
 }
@@ -162,9 +186,10 @@ __global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst)
     return;
   }
   bool isMax = true;
+  const uint32_t first = firstOf(w, b);
   for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
-    int j = neighbour(w, b, e);
-    if (j >= 0 && (src[j] & PENDING) && !beats(w, b, uint32_t(j))) isMax = false;
+    int j = neighbour(w, e);
+    if (j >= 0 && (src[j] & PENDING) && !beats(b, uint32_t(j), first)) isMax = false;
   }
   if (isMax) {
     uint32_t fr = smallestFree(w, b, src, true);
@@ -175,48 +200,55 @@ __global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst)
 }
 
 // 4. Per-chunk histogram and each body's rank within its chunk's colour (keeps index order inside a bucket)
+/** Bucket key of a coloured body: its colour's heavy or light half (engine_types.h, COLOR_KEYS). */
+AVBD_D uint32_t bucketKey(const DeviceWorld &w, uint32_t b, uint32_t col) {
+  return 2u * col + (w.adjStart[b + 1] - w.adjStart[b] > LANES ? 0u : 1u);
+}
+
 __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
-  __shared__ uint32_t hist[MAX_COLORS];
-  __shared__ uint32_t chunkColors[COLOR_WG];
+  __shared__ uint32_t hist[COLOR_KEYS];
+  __shared__ uint32_t chunkKeys[COLOR_WG];
   uint32_t lid = threadIdx.x;
-  if (lid < MAX_COLORS) hist[lid] = 0;
+  if (lid < COLOR_KEYS) hist[lid] = 0;
   uint32_t b = blockIdx.x * COLOR_WG + lid;
-  uint32_t col = NO_COLOR;
+  uint32_t key = ~0u;
   if (b < w.nBodies && dynamicBody(w, int(b))) {
     uint32_t s = w.colorA[b];
-    col = s & 0xffu;
+    uint32_t col = s & 0xffu;
     if (s & PENDING) atomicAdd(&w.counters[C_CLASHES], 1u);
     if (col == NO_COLOR) col = w.colorCap - 1;
     w.colorA[b] = col;
     atomicMax(&w.counters[C_NUM_COLORS], col + 1);
+    key = bucketKey(w, b, col);
   }
-  chunkColors[lid] = col;
+  chunkKeys[lid] = key;
   __syncthreads();
-  if (col != NO_COLOR) {
+  if (key != ~0u) {
     uint32_t rank = 0;
-    for (uint32_t j = 0; j < lid; j++) rank += chunkColors[j] == col ? 1u : 0u;
+    for (uint32_t j = 0; j < lid; j++) rank += chunkKeys[j] == key ? 1u : 0u;
     w.colorRank[b] = rank;
-    atomicAdd(&hist[col], 1u);
+    atomicAdd(&hist[key], 1u);
   }
   __syncthreads();
-  if (lid < MAX_COLORS) w.colorHist[lid * w.colorGroups + blockIdx.x] = hist[lid];
-  if (blockIdx.x == 0 && lid == 0) w.colorHist[MAX_COLORS * w.colorGroups] = 0;
+  if (lid < COLOR_KEYS) w.colorHist[lid * w.colorGroups + blockIdx.x] = hist[lid];
+  if (blockIdx.x == 0 && lid == 0) w.colorHist[COLOR_KEYS * w.colorGroups] = 0;
 }
 
-// 5. Colour c starts at its first chunk's offset
+// 5. Bucket k starts at its first chunk's offset
 __global__ void k_color_starts(DeviceWorld w) {
-  uint32_t c = threadIdx.x;
-  if (c < MAX_COLORS) w.colorStart[c] = w.colorHist[c * w.colorGroups];
-  if (c == 0) w.colorStart[MAX_COLORS] = w.colorHist[MAX_COLORS * w.colorGroups];
+  uint32_t k = threadIdx.x;
+  if (k < COLOR_KEYS) w.colorStart[k] = w.colorHist[k * w.colorGroups];
+  if (k == 0) w.colorStart[COLOR_KEYS] = w.colorHist[COLOR_KEYS * w.colorGroups];
 }
 
-// 6. Scatter: chunk offset for the body's colour plus its rank
+// 6. Scatter: chunk offset for the body's bucket plus its rank
 __global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
   uint32_t b = blockIdx.x * COLOR_WG + threadIdx.x;
   if (b >= w.nBodies || !dynamicBody(w, int(b))) return;
   uint32_t col = w.colorA[b] & 0xffu;
-  uint32_t slot = w.colorHist[col * w.colorGroups + blockIdx.x] + w.colorRank[b];
+  uint32_t slot = w.colorHist[bucketKey(w, b, col) * w.colorGroups + blockIdx.x] + w.colorRank[b];
   w.colorBodies[slot] = b;
+  w.colorSlot[slot] = make_uint4(b, w.adjStart[b], umin(w.adjStart[b + 1], w.adjCapacity), w.bodyWorld[b]);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -247,8 +279,8 @@ void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s) {
     else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA);
   }
   k_color_count<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
-  launch_scan_exclusive(w.colorHist, MAX_COLORS * w.colorGroups + 1, w.scanScratch, s);
-  k_color_starts<<<1, 64, 0, s>>>(w);
+  launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s);
+  k_color_starts<<<1, COLOR_KEYS, 0, s>>>(w);
   k_color_scatter<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
 }
 

@@ -154,6 +154,43 @@ def jointed_mix(avbd, steps, warmup):
             "step_ms_p99": float(np.percentile(ms, 99)), "counters": counters, "phases_ms": prof}
 
 
+def serving(avbd, requests, rate, pool, max_active):
+    """The serving workload: Poisson arrivals (`rate` per second) of rollout requests (U{1..120} steps) for a pool of
+    worlds of 16..50,000 boxes (log-uniform sizes, the single large world's body distributions), admitted by the
+    serving loop against a 200 GB HBM budget and run concurrently (one stream per world)."""
+    sizes = avbd.configs.serving_pool(pool)
+    server = avbd.serve.Server(hbmBudget=200 * 10**9, maxActive=max_active)
+    ids = [server.addWorld(avbd.configs.boxes_world(int(n), avbd.configs.SEED + 100 + k)) for k, n in enumerate(sizes)]
+    warm = [server.submit(w, 1) for w in ids]  # admission and graph capture of every world
+    for r in warm:
+        server.wait(r)
+    print(f"# serving: {pool} worlds ({int(sizes.sum())} bodies) resident", file=sys.stderr, flush=True)
+    server.resetStats()
+    t, world, steps = avbd.configs.serving_requests(requests, rate, pool)
+    t0 = time.perf_counter()
+    rids = []
+    for k in range(requests):
+        delay = t0 + t[k] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        rids.append(server.submit(ids[int(world[k])], int(steps[k])))
+    print(f"# serving: {requests} requests submitted in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    for r in rids:
+        server.wait(r)
+    st = server.stats()
+    server.destroy()
+    return {"workload": f"Poisson arrivals at {rate:g} req/s of rollouts U{{1..120}} steps for {pool} worlds of "
+                        "16..50,000 boxes (log-uniform), 200 GB HBM budget, 10 iterations",
+            "requests": requests, "worlds": pool, "pool_bodies": int(sizes.sum()),
+            "offered_body_steps_per_s": float(rate * np.mean(steps) * sizes[world].mean()),
+            "body_steps_per_s": st["body_steps"] / max(st["wall_seconds"], 1e-9),
+            "requests_per_s": st["completed"] / max(st["wall_seconds"], 1e-9), "wall_s": st["wall_seconds"],
+            "step_ms_p50": st["step_ms_p50"], "step_ms_p99": st["step_ms_p99"], "step_ms_max": st["step_ms_max"],
+            "request_ms_p50": st["request_ms_p50"], "request_ms_p99": st["request_ms_p99"],
+            "evictions": st["evictions"], "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
+            "max_active": max_active}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +204,11 @@ def main():
     ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-pool", type=int, default=64)
+    ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--mix-steps", type=int, default=200)
     ap.add_argument("--mix-warmup", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=280)
@@ -231,6 +273,8 @@ def main():
 
     world.destroy()
     mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
+    served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
+                                                  args.serve_active)
     batched = None
     if not args.no_batched:
         batched = batched_worlds(avbd, args.batch_steps, args.batch_warmup, args.batch_sample)
@@ -268,6 +312,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "jointed_mix": mix,
+        "serving": served,
     }
     print(json.dumps(line))
 

@@ -174,6 +174,48 @@ int avbd_world_profile_step(avbd_world *world, float *out);
 /** Device bytes held by the world. */
 int avbd_world_device_bytes(const avbd_world *world, uint64_t *out);
 
+/* ---- Serving loop ------------------------------------------------------------------------------------------------
+ * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):
+ * a server takes step/rollout requests for many registered worlds as they arrive, keeps world state resident in HBM
+ * under a byte budget (least recently used idle worlds are copied to host memory and restored exactly when next
+ * requested), and runs admitted requests concurrently, one HIP stream per world, launching the next step of every
+ * running request each turn (continuous batching at step granularity). Requests of one world run in submission order,
+ * so a world's result does not depend on what else is served. Served worlds do not adapt their colour cap. */
+
+typedef struct avbd_server avbd_server;
+
+typedef struct AvbdServerOptions {
+  uint64_t hbm_budget_bytes; /* 0: 200 GiB */
+  uint32_t max_active;       /* requests running at once (0: 32) */
+  uint32_t lookahead;        /* steps in flight per running request (0: 2) */
+  int32_t device;
+  uint32_t color_cap, color_rounds, use_graph, broadphase; /* per world, as in AvbdOptions (0: defaults, graph on) */
+} AvbdServerOptions;
+
+typedef struct AvbdServerStats {
+  uint64_t submitted, completed, admissions, evictions, restores, steps, body_steps;
+  double wall_seconds;                    /* first submission (since reset) to last completion */
+  double step_ms_p50, step_ms_p99, step_ms_max; /* device time per step (HIP events) */
+  double request_ms_p50, request_ms_p99;  /* submission to completion */
+  uint64_t resident_bytes, peak_resident_bytes;
+  uint32_t resident_worlds, worlds;
+} AvbdServerStats;
+
+int avbd_server_create(const AvbdServerOptions *options, avbd_server **out);
+int avbd_server_destroy(avbd_server *server);
+/** Register a world (held on the host until a request needs it); *world_id is its handle. */
+int avbd_server_add_world(avbd_server *server, const AvbdSceneDesc *scene, uint32_t *world_id);
+int avbd_server_world_body_count(avbd_server *server, uint32_t world_id, uint32_t *out);
+/** Queue `steps` steps of a world (a step request is a rollout of length 1). */
+int avbd_server_submit(avbd_server *server, uint32_t world_id, uint32_t steps, uint64_t *request_id);
+/** Wait for a request; latency_ms (may be NULL) = submission to completion. */
+int avbd_server_wait(avbd_server *server, uint64_t request_id, double *latency_ms);
+int avbd_server_poll(avbd_server *server, uint64_t request_id, int *done);
+/** The world's bodies (40 f32 each) after its queued requests have finished. */
+int avbd_server_read_bodies(avbd_server *server, uint32_t world_id, float *out);
+int avbd_server_stats(avbd_server *server, AvbdServerStats *out);
+int avbd_server_reset_stats(avbd_server *server);
+
 #ifdef __cplusplus
 }
 #endif

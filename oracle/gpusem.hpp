@@ -767,9 +767,10 @@ template <class R> class World {
     }
   }
 
-  /** Lanes per body: lane k folds adjacency entries k, k + LANES, ... (lane 0 also the inertia terms), then the
-   * partial systems are summed by the fixed tree k += k + 4, k += k + 2, k += k + 1 (the HIP primal's order). */
-  static constexpr uint32_t LANES = 8;
+  /** Lanes per body: W = LANES for a body with at most LANES adjacency entries, else HEAVY_LANES. Lane k folds
+   * entries k, k + W, ... (lane 0 also the inertia terms), then the partial systems are summed by the fixed tree
+   * k += k + W/2, ..., k += k + 1 (the HIP primal's order). */
+  static constexpr uint32_t LANES = 8, HEAVY_LANES = 32;
 
   void addEntry(Acc &acc, uint32_t i, uint32_t id, R alpha) const {
     const uint32_t J = uint32_t(joints.size());
@@ -810,14 +811,15 @@ template <class R> class World {
     R dt2 = dt * dt;
     R m = b.mass / dt2;
     V3<R> I = divs(b.moment, dt2);
-    Acc lanes[LANES];
+    const uint32_t W = adjStart[i + 1] - adjStart[i] > LANES ? HEAVY_LANES : LANES;
+    Acc lanes[HEAVY_LANES];
     lanes[0].lin = diag(m, m, m);
     lanes[0].ang = diag(I.x, I.y, I.z);
     lanes[0].rLin = scale(sub(b.pos, b.inertialPos), m);
     lanes[0].rAng = mulv3(I, qsub(b.rot, b.inertialRot));
-    for (uint32_t lane = 0; lane < LANES; lane++)
-      for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += LANES) addEntry(lanes[lane], i, adjList[e], alpha);
-    for (uint32_t s = LANES / 2; s > 0; s >>= 1)
+    for (uint32_t lane = 0; lane < W; lane++)
+      for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += W) addEntry(lanes[lane], i, adjList[e], alpha);
+    for (uint32_t s = W / 2; s > 0; s >>= 1)
       for (uint32_t lane = 0; lane < s; lane++) mergeInto(lanes[lane], lanes[lane + s]);
     return lanes[0];
   }

@@ -54,6 +54,12 @@ def test_world_creation_without_a_gpu_fails_loudly(avbd):
         avbd.createSolverPipeline("hip").createSolver(scene)
 
 
+@pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
+def test_server_creation_without_a_gpu_fails_loudly(avbd):
+    with pytest.raises(avbd.AvbdError, match="no HIP device"):
+        avbd.serve.Server()
+
+
 def test_scene_validation(avbd):
     s = avbd.AvbdScene3D()
     with pytest.raises(ValueError):
