@@ -218,13 +218,35 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
   return e;
 }
 
+/** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
+AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
+                    uint32_t p, float alpha) {
+  if (head.z & ENTRY_JOINT) {
+    addJoint(w, acc, head.y, alpha, i);
+    return;
+  }
+  const bool isA = (head.z & ENTRY_IS_A) != 0;
+  const PairBody other = pairBody(w, head.x);
+  const PairBody PA = isA ? self : other;
+  const PairBody PB = isA ? other : self;
+  Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
+  float sg = isA ? 1.f : -1.f;
+  f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
+  Contact k = w.contacts[head.y + p];
+  Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha);
+  f3 r = isA ? ev.rAW : ev.rBW;
+  addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+  addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+  addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+}
+
 /**
- * Lane `lane` of body i's system (accumulate, wgsl-solve.ts:351-413): inertia on lane 0, then adjacency entries
- * lane, lane + W, ... in the body's sorted order (W lanes per body: LANES, or HEAVY_LANES for a body with more than
- * LANES entries). The partial systems are then summed by a fixed tree (k += k + W/2, ..., k += k + 1), so the result
- * does not depend on scheduling. Entries come as records (partner, contact range, normal,
- * friction), so a contact entry needs one load for the record and then the partner body and the contact points
- * together; the body's own pose is read once.
+ * Lane `lane` of body i's system (accumulate, wgsl-solve.ts:351-413), W lanes per body (LANES, or HEAVY_LANES for a
+ * body with more than LANES adjacency entries). Lane 0 holds the inertia terms. The body's entries are taken in chunks
+ * of W, one entry record per lane; the chunk's work items (a joint, or one contact point) are numbered in entry order
+ * by a prefix sum over the group and item t goes to lane t mod W, which finds its entry by a binary search over the
+ * group's prefix sums and takes the record from the lane that loaded it. A manifold with many points is thus spread
+ * over several lanes. The partial systems are then summed by a fixed tree, so the result does not depend on scheduling.
  */
 template <uint32_t W>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
@@ -245,31 +267,37 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
     acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
     acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
   }
-  uint32_t e = first + lane;
-  if (e >= end) return;
+  if (first >= end) return;  // the same for every lane of the group
   const PairBody self = pairBody(w, i);
-  for (; e < end; e += W) {
-    const uint4 head = w.adjEntry[2ull * e], tail = w.adjEntry[2ull * e + 1];
-    if (head.z & ENTRY_JOINT) {
-      addJoint(w, acc, head.y, alpha, i);
-    } else {
-      const bool isA = (head.z & ENTRY_IS_A) != 0;
-      const PairBody other = pairBody(w, head.x);
-      const PairBody PA = isA ? self : other;
-      const PairBody PB = isA ? other : self;
-      Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
-      const float friction = __uint_as_float(head.w);
-      float sg = isA ? 1.f : -1.f;
-      f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
-      uint32_t cEnd = head.y + (head.z & 15u);
-      for (uint32_t c = head.y; c < cEnd; c++) {
-        Contact k = w.contacts[c];
-        Eval ev = evalContact(k, basis, friction, PA, PB, alpha);
-        f3 r = isA ? ev.rAW : ev.rBW;
-        addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-        addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-        addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+  for (uint32_t c0 = first; c0 < end; c0 += W) {
+    const uint32_t e = c0 + lane;
+    uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
+    if (e < end) {
+      head = w.adjEntry[2ull * e];
+      tail = w.adjEntry[2ull * e + 1];
+    }
+    const uint32_t items = e < end ? ((head.z & ENTRY_JOINT) ? 1u : (head.z & 15u)) : 0u;
+    uint32_t incl = items;
+#pragma unroll
+    for (uint32_t s = 1; s < W; s <<= 1) {
+      uint32_t o = __shfl_up(incl, s, W);
+      if (lane >= s) incl += o;
+    }
+    const uint32_t excl = incl - items;
+    const uint32_t total = __shfl(incl, W - 1, W);
+    for (uint32_t r0 = 0; r0 < total; r0 += W) {  // the same turns for every lane of the group
+      const uint32_t t = r0 + lane;
+      uint32_t owner = 0;  // the last lane whose first item is at or before t (prefix sums never decrease)
+#pragma unroll
+      for (uint32_t step = W / 2; step > 0; step >>= 1) {
+        const uint32_t ex = __shfl(excl, owner + step, W);
+        if (ex <= t) owner += step;
       }
+      const uint4 h = make_uint4(__shfl(head.x, owner, W), __shfl(head.y, owner, W), __shfl(head.z, owner, W),
+                                 __shfl(head.w, owner, W));
+      const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W), 0u);
+      const uint32_t p = t - __shfl(excl, owner, W);
+      if (t < total) addItem(w, acc, i, self, h, tl, p, alpha);
     }
   }
 }

@@ -767,12 +767,20 @@ template <class R> class World {
     }
   }
 
-  /** Lanes per body: W = LANES for a body with at most LANES adjacency entries, else HEAVY_LANES. Lane k folds
-   * entries k, k + W, ... (lane 0 also the inertia terms), then the partial systems are summed by the fixed tree
+  /** Lanes per body: W = LANES for a body with at most LANES adjacency entries, else HEAVY_LANES. Entries are taken
+   * in chunks of W; a chunk's work items (a joint, or one contact point) are numbered in entry order and item t goes to
+   * lane t mod W (lane 0 also holds the inertia terms); the partial systems are then summed by the fixed tree
    * k += k + W/2, ..., k += k + 1 (the HIP primal's order). */
   static constexpr uint32_t LANES = 8, HEAVY_LANES = 32;
 
-  void addEntry(Acc &acc, uint32_t i, uint32_t id, R alpha) const {
+  /** Work items of adjacency entry `id`: a joint is one item, a manifold one item per contact point. */
+  uint32_t itemCount(uint32_t id) const {
+    const uint32_t J = uint32_t(joints.size());
+    return id < J ? 1u : manifolds[id - J].count();
+  }
+
+  /** Item `p` of entry `id` of body i: the joint's rows, or the three rows of contact point p of the manifold. */
+  void addItem(Acc &acc, uint32_t i, uint32_t id, uint32_t p, R alpha) const {
     const uint32_t J = uint32_t(joints.size());
     if (id < J) {
       addJoint(acc, joints[id], alpha, i);
@@ -784,14 +792,12 @@ template <class R> class World {
     M3<R> basis = orthonormal(mf.normal);
     R sg = isA ? R(1) : R(-1);
     V3<R> l0 = scale(row(basis, 0), sg), l1 = scale(row(basis, 1), sg), l2 = scale(row(basis, 2), sg);
-    for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
-      const Contact<R> &k = contacts[c];
-      Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
-      V3<R> r = isA ? ev.rAW : ev.rBW;
-      addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-      addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-      addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
-    }
+    const Contact<R> &k = contacts[mf.first + p];
+    Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
+    V3<R> r = isA ? ev.rAW : ev.rBW;
+    addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
+    addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
+    addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
   }
 
   static void mergeInto(Acc &a, const Acc &o) {
@@ -817,8 +823,12 @@ template <class R> class World {
     lanes[0].ang = diag(I.x, I.y, I.z);
     lanes[0].rLin = scale(sub(b.pos, b.inertialPos), m);
     lanes[0].rAng = mulv3(I, qsub(b.rot, b.inertialRot));
-    for (uint32_t lane = 0; lane < W; lane++)
-      for (uint32_t e = adjStart[i] + lane; e < adjStart[i + 1]; e += W) addEntry(lanes[lane], i, adjList[e], alpha);
+    // Entries in chunks of W; the chunk's work items (joints, contact points, in entry order) go to lanes t mod W
+    for (uint32_t c0 = adjStart[i]; c0 < adjStart[i + 1]; c0 += W) {
+      uint32_t t = 0;
+      for (uint32_t e = c0; e < std::min(c0 + W, adjStart[i + 1]); e++)
+        for (uint32_t p = 0; p < itemCount(adjList[e]); p++, t++) addItem(lanes[t % W], i, adjList[e], p, alpha);
+    }
     for (uint32_t s = W / 2; s > 0; s >>= 1)
       for (uint32_t lane = 0; lane < s; lane++) mergeInto(lanes[lane], lanes[lane + s]);
     return lanes[0];
