@@ -100,31 +100,32 @@ struct Acc {
   f3 rLin, rAng;
 };
 
-/** addRow (wgsl-solve.ts:53-59): lin += l lᵀ k, ang += a aᵀ k, cross += a lᵀ k, rhs += (l, a) f. */
+/** addRow (wgsl-solve.ts:53-59): lin += (l lᵀ) k, ang += (a aᵀ) k, cross += (a lᵀ) k, rhs += (l, a) f, each
+ * accumulation one fused multiply-add (the CPU restatement uses std::fma at the same places). */
 AVBD_D void addRow(Acc &s, f3 l, f3 a, float k, float f) {
-  s.l00 += l.x * l.x * k;
-  s.l10 += l.y * l.x * k;
-  s.l11 += l.y * l.y * k;
-  s.l20 += l.z * l.x * k;
-  s.l21 += l.z * l.y * k;
-  s.l22 += l.z * l.z * k;
-  s.a00 += a.x * a.x * k;
-  s.a10 += a.y * a.x * k;
-  s.a11 += a.y * a.y * k;
-  s.a20 += a.z * a.x * k;
-  s.a21 += a.z * a.y * k;
-  s.a22 += a.z * a.z * k;
-  s.c[0] += a.x * l.x * k;
-  s.c[1] += a.x * l.y * k;
-  s.c[2] += a.x * l.z * k;
-  s.c[3] += a.y * l.x * k;
-  s.c[4] += a.y * l.y * k;
-  s.c[5] += a.y * l.z * k;
-  s.c[6] += a.z * l.x * k;
-  s.c[7] += a.z * l.y * k;
-  s.c[8] += a.z * l.z * k;
-  s.rLin = add(s.rLin, scale(l, f));
-  s.rAng = add(s.rAng, scale(a, f));
+  s.l00 = fmaf(l.x * l.x, k, s.l00);
+  s.l10 = fmaf(l.y * l.x, k, s.l10);
+  s.l11 = fmaf(l.y * l.y, k, s.l11);
+  s.l20 = fmaf(l.z * l.x, k, s.l20);
+  s.l21 = fmaf(l.z * l.y, k, s.l21);
+  s.l22 = fmaf(l.z * l.z, k, s.l22);
+  s.a00 = fmaf(a.x * a.x, k, s.a00);
+  s.a10 = fmaf(a.y * a.x, k, s.a10);
+  s.a11 = fmaf(a.y * a.y, k, s.a11);
+  s.a20 = fmaf(a.z * a.x, k, s.a20);
+  s.a21 = fmaf(a.z * a.y, k, s.a21);
+  s.a22 = fmaf(a.z * a.z, k, s.a22);
+  s.c[0] = fmaf(a.x * l.x, k, s.c[0]);
+  s.c[1] = fmaf(a.x * l.y, k, s.c[1]);
+  s.c[2] = fmaf(a.x * l.z, k, s.c[2]);
+  s.c[3] = fmaf(a.y * l.x, k, s.c[3]);
+  s.c[4] = fmaf(a.y * l.y, k, s.c[4]);
+  s.c[5] = fmaf(a.y * l.z, k, s.c[5]);
+  s.c[6] = fmaf(a.z * l.x, k, s.c[6]);
+  s.c[7] = fmaf(a.z * l.y, k, s.c[7]);
+  s.c[8] = fmaf(a.z * l.z, k, s.c[8]);
+  s.rLin = mk3(fmaf(l.x, f, s.rLin.x), fmaf(l.y, f, s.rLin.y), fmaf(l.z, f, s.rLin.z));
+  s.rAng = mk3(fmaf(a.x, f, s.rAng.x), fmaf(a.y, f, s.rAng.y), fmaf(a.z, f, s.rAng.z));
 }
 
 AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, uint32_t i) {
@@ -302,25 +303,37 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
   }
 }
 
-/** acc += the partial system of the lane `s` above, within groups of W lanes. */
-template <uint32_t W> AVBD_D void mergeFromAbove(Acc &acc, uint32_t lane, uint32_t s) {
+/**
+ * acc += the partial system of the lane S above (lanes below S of each group of W use the result). Within a row of 16
+ * lanes the value comes by DPP row shift (row_shl:S, folded into the add); S = 16 crosses rows and goes through
+ * ds_bpermute. Lanes at or above S add garbage that no later level reads.
+ */
+template <uint32_t W, uint32_t S> AVBD_D void mergeFromAbove(Acc &acc) {
   float *a = reinterpret_cast<float *>(&acc);
   constexpr int N = sizeof(Acc) / sizeof(float);
 #pragma unroll
   for (int q = 0; q < N; q++) {
-    float o = __shfl_down(a[q], s, W);
-    if (lane < s) a[q] = a[q] + o;
+    float o;
+    if constexpr (S < 16)
+      o = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a[q]), 0x100 + S, 0xF, 0xF, false));
+    else
+      o = __shfl_down(a[q], S, W);
+    a[q] = a[q] + o;
   }
 }
 
-template <uint32_t W> AVBD_D void mergeTree(Acc &acc, uint32_t lane) {
-#pragma unroll
-  for (uint32_t s = W / 2; s > 0; s >>= 1) mergeFromAbove<W>(acc, lane, s);
+template <uint32_t W> AVBD_D void mergeTree(Acc &acc) {
+  if constexpr (W >= 32) mergeFromAbove<W, 16>(acc);
+  if constexpr (W >= 16) mergeFromAbove<W, 8>(acc);
+  if constexpr (W >= 8) mergeFromAbove<W, 4>(acc);
+  if constexpr (W >= 4) mergeFromAbove<W, 2>(acc);
+  mergeFromAbove<W, 1>(acc);
 }
 
 /**
  * LDLᵀ of the symmetric 6x6 system (lower triangle `A`, row-major by rows 1..6) with each pivot floored at
- * PIVOT_FLOOR of its diagonal (wgsl-solve.ts:416-470), then x = A⁻¹ rhs. `floored` reports whether any pivot took its
+ * PIVOT_FLOOR of its diagonal (wgsl-solve.ts:416-470), then x = A⁻¹ rhs. Each pivot is inverted once and the column
+ * scaled by the reciprocal (6 correctly rounded divisions instead of 21; the CPU restatement does the same). `floored` reports whether any pivot took its
  * floor (the unfloored pivot was not above it).
  */
 template <class T> AVBD_D void ldlt6(const T (&A)[21], const T (&b)[6], T (&x)[6], bool &floored) {
@@ -336,30 +349,36 @@ template <class T> AVBD_D void ldlt6(const T (&A)[21], const T (&b)[6], T (&x)[6
   const T A41 = A[6], A42 = A[7], A43 = A[8], A44 = A[9];
   const T A51 = A[10], A52 = A[11], A53 = A[12], A54 = A[13], A55 = A[14];
   const T A61 = A[15], A62 = A[16], A63 = A[17], A64 = A[18], A65 = A[19], A66 = A[20];
-  const T L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
   const T D1 = A11;
+  const T r1 = T(1) / D1;
+  const T L21 = A21 * r1, L31 = A31 * r1, L41 = A41 * r1, L51 = A51 * r1, L61 = A61 * r1;
   const T D2 = pivot(A22 - L21 * L21 * D1, A22);
-  const T L32 = (A32 - L21 * L31 * D1) / D2;
-  const T L42 = (A42 - L21 * L41 * D1) / D2;
-  const T L52 = (A52 - L21 * L51 * D1) / D2;
-  const T L62 = (A62 - L21 * L61 * D1) / D2;
+  const T r2 = T(1) / D2;
+  const T L32 = (A32 - L21 * L31 * D1) * r2;
+  const T L42 = (A42 - L21 * L41 * D1) * r2;
+  const T L52 = (A52 - L21 * L51 * D1) * r2;
+  const T L62 = (A62 - L21 * L61 * D1) * r2;
   const T D3 = pivot(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33);
-  const T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
-  const T L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
-  const T L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+  const T r3 = T(1) / D3;
+  const T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) * r3;
+  const T L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) * r3;
+  const T L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) * r3;
   const T D4 = pivot(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44);
-  const T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
-  const T L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+  const T r4 = T(1) / D4;
+  const T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) * r4;
+  const T L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) * r4;
   const T D5 = pivot(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55);
-  const T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+  const T r5 = T(1) / D5;
+  const T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) * r5;
   const T D6 = pivot(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66);
+  const T r6 = T(1) / D6;
   const T y1 = b[0];
   const T y2 = b[1] - L21 * y1;
   const T y3 = b[2] - L31 * y1 - L32 * y2;
   const T y4 = b[3] - L41 * y1 - L42 * y2 - L43 * y3;
   const T y5 = b[4] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
   const T y6 = b[5] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
-  const T z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
+  const T z1 = y1 * r1, z2 = y2 * r2, z3 = y3 * r3, z4 = y4 * r4, z5 = y5 * r5, z6 = y6 * r6;
   x[5] = z6;
   x[4] = z5 - L65 * x[5];
   x[3] = z4 - L54 * x[4] - L64 * x[5];
@@ -418,7 +437,7 @@ template <uint32_t W> AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, 
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
   }
-  mergeTree<W>(acc, lane);
+  mergeTree<W>(acc);
   if (live && lane == 0) finishBody(w, slot.x, acc);
 }
 

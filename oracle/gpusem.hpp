@@ -666,15 +666,17 @@ template <class R> class World {
     M3<R> lin, ang, cross;
     V3<R> rLin, rAng;
   };
+  /** addRow (wgsl-solve.ts:53-59) with each accumulation as one fused multiply-add (the HIP primal's v_fma_f32):
+   * lin += (l lᵀ) k, ang += (a aᵀ) k, cross += (a lᵀ) k, rhs += (l, a) f. */
   static void addRow(Acc &acc, V3<R> l, V3<R> a, R k, R f) {
     for (int r = 0; r < 3; r++)
       for (int c = 0; c < 3; c++) {
-        acc.lin.m[r * 3 + c] += l[r] * l[c] * k;
-        acc.ang.m[r * 3 + c] += a[r] * a[c] * k;
-        acc.cross.m[r * 3 + c] += a[r] * l[c] * k;
+        acc.lin.m[r * 3 + c] = std::fma(l[r] * l[c], k, acc.lin.m[r * 3 + c]);
+        acc.ang.m[r * 3 + c] = std::fma(a[r] * a[c], k, acc.ang.m[r * 3 + c]);
+        acc.cross.m[r * 3 + c] = std::fma(a[r] * l[c], k, acc.cross.m[r * 3 + c]);
       }
-    acc.rLin = add(acc.rLin, scale(l, f));
-    acc.rAng = add(acc.rAng, scale(a, f));
+    acc.rLin = V3<R>(std::fma(l.x, f, acc.rLin.x), std::fma(l.y, f, acc.rLin.y), std::fma(l.z, f, acc.rLin.z));
+    acc.rAng = V3<R>(std::fma(a.x, f, acc.rAng.x), std::fma(a.y, f, acc.rAng.y), std::fma(a.z, f, acc.rAng.z));
   }
 
   struct PairBody {
@@ -842,7 +844,7 @@ template <class R> class World {
     Body<R> &b = bodies[i];
     V3<R> xl, xa;
     bool floored = false;
-    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f), &floored);
+    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f), &floored, true);
     if (floored && sizeof(R) == sizeof(float)) {
       auto up = [](const M3<R> &m) {
         M3<double> d;
@@ -851,7 +853,8 @@ template <class R> class World {
       };
       auto upv = [](V3<R> v) { return V3<double>(double(v.x), double(v.y), double(v.z)); };
       V3<double> dl, da;
-      solve6(up(acc.lin), up(acc.ang), up(acc.cross), upv(acc.rLin), upv(acc.rAng), dl, da, double(1e-5f));
+      solve6(up(acc.lin), up(acc.ang), up(acc.cross), upv(acc.rLin), upv(acc.rAng), dl, da, double(1e-5f), nullptr,
+             true);
       xl = V3<R>(R(dl.x), R(dl.y), R(dl.z));
       xa = V3<R>(R(da.x), R(da.y), R(da.z));
       pivotFallbacks++;

@@ -171,10 +171,12 @@ template <class R> inline V3<R> row(const M3<R> &a, int k) { return {a.m[k * 3],
  * LDLᵀ solve of the 6x6 SPD system [aLin aCrossᵀ; aCross aAng]·x = b reading only the lower triangle
  * (math.ts:227-285). With pivotFloor > 0 every pivot D_k is floored at pivotFloor·A_kk, the GPU solver's
  * finishBody (wgsl-solve.ts:416-470, PIVOT_FLOOR layout.ts:106); pivotFloor == 0 is the upstream solve6.
+ * `floored` reports a pivot that took its floor; `reciprocal` divides by each pivot through its reciprocal, formed
+ * once (the HIP finishBody's form; the upstream restatement divides).
  */
 template <class R>
 inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3<R> bLin, V3<R> bAng, V3<R> &xLin,
-                   V3<R> &xAng, R pivotFloor = 0, bool *floored = nullptr) {
+                   V3<R> &xAng, R pivotFloor = 0, bool *floored = nullptr, bool reciprocal = false) {
   const R A11 = aLin.m[0];
   const R A21 = aLin.m[3], A22 = aLin.m[4];
   const R A31 = aLin.m[6], A32 = aLin.m[7], A33 = aLin.m[8];
@@ -186,24 +188,33 @@ inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3
     if (floored && !(d > a * pivotFloor)) *floored = true;
     return rmax(d, a * pivotFloor);
   };
+  // x / D, or x * (1 / D) with the reciprocal formed once per pivot (the GPU primal's form)
+  auto over = [&](R x, R d, R r) { return reciprocal ? x * r : x / d; };
 
-  const R L21 = A21 / A11, L31 = A31 / A11, L41 = A41 / A11, L51 = A51 / A11, L61 = A61 / A11;
   const R D1 = A11;
+  const R r1 = R(1) / D1;
+  const R L21 = over(A21, A11, r1), L31 = over(A31, A11, r1), L41 = over(A41, A11, r1), L51 = over(A51, A11, r1),
+          L61 = over(A61, A11, r1);
   const R D2 = floorp(A22 - L21 * L21 * D1, A22);
-  const R L32 = (A32 - L21 * L31 * D1) / D2;
-  const R L42 = (A42 - L21 * L41 * D1) / D2;
-  const R L52 = (A52 - L21 * L51 * D1) / D2;
-  const R L62 = (A62 - L21 * L61 * D1) / D2;
+  const R r2 = R(1) / D2;
+  const R L32 = over(A32 - L21 * L31 * D1, D2, r2);
+  const R L42 = over(A42 - L21 * L41 * D1, D2, r2);
+  const R L52 = over(A52 - L21 * L51 * D1, D2, r2);
+  const R L62 = over(A62 - L21 * L61 * D1, D2, r2);
   const R D3 = floorp(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33);
-  const R L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
-  const R L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
-  const R L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+  const R r3 = R(1) / D3;
+  const R L43 = over(A43 - L31 * L41 * D1 - L32 * L42 * D2, D3, r3);
+  const R L53 = over(A53 - L31 * L51 * D1 - L32 * L52 * D2, D3, r3);
+  const R L63 = over(A63 - L31 * L61 * D1 - L32 * L62 * D2, D3, r3);
   const R D4 = floorp(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44);
-  const R L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
-  const R L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+  const R r4 = R(1) / D4;
+  const R L54 = over(A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3, D4, r4);
+  const R L64 = over(A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3, D4, r4);
   const R D5 = floorp(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55);
-  const R L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+  const R r5 = R(1) / D5;
+  const R L65 = over(A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4, D5, r5);
   const R D6 = floorp(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66);
+  const R r6 = R(1) / D6;
 
   const R y1 = bLin.x;
   const R y2 = bLin.y - L21 * y1;
@@ -212,7 +223,8 @@ inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3
   const R y5 = bAng.y - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
   const R y6 = bAng.z - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
 
-  const R z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
+  const R z1 = over(y1, D1, r1), z2 = over(y2, D2, r2), z3 = over(y3, D3, r3), z4 = over(y4, D4, r4),
+          z5 = over(y5, D5, r5), z6 = over(y6, D6, r6);
 
   xAng.z = z6;
   xAng.y = z5 - L65 * xAng.z;
