@@ -225,6 +225,8 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   bvh_.nodes = dallocT<BvhNode>(plan_.nodeCount);
   bvh_.trav = dallocT<BvhNode>(plan_.nodeCount);
   bvh_.order = dallocT<uint32_t>(n);
+  bvh_.rank = dallocT<uint32_t>(n);
+  bvh_.maxPos = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.keys = dallocT<uint64_t>(n);

@@ -33,6 +33,8 @@ struct Bvh {
   BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
   BvhNode *trav;       // traversal copy: lo + left/leaf tag, hi + escape rope
   uint32_t *order;     // bodies in Morton order (the last build's)
+  uint32_t *rank;      // per body: its position in `order`
+  uint32_t *maxPos;    // per node: highest rank of a dynamic leaf below it, | MAXPOS_STATIC if a static leaf is below
   uint32_t *parent;    // per node
   uint32_t *escape;    // per node
   uint64_t *keys, *keysAlt;
@@ -44,6 +46,9 @@ struct Bvh {
   BvhGroup *groups;    // every level's groups
   uint32_t nodeCount;
 };
+
+/** maxPos flag: the subtree holds a static leaf (every dynamic query must look there). */
+constexpr uint32_t MAXPOS_STATIC = 0x80000000u;
 
 /** The plan for worlds of the given body counts (bodies of world k follow those of worlds 0..k-1). */
 inline BvhPlan makeBvhPlan(const std::vector<uint32_t> &counts) {

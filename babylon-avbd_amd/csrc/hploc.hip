@@ -253,7 +253,21 @@ __global__ void __launch_bounds__(64) k_bvh_level(Bvh t, const BvhGroup *groups,
   }
 }
 
-/** Refit one level's treelets: nodes in creation order (children before parents). */
+/** Leaf ranks for the traversal's pruning: a dynamic leaf's position in Morton order, a static leaf's flag. */
+__global__ void k_bvh_rank(DeviceWorld w, Bvh t) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < w.nBodies) t.rank[t.order[q]] = q;
+}
+__global__ void k_bvh_leafmax(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < w.nBodies) t.maxPos[i] = w.bodies[i].size.w > 0.f ? t.rank[i] : MAXPOS_STATIC;
+}
+
+AVBD_D uint32_t combineMaxPos(uint32_t a, uint32_t b) {
+  return umax(a & ~MAXPOS_STATIC, b & ~MAXPOS_STATIC) | ((a | b) & MAXPOS_STATIC);
+}
+
+/** Refit one level's treelets: nodes in creation order (children before parents); bounds and maxPos. */
 __global__ void k_bvh_refit_level(Bvh t, const BvhGroup *groups, uint32_t nGroups) {
   uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= nGroups) return;
@@ -261,9 +275,11 @@ __global__ void k_bvh_refit_level(Bvh t, const BvhGroup *groups, uint32_t nGroup
   for (uint32_t j = 0; j + 1 < grp.count; j++) {
     uint32_t nid = grp.nodeBase + j;
     BvhNode n = t.nodes[nid];
-    BvhNode a = t.nodes[__float_as_uint(n.lo.w)];
-    BvhNode b = t.nodes[__float_as_uint(n.hi.w)];
+    uint32_t l = __float_as_uint(n.lo.w), r = __float_as_uint(n.hi.w);
+    BvhNode a = t.nodes[l];
+    BvhNode b = t.nodes[r];
     t.nodes[nid] = BvhNode{mk4(fmin3(xyz(a.lo), xyz(b.lo)), n.lo.w), mk4(fmax3(xyz(a.hi), xyz(b.hi)), n.hi.w)};
+    t.maxPos[nid] = combineMaxPos(t.maxPos[l], t.maxPos[r]);
   }
 }
 
@@ -298,10 +314,12 @@ __global__ void k_bvh_trav(Bvh t, uint32_t nNodes) {
 }
 
 /**
- * Pairs of one dynamic body from its world's tree: leaves overlapping its (padded) box, dynamic partners of lower
- * index and static partners of any index (gpuHploc.js:37-55), each through the pair predicate. Threads take bodies in
- * Morton order (neighbouring threads walk neighbouring paths). STASH pass: count, keeping the first PAIR_STASH
- * partners; write pass: copy the stash sorted by partner, or walk again for bodies with more.
+ * Pairs of one dynamic body from its world's tree: leaves overlapping its (padded) box (gpuHploc.js:37-55), each through
+ * the pair predicate as (higher index, lower index). Every static leaf is looked for, but of two dynamic bodies only
+ * the one earlier in Morton order looks for the other: subtrees holding no static leaf and no dynamic leaf later than
+ * the query are skipped (maxPos), so each dynamic pair is found once. Threads take bodies in Morton order
+ * (neighbouring threads walk neighbouring paths). STASH pass: count, keeping the first PAIR_STASH partners; write
+ * pass: copy the stash sorted by partner, or walk again for bodies with more.
  */
 template <bool STASH> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -321,15 +339,16 @@ template <bool STASH> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
     // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
     for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
       BvhNode nd = t.trav[at];
+      uint32_t mp = t.maxPos[at];
       uint32_t tag = __float_as_uint(nd.lo.w);
       if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
-          nd.lo.z > qhi.z) {
+          nd.lo.z > qhi.z || !((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q)) {
         at = __float_as_uint(nd.hi.w);
         continue;
       }
       if (tag & LEAF) {
         uint32_t j = tag & ~LEAF;
-        if (j != i && (j < i || w.bodies[j].size.w <= 0.f)) {
+        {
           uint32_t a = umax(i, j), b = umin(i, j);
           if (acceptPair(w, a, b)) {
             if (STASH) {
@@ -396,11 +415,16 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   k_bvh_copy<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t.roots, in, w.nWorlds);
   k_bvh_ropes<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
   k_bvh_copy<<<blocks(n, 256), 256, 0, s>>>(t.order, v0, n);
+  k_bvh_rank<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  k_bvh_leafmax<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  for (const BvhLevel &lev : plan.levels)
+    k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
   k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
 }
 
 void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
   k_bvh_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
+  k_bvh_leafmax<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
   k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
