@@ -378,8 +378,8 @@ int avbd_world_profile_step(avbd_world *world, float *out) {
     CHECK_WORLD(world);
     if (!out) throw std::invalid_argument("out is NULL");
     PhaseTimes p = world->engine->profileStep();
-    const float v[10] = {p.total, p.collision, p.adjacency, p.coloring, p.warmStart, p.primal, p.dual, p.velocities,
-                         float(p.primalLaunches), float(p.dualLaunches)};
+    const float v[13] = {p.total, p.collision, p.adjacency, p.coloring, p.warmStart, p.primal, p.dual, p.velocities,
+                         float(p.primalLaunches), float(p.dualLaunches), p.broadphase, p.narrowphase, p.pairSearch};
     std::memcpy(out, v, sizeof v);
     return AVBD_OK;
   });

@@ -38,6 +38,8 @@ struct EngineOptions {
 struct PhaseTimes {
   float total, collision, adjacency, coloring, warmStart, primal, dual, velocities;
   uint32_t primalLaunches, dualLaunches;
+  float broadphase, narrowphase;  // parts of collision: tree or grid and pair search; contacts and compaction
+  float pairSearch;               // part of broadphase: the pair search alone
 };
 
 /** Everything the next step of a batch depends on, in host memory (a world evicted from HBM keeps this). */

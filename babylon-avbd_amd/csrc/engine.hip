@@ -160,6 +160,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.joints = dallocT<Joint>(nJoints_);
   w.info = dallocT<int4>(nJoints_);
   w.noCollide = dallocT<uint32_t>(3ull * noCollideCount_);
+  w.ncStart = dallocT<uint32_t>(uint64_t(nBodies_) + 1);
   w.counters = dallocT<uint32_t>(COUNTER_WORDS);
   w.gridCount = dallocT<uint32_t>(uint64_t(tableSize_) + 2);
   w.gridCursor = dallocT<uint32_t>(tableSize_);
@@ -202,6 +203,10 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
     nc.push_back(std::get<2>(t));
   }
   up(w.noCollide, nc.data(), nc.size() * 4);
+  std::vector<uint32_t> ncStart(uint64_t(nBodies_) + 1, 0);
+  for (auto &t : noCollide) ncStart[std::get<0>(t) + 1]++;
+  for (uint32_t i = 0; i < nBodies_; i++) ncStart[i + 1] += ncStart[i];
+  up(w.ncStart, ncStart.data(), ncStart.size() * 4);
   up(w.largeStart, largeStart.data(), largeStart.size() * 4);
   up(w.largeList, largeList.data(), largeList.size() * 4);
   std::vector<uint32_t> noColor(nBodies_, NO_COLOR);
@@ -227,6 +232,9 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   bvh_.order = dallocT<uint32_t>(n);
   bvh_.rank = dallocT<uint32_t>(n);
   bvh_.maxPos = dallocT<uint32_t>(plan_.nodeCount);
+  bvh_.candStash = dallocT<uint32_t>(uint64_t(n) * BVH_CAND);
+  bvh_.candCount = dallocT<uint32_t>(n);
+  bvh_.candMask = dallocT<uint32_t>(n);
   bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.keys = dallocT<uint64_t>(n);
@@ -432,14 +440,31 @@ void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }
 PhaseTimes Engine::profileStep() {
   HIP_CHECK(hipSetDevice(device_));
   const uint32_t launches = iterations_ * (colorCap_ + 1);
-  std::vector<hipEvent_t> ev(6 + 2 * launches);
+  std::vector<hipEvent_t> ev(9 + 2 * launches);
   for (auto &x : ev) HIP_CHECK(hipEventCreate(&x));
   hipStream_t s = stream_;
   DeviceWorld &w = views_[parity_];
   w.colorCap = colorCap_;
   const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
   HIP_CHECK(hipEventRecord(ev[0], s));
-  encodeCollision(w, rebuild);
+  // encodeCollision, with events after the frame set-up and after the broadphase
+  launch_begin_frame(w, s);
+  launch_update_refs(w, s);
+  launch_hash_build(w, pairCap_, s);
+  HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 4], s));
+  if (useBvh_) {
+    if (rebuild) launch_bvh_build(w, bvh_, plan_, s);
+    else launch_bvh_refit(w, bvh_, plan_, s);
+    HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 6], s));
+    launch_bvh_pairs(w, bvh_, s);
+    launch_pairs_total(w, s);
+  } else {
+    launch_grid(w, s);
+    HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 6], s));
+    launch_pairs(w, s);
+  }
+  HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 5], s));
+  launch_narrowphase(w, s);
   HIP_CHECK(hipEventRecord(ev[1], s));
   launch_adjacency(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2], s));
@@ -471,6 +496,9 @@ PhaseTimes Engine::profileStep() {
   PhaseTimes p{};
   p.total = ms(0, 5);
   p.collision = ms(0, 1);
+  p.broadphase = ms(2 + 2 * launches + 4, 2 + 2 * launches + 5);
+  p.pairSearch = ms(2 + 2 * launches + 6, 2 + 2 * launches + 5);
+  p.narrowphase = ms(2 + 2 * launches + 5, 1);
   p.adjacency = ms(1, 2);
   p.coloring = ms(2, 3);
   p.warmStart = ms(3, 4);

@@ -314,62 +314,115 @@ __global__ void k_bvh_trav(Bvh t, uint32_t nNodes) {
 }
 
 /**
- * Pairs of one dynamic body from its world's tree: leaves overlapping its (padded) box (gpuHploc.js:37-55), each through
- * the pair predicate as (higher index, lower index). Every static leaf is looked for, but of two dynamic bodies only
- * the one earlier in Morton order looks for the other: subtrees holding no static leaf and no dynamic leaf later than
- * the query are skipped (maxPos), so each dynamic pair is found once. Threads take bodies in Morton order
- * (neighbouring threads walk neighbouring paths). STASH pass: count, keeping the first PAIR_STASH partners; write
- * pass: copy the stash sorted by partner, or walk again for bodies with more.
+ * Walk body i's world tree (stackless, escape ropes) and call visit(j) for each leaf j whose padded box overlaps i's
+ * (gpuHploc.js:37-55). Every static leaf is visited, but of two dynamic bodies only the one earlier in Morton order
+ * (rank q) visits the other: subtrees holding no static leaf and no dynamic leaf later than q are skipped (maxPos),
+ * so each dynamic pair comes up once.
  */
-template <bool STASH> __global__ void k_bvh_pairs(DeviceWorld w, Bvh t) {
+template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint32_t i, uint32_t q, F &&visit) {
+  BvhNode self = t.trav[i];
+  f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
+  uint32_t at = t.roots[w.bodyWorld[i]];
+  // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
+  for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
+    BvhNode nd = t.trav[at];
+    uint32_t mp = t.maxPos[at];
+    uint32_t tag = __float_as_uint(nd.lo.w);
+    if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
+        nd.lo.z > qhi.z || !((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q)) {
+      at = __float_as_uint(nd.hi.w);
+      continue;
+    }
+    if (tag & LEAF) {
+      visit(tag & ~LEAF);
+      at = __float_as_uint(nd.hi.w);
+    } else {
+      at = tag;
+    }
+  }
+}
+
+/**
+ * Pass 1: each dynamic body's query walks the tree with the box test only and keeps its first BVH_CAND candidate
+ * leaves. Threads take bodies in Morton order (neighbouring threads walk neighbouring paths).
+ */
+__global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= w.nBodies) return;
   uint32_t i = t.order[q];
-  uint32_t base = STASH ? 0 : w.pairCount[i];
-  uint32_t total = STASH ? 0 : w.pairCount[i + 1] - base;
-  if (!STASH && total <= PAIR_STASH) {
-    writeStashedPairs(w, i, base, total);
-    return;
-  }
   uint32_t n = 0;
   if (w.bodies[i].size.w > 0.f) {
-    BvhNode self = t.trav[i];
-    f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
-    uint32_t at = t.roots[w.bodyWorld[i]];
-    // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
-    for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
-      BvhNode nd = t.trav[at];
-      uint32_t mp = t.maxPos[at];
-      uint32_t tag = __float_as_uint(nd.lo.w);
-      if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
-          nd.lo.z > qhi.z || !((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q)) {
-        at = __float_as_uint(nd.hi.w);
-        continue;
-      }
-      if (tag & LEAF) {
-        uint32_t j = tag & ~LEAF;
-        {
-          uint32_t a = umax(i, j), b = umin(i, j);
-          if (acceptPair(w, a, b)) {
-            if (STASH) {
-              if (n < PAIR_STASH) w.pairStash[size_t(i) * PAIR_STASH + n] = j;
-            } else if (base + n < w.pairCapacity) {
-              w.pairs[base + n] = make_uint2(a, b);
-            }
-            n++;
-          }
-        }
-        at = __float_as_uint(nd.hi.w);
-      } else {
-        at = tag;
-      }
-    }
+    uint32_t *out = t.candStash + size_t(i) * BVH_CAND;
+    walkTree(w, t, i, q, [&](uint32_t j) {
+      if (n < BVH_CAND) out[n] = j;
+      n++;
+    });
   }
-  if (STASH) {
-    w.pairCount[i] = n;
+  t.candCount[i] = n;
+}
+
+/**
+ * Pass 2: the pair predicate on every kept candidate at once, BVH_CAND lanes per body (two bodies a wave); the group's
+ * ballot is the body's accepted mask and its population the body's pair count. A body with more candidates than it
+ * kept is counted by the overflow pass instead.
+ */
+__global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
+  const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t i = uint32_t(g / BVH_CAND), slot = uint32_t(g % BVH_CAND);
+  const bool live = i < w.nBodies;
+  const uint32_t count = live ? t.candCount[i] : 0u;
+  bool accept = false;
+  if (live && count <= BVH_CAND && slot < count) {
+    uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
+    accept = acceptPair(w, umax(i, j), umin(i, j));
+  }
+  const uint64_t ballot = __ballot(accept);
+  const uint32_t bits = uint32_t(ballot >> (threadIdx.x % 64 / BVH_CAND * BVH_CAND));
+  if (live && slot == 0) {
+    t.candMask[i] = bits;
+    if (count <= BVH_CAND) w.pairCount[i] = uint32_t(__popc(bits));
+  }
+}
+
+/** Pass 2b: bodies with more candidates than kept walk again, testing as they go, and count their pairs. */
+__global__ void k_bvh_overflow_count(DeviceWorld w, Bvh t) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= w.nBodies) return;
+  uint32_t i = t.order[q];
+  if (t.candCount[i] <= BVH_CAND) return;
+  uint32_t n = 0;
+  walkTree(w, t, i, q, [&](uint32_t j) {
+    if (acceptPair(w, umax(i, j), umin(i, j))) n++;
+  });
+  w.pairCount[i] = n;
+}
+
+/** Pass 3 (after the scan of the counts): write each body's pairs (a > b) in candidate order. */
+__global__ void k_bvh_write(DeviceWorld w, Bvh t) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= w.nBodies) return;
+  uint32_t i = t.order[q];
+  uint32_t base = w.pairCount[i];
+  uint32_t count = t.candCount[i];
+  uint32_t n = 0;
+  if (count <= BVH_CAND) {
+    uint32_t bits = t.candMask[i];
+    while (bits) {
+      uint32_t slot = uint32_t(__builtin_ctz(bits));
+      bits &= bits - 1u;
+      uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
+      if (base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(umax(i, j), umin(i, j));
+      n++;
+    }
     return;
   }
-  sortBodyPairs(w, i, base, n);
+  walkTree(w, t, i, q, [&](uint32_t j) {
+    uint32_t a = umax(i, j), b = umin(i, j);
+    if (acceptPair(w, a, b)) {
+      if (base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(a, b);
+      n++;
+    }
+  });
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -431,9 +484,11 @@ void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
 }
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
-  k_bvh_pairs<true><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  k_bvh_cand<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * BVH_CAND, 256), 256, 0, s>>>(w, t);
+  k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
   launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
-  k_bvh_pairs<false><<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  k_bvh_write<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
 }
 
 }  // namespace avbd

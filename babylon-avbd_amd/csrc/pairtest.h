@@ -39,21 +39,14 @@ AVBD_D bool faceSeparated(const Shape &A, const Shape &B, f3 d) {
   return false;
 }
 
-/** Binary search of the (hi, lo) no-collide list; false if every connecting joint has broken (wgsl-collision.ts:112). */
+/**
+ * The pair is on the no-collide list (an ignored pair, or a joint that has not broken; wgsl-collision.ts:112). The
+ * list is grouped by the higher body, so this scans hi's few entries instead of searching the whole list.
+ */
 AVBD_D bool ignored(const DeviceWorld &w, uint32_t hi, uint32_t lo) {
-  uint32_t first = 0, count = w.noCollideCount;
-  while (count > 0) {
-    uint32_t step = count / 2, mid = first + step;
-    uint32_t mh = w.noCollide[3 * mid], ml = w.noCollide[3 * mid + 1];
-    if (mh < hi || (mh == hi && ml < lo)) {
-      first = mid + 1;
-      count -= step + 1;
-    } else {
-      count = step;
-    }
-  }
-  for (uint32_t k = first; k < w.noCollideCount; k++) {
-    if (w.noCollide[3 * k] != hi || w.noCollide[3 * k + 1] != lo) break;
+  const uint32_t end = w.ncStart[hi + 1];
+  for (uint32_t k = w.ncStart[hi]; k < end; k++) {
+    if (w.noCollide[3 * k + 1] != lo) continue;
     uint32_t j = w.noCollide[3 * k + 2];
     if (j == 0xffffffffu) return true;
     if (w.joints[j].penLin.w != 0.f || w.joints[j].penAng.w != 0.f) return true;

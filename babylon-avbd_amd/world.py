@@ -291,10 +291,10 @@ class HipWorld:
     def profileStep(self):
         """One step with HIP events per phase and per primal/dual launch (ms)."""
         self._alive()
-        out = np.zeros(10, np.float32)
+        out = np.zeros(13, np.float32)
         check(self._lib.avbd_world_profile_step(self._h, ptr(out, ctypes.c_float)))
         keys = ("total", "collision", "adjacency", "coloring", "warmStart", "primal", "dual", "velocities",
-                "primalLaunches", "dualLaunches")
+                "primalLaunches", "dualLaunches", "broadphase", "narrowphase", "pairSearch")
         return {k: float(v) for k, v in zip(keys, out)}
 
     def deviceBytes(self):

@@ -166,8 +166,10 @@ int avbd_world_adapt(avbd_world *world, int *changed);
 
 /**
  * Run one step with HIP events around each phase (the reference's per-phase timestamp queries, solver.ts:1389-1416)
- * and around every primal and dual launch. out[10] (ms): total, collision, adjacency, colouring, warm start,
- * primal (sum over launches), dual (sum), velocities, primal launches, dual launches.
+ * and around every primal and dual launch. out[13] (ms): total, collision, adjacency, colouring, warm start,
+ * primal (sum over launches), dual (sum), velocities, primal launches, dual launches, broadphase (tree or grid and
+ * the pair search, part of collision), narrowphase (contacts and manifold compaction, part of collision), pair search
+ * (part of broadphase).
  */
 int avbd_world_profile_step(avbd_world *world, float *out);
 
