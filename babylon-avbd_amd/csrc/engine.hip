@@ -234,7 +234,7 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   bvh_.maxPos = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.candStash = dallocT<uint32_t>(uint64_t(n) * BVH_CAND);
   bvh_.candCount = dallocT<uint32_t>(n);
-  bvh_.candMask = dallocT<uint32_t>(n);
+  bvh_.candMask = dallocT<uint32_t>(uint64_t(n) * BVH_CAND_WORDS);
   bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.keys = dallocT<uint64_t>(n);

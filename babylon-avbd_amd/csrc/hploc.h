@@ -37,7 +37,7 @@ struct Bvh {
   uint32_t *maxPos;    // per node: highest rank of a dynamic leaf below it, | MAXPOS_STATIC if a static leaf is below
   uint32_t *candStash; // per body: up to BVH_CAND candidate leaves of its query (boxes overlap), in traversal order
   uint32_t *candCount; // per body: its query's candidate count (may exceed BVH_CAND)
-  uint32_t *candMask;  // per body: bit k = candidate k passed the pair predicate
+  uint32_t *candMask;  // per body: BVH_CAND / 32 words, bit k = candidate k passed the pair predicate
   uint32_t *parent;    // per node
   uint32_t *escape;    // per node
   uint64_t *keys, *keysAlt;
@@ -51,7 +51,8 @@ struct Bvh {
 };
 
 /** Candidate leaves kept per query; a query with more is answered by a second walk that tests as it goes. */
-constexpr uint32_t BVH_CAND = 32;
+constexpr uint32_t BVH_CAND = 128;
+constexpr uint32_t BVH_CAND_WORDS = BVH_CAND / 32;
 
 /** maxPos flag: the subtree holds a static leaf (every dynamic query must look there). */
 constexpr uint32_t MAXPOS_STATIC = 0x80000000u;

@@ -362,26 +362,31 @@ __global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
 }
 
 /**
- * Pass 2: the pair predicate on every kept candidate at once, BVH_CAND lanes per body (two bodies a wave); the group's
- * ballot is the body's accepted mask and its population the body's pair count. A body with more candidates than it
- * kept is counted by the overflow pass instead.
+ * Pass 2: the pair predicate on every kept candidate, 32 lanes per body (two bodies a wave), 32 candidates a round;
+ * each round's ballot is a word of the body's accepted mask and the mask's population its pair count. A body with
+ * more candidates than it kept is counted by the overflow pass instead.
  */
 __global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
   const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint32_t i = uint32_t(g / BVH_CAND), slot = uint32_t(g % BVH_CAND);
+  const uint32_t i = uint32_t(g / 32), lane = uint32_t(g % 32);
   const bool live = i < w.nBodies;
   const uint32_t count = live ? t.candCount[i] : 0u;
-  bool accept = false;
-  if (live && count <= BVH_CAND && slot < count) {
-    uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
-    accept = acceptPair(w, umax(i, j), umin(i, j));
+  const uint32_t kept = count <= BVH_CAND ? count : 0u;
+  // Rounds are the same for both bodies of a wave (the ballot needs every lane): the larger of the two
+  const uint32_t rounds = (umax(kept, __shfl_xor(kept, 32)) + 31) / 32;
+  uint32_t pairs = 0;
+  for (uint32_t r = 0; r < rounds; r++) {
+    const uint32_t slot = r * 32 + lane;
+    bool accept = false;
+    if (live && slot < kept) {
+      uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
+      accept = acceptPair(w, umax(i, j), umin(i, j));
+    }
+    const uint32_t bits = uint32_t(__ballot(accept) >> (threadIdx.x % 64 / 32 * 32));
+    if (live && lane == 0 && r < BVH_CAND_WORDS) t.candMask[size_t(i) * BVH_CAND_WORDS + r] = bits;
+    pairs += uint32_t(__popc(bits));
   }
-  const uint64_t ballot = __ballot(accept);
-  const uint32_t bits = uint32_t(ballot >> (threadIdx.x % 64 / BVH_CAND * BVH_CAND));
-  if (live && slot == 0) {
-    t.candMask[i] = bits;
-    if (count <= BVH_CAND) w.pairCount[i] = uint32_t(__popc(bits));
-  }
+  if (live && lane == 0 && count <= BVH_CAND) w.pairCount[i] = pairs;
 }
 
 /** Pass 2b: bodies with more candidates than kept walk again, testing as they go, and count their pairs. */
@@ -406,13 +411,15 @@ __global__ void k_bvh_write(DeviceWorld w, Bvh t) {
   uint32_t count = t.candCount[i];
   uint32_t n = 0;
   if (count <= BVH_CAND) {
-    uint32_t bits = t.candMask[i];
-    while (bits) {
-      uint32_t slot = uint32_t(__builtin_ctz(bits));
-      bits &= bits - 1u;
-      uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
-      if (base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(umax(i, j), umin(i, j));
-      n++;
+    for (uint32_t r = 0; r * 32 < count; r++) {
+      uint32_t bits = t.candMask[size_t(i) * BVH_CAND_WORDS + r];
+      while (bits) {
+        uint32_t slot = r * 32 + uint32_t(__builtin_ctz(bits));
+        bits &= bits - 1u;
+        uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
+        if (base + n < w.pairCapacity) w.pairs[base + n] = make_uint2(umax(i, j), umin(i, j));
+        n++;
+      }
     }
     return;
   }
@@ -485,7 +492,7 @@ void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
   k_bvh_cand<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
-  k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * BVH_CAND, 256), 256, 0, s>>>(w, t);
+  k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
   k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
   launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
   k_bvh_write<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
