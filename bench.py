@@ -61,34 +61,37 @@ def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
                        f"colour-ordered Gauss-Seidel), {dt:.1f} s")
 
 
-def batched_worlds(avbd, steps, warmup, sample):
-    """The batched-worlds workload: 4,096 box piles (body counts log-uniform in [8, 2048], friction U(0.2, 0.9)) stepped
-    together, one launch per phase from a captured hipGraph; per-step device latency from HIP events around each step.
-    Beside it, one world per launch: the first `sample` worlds of the same draw, each its own world (own stream and
-    graph), stepped round-robin over the same step window; its aggregate rate is the sample's body-steps per second."""
+def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
+    """The batched-worlds workload: 4,096 box piles (body counts log-uniform in [8, 2048], friction U(0.2, 0.9)), one
+    launch per phase per batch from captured hipGraphs. The autotuner (babylon-avbd_amd/tune.py) first times short
+    runs of the worlds split into 1, 8, 16 and 32 batches and keeps the fastest split whose per-step latency (device
+    time of one batch's step, HIP events) meets the 5 ms target at p99; the timed window then runs that split, the
+    batches stepped one after another each round. Beside it, one world per launch: the first `sample` worlds of the
+    same draw, each its own world (own stream and graph), stepped round-robin over the same step window."""
     opts = {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True}
+    pipe = avbd.createSolverPipeline("hip")
+    log = lambda m: print("# " + m, file=sys.stderr, flush=True)  # noqa: E731
+    groups, table = avbd.tune.batch_groups(pipe, avbd.configs.batched_piles, opts, target_p99_ms=target_p99_ms,
+                                           candidates=(1, 8, 16, 32), log=log)
     scenes = avbd.configs.batched_piles()
     bodies = sum(s.body_count for s in scenes)
-    pipe = avbd.createSolverPipeline("hip")
-    batch = pipe.createBatch(scenes, opts)
-    print(f"# batch: {len(scenes)} worlds, {bodies} bodies, {batch.deviceBytes() / 2**30:.1f} GiB on device",
-          file=sys.stderr, flush=True)
+    worlds = [pipe.createBatch(part, opts) for part in avbd.tune.split_balanced(scenes, groups)]
+    log(f"batch: {len(scenes)} worlds, {bodies} bodies in {len(worlds)} batches, "
+        f"{sum(w.deviceBytes() for w in worlds) / 2**30:.1f} GiB on device")
+    avbd.tune.run_groups(worlds, warmup)
+    ms, elapsed = [], 0.0
     done = 0
-    while done < warmup:
-        k = min(50, warmup - done)
-        batch.step(k)
-        batch.synchronize()
+    while done < steps:
+        k = min(50, steps - done)
+        m, e = avbd.tune.run_groups(worlds, k)
+        ms += m
+        elapsed += e
         done += k
-        print(f"# batch warm-up {done}: {batch.readCounters()}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    ms = []
-    while len(ms) < steps:
-        ms.extend(batch.stepTimed(min(50, steps - len(ms))).tolist())
-        print(f"# batch step {warmup + len(ms)}: last {ms[-1]:.2f} ms", file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
-    counters = batch.readCounters()
-    prof = batch.profileStep()
-    batch.destroy()
+        log(f"batch step {warmup + done}: {len(worlds)} batches, last step {m[-1]:.2f} ms")
+    counters = [w.readCounters() for w in worlds]
+    prof = worlds[0].profileStep()
+    for w in worlds:
+        w.destroy()
     ms = np.asarray(ms)
     agg = bodies * steps / elapsed
 
@@ -99,25 +102,29 @@ def batched_worlds(avbd, steps, warmup, sample):
             w.step(1)
     for w in solos:
         w.synchronize()
-    print(f"# one world per launch: {sample} worlds, {solo_bodies} bodies warmed up", file=sys.stderr, flush=True)
+    log(f"one world per launch: {sample} worlds, {solo_bodies} bodies warmed up")
     t0 = time.perf_counter()
     for k in range(steps):
         for w in solos:
             w.step(1)
         if (k + 1) % 50 == 0:
-            print(f"# one world per launch: step {warmup + k + 1}", file=sys.stderr, flush=True)
+            log(f"one world per launch: step {warmup + k + 1}")
     for w in solos:
         w.synchronize()
-    solo_elapsed = time.perf_counter() - t0
-    solo_rate = solo_bodies * steps / solo_elapsed
+    solo_rate = solo_bodies * steps / (time.perf_counter() - t0)
     pipe.destroy()
+    single = next(r for r in table if r[0] == 1)
     return {
         "workload": "4,096 box piles, body counts log-uniform in [8, 2048], friction U(0.2, 0.9), 10 iterations, "
-                    "one launch per phase (hipGraph)",
-        "worlds": len(scenes), "bodies": bodies, "steps_timed": f"{warmup + 1}-{warmup + steps}",
-        "body_steps_per_s": agg, "ms_per_step": elapsed * 1e3 / steps,
+                    "one launch per phase per batch (hipGraph)",
+        "worlds": len(scenes), "bodies": bodies, "batches": len(worlds), "steps_timed": f"{warmup + 1}-{warmup + steps}",
+        "body_steps_per_s": agg, "ms_per_round": elapsed * 1e3 / steps,
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
-        "step_ms_max": float(ms.max()), "counters": counters, "phases_ms": prof,
+        "step_ms_max": float(ms.max()), "p99_target_ms": target_p99_ms,
+        "tuning": [{"batches": g, "body_steps_per_s": r, "step_ms_p50": p50, "step_ms_p99": p99,
+                    "steps": "11-30"} for g, r, p50, p99 in table],
+        "single_batch_probe": {"body_steps_per_s": single[1], "step_ms_p99": single[3]},
+        "counters_first_batch": counters[0], "phases_ms_first_batch": prof,
         "one_world_per_launch": {"worlds": sample, "bodies": solo_bodies, "body_steps_per_s": solo_rate,
                                  "sample": f"worlds 0-{sample - 1} of the same draw, each its own world, "
                                            f"steps {warmup + 1}-{warmup + steps} round-robin"},
