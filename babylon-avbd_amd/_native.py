@@ -100,6 +100,8 @@ SIGNATURES = {
     "avbd_world_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_world_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "avbd_world_step_timed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, f32p]),
+    "avbd_world_step_timed_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "avbd_world_step_timed_end": (ctypes.c_int, [ctypes.c_void_p, f32p, ctypes.c_uint32, u32p]),
     "avbd_world_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_world_body_count": (ctypes.c_int, [ctypes.c_void_p, u32p]),
     "avbd_world_joint_count": (ctypes.c_int, [ctypes.c_void_p, u32p]),

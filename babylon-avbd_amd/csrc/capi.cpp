@@ -166,6 +166,28 @@ int avbd_world_step_timed(avbd_world *world, uint32_t n, float *ms) {
   });
 }
 
+int avbd_world_step_timed_begin(avbd_world *world, uint32_t n) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (n > 4096) throw std::invalid_argument("at most 4096 timed steps at once");
+    if (world->engine->timedPending()) throw std::invalid_argument("timed steps already pending: end them first");
+    world->engine->stepTimedBegin(n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_step_timed_end(avbd_world *world, float *ms, uint32_t capacity, uint32_t *count) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!count) throw std::invalid_argument("count is NULL");
+    const uint32_t n = world->engine->timedPending();
+    if (n > capacity) throw std::length_error("ms capacity below the number of timed steps");
+    if (n && !ms) throw std::invalid_argument("ms is NULL");
+    *count = world->engine->stepTimedEnd(ms);
+    return AVBD_OK;
+  });
+}
+
 int avbd_world_synchronize(avbd_world *world) {
   return guarded([&] {
     CHECK_WORLD(world);

@@ -67,6 +67,10 @@ class Engine {
   void step(uint32_t n);
   /** n steps with HIP events around each on the engine's stream; ms[k] = step k's device time. Waits. */
   void stepTimed(uint32_t n, float *ms);
+  /** stepTimed in two halves: enqueue n event-bracketed steps; later wait and read their times (returns n). */
+  void stepTimedBegin(uint32_t n);
+  uint32_t stepTimedEnd(float *ms);
+  uint32_t timedPending() const { return timedSteps_; }
   void synchronize();
   /** One step (not from a graph) with HIP events around each phase and each primal/dual launch. */
   PhaseTimes profileStep();
@@ -159,6 +163,7 @@ class Engine {
   uint32_t counterSlot_ = 0;
   uint32_t shrinkVotes_ = 0;
   std::vector<hipEvent_t> timingEvents_;
+  uint32_t timedSteps_ = 0;
 };
 
 /** Largest bounding radius still sized into grid cells (reference/three-avbd/src/avbd3d/gpu/radii.ts:47-55). */

@@ -419,6 +419,11 @@ void Engine::step(uint32_t n) {
 }
 
 void Engine::stepTimed(uint32_t n, float *ms) {
+  stepTimedBegin(n);
+  stepTimedEnd(ms);
+}
+
+void Engine::stepTimedBegin(uint32_t n) {
   HIP_CHECK(hipSetDevice(device_));
   while (timingEvents_.size() < 2ull * n) {
     hipEvent_t e;
@@ -430,9 +435,17 @@ void Engine::stepTimed(uint32_t n, float *ms) {
     step(1);
     HIP_CHECK(hipEventRecord(timingEvents_[2 * k + 1], stream_));
   }
+  timedSteps_ = n;
+}
+
+uint32_t Engine::stepTimedEnd(float *ms) {
+  const uint32_t n = timedSteps_;
+  if (n == 0) return 0;
   HIP_CHECK(hipEventSynchronize(timingEvents_[2 * n - 1]));
   for (uint32_t k = 0; k < n; k++)
     HIP_CHECK(hipEventElapsedTime(&ms[k], timingEvents_[2 * k], timingEvents_[2 * k + 1]));
+  timedSteps_ = 0;
+  return n;
 }
 
 void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }

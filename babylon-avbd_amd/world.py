@@ -142,6 +142,22 @@ class HipWorld:
         check(self._lib.avbd_world_step_timed(self._h, int(n), ptr(ms, ctypes.c_float)))
         return ms
 
+    def stepTimedBegin(self, n=1):
+        """Enqueue n event-bracketed steps without waiting (pair with stepTimedEnd)."""
+        self._alive()
+        check(self._lib.avbd_world_step_timed_begin(self._h, int(n)))
+        self._timed = int(n)
+
+    def stepTimedEnd(self):
+        """Wait for the steps of the last stepTimedBegin; returns their device milliseconds."""
+        self._alive()
+        n = getattr(self, "_timed", 0)
+        ms = np.zeros(max(n, 1), np.float32)
+        got = ctypes.c_uint32()
+        check(self._lib.avbd_world_step_timed_end(self._h, ptr(ms, ctypes.c_float), len(ms), ctypes.byref(got)))
+        self._timed = 0
+        return ms[:got.value]
+
     def synchronize(self):
         self._alive()
         check(self._lib.avbd_world_synchronize(self._h))

@@ -64,30 +64,30 @@ def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
 def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
     """The batched-worlds workload: 4,096 box piles (body counts log-uniform in [8, 2048], friction U(0.2, 0.9)), one
     launch per phase per batch from captured hipGraphs. The autotuner (babylon-avbd_amd/tune.py) first times short
-    runs of the worlds split into 1, 8, 16 and 32 batches and keeps the fastest split whose per-step latency (device
-    time of one batch's step, HIP events) meets the 5 ms target at p99; the timed window then runs that split, the
-    batches stepped one after another each round. Beside it, one world per launch: the first `sample` worlds of the
+    runs of the worlds split into several batches stepped a few at a time (each on its own stream) and keeps the
+    fastest split whose per-step latency (device time of one batch's step, HIP events) meets the 5 ms target at p99;
+    the timed window then runs that split. Beside it, one world per launch: the first `sample` worlds of the
     same draw, each its own world (own stream and graph), stepped round-robin over the same step window."""
     opts = {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True}
     pipe = avbd.createSolverPipeline("hip")
     log = lambda m: print("# " + m, file=sys.stderr, flush=True)  # noqa: E731
-    groups, table = avbd.tune.batch_groups(pipe, avbd.configs.batched_piles, opts, target_p99_ms=target_p99_ms,
-                                           candidates=(1, 8, 16, 32), log=log)
+    (groups, concurrency), table = avbd.tune.batch_groups(pipe, avbd.configs.batched_piles, opts,
+                                                          target_p99_ms=target_p99_ms, log=log)
     scenes = avbd.configs.batched_piles()
     bodies = sum(s.body_count for s in scenes)
     worlds = [pipe.createBatch(part, opts) for part in avbd.tune.split_balanced(scenes, groups)]
     log(f"batch: {len(scenes)} worlds, {bodies} bodies in {len(worlds)} batches, "
         f"{sum(w.deviceBytes() for w in worlds) / 2**30:.1f} GiB on device")
-    avbd.tune.run_groups(worlds, warmup)
+    avbd.tune.run_groups(worlds, warmup, concurrency)
     ms, elapsed = [], 0.0
     done = 0
     while done < steps:
         k = min(50, steps - done)
-        m, e = avbd.tune.run_groups(worlds, k)
+        m, e = avbd.tune.run_groups(worlds, k, concurrency)
         ms += m
         elapsed += e
         done += k
-        log(f"batch step {warmup + done}: {len(worlds)} batches, last step {m[-1]:.2f} ms")
+        log(f"batch step {warmup + done}: {len(worlds)} batches, {concurrency} at a time, last step {m[-1]:.2f} ms")
     counters = [w.readCounters() for w in worlds]
     prof = worlds[0].profileStep()
     for w in worlds:
@@ -117,13 +117,14 @@ def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
     return {
         "workload": "4,096 box piles, body counts log-uniform in [8, 2048], friction U(0.2, 0.9), 10 iterations, "
                     "one launch per phase per batch (hipGraph)",
-        "worlds": len(scenes), "bodies": bodies, "batches": len(worlds), "steps_timed": f"{warmup + 1}-{warmup + steps}",
+        "worlds": len(scenes), "bodies": bodies, "batches": len(worlds), "concurrent_batches": concurrency,
+        "steps_timed": f"{warmup + 1}-{warmup + steps}",
         "body_steps_per_s": agg, "ms_per_round": elapsed * 1e3 / steps,
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
         "step_ms_max": float(ms.max()), "p99_target_ms": target_p99_ms,
-        "tuning": [{"batches": g, "body_steps_per_s": r, "step_ms_p50": p50, "step_ms_p99": p99,
-                    "steps": "11-30"} for g, r, p50, p99 in table],
-        "single_batch_probe": {"body_steps_per_s": single[1], "step_ms_p99": single[3]},
+        "tuning": [{"batches": g, "concurrent": c, "body_steps_per_s": r, "step_ms_p50": p50, "step_ms_p99": p99,
+                    "steps": "61-80"} for g, c, r, p50, p99 in table],
+        "single_batch_probe": {"body_steps_per_s": single[2], "step_ms_p99": single[4]},
         "counters_first_batch": counters[0], "phases_ms_first_batch": prof,
         "one_world_per_launch": {"worlds": sample, "bodies": solo_bodies, "body_steps_per_s": solo_rate,
                                  "sample": f"worlds 0-{sample - 1} of the same draw, each its own world, "

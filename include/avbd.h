@@ -113,6 +113,10 @@ int avbd_world_step(avbd_world *world, uint32_t n);
  * wait: ms[k] = device milliseconds of step k. For step-latency percentiles of worlds and batches.
  */
 int avbd_world_step_timed(avbd_world *world, uint32_t n, float *ms);
+/** avbd_world_step_timed in two halves, so several worlds' steps can be in flight at once: enqueue n event-bracketed
+ * steps (at most 4096), then wait and copy their device times (count = n; capacity must be at least n). */
+int avbd_world_step_timed_begin(avbd_world *world, uint32_t n);
+int avbd_world_step_timed_end(avbd_world *world, float *ms, uint32_t capacity, uint32_t *count);
 /** Wait for enqueued steps. */
 int avbd_world_synchronize(avbd_world *world);
 
