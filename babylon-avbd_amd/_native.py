@@ -89,6 +89,24 @@ class ServerStats(ctypes.Structure):
                 ("resident_worlds", ctypes.c_uint32), ("worlds", ctypes.c_uint32)]
 
 
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("up", ctypes.c_float * 3),
+        ("alpha", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("beta_lin", ctypes.c_float),
+        ("beta_ang", ctypes.c_float),
+        ("iterations", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+COMMAND_WORDS = 12
+CMD_LINEAR_VELOCITY, CMD_ANGULAR_VELOCITY, CMD_IMPULSE, CMD_FORCE, CMD_TELEPORT, CMD_ANGULAR_IMPULSE, CMD_TORQUE = range(7)
+
+
 # name -> (restype, argtypes): every entry point of include/avbd.h
 SIGNATURES = {
     "avbd_last_error": (ctypes.c_char_p, []),
@@ -122,6 +140,10 @@ SIGNATURES = {
     "avbd_world_adapt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),
     "avbd_world_device_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "avbd_world_enqueue_commands": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),
+    "avbd_world_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),
+    "avbd_world_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),
+    "avbd_world_set_filters": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_uint32]),
     "avbd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "avbd_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_server_add_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), u32p]),

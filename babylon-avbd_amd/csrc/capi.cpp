@@ -414,6 +414,66 @@ int avbd_world_device_bytes(const avbd_world *world, uint64_t *out) {
   return AVBD_OK;
 }
 
+int avbd_world_enqueue_commands(avbd_world *world, const uint32_t *commands, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->applyCommands(commands, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_get_params(avbd_world *world, uint32_t world_index, AvbdParams *out) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    if (world_index >= world->engine->worldCount()) throw std::invalid_argument("world index out of range");
+    const WorldParams p = world->engine->readWorldParams()[world_index];
+    out->dt = p.dt;
+    out->gravity = p.gravity;
+    out->up[0] = p.up[0];
+    out->up[1] = p.up[1];
+    out->up[2] = p.up[2];
+    out->alpha = p.alpha;
+    out->gamma = p.gamma;
+    out->beta_lin = p.betaLin;
+    out->beta_ang = p.betaAng;
+    out->iterations = world->engine->iterations();
+    out->flags = p.flags;
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_params(avbd_world *world, uint32_t world_index, const AvbdParams *params) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (!params) throw std::invalid_argument("params is NULL");
+    ParamEdit e{};
+    e.dt = params->dt;
+    e.gravity = params->gravity;
+    e.betaLin = params->beta_lin;
+    e.betaAng = params->beta_ang;
+    e.gamma = params->gamma;
+    e.alpha = params->alpha;
+    e.up[0] = params->up[0];
+    e.up[1] = params->up[1];
+    e.up[2] = params->up[2];
+    e.flags = params->flags;
+    if (params->iterations == 0) throw std::invalid_argument("iterations must be a positive integer");
+    world->engine->setParams(world_index, e);
+    world->engine->setIterations(params->iterations);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_filters(avbd_world *world, const uint32_t *indices, const uint32_t *groups,
+                           const uint32_t *collides_with, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->setFilters(indices, groups, collides_with, n);
+    return AVBD_OK;
+  });
+}
+
 // --- serving loop ----------------------------------------------------------------------------------------------------
 
 #define CHECK_SERVER(s) \

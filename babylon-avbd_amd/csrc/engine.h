@@ -109,6 +109,19 @@ class Engine {
   SavedState save();
   void restore(const SavedState &s);
 
+  /**
+   * Body commands, 12 words each (body, kind, has point, 0 | value xyzw | point xyzw; bodies index the whole batch),
+   * checked on the host, then applied on the stream in submission order: after the steps enqueued before, before the
+   * steps enqueued after. A teleport also drops the warm start of the last step's manifolds on the teleported body.
+   */
+  void applyCommands(const uint32_t *words, uint32_t n);
+  /** Collision filters (groups, collides-with) of n bodies, from the next step on. */
+  void setFilters(const uint32_t *indices, const uint32_t *groups, const uint32_t *masks, uint32_t n);
+  /** Parameters of world k of the batch, from the next step on. */
+  void setParams(uint32_t world, const ParamEdit &e);
+  /** Solver iterations per step (all worlds of the batch). */
+  void setIterations(uint32_t n);
+
   void setColorCap(uint32_t cap);
   void setColorRounds(uint32_t rounds);
   uint32_t colorCap() const { return colorCap_; }
@@ -132,6 +145,8 @@ class Engine {
   template <class T> T *dallocT(size_t count) { return static_cast<T *>(dalloc(count * sizeof(T))); }
   void resetGraphs();
   void pollCounters();
+  /** Copy words to a device buffer on the stream (through one of two pinned staging buffers); returns the buffer. */
+  uint32_t *uploadEdit(const uint32_t *words, size_t n);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -164,6 +179,15 @@ class Engine {
   uint32_t shrinkVotes_ = 0;
   std::vector<hipEvent_t> timingEvents_;
   uint32_t timedSteps_ = 0;
+  // Edits: device upload buffer, pinned staging pair with the events of their last copies, teleport marks per body
+  uint32_t *editDev_ = nullptr;
+  size_t editDevWords_ = 0;
+  uint32_t *editHost_[2] = {nullptr, nullptr};
+  size_t editHostWords_[2] = {0, 0};
+  hipEvent_t editEvents_[2] = {nullptr, nullptr};
+  bool editPending_[2] = {false, false};
+  int editSlot_ = 0;
+  uint32_t *marks_ = nullptr;
 };
 
 /** Largest bounding radius still sized into grid cells (reference/three-avbd/src/avbd3d/gpu/radii.ts:47-55). */

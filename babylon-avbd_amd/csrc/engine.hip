@@ -327,6 +327,11 @@ Engine::~Engine() {
   for (auto &ev : timingEvents_) (void)hipEventDestroy(ev);
   if (hostCounters_) (void)hipHostFree(hostCounters_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (int k = 0; k < 2; k++) {
+    if (editEvents_[k]) (void)hipEventDestroy(editEvents_[k]);
+    if (editHost_[k]) (void)hipHostFree(editHost_[k]);
+  }
+  if (editDev_) (void)hipFree(editDev_);
   for (void *p : contactAllocations_) (void)hipFree(p);
   for (void *p : allocations_) (void)hipFree(p);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -699,6 +704,141 @@ void Engine::restore(const SavedState &s) {
   writeBodies(0, nBodies_, s.bodies.data());
   writeJoints(0, nJoints_, s.joints.data());
   writeState(s.refs.data(), s.colors.data(), s.manifolds, s.contacts, s.steps.data());
+}
+
+// --- edits between steps --------------------------------------------------------------------------------------------
+
+uint32_t *Engine::uploadEdit(const uint32_t *words, size_t n) {
+  const int s = editSlot_;
+  editSlot_ ^= 1;
+  if (!editEvents_[s]) HIP_CHECK(hipEventCreateWithFlags(&editEvents_[s], hipEventDisableTiming));
+  if (editPending_[s]) {
+    HIP_CHECK(hipEventSynchronize(editEvents_[s]));  // the staging buffer's last copy has been read
+    editPending_[s] = false;
+  }
+  if (editHostWords_[s] < n) {
+    if (editHost_[s]) HIP_CHECK(hipHostFree(editHost_[s]));
+    editHost_[s] = nullptr;
+    const size_t cap = std::max<size_t>(2 * n, 1024);
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&editHost_[s]), cap * 4, hipHostMallocDefault));
+    editHostWords_[s] = cap;
+  }
+  std::memcpy(editHost_[s], words, n * 4);
+  if (editDevWords_ < n) {
+    HIP_CHECK(hipStreamSynchronize(stream_));  // edits enqueued earlier may still read the old buffer
+    if (editDev_) {
+      HIP_CHECK(hipFree(editDev_));
+      deviceBytes_ -= editDevWords_ * 4;
+    }
+    editDev_ = nullptr;
+    const size_t cap = std::max<size_t>(2 * n, 1024);
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&editDev_), cap * 4));
+    editDevWords_ = cap;
+    deviceBytes_ += cap * 4;
+  }
+  HIP_CHECK(hipMemcpyAsync(editDev_, editHost_[s], n * 4, hipMemcpyHostToDevice, stream_));
+  HIP_CHECK(hipEventRecord(editEvents_[s], stream_));
+  editPending_[s] = true;
+  return editDev_;
+}
+
+namespace {
+bool finite(const float *v, int n) {
+  for (int k = 0; k < n; k++)
+    if (!std::isfinite(v[k])) return false;
+  return true;
+}
+const char *const kCommandNames[7] = {"linearVelocity", "angularVelocity", "impulse", "force",
+                                      "position",       "angularImpulse",  "torque"};
+}  // namespace
+
+void Engine::applyCommands(const uint32_t *words, uint32_t n) {
+  if (n == 0) return;
+  if (!words) throw std::invalid_argument("commands is NULL");
+  HIP_CHECK(hipSetDevice(device_));
+  bool teleports = false;
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t *c = words + 12ull * k;
+    float f[8];
+    std::memcpy(f, c + 4, sizeof f);
+    if (c[0] >= nBodies_) throw std::invalid_argument("A live 3D GPU body index is required");
+    if (c[1] > 6) throw std::invalid_argument("command kind must be 0 to 6");
+    if (c[2] > 1) throw std::invalid_argument("has_point must be 0 or 1");
+    const uint32_t kind = c[1];
+    const bool hasPoint = c[2] != 0;
+    const std::string name = kCommandNames[kind];
+    if (!finite(f, 3)) throw std::invalid_argument(name + " requires three finite components");
+    if ((kind == 3 || kind == 6) && !finite(f + 3, 1)) throw std::invalid_argument(name + " requires a finite time step");
+    if ((kind == 2 || kind == 3) && hasPoint && !finite(f + 4, 3))
+      throw std::invalid_argument("point requires three finite components");
+    if (kind == 4 && hasPoint) {
+      if (!finite(f + 4, 4)) throw std::invalid_argument("rotation requires four finite components");
+      if (f[4] == 0.f && f[5] == 0.f && f[6] == 0.f && f[7] == 0.f)
+        throw std::invalid_argument("rotation must be nonzero");
+    }
+    teleports = teleports || kind == 4;
+  }
+  // Group by body, each body's commands in submission order (commands of different bodies commute)
+  std::vector<uint32_t> order(n);
+  for (uint32_t k = 0; k < n; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return words[12ull * x] < words[12ull * y]; });
+  std::vector<uint32_t> buf(12ull * n);
+  std::vector<uint32_t> ranges;
+  for (uint32_t k = 0; k < n; k++) {
+    std::memcpy(&buf[12ull * k], words + 12ull * order[k], 48);
+    if (k == 0 || words[12ull * order[k]] != words[12ull * order[k - 1]]) {
+      ranges.push_back(k);
+      ranges.push_back(0);
+    }
+    ranges.back()++;
+  }
+  const uint32_t nRanges = uint32_t(ranges.size() / 2);
+  buf.insert(buf.end(), ranges.begin(), ranges.end());
+  if (teleports && !marks_) marks_ = dallocT<uint32_t>(nBodies_);
+  uint32_t *dev = uploadEdit(buf.data(), buf.size());
+  const uint2 *devRanges = reinterpret_cast<const uint2 *>(dev + 12ull * n);
+  launch_body_commands(base_.bodies, dev, devRanges, nRanges, marks_, stream_);
+  if (teleports) {
+    launch_invalidate_manifolds(views_[parity_].prevManifolds, base_.counters, pairCap_, marks_, stream_);
+    launch_clear_marks(dev, devRanges, nRanges, marks_, stream_);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::setFilters(const uint32_t *indices, const uint32_t *groups, const uint32_t *masks, uint32_t n) {
+  if (n == 0) return;
+  if (!indices || !groups || !masks) throw std::invalid_argument("indices, groups or collides_with is NULL");
+  HIP_CHECK(hipSetDevice(device_));
+  std::vector<uint32_t> buf(3ull * n);
+  for (uint32_t k = 0; k < n; k++) {
+    if (indices[k] >= nBodies_) throw std::invalid_argument("Collision filters require live body indices");
+    buf[3 * k] = indices[k];
+    buf[3 * k + 1] = groups[k];
+    buf[3 * k + 2] = masks[k];
+  }
+  launch_set_filters(base_.filters, uploadEdit(buf.data(), buf.size()), n, stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::setParams(uint32_t world, const ParamEdit &e) {
+  if (world >= nWorlds_) throw std::invalid_argument("world index out of range");
+  if (!(e.dt > 0.f) || !std::isfinite(e.dt)) throw std::invalid_argument("timeStep must be positive");
+  const float values[9] = {e.gravity, e.betaLin, e.betaAng, e.gamma, e.alpha, e.up[0], e.up[1], e.up[2], 0.f};
+  if (!finite(values, 8)) throw std::invalid_argument("parameters must be finite");
+  const float upLen = std::sqrt(e.up[0] * e.up[0] + e.up[1] * e.up[1] + e.up[2] * e.up[2]);
+  if (std::fabs(upLen - 1.f) > 1e-5f) throw std::invalid_argument("up must be a unit vector");
+  HIP_CHECK(hipSetDevice(device_));
+  launch_set_params(base_.worlds + world, e, stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::setIterations(uint32_t n) {
+  if (n == 0) throw std::invalid_argument("iterations must be a positive integer");
+  if (n != iterations_) {
+    iterations_ = n;
+    resetGraphs();
+  }
 }
 
 void Engine::setColorCap(uint32_t cap) {

@@ -69,6 +69,13 @@ struct DeviceWorld {
   uint32_t colorCap;
 };
 
+/** The writable parameters of a world (its step counter, body range and large-body radius are kept). */
+struct ParamEdit {
+  float dt, gravity, betaLin, betaAng, gamma, alpha;
+  float up[3];
+  uint32_t flags;
+};
+
 constexpr int COLOR_WG = 256;
 
 /**

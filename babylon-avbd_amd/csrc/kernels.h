@@ -33,4 +33,17 @@ void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
 void launch_velocities(const DeviceWorld &w, hipStream_t s);
 
+// Edits between steps
+/** Body commands (3 uint4 each) grouped by body: ranges[r] = (first command, count), each body's in submission
+ * order; teleported bodies get marks[body] = 1. */
+void launch_body_commands(Body *bodies, const uint32_t *commands, const uint2 *ranges, uint32_t n, uint32_t *marks,
+                          hipStream_t s);
+/** Zero the count/generation word of the last step's manifolds (counters[C_MANIFOLDS] of them) touching a marked body. */
+void launch_invalidate_manifolds(Manifold *manifolds, const uint32_t *counters, uint32_t capacity,
+                                 const uint32_t *marks, hipStream_t s);
+void launch_clear_marks(const uint32_t *commands, const uint2 *ranges, uint32_t n, uint32_t *marks, hipStream_t s);
+/** words: n (body, groups, collides-with) triples. */
+void launch_set_filters(uint2 *filters, const uint32_t *words, uint32_t n, hipStream_t s);
+void launch_set_params(WorldParams *p, const ParamEdit &e, hipStream_t s);
+
 }  // namespace avbd

@@ -391,6 +391,103 @@ napi_value DeviceBytes(napi_env env, napi_callback_info info) {
   return v;
 }
 
+napi_value EnqueueCommands(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *data;
+  size_t length;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &data, &length, false)) return nullptr;
+  if (length % AVBD_COMMAND_WORDS) {
+    napi_throw_range_error(env, nullptr, "commands are 12 words each");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_enqueue_commands(w, static_cast<const uint32_t *>(data),
+                                               uint32_t(length / AVBD_COMMAND_WORDS)));
+  return nullptr;
+}
+
+napi_value SetFilters(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!args(env, info, 4, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx, *groups, *masks;
+  size_t n, ng, nm;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false) ||
+      !typed(env, argv[2], napi_uint32_array, &groups, &ng, false) ||
+      !typed(env, argv[3], napi_uint32_array, &masks, &nm, false))
+    return nullptr;
+  if (ng != n || nm != n) {
+    napi_throw_range_error(env, nullptr, "Each body index needs a group and collision mask");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_set_filters(w, static_cast<const uint32_t *>(idx), static_cast<const uint32_t *>(groups),
+                                          static_cast<const uint32_t *>(masks), uint32_t(n)));
+  return nullptr;
+}
+
+napi_value GetParams(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t k;
+  if (!w || !u32(env, argv[1], &k)) return nullptr;
+  AvbdParams p{};
+  AVBD_OK_OR_THROW(avbd_world_get_params(w, k, &p));
+  napi_value o, v, up;
+  NAPI_OK(napi_create_object(env, &o));
+  const struct {
+    const char *name;
+    double value;
+  } fields[] = {{"dt", p.dt},       {"gravity", p.gravity},   {"alpha", p.alpha},
+                {"gamma", p.gamma}, {"betaLin", p.beta_lin},  {"betaAng", p.beta_ang},
+                {"iterations", double(p.iterations)}, {"flags", double(p.flags)}};
+  for (const auto &f : fields) {
+    NAPI_OK(napi_create_double(env, f.value, &v));
+    NAPI_OK(napi_set_named_property(env, o, f.name, v));
+  }
+  NAPI_OK(napi_create_array_with_length(env, 3, &up));
+  for (uint32_t i = 0; i < 3; i++) {
+    NAPI_OK(napi_create_double(env, p.up[i], &v));
+    NAPI_OK(napi_set_element(env, up, i, v));
+  }
+  NAPI_OK(napi_set_named_property(env, o, "up", up));
+  return o;
+}
+
+/** setParams(handle, world, {dt, gravity, up, alpha, gamma, betaLin, betaAng, iterations, flags}); missing fields keep
+ * their current values. */
+napi_value SetParams(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t k;
+  if (!w || !u32(env, argv[1], &k)) return nullptr;
+  AvbdParams p{};
+  AVBD_OK_OR_THROW(avbd_world_get_params(w, k, &p));
+  p.dt = float(numberProp(env, argv[2], "dt", p.dt));
+  p.gravity = float(numberProp(env, argv[2], "gravity", p.gravity));
+  p.alpha = float(numberProp(env, argv[2], "alpha", p.alpha));
+  p.gamma = float(numberProp(env, argv[2], "gamma", p.gamma));
+  p.beta_lin = float(numberProp(env, argv[2], "betaLin", p.beta_lin));
+  p.beta_ang = float(numberProp(env, argv[2], "betaAng", p.beta_ang));
+  p.iterations = uint32_t(numberProp(env, argv[2], "iterations", p.iterations));
+  p.flags = uint32_t(numberProp(env, argv[2], "flags", p.flags));
+  napi_value up = prop(env, argv[2], "up");
+  bool isArray = false;
+  if (up) napi_is_array(env, up, &isArray);
+  if (isArray) {
+    for (uint32_t i = 0; i < 3; i++) {
+      napi_value e;
+      double d = p.up[i];
+      if (napi_get_element(env, up, i, &e) == napi_ok) napi_get_value_double(env, e, &d);
+      p.up[i] = float(d);
+    }
+  }
+  AVBD_OK_OR_THROW(avbd_world_set_params(w, k, &p));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   const napi_property_descriptor props[] = {
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
@@ -416,6 +513,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"adapt", nullptr, Adapt, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setColoring", nullptr, SetColoring, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"deviceBytes", nullptr, DeviceBytes, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"enqueueCommands", nullptr, EnqueueCommands, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setFilters", nullptr, SetFilters, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
   if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return nullptr;
   return exports;

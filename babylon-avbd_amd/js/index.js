@@ -291,6 +291,62 @@ function nativeOptions(options) {
   };
 }
 
+// --- body edits (src/native/runtime.js:16-175, solver3D.js:130-213, packedPoses.js:3-72) ---------------------------
+
+const EDIT_KEYS = ['linearVelocity', 'angularVelocity', 'impulse', 'force', '', 'angularImpulse', 'torque'];
+const TELEPORT = 4;
+
+/** An {x, y, z[, w]} object or an array-like of exactly `count` finite numbers (runtime.js:16-32). */
+function components(value, count, name) {
+  const a = value && typeof value.x === 'number'
+    ? [value.x, value.y].concat(count >= 3 ? [value.z] : [], count === 4 ? [value.w] : [])
+    : Array.from(value || []);
+  if (a.length !== count || !a.every(Number.isFinite)) throw Error(`${name} requires ${count} finite components`);
+  return a;
+}
+
+function unitRotation(rotation) {
+  const q = components(rotation, 4, 'rotation');
+  const l = Math.hypot(q[0], q[1], q[2], q[3]);
+  if (!l) throw Error('rotation must be nonzero');
+  return q.map((v) => v / l);
+}
+
+function mask32(v) {
+  if (!Number.isInteger(v) || v < 0 || v > 0xffffffff) throw Error('Collision masks must be unsigned 32-bit integers');
+  return v;
+}
+
+/** Growable 12-word command records: body, kind, has point, 0 | value xyzw (f32) | point xyzw (f32). */
+class CommandQueue {
+  constructor() { this.count = 0; this.u32 = new Uint32Array(64 * 12); this.f32 = new Float32Array(this.u32.buffer); }
+  reserve(n) {
+    if (n * 12 <= this.u32.length) return;
+    const u = new Uint32Array(Math.max(n, 2 * this.u32.length / 12) * 12);
+    u.set(this.u32.subarray(0, this.count * 12));
+    this.u32 = u;
+    this.f32 = new Float32Array(u.buffer);
+  }
+  append(index, kind, value, point, hasPoint) {
+    this.reserve(this.count + 1);
+    const o = this.count++ * 12;
+    this.u32.fill(0, o, o + 12);
+    this.u32[o] = index;
+    this.u32[o + 1] = kind;
+    this.u32[o + 2] = hasPoint ? 1 : 0;
+    for (let k = 0; k < 4; k++) {
+      this.f32[o + 4 + k] = value[k] === undefined ? 0 : value[k];
+      this.f32[o + 8 + k] = point && point[k] !== undefined ? point[k] : 0;
+    }
+  }
+  extend(other) {
+    this.reserve(this.count + other.count);
+    this.u32.set(other.u32.subarray(0, other.count * 12), this.count * 12);
+    this.count += other.count;
+  }
+  records() { return this.u32.subarray(0, this.count * 12); }
+}
+
 /** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */
 class HipSolver3D {
   constructor(scenes, options, device) {
@@ -309,12 +365,159 @@ class HipSolver3D {
     this.params = Object.assign({}, list[0].params, { up: list[0].up.slice() });
     this.timeAccumulator = 0;
     this.destroyed = false;
+    this.firstBodies = [];
+    this.dts = [];
+    let first = 0;
+    for (const s of list) {
+      this.firstBodies.push(first);
+      this.dts.push(s.params.dt);
+      first += s.bodyCount;
+    }
+    this.commands = new CommandQueue();
   }
 
   alive() { if (this.destroyed) throw Error('The solver has been destroyed'); }
 
-  /** Submit one step without waiting (solver3D.js:350-359). */
-  step() { this.alive(); native().step(this.handle, 1); }
+  /** Alive, with queued edits submitted (the reference flushes them before steps and reads). */
+  ready() { this.alive(); this.flushEdits(); }
+
+  /** Submit one step without waiting (solver3D.js:350-359); queued edits go first. */
+  step() { this.ready(); native().step(this.handle, 1); }
+
+  liveIndex(index) {
+    this.alive();
+    if (!Number.isInteger(index) || index < 0 || index >= this.bodyCount) throw Error('A live 3D GPU body index is required');
+    return index;
+  }
+
+  dtOf(index) {
+    let k = this.firstBodies.length - 1;
+    while (k > 0 && this.firstBodies[k] > index) k--;
+    return this.dts[k];
+  }
+
+  queueBodyCommand(index, kind, value, point, hasPoint) {
+    this.commands.append(this.liveIndex(index), kind, value, point, hasPoint);
+    return this;
+  }
+  setLinearVelocity(index, v) { return this.queueBodyCommand(index, 0, components(v, 3, 'velocity')); }
+  setAngularVelocity(index, v) { return this.queueBodyCommand(index, 1, components(v, 3, 'angular velocity')); }
+  applyImpulse(index, v, point) {
+    const p = point === undefined ? undefined : components(point, 3, 'point');
+    return this.queueBodyCommand(index, 2, components(v, 3, 'impulse'), p, p !== undefined);
+  }
+  applyForce(index, v, point) {
+    const p = point === undefined ? undefined : components(point, 3, 'point');
+    return this.queueBodyCommand(index, 3, components(v, 3, 'force').concat([this.dtOf(this.liveIndex(index))]), p,
+      p !== undefined);
+  }
+  applyAngularImpulse(index, v) { return this.queueBodyCommand(index, 5, components(v, 3, 'angular impulse')); }
+  applyTorque(index, v) {
+    return this.queueBodyCommand(index, 6, components(v, 3, 'torque').concat([this.dtOf(this.liveIndex(index))]));
+  }
+  teleport(index, position, rotation) {
+    const p = components(position, 3, 'position');
+    const q = rotation === undefined ? undefined : unitRotation(rotation);
+    return this.queueBodyCommand(index, TELEPORT, p, q, q !== undefined);
+  }
+
+  /** Packed teleports: unique ascending Uint32 slots, Float32 xyz positions, optional Float32 xyzw rotations. */
+  setPoses(indices, positions, rotations) {
+    this.alive();
+    if (!(indices instanceof Uint32Array) || !(positions instanceof Float32Array) ||
+        (rotations !== undefined && !(rotations instanceof Float32Array)))
+      throw Error('Pose batches require Uint32 indices and Float32 pose arrays');
+    if (positions.length !== indices.length * 3 || (rotations && rotations.length !== indices.length * 4))
+      throw Error('Provide xyz positions and optional xyzw rotations per body');
+    let previous = -1;
+    for (let i = 0; i < indices.length; i++) {
+      this.liveIndex(indices[i]);
+      if (indices[i] <= previous) throw Error('Pose batch slots must be unique and ascending');
+      previous = indices[i];
+    }
+    if (!positions.every(Number.isFinite)) throw Error('Pose positions must be finite');
+    const inverse = [];
+    for (let i = 0; rotations && i < indices.length; i++) {
+      const l = Math.hypot(rotations[4 * i], rotations[4 * i + 1], rotations[4 * i + 2], rotations[4 * i + 3]);
+      if (!Number.isFinite(l) || l <= 0) throw Error('Pose rotations must be finite and nonzero');
+      inverse.push(1 / l);
+    }
+    const staged = new CommandQueue();
+    for (let i = 0; i < indices.length; i++) {
+      const p = [positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]];
+      const q = rotations ? [0, 1, 2, 3].map((k) => rotations[4 * i + k] * inverse[i]) : [0, 0, 0, 1];
+      staged.append(indices[i], TELEPORT, p, q, !!rotations);
+    }
+    this.commands.extend(staged);
+    return this;
+  }
+
+  /** editBodies (runtime.js:45-175): the batch is checked whole before anything is queued; per record a teleport
+   * precedes velocity replacements, impulses and forces; group / collidesWith become setFilters. */
+  editBodies(edits) {
+    this.alive();
+    const staged = new CommandQueue();
+    const properties = [];
+    for (const edit of edits) {
+      const index = this.liveIndex(edit.body);
+      for (const key of ['isTrigger', 'restitution', 'allowSleep'])
+        if (edit[key] !== undefined) throw Error(`${key} edits are not available on the hip backend`);
+      if (edit.group !== undefined || edit.collidesWith !== undefined) {
+        if (edit.collidesWith !== undefined && edit.group === undefined)
+          throw Error('A collision-mask edit requires group as well');
+        properties.push([index, mask32(edit.group), edit.collidesWith === undefined ? 0xffffffff : mask32(edit.collidesWith)]);
+      }
+      const point = edit.worldPoint === undefined ? undefined : components(edit.worldPoint, 3, 'worldPoint');
+      if (edit.position !== undefined) {
+        const q = edit.rotation === undefined ? undefined : unitRotation(edit.rotation);
+        staged.append(index, TELEPORT, components(edit.position, 3, 'position'), q, q !== undefined);
+      } else if (edit.rotation !== undefined) {
+        throw Error('A rotation edit requires position; use teleport for a full pose');
+      }
+      for (let kind = 0; kind < EDIT_KEYS.length; kind++) {
+        if (kind === TELEPORT || edit[EDIT_KEYS[kind]] === undefined) continue;
+        const v = components(edit[EDIT_KEYS[kind]], 3, EDIT_KEYS[kind]);
+        const withPoint = (kind === 2 || kind === 3) && point !== undefined;
+        if (kind === 3 || kind === 6) v.push(this.dtOf(index));
+        staged.append(index, kind, v, withPoint ? point : undefined, withPoint);
+      }
+    }
+    this.commands.extend(staged);
+    for (const [index, group, collides] of properties) this.setFilters([index], [group], [collides]);
+    return this;
+  }
+
+  /** setFilters(indices, groups, collidesWith) (appGpuSolver3D.js:958-983). */
+  setFilters(indices, groups, collidesWith) {
+    this.alive();
+    if (indices.length !== groups.length || indices.length !== collidesWith.length)
+      throw Error('Each body index needs a group and collision mask');
+    const idx = Uint32Array.from(indices, (i) => this.liveIndex(i));
+    native().setFilters(this.handle, idx, Uint32Array.from(groups, mask32), Uint32Array.from(collidesWith, mask32));
+    return this;
+  }
+
+  /** Submit queued body commands without stepping (solver3D.js:208-214). */
+  flushEdits() {
+    this.alive();
+    if (this.commands.count) {
+      const words = this.commands.records().slice();
+      this.commands.count = 0;
+      native().enqueueCommands(this.handle, words);
+    }
+    return this;
+  }
+
+  /** Step parameters of world k of the batch; setParams changes them from the next step on. */
+  getParams(k) { this.alive(); return native().getParams(this.handle, k || 0); }
+  setParams(changes, k) {
+    this.ready();
+    native().setParams(this.handle, k || 0, changes || {});
+    const p = native().getParams(this.handle, k || 0);
+    this.dts[k || 0] = p.dt;
+    if (!k) Object.assign(this.params, { dt: p.dt, gravity: p.gravity, iterations: p.iterations });
+    return this;
+  }
 
   /** Fixed-step accumulator; returns the steps submitted (runtime.js:255-272). */
   advance(elapsed, options) {
@@ -337,19 +540,19 @@ class HipSolver3D {
   }
 
   /** Per-step device milliseconds of n steps (waits). */
-  stepTimed(n) { this.alive(); return native().stepTimed(this.handle, n); }
+  stepTimed(n) { this.ready(); return native().stepTimed(this.handle, n); }
 
   /** All bodies, 40 f32 each, in slot order (solver.ts:1421-1423). */
-  readBodies() { this.alive(); return Promise.resolve(native().readBodies(this.handle, 0, this.bodyCount)); }
+  readBodies() { this.ready(); return Promise.resolve(native().readBodies(this.handle, 0, this.bodyCount)); }
 
   /** readSelectedBodies (solver3D.js:223-230): 40 or (posesOnly) 8 f32 per body. */
   readSelectedBodies(indices, options) {
-    this.alive();
+    this.ready();
     const idx = indices instanceof Uint32Array ? indices : Uint32Array.from(indices);
     return Promise.resolve(native().readSelected(this.handle, idx, !!(options && options.posesOnly)));
   }
 
-  readJoints() { this.alive(); return Promise.resolve(native().readJoints(this.handle, 0, this.jointCount)); }
+  readJoints() { this.ready(); return Promise.resolve(native().readJoints(this.handle, 0, this.jointCount)); }
 
   /** {pairs, contacts, manifolds, joints, overflow, clashes, colors, sleeping} (world.js:672-692). */
   readCounters() { this.alive(); return Promise.resolve(native().readCounters(this.handle)); }
@@ -359,7 +562,7 @@ class HipSolver3D {
 
   /** readContactList (solver.ts:1435-1469): {a, b, feature, stick, rA, rB, pen, lam, normal} per contact point. */
   readContactList() {
-    this.alive();
+    this.ready();
     const { manifolds, contacts } = native().readContacts(this.handle);
     const f32 = new Float32Array(contacts.buffer);
     const mf32 = new Float32Array(manifolds.buffer);
@@ -435,4 +638,4 @@ async function createSolverPipeline(options) {
   };
 }
 
-module.exports = { createSolverPipeline, AvbdScene3D, HipSolver3D, native, FLAGS_DEFAULT };
+module.exports = { createSolverPipeline, AvbdScene3D, HipSolver3D, CommandQueue, native, FLAGS_DEFAULT };

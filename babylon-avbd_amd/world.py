@@ -7,7 +7,7 @@ import math
 
 import numpy as np
 
-from . import _native
+from . import _native, edits as _edits
 from ._native import check, ptr
 from .scene import AvbdScene3D
 
@@ -91,12 +91,19 @@ class HipWorld:
         check(lib.avbd_world_joint_count(self._h, ctypes.byref(n)))
         self.jointCount = n.value
         self.worldCount = len(scenes)
+        self._dts = [float(sc.params["dt"]) for sc in scenes]
+        self._pending = _edits.CommandStaging()
 
     # --- lifetime ---------------------------------------------------------------------------------------------------
 
     def _alive(self):
         if self.destroyed:
             raise RuntimeError("The solver has been destroyed")
+
+    def _ready(self):
+        """Alive, with queued edits submitted (the reference flushes them before steps and reads)."""
+        self._alive()
+        self.flushEdits()
 
     def destroy(self):
         if self.destroyed:
@@ -115,8 +122,8 @@ class HipWorld:
     # --- stepping ---------------------------------------------------------------------------------------------------
 
     def step(self, n=1):
-        """Submit n steps without waiting (solver3D.js:350-359)."""
-        self._alive()
+        """Submit n steps without waiting (solver3D.js:350-359); queued edits go first."""
+        self._ready()
         check(self._lib.avbd_world_step(self._h, int(n)))
 
     def advance(self, elapsed, maxSubSteps=6, maxFrameTime=0.05):
@@ -137,14 +144,14 @@ class HipWorld:
 
     def stepTimed(self, n=1):
         """Run n steps and wait; returns each step's device milliseconds (HIP events around each step)."""
-        self._alive()
+        self._ready()
         ms = np.zeros(int(n), np.float32)
         check(self._lib.avbd_world_step_timed(self._h, int(n), ptr(ms, ctypes.c_float)))
         return ms
 
     def stepTimedBegin(self, n=1):
         """Enqueue n event-bracketed steps without waiting (pair with stepTimedEnd)."""
-        self._alive()
+        self._ready()
         check(self._lib.avbd_world_step_timed_begin(self._h, int(n)))
         self._timed = int(n)
 
@@ -159,26 +166,147 @@ class HipWorld:
         return ms[:got.value]
 
     def synchronize(self):
-        self._alive()
+        self._ready()
         check(self._lib.avbd_world_synchronize(self._h))
+
+    # --- edits between steps (solver3D.js:130-213, runtime.js:45-175) --------------------------------------------
+
+    def liveIndex(self, index):
+        self._alive()
+        if isinstance(index, bool) or not isinstance(index, (int, np.integer)) or not 0 <= index < self.bodyCount:
+            raise ValueError("A live 3D GPU body index is required")
+        return int(index)
+
+    def _world_of(self, index):
+        return int(np.searchsorted(self.world_first_body, index, side="right") - 1)
+
+    def _dt_of(self, index):
+        return self._dts[self._world_of(index)]
+
+    def queueBodyCommand(self, index, kind, value, point=None, hasPoint=False):
+        self._pending.append(self.liveIndex(index), kind, value, point, hasPoint)
+        return self
+
+    def setLinearVelocity(self, index, v):
+        return self.queueBodyCommand(index, _edits.LINEAR_VELOCITY, _edits.components(v, 3, "velocity"))
+
+    def setAngularVelocity(self, index, v):
+        return self.queueBodyCommand(index, _edits.ANGULAR_VELOCITY, _edits.components(v, 3, "angular velocity"))
+
+    def applyImpulse(self, index, v, point=None):
+        p = None if point is None else _edits.components(point, 3, "point")
+        return self.queueBodyCommand(index, _edits.IMPULSE, _edits.components(v, 3, "impulse"), p, p is not None)
+
+    def applyForce(self, index, v, point=None):
+        """A force over the next step: the impulse v × dt (dt of the body's world)."""
+        p = None if point is None else _edits.components(point, 3, "point")
+        index = self.liveIndex(index)
+        return self.queueBodyCommand(index, _edits.FORCE, _edits.components(v, 3, "force") + [self._dt_of(index)], p,
+                                     p is not None)
+
+    def applyAngularImpulse(self, index, v):
+        return self.queueBodyCommand(index, _edits.ANGULAR_IMPULSE, _edits.components(v, 3, "angular impulse"))
+
+    def applyTorque(self, index, v):
+        index = self.liveIndex(index)
+        return self.queueBodyCommand(index, _edits.TORQUE, _edits.components(v, 3, "torque") + [self._dt_of(index)])
+
+    def teleport(self, index, position, rotation=None):
+        p = _edits.components(position, 3, "position")
+        q = None if rotation is None else _edits.unit_rotation(rotation)
+        return self.queueBodyCommand(index, _edits.TELEPORT, p, q, q is not None)
+
+    def setPoses(self, indices, positions, rotations=None):
+        """Packed teleports (solver3D.js:188-206): unique ascending u32 slots, f32 xyz positions, optional f32 xyzw."""
+        self._alive()
+        staged = _edits.CommandStaging()
+        _edits.stage_poses(indices, positions, rotations, self.bodyCount, staged)
+        self._pending.extend(staged)
+        return self
+
+    def editBodies(self, edits):
+        """editBodies (runtime.js:45-175): the whole batch is checked before anything is queued; per record a teleport
+        precedes velocity replacements, impulses and forces; group/collidesWith edits become setFilters."""
+        self._alive()
+        staged = _edits.CommandStaging()
+        properties = _edits.stage_edits(edits, self.liveIndex, self._dt_of, staged)
+        self._pending.extend(staged)
+        for index, group, collides in properties:
+            self.setFilters([index], [group], [collides])
+        return self
+
+    def setFilters(self, indices, groups, collidesWith):
+        """setFilters (appGpuSolver3D.js:958-983): collision group and mask per body, from the next step on."""
+        self._alive()
+        if not (len(indices) == len(groups) == len(collidesWith)):
+            raise ValueError("Each body index needs a group and collision mask")
+        idx = np.ascontiguousarray([self.liveIndex(i) for i in indices], np.uint32)
+        g = np.ascontiguousarray([_edits.mask(int(v) if isinstance(v, np.integer) else v) for v in groups], np.uint32)
+        m = np.ascontiguousarray([_edits.mask(int(v) if isinstance(v, np.integer) else v) for v in collidesWith],
+                                 np.uint32)
+        check(self._lib.avbd_world_set_filters(self._h, ptr(idx, ctypes.c_uint32), ptr(g, ctypes.c_uint32),
+                                               ptr(m, ctypes.c_uint32), len(idx)))
+        return self
+
+    def flushEdits(self):
+        """Submit queued body commands without stepping (solver3D.js:208-214)."""
+        self._alive()
+        if self._pending.count:
+            words = np.ascontiguousarray(self._pending.records())
+            self._pending.clear()
+            check(self._lib.avbd_world_enqueue_commands(self._h, ptr(words, ctypes.c_uint32), len(words)))
+        return self
+
+    def getParams(self, world=0):
+        """Step parameters of world `world` of the batch (GpuParams3D, solver.ts:350)."""
+        self._alive()
+        p = _native.Params()
+        check(self._lib.avbd_world_get_params(self._h, int(world), ctypes.byref(p)))
+        return dict(dt=p.dt, gravity=p.gravity, up=list(p.up), alpha=p.alpha, gamma=p.gamma, betaLin=p.beta_lin,
+                    betaAng=p.beta_ang, iterations=p.iterations, flags=p.flags)
+
+    def setParams(self, world=0, **changes):
+        """Change step parameters (dt, gravity, up, alpha, gamma, betaLin, betaAng, iterations, flags) of world
+        `world` from the next step on; iterations apply to every world of a batch."""
+        self._ready()
+        unknown = set(changes) - {"dt", "gravity", "up", "alpha", "gamma", "betaLin", "betaAng", "iterations", "flags"}
+        if unknown:
+            raise ValueError(f"Unknown parameters: {sorted(unknown)}")
+        cur = self.getParams(world)
+        cur.update(changes)
+        p = _native.Params()
+        p.dt = cur["dt"]
+        p.gravity = cur["gravity"]
+        p.up[:] = [float(v) for v in cur["up"]]
+        p.alpha = cur["alpha"]
+        p.gamma = cur["gamma"]
+        p.beta_lin = cur["betaLin"]
+        p.beta_ang = cur["betaAng"]
+        p.iterations = int(cur["iterations"])
+        p.flags = int(cur["flags"])
+        check(self._lib.avbd_world_set_params(self._h, int(world), ctypes.byref(p)))
+        self._dts[int(world)] = float(np.float32(p.dt))
+        if int(world) == 0:
+            self.params.update(dt=self._dts[0], gravity=p.gravity, iterations=p.iterations)
+        return self
 
     # --- reads ----------------------------------------------------------------------------------------------------
 
     def readBodies(self):
         """All bodies, 40 f32 each, in slot order (solver.ts:1421-1423)."""
-        self._alive()
+        self._ready()
         out = np.zeros((self.bodyCount, 40), np.float32)
         check(self._lib.avbd_world_read_bodies(self._h, 0, self.bodyCount, ptr(out, ctypes.c_float)))
         return out
 
     def writeBodies(self, first, records):
-        self._alive()
+        self._ready()
         records = np.ascontiguousarray(records, np.float32).reshape(-1, 40)
         check(self._lib.avbd_world_write_bodies(self._h, int(first), len(records), ptr(records, ctypes.c_float)))
 
     def readSelectedBodies(self, indices, posesOnly=False):
         """readSelectedBodies (solver3D.js:223-230): 40 or (posesOnly) 8 f32 per body."""
-        self._alive()
+        self._ready()
         idx = np.ascontiguousarray(indices, np.uint32)
         out = np.zeros((len(idx), 8 if posesOnly else 40), np.float32)
         check(self._lib.avbd_world_read_selected(self._h, ptr(idx, ctypes.c_uint32), len(idx), 1 if posesOnly else 0,
@@ -186,14 +314,14 @@ class HipWorld:
         return out
 
     def readJoints(self):
-        self._alive()
+        self._ready()
         out = np.zeros((self.jointCount, 32), np.float32)
         if self.jointCount:
             check(self._lib.avbd_world_read_joints(self._h, 0, self.jointCount, ptr(out, ctypes.c_float)))
         return out
 
     def writeJoints(self, first, records):
-        self._alive()
+        self._ready()
         records = np.ascontiguousarray(records, np.float32).reshape(-1, 32)
         check(self._lib.avbd_world_write_joints(self._h, int(first), len(records), ptr(records, ctypes.c_float)))
 
@@ -216,7 +344,7 @@ class HipWorld:
 
     def readContactRecords(self):
         """The last step's manifolds (m, 8) u32 and contacts (c, 16) u32, contacts packed densely per manifold."""
-        self._alive()
+        self._ready()
         m = ctypes.c_uint32()
         c = ctypes.c_uint32()
         check(self._lib.avbd_world_read_contacts(self._h, None, 0, ctypes.byref(m), None, 0, ctypes.byref(c)))
@@ -277,7 +405,7 @@ class HipWorld:
 
     def writeState(self, state):
         """seedFrom (solver.ts:905-948): bodies, joints and the persistent state of readState()."""
-        self._alive()
+        self._ready()
         self.writeBodies(0, state["bodies"])
         if self.jointCount:
             self.writeJoints(0, state["joints"])
@@ -306,7 +434,7 @@ class HipWorld:
 
     def profileStep(self):
         """One step with HIP events per phase and per primal/dual launch (ms)."""
-        self._alive()
+        self._ready()
         out = np.zeros(13, np.float32)
         check(self._lib.avbd_world_profile_step(self._h, ptr(out, ctypes.c_float)))
         keys = ("total", "collision", "adjacency", "coloring", "warmStart", "primal", "dual", "velocities",
@@ -357,6 +485,11 @@ class SolverPipeline:
         w = HipWorld(scenes, opts)
         self.worlds.add(w)
         return w
+
+    def _ready(self):
+        """Alive, with queued edits submitted (the reference flushes them before steps and reads)."""
+        self._alive()
+        self.flushEdits()
 
     def destroy(self):
         if self.destroyed:

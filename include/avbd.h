@@ -180,6 +180,54 @@ int avbd_world_profile_step(avbd_world *world, float *out);
 /** Device bytes held by the world. */
 int avbd_world_device_bytes(const avbd_world *world, uint64_t *out);
 
+/* ---- Edits between steps ---------------------------------------------------------------------------------------- */
+
+/**
+ * Body command record, 12 words (CommandStaging / writeCommand, src/gpu/bodyCommands2D.js:16-58; struct Command of
+ * src/native/bodyCommands3D.wgsl:2): u32 body, u32 kind, u32 has_point (0 or 1), u32 0 | f32 value[4] | f32 point[4].
+ *   0 linear velocity := value.xyz          1 angular velocity := value.xyz
+ *   2 impulse value.xyz at point.xyz (at the centre without a point)
+ *   3 force value.xyz over value.w seconds (the step's dt; applied as the impulse value.xyz × value.w)
+ *   4 teleport to value.xyz, rotation := point.xyzw if has_point (unit quaternion, normalised by the caller)
+ *   5 angular impulse value.xyz             6 torque value.xyz over value.w seconds
+ * Impulses, forces and torques leave static bodies (mass 0) unchanged.
+ */
+#define AVBD_COMMAND_WORDS 12
+#define AVBD_CMD_LINEAR_VELOCITY 0u
+#define AVBD_CMD_ANGULAR_VELOCITY 1u
+#define AVBD_CMD_IMPULSE 2u
+#define AVBD_CMD_FORCE 3u
+#define AVBD_CMD_TELEPORT 4u
+#define AVBD_CMD_ANGULAR_IMPULSE 5u
+#define AVBD_CMD_TORQUE 6u
+
+/**
+ * Queue n body commands (the commands of NativeGpuSolver3D.setLinearVelocity … teleport, setPoses and editBodies,
+ * src/native/solver3D.js:130-206, src/native/runtime.js:45-175, applied by bodyCommands3D.wgsl:14-50). All n are
+ * checked first (live body index, kind, finite components, nonzero rotation); on an error none is applied. They are
+ * applied on the world's stream after the steps enqueued before this call and before those enqueued after it, each
+ * body's commands in submission order. A teleport also resets the body's step-start and inertial poses and drops the
+ * warm start of the last step's manifolds that touch it. Bodies index the whole batch.
+ */
+int avbd_world_enqueue_commands(avbd_world *world, const uint32_t *commands, uint32_t n);
+
+/** Step parameters of one world of a batch (GpuParams3D, solver.ts:350, written before every step by writeParams,
+ * solver.ts:1214-1254). iterations is shared by all worlds of a batch. */
+typedef struct AvbdParams {
+  float dt, gravity;
+  float up[3];
+  float alpha, gamma, beta_lin, beta_ang;
+  uint32_t iterations;
+  uint32_t flags; /* AVBD_FLAG_* */
+} AvbdParams;
+
+int avbd_world_get_params(avbd_world *world, uint32_t world_index, AvbdParams *out);
+/** Takes effect from the next step on (a changed iteration count re-records the step graphs). */
+int avbd_world_set_params(avbd_world *world, uint32_t world_index, const AvbdParams *params);
+/** setFilters(indices, groups, collidesWith) (appGpuSolver3D.js:958-983): collision groups and masks of n bodies. */
+int avbd_world_set_filters(avbd_world *world, const uint32_t *indices, const uint32_t *groups,
+                           const uint32_t *collides_with, uint32_t n);
+
 /* ---- Serving loop ------------------------------------------------------------------------------------------------
  * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):
  * a server takes step/rollout requests for many registered worlds as they arrive, keeps world state resident in HBM

@@ -343,6 +343,48 @@ void gs_read_refs(void *p, float *out) {
   }
 }
 
+// --- edits between steps ---------------------------------------------------------------------------------------------
+
+/** Body commands (12 words each, World::applyCommands), in submission order. */
+void gs_apply_commands(void *p, const uint32_t *words, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->applyCommands(words, n);
+  if (h->d) h->d->applyCommands(words, n);
+}
+
+/** World k's parameters from 10 words: dt, gravity, betaLin, betaAng, gamma, alpha, up xyz, flags (bits); the step
+ * counter is kept. */
+void gs_set_params(void *p, uint32_t k, const float *v) {
+  auto *h = static_cast<Handle *>(p);
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<WorldParams> &params = pass == 0 ? h->f->params : h->d ? h->d->params : h->f->params;
+    if (pass == 1 && !h->d) break;
+    WorldParams &q = params[k];
+    q.dt = v[0]; q.gravity = v[1]; q.betaLin = v[2]; q.betaAng = v[3]; q.gamma = v[4]; q.alpha = v[5];
+    q.up[0] = v[6]; q.up[1] = v[7]; q.up[2] = v[8];
+    q.flags = floatToBits(v[9]);
+  }
+}
+
+void gs_set_iterations(void *p, int iterations) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->iterations = iterations;
+  if (h->d) h->d->iterations = iterations;
+}
+
+/** Collision filters (groups, collides-with) of n bodies. */
+void gs_set_filters(void *p, const uint32_t *indices, const uint32_t *groups, const uint32_t *masks, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  for (uint32_t k = 0; k < n; k++) {
+    h->f->filters[2 * size_t(indices[k])] = groups[k];
+    h->f->filters[2 * size_t(indices[k]) + 1] = masks[k];
+    if (h->d) {
+      h->d->filters[2 * size_t(indices[k])] = groups[k];
+      h->d->filters[2 * size_t(indices[k]) + 1] = masks[k];
+    }
+  }
+}
+
 // --- unit-level probes (for tests of single functions against the upstream restatement) -----------------------------
 
 /**

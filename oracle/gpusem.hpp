@@ -967,6 +967,64 @@ template <class R> class World {
     collisionAndTopology();
     solve();
   }
+
+  // --- edits between steps -----------------------------------------------------------------------------------------
+
+  /** ω change of an impulse about the centre: world → body frame, ÷ max(moment, 1e-10), back (bodyCommands3D.wgsl:9-12). */
+  static V3<R> angularImpulse(const Body<R> &b, V3<R> impulse) {
+    V3<R> local = rotate(conj(b.rot), impulse);
+    const R floor = R(1e-10f);
+    V3<R> scaled{local.x / rmax(b.moment.x, floor), local.y / rmax(b.moment.y, floor), local.z / rmax(b.moment.z, floor)};
+    return rotate(b.rot, scaled);
+  }
+
+  /**
+   * Body commands, 12 words each: body, kind, has point, 0 | value xyzw (f32) | point xyzw (f32), applied in
+   * submission order (bodyCommands3D.wgsl:14-36 applies each body's commands in order, and commands of different
+   * bodies commute). Kinds: 0 set linear velocity, 1 set angular velocity, 2 impulse (at the point if given),
+   * 3 force × value.w (= dt), 4 teleport (position; rotation = point if given), 5 angular impulse, 6 torque × value.w.
+   * Impulses leave static bodies (mass 0) unchanged. A teleport resets the step-start and inertial poses, and the last
+   * step's manifolds that touch a teleported body lose their warm start (count and generation word = 0,
+   * bodyCommands3D.wgsl:38-44).
+   */
+  void applyCommands(const uint32_t *words, uint32_t count) {
+    std::vector<uint8_t> teleported(n(), 0);
+    for (uint32_t k = 0; k < count; k++) {
+      const uint32_t *c = words + 12 * k;
+      float f[8];
+      std::memcpy(f, c + 4, sizeof f);
+      Body<R> &b = bodies[c[0]];
+      const uint32_t kind = c[1];
+      const bool hasPoint = c[2] != 0;
+      V3<R> value{R(f[0]), R(f[1]), R(f[2])};
+      if (kind == 0) {
+        b.vel = value;
+      } else if (kind == 1) {
+        b.angVel = value;
+      } else if (kind == 2 || kind == 3) {
+        if (b.mass > 0) {
+          V3<R> impulse = scale(value, kind == 3 ? R(f[3]) : R(1));
+          b.vel = add(b.vel, divs(impulse, b.mass));
+          if (hasPoint) {
+            V3<R> point{R(f[4]), R(f[5]), R(f[6])};
+            b.angVel = add(b.angVel, angularImpulse(b, cross(sub(point, b.pos), impulse)));
+          }
+        }
+      } else if (kind == 5 || kind == 6) {
+        if (b.mass > 0) b.angVel = add(b.angVel, angularImpulse(b, scale(value, kind == 6 ? R(f[3]) : R(1))));
+      } else if (kind == 4) {
+        b.pos = value;
+        if (hasPoint) b.rot = Q4<R>(R(f[4]), R(f[5]), R(f[6]), R(f[7]));
+        b.ipos = b.pos;
+        b.irot = b.rot;
+        b.inertialPos = b.pos;
+        b.inertialRot = b.rot;
+        teleported[c[0]] = 1;
+      }
+    }
+    for (Manifold<R> &m : manifolds)
+      if (m.count() != 0 && (teleported[m.a] || teleported[m.b])) m.countw = 0;
+  }
 };
 
 }  // namespace gpu

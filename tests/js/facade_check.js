@@ -20,6 +20,22 @@ function build(spec) {
   return s;
 }
 
+/** Apply edit calls ([method, ...args]) to a world; typed-array arguments of setPoses arrive as plain arrays. */
+function applyEdits(world, ops) {
+  for (const [method, ...a] of ops) {
+    if (method === 'setPoses') world.setPoses(Uint32Array.from(a[0]), Float32Array.from(a[1]), a[2] ? Float32Array.from(a[2]) : undefined);
+    else world[method](...a);
+  }
+}
+
+/** A world object with only what the edit staging reads (no device): for comparing staged commands on the CPU. */
+function stagingOnly(bodyCount, firstBodies, dts) {
+  const w = Object.create(avbd.HipSolver3D.prototype);
+  Object.assign(w, { destroyed: false, bodyCount, firstBodies, dts, commands: new avbd.CommandQueue() });
+  w.setFilters = function (indices, groups, masks) { (this.filterCalls = this.filterCalls || []).push([indices, groups, masks]); return this; };
+  return w;
+}
+
 const bits = (t) => (t ? Array.from(new Uint32Array(t.buffer, t.byteOffset, t.length)) : null);
 
 async function errorOf(f) {
@@ -53,6 +69,21 @@ async function main() {
     out.create = avbd.native().deviceCount() > 0 ? 'has-gpu' : await errorOf(() => pipe.createSolver(s));
     out.editableAfterFailure = await errorOf(() => s.addBox([1, 1, 1]));
     console.log(JSON.stringify(out));
+  } else if (mode === 'stage') {
+    const spec = specs;
+    const w = stagingOnly(spec.bodyCount, spec.firstBodies, spec.dts);
+    let error = null;
+    try { applyEdits(w, spec.ops); } catch (e) { error = String(e.message); }
+    console.log(JSON.stringify({ words: Array.from(w.commands.records()), filters: w.filterCalls || [], error }));
+  } else if (mode === 'edits') {
+    const pipe = await avbd.createSolverPipeline({ backend: 'hip' });
+    const spec = specs;
+    const w = pipe.createSolver(build(spec.scene), { colorCap: 32, colorRounds: 16 });
+    for (let k = 0; k < spec.before; k++) w.step();
+    applyEdits(w, spec.ops);
+    for (let k = 0; k < spec.after; k++) w.step();
+    console.log(JSON.stringify({ bodies: bits(await w.readBodies()), pairs: Array.from(await w.readPairs()) }));
+    pipe.destroy();
   } else if (mode === 'step') {
     const pipe = await avbd.createSolverPipeline({ backend: 'hip' });
     const steps = Number(process.argv[4]);

@@ -70,6 +70,10 @@ def lib():
             "gs_probe_solve6": (None, [f64p, f64p, ctypes.c_double, f64p]),
             "gs_probe_pair_contacts": (ctypes.c_int, [ctypes.c_int, f32p, f32p, ctypes.c_int, u32p, f64p, f64p]),
             "gs_probe_priority": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "gs_apply_commands": (None, [vp, u32p, ctypes.c_uint32]),
+            "gs_set_params": (None, [vp, ctypes.c_uint32, f32p]),
+            "gs_set_iterations": (None, [vp, ctypes.c_int]),
+            "gs_set_filters": (None, [vp, u32p, u32p, u32p, ctypes.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -225,3 +229,27 @@ class OracleWorld:
         out = np.zeros((self.n, 8), np.float32)
         lib().gs_read_refs(self.h, P(out, ctypes.c_float))
         return out
+
+    def apply_commands(self, words):
+        """Body commands (n x 12 u32 records), in submission order."""
+        w = np.ascontiguousarray(words, np.uint32).reshape(-1, 12)
+        lib().gs_apply_commands(self.h, P(w, ctypes.c_uint32), len(w))
+
+    def set_params(self, k, dt, gravity, beta_lin, beta_ang, gamma, alpha, up, flags):
+        """World k's parameters (its step counter is kept)."""
+        v = np.array([dt, gravity, beta_lin, beta_ang, gamma, alpha, *up, 0], np.float32)
+        v[9:10].view(np.uint32)[0] = flags
+        self.params[k, :10] = v
+        lib().gs_set_params(self.h, k, P(v, ctypes.c_float))
+
+    def set_iterations(self, n):
+        self._iters = int(n)
+        lib().gs_set_iterations(self.h, int(n))
+
+    def set_filters(self, indices, groups, masks):
+        idx = np.ascontiguousarray(indices, np.uint32)
+        g = np.ascontiguousarray(groups, np.uint32)
+        m = np.ascontiguousarray(masks, np.uint32)
+        self.filters[idx, 0] = g
+        self.filters[idx, 1] = m
+        lib().gs_set_filters(self.h, P(idx, ctypes.c_uint32), P(g, ctypes.c_uint32), P(m, ctypes.c_uint32), len(idx))

@@ -124,3 +124,35 @@ def test_js_world_steps_like_the_python_world(avbd, tmp_path):
     assert js["poses"] == sel.view(np.uint32).ravel().tolist()
     assert js["contacts"] == world.readCounters()["contacts"]
     world.destroy()
+
+
+@pytest.mark.gpu
+def test_js_edits_land_like_the_python_edits(avbd, tmp_path):
+    """The same edit calls through the JS facade and the Python facade, between steps: bit-identical bodies."""
+    ops = [["box", [30, 1, 30], {"density": 0, "friction": 0.5, "position": [0, -0.5, 0]}]]
+    for k in range(27):
+        ops.append(["box", [1, 1, 1], {"position": [(k % 3) * 1.1 - 1.1, 0.5 + 1.05 * (k // 9), (k // 3 % 3) * 1.1 - 1.1]}])
+    scene = {"params": {"gravity": -9.81}, "ops": ops}
+    edits = [["setLinearVelocity", 3, [0, 4, 0]], ["applyImpulse", 5, [2, 0, 0], [0.2, 1.0, 0.1]],
+             ["applyForce", 6, [0, 80, 0]], ["teleport", 7, [4, 3, 4], [0.1, 0.2, 0.3, 0.9]],
+             ["editBodies", [{"body": 8, "position": [-4, 2, 0], "linearVelocity": [1, 0, 0], "torque": [0, 3, 0]},
+                             {"body": 9, "group": 2, "collidesWith": 1}]],
+             ["setPoses", [10, 11], [0, 6, 0, 1, 7, 1], [0, 0, 0, 1, 0.5, 0.5, 0.5, 0.5]],
+             ["applyAngularImpulse", 12, [0, 0, 1]], ["setAngularVelocity", 13, [3, 0, 0]]]
+    spec = {"scene": scene, "before": 20, "after": 30, "ops": edits}
+    f = tmp_path / "spec.json"
+    f.write_text(json.dumps(spec))
+    r = subprocess.run([NODE, str(CHECK), "edits", str(f)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    world = avbd.createSolverPipeline("hip").createSolver(build_py(avbd, scene), dict(colorCap=32, colorRounds=16))
+    world.step(20)
+    for method, *a in edits:
+        if method == "setPoses":
+            world.setPoses(np.array(a[0], np.uint32), np.array(a[1], np.float32), np.array(a[2], np.float32))
+        else:
+            getattr(world, method)(*a)
+    world.step(30)
+    assert js["bodies"] == world.readBodies().view(np.uint32).ravel().tolist()
+    assert js["pairs"] == world.readPairs().ravel().tolist()
+    world.destroy()
