@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import OracleWorld
-from test_gpu_parity import compare_step
+from test_gpu_parity import _manifold_map, compare_step
 
 pytestmark = pytest.mark.gpu
 
@@ -75,8 +75,13 @@ def test_commands_are_bit_identical_to_the_oracle_and_steps_stay_so(avbd):
     oracle.apply_commands(cmds)
     _bodies_equal(world, oracle, "after the commands")
     # the teleported bodies' manifolds lost their warm start exactly as the oracle's did
-    mf, _ = world.readContactRecords()
-    assert np.array_equal(mf, oracle.manifolds()), "manifold records after the invalidation differ"
+    mg = _manifold_map(*world.readContactRecords())
+    mc = _manifold_map(oracle.manifolds(), oracle.contact_words())
+    assert set(mg) == set(mc)
+    assert any(mc[k][0] == 0 for k in mc), "no manifold lost its warm start"
+    for k in mc:
+        assert mg[k][0] == mc[k][0] and mg[k][1] == mc[k][1], f"manifold {k} differs after the invalidation"
+        assert np.array_equal(mg[k][2], mc[k][2]), f"contacts of manifold {k} differ"
     world.step(40)
     oracle.step(40)
     got, want = world.readBodies(), oracle.bodies().astype(np.float32)
@@ -233,6 +238,11 @@ def test_filters_change_the_pair_set_like_the_oracle(avbd):
     pairs = world.readPairs()
     assert stats["pairs"] > 0
     assert not ((pairs[:, 0] % 2 == 1) & (pairs[:, 1] % 2 == 1)).any(), "filtered pairs were emitted"
-    with pytest.raises(native.AvbdError):
+    with pytest.raises(ValueError, match="live"):
         world.setFilters([n], [1], [1])
+    one = np.array([n], np.uint32)
+    u = ctypes.c_uint32
+    with pytest.raises(native.AvbdError, match="live body indices"):
+        native.check(world._lib.avbd_world_set_filters(world._h, native.ptr(one, u), native.ptr(one, u),
+                                                       native.ptr(one, u), 1))
     world.destroy()
