@@ -504,13 +504,20 @@ AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
   w.joints[j] = k;
 }
 
-AVBD_D void dualManifold(const DeviceWorld &w, uint32_t m) {
-  Manifold mf = w.manifolds[m];
+/**
+ * Contact points slot, slot + DUAL_SLOTS, ... of manifold m (dual, wgsl-solve.ts:474-535): the penalty ramp where the
+ * row is active, λ = the clamped force, and the stick flag. Points are independent of each other, so the manifold's
+ * points are spread over DUAL_SLOTS threads.
+ */
+constexpr uint32_t DUAL_SLOTS = 4;
+AVBD_D void dualManifold(const DeviceWorld &w, uint32_t m, uint32_t slot) {
+  const Manifold mf = w.manifolds[m];
+  const uint32_t count = mf.ids.w & 15u;
+  if (slot >= count) return;
   const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
   PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
-  uint32_t end = mf.ids.z + (mf.ids.w & 15u);
-  for (uint32_t c = mf.ids.z; c < end; c++) {
+  for (uint32_t c = mf.ids.z + slot; c < mf.ids.z + count; c += DUAL_SLOTS) {
     Contact k = w.contacts[c];
     Eval e = evalContact(k, basis, mf.geo.w, PA, PB, p.alpha);
     f3 pen = xyz(k.pen);
@@ -528,12 +535,14 @@ AVBD_D void dualManifold(const DeviceWorld &w, uint32_t m) {
   }
 }
 
+/** Joints first (one thread each), then DUAL_SLOTS threads per manifold. */
 __global__ void k_dual(DeviceWorld w) {
   if (held(w)) return;
-  uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
-  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
-    if (id < w.nJoints) dualJoint(w, id);
-    else dualManifold(w, id - w.nJoints);
+  const uint32_t J = w.nJoints;
+  const uint64_t total = uint64_t(J) + uint64_t(w.counters[C_MANIFOLDS]) * DUAL_SLOTS;
+  for (uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; id < total; id += uint64_t(gridDim.x) * blockDim.x) {
+    if (id < J) dualJoint(w, uint32_t(id));
+    else dualManifold(w, uint32_t((id - J) / DUAL_SLOTS), uint32_t((id - J) % DUAL_SLOTS));
   }
 }
 
@@ -569,7 +578,7 @@ void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies
   k_primal<<<blocks(uint64_t(maxColorBodies) * LANES, PRIMAL_BLOCK, 4096), PRIMAL_BLOCK, 0, s>>>(w, color);
 }
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
-  k_dual<<<blocks(uint64_t(w.nJoints) + manifoldCapacity, 64, 8192), 64, 0, s>>>(w);
+  k_dual<<<blocks(uint64_t(w.nJoints) + uint64_t(manifoldCapacity) * DUAL_SLOTS, 256, 16384), 256, 0, s>>>(w);
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {
   k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
