@@ -228,7 +228,9 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   const uint32_t tiles = (n + 255) / 256;
   bvh_.nodeCount = plan_.nodeCount;
   bvh_.nodes = dallocT<BvhNode>(plan_.nodeCount);
-  bvh_.trav = dallocT<BvhNode>(plan_.nodeCount);
+  bvh_.pairs = dallocT<BvhPair>(plan_.nodeCount);
+  if (plan_.height > 250) throw std::length_error("BVH too deep for the traversal stack");
+  bvh_.stackDepth = plan_.height;
   bvh_.order = dallocT<uint32_t>(n);
   bvh_.rank = dallocT<uint32_t>(n);
   bvh_.maxPos = dallocT<uint32_t>(plan_.nodeCount);
@@ -236,7 +238,6 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   bvh_.candCount = dallocT<uint32_t>(n);
   bvh_.candMask = dallocT<uint32_t>(uint64_t(n) * BVH_CAND_WORDS);
   bvh_.parent = dallocT<uint32_t>(plan_.nodeCount);
-  bvh_.escape = dallocT<uint32_t>(plan_.nodeCount);
   bvh_.keys = dallocT<uint64_t>(n);
   bvh_.keysAlt = dallocT<uint64_t>(n);
   bvh_.vals = dallocT<uint32_t>(n);

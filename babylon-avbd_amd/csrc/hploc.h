@@ -12,6 +12,12 @@ struct BvhNode {
   float4 lo, hi;
 };
 
+/** An internal node as the pair search reads it: both children's boxes, tags (node id, or body | LEAF) in the lo.w
+ * words and subtree maxPos in the hi.w words, so one 64-byte load decides both descents. */
+struct BvhPair {
+  float4 aLo, aHi, bLo, bHi;
+};
+
 /** One treelet of a level: its input clusters [inStart, inStart + count), output slot, first node it creates. */
 struct BvhGroup {
   uint32_t inStart, count, outIndex, nodeBase;
@@ -27,11 +33,12 @@ struct BvhPlan {
   std::vector<BvhGroup> groups;
   uint32_t nodeCount = 0;
   uint32_t sortPasses = 4;
+  uint32_t height = 1;  // bound on the tree height: a level's treelet of k clusters adds at most k - 1 levels
 };
 
 struct Bvh {
   BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
-  BvhNode *trav;       // traversal copy: lo + left/leaf tag, hi + escape rope
+  BvhPair *pairs;      // per internal node: its children's boxes, tags and maxPos (leaves' entries unused)
   uint32_t *order;     // bodies in Morton order (the last build's)
   uint32_t *rank;      // per body: its position in `order`
   uint32_t *maxPos;    // per node: highest rank of a dynamic leaf below it, | MAXPOS_STATIC if a static leaf is below
@@ -39,7 +46,6 @@ struct Bvh {
   uint32_t *candCount; // per body: its query's candidate count (may exceed BVH_CAND)
   uint32_t *candMask;  // per body: BVH_CAND / 32 words, bit k = candidate k passed the pair predicate
   uint32_t *parent;    // per node
-  uint32_t *escape;    // per node
   uint64_t *keys, *keysAlt;
   uint32_t *vals, *valsAlt;
   uint32_t *clustersA, *clustersB;
@@ -48,6 +54,7 @@ struct Bvh {
   uint32_t *sortHist;  // 256 × tiles + 1
   BvhGroup *groups;    // every level's groups
   uint32_t nodeCount;
+  uint32_t stackDepth; // traversal stack entries per query (the plan's height bound)
 };
 
 /** Candidate leaves kept per query; a query with more is answered by a second walk that tests as it goes. */
@@ -77,6 +84,7 @@ inline BvhPlan makeBvhPlan(const std::vector<uint32_t> &counts) {
     BvhLevel lev{uint32_t(plan.groups.size()), 0};
     uint32_t inOff = 0, outOff = 0;
     std::vector<uint32_t> next(c.size());
+    uint32_t widest = 1;
     for (size_t k = 0; k < c.size(); k++) {
       uint32_t g = (c[k] + 15) / 16;
       for (uint32_t q = 0; q < g; q++) {
@@ -84,12 +92,14 @@ inline BvhPlan makeBvhPlan(const std::vector<uint32_t> &counts) {
         plan.groups.push_back(BvhGroup{inOff + 16 * q, count, outOff + q, nodeBase});
         nodeBase += count - 1;
         lev.groupCount++;
+        widest = std::max(widest, count);
       }
       inOff += c[k];
       outOff += g;
       next[k] = g;
     }
     plan.levels.push_back(lev);
+    plan.height += widest - 1;
     c = next;
   }
   return plan;

@@ -1,8 +1,8 @@
 // H-PLOC bounding-volume hierarchy broadphase on gfx950: per-step leaf refit, Morton ordering (world id above the
 // 30-bit code, so each world's leaves stay contiguous), a stable LSD radix sort, 16-cluster treelets merged by mutual
-// nearest neighbours under the surface-area cost, level by level until one root per world, escape ropes for stackless
-// traversal, and queries that feed the same pair predicate as the grid (so the pair set does not depend on the
-// broadphase). Semantics of the reference's portable variant: src/gpu/hplocpp.wgsl:35-268, hplocSort.wgsl:11-48,
+// nearest neighbours under the surface-area cost, level by level until one root per world, child-pair node records
+// (both children's boxes in one 64-byte load) for a depth-first traversal with an LDS stack, and queries that feed the
+// same pair predicate as the grid (so the pair set does not depend on the broadphase). Semantics of the reference's portable variant: src/gpu/hplocpp.wgsl:35-268, hplocSort.wgsl:11-48,
 // gpuHploc.js:24-56 (findTreePairs), 207-375; rebuild every 64 steps, refit every step (docs/HPLOC.md:14-33).
 //
 // The level plan (which clusters each treelet merges, where its nodes go) depends only on the body count of each
@@ -283,61 +283,65 @@ __global__ void k_bvh_refit_level(Bvh t, const BvhGroup *groups, uint32_t nGroup
   }
 }
 
-/** Escape rope of each node: the right sibling of the first ancestor it descends from on the left (END at a root). */
-__global__ void k_bvh_ropes(Bvh t, uint32_t nNodes) {
-  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nNodes) return;
-  uint32_t at = id, escape = END;
-  for (int depth = 0; depth < 256; depth++) {
-    uint32_t p = t.parent[at];
-    if (p == END) break;
-    BvhNode parent = t.nodes[p];
-    if (__float_as_uint(parent.lo.w) == at) {
-      escape = __float_as_uint(parent.hi.w);
-      break;
-    }
-    at = p;
-  }
-  t.escape[id] = escape;
-}
-
-/** Traversal copy of the tree: lo + (left child, or body index | LEAF for a leaf), hi + escape rope. */
+/** Tag of a child in a pair record: its node id, or for a leaf its body index | LEAF (leaf ids are body indices). */
 constexpr uint32_t LEAF = 0x80000000u;
 
-__global__ void k_bvh_trav(Bvh t, uint32_t nNodes) {
-  uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nNodes) return;
+/** Pair records of the internal nodes [N, nodeCount) from the refitted boxes and maxPos. */
+__global__ void k_bvh_pairnodes(Bvh t, uint32_t nLeaves) {
+  uint32_t id = nLeaves + blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= t.nodeCount) return;
   BvhNode n = t.nodes[id];
-  uint32_t left = __float_as_uint(n.lo.w), right = __float_as_uint(n.hi.w);
-  uint32_t tag = right == END ? (left | LEAF) : left;
-  t.trav[id] = BvhNode{mk4(xyz(n.lo), __uint_as_float(tag)), mk4(xyz(n.hi), __uint_as_float(t.escape[id]))};
+  uint32_t l = __float_as_uint(n.lo.w), r = __float_as_uint(n.hi.w);
+  BvhNode A = t.nodes[l], B = t.nodes[r];
+  uint32_t tagA = l < nLeaves ? (l | LEAF) : l, tagB = r < nLeaves ? (r | LEAF) : r;
+  t.pairs[id] = BvhPair{mk4(xyz(A.lo), __uint_as_float(tagA)), mk4(xyz(A.hi), __uint_as_float(t.maxPos[l])),
+                        mk4(xyz(B.lo), __uint_as_float(tagB)), mk4(xyz(B.hi), __uint_as_float(t.maxPos[r]))};
 }
 
+/** A query's traversal stack in LDS: entry d of lane x at stack[d * blockDim.x + x] (no bank conflicts). */
+extern __shared__ uint32_t bvhStack[];
+
 /**
- * Walk body i's world tree (stackless, escape ropes) and call visit(j) for each leaf j whose padded box overlaps i's
- * (gpuHploc.js:37-55). Every static leaf is visited, but of two dynamic bodies only the one earlier in Morton order
- * (rank q) visits the other: subtrees holding no static leaf and no dynamic leaf later than q are skipped (maxPos),
- * so each dynamic pair comes up once.
+ * Walk body i's world tree depth first and call visit(j) for each leaf j whose padded box overlaps i's
+ * (gpuHploc.js:37-55). Each internal node is one 64-byte load holding both children's boxes; a child leaf is
+ * visited on the spot, an internal child descended into (the other one kept on the stack, which never holds more
+ * entries than the tree is high). Every static leaf is visited, but of two dynamic bodies only the one earlier in
+ * Morton order (rank q) visits the other: subtrees holding no static leaf and no dynamic leaf later than q are
+ * skipped (maxPos), so each dynamic pair comes up once. A one-body world has no pair.
  */
 template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint32_t i, uint32_t q, F &&visit) {
-  BvhNode self = t.trav[i];
-  f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
-  uint32_t at = t.roots[w.bodyWorld[i]];
-  // Stackless traversal visits each node at most twice; the bound only guards against a malformed tree
-  for (uint32_t guard = 0; at != END && guard < 2 * t.nodeCount + 2; guard++) {
-    BvhNode nd = t.trav[at];
-    uint32_t mp = t.maxPos[at];
-    uint32_t tag = __float_as_uint(nd.lo.w);
-    if (nd.hi.x < qlo.x || nd.hi.y < qlo.y || nd.hi.z < qlo.z || nd.lo.x > qhi.x || nd.lo.y > qhi.y ||
-        nd.lo.z > qhi.z || !((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q)) {
-      at = __float_as_uint(nd.hi.w);
-      continue;
+  const BvhNode self = t.nodes[i];
+  const f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
+  const uint32_t root = t.roots[w.bodyWorld[i]];
+  if (root < w.nBodies) return;
+  auto hit = [&](float4 lo, float4 hi) {
+    const uint32_t mp = __float_as_uint(hi.w);
+    return !(hi.x < qlo.x || hi.y < qlo.y || hi.z < qlo.z || lo.x > qhi.x || lo.y > qhi.y || lo.z > qhi.z) &&
+           ((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q);
+  };
+  uint32_t *stack = bvhStack + threadIdx.x;
+  uint32_t at = root, sp = 0;
+  // Each internal node is entered at most once; the bound only guards against a malformed tree
+  for (uint32_t guard = 0; guard < t.nodeCount; guard++) {
+    const BvhPair nd = t.pairs[at];
+    const uint32_t tagA = __float_as_uint(nd.aLo.w), tagB = __float_as_uint(nd.bLo.w);
+    bool a = hit(nd.aLo, nd.aHi), b = hit(nd.bLo, nd.bHi);
+    if (a && (tagA & LEAF)) {
+      visit(tagA & ~LEAF);
+      a = false;
     }
-    if (tag & LEAF) {
-      visit(tag & ~LEAF);
-      at = __float_as_uint(nd.hi.w);
+    if (b && (tagB & LEAF)) {
+      visit(tagB & ~LEAF);
+      b = false;
+    }
+    if (a) {
+      if (b && sp < t.stackDepth) stack[(sp++) * blockDim.x] = tagB;
+      at = tagA;
+    } else if (b) {
+      at = tagB;
     } else {
-      at = tag;
+      if (sp == 0) break;
+      at = stack[(--sp) * blockDim.x];
     }
   }
 }
@@ -473,13 +477,12 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   }
   // The last level's outputs are the world roots (a one-body world's root is its leaf: its plan has a copy level)
   k_bvh_copy<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t.roots, in, w.nWorlds);
-  k_bvh_ropes<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
   k_bvh_copy<<<blocks(n, 256), 256, 0, s>>>(t.order, v0, n);
   k_bvh_rank<<<blocks(n, 256), 256, 0, s>>>(w, t);
   k_bvh_leafmax<<<blocks(n, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
-  k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
+  k_bvh_pairnodes<<<blocks(plan.nodeCount - n, 256), 256, 0, s>>>(t, n);
 }
 
 void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
@@ -487,15 +490,16 @@ void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   k_bvh_leafmax<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
-  k_bvh_trav<<<blocks(plan.nodeCount, 256), 256, 0, s>>>(t, plan.nodeCount);
+  k_bvh_pairnodes<<<blocks(plan.nodeCount - w.nBodies, 256), 256, 0, s>>>(t, w.nBodies);
 }
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
-  k_bvh_cand<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  const size_t stackBytes = size_t(64) * t.stackDepth * sizeof(uint32_t);
+  k_bvh_cand<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
-  k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
-  k_bvh_write<<<blocks(w.nBodies, 64), 64, 0, s>>>(w, t);
+  k_bvh_write<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
 }
 
 }  // namespace avbd
