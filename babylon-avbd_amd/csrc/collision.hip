@@ -260,7 +260,11 @@ AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t p
   return true;
 }
 
-__global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
+/**
+ * Contacts of every pair slot (wgsl-collision.ts:617-808). kRounded = false compiles the box/box path only (a batch
+ * without spheres or capsules), which keeps the kernel's code small enough to stay in the instruction cache.
+ */
+template <bool kRounded> __global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
   uint32_t P = w.counters[C_PAIRS];
   for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
     uint2 pr = w.pairs[pslot];
@@ -275,8 +279,8 @@ __global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
     sat.n = mk3(0.f, 0.f, 0.f);
     Found found;
     found.count = 0;
-    bool sphereA = BA.angVel.w == SHAPE_SPHERE, sphereB = BB.angVel.w == SHAPE_SPHERE;
-    bool capsuleA = BA.angVel.w == SHAPE_CAPSULE, capsuleB = BB.angVel.w == SHAPE_CAPSULE;
+    const bool sphereA = kRounded && BA.angVel.w == SHAPE_SPHERE, sphereB = kRounded && BB.angVel.w == SHAPE_SPHERE;
+    const bool capsuleA = kRounded && BA.angVel.w == SHAPE_CAPSULE, capsuleB = kRounded && BB.angVel.w == SHAPE_CAPSULE;
     if (capsuleA || capsuleB) {
       // The capsule is the first argument; a capsule B flips the result back (capsuleCollision.js:17-29)
       if (capsuleA && capsuleB) {
@@ -438,7 +442,10 @@ void launch_pairs(const DeviceWorld &w, hipStream_t s) {
 }
 void launch_pairs_total(const DeviceWorld &w, hipStream_t s) { k_pairs_total<<<1, 1, 0, s>>>(w); }
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
-  k_narrowphase<<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
+  if (w.roundedShapes)
+    k_narrowphase<true><<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
+  else
+    k_narrowphase<false><<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
   k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);
   launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s);
   k_manifold_compact<<<blocks(w.pairCapacity, 256, 4096), 256, 0, s>>>(w);

@@ -152,6 +152,9 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.nJoints = nJoints_;
   w.nWorlds = nWorlds_;
   w.noCollideCount = noCollideCount_;
+  w.roundedShapes = 0;
+  for (const Body &b : bodies)
+    if (b.angVel.w == SHAPE_SPHERE || b.angVel.w == SHAPE_CAPSULE) w.roundedShapes = 1;
   w.bodies = dallocT<Body>(nBodies_);
   w.bodyWorld = dallocT<uint32_t>(nBodies_);
   w.worlds = dallocT<WorldParams>(nWorlds_);
@@ -541,6 +544,14 @@ void Engine::readBodies(uint32_t first, uint32_t count, Body *out) {
 }
 void Engine::writeBodies(uint32_t first, uint32_t count, const Body *in) {
   if (size_t(first) + count > nBodies_) throw std::out_of_range("body range");
+  if (!base_.roundedShapes)
+    for (uint32_t k = 0; k < count; k++)
+      if (in[k].angVel.w == SHAPE_SPHERE || in[k].angVel.w == SHAPE_CAPSULE) {
+        // A sphere or capsule appears: the narrowphase switches to its full variant from the next step on
+        base_.roundedShapes = views_[0].roundedShapes = views_[1].roundedShapes = 1;
+        resetGraphs();
+        break;
+      }
   HIP_CHECK(hipMemcpyAsync(base_.bodies + first, in, size_t(count) * sizeof(Body), hipMemcpyHostToDevice, stream_));
   synchronize();
 }

@@ -18,6 +18,7 @@ struct DeviceWorld {
   uint32_t *noCollide;   // (hi, lo, joint slot or ~0) triples, sorted by (hi, lo)
   uint32_t *ncStart;     // nBodies + 1: body hi's triples are noCollide[ncStart[hi] .. ncStart[hi + 1])
   uint32_t nBodies, nJoints, nWorlds, noCollideCount;
+  uint32_t roundedShapes;  // any sphere or capsule in the batch (else the narrowphase runs its box-only variant)
   uint32_t *counters;    // COUNTER_WORDS
   uint32_t *scanScratch; // block sums of the multi-level prefix scan
 
