@@ -264,7 +264,7 @@ AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t p
  * Contacts of every pair slot (wgsl-collision.ts:617-808). kRounded = false compiles the box/box path only (a batch
  * without spheres or capsules), which keeps the kernel's code small enough to stay in the instruction cache.
  */
-template <bool kRounded> __global__ void __launch_bounds__(64) k_narrowphase(DeviceWorld w) {
+template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narrowphase(DeviceWorld w) {
   uint32_t P = w.counters[C_PAIRS];
   for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
     uint2 pr = w.pairs[pslot];
@@ -292,11 +292,7 @@ template <bool kRounded> __global__ void __launch_bounds__(64) k_narrowphase(Dev
         Capsule C = capsuleOf(BB);
         found = sphereA ? capsuleSphere(C, A.c, A.h.x, sat) : capsuleBox(C, A, BA.rot, sat);
         sat.n = neg(sat.n);
-        for (int c = 0; c < found.count; c++) {
-          f3 t = found.xA[c];
-          found.xA[c] = found.xB[c];
-          found.xB[c] = t;
-        }
+        swapFoundSides(found);
       }
     } else if (sphereA && sphereB) {
       found = sphereSphere(A, B, sat);
@@ -334,13 +330,16 @@ template <bool kRounded> __global__ void __launch_bounds__(64) k_narrowphase(Dev
     float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
     float pmin = contactPenaltyMin(w, p, a, b);
     for (int i = 0; i < found.count; i++) {
-      f3 rA = qrotate(qconj(qA), sub(found.xA[i], A.c));
-      f3 rB = qrotate(qconj(qB), sub(found.xB[i], B.c));
+      f3 xA, xB;
+      uint32_t feature;
+      foundPoint(found, i, xA, xB, feature);
+      f3 rA = qrotate(qconj(qA), sub(xA, A.c));
+      f3 rB = qrotate(qconj(qB), sub(xB, B.c));
       f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);
       uint32_t stick = 0;
       int j = -1;
       for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
-        if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == found.feature[i]) {
+        if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == feature) {
           j = int(c);
           break;
         }
@@ -372,7 +371,7 @@ template <bool kRounded> __global__ void __launch_bounds__(64) k_narrowphase(Dev
       pen = clampPen(pen, p.gamma);
       pen.x = rmax(pen.x, pmin);
       Contact k;
-      k.rA = mk4(rA, bitsf(found.feature[i] | stick));
+      k.rA = mk4(rA, bitsf(feature | stick));
       k.rB = mk4(rB, c0x);
       k.pen = mk4(pen, c0y);
       k.lam = mk4(lam, c0z);
@@ -443,9 +442,9 @@ void launch_pairs(const DeviceWorld &w, hipStream_t s) {
 void launch_pairs_total(const DeviceWorld &w, hipStream_t s) { k_pairs_total<<<1, 1, 0, s>>>(w); }
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
   if (w.roundedShapes)
-    k_narrowphase<true><<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
+    k_narrowphase<true><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
   else
-    k_narrowphase<false><<<blocks(w.pairCapacity, 64, 8192), 64, 0, s>>>(w);
+    k_narrowphase<false><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
   k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);
   launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s);
   k_manifold_compact<<<blocks(w.pairCapacity, 256, 4096), 256, 0, s>>>(w);

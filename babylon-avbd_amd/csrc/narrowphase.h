@@ -9,18 +9,20 @@ namespace avbd {
 
 constexpr uint32_t AXIS_FACE_A = 0, AXIS_FACE_B = 1, AXIS_EDGE = 2;
 
+/** An oriented box: centre, half extents and the three world axes (named members, so that choosing an axis by a
+ * runtime index is a register select and never an indexed stack access). */
 struct Box {
   f3 c, h;
-  f3 ax[3];
+  f3 ax0, ax1, ax2;
 };
 
 AVBD_D Box makeBox(f3 pos, float4 rot, f3 size) {
   Box b;
   b.c = pos;
   b.h = scale(size, 0.5f);
-  b.ax[0] = qrotate(rot, mk3(1.f, 0.f, 0.f));
-  b.ax[1] = qrotate(rot, mk3(0.f, 1.f, 0.f));
-  b.ax[2] = qrotate(rot, mk3(0.f, 0.f, 1.f));
+  b.ax0 = qrotate(rot, mk3(1.f, 0.f, 0.f));
+  b.ax1 = qrotate(rot, mk3(0.f, 1.f, 0.f));
+  b.ax2 = qrotate(rot, mk3(0.f, 0.f, 1.f));
   return b;
 }
 
@@ -38,21 +40,61 @@ struct Found {
   int count;
 };
 
+/**
+ * Append a point unless it lies within the merge distance of one already found. The loops run over every slot with a
+ * guard, so the arrays are indexed by constants and stay in registers.
+ */
 AVBD_D void addFound(Found &f, f3 xA, f3 xB, uint32_t feature) {
   f3 mid = scale(add(xA, xB), 0.5f);
-  for (int i = 0; i < f.count; i++) {
-    f3 d = sub(mid, scale(add(f.xA[i], f.xB[i]), 0.5f));
-    if (dot(d, d) < CONTACT_MERGE_DIST_SQ) return;
+  bool near = false;
+#pragma unroll
+  for (int i = 0; i < MAX_CONTACTS; i++) {
+    if (i < f.count) {
+      f3 d = sub(mid, scale(add(f.xA[i], f.xB[i]), 0.5f));
+      if (dot(d, d) < CONTACT_MERGE_DIST_SQ) near = true;
+    }
   }
-  if (f.count >= MAX_CONTACTS) return;
-  f.feature[f.count] = feature;
-  f.xA[f.count] = xA;
-  f.xB[f.count] = xB;
+  if (near || f.count >= MAX_CONTACTS) return;
+#pragma unroll
+  for (int i = 0; i < MAX_CONTACTS; i++) {
+    if (i == f.count) {
+      f.feature[i] = feature;
+      f.xA[i] = xA;
+      f.xB[i] = xB;
+    }
+  }
   f.count++;
 }
 
+/** Point i of a Found (i < count), read through a select over the slots (no indexed register access). */
+AVBD_D void foundPoint(const Found &f, int i, f3 &xA, f3 &xB, uint32_t &feature) {
+  xA = f.xA[0];
+  xB = f.xB[0];
+  feature = f.feature[0];
+#pragma unroll
+  for (int k = 1; k < MAX_CONTACTS; k++) {
+    if (i == k) {
+      xA = f.xA[k];
+      xB = f.xB[k];
+      feature = f.feature[k];
+    }
+  }
+}
+
+/** Swap the two sides of every point (a capsule argument reversed). */
+AVBD_D void swapFoundSides(Found &f) {
+#pragma unroll
+  for (int k = 0; k < MAX_CONTACTS; k++) {
+    if (k < f.count) {
+      f3 t = f.xA[k];
+      f.xA[k] = f.xB[k];
+      f.xB[k] = t;
+    }
+  }
+}
+
 AVBD_D float projRadius(const Box &b, f3 n) {
-  return b.h.x * fabsf(dot(n, b.ax[0])) + b.h.y * fabsf(dot(n, b.ax[1])) + b.h.z * fabsf(dot(n, b.ax[2]));
+  return b.h.x * fabsf(dot(n, b.ax0)) + b.h.y * fabsf(dot(n, b.ax1)) + b.h.z * fabsf(dot(n, b.ax2));
 }
 
 AVBD_D bool testAxis(const Box &A, const Box &B, f3 delta, f3 axis, uint32_t kind, uint32_t ia, uint32_t ib,
@@ -76,21 +118,40 @@ AVBD_D bool testAxis(const Box &A, const Box &B, f3 delta, f3 axis, uint32_t kin
 }
 
 AVBD_D f3 support(const Box &b, f3 dir) {
-  float sx = dot(dir, b.ax[0]) >= 0.f ? 1.f : -1.f;
-  float sy = dot(dir, b.ax[1]) >= 0.f ? 1.f : -1.f;
-  float sz = dot(dir, b.ax[2]) >= 0.f ? 1.f : -1.f;
-  f3 p = addScaled(b.c, b.ax[0], b.h.x * sx);
-  p = addScaled(p, b.ax[1], b.h.y * sy);
-  return addScaled(p, b.ax[2], b.h.z * sz);
+  float sx = dot(dir, b.ax0) >= 0.f ? 1.f : -1.f;
+  float sy = dot(dir, b.ax1) >= 0.f ? 1.f : -1.f;
+  float sz = dot(dir, b.ax2) >= 0.f ? 1.f : -1.f;
+  f3 p = addScaled(b.c, b.ax0, b.h.x * sx);
+  p = addScaled(p, b.ax1, b.h.y * sy);
+  return addScaled(p, b.ax2, b.h.z * sz);
 }
 
-AVBD_D int clipPlane(const f3 *src, int n, f3 *dst, f3 pn, float offset) {
+/**
+ * Clipping polygons live in LDS, one slice per lane (the narrowphase runs 64-lane blocks): component c of vertex k of
+ * polygon buffer b at clipPoly[b][c][k][lane], so the per-vertex reads and writes of varying index stay out of
+ * scratch memory.
+ */
+constexpr uint32_t NARROW_BLOCK = 64;
+__shared__ float clipPoly[2][3][MAX_POLY][NARROW_BLOCK];
+AVBD_D f3 polyGet(int buf, int k) {
+  const uint32_t l = threadIdx.x;
+  return mk3(clipPoly[buf][0][k][l], clipPoly[buf][1][k][l], clipPoly[buf][2][k][l]);
+}
+AVBD_D void polySet(int buf, int k, f3 v) {
+  const uint32_t l = threadIdx.x;
+  clipPoly[buf][0][k][l] = v.x;
+  clipPoly[buf][1][k][l] = v.y;
+  clipPoly[buf][2][k][l] = v.z;
+}
+
+/** Sutherland–Hodgman against one plane: polygon buffer src (n vertices) into buffer dst; returns the vertex count. */
+AVBD_D int clipPlane(int src, int n, int dst, f3 pn, float offset) {
   if (n == 0) return 0;
   int count = 0;
-  f3 a = src[n - 1];
+  f3 a = polyGet(src, n - 1);
   float da = dot(pn, a) - offset;
   for (int i = 0; i < n; i++) {
-    f3 b = src[i];
+    f3 b = polyGet(src, i);
     float db = dot(pn, b) - offset;
     bool aIn = da <= PLANE_EPSILON;
     bool bIn = db <= PLANE_EPSILON;
@@ -98,9 +159,9 @@ AVBD_D int clipPlane(const f3 *src, int n, f3 *dst, f3 pn, float offset) {
       float t = 0.f;
       float denom = da - db;
       if (fabsf(denom) > SAT_AXIS_EPSILON) t = rclamp(da / denom, 0.f, 1.f);
-      if (count < MAX_POLY) dst[count++] = addScaled(a, sub(b, a), t);
+      if (count < MAX_POLY) polySet(dst, count++, addScaled(a, sub(b, a), t));
     }
-    if (bIn && count < MAX_POLY) dst[count++] = b;
+    if (bIn && count < MAX_POLY) polySet(dst, count++, b);
     a = b;
     da = db;
   }
@@ -109,13 +170,28 @@ AVBD_D int clipPlane(const f3 *src, int n, f3 *dst, f3 pn, float offset) {
 
 AVBD_D int faceU(int k) { return k == 0 ? 1 : 0; }
 AVBD_D int faceV(int k) { return k != 2 ? 2 : 1; }
-AVBD_D f3 boxAxis(const Box &b, int k) { return k == 0 ? b.ax[0] : (k == 1 ? b.ax[1] : b.ax[2]); }
+/** c ? x : y member by member (a select of values; a select of references would keep both boxes in memory). */
+AVBD_D f3 pick3(bool c, f3 x, f3 y) { return mk3(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z); }
+AVBD_D Box pickBox(bool c, const Box &x, const Box &y) {
+  Box b;
+  b.c = pick3(c, x.c, y.c);
+  b.h = pick3(c, x.h, y.h);
+  b.ax0 = pick3(c, x.ax0, y.ax0);
+  b.ax1 = pick3(c, x.ax1, y.ax1);
+  b.ax2 = pick3(c, x.ax2, y.ax2);
+  return b;
+}
+
+AVBD_D f3 boxAxis(const Box &b, int k) {
+  const f3 a = k == 0 ? b.ax0 : b.ax1;
+  return k == 2 ? b.ax2 : a;
+}
 
 AVBD_D Found faceManifold(const Box &A, const Box &B, bool refIsA, int refAxis, f3 nAB) {
   Found found;
   found.count = 0;
-  const Box &R = refIsA ? A : B;
-  const Box &I = refIsA ? B : A;
+  const Box R = pickBox(refIsA, A, B);
+  const Box I = pickBox(refIsA, B, A);
   f3 outward = refIsA ? nAB : neg(nAB);
   f3 rax = boxAxis(R, refAxis);
   float rs = dot(outward, rax) >= 0.f ? 1.f : -1.f;
@@ -139,21 +215,20 @@ AVBD_D Found faceManifold(const Box &A, const Box &B, bool refIsA, int refAxis, 
   f3 ic = addScaled(I.c, fn, comp(I.h, incAxis));
   f3 u = boxAxis(I, faceU(incAxis)), v = boxAxis(I, faceV(incAxis));
   float eU = comp(I.h, faceU(incAxis)), eV = comp(I.h, faceV(incAxis));
-  f3 poly[MAX_POLY], tmp[MAX_POLY];
-  poly[0] = addScaled(addScaled(ic, u, 1.f * eU), v, 1.f * eV);
-  poly[1] = addScaled(addScaled(ic, u, -1.f * eU), v, 1.f * eV);
-  poly[2] = addScaled(addScaled(ic, u, -1.f * eU), v, -1.f * eV);
-  poly[3] = addScaled(addScaled(ic, u, 1.f * eU), v, -1.f * eV);
+  polySet(0, 0, addScaled(addScaled(ic, u, 1.f * eU), v, 1.f * eV));
+  polySet(0, 1, addScaled(addScaled(ic, u, -1.f * eU), v, 1.f * eV));
+  polySet(0, 2, addScaled(addScaled(ic, u, -1.f * eU), v, -1.f * eV));
+  polySet(0, 3, addScaled(addScaled(ic, u, 1.f * eU), v, -1.f * eV));
   int n = 4;
-  n = clipPlane(poly, n, tmp, ru, dot(ru, rc) + eu);
-  n = clipPlane(tmp, n, poly, neg(ru), dot(neg(ru), rc) + eu);
-  n = clipPlane(poly, n, tmp, rv, dot(rv, rc) + ev);
-  n = clipPlane(tmp, n, poly, neg(rv), dot(neg(rv), rc) + ev);
+  n = clipPlane(0, n, 1, ru, dot(ru, rc) + eu);
+  n = clipPlane(1, n, 0, neg(ru), dot(neg(ru), rc) + eu);
+  n = clipPlane(0, n, 1, rv, dot(rv, rc) + ev);
+  n = clipPlane(1, n, 0, neg(rv), dot(neg(rv), rc) + ev);
   if (n == 0) return found;
 
   uint32_t prefix = ((refIsA ? AXIS_FACE_A : AXIS_FACE_B) << 24) | (uint32_t(refAxis) << 16) | (uint32_t(incAxis) << 8);
   for (int i = 0; i < n && found.count < MAX_CONTACTS; i++) {
-    f3 p = poly[i];
+    f3 p = polyGet(0, i);
     float dist = dot(sub(p, rc), rn);
     if (dist > PLANE_EPSILON) continue;
     f3 onRef = addScaled(p, rn, -dist);
@@ -220,13 +295,18 @@ AVBD_D Found collideBoxes(const Box &A, const Box &B, bool faceBias, Sat &sat) {
   face.kind = edge.kind = face.ia = face.ib = edge.ia = edge.ib = 0;
   face.sep = edge.sep = 0.f;
   face.n = edge.n = mk3(0.f, 0.f, 0.f);
+#pragma unroll
   for (int i = 0; i < 3; i++)
-    if (!testAxis(A, B, delta, A.ax[i], AXIS_FACE_A, uint32_t(i), 0u, face)) return none;
+    if (!testAxis(A, B, delta, boxAxis(A, i), AXIS_FACE_A, uint32_t(i), 0u, face)) return none;
+#pragma unroll
   for (int i = 0; i < 3; i++)
-    if (!testAxis(A, B, delta, B.ax[i], AXIS_FACE_B, 0u, uint32_t(i), face)) return none;
+    if (!testAxis(A, B, delta, boxAxis(B, i), AXIS_FACE_B, 0u, uint32_t(i), face)) return none;
+#pragma unroll
   for (int i = 0; i < 3; i++)
+#pragma unroll
     for (int j = 0; j < 3; j++)
-      if (!testAxis(A, B, delta, cross(A.ax[i], B.ax[j]), AXIS_EDGE, uint32_t(i), uint32_t(j), edge)) return none;
+      if (!testAxis(A, B, delta, cross(boxAxis(A, i), boxAxis(B, j)), AXIS_EDGE, uint32_t(i), uint32_t(j), edge))
+        return none;
   if (!face.valid) return none;
   Sat best = face;
   if (edge.valid) {
