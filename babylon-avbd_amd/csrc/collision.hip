@@ -329,10 +329,12 @@ template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narro
     bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
     float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
     float pmin = contactPenaltyMin(w, p, a, b);
-    for (int i = 0; i < found.count; i++) {
-      f3 xA, xB;
-      uint32_t feature;
-      foundPoint(found, i, xA, xB, feature);
+    // Unrolled over the slots, so the found points are read at constant indices and stay in registers
+#pragma unroll
+    for (int i = 0; i < MAX_CONTACTS; i++) {
+      if (i >= found.count) break;
+      const f3 xA = found.xA[i], xB = found.xB[i];
+      const uint32_t feature = found.feature[i];
       f3 rA = qrotate(qconj(qA), sub(xA, A.c));
       f3 rB = qrotate(qconj(qB), sub(xB, B.c));
       f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);

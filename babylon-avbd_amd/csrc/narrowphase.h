@@ -66,21 +66,6 @@ AVBD_D void addFound(Found &f, f3 xA, f3 xB, uint32_t feature) {
   f.count++;
 }
 
-/** Point i of a Found (i < count), read through a select over the slots (no indexed register access). */
-AVBD_D void foundPoint(const Found &f, int i, f3 &xA, f3 &xB, uint32_t &feature) {
-  xA = f.xA[0];
-  xB = f.xB[0];
-  feature = f.feature[0];
-#pragma unroll
-  for (int k = 1; k < MAX_CONTACTS; k++) {
-    if (i == k) {
-      xA = f.xA[k];
-      xB = f.xB[k];
-      feature = f.feature[k];
-    }
-  }
-}
-
 /** Swap the two sides of every point (a capsule argument reversed). */
 AVBD_D void swapFoundSides(Found &f) {
 #pragma unroll
