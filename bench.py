@@ -18,6 +18,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_primal_100k_boxes_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_primal launch on the 100k-box world, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    measurement (calibrated against a copy kernel of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def primal_bytes_per_iteration(world):
@@ -35,6 +46,31 @@ def primal_bytes_per_iteration(world):
     joints = int((~is_contact).sum())
     return dynamic * 160 + entries * 100 + points * 64 + joints * 128, dict(
         dynamic=dynamic, entries=entries, contact_reads=points, joint_entries=joints)
+
+
+def summarize_counters(directory, kernel, per_step):
+    """Per-step sums and per-dispatch means of the counters in a rocprofv3 --pmc CSV under `directory`, over the
+    dispatches of kernels whose name contains `kernel` (the last whole step of them when per_step > 0)."""
+    import csv
+    import glob
+    paths = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
+    if not paths:
+        raise SystemExit(f"no counter_collection.csv under {directory}")
+    values = {}
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if kernel not in r["Kernel_Name"]:
+                    continue
+                d = values.setdefault(int(r["Dispatch_Id"]), {})
+                d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    ids = sorted(values)
+    if per_step > 0:
+        ids = ids[-per_step:]
+    names = sorted({c for d in ids for c in values[d]})
+    total = {c: sum(values[d].get(c, 0.0) for d in ids) for c in names}
+    return {"kernel": kernel, "dispatches": len(ids), "sum": total,
+            "mean_per_dispatch": {c: total[c] / max(len(ids), 1) for c in names}}
 
 
 def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
@@ -217,12 +253,22 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-pool", type=int, default=64)
     ap.add_argument("--serve-active", type=int, default=32)
+    ap.add_argument("--state", default=None,
+                    help="settled-state file: loaded instead of the warm-up when it exists, written after it otherwise")
+    ap.add_argument("--summarize-counters", default=None, metavar="DIR",
+                    help="print per-step sums of the counters rocprofv3 --pmc wrote under DIR for --kernel, then exit")
+    ap.add_argument("--kernel", default="k_primal")
+    ap.add_argument("--launches-per-step", type=int, default=0,
+                    help="with --summarize-counters: dispatches of --kernel per step (0: one step = all dispatches)")
     ap.add_argument("--mix-steps", type=int, default=200)
     ap.add_argument("--mix-warmup", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=280)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-sample", type=int, default=128)
     args = ap.parse_args()
+    if args.summarize_counters:
+        print(json.dumps(summarize_counters(args.summarize_counters, args.kernel, args.launches_per_step)))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the AVBD step here is single-GPU; "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
@@ -246,13 +292,21 @@ def main():
     print(f"# world: {n_bodies} bodies, {world.deviceBytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
 
     # Warm-up: the pile falls and lands; the colour/capacity policy adapts from counters read back asynchronously
-    done = 0
-    while done < args.warmup:
-        k = min(50, args.warmup - done)
-        world.step(k)
-        world.synchronize()
-        done += k
-        print(f"# warm-up {done}: {world.readCounters()}", file=sys.stderr, flush=True)
+    if args.state and os.path.exists(args.state):
+        saved = dict(np.load(args.state))
+        world.writeState(saved)
+        world.setColoring(int(saved["colorsUsed"]) + 3, 16)
+        print(f"# state loaded from {args.state}", file=sys.stderr, flush=True)
+    else:
+        done = 0
+        while done < args.warmup:
+            k = min(50, args.warmup - done)
+            world.step(k)
+            world.synchronize()
+            done += k
+            print(f"# warm-up {done}: {world.readCounters()}", file=sys.stderr, flush=True)
+        if args.state:
+            np.savez(args.state, colorsUsed=world.readCounters()["colors"], **world.readState())
     counters = world.readCounters()
     state0 = None if args.no_cpu else world.readState()
 
@@ -310,7 +364,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic() if args.workload == "large" else None,
+            "traffic_source": "profiles/k_primal_100k_boxes_traffic.json (rocprofv3 2 x FETCH_SIZE + WRITE_SIZE per "
+                              "launch, copy-kernel calibrated)",
             "kernel": "k_primal (one colour's bodies)",
             "bytes_per_iteration": bytes_iter,
             "launches_per_iteration": launches,
