@@ -393,18 +393,30 @@ __global__ void k_manifold_compact(DeviceWorld w) {
   // The loop bound is uniform per block (the wavefront reduction below needs every lane)
   for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
     uint32_t p = base + threadIdx.x;
-    uint32_t points = 0;
+    uint32_t points = 0, o = 0;
     if (p < P) {
-      uint32_t o = w.mfOffset[p];
+      o = w.mfOffset[p];
       if (w.mfOffset[p + 1] != o) {
         Manifold m = w.mfSlots[p];
         w.manifolds[o] = m;
         points = m.ids.w & 15u;
       }
     }
-    // One atomic per wavefront for the contact total
-    for (int d = 32; d > 0; d >>= 1) points += __shfl_xor(points, d, 64);
-    if ((threadIdx.x & 63) == 0 && points) atomicAdd(&w.counters[C_CONTACTS], points);
+    // The wavefront's points take one range of the dense point list (one atomic per wavefront, which also counts
+    // the contact total); each manifold's points follow its wavefront prefix. The order of the ranges varies from
+    // run to run, the dual's result does not (every point is updated on its own).
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = points;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t first = 0;
+    if (lane == 63 && total) first = atomicAdd(&w.counters[C_CONTACTS], total);
+    first = __shfl(first, 63, 64) + incl - points;
+    for (uint32_t k = 0; k < points; k++) w.pointMap[first + k] = (o << 3) | k;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
 }

@@ -287,6 +287,7 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.pairCapacity = pairCap_;
   w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
   w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
+  w.pointMap = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
   w.mfOffset = static_cast<uint32_t *>(alloc((size_t(pairCap_) + 1) * 4));
   for (int k = 0; k < 2; k++) {
     manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));

@@ -47,6 +47,7 @@ struct DeviceWorld {
   Manifold *manifolds;
   Manifold *prevManifolds;
   Contact *contacts;      // 8 per pair slot
+  uint32_t *pointMap;     // the current contact points, densely: manifold index << 3 | point (C_CONTACTS of them)
   Contact *prevContacts;
   uint32_t *table;        // last step's manifolds by pair hash (value = index + 1)
   uint32_t hashMask;

@@ -504,45 +504,42 @@ AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
   w.joints[j] = k;
 }
 
-/**
- * Contact points slot, slot + DUAL_SLOTS, ... of manifold m (dual, wgsl-solve.ts:474-535): the penalty ramp where the
- * row is active, λ = the clamped force, and the stick flag. Points are independent of each other, so the manifold's
- * points are spread over DUAL_SLOTS threads.
- */
-constexpr uint32_t DUAL_SLOTS = 4;
-AVBD_D void dualManifold(const DeviceWorld &w, uint32_t m, uint32_t slot) {
+/** Point k of manifold m (dual, wgsl-solve.ts:474-535): the penalty ramp where the row is active, λ = the clamped
+ * force, and the stick flag. Points are independent of each other, so every point is a thread of its own. */
+AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   const Manifold mf = w.manifolds[m];
-  const uint32_t count = mf.ids.w & 15u;
-  if (slot >= count) return;
   const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
   PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
-  for (uint32_t c = mf.ids.z + slot; c < mf.ids.z + count; c += DUAL_SLOTS) {
-    Contact k = w.contacts[c];
-    Eval e = evalContact(k, basis, mf.geo.w, PA, PB, p.alpha);
-    f3 pen = xyz(k.pen);
-    if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
-    uint32_t key = fbits(k.rA.w);
-    if (e.frictionScale <= e.bounds) {
-      pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
-      pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
-      bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
-      key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
-      w.contacts[c].rA.w = bitsf(key);
-    }
-    w.contacts[c].pen = mk4(pen, k.pen.w);
-    w.contacts[c].lam = mk4(e.F, k.lam.w);
+  const uint32_t c = mf.ids.z + k;
+  Contact kc = w.contacts[c];
+  Eval e = evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha);
+  f3 pen = xyz(kc.pen);
+  if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
+  uint32_t key = fbits(kc.rA.w);
+  if (e.frictionScale <= e.bounds) {
+    pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
+    pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
+    bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
+    key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+    w.contacts[c].rA.w = bitsf(key);
   }
+  w.contacts[c].pen = mk4(pen, kc.pen.w);
+  w.contacts[c].lam = mk4(e.F, kc.lam.w);
 }
 
-/** Joints first (one thread each), then DUAL_SLOTS threads per manifold. */
+/** Joints first (one thread each), then one thread per contact point of the dense point list. */
 __global__ void k_dual(DeviceWorld w) {
   if (held(w)) return;
   const uint32_t J = w.nJoints;
-  const uint64_t total = uint64_t(J) + uint64_t(w.counters[C_MANIFOLDS]) * DUAL_SLOTS;
+  const uint64_t total = uint64_t(J) + w.counters[C_CONTACTS];
   for (uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; id < total; id += uint64_t(gridDim.x) * blockDim.x) {
-    if (id < J) dualJoint(w, uint32_t(id));
-    else dualManifold(w, uint32_t((id - J) / DUAL_SLOTS), uint32_t((id - J) % DUAL_SLOTS));
+    if (id < J) {
+      dualJoint(w, uint32_t(id));
+    } else {
+      const uint32_t mk = w.pointMap[id - J];
+      dualPoint(w, mk >> 3, mk & 7u);
+    }
   }
 }
 
@@ -578,7 +575,7 @@ void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies
   k_primal<<<blocks(uint64_t(maxColorBodies) * LANES, PRIMAL_BLOCK, 4096), PRIMAL_BLOCK, 0, s>>>(w, color);
 }
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
-  k_dual<<<blocks(uint64_t(w.nJoints) + uint64_t(manifoldCapacity) * DUAL_SLOTS, 256, 16384), 256, 0, s>>>(w);
+  k_dual<<<blocks(uint64_t(w.nJoints) + uint64_t(manifoldCapacity) * 2, 256, 16384), 256, 0, s>>>(w);
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {
   k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
