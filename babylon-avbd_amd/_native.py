@@ -58,13 +58,14 @@ class Options(ctypes.Structure):
         ("adaptive", ctypes.c_uint32),
         ("broadphase", ctypes.c_uint32),
         ("bvh_rebuild_interval", ctypes.c_uint32),
+        ("solver", ctypes.c_uint32),
     ]
 
 
 class Counters(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in
                 ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "joints", "bodies",
-                 "pivot_fallbacks")]
+                 "pivot_fallbacks", "solve_aborts")]
 
 
 class ServerOptions(ctypes.Structure):

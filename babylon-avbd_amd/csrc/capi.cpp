@@ -97,6 +97,8 @@ EngineOptions toOptions(const AvbdOptions *o) {
     if (o->broadphase > 2) throw std::invalid_argument("broadphase must be 0 (auto), 1 (grid) or 2 (hploc)");
     e.broadphase = o->broadphase;
     if (o->bvh_rebuild_interval) e.rebuildInterval = o->bvh_rebuild_interval;
+    if (o->solver > 2) throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches) or 2 (dataflow)");
+    e.solver = o->solver;
     e.device = o->device;
   }
   if (e.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");
@@ -282,6 +284,7 @@ int avbd_world_read_counters(avbd_world *world, AvbdCounters *out) {
     out->joints = world->engine->jointCount();
     out->bodies = world->engine->bodyCount();
     out->pivot_fallbacks = c.pivot64;
+    out->solve_aborts = c.solveAborts;
     return AVBD_OK;
   });
 }

@@ -32,6 +32,8 @@ struct EngineOptions {
   int device = 0;
   uint32_t broadphase = 0;      // 0 auto (H-PLOC for one world of >= 10,000 bodies, else grid), 1 grid, 2 H-PLOC
   uint32_t rebuildInterval = 64;  // H-PLOC: rebuild every this many steps, refit in between
+  uint32_t solver = 0;          // 0 auto (colour launches), 1 one launch per colour, 2 dataflow (one launch for all
+                                // iterations; batches without joints, one engine on the device at a time)
 };
 
 /** Milliseconds of one profiled step by phase, measured with HIP events on the engine's stream. */
@@ -55,6 +57,7 @@ struct SavedState {
 
 struct Counters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
+  uint32_t solveAborts;  // dataflow solve waits given up in the last step (0 unless something is wrong)
 };
 
 class Engine {
@@ -166,6 +169,8 @@ class Engine {
   Contact *contactBuf_[2] = {nullptr, nullptr};
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
   bool useBvh_ = false;
+  bool useDataflow_ = false;
+  uint32_t dfGrid_ = 0;
   uint32_t rebuildInterval_ = 64;
   uint64_t stepIndex_ = 0;
   Bvh bvh_{};

@@ -146,6 +146,11 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   if (opt.broadphase == 2) useBvh_ = true;
   else if (opt.broadphase == 1) useBvh_ = false;
   else useBvh_ = nWorlds_ == 1 && nBodies_ >= 10000;
+  if (opt.solver > 2) throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches) or 2 (dataflow)");
+  if (opt.solver == 2 && nJoints_ == 0) {
+    dfGrid_ = dataflow_grid();
+    useDataflow_ = dfGrid_ > 0;
+  }
 
   DeviceWorld &w = base_;
   w.nBodies = nBodies_;
@@ -159,6 +164,8 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.bodyWorld = dallocT<uint32_t>(nBodies_);
   w.worlds = dallocT<WorldParams>(nWorlds_);
   w.refs = dallocT<float4>(2ull * nBodies_);
+  w.bodyDone = dallocT<uint32_t>(nBodies_);
+  w.dfPlan = dallocT<DataflowPlan>(1);
   w.filters = dallocT<uint2>(nBodies_);
   w.joints = dallocT<Joint>(nJoints_);
   w.info = dallocT<int4>(nJoints_);
@@ -288,6 +295,7 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
   w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
   w.pointMap = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
+  w.pointDone = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
   w.mfOffset = static_cast<uint32_t *>(alloc((size_t(pairCap_) + 1) * 4));
   for (int k = 0; k < 2; k++) {
     manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
@@ -373,9 +381,13 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
   launch_adjacency(w, pairCap_, s);
   launch_coloring(w, colorRounds_, s);
   launch_warm_start(w, s);
-  for (uint32_t it = 0; it < iterations_; it++) {
-    for (uint32_t c = 0; c < colorCap_; c++) launch_primal(w, c, maxDynamicBodies_, s);
-    launch_dual(w, pairCap_, s);
+  if (useDataflow_) {
+    launch_dataflow(w, iterations_, dfGrid_, s);
+  } else {
+    for (uint32_t it = 0; it < iterations_; it++) {
+      for (uint32_t c = 0; c < colorCap_; c++) launch_primal(w, c, maxDynamicBodies_, s);
+      launch_dual(w, pairCap_, s);
+    }
   }
   launch_velocities(w, s);
 }
@@ -389,7 +401,7 @@ void Engine::pollCounters() {
     counterPending_[slot] = false;
     const uint32_t *c = hostCounters_ + slot * COUNTER_WORDS;
     Counters cc{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                c[C_PREV_MANIFOLDS], c[C_PIVOT64]};
+                c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT]};
     adaptFrom(cc);
   }
 }
@@ -496,15 +508,22 @@ PhaseTimes Engine::profileStep() {
   launch_warm_start(w, s);
   HIP_CHECK(hipEventRecord(ev[4], s));
   uint32_t k = 6;
-  for (uint32_t it = 0; it < iterations_; it++) {
-    for (uint32_t c = 0; c < colorCap_; c++) {
+  if (useDataflow_) {
+    // One launch for every iteration: its time is reported as the primal's, the dual's is in it
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+    launch_dataflow(w, iterations_, dfGrid_, s);
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+  } else {
+    for (uint32_t it = 0; it < iterations_; it++) {
+      for (uint32_t c = 0; c < colorCap_; c++) {
+        HIP_CHECK(hipEventRecord(ev[k++], s));
+        launch_primal(w, c, maxDynamicBodies_, s);
+        HIP_CHECK(hipEventRecord(ev[k++], s));
+      }
       HIP_CHECK(hipEventRecord(ev[k++], s));
-      launch_primal(w, c, maxDynamicBodies_, s);
+      launch_dual(w, pairCap_, s);
       HIP_CHECK(hipEventRecord(ev[k++], s));
     }
-    HIP_CHECK(hipEventRecord(ev[k++], s));
-    launch_dual(w, pairCap_, s);
-    HIP_CHECK(hipEventRecord(ev[k++], s));
   }
   launch_velocities(w, s);
   HIP_CHECK(hipEventRecord(ev[5], s));
@@ -526,14 +545,21 @@ PhaseTimes Engine::profileStep() {
   p.coloring = ms(2, 3);
   p.warmStart = ms(3, 4);
   k = 6;
-  for (uint32_t it = 0; it < iterations_; it++) {
-    for (uint32_t c = 0; c < colorCap_; c++, k += 2) p.primal += ms(k, k + 1);
-    p.dual += ms(k, k + 1);
-    k += 2;
+  if (useDataflow_) {
+    p.primal = ms(6, 7);
+    p.velocities = ms(7, 5);
+    p.primalLaunches = 1;
+    p.dualLaunches = 0;
+  } else {
+    for (uint32_t it = 0; it < iterations_; it++) {
+      for (uint32_t c = 0; c < colorCap_; c++, k += 2) p.primal += ms(k, k + 1);
+      p.dual += ms(k, k + 1);
+      k += 2;
+    }
+    p.velocities = ms(k - 1, 5);
+    p.primalLaunches = iterations_ * colorCap_;
+    p.dualLaunches = iterations_;
   }
-  p.velocities = ms(k - 1, 5);
-  p.primalLaunches = iterations_ * colorCap_;
-  p.dualLaunches = iterations_;
   for (auto &x : ev) HIP_CHECK(hipEventDestroy(x));
   return p;
 }
@@ -574,7 +600,7 @@ Counters Engine::readCounters() {
   HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
   synchronize();
   return Counters{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                  c[C_PREV_MANIFOLDS], c[C_PIVOT64]};
+                  c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT]};
 }
 
 std::vector<uint2> Engine::readPairs() {

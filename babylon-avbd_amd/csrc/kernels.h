@@ -32,6 +32,10 @@ void launch_warm_start(const DeviceWorld &w, hipStream_t s);
 void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s);
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
 void launch_velocities(const DeviceWorld &w, hipStream_t s);
+/** Blocks of the dataflow solve that are resident at once on the current device (0 if unknown). */
+uint32_t dataflow_grid();
+/** Every iteration's primal and dual in one launch of `grid` blocks (at most dataflow_grid()). */
+void launch_dataflow(const DeviceWorld &w, uint32_t iterations, uint32_t grid, hipStream_t s);
 
 // Edits between steps
 /** Body commands (3 uint4 each) grouped by body: ranges[r] = (first command, count), each body's in submission

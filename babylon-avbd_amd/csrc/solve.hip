@@ -12,6 +12,66 @@ namespace avbd {
 
 AVBD_D bool held(const DeviceWorld &w) { return w.counters[C_OVERFLOW] != 0u; }
 
+// --- device-scope access for the dataflow solve -------------------------------------------------------------------------
+// Within one launch, waves on different XCDs exchange poses and contact multipliers. Those are read and written with
+// device-scope (sc1) accesses, which are coherent across the XCDs' L2 caches; a writer waits for its stores to complete
+// before it publishes a stamp, and a reader reads only after it has seen the stamp.
+
+/** The dataflow solve's position in the step: iteration, colour of the running primal, the step's stamp base. */
+struct DfCtx {
+  uint32_t it, color, base;
+};
+
+AVBD_D uint32_t ldAgent(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+AVBD_D void stAgent(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+AVBD_D float4 ldAgent4(const float4 *p) {
+  const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+  const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float4(__uint_as_float(uint32_t(a)), __uint_as_float(uint32_t(a >> 32)), __uint_as_float(uint32_t(b)),
+                     __uint_as_float(uint32_t(b >> 32)));
+}
+AVBD_D void stAgent4(float4 *p, float4 v) {
+  uint64_t *q = reinterpret_cast<uint64_t *>(p);
+  __hip_atomic_store(q, uint64_t(__float_as_uint(v.x)) | (uint64_t(__float_as_uint(v.y)) << 32), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, uint64_t(__float_as_uint(v.z)) | (uint64_t(__float_as_uint(v.w)) << 32), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+/** This wave's earlier stores are complete before any later one is issued. */
+AVBD_D void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+/** Polls of one wait before it gives up (about 65 ms); a wait given up sets C_DF_ABORT, which ends every other wait. */
+constexpr uint32_t DF_SPIN = 1u << 16;
+
+/** Wait until *s >= ts and *r >= tr, both polled in one round trip. Every wait ends like waitFor's. */
+AVBD_D void waitFor2(const DeviceWorld &w, const uint32_t *s, uint32_t ts, const uint32_t *r, uint32_t tr) {
+  for (uint32_t spin = 0;; spin++) {
+    const uint32_t a = ldAgent(s), b = ldAgent(r);
+    if (a >= ts && b >= tr) break;
+    if (spin >= DF_SPIN) {
+      atomicOr(&w.counters[C_DF_ABORT], 1u);
+      break;
+    }
+    if ((spin & 31u) == 31u && ldAgent(&w.counters[C_DF_ABORT]) != 0u) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  asm volatile("" ::: "memory");
+}
+
+/** Wait until *s >= target. Every wait ends: at DF_SPIN polls, or at once when an earlier wait gave up. */
+AVBD_D void waitFor(const DeviceWorld &w, const uint32_t *s, uint32_t target) {
+  for (uint32_t spin = 0; ldAgent(s) < target; spin++) {
+    if (spin >= DF_SPIN) {
+      atomicOr(&w.counters[C_DF_ABORT], 1u);
+      break;
+    }
+    if ((spin & 31u) == 31u && ldAgent(&w.counters[C_DF_ABORT]) != 0u) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  asm volatile("" ::: "memory");  // later loads are not moved above the wait
+}
+
 // --- joints ------------------------------------------------------------------------------------------------------------
 
 AVBD_D f3 anchorA(const DeviceWorld &w, const Joint &k, int a) {
@@ -173,12 +233,13 @@ struct PairBody {
   float4 rot, irot;
   f3 dLin, dAng;
 };
-AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
+template <bool DF = false> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
   const Body &B = w.bodies[i];
   PairBody pb;
-  pb.rot = B.rot;
+  const float4 pos = DF ? ldAgent4(&B.pos) : B.pos;
+  pb.rot = DF ? ldAgent4(&B.rot) : B.rot;
   pb.irot = B.irot;
-  pb.dLin = sub(xyz(B.pos), xyz(B.ipos));
+  pb.dLin = sub(xyz(pos), xyz(B.ipos));
   pb.dAng = qsub(pb.rot, pb.irot);
   return pb;
 }
@@ -220,20 +281,40 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
 }
 
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
+template <bool DF>
 AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
-                    uint32_t p, float alpha) {
+                    uint32_t p, float alpha, const DfCtx &df) {
   if (head.z & ENTRY_JOINT) {
-    addJoint(w, acc, head.y, alpha, i);
+    addJoint(w, acc, head.y, alpha, i);  // the dataflow solve runs only for batches without joints
     return;
   }
+  if constexpr (DF) {
+    // The partner's pose: this iteration's if its colour comes first, else the last iteration's; and the point's
+    // multipliers from the last iteration's dual. A partner of an earlier colour has already waited for the same
+    // points' dual before it published this iteration's pose, so its stamp covers them.
+    const bool dynamic = (head.z & ENTRY_DYNAMIC) != 0;
+    const bool earlier = dynamic && w.colorA[head.x] < df.color;
+    const uint32_t target = earlier ? df.it + 1u : df.it;
+    const bool needBody = dynamic && target != 0u, needPoint = df.it != 0u && !earlier;
+    if (needBody && needPoint)
+      waitFor2(w, &w.bodyDone[head.x], df.base + target, &w.pointDone[head.y + p], df.base + df.it);
+    else if (needBody)
+      waitFor(w, &w.bodyDone[head.x], df.base + target);
+    else if (needPoint)
+      waitFor(w, &w.pointDone[head.y + p], df.base + df.it);
+  }
   const bool isA = (head.z & ENTRY_IS_A) != 0;
-  const PairBody other = pairBody(w, head.x);
+  const PairBody other = pairBody<DF>(w, head.x);
   const PairBody PA = isA ? self : other;
   const PairBody PB = isA ? other : self;
   Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
   float sg = isA ? 1.f : -1.f;
   f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
   Contact k = w.contacts[head.y + p];
+  if constexpr (DF) {
+    k.pen = ldAgent4(&w.contacts[head.y + p].pen);
+    k.lam = ldAgent4(&w.contacts[head.y + p].lam);
+  }
   Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha);
   f3 r = isA ? ev.rAW : ev.rBW;
   addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
@@ -249,10 +330,15 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
  * group's prefix sums and takes the record from the lane that loaded it. A manifold with many points is thus spread
  * over several lanes. The partial systems are then summed by a fixed tree, so the result does not depend on scheduling.
  */
-template <uint32_t W>
+template <uint32_t W, bool DF>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
-                       float dt, Acc &acc) {
+                       float dt, Acc &acc, const DfCtx &df) {
   const Body &B = w.bodies[i];
+  if constexpr (DF) {
+    if (df.it) waitFor(w, &w.bodyDone[i], df.base + df.it);  // the body's own last iteration
+  }
+  const float4 bpos = DF ? ldAgent4(&B.pos) : B.pos;
+  const float4 brot = DF ? ldAgent4(&B.rot) : B.rot;
   acc.l00 = acc.l10 = acc.l11 = acc.l20 = acc.l21 = acc.l22 = 0.f;
   acc.a00 = acc.a10 = acc.a11 = acc.a20 = acc.a21 = acc.a22 = 0.f;
 #pragma unroll
@@ -265,11 +351,11 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
     f3 I = divs(xyz(B.moment), dt2);
     acc.l00 = m; acc.l11 = m; acc.l22 = m;
     acc.a00 = I.x; acc.a11 = I.y; acc.a22 = I.z;
-    acc.rLin = scale(sub(xyz(B.pos), xyz(B.inertialPos)), m);
-    acc.rAng = mul3(I, qsub(B.rot, B.inertialRot));
+    acc.rLin = scale(sub(xyz(bpos), xyz(B.inertialPos)), m);
+    acc.rAng = mul3(I, qsub(brot, B.inertialRot));
   }
   if (first >= end) return;  // the same for every lane of the group
-  const PairBody self = pairBody(w, i);
+  const PairBody self = pairBody<DF>(w, i);
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
     uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
@@ -298,7 +384,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
                                  __shfl(head.w, owner, W));
       const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W), 0u);
       const uint32_t p = t - __shfl(excl, owner, W);
-      if (t < total) addItem(w, acc, i, self, h, tl, p, alpha);
+      if (t < total) addItem<DF>(w, acc, i, self, h, tl, p, alpha, df);
     }
   }
 }
@@ -405,7 +491,7 @@ AVBD_D void ldlt6Double(const float (&A)[21], const float (&b)[6], float (&x)[6]
  * first; a near-singular system (a pivot at its floor, as stiff joint chains give) is solved again in FP64 from the
  * same FP32 entries, with the same floor.
  */
-AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
+template <bool DF> AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df) {
   const float A[21] = {s.l00, s.l10, s.l11, s.l20, s.l21, s.l22, s.c[0], s.c[1], s.c[2], s.a00, s.c[3],
                        s.c[4], s.c[5], s.a10, s.a11, s.c[6], s.c[7], s.c[8], s.a20, s.a21, s.a22};
   const float b[6] = {s.rLin.x, s.rLin.y, s.rLin.z, s.rAng.x, s.rAng.y, s.rAng.z};
@@ -417,28 +503,37 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s) {
     atomicAdd(&w.counters[C_PIVOT64], 1u);
   }
   Body &B = w.bodies[i];
-  float4 pos = B.pos;
-  B.pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
-  B.rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
+  if constexpr (DF) {
+    const float4 pos = ldAgent4(&B.pos);
+    stAgent4(&B.pos, make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w));
+    stAgent4(&B.rot, qadd(ldAgent4(&B.rot), mk3(-x[3], -x[4], -x[5])));
+    storesDone();
+    stAgent(&w.bodyDone[i], df.base + df.it + 1u);
+  } else {
+    float4 pos = B.pos;
+    B.pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
+    B.rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
+  }
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;
 constexpr uint32_t WAVE = 64;
 
 /** Bucket position k (of a colour half ending at `end`) solved by a group of W lanes; `lane` is the lane in the group. */
-template <uint32_t W> AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t lane) {
+template <uint32_t W, bool DF = false>
+AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t lane, const DfCtx &df = DfCtx{}) {
   const bool live = k < end;
   const uint4 slot = live ? w.colorSlot[k] : make_uint4(0u, 0u, 0u, 0u);
   Acc acc;
   if (live) {
     const WorldParams &p = w.worlds[slot.w];
-    accumulate<W>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, acc);
+    accumulate<W, DF>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, acc, df);
   } else {
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
   }
   mergeTree<W>(acc);
-  if (live && lane == 0) finishBody(w, slot.x, acc);
+  if (live && lane == 0) finishBody<DF>(w, slot.x, acc, df);
 }
 
 /**
@@ -506,26 +601,48 @@ AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
 
 /** Point k of manifold m (dual, wgsl-solve.ts:474-535): the penalty ramp where the row is active, λ = the clamped
  * force, and the stick flag. Points are independent of each other, so every point is a thread of its own. */
-AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
+template <bool DF = false> AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k, const DfCtx &df = DfCtx{}) {
   const Manifold mf = w.manifolds[m];
-  const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
-  PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
-  Basis basis = orthonormal(xyz(mf.geo));
   const uint32_t c = mf.ids.z + k;
+  if constexpr (DF) {
+    // Both bodies' poses of this iteration (a static body's stamp is never written: it is not waited for)
+    const bool dynA = w.bodies[mf.ids.x].size.w > 0.f, dynB = w.bodies[mf.ids.y].size.w > 0.f;
+    const uint32_t t = df.base + df.it + 1u;
+    if (dynA && dynB)
+      waitFor2(w, &w.bodyDone[mf.ids.x], t, &w.bodyDone[mf.ids.y], t);
+    else if (dynA || dynB)
+      waitFor(w, &w.bodyDone[dynA ? mf.ids.x : mf.ids.y], t);
+  }
+  const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
+  PairBody PA = pairBody<DF>(w, mf.ids.x), PB = pairBody<DF>(w, mf.ids.y);
+  Basis basis = orthonormal(xyz(mf.geo));
   Contact kc = w.contacts[c];
+  if constexpr (DF) {
+    kc.pen = ldAgent4(&w.contacts[c].pen);
+    kc.lam = ldAgent4(&w.contacts[c].lam);
+  }
   Eval e = evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha);
   f3 pen = xyz(kc.pen);
   if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
   uint32_t key = fbits(kc.rA.w);
-  if (e.frictionScale <= e.bounds) {
+  const bool ramp = e.frictionScale <= e.bounds;
+  if (ramp) {
     pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
     pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
     bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
     key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
-    w.contacts[c].rA.w = bitsf(key);
   }
-  w.contacts[c].pen = mk4(pen, kc.pen.w);
-  w.contacts[c].lam = mk4(e.F, kc.lam.w);
+  if constexpr (DF) {
+    if (ramp) stAgent(reinterpret_cast<uint32_t *>(&w.contacts[c].rA.w), key);
+    stAgent4(&w.contacts[c].pen, mk4(pen, kc.pen.w));
+    stAgent4(&w.contacts[c].lam, mk4(e.F, kc.lam.w));
+    storesDone();
+    stAgent(&w.pointDone[c], df.base + df.it + 1u);
+  } else {
+    if (ramp) w.contacts[c].rA.w = bitsf(key);
+    w.contacts[c].pen = mk4(pen, kc.pen.w);
+    w.contacts[c].lam = mk4(e.F, kc.lam.w);
+  }
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */
@@ -558,6 +675,74 @@ __global__ void k_velocities(DeviceWorld w) {
   }
 }
 
+// --- dataflow solve: every iteration's colours and duals in one launch -------------------------------------------------
+
+/**
+ * The step's work list for the dataflow solve (one thread): wave counts per colour from the colour buckets, the dual's
+ * point blocks from the contact count, and the stamp base past the last step's stamps.
+ */
+__global__ void k_dataflow_plan(DeviceWorld w, uint32_t iterations) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  DataflowPlan &p = *w.dfPlan;
+  p.base = p.base + p.iterations + 1u;
+  p.iterations = iterations;
+  uint32_t waves = 0;
+  for (uint32_t c = 0; c < w.colorCap; c++) {
+    const uint32_t heavy = w.colorStart[2 * c], light = w.colorStart[2 * c + 1], end = w.colorStart[2 * c + 2];
+    const uint32_t hw = (light - heavy + (WAVE / HEAVY_LANES) - 1) / (WAVE / HEAVY_LANES);
+    p.waveStart[c] = waves;
+    p.heavyWaves[c] = hw;
+    waves += hw + (end - light + (WAVE / LANES) - 1) / (WAVE / LANES);
+  }
+  p.waveStart[w.colorCap] = waves;
+  p.dualStart = waves;
+  p.points = w.counters[C_CONTACTS];
+  p.unitsPerIter = waves + (p.points + WAVE - 1) / WAVE;
+  p.total = p.unitsPerIter * iterations;
+}
+
+/**
+ * Every iteration's primal (colour by colour) and dual in one launch. Unit u (see DataflowPlan) goes to wave
+ * u mod (waves of the grid), and each wave takes its units in increasing order. A unit waits only for units before it
+ * (a body for its neighbours of earlier colours in this iteration, of later colours in the last one, for its own last
+ * iteration and for its points' last dual; a dual point for both bodies' poses of this iteration), so the solve
+ * computes exactly what the colour-by-colour launches do. Every wave of the grid is resident at once (the grid is
+ * sized by the occupancy), so the unit everything else waits for always runs; and every wait is bounded.
+ */
+__global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld w) {
+  if (held(w)) return;
+  const DataflowPlan &plan = *w.dfPlan;
+  const uint32_t lane = threadIdx.x % WAVE;
+  const uint32_t waves = gridDim.x * (PRIMAL_BLOCK / WAVE);
+  const uint32_t total = plan.total, perIter = plan.unitsPerIter, dualStart = plan.dualStart;
+  DfCtx df;
+  df.base = plan.base;
+  for (uint32_t u = blockIdx.x * (PRIMAL_BLOCK / WAVE) + threadIdx.x / WAVE; u < total; u += waves) {
+    df.it = u / perIter;
+    const uint32_t r = u % perIter;
+    if (r < dualStart) {
+      uint32_t c = 0;
+      while (plan.waveStart[c + 1] <= r) c++;
+      df.color = c;
+      const uint32_t v = r - plan.waveStart[c];
+      const uint32_t heavyStart = w.colorStart[2 * c], lightStart = w.colorStart[2 * c + 1],
+                     lightEnd = w.colorStart[2 * c + 2];
+      const uint32_t hw = plan.heavyWaves[c];
+      if (v < hw)
+        primalGroup<HEAVY_LANES, true>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
+                                       lane % HEAVY_LANES, df);
+      else
+        primalGroup<LANES, true>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
+    } else {
+      const uint32_t q = (r - dualStart) * WAVE + lane;
+      if (q < plan.points) {
+        const uint32_t mk = w.pointMap[q];
+        dualPoint<true>(w, mk >> 3, mk & 7u, df);
+      }
+    }
+  }
+}
+
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
 
 static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) {
@@ -576,6 +761,19 @@ void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies
 }
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
   k_dual<<<blocks(uint64_t(w.nJoints) + uint64_t(manifoldCapacity) * 2, 256, 16384), 256, 0, s>>>(w);
+}
+uint32_t dataflow_grid() {
+  int device = 0, perCu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void *>(k_solve_dataflow),
+                                                   PRIMAL_BLOCK, 0) != hipSuccess)
+    return 0;
+  return uint32_t(perCu) * uint32_t(prop.multiProcessorCount);
+}
+void launch_dataflow(const DeviceWorld &w, uint32_t iterations, uint32_t grid, hipStream_t s) {
+  k_dataflow_plan<<<1, 64, 0, s>>>(w, iterations);
+  k_solve_dataflow<<<grid, PRIMAL_BLOCK, 0, s>>>(w);
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {
   k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);

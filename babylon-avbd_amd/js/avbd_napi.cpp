@@ -131,7 +131,7 @@ napi_value DeviceCount(napi_env env, napi_callback_info) {
  * createWorld(scenes, options): scenes = [{bodies: Float32Array, joints: Float32Array, jointInfo: Int32Array,
  * ignore: Uint32Array, filters: Uint32Array | null, dt, gravity, up: [3], alpha, gamma, betaLin, betaAng, iterations,
  * flags}]; options = {colorCap, colorRounds, pairCapacity, graph, device, adaptive, broadphase (0 auto, 1 grid,
- * 2 hploc), bvhRebuildInterval}. Returns the world handle (avbd_batch_create).
+ * 2 hploc), bvhRebuildInterval, solver (0 auto, 1 colour launches, 2 dataflow)}. Returns the world handle (avbd_batch_create).
  */
 napi_value CreateWorld(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -190,6 +190,7 @@ napi_value CreateWorld(napi_env env, napi_callback_info info) {
   o.adaptive = uint32_t(numberProp(env, argv[1], "adaptive", 0));
   o.broadphase = uint32_t(numberProp(env, argv[1], "broadphase", 0));
   o.bvh_rebuild_interval = uint32_t(numberProp(env, argv[1], "bvhRebuildInterval", 0));
+  o.solver = uint32_t(numberProp(env, argv[1], "solver", 0));
   avbd_world *w = nullptr;
   AVBD_OK_OR_THROW(avbd_batch_create(descs.data(), n, &o, &w));
   napi_value handle;
@@ -314,7 +315,8 @@ napi_value ReadCounters(napi_env env, napi_callback_info info) {
     uint32_t value;
   } fields[] = {{"pairs", c.pairs},       {"contacts", c.contacts}, {"manifolds", c.manifolds},
                 {"joints", c.joints},     {"overflow", c.overflow}, {"clashes", c.clashes},
-                {"colors", c.colors},     {"sleeping", 0},          {"pivotFallbacks", c.pivot_fallbacks}};
+                {"colors", c.colors},     {"sleeping", 0},          {"pivotFallbacks", c.pivot_fallbacks},
+                {"solveAborts", c.solve_aborts}};
   for (const auto &f : fields) {
     NAPI_OK(napi_create_uint32(env, f.value, &v));
     NAPI_OK(napi_set_named_property(env, o, f.name, v));

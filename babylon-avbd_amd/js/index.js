@@ -288,6 +288,7 @@ function nativeOptions(options) {
     adaptive: o.adaptive ? 1 : 0,
     broadphase: code,
     bvhRebuildInterval: (o.bvh && o.bvh.rebuildInterval) || 0,
+    solver: solverCode(o.solver),
   };
 }
 
@@ -345,6 +346,12 @@ class CommandQueue {
     this.count += other.count;
   }
   records() { return this.u32.subarray(0, this.count * 12); }
+}
+
+function solverCode(solver) {
+  const code = { auto: 0, colors: 1, dataflow: 2 }[solver === undefined ? 'auto' : solver];
+  if (code === undefined) throw Error(`Unknown solver: ${solver}`);
+  return code;
 }
 
 /** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */

@@ -52,6 +52,10 @@ def _options(options):
         raise ValueError(f"Unknown broadphase: {bp}")
     o.broadphase = {"auto": 0, "grid": 1, "hploc": 2}[bp]
     o.bvh_rebuild_interval = int((options.get("bvh") or {}).get("rebuildInterval", 0))
+    solver = options.get("solver", "auto")
+    if solver not in ("auto", "colors", "dataflow"):
+        raise ValueError(f"Unknown solver: {solver}")
+    o.solver = {"auto": 0, "colors": 1, "dataflow": 2}[solver]
     return o
 
 
@@ -331,7 +335,8 @@ class HipWorld:
         c = _native.Counters()
         check(self._lib.avbd_world_read_counters(self._h, ctypes.byref(c)))
         return dict(pairs=c.pairs, contacts=c.contacts, manifolds=c.manifolds, joints=c.joints, overflow=c.overflow,
-                    clashes=c.clashes, colors=c.colors, sleeping=0, pivotFallbacks=c.pivot_fallbacks)
+                    clashes=c.clashes, colors=c.colors, sleeping=0, pivotFallbacks=c.pivot_fallbacks,
+                    solveAborts=c.solve_aborts)
 
     def readPairs(self):
         """The last step's broadphase pairs (a > b) as an (n, 2) u32 array, in emission order."""

@@ -80,12 +80,16 @@ typedef struct AvbdOptions {
   uint32_t broadphase;      /* 0 auto (H-PLOC for one world of >= 10,000 bodies), 1 uniform grid, 2 H-PLOC BVH
                                (AppGpuSolver3D broadphase "auto" | "grid" | "hploc", appGpuSolver3D.js:333-402) */
   uint32_t bvh_rebuild_interval; /* H-PLOC rebuild period in steps, refit in between (0: 64; gpuHploc.js:62-80) */
+  uint32_t solver;          /* 0 auto (colour launches), 1 one launch per colour and a dual launch per iteration,
+                               2 dataflow: every iteration's colours and duals in one launch, each body waiting only
+                               for its neighbours (same result; batches without joints; needs the device to itself) */
 } AvbdOptions;
 
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */
 typedef struct AvbdCounters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, joints, bodies;
   uint32_t pivot_fallbacks; /* bodies whose 6x6 solve was redone in FP64 in the last step (a pivot hit its floor) */
+  uint32_t solve_aborts;    /* dataflow solve: waits given up in the last step (nonzero: that step is not valid) */
 } AvbdCounters;
 
 /** Thread-local message of the last failed call ("" if none). */

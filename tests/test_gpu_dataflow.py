@@ -1,0 +1,82 @@
+"""The dataflow solve (every iteration's colours and duals in one launch, each body waiting only for its neighbours)
+computes exactly what the colour-by-colour launches compute: bit-identical to the oracle (FP32 mirror) and to the
+launch path, alone, in a batch and under graph replay; and it never gives up a wait."""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleWorld
+from test_gpu_parity import compare_step
+
+pytestmark = pytest.mark.gpu
+
+
+def _world(avbd, scene, solver="dataflow", **opts):
+    return avbd.createSolverPipeline("hip").createSolver(scene, dict(colorCap=32, colorRounds=16, solver=solver, **opts))
+
+
+def test_trajectory_bit_exact_small_pile(avbd):
+    scene = lambda: avbd.configs.box_pile(counts=(4, 4, 4), seed=7)  # noqa: E731
+    world = _world(avbd, scene())
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    world.step(100)
+    oracle.step(100)
+    got = world.readBodies()
+    want = oracle.bodies().astype(np.float32)
+    assert world.readCounters()["solveAborts"] == 0
+    assert np.array_equal(got[:, 0:8], want[:, 0:8]), f"max diff {np.abs(got[:, :8] - want[:, :8]).max()}"
+    world.destroy()
+
+
+def test_steps_stay_in_parity_1000_pile(avbd):
+    scene = avbd.configs.pile_1000
+    world = _world(avbd, scene())
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    for s in range(40):
+        oracle.seed(world.readState())
+        world.step()
+        oracle.step()
+        compare_step(world, oracle)
+        assert world.readCounters()["solveAborts"] == 0
+    world.destroy()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_same_bits_as_the_colour_launches(avbd, graph):
+    scene = avbd.configs.pile_1000
+    a = _world(avbd, scene(), solver="colors", graph=graph)
+    b = _world(avbd, scene(), solver="dataflow", graph=graph)
+    for _ in range(4):
+        a.step(30)
+        b.step(30)
+        assert np.array_equal(a.readBodies(), b.readBodies())
+        assert b.readCounters()["solveAborts"] == 0
+    ma, ca = a.readContactRecords()
+    mb, cb = b.readContactRecords()
+    assert np.array_equal(ca[:, 12:15], cb[:, 12:15]) and np.array_equal(ma[:, :2], mb[:, :2])
+    a.destroy()
+    b.destroy()
+
+
+def test_batch_worlds_match_their_solo_runs(avbd):
+    scenes = [avbd.configs.box_pile(counts=(4, 4, 4), seed=s) for s in (1, 2, 3)]
+    batch = avbd.createSolverPipeline("hip").createBatch(scenes, dict(colorCap=32, colorRounds=16, solver="dataflow"))
+    batch.step(40)
+    got = batch.readBodies()
+    n = scenes[0].body_count
+    for k, s in enumerate((1, 2, 3)):
+        solo = _world(avbd, avbd.configs.box_pile(counts=(4, 4, 4), seed=s), solver="colors")
+        solo.step(40)
+        assert np.array_equal(got[k * n:(k + 1) * n], solo.readBodies()), f"world {k}"
+        solo.destroy()
+    batch.destroy()
+
+
+def test_joints_fall_back_to_colour_launches(avbd):
+    mix = lambda: avbd.configs.jointed_mix(n=6, spheres=30, capsules=20, area=3.0, capsule_grid=(4, 1, 5))  # noqa: E731
+    a = _world(avbd, mix(), solver="dataflow")
+    b = _world(avbd, mix(), solver="colors")
+    a.step(30)
+    b.step(30)
+    assert np.array_equal(a.readBodies(), b.readBodies())
+    a.destroy()
+    b.destroy()
