@@ -52,7 +52,7 @@ enum Counter : uint32_t {
   C_PREV_MANIFOLDS = 7,
   C_CANDIDATES = 8,
   C_PIVOT64 = 9,  // bodies solved again in FP64 this step (an FP32 pivot took its floor)
-  C_DF_ABORT = 10,  // the dataflow solve gave up a wait (never expected: the step's result is then not valid)
+  C_DF_ABORT = 10,  // a bounded wait gave up (dataflow solve, prefix scan; never expected: the step is then not valid)
   COUNTER_WORDS = 16
 };
 

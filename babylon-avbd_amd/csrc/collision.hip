@@ -389,37 +389,70 @@ __global__ void k_manifold_flags(DeviceWorld w) {
     w.mfOffset[p] = (p < P && (w.mfSlots[p].ids.w & 15u) != 0u) ? 1u : 0u;
 }
 
-__global__ void k_manifold_compact(DeviceWorld w) {
-  uint32_t P = w.counters[C_PAIRS];
-  // The loop bound is uniform per block (the wavefront reduction below needs every lane)
-  for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
-    uint32_t p = base + threadIdx.x;
+constexpr uint32_t COMPACT_BLOCK = 256, COMPACT_GRID = 512;
+
+/** Exclusive prefix of v over the 256 threads of a block, and the block total (every thread must call it). */
+AVBD_D uint32_t blockExclusive(uint32_t v, uint32_t &total, uint32_t *waveTot) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  __syncthreads();  // waveTot is free (an earlier call has read it)
+  if (lane == 63u) waveTot[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < COMPACT_BLOCK / 64; k++) {
+    if (k < wave) before += waveTot[k];
+    total += waveTot[k];
+  }
+  return before + incl - v;
+}
+
+/**
+ * Dense manifolds (at their scanned offsets) and the dense point list (manifold << 3 | point). Each block takes one
+ * contiguous range of pair slots: it counts the range's points, takes their range of the point list with one atomic,
+ * then writes tile by tile. The order of the blocks' ranges varies from run to run; the dual's result does not (every
+ * point is updated on its own).
+ */
+__global__ void __launch_bounds__(COMPACT_BLOCK) k_manifold_compact(DeviceWorld w) {
+  __shared__ uint32_t waveTot[COMPACT_BLOCK / 64];
+  __shared__ uint32_t rangeFirst;
+  const uint32_t P = w.counters[C_PAIRS];
+  const uint32_t per = (P + gridDim.x - 1) / gridDim.x;
+  const uint32_t chunk = (per + COMPACT_BLOCK - 1) / COMPACT_BLOCK * COMPACT_BLOCK;
+  const uint32_t lo = blockIdx.x * chunk;
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
+  if (lo >= P) return;  // the whole block
+  const uint32_t hi = umin(P, lo + chunk);
+  uint32_t mine = 0;
+  for (uint32_t p = lo + threadIdx.x; p < hi; p += COMPACT_BLOCK)
+    if (w.mfOffset[p + 1] != w.mfOffset[p]) mine += w.mfSlots[p].ids.w & 15u;
+  uint32_t total = 0;
+  blockExclusive(mine, total, waveTot);
+  if (threadIdx.x == 0) rangeFirst = total ? atomicAdd(&w.counters[C_CONTACTS], total) : 0u;
+  __syncthreads();
+  uint32_t running = rangeFirst;
+  for (uint32_t base = lo; base < hi; base += COMPACT_BLOCK) {  // the same turns for every thread of the block
+    const uint32_t p = base + threadIdx.x;
     uint32_t points = 0, o = 0;
-    if (p < P) {
+    if (p < hi) {
       o = w.mfOffset[p];
       if (w.mfOffset[p + 1] != o) {
-        Manifold m = w.mfSlots[p];
+        const Manifold m = w.mfSlots[p];
         w.manifolds[o] = m;
         points = m.ids.w & 15u;
       }
     }
-    // The wavefront's points take one range of the dense point list (one atomic per wavefront, which also counts
-    // the contact total); each manifold's points follow its wavefront prefix. The order of the ranges varies from
-    // run to run, the dual's result does not (every point is updated on its own).
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t incl = points;
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-      const uint32_t v = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += v;
-    }
-    const uint32_t total = __shfl(incl, 63, 64);
-    uint32_t first = 0;
-    if (lane == 63 && total) first = atomicAdd(&w.counters[C_CONTACTS], total);
-    first = __shfl(first, 63, 64) + incl - points;
+    uint32_t tileTotal = 0;
+    const uint32_t first = running + blockExclusive(points, tileTotal, waveTot);
     for (uint32_t k = 0; k < points; k++) w.pointMap[first + k] = (o << 3) | k;
+    running += tileTotal;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -445,12 +478,12 @@ void launch_grid(const DeviceWorld &w, hipStream_t s) {
   launch_fill_u32(w.gridCount, 0u, uint64_t(w.tableMask) + 2, s);
   launch_fill_u32(w.gridCursor, 0u, uint64_t(w.tableMask) + 1, s);
   k_grid_count<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
-  launch_scan_exclusive(w.gridCount, w.tableMask + 2, w.scanScratch, s);
+  launch_scan_exclusive(w.gridCount, w.tableMask + 2, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_grid_scatter<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
 }
 void launch_pairs(const DeviceWorld &w, hipStream_t s) {
   k_pairs<true><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_pairs<false><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
   k_pairs_total<<<1, 1, 0, s>>>(w);
 }
@@ -461,8 +494,8 @@ void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
   else
     k_narrowphase<false><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
   k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);
-  launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s);
-  k_manifold_compact<<<blocks(w.pairCapacity, 256, 4096), 256, 0, s>>>(w);
+  launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
+  k_manifold_compact<<<COMPACT_GRID, COMPACT_BLOCK, 0, s>>>(w);
 }
 
 }  // namespace avbd

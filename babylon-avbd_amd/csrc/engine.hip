@@ -165,6 +165,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.worlds = dallocT<WorldParams>(nWorlds_);
   w.refs = dallocT<float4>(2ull * nBodies_);
   w.bodyDone = dallocT<uint32_t>(nBodies_);
+  w.dualDone = dallocT<uint32_t>(nBodies_);
   w.dfPlan = dallocT<DataflowPlan>(1);
   w.filters = dallocT<uint2>(nBodies_);
   w.joints = dallocT<Joint>(nJoints_);
@@ -295,7 +296,6 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
   w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
   w.pointMap = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
-  w.pointDone = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
   w.mfOffset = static_cast<uint32_t *>(alloc((size_t(pairCap_) + 1) * 4));
   for (int k = 0; k < 2; k++) {
     manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
@@ -649,14 +649,17 @@ void Engine::readContacts(std::vector<Manifold> &manifolds, std::vector<Contact>
   HIP_CHECK(hipMalloc(&dense, size_t(m) * MAX_CONTACTS * sizeof(Contact)));
   HIP_CHECK(hipMalloc(&mout, size_t(m) * sizeof(Manifold)));
   HIP_CHECK(hipMemsetAsync(offsets, 0, (size_t(m) + 1) * 4, stream_));
+  HIP_CHECK(hipMemsetAsync(scratch, 0, size_t(words) * 4, stream_));  // the scan's tile statuses start at zero
   uint32_t g = std::min<uint32_t>((m + 255) / 256, 4096);
   k_contact_counts<<<g, 256, 0, stream_>>>(dm, m, offsets);
   launch_scan_exclusive(offsets, m + 1, scratch, stream_);
   k_gather_contacts<<<g, 256, 0, stream_>>>(dm, m, offsets, dc, dense, mout);
-  uint32_t total = 0;
+  uint32_t total = 0, gaveUp = 0;
   HIP_CHECK(hipMemcpyAsync(&total, offsets + m, 4, hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(&gaveUp, scratch + 2, 4, hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipMemcpyAsync(manifolds.data(), mout, size_t(m) * sizeof(Manifold), hipMemcpyDeviceToHost, stream_));
   synchronize();
+  if (gaveUp) total = 0;
   contacts.resize(total);
   if (total)
     HIP_CHECK(hipMemcpyAsync(contacts.data(), dense, size_t(total) * sizeof(Contact), hipMemcpyDeviceToHost, stream_));
@@ -665,6 +668,7 @@ void Engine::readContacts(std::vector<Manifold> &manifolds, std::vector<Contact>
   HIP_CHECK(hipFree(scratch));
   HIP_CHECK(hipFree(dense));
   HIP_CHECK(hipFree(mout));
+  if (gaveUp) throw std::runtime_error("reading contacts: the offset scan gave up a wait");
 }
 
 std::vector<uint32_t> Engine::readColors() {

@@ -7,12 +7,12 @@
 namespace avbd {
 
 /**
- * Work of one step's dataflow solve (written on the device before it): unit u = it × unitsPerIter + r is, for
- * r < dualStart, wave r - waveStart[c] of colour c's primal (its heavy waves first), else the dual's point block
- * r - dualStart (64 points). Stamps of the step count from `base` (each step's base is past the last one's stamps).
+ * Work of one step's dataflow solve (written on the device before it): unit u = it × unitsPerIter + r is wave
+ * r - waveStart[c] of colour c's primal (its heavy waves first) in iteration it, with the duals its bodies finish.
+ * Stamps of the step count from `base` (each step's base is past the last one's stamps).
  */
 struct DataflowPlan {
-  uint32_t base, iterations, unitsPerIter, dualStart, points, total;
+  uint32_t base, iterations, unitsPerIter, total;
   uint32_t waveStart[65];
   uint32_t heavyWaves[64];
 };
@@ -59,8 +59,8 @@ struct DeviceWorld {
   Manifold *prevManifolds;
   Contact *contacts;      // 8 per pair slot
   uint32_t *pointMap;     // the current contact points, densely: manifold index << 3 | point (C_CONTACTS of them)
-  uint32_t *pointDone;    // per contact slot: stamp of its last dual update (dataflow solve)
   uint32_t *bodyDone;     // per body: stamp of its last primal update (dataflow solve)
+  uint32_t *dualDone;     // per body: stamp of its last dual of the pairs it finishes (dataflow solve)
   DataflowPlan *dfPlan;
   Contact *prevContacts;
   uint32_t *table;        // last step's manifolds by pair hash (value = index + 1)

@@ -267,35 +267,71 @@ AVBD_D uint32_t combineMaxPos(uint32_t a, uint32_t b) {
   return umax(a & ~MAXPOS_STATIC, b & ~MAXPOS_STATIC) | ((a | b) & MAXPOS_STATIC);
 }
 
-/** Refit one level's treelets: nodes in creation order (children before parents); bounds and maxPos. */
-__global__ void k_bvh_refit_level(Bvh t, const BvhGroup *groups, uint32_t nGroups) {
-  uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= nGroups) return;
-  BvhGroup grp = groups[gi];
-  for (uint32_t j = 0; j + 1 < grp.count; j++) {
-    uint32_t nid = grp.nodeBase + j;
-    BvhNode n = t.nodes[nid];
-    uint32_t l = __float_as_uint(n.lo.w), r = __float_as_uint(n.hi.w);
-    BvhNode a = t.nodes[l];
-    BvhNode b = t.nodes[r];
-    t.nodes[nid] = BvhNode{mk4(fmin3(xyz(a.lo), xyz(b.lo)), n.lo.w), mk4(fmax3(xyz(a.hi), xyz(b.hi)), n.hi.w)};
-    t.maxPos[nid] = combineMaxPos(t.maxPos[l], t.maxPos[r]);
-  }
-}
-
 /** Tag of a child in a pair record: its node id, or for a leaf its body index | LEAF (leaf ids are body indices). */
 constexpr uint32_t LEAF = 0x80000000u;
 
-/** Pair records of the internal nodes [N, nodeCount) from the refitted boxes and maxPos. */
-__global__ void k_bvh_pairnodes(Bvh t, uint32_t nLeaves) {
-  uint32_t id = nLeaves + blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= t.nodeCount) return;
-  BvhNode n = t.nodes[id];
-  uint32_t l = __float_as_uint(n.lo.w), r = __float_as_uint(n.hi.w);
-  BvhNode A = t.nodes[l], B = t.nodes[r];
-  uint32_t tagA = l < nLeaves ? (l | LEAF) : l, tagB = r < nLeaves ? (r | LEAF) : r;
-  t.pairs[id] = BvhPair{mk4(xyz(A.lo), __uint_as_float(tagA)), mk4(xyz(A.hi), __uint_as_float(t.maxPos[l])),
-                        mk4(xyz(B.lo), __uint_as_float(tagB)), mk4(xyz(B.hi), __uint_as_float(t.maxPos[r]))};
+/**
+ * Refit one level, a 16-lane group per treelet (four a block): the lanes read their nodes' child ids and the boxes and
+ * maxPos of children outside the treelet in parallel; one lane then refits the nodes in creation order (children
+ * before parents) in LDS; the lanes write the nodes, their maxPos and their pair records.
+ */
+__global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup *groups, uint32_t nGroups,
+                                                           uint32_t nLeaves) {
+  __shared__ float4 nodeLo[64], nodeHi[64];         // refitted box of each node slot
+  __shared__ uint32_t nodeMp[64];
+  __shared__ float4 childLo[2][64], childHi[2][64];  // per node slot: its two children's boxes and maxPos
+  __shared__ uint32_t childMp[2][64], childId[2][64];
+  const uint32_t lane = threadIdx.x % WIDTH, g = threadIdx.x / WIDTH, slot = threadIdx.x;
+  const uint32_t gi = blockIdx.x * 4 + g;
+  const bool groupLive = gi < nGroups;
+  const BvhGroup grp = groupLive ? groups[gi] : BvhGroup{0, 0, 0, 0};
+  const uint32_t nodes = grp.count > 0 ? grp.count - 1 : 0;  // a treelet of k clusters holds k - 1 nodes
+  const bool live = groupLive && lane < nodes;
+  if (live) {
+    const BvhNode n = t.nodes[grp.nodeBase + lane];
+    const uint32_t ids[2] = {__float_as_uint(n.lo.w), __float_as_uint(n.hi.w)};
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      childId[k][slot] = ids[k];
+      if (ids[k] - grp.nodeBase >= nodes) {  // outside the treelet (a leaf or a lower level's root)
+        const BvhNode c = t.nodes[ids[k]];
+        childLo[k][slot] = c.lo;
+        childHi[k][slot] = c.hi;
+        childMp[k][slot] = t.maxPos[ids[k]];
+      }
+    }
+  }
+  __syncthreads();
+  if (groupLive && lane == 0) {
+    const uint32_t base = g * WIDTH;
+    for (uint32_t j = 0; j < nodes; j++) {
+      const uint32_t sj = base + j;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const uint32_t in = childId[k][sj] - grp.nodeBase;
+        if (in < nodes) {  // an earlier node of the treelet
+          childLo[k][sj] = nodeLo[base + in];
+          childHi[k][sj] = nodeHi[base + in];
+          childMp[k][sj] = nodeMp[base + in];
+        }
+      }
+      nodeLo[sj] = mk4(fmin3(xyz(childLo[0][sj]), xyz(childLo[1][sj])), 0.f);
+      nodeHi[sj] = mk4(fmax3(xyz(childHi[0][sj]), xyz(childHi[1][sj])), 0.f);
+      nodeMp[sj] = combineMaxPos(childMp[0][sj], childMp[1][sj]);
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const uint32_t nid = grp.nodeBase + lane;
+    const uint32_t l = childId[0][slot], r = childId[1][slot];
+    t.nodes[nid] = BvhNode{mk4(xyz(nodeLo[slot]), __uint_as_float(l)), mk4(xyz(nodeHi[slot]), __uint_as_float(r))};
+    t.maxPos[nid] = nodeMp[slot];
+    const uint32_t tagA = l < nLeaves ? (l | LEAF) : l, tagB = r < nLeaves ? (r | LEAF) : r;
+    t.pairs[nid] = BvhPair{mk4(xyz(childLo[0][slot]), __uint_as_float(tagA)),
+                           mk4(xyz(childHi[0][slot]), __uint_as_float(childMp[0][slot])),
+                           mk4(xyz(childLo[1][slot]), __uint_as_float(tagB)),
+                           mk4(xyz(childHi[1][slot]), __uint_as_float(childMp[1][slot]))};
+  }
 }
 
 /** A query's traversal stack in LDS: entry d of lane x at stack[d * blockDim.x + x] (no bank conflicts). */
@@ -462,7 +498,7 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   for (uint32_t pass = 0; pass < plan.sortPasses; pass++) {
     uint32_t shift = 8 * pass;
     k_sort_count<<<tiles, SORT_TILE, 0, s>>>(k0, n, shift, tiles, t.sortHist);
-    launch_scan_exclusive(t.sortHist, 256 * tiles + 1, w.scanScratch, s);
+    launch_scan_exclusive(t.sortHist, 256 * tiles + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
     k_sort_scatter<<<tiles, SORT_TILE, 0, s>>>(k0, v0, n, shift, tiles, t.sortHist, k1, v1);
     std::swap(k0, k1);
     std::swap(v0, v1);
@@ -481,16 +517,15 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   k_bvh_rank<<<blocks(n, 256), 256, 0, s>>>(w, t);
   k_bvh_leafmax<<<blocks(n, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
-    k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
-  k_bvh_pairnodes<<<blocks(plan.nodeCount - n, 256), 256, 0, s>>>(t, n);
+    k_bvh_refit_treelets<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount, n);
 }
 
 void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
   k_bvh_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   k_bvh_leafmax<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
-    k_bvh_refit_level<<<blocks(lev.groupCount, 64), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount);
-  k_bvh_pairnodes<<<blocks(plan.nodeCount - w.nBodies, 256), 256, 0, s>>>(t, w.nBodies);
+    k_bvh_refit_treelets<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount,
+                                                                   w.nBodies);
 }
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
@@ -498,7 +533,7 @@ void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
   k_bvh_cand<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
   k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_bvh_write<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
 }
 

@@ -7,8 +7,13 @@
 
 namespace avbd {
 
-/** In-place exclusive prefix sum of n u32 values; scratch holds at least scan_scratch_words(n) words. */
-void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s);
+/**
+ * In-place exclusive prefix sum of n u32 values in one launch. `scratch` holds at least scan_scratch_words(n) words,
+ * all zero before its first scan (each scan leaves it zero again); scans that share it run one after another on one
+ * stream. A wait given up (never expected) sets *abortFlag, or scratch word 2 when abortFlag is null.
+ */
+void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s,
+                           uint32_t *abortFlag = nullptr);
 uint32_t scan_scratch_words(uint32_t n);
 /** Fill n words with a value (a kernel, so it can be captured in a graph). */
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s);

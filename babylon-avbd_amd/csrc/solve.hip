@@ -280,6 +280,12 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
   return e;
 }
 
+/**
+ * Whether the dynamic body (colour ca, index a) finishes its pair with the body (colour cb, index b) in the dataflow
+ * solve: the later colour runs the pair's dual right after its primal, ties (a clash) go to the higher index.
+ */
+AVBD_D bool finishesPair(uint32_t ca, uint32_t a, uint32_t cb, uint32_t b) { return ca > cb || (ca == cb && a > b); }
+
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
 template <bool DF>
 AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
@@ -289,19 +295,17 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     return;
   }
   if constexpr (DF) {
-    // The partner's pose: this iteration's if its colour comes first, else the last iteration's; and the point's
-    // multipliers from the last iteration's dual. A partner of an earlier colour has already waited for the same
-    // points' dual before it published this iteration's pose, so its stamp covers them.
-    const bool dynamic = (head.z & ENTRY_DYNAMIC) != 0;
-    const bool earlier = dynamic && w.colorA[head.x] < df.color;
-    const uint32_t target = earlier ? df.it + 1u : df.it;
-    const bool needBody = dynamic && target != 0u, needPoint = df.it != 0u && !earlier;
-    if (needBody && needPoint)
-      waitFor2(w, &w.bodyDone[head.x], df.base + target, &w.pointDone[head.y + p], df.base + df.it);
-    else if (needBody)
-      waitFor(w, &w.bodyDone[head.x], df.base + target);
-    else if (needPoint)
-      waitFor(w, &w.pointDone[head.y + p], df.base + df.it);
+    // The partner's pose: this iteration's if its colour comes first, else the last iteration's. The point's
+    // multipliers come from the last iteration's dual, which the pair's finisher ran right after its primal: this
+    // body (its own stamp was waited for) or the partner, whose dual stamp then covers both.
+    if (head.z & ENTRY_DYNAMIC) {
+      const uint32_t pc = w.colorA[head.x];
+      if (pc < df.color)
+        waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
+      else if (df.it != 0u)
+        waitFor(w, finishesPair(pc, head.x, df.color, i) ? &w.dualDone[head.x] : &w.bodyDone[head.x],
+                df.base + df.it);
+    }
   }
   const bool isA = (head.z & ENTRY_IS_A) != 0;
   const PairBody other = pairBody<DF>(w, head.x);
@@ -335,7 +339,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
                        float dt, Acc &acc, const DfCtx &df) {
   const Body &B = w.bodies[i];
   if constexpr (DF) {
-    if (df.it) waitFor(w, &w.bodyDone[i], df.base + df.it);  // the body's own last iteration
+    if (df.it) waitFor(w, &w.dualDone[i], df.base + df.it);  // the body's own last iteration and its duals
   }
   const float4 bpos = DF ? ldAgent4(&B.pos) : B.pos;
   const float4 brot = DF ? ldAgent4(&B.rot) : B.rot;
@@ -491,7 +495,9 @@ AVBD_D void ldlt6Double(const float (&A)[21], const float (&b)[6], float (&x)[6]
  * first; a near-singular system (a pivot at its floor, as stiff joint chains give) is solved again in FP64 from the
  * same FP32 entries, with the same floor.
  */
-template <bool DF> AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df) {
+template <bool DF>
+AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df, float4 *newPos = nullptr,
+                       float4 *newRot = nullptr) {
   const float A[21] = {s.l00, s.l10, s.l11, s.l20, s.l21, s.l22, s.c[0], s.c[1], s.c[2], s.a00, s.c[3],
                        s.c[4], s.c[5], s.a10, s.a11, s.c[6], s.c[7], s.c[8], s.a20, s.a21, s.a22};
   const float b[6] = {s.rLin.x, s.rLin.y, s.rLin.z, s.rAng.x, s.rAng.y, s.rAng.z};
@@ -505,8 +511,10 @@ template <bool DF> AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, cons
   Body &B = w.bodies[i];
   if constexpr (DF) {
     const float4 pos = ldAgent4(&B.pos);
-    stAgent4(&B.pos, make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w));
-    stAgent4(&B.rot, qadd(ldAgent4(&B.rot), mk3(-x[3], -x[4], -x[5])));
+    *newPos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
+    *newRot = qadd(ldAgent4(&B.rot), mk3(-x[3], -x[4], -x[5]));
+    stAgent4(&B.pos, *newPos);
+    stAgent4(&B.rot, *newRot);
     storesDone();
     stAgent(&w.bodyDone[i], df.base + df.it + 1u);
   } else {
@@ -514,6 +522,101 @@ template <bool DF> AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, cons
     B.pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
     B.rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
   }
+}
+
+/**
+ * The dual update of contact slot c (dual, wgsl-solve.ts:474-535) from its record kc and its evaluation at this
+ * iteration's poses: the penalty ramp where the row is active, λ = the clamped force, and the stick flag.
+ */
+template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, const Contact &kc, const Eval &e,
+                                          const WorldParams &p) {
+  f3 pen = xyz(kc.pen);
+  if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
+  uint32_t key = fbits(kc.rA.w);
+  const bool ramp = e.frictionScale <= e.bounds;
+  if (ramp) {
+    pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
+    pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
+    bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
+    key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
+  }
+  if constexpr (DF) {
+    if (ramp) stAgent(reinterpret_cast<uint32_t *>(&w.contacts[c].rA.w), key);
+    stAgent4(&w.contacts[c].pen, mk4(pen, kc.pen.w));
+    stAgent4(&w.contacts[c].lam, mk4(e.F, kc.lam.w));
+  } else {
+    if (ramp) w.contacts[c].rA.w = bitsf(key);
+    w.contacts[c].pen = mk4(pen, kc.pen.w);
+    w.contacts[c].lam = mk4(e.F, kc.lam.w);
+  }
+}
+
+/**
+ * The dataflow solve's dual for the pairs body i finishes (its partner is static, of an earlier colour, or of its
+ * colour with a lower index), right after its primal: each such contact point is evaluated at the body's new pose
+ * (pos, rot, held by every lane of the group) and the partner's pose of this iteration (an earlier colour's was waited
+ * for by the primal), and updated as the dual pass does. The items are spread over the group's lanes as in
+ * accumulate. Then the body's dual stamp.
+ */
+template <uint32_t W>
+AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 pos, float4 rot, const DfCtx &df) {
+  const uint32_t i = slot.x, first = slot.y, end = slot.z;
+  const WorldParams &p = w.worlds[slot.w];
+  const Body &B = w.bodies[i];
+  PairBody self;
+  self.rot = rot;
+  self.irot = B.irot;
+  self.dLin = sub(xyz(pos), xyz(B.ipos));
+  self.dAng = qsub(rot, self.irot);
+  for (uint32_t c0 = first; c0 < end; c0 += W) {
+    const uint32_t e = c0 + lane;
+    uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t items = 0;
+    if (e < end) {
+      head = w.adjEntry[2ull * e];
+      tail = w.adjEntry[2ull * e + 1];
+      if (!(head.z & ENTRY_JOINT) &&
+          (!(head.z & ENTRY_DYNAMIC) || finishesPair(df.color, i, w.colorA[head.x], head.x)))
+        items = head.z & 15u;
+    }
+    uint32_t incl = items;
+#pragma unroll
+    for (uint32_t s = 1; s < W; s <<= 1) {
+      uint32_t o = __shfl_up(incl, s, W);
+      if (lane >= s) incl += o;
+    }
+    const uint32_t excl = incl - items;
+    const uint32_t total = __shfl(incl, W - 1, W);
+    for (uint32_t r0 = 0; r0 < total; r0 += W) {  // the same turns for every lane of the group
+      const uint32_t t = r0 + lane;
+      uint32_t owner = 0;
+#pragma unroll
+      for (uint32_t step = W / 2; step > 0; step >>= 1) {
+        const uint32_t ex = __shfl(excl, owner + step, W);
+        if (ex <= t) owner += step;
+      }
+      const uint4 h = make_uint4(__shfl(head.x, owner, W), __shfl(head.y, owner, W), __shfl(head.z, owner, W),
+                                 __shfl(head.w, owner, W));
+      const f3 normal = mk3(__uint_as_float(__shfl(tail.x, owner, W)), __uint_as_float(__shfl(tail.y, owner, W)),
+                            __uint_as_float(__shfl(tail.z, owner, W)));
+      const uint32_t pt = t - __shfl(excl, owner, W);
+      if (t < total) {
+        // A partner of this colour (a clash: the colouring ran out of colours) runs beside this body and is read as
+        // it stands, as the primal reads it; waiting for it could wait for a unit queued behind this one
+        const bool isA = (h.z & ENTRY_IS_A) != 0;
+        const PairBody other = pairBody<true>(w, h.x);
+        const uint32_t c = h.y + pt;
+        Contact kc = w.contacts[c];
+        kc.pen = ldAgent4(&w.contacts[c].pen);
+        kc.lam = ldAgent4(&w.contacts[c].lam);
+        const Eval ev = evalContact(kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
+                                    isA ? other : self, p.alpha);
+        dualUpdate<true>(w, c, kc, ev, p);
+      }
+    }
+  }
+  storesDone();
+  if (lane == 0) stAgent(&w.dualDone[i], df.base + df.it + 1u);
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;
@@ -533,7 +636,16 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
   }
   mergeTree<W>(acc);
-  if (live && lane == 0) finishBody<DF>(w, slot.x, acc, df);
+  if constexpr (DF) {
+    float4 pos = make_float4(0.f, 0.f, 0.f, 0.f), rot = pos;
+    if (live && lane == 0) finishBody<true>(w, slot.x, acc, df, &pos, &rot);
+    // The new pose from the group's lane 0 to all of its lanes
+    pos = make_float4(__shfl(pos.x, 0, W), __shfl(pos.y, 0, W), __shfl(pos.z, 0, W), __shfl(pos.w, 0, W));
+    rot = make_float4(__shfl(rot.x, 0, W), __shfl(rot.y, 0, W), __shfl(rot.z, 0, W), __shfl(rot.w, 0, W));
+    if (live) fusedDual<W>(w, slot, lane, pos, rot, df);
+  } else {
+    if (live && lane == 0) finishBody<false>(w, slot.x, acc, df);
+  }
 }
 
 /**
@@ -599,50 +711,15 @@ AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
   w.joints[j] = k;
 }
 
-/** Point k of manifold m (dual, wgsl-solve.ts:474-535): the penalty ramp where the row is active, λ = the clamped
- * force, and the stick flag. Points are independent of each other, so every point is a thread of its own. */
-template <bool DF = false> AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k, const DfCtx &df = DfCtx{}) {
+/** Point k of manifold m: every point is a thread of its own (the points are independent of each other). */
+AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   const Manifold mf = w.manifolds[m];
   const uint32_t c = mf.ids.z + k;
-  if constexpr (DF) {
-    // Both bodies' poses of this iteration (a static body's stamp is never written: it is not waited for)
-    const bool dynA = w.bodies[mf.ids.x].size.w > 0.f, dynB = w.bodies[mf.ids.y].size.w > 0.f;
-    const uint32_t t = df.base + df.it + 1u;
-    if (dynA && dynB)
-      waitFor2(w, &w.bodyDone[mf.ids.x], t, &w.bodyDone[mf.ids.y], t);
-    else if (dynA || dynB)
-      waitFor(w, &w.bodyDone[dynA ? mf.ids.x : mf.ids.y], t);
-  }
   const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
-  PairBody PA = pairBody<DF>(w, mf.ids.x), PB = pairBody<DF>(w, mf.ids.y);
+  PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
-  Contact kc = w.contacts[c];
-  if constexpr (DF) {
-    kc.pen = ldAgent4(&w.contacts[c].pen);
-    kc.lam = ldAgent4(&w.contacts[c].lam);
-  }
-  Eval e = evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha);
-  f3 pen = xyz(kc.pen);
-  if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
-  uint32_t key = fbits(kc.rA.w);
-  const bool ramp = e.frictionScale <= e.bounds;
-  if (ramp) {
-    pen.y = rmin(pen.y + p.betaLin * fabsf(e.C.y), PENALTY_MAX);
-    pen.z = rmin(pen.z + p.betaLin * fabsf(e.C.z), PENALTY_MAX);
-    bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
-    key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
-  }
-  if constexpr (DF) {
-    if (ramp) stAgent(reinterpret_cast<uint32_t *>(&w.contacts[c].rA.w), key);
-    stAgent4(&w.contacts[c].pen, mk4(pen, kc.pen.w));
-    stAgent4(&w.contacts[c].lam, mk4(e.F, kc.lam.w));
-    storesDone();
-    stAgent(&w.pointDone[c], df.base + df.it + 1u);
-  } else {
-    if (ramp) w.contacts[c].rA.w = bitsf(key);
-    w.contacts[c].pen = mk4(pen, kc.pen.w);
-    w.contacts[c].lam = mk4(e.F, kc.lam.w);
-  }
+  const Contact kc = w.contacts[c];
+  dualUpdate<false>(w, c, kc, evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha), p);
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */
@@ -678,8 +755,8 @@ __global__ void k_velocities(DeviceWorld w) {
 // --- dataflow solve: every iteration's colours and duals in one launch -------------------------------------------------
 
 /**
- * The step's work list for the dataflow solve (one thread): wave counts per colour from the colour buckets, the dual's
- * point blocks from the contact count, and the stamp base past the last step's stamps.
+ * The step's work list for the dataflow solve (one thread): wave counts per colour from the colour buckets, and the
+ * stamp base past the last step's stamps.
  */
 __global__ void k_dataflow_plan(DeviceWorld w, uint32_t iterations) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -695,51 +772,42 @@ __global__ void k_dataflow_plan(DeviceWorld w, uint32_t iterations) {
     waves += hw + (end - light + (WAVE / LANES) - 1) / (WAVE / LANES);
   }
   p.waveStart[w.colorCap] = waves;
-  p.dualStart = waves;
-  p.points = w.counters[C_CONTACTS];
-  p.unitsPerIter = waves + (p.points + WAVE - 1) / WAVE;
+  p.unitsPerIter = waves;
   p.total = p.unitsPerIter * iterations;
 }
 
 /**
  * Every iteration's primal (colour by colour) and dual in one launch. Unit u (see DataflowPlan) goes to wave
- * u mod (waves of the grid), and each wave takes its units in increasing order. A unit waits only for units before it
- * (a body for its neighbours of earlier colours in this iteration, of later colours in the last one, for its own last
- * iteration and for its points' last dual; a dual point for both bodies' poses of this iteration), so the solve
- * computes exactly what the colour-by-colour launches do. Every wave of the grid is resident at once (the grid is
- * sized by the occupancy), so the unit everything else waits for always runs; and every wait is bounded.
+ * u mod (waves of the grid), and each wave takes its units in increasing order. A unit is one colour wave of one
+ * iteration; each of its bodies runs the dual of the pairs it finishes right after its primal (fusedDual). A unit
+ * waits only for units before it (a body for its neighbours of earlier colours in this iteration, for the primal and
+ * dual of its neighbours of later colours in the last one, and for its own last iteration), so the solve computes
+ * exactly what the colour-by-colour launches and the dual pass do. Every wave of the grid is resident at once (the
+ * grid is sized by the occupancy), so the unit everything else waits for always runs; and every wait is bounded.
  */
 __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld w) {
   if (held(w)) return;
   const DataflowPlan &plan = *w.dfPlan;
   const uint32_t lane = threadIdx.x % WAVE;
   const uint32_t waves = gridDim.x * (PRIMAL_BLOCK / WAVE);
-  const uint32_t total = plan.total, perIter = plan.unitsPerIter, dualStart = plan.dualStart;
+  const uint32_t total = plan.total, perIter = plan.unitsPerIter;
   DfCtx df;
   df.base = plan.base;
   for (uint32_t u = blockIdx.x * (PRIMAL_BLOCK / WAVE) + threadIdx.x / WAVE; u < total; u += waves) {
     df.it = u / perIter;
     const uint32_t r = u % perIter;
-    if (r < dualStart) {
-      uint32_t c = 0;
-      while (plan.waveStart[c + 1] <= r) c++;
-      df.color = c;
-      const uint32_t v = r - plan.waveStart[c];
-      const uint32_t heavyStart = w.colorStart[2 * c], lightStart = w.colorStart[2 * c + 1],
-                     lightEnd = w.colorStart[2 * c + 2];
-      const uint32_t hw = plan.heavyWaves[c];
-      if (v < hw)
-        primalGroup<HEAVY_LANES, true>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
-                                       lane % HEAVY_LANES, df);
-      else
-        primalGroup<LANES, true>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
-    } else {
-      const uint32_t q = (r - dualStart) * WAVE + lane;
-      if (q < plan.points) {
-        const uint32_t mk = w.pointMap[q];
-        dualPoint<true>(w, mk >> 3, mk & 7u, df);
-      }
-    }
+    uint32_t c = 0;
+    while (plan.waveStart[c + 1] <= r) c++;
+    df.color = c;
+    const uint32_t v = r - plan.waveStart[c];
+    const uint32_t heavyStart = w.colorStart[2 * c], lightStart = w.colorStart[2 * c + 1],
+                   lightEnd = w.colorStart[2 * c + 2];
+    const uint32_t hw = plan.heavyWaves[c];
+    if (v < hw)
+      primalGroup<HEAVY_LANES, true>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
+                                     lane % HEAVY_LANES, df);
+    else
+      primalGroup<LANES, true>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
   }
 }
 

@@ -26,6 +26,13 @@ AVBD_D int2 endpoints(const DeviceWorld &w, uint32_t id) {
 
 // --- adjacency ---------------------------------------------------------------------------------------------------------
 
+__global__ void k_adj_clear(DeviceWorld w) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= w.nBodies; i += gridDim.x * blockDim.x) {
+    w.adjStart[i] = 0u;
+    if (i < w.nBodies) w.adjFill[i] = 0u;
+  }
+}
+
 __global__ void k_adj_degree(DeviceWorld w) {
   uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
   for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
@@ -59,12 +66,64 @@ AVBD_D uint64_t adjKey(const DeviceWorld &w, uint32_t b, uint32_t id) {
   return uint64_t(uint32_t(other + 2)) << 32;
 }
 
-/** Sort body b's list into key order, then write the entry records the colouring and the primal read. */
-__global__ void k_adj_sort(DeviceWorld w) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= w.nBodies) return;
+constexpr uint32_t ADJ_SORT_BLOCK = 64, ADJ_SORT_LDS = 32;
+
+/** Entry records of body b's slot e (adjacency id `id`): what the colouring and the primal read. */
+AVBD_D void writeEntry(const DeviceWorld &w, uint32_t b, uint32_t e, uint32_t id) {
+  uint4 head, tail;
+  if (id < w.nJoints) {
+    int4 f = w.info[id];
+    int other = f.y == int(b) ? f.z : f.y;
+    head = make_uint4(other < 0 ? ENTRY_NONE : uint32_t(other), id,
+                      ENTRY_JOINT | (dynamicBody(w, other) ? ENTRY_DYNAMIC : 0u), 0u);
+    tail = make_uint4(0u, 0u, 0u, id);
+  } else {
+    const Manifold mf = w.manifolds[id - w.nJoints];
+    bool isA = mf.ids.x == b;
+    uint32_t other = isA ? mf.ids.y : mf.ids.x;
+    head = make_uint4(other, mf.ids.z,
+                      (mf.ids.w & 15u) | (isA ? ENTRY_IS_A : 0u) | (dynamicBody(w, int(other)) ? ENTRY_DYNAMIC : 0u),
+                      __float_as_uint(mf.geo.w));
+    tail = make_uint4(__float_as_uint(mf.geo.x), __float_as_uint(mf.geo.y), __float_as_uint(mf.geo.z), id);
+  }
+  w.adjEntry[2ull * e] = head;
+  w.adjEntry[2ull * e + 1] = tail;
+}
+
+/**
+ * Sort body b's list into key order (keys are distinct, so the order is unique), then write the entry records. A list
+ * of up to ADJ_SORT_LDS entries is sorted in this thread's LDS column with its keys read once; a longer one in place.
+ */
+__global__ void __launch_bounds__(ADJ_SORT_BLOCK) k_adj_sort(DeviceWorld w) {
+  __shared__ uint64_t keys[ADJ_SORT_LDS][ADJ_SORT_BLOCK];
+  __shared__ uint32_t ids[ADJ_SORT_LDS][ADJ_SORT_BLOCK];
+  const uint32_t t = threadIdx.x;
+  uint32_t b = blockIdx.x * blockDim.x + t;
+  if (b >= w.nBodies) return;  // no block-wide synchronisation below
   uint32_t lo = w.adjStart[b], hi = w.adjStart[b + 1];
   if (hi > w.adjCapacity) hi = w.adjCapacity;
+  if (hi <= lo) return;
+  const uint32_t n = hi - lo;
+  if (n <= ADJ_SORT_LDS) {
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t id = w.adjList[lo + k];
+      const uint64_t key = adjKey(w, b, id);
+      uint32_t j = k;
+      while (j > 0 && key < keys[j - 1][t]) {
+        keys[j][t] = keys[j - 1][t];
+        ids[j][t] = ids[j - 1][t];
+        j--;
+      }
+      keys[j][t] = key;
+      ids[j][t] = id;
+    }
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t id = ids[k][t];
+      w.adjList[lo + k] = id;
+      writeEntry(w, b, lo + k, id);
+    }
+    return;
+  }
   for (uint32_t i = lo + 1; i < hi; i++) {
     uint32_t id = w.adjList[i];
     uint64_t key = adjKey(w, b, id);
@@ -77,27 +136,7 @@ __global__ void k_adj_sort(DeviceWorld w) {
     }
     w.adjList[j] = id;
   }
-  for (uint32_t e = lo; e < hi; e++) {
-    uint32_t id = w.adjList[e];
-    uint4 head, tail;
-    if (id < w.nJoints) {
-      int4 f = w.info[id];
-      int other = f.y == int(b) ? f.z : f.y;
-      head = make_uint4(other < 0 ? ENTRY_NONE : uint32_t(other), id,
-                        ENTRY_JOINT | (dynamicBody(w, other) ? ENTRY_DYNAMIC : 0u), 0u);
-      tail = make_uint4(0u, 0u, 0u, id);
-    } else {
-      const Manifold mf = w.manifolds[id - w.nJoints];
-      bool isA = mf.ids.x == b;
-      uint32_t other = isA ? mf.ids.y : mf.ids.x;
-      head = make_uint4(other, mf.ids.z,
-                        (mf.ids.w & 15u) | (isA ? ENTRY_IS_A : 0u) | (dynamicBody(w, int(other)) ? ENTRY_DYNAMIC : 0u),
-                        __float_as_uint(mf.geo.w));
-      tail = make_uint4(__float_as_uint(mf.geo.x), __float_as_uint(mf.geo.y), __float_as_uint(mf.geo.z), id);
-    }
-    w.adjEntry[2ull * e] = head;
-    w.adjEntry[2ull * e + 1] = tail;
-  }
+  for (uint32_t e = lo; e < hi; e++) writeEntry(w, b, e, w.adjList[e]);
 }
 
 // --- colouring ---------------------------------------------------------------------------------------------------------
@@ -212,15 +251,18 @@ __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
   if (lid < COLOR_KEYS) hist[lid] = 0;
   uint32_t b = blockIdx.x * COLOR_WG + lid;
   uint32_t key = ~0u;
+  bool clash = false;
   if (b < w.nBodies && dynamicBody(w, int(b))) {
     uint32_t s = w.colorA[b];
     uint32_t col = s & 0xffu;
-    if (s & PENDING) atomicAdd(&w.counters[C_CLASHES], 1u);
+    clash = (s & PENDING) != 0u;
     if (col == NO_COLOR) col = w.colorCap - 1;
     w.colorA[b] = col;
-    atomicMax(&w.counters[C_NUM_COLORS], col + 1);
     key = bucketKey(w, b, col);
   }
+  // One atomic per wavefront that holds a clash (the colour count comes from the buckets, k_color_starts)
+  const uint32_t clashes = uint32_t(__popcll(__ballot(clash)));
+  if ((lid & 63u) == 0u && clashes) atomicAdd(&w.counters[C_CLASHES], clashes);
   chunkKeys[lid] = key;
   __syncthreads();
   if (key != ~0u) {
@@ -235,10 +277,18 @@ __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
 }
 
 // 5. Bucket k starts at its first chunk's offset
-__global__ void k_color_starts(DeviceWorld w) {
+// and the colour count: one more than the highest colour with a body (either of its buckets non-empty)
+__global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w) {
+  __shared__ uint32_t used;
   uint32_t k = threadIdx.x;
-  if (k < COLOR_KEYS) w.colorStart[k] = w.colorHist[k * w.colorGroups];
+  if (k == 0) used = 0;
+  __syncthreads();
+  const uint32_t start = w.colorHist[k * w.colorGroups], next = w.colorHist[(k + 1) * w.colorGroups];
+  w.colorStart[k] = start;
   if (k == 0) w.colorStart[COLOR_KEYS] = w.colorHist[COLOR_KEYS * w.colorGroups];
+  if (next > start) atomicMax(&used, k / 2 + 1);
+  __syncthreads();
+  if (k == 0) w.counters[C_NUM_COLORS] = used;
 }
 
 // 6. Scatter: chunk offset for the body's bucket plus its rank
@@ -261,13 +311,12 @@ static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) 
 }
 
 void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
-  launch_fill_u32(w.adjStart, 0u, uint64_t(w.nBodies) + 1, s);
-  launch_fill_u32(w.adjFill, 0u, w.nBodies, s);
+  k_adj_clear<<<blocks(uint64_t(w.nBodies) + 1, 256, 4096), 256, 0, s>>>(w);
   uint64_t items = uint64_t(w.nJoints) + manifoldCapacity;
   k_adj_degree<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
-  launch_scan_exclusive(w.adjStart, w.nBodies + 1, w.scanScratch, s);
+  launch_scan_exclusive(w.adjStart, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_adj_fill<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
-  k_adj_sort<<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
+  k_adj_sort<<<blocks(w.nBodies, ADJ_SORT_BLOCK), ADJ_SORT_BLOCK, 0, s>>>(w);
 }
 
 void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s) {
@@ -279,7 +328,8 @@ void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s) {
     else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA);
   }
   k_color_count<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
-  launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s);
+  launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s,
+                        w.counters + C_DF_ABORT);
   k_color_starts<<<1, COLOR_KEYS, 0, s>>>(w);
   k_color_scatter<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
 }

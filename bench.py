@@ -2,8 +2,9 @@
 
 One "step" is one full AVBD step of the world (collision, adjacency, colouring, warm starts, 10 × (primal per colour,
 dual), velocities) with the bodies resident in HBM. Prints one JSON line (the driver's contract) with a `roofline`
-object for the dominant kernel (the per-colour primal) and a `cpu_baseline` object (the CPU restatement of the same
-step, timed on a bounded sample on this host).
+object for the dominant kernel (the solve: one dataflow launch for every iteration, or with --solver colors the
+per-colour primal) and a `cpu_baseline` object (the CPU restatement of the same step, timed on a bounded sample on
+this host's cores and on one thread).
 """
 import argparse
 import importlib
@@ -18,14 +19,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_primal_100k_boxes_traffic.json")
 
 
-def measured_traffic():
-    """HBM bytes per k_primal launch on the 100k-box world, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+def traffic_profile(kernel):
+    return os.path.join("profiles", f"{kernel}_100k_boxes_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 100k-box world, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     measurement (calibrated against a copy kernel of known size), or None."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(os.path.join(ROOT, traffic_profile(kernel))) as f:
             return float(json.load(f)["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
@@ -46,6 +50,18 @@ def primal_bytes_per_iteration(world):
     joints = int((~is_contact).sum())
     return dynamic * 160 + entries * 100 + points * 64 + joints * 128, dict(
         dynamic=dynamic, entries=entries, contact_reads=points, joint_entries=joints)
+
+
+def dual_bytes_per_iteration(world):
+    """Algorithmic bytes one iteration's dual pass moves: per manifold its 32 B record and both bodies' 64 B pose
+    spans; per contact point its 4 B list entry, its 64 B record read and 28 B written (force, penalty, key); per
+    joint its 128 B record read and 64 B written."""
+    mf, _ = world.readContactRecords()
+    counts = (mf[:, 3] & 15).astype(np.int64)
+    manifolds = int((counts > 0).sum())
+    points = int(counts.sum())
+    J = world.jointCount
+    return manifolds * (32 + 128) + points * (4 + 64 + 28) + J * 192, dict(manifolds=manifolds, points=points)
 
 
 def summarize_counters(directory, kernel, per_step):
@@ -73,28 +89,51 @@ def summarize_counters(directory, kernel, per_step):
             "mean_per_dispatch": {c: total[c] / max(len(ids), 1) for c in names}}
 
 
+def host_threads():
+    """Host cores this process may use: the OpenMP thread count the host sets, else the CPUs it is pinned to."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
-    """The sequential CPU restatement (FP32) of the same step, started from the GPU's state at the start of the timed
-    window and run for a bounded sample (whole steps until `budget_s` seconds or `max_steps`), one thread."""
+    """The CPU restatement (FP32) of the same step, started from the GPU's state at the start of the timed window and
+    run for a bounded sample (whole steps until `budget_s` seconds or `max_steps`): once on every host core (the
+    colour-parallel OpenMP variant: bodies of a colour, pairs and manifolds in parallel, the same results) and once
+    on one thread. The all-cores figure is the baseline."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import OracleWorld  # the checker, timed here as the CPU baseline
+    import oracle_lib  # the checker, timed here as the CPU baseline
 
     scene = scene_fn()
-    oracle = OracleWorld([scene], precision=32, color_cap=colour_cap, color_rounds=16)
-    oracle.seed(state)
-    steps = 0
-    t0 = time.perf_counter()
-    while steps < max_steps:
-        oracle.step(1)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
     n = scene.body_count
-    return dict(value=n * steps / dt, unit="body-steps/s", cores=1, kind="port",
-                sample=f"steps {first_step + 1}-{first_step + steps} of the same {n}-body world from the GPU's state "
-                       f"(the timed window's start), sequential FP32 restatement of the step (grid broadphase, "
-                       f"colour-ordered Gauss-Seidel), {dt:.1f} s")
+
+    def timed(threads):
+        oracle_lib.set_threads(threads)
+        try:
+            oracle = oracle_lib.OracleWorld([scene], precision=32, color_cap=colour_cap, color_rounds=16)
+            oracle.seed(state)
+            steps = 0
+            t0 = time.perf_counter()
+            while steps < max_steps:
+                oracle.step(1)
+                steps += 1
+                if time.perf_counter() - t0 > budget_s:
+                    break
+            return steps, time.perf_counter() - t0
+        finally:
+            oracle_lib.set_threads(1)
+
+    cores = host_threads()
+    steps_n, dt_n = timed(cores)
+    steps_1, dt_1 = timed(1)
+    what = (f"of the same {n}-body world from the GPU's state (the timed window's start), FP32 restatement of the "
+            f"step (grid broadphase, colour-ordered Gauss-Seidel)")
+    return dict(value=n * steps_n / dt_n, unit="body-steps/s", cores=cores, kind="port",
+                sample=f"steps {first_step + 1}-{first_step + steps_n} {what}, {cores} OpenMP threads "
+                       f"(colour-parallel), {dt_n:.1f} s",
+                single_thread=dict(value=n * steps_1 / dt_1, unit="body-steps/s", cores=1,
+                                   sample=f"steps {first_step + 1}-{first_step + steps_1}, one thread, {dt_1:.1f} s"))
 
 
 def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
@@ -326,10 +365,26 @@ def main():
     prof = world.profileStep()
     bytes_iter, parts = primal_bytes_per_iteration(world)
     iters = scene.params["iterations"]
-    primal_ms_iter = prof["primal"] / iters
-    achieved = bytes_iter / (primal_ms_iter * 1e-3) / 1e9
-    launches = prof["primalLaunches"] / iters
-    print(f"# profile: {json.dumps(prof)}; primal bytes/iteration {bytes_iter} {parts}", file=sys.stderr, flush=True)
+    dataflow = prof["primalLaunches"] == 1 and prof["dualLaunches"] == 0
+    if dataflow:
+        # One persistent launch runs every iteration's colours and duals: its bytes are both passes', 10 times
+        dual_iter, dual_parts = dual_bytes_per_iteration(world)
+        parts.update(dual_parts)
+        kernel = "k_solve_dataflow"
+        kernel_desc = "k_solve_dataflow (every iteration's primal colours and dual in one launch)"
+        launch_bytes = iters * (bytes_iter + dual_iter)
+        launch_ms = prof["primal"]
+        launches = 1.0 / iters
+        bytes_note = {"primal_bytes_per_iteration": bytes_iter, "dual_bytes_per_iteration": dual_iter}
+    else:
+        kernel = "k_primal"
+        kernel_desc = "k_primal (one colour's bodies)"
+        launches = prof["primalLaunches"] / iters
+        launch_bytes = bytes_iter / launches
+        launch_ms = prof["primal"] / max(prof["primalLaunches"], 1)
+        bytes_note = {"primal_bytes_per_iteration": bytes_iter}
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    print(f"# profile: {json.dumps(prof)}; {kernel} bytes/launch {launch_bytes} {parts}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -366,13 +421,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic() if args.workload == "large" else None,
-            "traffic_source": "profiles/k_primal_100k_boxes_traffic.json (rocprofv3 2 x FETCH_SIZE + WRITE_SIZE per "
-                              "launch, copy-kernel calibrated)",
-            "kernel": "k_primal (one colour's bodies)",
-            "bytes_per_iteration": bytes_iter,
+            "traffic": measured_traffic(kernel) if args.workload == "large" else None,
+            "traffic_source": f"{traffic_profile(kernel)} (rocprofv3 2 x FETCH_SIZE + WRITE_SIZE per launch, "
+                              "copy-kernel calibrated)",
+            "kernel": kernel_desc,
+            "bytes_per_launch": launch_bytes,
+            **bytes_note,
             "launches_per_iteration": launches,
-            "avg_launch_ms": prof["primal"] / max(prof["primalLaunches"], 1),
+            "avg_launch_ms": launch_ms,
         },
         "phases_ms": prof,
         "cpu_baseline": cpu,

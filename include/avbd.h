@@ -89,7 +89,8 @@ typedef struct AvbdOptions {
 typedef struct AvbdCounters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, joints, bodies;
   uint32_t pivot_fallbacks; /* bodies whose 6x6 solve was redone in FP64 in the last step (a pivot hit its floor) */
-  uint32_t solve_aborts;    /* dataflow solve: waits given up in the last step (nonzero: that step is not valid) */
+  uint32_t solve_aborts;    /* waits given up in the last step, by the dataflow solve or a prefix scan (nonzero: that
+                               step is not valid; never expected) */
 } AvbdCounters;
 
 /** Thread-local message of the last failed call ("" if none). */

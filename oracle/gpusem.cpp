@@ -7,7 +7,15 @@
 #include <cstring>
 #include <memory>
 
+#include <omp.h>
+
 #include "gpusem.hpp"
+
+namespace {
+struct OneThreadByDefault {
+  OneThreadByDefault() { omp_set_num_threads(1); }
+} oneThreadByDefault;
+}  // namespace
 
 using namespace oracle;
 using namespace oracle::gpu;
@@ -365,6 +373,10 @@ void gs_set_params(void *p, uint32_t k, const float *v) {
     q.flags = floatToBits(v[9]);
   }
 }
+
+/** Threads of the colour-parallel variant (OpenMP); 1 (the default) runs every loop in order. Results are the same. */
+void gs_set_threads(int threads) { omp_set_num_threads(threads < 1 ? 1 : threads); }
+int gs_get_threads() { return omp_get_max_threads(); }
 
 void gs_set_iterations(void *p, int iterations) {
   auto *h = static_cast<Handle *>(p);

@@ -184,6 +184,7 @@ template <class R> class World {
   }
 
   void updateRefs() {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n(); i++) {
       const WorldParams &p = P(i);
       if (!(p.flags & FLAG_REUSE_CONTACTS)) continue;
@@ -303,28 +304,37 @@ template <class R> class World {
       cellOf(i, c);
       grid[key(world[i], c[0], c[1], c[2])].push_back(i);
     }
-    std::vector<std::pair<uint32_t, uint32_t>> cand;
+    // Pairs (i, j), j < i, grouped by i: a small body's grid neighbours and the large bodies of its world below it; a
+    // large body's whole world below it. Each group sorted and unique, so the groups in order are the sorted pair set.
+    std::vector<std::vector<uint32_t>> below(N);
+#pragma omp parallel for schedule(dynamic, 256)
     for (uint32_t i = 0; i < N; i++) {
-      if (large[i]) continue;
-      int64_t c[3];
-      cellOf(i, c);
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            auto it = grid.find(key(world[i], c[0] + dx, c[1] + dy, c[2] + dz));
-            if (it == grid.end()) continue;
-            for (uint32_t j : it->second)
-              if (j < i) cand.push_back({i, j});
-          }
+      std::vector<uint32_t> &js = below[i];
+      if (large[i]) {
+        for (uint32_t j = worldBase[world[i]]; j < i; j++) js.push_back(j);
+      } else {
+        int64_t c[3];
+        cellOf(i, c);
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              auto it = grid.find(key(world[i], c[0] + dx, c[1] + dy, c[2] + dz));
+              if (it == grid.end()) continue;
+              for (uint32_t j : it->second)
+                if (j < i) js.push_back(j);
+            }
+        for (uint32_t l : largeOf[world[i]])
+          if (l < i) js.push_back(l);
+        std::sort(js.begin(), js.end());
+        js.erase(std::unique(js.begin(), js.end()), js.end());
+      }
+      size_t kept = 0;
+      for (uint32_t j : js)
+        if (acceptPair(i, j)) js[kept++] = j;
+      js.resize(kept);
     }
-    for (auto &lst : largeOf)
-      for (uint32_t l : lst)
-        for (uint32_t j = worldBase[world[l]]; j < N && world[j] == world[l]; j++)
-          if (j != l && (!large[j] || j < l)) cand.push_back({rmax(l, j), rmin(l, j)});
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    for (auto &p : cand)
-      if (acceptPair(p.first, p.second)) pairs.push_back(p);
+    for (uint32_t i = 0; i < N; i++)
+      for (uint32_t j : below[i]) pairs.push_back({i, j});
   }
 
   R contactPenaltyMin(uint32_t a, uint32_t b) const {
@@ -336,20 +346,36 @@ template <class R> class World {
     return rclamp(rmin(mA, mB) / (dt * dt), R(1), R(1e10f));
   }
 
-  std::map<std::pair<uint32_t, uint32_t>, uint32_t> prevIndex;
+  /** The last step's manifolds by pair: (a << 32 | b, index) sorted by key; a repeated pair keeps its last index. */
+  std::vector<std::pair<uint64_t, uint32_t>> prevIndex;
 
-  bool reuseContacts(uint32_t a, uint32_t b) {
-    auto it = prevIndex.find({a, b});
-    if (it == prevIndex.end()) return false;
-    const Manifold<R> prev = prevManifolds[it->second];
+  /** Index of the last step's manifold of pair (a, b), or -1. */
+  int64_t previousManifold(uint32_t a, uint32_t b) const {
+    const uint64_t k = uint64_t(a) << 32 | b;
+    auto it = std::upper_bound(prevIndex.begin(), prevIndex.end(), std::make_pair(k, UINT32_MAX));
+    if (it == prevIndex.begin() || (it - 1)->first != k) return -1;
+    return (it - 1)->second;
+  }
+
+  /** One pair's narrowphase output: its manifold (first filled in when appended) and points; count < 0: none. */
+  struct PairOut {
+    Manifold<R> m;
+    Contact<R> c[8];
+    int count = -1;
+  };
+  std::vector<PairOut> pairOuts;
+
+  bool reuseContacts(uint32_t a, uint32_t b, PairOut &o) const {
+    int64_t at = previousManifold(a, b);
+    if (at < 0) return false;
+    const Manifold<R> prev = prevManifolds[size_t(at)];
     uint32_t count = prev.count();
     uint32_t generated = prev.countw >> 4;
     uint32_t moved = rmax(bodies[a].stamp, bodies[b].stamp);
     if (count == 0 || generated < moved) return false;
     const WorldParams &p = P(a);
-    Manifold<R> m = prev;
-    m.first = uint32_t(contacts.size());
-    manifolds.push_back(m);
+    o.m = prev;
+    o.count = int(count);
     const Body<R> &A = bodies[a], &B = bodies[b];
     M3<R> basis = orthonormal(prev.normal);
     R alpha = R(p.alpha), gamma = R(p.gamma);
@@ -363,7 +389,7 @@ template <class R> class World {
       k.lam = scale(scale(k.lam, alpha), gamma);
       k.pen = V3<R>(rclampW(k.pen.x * gamma), rclampW(k.pen.y * gamma), rclampW(k.pen.z * gamma));
       k.pen.x = rmaxW(k.pen.x, pmin);
-      contacts.push_back(k);
+      o.c[i] = k;
     }
     return true;
   }
@@ -371,85 +397,100 @@ template <class R> class World {
   static R rclampW(R x) { return rmin(rmax(x, R(1)), R(1e10f)); }
   static R rmaxW(R a, R b) { return a > b ? a : b; }
 
-  void narrowphase() {
-    prevIndex.clear();
-    for (uint32_t m = 0; m < prevManifolds.size(); m++) prevIndex[{prevManifolds[m].a, prevManifolds[m].b}] = m;
-    for (auto &pr : pairs) {
-      uint32_t a = pr.first, b = pr.second;
-      const WorldParams &p = P(a);
-      if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b)) continue;
-      const Body<R> &BA = bodies[a], &BB = bodies[b];
-      Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
-      Sat<R> sat;
-      Found<R> found;
-      found = shapeContacts(BA, BB, A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
-      bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
-      bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
-      if (found.count == 0) continue;
-      Manifold<R> mf;
-      mf.a = a;
-      mf.b = b;
-      mf.first = uint32_t(contacts.size());
-      mf.countw = uint32_t(found.count) | (p.step << 4);
-      mf.normal = neg(sat.n);
-      mf.friction = std::sqrt(BA.friction * BB.friction);
-      manifolds.push_back(mf);
-      M3<R> basis = orthonormal(mf.normal);
-      bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
-      uint32_t prevFirst = 0, prevCount = 0;
-      auto it = prevIndex.find({a, b});
-      if (it != prevIndex.end()) {
-        prevFirst = prevManifolds[it->second].first;
-        prevCount = prevManifolds[it->second].count();
-      }
-      bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
-      R minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * R(2);
-      R pmin = contactPenaltyMin(a, b);
-      R alpha = R(p.alpha), gamma = R(p.gamma);
-      for (int i = 0; i < found.count; i++) {
-        V3<R> rA = rotate(conj(BA.rot), sub(found.xA[i], A.c));
-        V3<R> rB = rotate(conj(BB.rot), sub(found.xB[i], B.c));
-        V3<R> pen, lam;
-        uint32_t stick = 0;
-        int64_t j = -1;
-        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
-          if ((prevContacts[c].key & ~STICK_BIT) == found.feature[i]) {
+  /** One pair's contacts (wgsl-collision.ts:617-808) into `o`, from the bodies and the last step's points only. */
+  void pairContacts(uint32_t a, uint32_t b, PairOut &o) const {
+    o.count = -1;
+    const WorldParams &p = P(a);
+    if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(a, b, o)) return;
+    const Body<R> &BA = bodies[a], &BB = bodies[b];
+    Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
+    Sat<R> sat;
+    Found<R> found;
+    found = shapeContacts(BA, BB, A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
+    bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
+    bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
+    if (found.count == 0) return;
+    Manifold<R> &mf = o.m;
+    mf.a = a;
+    mf.b = b;
+    mf.first = 0;
+    mf.countw = uint32_t(found.count) | (p.step << 4);
+    mf.normal = neg(sat.n);
+    mf.friction = std::sqrt(BA.friction * BB.friction);
+    o.count = found.count;
+    M3<R> basis = orthonormal(mf.normal);
+    bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
+    uint32_t prevFirst = 0, prevCount = 0;
+    int64_t at = previousManifold(a, b);
+    if (at >= 0) {
+      prevFirst = prevManifolds[size_t(at)].first;
+      prevCount = prevManifolds[size_t(at)].count();
+    }
+    bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
+    R minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * R(2);
+    R pmin = contactPenaltyMin(a, b);
+    R alpha = R(p.alpha), gamma = R(p.gamma);
+    for (int i = 0; i < found.count; i++) {
+      V3<R> rA = rotate(conj(BA.rot), sub(found.xA[i], A.c));
+      V3<R> rB = rotate(conj(BB.rot), sub(found.xB[i], B.c));
+      V3<R> pen, lam;
+      uint32_t stick = 0;
+      int64_t j = -1;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+        if ((prevContacts[c].key & ~STICK_BIT) == found.feature[i]) {
+          j = c;
+          break;
+        }
+      if (j < 0 && matchNearest) {
+        R bestDist = R(0.05f) * minSide;
+        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+          R dist = length(sub(prevContacts[c].rA, rA));
+          if (dist <= bestDist) {
+            bestDist = dist;
             j = c;
-            break;
-          }
-        if (j < 0 && matchNearest) {
-          R bestDist = R(0.05f) * minSide;
-          for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
-            R dist = length(sub(prevContacts[c].rA, rA));
-            if (dist <= bestDist) {
-              bestDist = dist;
-              j = c;
-            }
           }
         }
-        if (j >= 0) {
-          const Contact<R> &pc = prevContacts[size_t(j)];
-          pen = pc.pen;
-          lam = pc.lam;
-          stick = pc.key & STICK_BIT;
-          if (stick != 0 && !anySphere) {
-            rA = pc.rA;
-            rB = pc.rB;
-          }
-        }
-        V3<R> d = sub(add(rotate(BA.rot, rA), A.c), add(rotate(BB.rot, rB), B.c));
-        Contact<R> k;
-        k.rA = rA;
-        k.rB = rB;
-        k.key = found.feature[i] | stick;
-        k.c0x = dot(row(basis, 0), d) + R(0.01f);
-        k.c0y = dot(row(basis, 1), d);
-        k.c0z = dot(row(basis, 2), d);
-        k.lam = scale(scale(lam, alpha), gamma);
-        k.pen = V3<R>(rclampW(pen.x * gamma), rclampW(pen.y * gamma), rclampW(pen.z * gamma));
-        k.pen.x = rmaxW(k.pen.x, pmin);
-        contacts.push_back(k);
       }
+      if (j >= 0) {
+        const Contact<R> &pc = prevContacts[size_t(j)];
+        pen = pc.pen;
+        lam = pc.lam;
+        stick = pc.key & STICK_BIT;
+        if (stick != 0 && !anySphere) {
+          rA = pc.rA;
+          rB = pc.rB;
+        }
+      }
+      V3<R> d = sub(add(rotate(BA.rot, rA), A.c), add(rotate(BB.rot, rB), B.c));
+      Contact<R> k;
+      k.rA = rA;
+      k.rB = rB;
+      k.key = found.feature[i] | stick;
+      k.c0x = dot(row(basis, 0), d) + R(0.01f);
+      k.c0y = dot(row(basis, 1), d);
+      k.c0z = dot(row(basis, 2), d);
+      k.lam = scale(scale(lam, alpha), gamma);
+      k.pen = V3<R>(rclampW(pen.x * gamma), rclampW(pen.y * gamma), rclampW(pen.z * gamma));
+      k.pen.x = rmaxW(k.pen.x, pmin);
+      o.c[i] = k;
+    }
+  }
+
+  /** Every pair's contacts (in parallel), then the manifolds and points appended in pair order. */
+  void narrowphase() {
+    prevIndex.resize(prevManifolds.size());
+    for (uint32_t m = 0; m < prevManifolds.size(); m++)
+      prevIndex[m] = {uint64_t(prevManifolds[m].a) << 32 | prevManifolds[m].b, m};
+    std::sort(prevIndex.begin(), prevIndex.end());
+    if (pairOuts.size() < pairs.size()) pairOuts.resize(pairs.size());
+#pragma omp parallel for schedule(dynamic, 128)
+    for (size_t q = 0; q < pairs.size(); q++) pairContacts(pairs[q].first, pairs[q].second, pairOuts[q]);
+    for (size_t q = 0; q < pairs.size(); q++) {
+      PairOut &o = pairOuts[q];
+      if (o.count < 0) continue;
+      o.m.first = uint32_t(contacts.size());
+      manifolds.push_back(o.m);
+      for (int i = 0; i < o.count; i++) contacts.push_back(o.c[i]);
     }
   }
 
@@ -520,6 +561,7 @@ template <class R> class World {
     const uint32_t N = uint32_t(n());
     std::vector<uint32_t> B(N);
     // 1. compaction (colorCompact)
+#pragma omp parallel for schedule(static)
     for (uint32_t b = 0; b < N; b++) {
       if (!dynamicBody(b)) {
         B[b] = NO_COLOR;
@@ -538,6 +580,7 @@ template <class R> class World {
       B[b] = col;
     }
     // 2. pending (colorMark)
+#pragma omp parallel for schedule(static)
     for (uint32_t b = 0; b < N; b++) {
       uint32_t col = B[b] & 0xffu;
       if (!dynamicBody(b)) {
@@ -555,7 +598,8 @@ template <class R> class World {
     for (uint32_t r = 0; r < colorRounds; r++) {
       const std::vector<uint32_t> &src = (r % 2 == 0) ? colorA : B;
       std::vector<uint32_t> &dst = (r % 2 == 0) ? B : colorA;
-      for (uint32_t b = 0; b < N; b++) {
+#pragma omp parallel for schedule(static)
+      for (uint32_t b = 0; b < N; b++) {  // Jacobi between the two buffers: bodies are independent
         uint32_t s = src[b];
         if (!(s & PENDING)) {
           dst[b] = s;
@@ -613,7 +657,9 @@ template <class R> class World {
   static bool hard(R stiffness) { return stiffness >= R(1e30f); }
 
   void warmStartJoints() {
-    for (Joint<R> &k : joints) {
+#pragma omp parallel for schedule(static)
+    for (size_t jj = 0; jj < joints.size(); jj++) {
+      Joint<R> &k = joints[jj];
       if (k.type != int32_t(T_JOINT)) continue;
       const WorldParams &p = P(uint32_t(k.b));
       k.c0Lin = jointLinC(k);
@@ -630,6 +676,7 @@ template <class R> class World {
   }
 
   void warmStartBodies() {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n(); i++) {
       Body<R> &b = bodies[i];
       const WorldParams &p = P(i);
@@ -857,6 +904,7 @@ template <class R> class World {
              true);
       xl = V3<R>(R(dl.x), R(dl.y), R(dl.z));
       xa = V3<R>(R(da.x), R(da.y), R(da.z));
+#pragma omp atomic
       pivotFallbacks++;
     }
     b.pos = sub(b.pos, xl);
@@ -915,6 +963,7 @@ template <class R> class World {
   }
 
   void updateVelocities() {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n(); i++) {
       Body<R> &b = bodies[i];
       const WorldParams &p = P(i);
@@ -950,13 +999,17 @@ template <class R> class World {
     warmStartBodies();
     for (int it = 0; it < iterations; it++) {
       for (uint32_t c = 0; c < colorCap; c++)
-        for (uint32_t s = colorStart[c]; s < colorStart[c + 1]; s++) {
+        // bodies still pending after the rounds (clashes) may share a colour with a neighbour: then all run in order
+#pragma omp parallel for schedule(dynamic, 64) if (clashes == 0)
+        for (uint32_t s = colorStart[c]; s < colorStart[c + 1]; s++) {  // a colour's bodies are independent
           uint32_t i = colorBodies[s];
           R alpha = R(P(i).alpha);
           Acc acc = accumulate(i, alpha);
           finishBody(i, acc);
         }
-      for (Joint<R> &k : joints) dualJoint(k, R(P(uint32_t(k.b)).alpha));
+#pragma omp parallel for schedule(static)
+      for (size_t j = 0; j < joints.size(); j++) dualJoint(joints[j], R(P(uint32_t(joints[j].b)).alpha));
+#pragma omp parallel for schedule(dynamic, 256)
       for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, R(P(manifolds[m].a).alpha));
     }
     updateVelocities();

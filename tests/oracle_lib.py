@@ -74,6 +74,8 @@ def lib():
             "gs_set_params": (None, [vp, ctypes.c_uint32, f32p]),
             "gs_set_iterations": (None, [vp, ctypes.c_int]),
             "gs_set_filters": (None, [vp, u32p, u32p, u32p, ctypes.c_uint32]),
+            "gs_set_threads": (None, [ctypes.c_int]),
+            "gs_get_threads": (ctypes.c_int, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -81,6 +83,15 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's colour-parallel variant (1 = every loop in order; the results are the same)."""
+    lib().gs_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().gs_get_threads())
 
 
 def P(a, t):

@@ -357,3 +357,28 @@ def test_rope_joints_hold(avbd):
         pb = b[i + 1, 0:3] + rot(b[i + 1, 4:8], [-0.5, 0, 0])
         err = max(err, np.linalg.norm(pa - pb))
     assert err < 0.02, f"joint error {err}"
+
+
+@pytest.mark.parametrize("cap", [16, 3])
+def test_colour_parallel_variant_equals_in_order(avbd, cap):
+    """The OpenMP variant of the oracle (the all-cores CPU timing) steps to the same bytes as one thread: bodies of a
+    colour, pairs and manifolds are independent, and a colouring with clashes (cap 3 forces them) solves in order."""
+    from oracle_lib import get_threads, set_threads
+    scenes = [avbd.configs.pile_of(400, seed=11), avbd.configs.jointed_mix(n=8, spheres=60, capsules=60, area=8.0)]
+    out = []
+    for threads in (1, 4):
+        set_threads(threads)
+        try:
+            assert get_threads() == threads
+            w = OracleWorld(scenes, precision=32, color_cap=cap, color_rounds=4)
+            w.step(40)
+            out.append((w.bodies(), w.joints(), w.contact_words(), w.counts()))
+        finally:
+            set_threads(1)
+    (b1, j1, c1, n1), (b4, j4, c4, n4) = out
+    assert n1 == n4
+    if cap == 3:
+        assert n1["clashes"] > 0
+    np.testing.assert_array_equal(b1, b4)
+    np.testing.assert_array_equal(j1, j4)
+    np.testing.assert_array_equal(c1, c4)
