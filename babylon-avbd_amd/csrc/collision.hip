@@ -6,6 +6,8 @@
 // Design for MI355X: pair emission is count → scan → write (no atomic append), each body's pairs sorted by partner, so
 // the pair list, and every manifold and contact slot derived from it, is the same every run. The narrowphase writes a
 // pair's manifold and its contacts into fixed slots (contacts at 8 × pair index), and a scan compacts the manifolds.
+#include <algorithm>
+
 #include "engine_types.h"
 #include "kernels.h"
 #include "narrowphase.h"
@@ -34,9 +36,7 @@ __global__ void k_begin_frame(DeviceWorld w) {
   }
 }
 
-__global__ void k_update_refs(DeviceWorld w) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies) return;
+AVBD_D void updateRef(const DeviceWorld &w, uint32_t i) {
   const WorldParams &p = w.worlds[w.bodyWorld[i]];
   if (!(p.flags & FLAG_REUSE_CONTACTS)) return;
   f3 pos = xyz(w.bodies[i].pos);
@@ -56,8 +56,20 @@ __global__ void k_update_refs(DeviceWorld w) {
 
 AVBD_D uint32_t pairHash(uint32_t a, uint32_t b) { return hash32((a * 0x9e3779b1u) ^ hash32(b)); }
 
-__global__ void k_hash_clear(DeviceWorld w) {
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h <= w.hashMask; h += gridDim.x * blockDim.x) w.table[h] = 0;
+/**
+ * Each body's contact-reuse reference (above), the cleared pair table, and the cleared adjacency counters (the
+ * manifold compaction counts degrees into them), in one pass.
+ */
+__global__ void k_refs_and_clear(DeviceWorld w) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= w.nBodies; i += stride) {
+    if (i < w.nBodies) {
+      updateRef(w, i);
+      w.adjFill[i] = 0u;
+    }
+    w.adjStart[i] = 0u;
+  }
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h <= w.hashMask; h += stride) w.table[h] = 0;
 }
 
 __global__ void k_hash_insert(DeviceWorld w) {
@@ -383,12 +395,6 @@ template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narro
   }
 }
 
-__global__ void k_manifold_flags(DeviceWorld w) {
-  uint32_t P = w.counters[C_PAIRS];
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= w.pairCapacity; p += gridDim.x * blockDim.x)
-    w.mfOffset[p] = (p < P && (w.mfSlots[p].ids.w & 15u) != 0u) ? 1u : 0u;
-}
-
 constexpr uint32_t COMPACT_BLOCK = 256, COMPACT_GRID = 512;
 
 /** Exclusive prefix of v over the 256 threads of a block, and the block total (every thread must call it). */
@@ -414,44 +420,63 @@ AVBD_D uint32_t blockExclusive(uint32_t v, uint32_t &total, uint32_t *waveTot) {
 }
 
 /**
- * Dense manifolds (at their scanned offsets) and the dense point list (manifold << 3 | point). Each block takes one
- * contiguous range of pair slots: it counts the range's points, takes their range of the point list with one atomic,
- * then writes tile by tile. The order of the blocks' ranges varies from run to run; the dual's result does not (every
- * point is updated on its own).
+ * Dense manifolds and the dense point list (manifold << 3 | point). Each block takes one contiguous range of pair
+ * slots: it counts the range's manifolds and points, takes their ranges of the two lists with one atomic each, then
+ * writes tile by tile (a tile's manifold and point offsets from one packed block scan). The order of the blocks' ranges
+ * varies from run to run; nothing downstream depends on it (adjacency lists are sorted, the hash finds manifolds by
+ * pair, every point is updated on its own). Each manifold and active joint adds to its dynamic ends' adjacency
+ * degrees (cleared by k_refs_and_clear).
  */
 __global__ void __launch_bounds__(COMPACT_BLOCK) k_manifold_compact(DeviceWorld w) {
   __shared__ uint32_t waveTot[COMPACT_BLOCK / 64];
-  __shared__ uint32_t rangeFirst;
+  __shared__ uint32_t mFirst, pFirst;
   const uint32_t P = w.counters[C_PAIRS];
   const uint32_t per = (P + gridDim.x - 1) / gridDim.x;
   const uint32_t chunk = (per + COMPACT_BLOCK - 1) / COMPACT_BLOCK * COMPACT_BLOCK;
   const uint32_t lo = blockIdx.x * chunk;
-  if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[C_MANIFOLDS] = w.mfOffset[P];
-  if (lo >= P) return;  // the whole block
-  const uint32_t hi = umin(P, lo + chunk);
-  uint32_t mine = 0;
-  for (uint32_t p = lo + threadIdx.x; p < hi; p += COMPACT_BLOCK)
-    if (w.mfOffset[p + 1] != w.mfOffset[p]) mine += w.mfSlots[p].ids.w & 15u;
-  uint32_t total = 0;
-  blockExclusive(mine, total, waveTot);
-  if (threadIdx.x == 0) rangeFirst = total ? atomicAdd(&w.counters[C_CONTACTS], total) : 0u;
-  __syncthreads();
-  uint32_t running = rangeFirst;
-  for (uint32_t base = lo; base < hi; base += COMPACT_BLOCK) {  // the same turns for every thread of the block
-    const uint32_t p = base + threadIdx.x;
-    uint32_t points = 0, o = 0;
-    if (p < hi) {
-      o = w.mfOffset[p];
-      if (w.mfOffset[p + 1] != o) {
-        const Manifold m = w.mfSlots[p];
-        w.manifolds[o] = m;
+  if (lo < P) {  // the whole block
+    const uint32_t hi = umin(P, lo + chunk);
+    uint32_t packed = 0;  // points << 16 | manifolds of this thread's slots
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += COMPACT_BLOCK) {
+      const uint32_t n = w.mfSlots[p].ids.w & 15u;
+      if (n) packed += (n << 16) | 1u;
+    }
+    uint32_t total = 0;
+    blockExclusive(packed, total, waveTot);
+    if (threadIdx.x == 0) {
+      mFirst = (total & 0xffffu) ? atomicAdd(&w.counters[C_MANIFOLDS], total & 0xffffu) : 0u;
+      pFirst = (total >> 16) ? atomicAdd(&w.counters[C_CONTACTS], total >> 16) : 0u;
+    }
+    __syncthreads();
+    uint32_t mRun = mFirst, pRun = pFirst;
+    for (uint32_t base = lo; base < hi; base += COMPACT_BLOCK) {  // the same turns for every thread of the block
+      const uint32_t p = base + threadIdx.x;
+      Manifold m;
+      uint32_t points = 0;
+      if (p < hi) {
+        m = w.mfSlots[p];
         points = m.ids.w & 15u;
       }
+      uint32_t tileTotal = 0;
+      const uint32_t off = blockExclusive(points ? (points << 16) | 1u : 0u, tileTotal, waveTot);
+      if (points) {
+        const uint32_t o = mRun + (off & 0xffffu), first = pRun + (off >> 16);
+        w.manifolds[o] = m;
+        for (uint32_t k = 0; k < points; k++) w.pointMap[first + k] = (o << 3) | k;
+        // The adjacency degree of each dynamic end
+        if (w.bodies[m.ids.x].size.w > 0.f) atomicAdd(&w.adjStart[m.ids.x], 1u);
+        if (w.bodies[m.ids.y].size.w > 0.f) atomicAdd(&w.adjStart[m.ids.y], 1u);
+      }
+      mRun += tileTotal & 0xffffu;
+      pRun += tileTotal >> 16;
     }
-    uint32_t tileTotal = 0;
-    const uint32_t first = running + blockExclusive(points, tileTotal, waveTot);
-    for (uint32_t k = 0; k < points; k++) w.pointMap[first + k] = (o << 3) | k;
-    running += tileTotal;
+  }
+  // Active joints' degrees (a joint is active while either stiffness is non-zero)
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < w.nJoints; j += gridDim.x * blockDim.x) {
+    const int4 f = w.info[j];
+    if (f.x == T_NONE || (w.joints[j].penLin.w == 0.f && w.joints[j].penAng.w == 0.f)) continue;
+    if (f.y >= 0 && w.bodies[f.y].size.w > 0.f) atomicAdd(&w.adjStart[f.y], 1u);
+    if (f.z >= 0 && w.bodies[f.z].size.w > 0.f) atomicAdd(&w.adjStart[f.z], 1u);
   }
 }
 
@@ -467,11 +492,9 @@ static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) 
 void launch_begin_frame(const DeviceWorld &w, hipStream_t s) {
   k_begin_frame<<<blocks(w.nWorlds, 256), 256, 0, s>>>(w);
 }
-void launch_update_refs(const DeviceWorld &w, hipStream_t s) {
-  k_update_refs<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
-}
-void launch_hash_build(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s) {
-  k_hash_clear<<<blocks(uint64_t(w.hashMask) + 1, 256, 4096), 256, 0, s>>>(w);
+void launch_refs_and_hash(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s) {
+  const uint64_t items = std::max<uint64_t>(w.nBodies, uint64_t(w.hashMask) + 1);
+  k_refs_and_clear<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
   k_hash_insert<<<blocks(prevCapacity, 256, 2048), 256, 0, s>>>(w);
 }
 void launch_grid(const DeviceWorld &w, hipStream_t s) {
@@ -493,9 +516,9 @@ void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
     k_narrowphase<true><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
   else
     k_narrowphase<false><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
-  k_manifold_flags<<<blocks(uint64_t(w.pairCapacity) + 1, 256, 4096), 256, 0, s>>>(w);
-  launch_scan_exclusive(w.mfOffset, w.pairCapacity + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
-  k_manifold_compact<<<COMPACT_GRID, COMPACT_BLOCK, 0, s>>>(w);
+  // Ranges of at most 4096 slots, so a range's manifold and point counts each fit 16 bits of the packed scan
+  const uint32_t grid = std::max<uint32_t>(COMPACT_GRID, (w.pairCapacity + 4095u) / 4096u);
+  k_manifold_compact<<<grid, COMPACT_BLOCK, 0, s>>>(w);
 }
 
 }  // namespace avbd

@@ -190,6 +190,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.adjFill = dallocT<uint32_t>(nBodies_);
   w.colorA = dallocT<uint32_t>(nBodies_);
   w.colorB = dallocT<uint32_t>(nBodies_);
+  w.colorFlags = dallocT<uint32_t>(COLOR_FLAG_WORDS);
   w.colorRank = dallocT<uint32_t>(nBodies_);
   w.colorHist = dallocT<uint32_t>(uint64_t(COLOR_KEYS) * colorGroups_ + 1);
   w.colorStart = dallocT<uint32_t>(COLOR_KEYS + 1);
@@ -240,8 +241,10 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
   bvh_.nodeCount = plan_.nodeCount;
   bvh_.nodes = dallocT<BvhNode>(plan_.nodeCount);
   bvh_.pairs = dallocT<BvhPair>(plan_.nodeCount);
-  if (plan_.height > 250) throw std::length_error("BVH too deep for the traversal stack");
-  bvh_.stackDepth = plan_.height;
+  bvh_.quads = dallocT<BvhQuad>(plan_.nodeCount - n);
+  if (plan_.height > 160) throw std::length_error("BVH too deep for the traversal stack");
+  // A walk descends two levels per quad and keeps at most three entries of each quad on its stack
+  bvh_.stackDepth = 3 * ((plan_.height + 1) / 2) + 1;
   bvh_.order = dallocT<uint32_t>(n);
   bvh_.rank = dallocT<uint32_t>(n);
   bvh_.maxPos = dallocT<uint32_t>(plan_.nodeCount);
@@ -296,7 +299,6 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
   w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
   w.pointMap = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
-  w.mfOffset = static_cast<uint32_t *>(alloc((size_t(pairCap_) + 1) * 4));
   for (int k = 0; k < 2; k++) {
     manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
     contactBuf_[k] = static_cast<Contact *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(Contact)));
@@ -361,13 +363,11 @@ void Engine::resetGraphs() {
 void Engine::encodeCollision(const DeviceWorld &w, bool rebuild) {
   hipStream_t s = stream_;
   launch_begin_frame(w, s);
-  launch_update_refs(w, s);
-  launch_hash_build(w, pairCap_, s);
+  launch_refs_and_hash(w, pairCap_, s);
   if (useBvh_) {
     if (rebuild) launch_bvh_build(w, bvh_, plan_, s);
     else launch_bvh_refit(w, bvh_, plan_, s);
-    launch_bvh_pairs(w, bvh_, s);
-    launch_pairs_total(w, s);
+    launch_bvh_pairs(w, bvh_, s);  // writes the pair count
   } else {
     launch_grid(w, s);
     launch_pairs(w, s);
@@ -379,10 +379,10 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
   hipStream_t s = stream_;
   encodeCollision(w, rebuild);
   launch_adjacency(w, pairCap_, s);
-  launch_coloring(w, colorRounds_, s);
+  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, s);
   launch_warm_start(w, s);
   if (useDataflow_) {
-    launch_dataflow(w, iterations_, dfGrid_, s);
+    launch_dataflow(w, dfGrid_, s);
   } else {
     for (uint32_t it = 0; it < iterations_; it++) {
       for (uint32_t c = 0; c < colorCap_; c++) launch_primal(w, c, maxDynamicBodies_, s);
@@ -484,15 +484,13 @@ PhaseTimes Engine::profileStep() {
   HIP_CHECK(hipEventRecord(ev[0], s));
   // encodeCollision, with events after the frame set-up and after the broadphase
   launch_begin_frame(w, s);
-  launch_update_refs(w, s);
-  launch_hash_build(w, pairCap_, s);
+  launch_refs_and_hash(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 4], s));
   if (useBvh_) {
     if (rebuild) launch_bvh_build(w, bvh_, plan_, s);
     else launch_bvh_refit(w, bvh_, plan_, s);
     HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 6], s));
-    launch_bvh_pairs(w, bvh_, s);
-    launch_pairs_total(w, s);
+    launch_bvh_pairs(w, bvh_, s);  // writes the pair count
   } else {
     launch_grid(w, s);
     HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 6], s));
@@ -503,7 +501,7 @@ PhaseTimes Engine::profileStep() {
   HIP_CHECK(hipEventRecord(ev[1], s));
   launch_adjacency(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2], s));
-  launch_coloring(w, colorRounds_, s);
+  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, s);
   HIP_CHECK(hipEventRecord(ev[3], s));
   launch_warm_start(w, s);
   HIP_CHECK(hipEventRecord(ev[4], s));
@@ -511,7 +509,7 @@ PhaseTimes Engine::profileStep() {
   if (useDataflow_) {
     // One launch for every iteration: its time is reported as the primal's, the dual's is in it
     HIP_CHECK(hipEventRecord(ev[k++], s));
-    launch_dataflow(w, iterations_, dfGrid_, s);
+    launch_dataflow(w, dfGrid_, s);
     HIP_CHECK(hipEventRecord(ev[k++], s));
   } else {
     for (uint32_t it = 0; it < iterations_; it++) {

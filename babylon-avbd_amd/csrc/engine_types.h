@@ -54,7 +54,6 @@ struct DeviceWorld {
 
   // Manifolds: per pair slot (narrowphase output), compacted list (current), last step's list, scan scratch
   Manifold *mfSlots;
-  uint32_t *mfOffset;  // pairCapacity + 1
   Manifold *manifolds;
   Manifold *prevManifolds;
   Contact *contacts;      // 8 per pair slot
@@ -76,6 +75,7 @@ struct DeviceWorld {
   // Colouring: state A, state B, rank, histogram (64 × groups + 1), starts (65), bucketed bodies
   uint32_t *colorA;
   uint32_t *colorB;
+  uint32_t *colorFlags;   // [0] the colouring state's buffer (0 colorA, 1 colorB), [1 + r] round r has work
   uint32_t *colorRank;
   uint32_t *colorHist;
   uint32_t *colorStart;
@@ -102,6 +102,7 @@ constexpr int COLOR_WG = 256;
 constexpr uint32_t LANES = 8;
 constexpr uint32_t HEAVY_LANES = 32;
 constexpr uint32_t COLOR_KEYS = 2 * MAX_COLORS;
+constexpr uint32_t COLOR_FLAG_WORDS = 64;  // colorFlags: rounds past 61 always run
 
 /**
  * Adjacency entry record, written once per step after the per-body sort (2 × uint4 = 32 B):

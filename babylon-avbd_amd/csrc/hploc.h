@@ -18,6 +18,13 @@ struct BvhPair {
   float4 aLo, aHi, bLo, bHi;
 };
 
+/** An internal node as the traversal reads it, with its grandchildren lifted: four entries (a leaf child itself, or
+ * an internal child's two children; an unused entry's box is empty), each box with its tag (node id, or body | LEAF)
+ * in lo.w and its subtree maxPos in hi.w. One 128-byte load decides up to four descents, two tree levels at a time. */
+struct BvhQuad {
+  float4 lo[4], hi[4];
+};
+
 /** One treelet of a level: its input clusters [inStart, inStart + count), output slot, first node it creates. */
 struct BvhGroup {
   uint32_t inStart, count, outIndex, nodeBase;
@@ -39,6 +46,7 @@ struct BvhPlan {
 struct Bvh {
   BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
   BvhPair *pairs;      // per internal node: its children's boxes, tags and maxPos (leaves' entries unused)
+  BvhQuad *quads;      // per internal node id - N: its four lifted entries (what the pair search walks)
   uint32_t *order;     // bodies in Morton order (the last build's)
   uint32_t *rank;      // per body: its position in `order`
   uint32_t *maxPos;    // per node: highest rank of a dynamic leaf below it, | MAXPOS_STATIC if a static leaf is below
@@ -54,7 +62,7 @@ struct Bvh {
   uint32_t *sortHist;  // 256 × tiles + 1
   BvhGroup *groups;    // every level's groups
   uint32_t nodeCount;
-  uint32_t stackDepth; // traversal stack entries per query (the plan's height bound)
+  uint32_t stackDepth; // traversal stack entries per query (three per two levels of the plan's height bound)
 };
 
 /** Candidate leaves kept per query; a query with more is answered by a second walk that tests as it goes. */

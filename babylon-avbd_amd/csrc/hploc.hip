@@ -32,15 +32,18 @@ AVBD_D f3 bodyHalf(const Body &b) {
 }
 
 /** Leaf boxes from the current poses, padded so that float rounding never drops a pair the exact test accepts. */
-__global__ void k_bvh_leaves(DeviceWorld w, Bvh t) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies) return;
+AVBD_D void leafBox(const DeviceWorld &w, const Bvh &t, uint32_t i) {
   const Body &b = w.bodies[i];
   f3 p = xyz(b.pos);
   f3 h = bodyHalf(b);
   float pad = 1e-3f + 1e-6f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + h.x + h.y + h.z);
   h = add(h, mk3(pad, pad, pad));
   t.nodes[i] = BvhNode{mk4(sub(p, h), __uint_as_float(i)), mk4(add(p, h), __uint_as_float(END))};
+}
+
+__global__ void k_bvh_leaves(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < w.nBodies) leafBox(w, t, i);
 }
 
 AVBD_D uint32_t expandBits(uint32_t v) {
@@ -262,6 +265,13 @@ __global__ void k_bvh_leafmax(DeviceWorld w, Bvh t) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < w.nBodies) t.maxPos[i] = w.bodies[i].size.w > 0.f ? t.rank[i] : MAXPOS_STATIC;
 }
+/** A refit's leaves: boxes from the current poses and maxPos from the last build's ranks. */
+__global__ void k_bvh_refit_leaves(DeviceWorld w, Bvh t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies) return;
+  leafBox(w, t, i);
+  t.maxPos[i] = w.bodies[i].size.w > 0.f ? t.rank[i] : MAXPOS_STATIC;
+}
 
 AVBD_D uint32_t combineMaxPos(uint32_t a, uint32_t b) {
   return umax(a & ~MAXPOS_STATIC, b & ~MAXPOS_STATIC) | ((a | b) & MAXPOS_STATIC);
@@ -334,22 +344,47 @@ __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup
   }
 }
 
+/** The quads of the internal nodes [N, nodeCount) from their pair records (after a refit wrote them). */
+__global__ void k_bvh_quads(Bvh t, uint32_t nLeaves) {
+  const uint32_t id = nLeaves + blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= t.nodeCount) return;
+  const BvhPair p = t.pairs[id];
+  const float4 emptyLo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), emptyHi = make_float4(-INFINITY, -INFINITY,
+                                                                                              -INFINITY, 0.f);
+  BvhQuad qd;
+  const uint32_t tagA = __float_as_uint(p.aLo.w), tagB = __float_as_uint(p.bLo.w);
+  if (tagA & LEAF) {
+    qd.lo[0] = p.aLo, qd.hi[0] = p.aHi, qd.lo[1] = emptyLo, qd.hi[1] = emptyHi;
+  } else {
+    const BvhPair c = t.pairs[tagA];
+    qd.lo[0] = c.aLo, qd.hi[0] = c.aHi, qd.lo[1] = c.bLo, qd.hi[1] = c.bHi;
+  }
+  if (tagB & LEAF) {
+    qd.lo[2] = p.bLo, qd.hi[2] = p.bHi, qd.lo[3] = emptyLo, qd.hi[3] = emptyHi;
+  } else {
+    const BvhPair c = t.pairs[tagB];
+    qd.lo[2] = c.aLo, qd.hi[2] = c.aHi, qd.lo[3] = c.bLo, qd.hi[3] = c.bHi;
+  }
+  t.quads[id - nLeaves] = qd;
+}
+
 /** A query's traversal stack in LDS: entry d of lane x at stack[d * blockDim.x + x] (no bank conflicts). */
 extern __shared__ uint32_t bvhStack[];
 
 /**
  * Walk body i's world tree depth first and call visit(j) for each leaf j whose padded box overlaps i's
- * (gpuHploc.js:37-55). Each internal node is one 64-byte load holding both children's boxes; a child leaf is
- * visited on the spot, an internal child descended into (the other one kept on the stack, which never holds more
- * entries than the tree is high). Every static leaf is visited, but of two dynamic bodies only the one earlier in
- * Morton order (rank q) visits the other: subtrees holding no static leaf and no dynamic leaf later than q are
- * skipped (maxPos), so each dynamic pair comes up once. A one-body world has no pair.
+ * (gpuHploc.js:37-55). Each step is one 128-byte quad load holding up to four boxes two levels down: a leaf entry is
+ * visited on the spot, the first internal entry descended into and the others kept on the stack. Every static leaf is
+ * visited, but of two dynamic bodies only the one earlier in Morton order (rank q) visits the other: subtrees holding
+ * no static leaf and no dynamic leaf later than q are skipped (maxPos), so each dynamic pair comes up once. A one-body
+ * world has no pair.
  */
 template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint32_t i, uint32_t q, F &&visit) {
   const BvhNode self = t.nodes[i];
   const f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
   const uint32_t root = t.roots[w.bodyWorld[i]];
-  if (root < w.nBodies) return;
+  const uint32_t N = w.nBodies;
+  if (root < N) return;
   auto hit = [&](float4 lo, float4 hi) {
     const uint32_t mp = __float_as_uint(hi.w);
     return !(hi.x < qlo.x || hi.y < qlo.y || hi.z < qlo.z || lo.x > qhi.x || lo.y > qhi.y || lo.z > qhi.z) &&
@@ -359,22 +394,18 @@ template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint
   uint32_t at = root, sp = 0;
   // Each internal node is entered at most once; the bound only guards against a malformed tree
   for (uint32_t guard = 0; guard < t.nodeCount; guard++) {
-    const BvhPair nd = t.pairs[at];
-    const uint32_t tagA = __float_as_uint(nd.aLo.w), tagB = __float_as_uint(nd.bLo.w);
-    bool a = hit(nd.aLo, nd.aHi), b = hit(nd.bLo, nd.bHi);
-    if (a && (tagA & LEAF)) {
-      visit(tagA & ~LEAF);
-      a = false;
+    const BvhQuad nd = t.quads[at - N];
+    uint32_t next = END;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (!hit(nd.lo[k], nd.hi[k])) continue;
+      const uint32_t tag = __float_as_uint(nd.lo[k].w);
+      if (tag & LEAF) visit(tag & ~LEAF);
+      else if (next == END) next = tag;
+      else if (sp < t.stackDepth) stack[(sp++) * blockDim.x] = tag;
     }
-    if (b && (tagB & LEAF)) {
-      visit(tagB & ~LEAF);
-      b = false;
-    }
-    if (a) {
-      if (b && sp < t.stackDepth) stack[(sp++) * blockDim.x] = tagB;
-      at = tagA;
-    } else if (b) {
-      at = tagB;
+    if (next != END) {
+      at = next;
     } else {
       if (sp == 0) break;
       at = stack[(--sp) * blockDim.x];
@@ -399,12 +430,19 @@ __global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
     });
   }
   t.candCount[i] = n;
+  if (n > BVH_CAND) {  // more candidates than kept: walk again, testing as it goes, and count the pairs
+    uint32_t pairs = 0;
+    walkTree(w, t, i, q, [&](uint32_t j) {
+      if (acceptPair(w, umax(i, j), umin(i, j))) pairs++;
+    });
+    w.pairCount[i] = pairs;
+  }
 }
 
 /**
  * Pass 2: the pair predicate on every kept candidate, 32 lanes per body (two bodies a wave), 32 candidates a round;
  * each round's ballot is a word of the body's accepted mask and the mask's population its pair count. A body with
- * more candidates than it kept is counted by the overflow pass instead.
+ * more candidates than it kept was counted by pass 1.
  */
 __global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
   const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -429,22 +467,20 @@ __global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
   if (live && lane == 0 && count <= BVH_CAND) w.pairCount[i] = pairs;
 }
 
-/** Pass 2b: bodies with more candidates than kept walk again, testing as they go, and count their pairs. */
-__global__ void k_bvh_overflow_count(DeviceWorld w, Bvh t) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= w.nBodies) return;
-  uint32_t i = t.order[q];
-  if (t.candCount[i] <= BVH_CAND) return;
-  uint32_t n = 0;
-  walkTree(w, t, i, q, [&](uint32_t j) {
-    if (acceptPair(w, umax(i, j), umin(i, j))) n++;
-  });
-  w.pairCount[i] = n;
-}
-
-/** Pass 3 (after the scan of the counts): write each body's pairs (a > b) in candidate order. */
+/**
+ * Pass 3 (after the scan of the counts): write each body's pairs (a > b) in candidate order; and the pair count,
+ * clamped to the capacity (overflow bit 0 when it is not).
+ */
 __global__ void k_bvh_write(DeviceWorld w, Bvh t) {
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) {
+    uint32_t total = w.pairCount[w.nBodies];
+    if (total > w.pairCapacity) {
+      atomicOr(&w.counters[C_OVERFLOW], 1u);
+      total = w.pairCapacity;
+    }
+    w.counters[C_PAIRS] = total;
+  }
   if (q >= w.nBodies) return;
   uint32_t i = t.order[q];
   uint32_t base = w.pairCount[i];
@@ -518,21 +554,22 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   k_bvh_leafmax<<<blocks(n, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_treelets<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount, n);
+  if (plan.nodeCount > n) k_bvh_quads<<<blocks(plan.nodeCount - n, 256), 256, 0, s>>>(t, n);
 }
 
 void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
-  k_bvh_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
-  k_bvh_leafmax<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
+  k_bvh_refit_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_treelets<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount,
                                                                    w.nBodies);
+  if (plan.nodeCount > w.nBodies)
+    k_bvh_quads<<<blocks(plan.nodeCount - w.nBodies, 256), 256, 0, s>>>(t, w.nBodies);
 }
 
 void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
   const size_t stackBytes = size_t(64) * t.stackDepth * sizeof(uint32_t);
   k_bvh_cand<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
-  k_bvh_overflow_count<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_bvh_write<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
 }

@@ -20,17 +20,23 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s);
 
 // Collision
 void launch_begin_frame(const DeviceWorld &w, hipStream_t s);
-void launch_update_refs(const DeviceWorld &w, hipStream_t s);
-void launch_hash_build(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s);
+/** Contact-reuse references (moved bodies restamped) and the last step's manifolds by pair in the hash table. */
+void launch_refs_and_hash(const DeviceWorld &w, uint32_t prevCapacity, hipStream_t s);
 void launch_grid(const DeviceWorld &w, hipStream_t s);
 void launch_pairs(const DeviceWorld &w, hipStream_t s);
 /** C_PAIRS from the scanned per-body counts (capacity-clamped, overflow bit 0). */
 void launch_pairs_total(const DeviceWorld &w, hipStream_t s);
+/** Contacts of every pair, then the dense manifold and point lists (and the adjacency counters cleared). */
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s);
 
 // Topology and colouring
+/** Per-body adjacency (after launch_narrowphase, which clears its counters). */
 void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
-void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s);
+/**
+ * Colouring and the per-colour buckets; with planIterations > 0 also the dataflow solve's work list. In a world
+ * without joints the last pass also warm-starts the bodies (launch_warm_start then has nothing left to do).
+ */
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, hipStream_t s);
 
 // Solve
 void launch_warm_start(const DeviceWorld &w, hipStream_t s);
@@ -39,8 +45,11 @@ void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s)
 void launch_velocities(const DeviceWorld &w, hipStream_t s);
 /** Blocks of the dataflow solve that are resident at once on the current device (0 if unknown). */
 uint32_t dataflow_grid();
-/** Every iteration's primal and dual in one launch of `grid` blocks (at most dataflow_grid()). */
-void launch_dataflow(const DeviceWorld &w, uint32_t iterations, uint32_t grid, hipStream_t s);
+/**
+ * Every iteration's primal and dual in one launch of `grid` blocks (at most dataflow_grid()), after a launch_coloring
+ * that wrote the work list.
+ */
+void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s);
 
 // Edits between steps
 /** Body commands (3 uint4 each) grouped by body: ranges[r] = (first command, count), each body's in submission

@@ -7,10 +7,10 @@
 // untouched), as the reference's protectSolverStep does (src/gpu/gpuSolverKernels.js:72-103).
 #include "engine_types.h"
 #include "kernels.h"
+#include "stepsetup.h"
 
 namespace avbd {
 
-AVBD_D bool held(const DeviceWorld &w) { return w.counters[C_OVERFLOW] != 0u; }
 
 // --- device-scope access for the dataflow solve -------------------------------------------------------------------------
 // Within one launch, waves on different XCDs exchange poses and contact multipliers. Those are read and written with
@@ -119,35 +119,7 @@ __global__ void k_warm_joints(DeviceWorld w) {
 __global__ void k_warm_bodies(DeviceWorld w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nBodies || held(w)) return;
-  const WorldParams &p = w.worlds[w.bodyWorld[i]];
-  Body &B = w.bodies[i];
-  float4 pos = B.pos, rot = B.rot, vel = B.vel;
-  f3 angVel = xyz(B.angVel);
-  float dt = p.dt, g = p.gravity;
-  f3 up = mk3(p.up[0], p.up[1], p.up[2]);
-  bool dyn = B.size.w > 0.f;
-  f3 inertialPos = add(xyz(pos), scale(xyz(vel), dt));
-  if (dyn) inertialPos = add(inertialPos, scale(up, g * (dt * dt)));
-  B.inertialPos = mk4(inertialPos, B.inertialPos.w);
-  B.inertialRot = qadd(rot, scale(angVel, dt));
-  float wgt = 0.f;
-  if (fabsf(g) > 0.f) {
-    float x = (dot(xyz(vel), up) - vel.w) / dt * rsign(g) / fabsf(g);
-    wgt = rmin(rmax(x, 0.f), 1.f);
-  }
-  bool slow = length(xyz(vel)) + length(angVel) * B.moment.w < REST_SPEED;
-  bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
-  B.ipos = pos;
-  B.irot = rot;
-  if (dyn && !atRest) {
-    B.pos = mk4(add(add(xyz(pos), scale(xyz(vel), dt)), scale(up, g * (wgt * dt * dt))), pos.w);
-    B.rot = qadd(rot, scale(angVel, dt));
-  }
-  bool moving = vel.x != 0.f || vel.y != 0.f || vel.z != 0.f || angVel.x != 0.f || angVel.y != 0.f || angVel.z != 0.f;
-  if (!dyn && moving) {
-    B.pos = mk4(add(xyz(pos), scale(xyz(vel), dt)), pos.w);
-    B.rot = qadd(rot, scale(angVel, dt));
-  }
+  warmBody(w, i);
 }
 
 // --- primal --------------------------------------------------------------------------------------------------------
@@ -620,7 +592,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;
-constexpr uint32_t WAVE = 64;
+constexpr uint32_t WAVE = DF_WAVE;
 
 /** Bucket position k (of a colour half ending at `end`) solved by a group of W lanes; `lane` is the lane in the group. */
 template <uint32_t W, bool DF = false>
@@ -754,27 +726,6 @@ __global__ void k_velocities(DeviceWorld w) {
 
 // --- dataflow solve: every iteration's colours and duals in one launch -------------------------------------------------
 
-/**
- * The step's work list for the dataflow solve (one thread): wave counts per colour from the colour buckets, and the
- * stamp base past the last step's stamps.
- */
-__global__ void k_dataflow_plan(DeviceWorld w, uint32_t iterations) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  DataflowPlan &p = *w.dfPlan;
-  p.base = p.base + p.iterations + 1u;
-  p.iterations = iterations;
-  uint32_t waves = 0;
-  for (uint32_t c = 0; c < w.colorCap; c++) {
-    const uint32_t heavy = w.colorStart[2 * c], light = w.colorStart[2 * c + 1], end = w.colorStart[2 * c + 2];
-    const uint32_t hw = (light - heavy + (WAVE / HEAVY_LANES) - 1) / (WAVE / HEAVY_LANES);
-    p.waveStart[c] = waves;
-    p.heavyWaves[c] = hw;
-    waves += hw + (end - light + (WAVE / LANES) - 1) / (WAVE / LANES);
-  }
-  p.waveStart[w.colorCap] = waves;
-  p.unitsPerIter = waves;
-  p.total = p.unitsPerIter * iterations;
-}
 
 /**
  * Every iteration's primal (colour by colour) and dual in one launch. Unit u (see DataflowPlan) goes to wave
@@ -821,7 +772,8 @@ static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) 
 }
 
 void launch_warm_start(const DeviceWorld &w, hipStream_t s) {
-  if (w.nJoints) k_warm_joints<<<blocks(w.nJoints, 256), 256, 0, s>>>(w);
+  if (!w.nJoints) return;  // the colouring's scatter pass warmed the bodies
+  k_warm_joints<<<blocks(w.nJoints, 256), 256, 0, s>>>(w);
   k_warm_bodies<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
 }
 void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s) {
@@ -839,8 +791,7 @@ uint32_t dataflow_grid() {
     return 0;
   return uint32_t(perCu) * uint32_t(prop.multiProcessorCount);
 }
-void launch_dataflow(const DeviceWorld &w, uint32_t iterations, uint32_t grid, hipStream_t s) {
-  k_dataflow_plan<<<1, 64, 0, s>>>(w, iterations);
+void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s) {
   k_solve_dataflow<<<grid, PRIMAL_BLOCK, 0, s>>>(w);
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {

@@ -6,6 +6,7 @@
 // colours the same way alone and inside a batch.
 #include "engine_types.h"
 #include "kernels.h"
+#include "stepsetup.h"
 
 namespace avbd {
 
@@ -25,23 +26,6 @@ AVBD_D int2 endpoints(const DeviceWorld &w, uint32_t id) {
 }
 
 // --- adjacency ---------------------------------------------------------------------------------------------------------
-
-__global__ void k_adj_clear(DeviceWorld w) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= w.nBodies; i += gridDim.x * blockDim.x) {
-    w.adjStart[i] = 0u;
-    if (i < w.nBodies) w.adjFill[i] = 0u;
-  }
-}
-
-__global__ void k_adj_degree(DeviceWorld w) {
-  uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
-  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
-    if (id < w.nJoints && !jointActive(w, id)) continue;
-    int2 e = endpoints(w, id);
-    if (dynamicBody(w, e.x)) atomicAdd(&w.adjStart[e.x], 1u);
-    if (dynamicBody(w, e.y)) atomicAdd(&w.adjStart[e.y], 1u);
-  }
-}
 
 __global__ void k_adj_fill(DeviceWorld w) {
   uint32_t total = w.nJoints + w.counters[C_MANIFOLDS];
@@ -105,22 +89,35 @@ __global__ void __launch_bounds__(ADJ_SORT_BLOCK) k_adj_sort(DeviceWorld w) {
   if (hi <= lo) return;
   const uint32_t n = hi - lo;
   if (n <= ADJ_SORT_LDS) {
-    for (uint32_t k = 0; k < n; k++) {
-      const uint32_t id = w.adjList[lo + k];
-      const uint64_t key = adjKey(w, b, id);
-      uint32_t j = k;
-      while (j > 0 && key < keys[j - 1][t]) {
-        keys[j][t] = keys[j - 1][t];
-        ids[j][t] = ids[j - 1][t];
-        j--;
+    constexpr uint32_t BATCH = 8;  // loads of a batch are independent, so they are in flight together
+    for (uint32_t k0 = 0; k0 < n; k0 += BATCH) {
+      uint32_t id[BATCH];
+      uint64_t key[BATCH];
+#pragma unroll
+      for (uint32_t q = 0; q < BATCH; q++) id[q] = k0 + q < n ? w.adjList[lo + k0 + q] : 0u;
+#pragma unroll
+      for (uint32_t q = 0; q < BATCH; q++) key[q] = k0 + q < n ? adjKey(w, b, id[q]) : 0ull;
+#pragma unroll
+      for (uint32_t q = 0; q < BATCH; q++) {
+        if (k0 + q >= n) break;
+        uint32_t j = k0 + q;
+        while (j > 0 && key[q] < keys[j - 1][t]) {
+          keys[j][t] = keys[j - 1][t];
+          ids[j][t] = ids[j - 1][t];
+          j--;
+        }
+        keys[j][t] = key[q];
+        ids[j][t] = id[q];
       }
-      keys[j][t] = key;
-      ids[j][t] = id;
     }
-    for (uint32_t k = 0; k < n; k++) {
-      const uint32_t id = ids[k][t];
-      w.adjList[lo + k] = id;
-      writeEntry(w, b, lo + k, id);
+    for (uint32_t k0 = 0; k0 < n; k0 += BATCH) {
+#pragma unroll
+      for (uint32_t q = 0; q < BATCH; q++) {
+        if (k0 + q >= n) break;
+        const uint32_t id = ids[k0 + q][t];
+        w.adjList[lo + k0 + q] = id;
+        writeEntry(w, b, lo + k0 + q, id);
+      }
     }
     return;
   }
@@ -176,6 +173,7 @@ AVBD_D uint32_t smallestFree(const DeviceWorld &w, uint32_t b, const uint32_t *s
 // 1. Compaction: a body whose priority beats all neighbours may drop to a smaller free colour (A → B)
 __global__ void k_color_compact(DeviceWorld w) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < COLOR_FLAG_WORDS) w.colorFlags[b] = 0u;  // the rounds' flags start clear (current state: colorA)
   if (b >= w.nBodies) return;
   if (!dynamicBody(w, int(b))) {
     w.colorB[b] = NO_COLOR;
@@ -212,12 +210,18 @@ __global__ void k_color_mark(DeviceWorld w) {
     if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(uint32_t(j), b, first)) pending = true;
   }
   w.colorA[b] = col | (pending ? PENDING : 0u);
+  if (pending) w.colorFlags[1] = 1u;  // round 0 has work
 }
 
-// 3. Jones–Plassmann round: a pending body beating all pending neighbours takes the smallest colour its settled
-// neighbours leave free; none left below the cap: the last colour, still pending (a counted clash)
-__global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst) {
+// 3. Jones–Plassmann round r: a pending body beating all pending neighbours takes the smallest colour its settled
+// neighbours leave free; none left below the cap: the last colour, still pending (a counted clash). A round whose
+// input holds no pending body would copy it unchanged, and so would every later round: it returns at once, and the
+// count pass reads the state from the buffer the last working round wrote (colorFlags[0]).
+__global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst, uint32_t r) {
+  const bool flagged = r + 2 < COLOR_FLAG_WORDS;
+  if (flagged && w.colorFlags[1 + r] == 0u) return;
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) w.colorFlags[0] = (r % 2 == 0) ? 1u : 0u;  // dst is colorB for even rounds
   if (b >= w.nBodies) return;
   uint32_t s = src[b];
   if (!(s & PENDING)) {
@@ -230,12 +234,13 @@ __global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst)
     int j = neighbour(w, e);
     if (j >= 0 && (src[j] & PENDING) && !beats(b, uint32_t(j), first)) isMax = false;
   }
+  uint32_t out = s;
   if (isMax) {
     uint32_t fr = smallestFree(w, b, src, true);
-    dst[b] = fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
-  } else {
-    dst[b] = s;
+    out = fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
   }
+  dst[b] = out;
+  if (flagged && (out & PENDING)) w.colorFlags[2 + r] = 1u;  // the next round has work
 }
 
 // 4. Per-chunk histogram and each body's rank within its chunk's colour (keeps index order inside a bucket)
@@ -253,7 +258,7 @@ __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
   uint32_t key = ~0u;
   bool clash = false;
   if (b < w.nBodies && dynamicBody(w, int(b))) {
-    uint32_t s = w.colorA[b];
+    uint32_t s = (w.colorFlags[0] ? w.colorB : w.colorA)[b];
     uint32_t col = s & 0xffu;
     clash = (s & PENDING) != 0u;
     if (col == NO_COLOR) col = w.colorCap - 1;
@@ -277,28 +282,41 @@ __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
 }
 
 // 5. Bucket k starts at its first chunk's offset
-// and the colour count: one more than the highest colour with a body (either of its buckets non-empty)
-__global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w) {
+// and the colour count: one more than the highest colour with a body (either of its buckets non-empty); with
+// planIterations > 0 also the dataflow solve's work list
+__global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w, uint32_t planIterations) {
   __shared__ uint32_t used;
+  __shared__ uint32_t starts[COLOR_KEYS + 1];
   uint32_t k = threadIdx.x;
   if (k == 0) used = 0;
   __syncthreads();
   const uint32_t start = w.colorHist[k * w.colorGroups], next = w.colorHist[(k + 1) * w.colorGroups];
   w.colorStart[k] = start;
-  if (k == 0) w.colorStart[COLOR_KEYS] = w.colorHist[COLOR_KEYS * w.colorGroups];
+  starts[k] = start;
+  if (k == 0) {
+    w.colorStart[COLOR_KEYS] = w.colorHist[COLOR_KEYS * w.colorGroups];
+    starts[COLOR_KEYS] = w.colorStart[COLOR_KEYS];
+  }
   if (next > start) atomicMax(&used, k / 2 + 1);
   __syncthreads();
-  if (k == 0) w.counters[C_NUM_COLORS] = used;
+  if (k == 0) {
+    w.counters[C_NUM_COLORS] = used;
+    if (planIterations) writeDataflowPlan(w, starts, planIterations);
+  }
 }
 
-// 6. Scatter: chunk offset for the body's bucket plus its rank
-__global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
+// 6. Scatter: chunk offset for the body's bucket plus its rank; with `warm`, also every body's warm start (the
+// scatter reads no pose, and no joint has to be warm-started first)
+template <bool warm> __global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
   uint32_t b = blockIdx.x * COLOR_WG + threadIdx.x;
-  if (b >= w.nBodies || !dynamicBody(w, int(b))) return;
-  uint32_t col = w.colorA[b] & 0xffu;
-  uint32_t slot = w.colorHist[bucketKey(w, b, col) * w.colorGroups + blockIdx.x] + w.colorRank[b];
-  w.colorBodies[slot] = b;
-  w.colorSlot[slot] = make_uint4(b, w.adjStart[b], umin(w.adjStart[b + 1], w.adjCapacity), w.bodyWorld[b]);
+  if (b >= w.nBodies) return;
+  if (dynamicBody(w, int(b))) {
+    uint32_t col = w.colorA[b] & 0xffu;
+    uint32_t slot = w.colorHist[bucketKey(w, b, col) * w.colorGroups + blockIdx.x] + w.colorRank[b];
+    w.colorBodies[slot] = b;
+    w.colorSlot[slot] = make_uint4(b, w.adjStart[b], umin(w.adjStart[b + 1], w.adjCapacity), w.bodyWorld[b]);
+  }
+  if (warm && !held(w)) warmBody(w, b);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -311,27 +329,27 @@ static inline uint32_t blocks(uint64_t n, uint32_t bs, uint32_t cap = 1u << 20) 
 }
 
 void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
-  k_adj_clear<<<blocks(uint64_t(w.nBodies) + 1, 256, 4096), 256, 0, s>>>(w);
+  // adjStart holds the degrees (counted by the manifold compaction) and adjFill is clear
   uint64_t items = uint64_t(w.nJoints) + manifoldCapacity;
-  k_adj_degree<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
   launch_scan_exclusive(w.adjStart, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
   k_adj_fill<<<blocks(items, 256, 4096), 256, 0, s>>>(w);
   k_adj_sort<<<blocks(w.nBodies, ADJ_SORT_BLOCK), ADJ_SORT_BLOCK, 0, s>>>(w);
 }
 
-void launch_coloring(const DeviceWorld &w, uint32_t rounds, hipStream_t s) {
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, hipStream_t s) {
   uint32_t g = blocks(w.nBodies, 256);
   k_color_compact<<<g, 256, 0, s>>>(w);
   k_color_mark<<<g, 256, 0, s>>>(w);
   for (uint32_t r = 0; r < rounds; r++) {
-    if (r % 2 == 0) k_color_round<<<g, 256, 0, s>>>(w, w.colorA, w.colorB);
-    else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA);
+    if (r % 2 == 0) k_color_round<<<g, 256, 0, s>>>(w, w.colorA, w.colorB, r);
+    else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA, r);
   }
   k_color_count<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
   launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s,
                         w.counters + C_DF_ABORT);
-  k_color_starts<<<1, COLOR_KEYS, 0, s>>>(w);
-  k_color_scatter<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+  k_color_starts<<<1, COLOR_KEYS, 0, s>>>(w, planIterations);
+  if (w.nJoints) k_color_scatter<false><<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+  else k_color_scatter<true><<<w.colorGroups, COLOR_WG, 0, s>>>(w);
 }
 
 }  // namespace avbd

@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
+    ap.add_argument("--bvh-rebuild", type=int, default=0,
+                    help="H-PLOC: rebuild the tree every this many steps, refit in between (0: the library default)")
     ap.add_argument("--solver", default="dataflow", choices=["auto", "colors", "dataflow"],
                     help="the large world's solve: one launch per colour, or every iteration in one dataflow launch")
     ap.add_argument("--no-batched", action="store_true")
@@ -329,7 +331,8 @@ def main():
     cap = 16
     world = avbd.createSolverPipeline("hip").createSolver(
         scene, {"colorCap": cap, "colorRounds": 16, "graph": not args.no_graph, "adaptive": True,
-                "broadphase": args.broadphase, "solver": args.solver})
+                "broadphase": args.broadphase, "solver": args.solver,
+                "bvh": {"rebuildInterval": args.bvh_rebuild}})
     print(f"# world: {n_bodies} bodies, {world.deviceBytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
 
     # Warm-up: the pile falls and lands; the colour/capacity policy adapts from counters read back asynchronously
