@@ -59,6 +59,7 @@ class Options(ctypes.Structure):
         ("broadphase", ctypes.c_uint32),
         ("bvh_rebuild_interval", ctypes.c_uint32),
         ("solver", ctypes.c_uint32),
+        ("coloring_rounds", ctypes.c_uint32),
     ]
 
 

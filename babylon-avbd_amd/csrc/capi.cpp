@@ -99,6 +99,9 @@ EngineOptions toOptions(const AvbdOptions *o) {
     if (o->bvh_rebuild_interval) e.rebuildInterval = o->bvh_rebuild_interval;
     if (o->solver > 2) throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches) or 2 (dataflow)");
     e.solver = o->solver;
+    if (o->coloring_rounds > 2)
+      throw std::invalid_argument("coloring_rounds must be 0 (auto), 1 (one launch per round) or 2 (one block)");
+    e.coloringRounds = o->coloring_rounds;
     e.device = o->device;
   }
   if (e.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");

@@ -33,6 +33,7 @@ __global__ void k_begin_frame(DeviceWorld w) {
     c[C_CANDIDATES] = 0;
     c[C_PIVOT64] = 0;
     c[C_DF_ABORT] = 0;
+    c[C_PENDING] = 0;
   }
 }
 

@@ -34,6 +34,8 @@ struct EngineOptions {
   uint32_t rebuildInterval = 64;  // H-PLOC: rebuild every this many steps, refit in between
   uint32_t solver = 0;          // 0 auto (colour launches), 1 one launch per colour, 2 dataflow (one launch for all
                                 // iterations; batches without joints, one engine on the device at a time)
+  uint32_t coloringRounds = 0;  // 0 auto (listed while few bodies are left pending, with adaptive), 1 one launch per
+                                // round, 2 always listed: one block over the pending bodies (same colours)
 };
 
 /** Milliseconds of one profiled step by phase, measured with HIP events on the engine's stream. */
@@ -58,6 +60,7 @@ struct SavedState {
 struct Counters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
   uint32_t solveAborts;  // dataflow solve waits given up in the last step (0 unless something is wrong)
+  uint32_t pending = 0;  // bodies the colouring's mark pass left pending
 };
 
 class Engine {
@@ -168,6 +171,9 @@ class Engine {
   Manifold *manifoldBuf_[2] = {nullptr, nullptr};
   Contact *contactBuf_[2] = {nullptr, nullptr};
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
+  bool listedRounds_ = false;  // colouring rounds over the pending list in one block (few pending bodies lately)
+  uint32_t coloringRounds_ = 0;
+  uint32_t listedVotes_ = 0;
   bool useBvh_ = false;
   bool useDataflow_ = false;
   uint32_t dfGrid_ = 0;

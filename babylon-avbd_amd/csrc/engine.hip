@@ -143,6 +143,8 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   tableSize_ = pow2AtLeast(2ull * std::max<uint32_t>(nBodies_, 1));
   colorGroups_ = (std::max<uint32_t>(nBodies_, 1) + COLOR_WG - 1) / COLOR_WG;
   rebuildInterval_ = std::max<uint32_t>(opt.rebuildInterval, 1);
+  coloringRounds_ = opt.coloringRounds;
+  listedRounds_ = coloringRounds_ == 2;
   if (opt.broadphase == 2) useBvh_ = true;
   else if (opt.broadphase == 1) useBvh_ = false;
   else useBvh_ = nWorlds_ == 1 && nBodies_ >= 10000;
@@ -191,6 +193,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.colorA = dallocT<uint32_t>(nBodies_);
   w.colorB = dallocT<uint32_t>(nBodies_);
   w.colorFlags = dallocT<uint32_t>(COLOR_FLAG_WORDS);
+  w.colorPending = dallocT<uint32_t>(nBodies_);
   w.colorRank = dallocT<uint32_t>(nBodies_);
   w.colorHist = dallocT<uint32_t>(uint64_t(COLOR_KEYS) * colorGroups_ + 1);
   w.colorStart = dallocT<uint32_t>(COLOR_KEYS + 1);
@@ -379,7 +382,7 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
   hipStream_t s = stream_;
   encodeCollision(w, rebuild);
   launch_adjacency(w, pairCap_, s);
-  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, s);
+  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, listedRounds_, s);
   launch_warm_start(w, s);
   if (useDataflow_) {
     launch_dataflow(w, dfGrid_, s);
@@ -401,7 +404,7 @@ void Engine::pollCounters() {
     counterPending_[slot] = false;
     const uint32_t *c = hostCounters_ + slot * COUNTER_WORDS;
     Counters cc{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT]};
+                c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
     adaptFrom(cc);
   }
 }
@@ -501,7 +504,7 @@ PhaseTimes Engine::profileStep() {
   HIP_CHECK(hipEventRecord(ev[1], s));
   launch_adjacency(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2], s));
-  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, s);
+  launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, listedRounds_, s);
   HIP_CHECK(hipEventRecord(ev[3], s));
   launch_warm_start(w, s);
   HIP_CHECK(hipEventRecord(ev[4], s));
@@ -598,7 +601,7 @@ Counters Engine::readCounters() {
   HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
   synchronize();
   return Counters{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                  c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT]};
+                  c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
 }
 
 std::vector<uint2> Engine::readPairs() {
@@ -927,6 +930,23 @@ bool Engine::adaptFrom(const Counters &c) {
   }
   if (rounds != colorRounds_) {
     setColorRounds(rounds);
+    changed = true;
+  }
+  // Colouring rounds in one block while the mark pass leaves few bodies pending (three quiet reads), in one launch
+  // per round again as soon as it leaves many (either way the same colours)
+  const uint32_t LISTED_MAX = 2048;
+  bool listed = listedRounds_;
+  if (coloringRounds_ != 0) {
+    listed = coloringRounds_ == 2;
+  } else if (c.pending > LISTED_MAX) {
+    listed = false;
+    listedVotes_ = 0;
+  } else if (!listed && ++listedVotes_ >= 3) {
+    listed = true;
+  }
+  if (listed != listedRounds_) {
+    listedRounds_ = listed;
+    resetGraphs();
     changed = true;
   }
   // Storage: grow to twice the demand when full beyond 80% or overflowed (solver.ts:1512-1518)

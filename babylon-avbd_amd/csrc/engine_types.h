@@ -76,6 +76,7 @@ struct DeviceWorld {
   uint32_t *colorA;
   uint32_t *colorB;
   uint32_t *colorFlags;   // [0] the colouring state's buffer (0 colorA, 1 colorB), [1 + r] round r has work
+  uint32_t *colorPending; // the bodies the mark pass left pending (C_PENDING of them, in no particular order)
   uint32_t *colorRank;
   uint32_t *colorHist;
   uint32_t *colorStart;

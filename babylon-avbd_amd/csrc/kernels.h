@@ -34,9 +34,12 @@ void launch_narrowphase(const DeviceWorld &w, hipStream_t s);
 void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
 /**
  * Colouring and the per-colour buckets; with planIterations > 0 also the dataflow solve's work list. In a world
- * without joints the last pass also warm-starts the bodies (launch_warm_start then has nothing left to do).
+ * without joints the last pass also warm-starts the bodies (launch_warm_start then has nothing left to do). With
+ * listedRounds the Jones–Plassmann rounds run in one block over the pending bodies (one launch instead of `rounds`;
+ * the same result, for steps that leave few bodies pending).
  */
-void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, hipStream_t s);
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, bool listedRounds,
+                     hipStream_t s);
 
 // Solve
 void launch_warm_start(const DeviceWorld &w, hipStream_t s);

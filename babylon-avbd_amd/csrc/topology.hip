@@ -210,37 +210,69 @@ __global__ void k_color_mark(DeviceWorld w) {
     if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(uint32_t(j), b, first)) pending = true;
   }
   w.colorA[b] = col | (pending ? PENDING : 0u);
-  if (pending) w.colorFlags[1] = 1u;  // round 0 has work
+  if (pending) {
+    w.colorFlags[1] = 1u;  // round 0 has work
+    w.colorPending[atomicAdd(&w.counters[C_PENDING], 1u)] = b;
+  }
 }
 
 // 3. Jones–Plassmann round r: a pending body beating all pending neighbours takes the smallest colour its settled
 // neighbours leave free; none left below the cap: the last colour, still pending (a counted clash). A round whose
 // input holds no pending body would copy it unchanged, and so would every later round: it returns at once, and the
 // count pass reads the state from the buffer the last working round wrote (colorFlags[0]).
-__global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst, uint32_t r) {
-  const bool flagged = r + 2 < COLOR_FLAG_WORDS;
-  if (flagged && w.colorFlags[1 + r] == 0u) return;
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0) w.colorFlags[0] = (r % 2 == 0) ? 1u : 0u;  // dst is colorB for even rounds
-  if (b >= w.nBodies) return;
+/** Body b's state after one round, from the round's input state src. */
+AVBD_D uint32_t roundState(const DeviceWorld &w, const uint32_t *src, uint32_t b) {
   uint32_t s = src[b];
-  if (!(s & PENDING)) {
-    dst[b] = s;
-    return;
-  }
+  if (!(s & PENDING)) return s;
   bool isMax = true;
   const uint32_t first = firstOf(w, b);
   for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
     int j = neighbour(w, e);
     if (j >= 0 && (src[j] & PENDING) && !beats(b, uint32_t(j), first)) isMax = false;
   }
-  uint32_t out = s;
-  if (isMax) {
-    uint32_t fr = smallestFree(w, b, src, true);
-    out = fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
-  }
+  if (!isMax) return s;
+  uint32_t fr = smallestFree(w, b, src, true);
+  return fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
+}
+
+__global__ void k_color_round(DeviceWorld w, const uint32_t *src, uint32_t *dst, uint32_t r) {
+  const bool flagged = r + 2 < COLOR_FLAG_WORDS;
+  if (flagged && w.colorFlags[1 + r] == 0u) return;
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) w.colorFlags[0] = (r % 2 == 0) ? 1u : 0u;  // dst is colorB for even rounds
+  if (b >= w.nBodies) return;
+  const uint32_t out = roundState(w, src, b);
   dst[b] = out;
   if (flagged && (out & PENDING)) w.colorFlags[2 + r] = 1u;  // the next round has work
+}
+
+/**
+ * All rounds in one block over the mark pass's pending list: a body off the list is settled and holds the same state
+ * in both buffers, so a round only has to update the listed bodies. The same result as the per-round launches, in one
+ * launch; the engine uses it while few bodies are left pending (any count is handled, a large one slowly).
+ */
+__global__ void __launch_bounds__(1024) k_color_rounds_listed(DeviceWorld w, uint32_t rounds) {
+  __shared__ uint32_t more;
+  const uint32_t P = w.counters[C_PENDING];
+  uint32_t current = 0;  // the buffer holding the state: 0 colorA, 1 colorB
+  for (uint32_t r = 0; r < rounds && P != 0; r++) {
+    const uint32_t *src = (r % 2 == 0) ? w.colorA : w.colorB;
+    uint32_t *dst = (r % 2 == 0) ? w.colorB : w.colorA;
+    if (threadIdx.x == 0) more = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < P; k += blockDim.x) {
+      const uint32_t b = w.colorPending[k];
+      const uint32_t out = roundState(w, src, b);
+      dst[b] = out;
+      if (out & PENDING) more = 1u;
+    }
+    __syncthreads();  // this round's states are written (block scope) before the next reads them
+    current = (r % 2 == 0) ? 1u : 0u;
+    const bool again = more != 0u;
+    __syncthreads();  // every thread has read `more` before the next round clears it
+    if (!again) break;
+  }
+  if (threadIdx.x == 0) w.colorFlags[0] = current;
 }
 
 // 4. Per-chunk histogram and each body's rank within its chunk's colour (keeps index order inside a bucket)
@@ -336,13 +368,18 @@ void launch_adjacency(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream
   k_adj_sort<<<blocks(w.nBodies, ADJ_SORT_BLOCK), ADJ_SORT_BLOCK, 0, s>>>(w);
 }
 
-void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, hipStream_t s) {
+void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterations, bool listedRounds,
+                     hipStream_t s) {
   uint32_t g = blocks(w.nBodies, 256);
   k_color_compact<<<g, 256, 0, s>>>(w);
   k_color_mark<<<g, 256, 0, s>>>(w);
-  for (uint32_t r = 0; r < rounds; r++) {
-    if (r % 2 == 0) k_color_round<<<g, 256, 0, s>>>(w, w.colorA, w.colorB, r);
-    else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA, r);
+  if (listedRounds) {
+    k_color_rounds_listed<<<1, 1024, 0, s>>>(w, rounds);
+  } else {
+    for (uint32_t r = 0; r < rounds; r++) {
+      if (r % 2 == 0) k_color_round<<<g, 256, 0, s>>>(w, w.colorA, w.colorB, r);
+      else k_color_round<<<g, 256, 0, s>>>(w, w.colorB, w.colorA, r);
+    }
   }
   k_color_count<<<w.colorGroups, COLOR_WG, 0, s>>>(w);
   launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s,

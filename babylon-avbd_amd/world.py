@@ -56,6 +56,10 @@ def _options(options):
     if solver not in ("auto", "colors", "dataflow"):
         raise ValueError(f"Unknown solver: {solver}")
     o.solver = {"auto": 0, "colors": 1, "dataflow": 2}[solver]
+    rounds = options.get("coloringRounds", "auto")
+    if rounds not in ("auto", "launches", "listed"):
+        raise ValueError(f"Unknown coloringRounds: {rounds}")
+    o.coloring_rounds = {"auto": 0, "launches": 1, "listed": 2}[rounds]
     return o
 
 

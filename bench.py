@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--broadphase", default="auto", choices=["auto", "grid", "hploc"])
+    ap.add_argument("--profile-steps", type=int, default=5,
+                    help="steps timed per launch with HIP events after the timed window (the roofline's average)")
     ap.add_argument("--bvh-rebuild", type=int, default=0,
                     help="H-PLOC: rebuild the tree every this many steps, refit in between (0: the library default)")
     ap.add_argument("--solver", default="dataflow", choices=["auto", "colors", "dataflow"],
@@ -364,8 +366,9 @@ def main():
     value = n_bodies * args.steps / elapsed
     counters = world.readCounters()
 
-    # Roofline of the dominant kernel: one profiled step (HIP events per launch on the engine's stream)
-    prof = world.profileStep()
+    # Roofline of the dominant kernel: profiled steps (HIP events per launch on the engine's stream), averaged
+    profs = [world.profileStep() for _ in range(args.profile_steps)]
+    prof = {k: sum(p[k] for p in profs) / len(profs) for k in profs[0]}
     bytes_iter, parts = primal_bytes_per_iteration(world)
     iters = scene.params["iterations"]
     dataflow = prof["primalLaunches"] == 1 and prof["dualLaunches"] == 0

@@ -83,6 +83,9 @@ typedef struct AvbdOptions {
   uint32_t solver;          /* 0 auto (colour launches), 1 one launch per colour and a dual launch per iteration,
                                2 dataflow: every iteration's colours and duals in one launch, each body waiting only
                                for its neighbours (same result; batches without joints; needs the device to itself) */
+  uint32_t coloring_rounds; /* 0 auto (one block over the pending bodies while few are left pending, with `adaptive`),
+                               1 one launch per Jones–Plassmann round, 2 always one block over the pending bodies
+                               (the same colours either way) */
 } AvbdOptions;
 
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */

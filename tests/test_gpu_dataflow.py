@@ -83,3 +83,20 @@ def test_joints_fall_back_to_colour_launches(avbd):
     assert np.array_equal(a.readBodies(), b.readBodies())
     a.destroy()
     b.destroy()
+
+
+@pytest.mark.parametrize("mode", ["listed", "launches"])
+def test_colouring_rounds_listed_and_launched_agree(avbd, mode):
+    """The Jones–Plassmann rounds over the pending list in one block give the colours the per-round launches give
+    (checked against the oracle on a pile whose colouring keeps changing), and the oracle's bodies."""
+    scene = lambda: avbd.configs.box_pile(counts=(6, 6, 6), seed=5)  # noqa: E731
+    world = _world(avbd, scene(), solver="colors", coloringRounds=mode, graph=True)
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    for _ in range(6):
+        world.step(20)
+        oracle.step(20)
+        assert np.array_equal(world.readColors(), oracle.colors()), mode
+        got, want = world.readBodies(), oracle.bodies().astype(np.float32)
+        assert np.array_equal(got[:, 0:8], want[:, 0:8]), mode  # poses, bit for bit
+        assert np.array_equal(got[:, 32:39], want[:, 32:39]), mode  # velocities
+    world.destroy()
