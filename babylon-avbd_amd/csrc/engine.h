@@ -176,6 +176,7 @@ class Engine {
   uint32_t listedVotes_ = 0;
   bool useBvh_ = false;
   bool useDataflow_ = false;
+  bool wantDataflow_ = false;  // asked for and possible; used while the arrays fit 32-bit offsets
   uint32_t dfGrid_ = 0;
   uint32_t rebuildInterval_ = 64;
   uint64_t stepIndex_ = 0;

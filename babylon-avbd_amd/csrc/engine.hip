@@ -153,6 +153,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
     dfGrid_ = dataflow_grid();
     useDataflow_ = dfGrid_ > 0;
   }
+  wantDataflow_ = useDataflow_;
 
   DeviceWorld &w = base_;
   w.nBodies = nBodies_;
@@ -271,6 +272,10 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
 }
 
 void Engine::allocateContacts(uint32_t pairCap) {
+  // The dataflow solve addresses bodies and contact slots with 32-bit buffer offsets
+  const uint64_t span = std::max<uint64_t>(uint64_t(nBodies_) * sizeof(Body),
+                                           uint64_t(pairCap) * MAX_CONTACTS * sizeof(Contact));
+  useDataflow_ = wantDataflow_ && span < (1ull << 32);
   // Keep last step's manifolds and contacts (the next step's warm-start source) across a resize
   std::vector<Manifold> keepM;
   std::vector<Contact> keepC;

@@ -24,20 +24,32 @@ struct DfCtx {
 
 AVBD_D uint32_t ldAgent(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 AVBD_D void stAgent(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-AVBD_D float4 ldAgent4(const float4 *p) {
-  const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
-  const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return make_float4(__uint_as_float(uint32_t(a)), __uint_as_float(uint32_t(a >> 32)), __uint_as_float(uint32_t(b)),
-                     __uint_as_float(uint32_t(b >> 32)));
+// Poses and contact multipliers move as whole 16-byte records: raw buffer loads and stores with the device-scope
+// cache policy (sc1), addressed from the arrays' bases (uniform, so the resource words stay scalar). The engine runs
+// the dataflow solve only while both arrays span less than 4 GiB (32-bit buffer offsets).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int BUF_SC1 = 1 << 4;            // cache policy bits: sc1 (device scope)
+constexpr int BUF_WORD3 = 0x00020000;      // resource word 3 of a raw (untyped) buffer on the gfx9 family
+AVBD_D __amdgpu_buffer_rsrc_t bodyRsrc(const DeviceWorld &w) {
+  return __builtin_amdgcn_make_buffer_rsrc(static_cast<void *>(w.bodies), 0, int(w.nBodies * uint32_t(sizeof(Body))),
+                                           BUF_WORD3);
 }
-AVBD_D void stAgent4(float4 *p, float4 v) {
-  uint64_t *q = reinterpret_cast<uint64_t *>(p);
-  __hip_atomic_store(q, uint64_t(__float_as_uint(v.x)) | (uint64_t(__float_as_uint(v.y)) << 32), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q + 1, uint64_t(__float_as_uint(v.z)) | (uint64_t(__float_as_uint(v.w)) << 32), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+AVBD_D __amdgpu_buffer_rsrc_t contactRsrc(const DeviceWorld &w) {
+  return __builtin_amdgcn_make_buffer_rsrc(static_cast<void *>(w.contacts), 0,
+                                           int(w.pairCapacity * uint32_t(MAX_CONTACTS * sizeof(Contact))), BUF_WORD3);
 }
+AVBD_D float4 ldDev4(__amdgpu_buffer_rsrc_t r, uint32_t byteOffset) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, byteOffset, 0, BUF_SC1);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+AVBD_D void stDev4(__amdgpu_buffer_rsrc_t r, uint32_t byteOffset, float4 f) {
+  const u32x4 v = {__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, byteOffset, 0, BUF_SC1);
+}
+AVBD_D uint32_t bodyOff(uint32_t i, uint32_t field) { return i * uint32_t(sizeof(Body)) + field; }
+AVBD_D uint32_t contactOff(uint32_t c, uint32_t field) { return c * uint32_t(sizeof(Contact)) + field; }
+constexpr uint32_t OFF_POS = 0, OFF_ROT = 16, OFF_KEY = 12, OFF_PEN = 32, OFF_LAM = 48;
+
 /** This wave's earlier stores are complete before any later one is issued. */
 AVBD_D void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
@@ -208,8 +220,8 @@ struct PairBody {
 template <bool DF = false> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
   const Body &B = w.bodies[i];
   PairBody pb;
-  const float4 pos = DF ? ldAgent4(&B.pos) : B.pos;
-  pb.rot = DF ? ldAgent4(&B.rot) : B.rot;
+  const float4 pos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
+  pb.rot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
   pb.irot = B.irot;
   pb.dLin = sub(xyz(pos), xyz(B.ipos));
   pb.dAng = qsub(pb.rot, pb.irot);
@@ -288,8 +300,8 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
   f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
   Contact k = w.contacts[head.y + p];
   if constexpr (DF) {
-    k.pen = ldAgent4(&w.contacts[head.y + p].pen);
-    k.lam = ldAgent4(&w.contacts[head.y + p].lam);
+    k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
+    k.lam = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_LAM));
   }
   Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha);
   f3 r = isA ? ev.rAW : ev.rBW;
@@ -313,8 +325,8 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
   if constexpr (DF) {
     if (df.it) waitFor(w, &w.dualDone[i], df.base + df.it);  // the body's own last iteration and its duals
   }
-  const float4 bpos = DF ? ldAgent4(&B.pos) : B.pos;
-  const float4 brot = DF ? ldAgent4(&B.rot) : B.rot;
+  const float4 bpos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
+  const float4 brot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
   acc.l00 = acc.l10 = acc.l11 = acc.l20 = acc.l21 = acc.l22 = 0.f;
   acc.a00 = acc.a10 = acc.a11 = acc.a20 = acc.a21 = acc.a22 = 0.f;
 #pragma unroll
@@ -482,11 +494,11 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfC
   }
   Body &B = w.bodies[i];
   if constexpr (DF) {
-    const float4 pos = ldAgent4(&B.pos);
+    const float4 pos = ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS));
     *newPos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
-    *newRot = qadd(ldAgent4(&B.rot), mk3(-x[3], -x[4], -x[5]));
-    stAgent4(&B.pos, *newPos);
-    stAgent4(&B.rot, *newRot);
+    *newRot = qadd(ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)), mk3(-x[3], -x[4], -x[5]));
+    stDev4(bodyRsrc(w), bodyOff(i, OFF_POS), *newPos);
+    stDev4(bodyRsrc(w), bodyOff(i, OFF_ROT), *newRot);
     storesDone();
     stAgent(&w.bodyDone[i], df.base + df.it + 1u);
   } else {
@@ -513,9 +525,9 @@ template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, cons
     key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
   }
   if constexpr (DF) {
-    if (ramp) stAgent(reinterpret_cast<uint32_t *>(&w.contacts[c].rA.w), key);
-    stAgent4(&w.contacts[c].pen, mk4(pen, kc.pen.w));
-    stAgent4(&w.contacts[c].lam, mk4(e.F, kc.lam.w));
+    if (ramp) __builtin_amdgcn_raw_buffer_store_b32(key, contactRsrc(w), contactOff(c, OFF_KEY), 0, BUF_SC1);
+    stDev4(contactRsrc(w), contactOff(c, OFF_PEN), mk4(pen, kc.pen.w));
+    stDev4(contactRsrc(w), contactOff(c, OFF_LAM), mk4(e.F, kc.lam.w));
   } else {
     if (ramp) w.contacts[c].rA.w = bitsf(key);
     w.contacts[c].pen = mk4(pen, kc.pen.w);
@@ -579,8 +591,8 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
         const PairBody other = pairBody<true>(w, h.x);
         const uint32_t c = h.y + pt;
         Contact kc = w.contacts[c];
-        kc.pen = ldAgent4(&w.contacts[c].pen);
-        kc.lam = ldAgent4(&w.contacts[c].lam);
+        kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
+        kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
         const Eval ev = evalContact(kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
                                     isA ? other : self, p.alpha);
         dualUpdate<true>(w, c, kc, ev, p);
