@@ -116,6 +116,12 @@ class AvbdScene3D {
 
   get bodyCount() { return this.records.length; }
 
+  /** scene.createSolver(pipeline, options) (scenes.js:652-702): a world of this scene on the pipeline's device. */
+  createSolver(pipeline, options) {
+    if (!pipeline || typeof pipeline.createSolver !== 'function') throw Error('A solver pipeline is required');
+    return pipeline.createSolver(this, options);
+  }
+
   assertEditable() {
     if (this.created) throw Error('Build this scene before createSolver; use the solver\'s edit methods afterwards');
   }
@@ -355,6 +361,48 @@ function solverCode(solver) {
 }
 
 /** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */
+/**
+ * A body handle (runtime.js:168-237 NativeBody): chainable edits of one body slot, and readState(). Triggers,
+ * restitution and sleeping are not built on this backend: turning them on throws, turning them off (their only state
+ * here) is accepted.
+ */
+class HipBody {
+  constructor(solver, index) {
+    this.solver = solver;
+    this.index = index;
+  }
+  get gpuIndex() { return this.solver.liveIndex(this.index); }
+  setLinearVelocity(v) { this.solver.setLinearVelocity(this.gpuIndex, v); return this; }
+  setAngularVelocity(v) { this.solver.setAngularVelocity(this.gpuIndex, v); return this; }
+  applyImpulse(v, p) { this.solver.applyImpulse(this.gpuIndex, v, p); return this; }
+  applyForce(v, p) { this.solver.applyForce(this.gpuIndex, v, p); return this; }
+  applyAngularImpulse(v) { this.solver.applyAngularImpulse(this.gpuIndex, v); return this; }
+  applyTorque(v) { this.solver.applyTorque(this.gpuIndex, v); return this; }
+  teleport(p, r) { this.solver.teleport(this.gpuIndex, p, r); return this; }
+  setTrigger(enabled) {
+    this.gpuIndex;
+    if (enabled) throw Error('Trigger bodies are not available on the hip backend');
+    return this;
+  }
+  setRestitution(value) {
+    this.gpuIndex;
+    if (!Number.isFinite(value) || value < 0) throw Error('Restitution must be a nonnegative finite number');
+    if (value !== 0) throw Error('Restitution is not available on the hip backend');
+    return this;
+  }
+  setCollisionGroups(group, collidesWith) {
+    this.solver.setFilters([this.gpuIndex], [group], [collidesWith === undefined ? 0xffffffff : collidesWith]);
+    return this;
+  }
+  setSleepEnabled(enabled) {
+    this.gpuIndex;
+    this.solver.sleepAllowed.set(this.index, !!enabled);  // recorded; this backend does not put bodies to sleep
+    return this;
+  }
+  wakeUp() { this.gpuIndex; return this; }
+  readState() { return this.solver.readBodyState(this.gpuIndex); }
+}
+
 class HipSolver3D {
   constructor(scenes, options, device) {
     const list = Array.isArray(scenes) ? scenes : [scenes];
@@ -381,9 +429,105 @@ class HipSolver3D {
       first += s.bodyCount;
     }
     this.commands = new CommandQueue();
+    this.handles = new Map();
+    this.sleepAllowed = new Map();
+    this.sceneBinding = null;
+    this.scenes = list;
   }
 
   alive() { if (this.destroyed) throw Error('The solver has been destroyed'); }
+
+  /** The slot of a handle of this world, a scene builder's body index, or a slot (runtime.js:326-345). */
+  bodyIndex(body) {
+    this.alive();
+    if (body instanceof HipBody) {
+      if (body.solver !== this) throw Error('Body belongs to another solver');
+      return body.gpuIndex;
+    }
+    return this.liveIndex(body);
+  }
+  /** The handle of a body (one per slot, created on first use). */
+  body(body) {
+    const index = this.bodyIndex(body);
+    if (!this.handles.has(index)) this.handles.set(index, new HipBody(this, index));
+    return this.handles.get(index);
+  }
+  gpuIndex(i) { return this.liveIndex(i); }
+
+  /** readBodyState: position, rotation, linear and angular velocity of one body (after queued edits). */
+  async readBodyState(index) {
+    const r = await this.readSelectedBodies([this.liveIndex(index)]);
+    return { position: Array.from(r.subarray(0, 3)), rotation: Array.from(r.subarray(4, 8)),
+             linearVelocity: Array.from(r.subarray(32, 35)), angularVelocity: Array.from(r.subarray(36, 39)) };
+  }
+
+  /** setVelocities(indices, {linear, angular}) (runtime.js:356-378): xyz triples per index, either or both. */
+  setVelocities(indices, values) {
+    this.alive();
+    const v = values || {};
+    if (v.linear === undefined && v.angular === undefined) throw Error('Provide linear and/or angular velocities');
+    const list = Array.from(indices, (i) => this.liveIndex(i));
+    // xyz per body: flat (3 numbers each) or one triple per body; the whole batch is checked before any is queued
+    const take = (arr, name) => {
+      if (arr === undefined) return undefined;
+      const items = Array.from(arr);
+      const flat = items.length && typeof items[0] === 'number' ? items : [].concat(...items.map((x) => Array.from(x)));
+      if (flat.length !== list.length * 3 || !flat.every(Number.isFinite))
+        throw Error(`${name} velocities must be finite xyz per body`);
+      return flat;
+    };
+    const lin = take(v.linear, 'Linear'), ang = take(v.angular, 'Angular');
+    const staged = new CommandQueue();
+    list.forEach((index, k) => {
+      if (lin) staged.append(index, 0, lin.slice(3 * k, 3 * k + 3));
+      if (ang) staged.append(index, 1, ang.slice(3 * k, 3 * k + 3));
+    });
+    this.commands.extend(staged);
+    return this;
+  }
+  setLinearVelocities(indices, values) { return this.setVelocities(indices, { linear: values }); }
+  setAngularVelocities(indices, values) { return this.setVelocities(indices, { angular: values }); }
+
+  /** Sleeping is not built on this backend: bodies never sleep, which is what disabling it asks for. */
+  disableSleeping() { this.alive(); return this; }
+  wakeAll() { this.alive(); return this; }
+
+  /**
+   * attachToScene(scene, {afterStep, maxSubSteps, maxFrameTime}) (runtime.js:275-311): step from the scene's
+   * before-render observable with its engine's frame time; destroy with the scene.
+   */
+  attachToScene(scene, options) {
+    this.alive();
+    const o = Object.assign({}, options || {});
+    const afterStep = o.afterStep;
+    delete o.afterStep;
+    if (!scene || !scene.onBeforeRenderObservable || !scene.onBeforeRenderObservable.add ||
+        !scene.onDisposeObservable || !scene.onDisposeObservable.addOnce || !scene.getEngine)
+      throw Error('A Babylon scene is required');
+    if (afterStep !== undefined && typeof afterStep !== 'function') throw Error('afterStep must be a function');
+    this.advance(0, o);  // validates the stepping limits before observers are registered
+    if (this.sceneBinding) {
+      if (this.sceneBinding.scene === scene) return this;
+      throw Error('The solver is already attached to another scene');
+    }
+    const before = scene.onBeforeRenderObservable.add(() => {
+      this.advance(Math.max(0, scene.getEngine().getDeltaTime() / 1000), o, afterStep);
+    });
+    const dispose = scene.onDisposeObservable.addOnce(() => this.destroy());
+    this.sceneBinding = { scene, before, dispose };
+    return this;
+  }
+  /** Remove the observers without destroying the world; reset the elapsed-time debt. */
+  detachFromScene() {
+    const b = this.sceneBinding;
+    if (b) {
+      b.scene.onBeforeRenderObservable.remove(b.before);
+      b.scene.onDisposeObservable.remove(b.dispose);
+      this.sceneBinding = null;
+      this.timeAccumulator = 0;
+    }
+    return this;
+  }
 
   /** Alive, with queued edits submitted (the reference flushes them before steps and reads). */
   ready() { this.alive(); this.flushEdits(); }
@@ -466,7 +610,7 @@ class HipSolver3D {
     const staged = new CommandQueue();
     const properties = [];
     for (const edit of edits) {
-      const index = this.liveIndex(edit.body);
+      const index = this.bodyIndex(edit.body);
       for (const key of ['isTrigger', 'restitution', 'allowSleep'])
         if (edit[key] !== undefined) throw Error(`${key} edits are not available on the hip backend`);
       if (edit.group !== undefined || edit.collidesWith !== undefined) {
@@ -527,7 +671,7 @@ class HipSolver3D {
   }
 
   /** Fixed-step accumulator; returns the steps submitted (runtime.js:255-272). */
-  advance(elapsed, options) {
+  advance(elapsed, options, afterStep) {
     this.alive();
     const o = options || {};
     const maxSubSteps = o.maxSubSteps === undefined ? 6 : o.maxSubSteps;
@@ -542,6 +686,7 @@ class HipSolver3D {
       this.step();
       this.timeAccumulator = Math.max(0, this.timeAccumulator - dt);
       count++;
+      if (afterStep) afterStep(this);
     }
     return count;
   }
@@ -598,8 +743,10 @@ class HipSolver3D {
 
   destroy() {
     if (this.destroyed) return;
+    this.detachFromScene();
     native().destroy(this.handle);
     this.destroyed = true;
+    this.handles.clear();
   }
 }
 

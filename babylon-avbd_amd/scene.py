@@ -157,6 +157,12 @@ class AvbdScene3D:
         self._count += 1
         return index
 
+    def createSolver(self, pipeline, options=None):
+        """scene.createSolver(pipeline, options) (scenes.js:652-702): a world of this scene on the pipeline's device."""
+        if not hasattr(pipeline, "createSolver"):
+            raise ValueError("A solver pipeline is required")
+        return pipeline.createSolver(self, options)
+
     def addBox(self, size, **options):
         """A centred box with full XYZ extents; zero density makes it static (scenes.js:131-144)."""
         s = _vector(size, 3, "size")

@@ -237,7 +237,7 @@ class HipWorld:
         precedes velocity replacements, impulses and forces; group/collidesWith edits become setFilters."""
         self._alive()
         staged = _edits.CommandStaging()
-        properties = _edits.stage_edits(edits, self.liveIndex, self._dt_of, staged)
+        properties = _edits.stage_edits(edits, self.bodyIndex, self._dt_of, staged)
         self._pending.extend(staged)
         for index, group, collides in properties:
             self.setFilters([index], [group], [collides])
@@ -457,9 +457,135 @@ class HipWorld:
 
     def gpuIndex(self, i):
         """Slot of the scene builder's body i (bodies keep builder order)."""
-        return int(i)
+        return self.liveIndex(i)
 
-    bodyIndex = gpuIndex
+    def bodyIndex(self, body):
+        """The slot of a handle of this world, or of a body index (runtime.js:326-345)."""
+        self._alive()
+        if isinstance(body, HipBody):
+            if body.solver is not self:
+                raise ValueError("Body belongs to another solver")
+            return body.gpuIndex
+        return self.liveIndex(body)
+
+    def body(self, body):
+        """The handle of a body (runtime.js:168-237), one per slot."""
+        index = self.bodyIndex(body)
+        handles = self.__dict__.setdefault("_handles", {})
+        if index not in handles:
+            handles[index] = HipBody(self, index)
+        return handles[index]
+
+    def readBodyState(self, index):
+        """Position, rotation, linear and angular velocity of one body (queued edits applied first)."""
+        r = self.readSelectedBodies([self.liveIndex(index)])[0]
+        return {"position": r[0:3].tolist(), "rotation": r[4:8].tolist(), "linearVelocity": r[32:35].tolist(),
+                "angularVelocity": r[36:39].tolist()}
+
+    def setVelocities(self, indices, linear=None, angular=None):
+        """setVelocities(indices, {linear, angular}) (runtime.js:356-378): xyz per body, the batch checked whole."""
+        self._alive()
+        if linear is None and angular is None:
+            raise ValueError("Provide linear and/or angular velocities")
+        idx = [self.liveIndex(i) for i in indices]
+
+        def take(values, name):
+            if values is None:
+                return None
+            a = np.asarray(values, np.float64).reshape(-1)
+            if a.size != 3 * len(idx) or not np.isfinite(a).all():
+                raise ValueError(f"{name} velocities must be finite xyz per body")
+            return a.reshape(-1, 3)
+        lin, ang = take(linear, "Linear"), take(angular, "Angular")
+        staged = _edits.CommandStaging()
+        for k, i in enumerate(idx):
+            if lin is not None:
+                staged.append(i, _edits.LINEAR_VELOCITY, lin[k].tolist())
+            if ang is not None:
+                staged.append(i, _edits.ANGULAR_VELOCITY, ang[k].tolist())
+        self._pending.extend(staged)
+        return self
+
+    def setLinearVelocities(self, indices, values):
+        return self.setVelocities(indices, linear=values)
+
+    def setAngularVelocities(self, indices, values):
+        return self.setVelocities(indices, angular=values)
+
+    def disableSleeping(self):
+        """Sleeping is not built on this backend: bodies never sleep, which is what disabling it asks for."""
+        self._alive()
+        return self
+
+
+class HipBody:
+    """A body handle (runtime.js:168-237 NativeBody): chainable edits of one slot and readState(). Trigger bodies,
+    restitution and sleeping are not built on this backend: turning them on raises, turning them off is accepted."""
+
+    def __init__(self, solver, index):
+        self.solver = solver
+        self.index = index
+
+    @property
+    def gpuIndex(self):
+        return self.solver.liveIndex(self.index)
+
+    def setLinearVelocity(self, v):
+        self.solver.setLinearVelocity(self.gpuIndex, v)
+        return self
+
+    def setAngularVelocity(self, v):
+        self.solver.setAngularVelocity(self.gpuIndex, v)
+        return self
+
+    def applyImpulse(self, v, point=None):
+        self.solver.applyImpulse(self.gpuIndex, v, point)
+        return self
+
+    def applyForce(self, v, point=None):
+        self.solver.applyForce(self.gpuIndex, v, point)
+        return self
+
+    def applyAngularImpulse(self, v):
+        self.solver.applyAngularImpulse(self.gpuIndex, v)
+        return self
+
+    def applyTorque(self, v):
+        self.solver.applyTorque(self.gpuIndex, v)
+        return self
+
+    def teleport(self, position, rotation=None):
+        self.solver.teleport(self.gpuIndex, position, rotation)
+        return self
+
+    def setCollisionGroups(self, group, collidesWith=0xFFFFFFFF):
+        self.solver.setFilters([self.gpuIndex], [group], [collidesWith])
+        return self
+
+    def setTrigger(self, enabled):
+        self.gpuIndex
+        if enabled:
+            raise ValueError("Trigger bodies are not available on the hip backend")
+        return self
+
+    def setRestitution(self, value):
+        self.gpuIndex
+        if not math.isfinite(value) or value < 0:
+            raise ValueError("Restitution must be a nonnegative finite number")
+        if value != 0:
+            raise ValueError("Restitution is not available on the hip backend")
+        return self
+
+    def setSleepEnabled(self, enabled):
+        self.gpuIndex  # recorded nowhere: this backend does not put bodies to sleep
+        return self
+
+    def wakeUp(self):
+        self.gpuIndex
+        return self
+
+    def readState(self):
+        return self.solver.readBodyState(self.gpuIndex)
 
 
 def createSolverPipeline(backend="hip", device=0):

@@ -156,3 +156,16 @@ def test_js_edits_land_like_the_python_edits(avbd, tmp_path):
     assert js["bodies"] == world.readBodies().view(np.uint32).ravel().tolist()
     assert js["pairs"] == world.readPairs().ravel().tolist()
     world.destroy()
+
+
+@pytest.mark.gpu
+def test_reference_api_contracts_hold_on_the_hip_backend():
+    """The reference's Babylon-free contracts (tests/backend-contract.js checkBackend/checkFloor,
+    tests/packed-poses-contract.js), restated in tests/js/contract_check.js, pass through the N-API addon."""
+    script = ROOT / "tests" / "js" / "contract_check.js"
+    r = subprocess.run([NODE, str(script)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["backend"]["passed"] and out["floor"]["passed"]
+    assert 0.35 < out["floor"]["floorHeight"] < 0.7
+    assert all(out["packedPoses"].values())
