@@ -146,6 +146,8 @@ SIGNATURES = {
     "avbd_world_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),
     "avbd_world_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),
     "avbd_world_set_filters": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_uint32]),
+    "avbd_world_set_restitution": (ctypes.c_int, [ctypes.c_void_p, u32p, f32p, ctypes.c_uint32]),
+    "avbd_world_get_restitution": (ctypes.c_int, [ctypes.c_void_p, f32p]),
     "avbd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "avbd_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_server_add_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), u32p]),

@@ -480,6 +480,23 @@ int avbd_world_set_filters(avbd_world *world, const uint32_t *indices, const uin
   });
 }
 
+int avbd_world_set_restitution(avbd_world *world, const uint32_t *indices, const float *values, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->setRestitution(indices, values, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_get_restitution(avbd_world *world, float *out) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    world->engine->getRestitution(out);
+    return AVBD_OK;
+  });
+}
+
 // --- serving loop ----------------------------------------------------------------------------------------------------
 
 #define CHECK_SERVER(s) \

@@ -86,6 +86,11 @@ __global__ void k_set_filters(uint2 *filters, const uint32_t *words, uint32_t n)
   if (k < n) filters[words[3 * k]] = make_uint2(words[3 * k + 1], words[3 * k + 2]);
 }
 
+__global__ void k_set_restitution(float *restitution, const uint32_t *words, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) restitution[words[2 * k]] = __uint_as_float(words[2 * k + 1]);
+}
+
 __global__ void k_set_params(WorldParams *p, ParamEdit e) {
   if (threadIdx.x != 0) return;
   p->dt = e.dt;
@@ -126,5 +131,9 @@ void launch_set_filters(uint2 *filters, const uint32_t *words, uint32_t n, hipSt
 }
 
 void launch_set_params(WorldParams *p, const ParamEdit &e, hipStream_t s) { k_set_params<<<1, 64, 0, s>>>(p, e); }
+
+void launch_set_restitution(float *restitution, const uint32_t *words, uint32_t n, hipStream_t s) {
+  k_set_restitution<<<(n + 255) / 256, 256, 0, s>>>(restitution, words, n);
+}
 
 }  // namespace avbd

@@ -125,6 +125,10 @@ class Engine {
   void setFilters(const uint32_t *indices, const uint32_t *groups, const uint32_t *masks, uint32_t n);
   /** Parameters of world k of the batch, from the next step on. */
   void setParams(uint32_t world, const ParamEdit &e);
+  /** Restitution coefficients in [0, 1] of n bodies, from the next step on (the bounce passes run while any body's
+   * coefficient is above 0). */
+  void setRestitution(const uint32_t *indices, const float *values, uint32_t n);
+  void getRestitution(float *out) const;
   /** Solver iterations per step (all worlds of the batch). */
   void setIterations(uint32_t n);
 
@@ -171,6 +175,8 @@ class Engine {
   Manifold *manifoldBuf_[2] = {nullptr, nullptr};
   Contact *contactBuf_[2] = {nullptr, nullptr};
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
+  std::vector<float> restitution_;  // host copy of the per-body coefficients
+  uint32_t bouncing_ = 0;           // bodies with a coefficient above 0
   bool listedRounds_ = false;  // colouring rounds over the pending list in one block (few pending bodies lately)
   uint32_t coloringRounds_ = 0;
   uint32_t listedVotes_ = 0;

@@ -167,6 +167,9 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.bodyWorld = dallocT<uint32_t>(nBodies_);
   w.worlds = dallocT<WorldParams>(nWorlds_);
   w.refs = dallocT<float4>(2ull * nBodies_);
+  w.restitution = dallocT<float>(nBodies_);
+  w.bounceVel = dallocT<float4>(4ull * nBodies_);
+  restitution_.assign(nBodies_, 0.f);
   w.bodyDone = dallocT<uint32_t>(nBodies_);
   w.dualDone = dallocT<uint32_t>(nBodies_);
   w.dfPlan = dallocT<DataflowPlan>(1);
@@ -313,6 +316,7 @@ void Engine::allocateContacts(uint32_t pairCap) {
   }
   w.table = static_cast<uint32_t *>(alloc(size_t(hashSize_) * 4));
   w.hashMask = hashSize_ - 1;
+  w.bounce = static_cast<float4 *>(alloc(size_t(pairCap_) * 3 * sizeof(float4)));
   w.adjCapacity = 2 * (nJoints_ + pairCap_);
   w.adjList = static_cast<uint32_t *>(alloc(size_t(w.adjCapacity) * 4));
   w.adjEntry = static_cast<uint4 *>(alloc(size_t(w.adjCapacity) * 2 * sizeof(uint4)));
@@ -385,6 +389,7 @@ void Engine::encodeCollision(const DeviceWorld &w, bool rebuild) {
 
 void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
   hipStream_t s = stream_;
+  if (bouncing_) launch_restitution_before(w, s);
   encodeCollision(w, rebuild);
   launch_adjacency(w, pairCap_, s);
   launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, listedRounds_, s);
@@ -398,6 +403,7 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
     }
   }
   launch_velocities(w, s);
+  if (bouncing_) launch_restitution_after(w, s);
 }
 
 void Engine::pollCounters() {
@@ -490,6 +496,7 @@ PhaseTimes Engine::profileStep() {
   w.colorCap = colorCap_;
   const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
   HIP_CHECK(hipEventRecord(ev[0], s));
+  if (bouncing_) launch_restitution_before(w, s);
   // encodeCollision, with events after the frame set-up and after the broadphase
   launch_begin_frame(w, s);
   launch_refs_and_hash(w, pairCap_, s);
@@ -532,6 +539,7 @@ PhaseTimes Engine::profileStep() {
     }
   }
   launch_velocities(w, s);
+  if (bouncing_) launch_restitution_after(w, s);  // timed with the velocities
   HIP_CHECK(hipEventRecord(ev[5], s));
   HIP_CHECK(hipEventSynchronize(ev[5]));
   stepIndex_++;
@@ -869,6 +877,30 @@ void Engine::setFilters(const uint32_t *indices, const uint32_t *groups, const u
   launch_set_filters(base_.filters, uploadEdit(buf.data(), buf.size()), n, stream_);
   HIP_CHECK(hipGetLastError());
 }
+
+void Engine::setRestitution(const uint32_t *indices, const float *values, uint32_t n) {
+  if (n == 0) return;
+  if (!indices || !values) throw std::invalid_argument("indices or values is NULL");
+  for (uint32_t k = 0; k < n; k++) {
+    if (indices[k] >= nBodies_) throw std::invalid_argument("Restitution requires live body indices");
+    if (!(values[k] >= 0.f && values[k] <= 1.f)) throw std::invalid_argument("restitution must be between 0 and 1");
+  }
+  HIP_CHECK(hipSetDevice(device_));
+  std::vector<uint32_t> buf(2ull * n);
+  const uint32_t before = bouncing_;
+  for (uint32_t k = 0; k < n; k++) {
+    float &r = restitution_[indices[k]];
+    bouncing_ = bouncing_ - (r > 0.f ? 1u : 0u) + (values[k] > 0.f ? 1u : 0u);
+    r = values[k];
+    buf[2 * k] = indices[k];
+    std::memcpy(&buf[2 * k + 1], &values[k], 4);
+  }
+  launch_set_restitution(base_.restitution, uploadEdit(buf.data(), buf.size()), n, stream_);
+  HIP_CHECK(hipGetLastError());
+  if ((before == 0) != (bouncing_ == 0)) resetGraphs();  // the step gains or loses its bounce passes
+}
+
+void Engine::getRestitution(float *out) const { std::copy(restitution_.begin(), restitution_.end(), out); }
 
 void Engine::setParams(uint32_t world, const ParamEdit &e) {
   if (world >= nWorlds_) throw std::invalid_argument("world index out of range");

@@ -58,6 +58,9 @@ struct DeviceWorld {
   Manifold *prevManifolds;
   Contact *contacts;      // 8 per pair slot
   uint32_t *pointMap;     // the current contact points, densely: manifold index << 3 | point (C_CONTACTS of them)
+  float *restitution;     // per body: restitution coefficient in [0, 1] (0: the body does not bounce)
+  float4 *bounceVel;      // per body, restitution passes: incoming linear, incoming angular, current linear, angular
+  float4 *bounce;         // per manifold slot, restitution passes: (impulse, delta, rebound, mass), lever A, lever B
   uint32_t *bodyDone;     // per body: stamp of its last primal update (dataflow solve)
   uint32_t *dualDone;     // per body: stamp of its last dual of the pairs it finishes (dataflow solve)
   DataflowPlan *dfPlan;

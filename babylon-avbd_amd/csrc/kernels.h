@@ -46,6 +46,10 @@ void launch_warm_start(const DeviceWorld &w, hipStream_t s);
 void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s);
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s);
 void launch_velocities(const DeviceWorld &w, hipStream_t s);
+/** Restitution (src/gpu/restitution.js): incoming velocities before the step; bounce rounds after it. */
+void launch_restitution_before(const DeviceWorld &w, hipStream_t s);
+void launch_restitution_after(const DeviceWorld &w, hipStream_t s);
+
 /** Blocks of the dataflow solve that are resident at once on the current device (0 if unknown). */
 uint32_t dataflow_grid();
 /**
@@ -65,6 +69,8 @@ void launch_invalidate_manifolds(Manifold *manifolds, const uint32_t *counters, 
 void launch_clear_marks(const uint32_t *commands, const uint2 *ranges, uint32_t n, uint32_t *marks, hipStream_t s);
 /** words: n (body, groups, collides-with) triples. */
 void launch_set_filters(uint2 *filters, const uint32_t *words, uint32_t n, hipStream_t s);
+/** Restitution coefficients of n bodies: words = (body, f32 bits) pairs. */
+void launch_set_restitution(float *restitution, const uint32_t *words, uint32_t n, hipStream_t s);
 void launch_set_params(WorldParams *p, const ParamEdit &e, hipStream_t s);
 
 }  // namespace avbd

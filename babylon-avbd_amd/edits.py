@@ -13,7 +13,7 @@ import numpy as np
 
 EDIT_KEYS = ("linearVelocity", "angularVelocity", "impulse", "force", "", "angularImpulse", "torque")
 LINEAR_VELOCITY, ANGULAR_VELOCITY, IMPULSE, FORCE, TELEPORT, ANGULAR_IMPULSE, TORQUE = range(7)
-_UNSUPPORTED = ("isTrigger", "restitution", "allowSleep")
+_UNSUPPORTED = ("isTrigger", "allowSleep")
 
 
 def components(value, count, name):
@@ -41,6 +41,17 @@ def unit_rotation(rotation):
     if not length:
         raise ValueError("rotation must be nonzero")
     return [v / length for v in q]
+
+
+def coefficient(value):
+    """A restitution coefficient: a finite number in [0, 1] (appGpuSolver3D.js:455-467)."""
+    try:
+        v = float(value)
+    except (TypeError, ValueError):
+        v = math.nan
+    if not (math.isfinite(v) and 0 <= v <= 1):
+        raise ValueError("restitution must be between 0 and 1")
+    return v
 
 
 def mask(value, name="Collision masks"):
@@ -96,7 +107,8 @@ class CommandStaging:
 
 def stage_edits(edits, body_index, dt_of, commands):
     """stageEdits (runtime.js:60-175) into `commands` (cleared first): per edit, a teleport first, then the remaining
-    kinds in editKeys order. Returns the property edits (body, group, collidesWith) to apply afterwards.
+    kinds in editKeys order. Returns the property edits (body, group, collidesWith, restitution) to apply afterwards
+    (group None: no filter change; restitution None: no coefficient change).
 
     body_index(body) -> live slot (raises otherwise); dt_of(slot) -> the body's world time step."""
     commands.clear()
@@ -107,10 +119,14 @@ def stage_edits(edits, body_index, dt_of, commands):
         if unsupported:
             raise NotImplementedError(f"{unsupported[0]} edits are not available on the hip backend")
         group, collides = edit.get("group"), edit.get("collidesWith")
-        if group is not None or collides is not None:
+        restitution = edit.get("restitution")
+        if restitution is not None:
+            restitution = coefficient(restitution)
+        if group is not None or collides is not None or restitution is not None:
             if collides is not None and group is None:
                 raise ValueError("A collision-mask edit requires group as well")
-            properties.append((index, mask(group), mask(collides) if collides is not None else 0xFFFFFFFF))
+            properties.append((index, None if group is None else mask(group),
+                               mask(collides) if collides is not None else 0xFFFFFFFF, restitution))
         point = edit.get("worldPoint")
         point = None if point is None else components(point, 3, "worldPoint")
         if edit.get("position") is not None:

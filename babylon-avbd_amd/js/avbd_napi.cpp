@@ -430,6 +430,37 @@ napi_value SetFilters(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+napi_value SetRestitution(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx, *values;
+  size_t n, nv;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false) ||
+      !typed(env, argv[2], napi_float32_array, &values, &nv, false))
+    return nullptr;
+  if (nv != n) {
+    napi_throw_range_error(env, nullptr, "Each body index needs a restitution value");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_set_restitution(w, static_cast<const uint32_t *>(idx), static_cast<const float *>(values),
+                                              uint32_t(n)));
+  return nullptr;
+}
+
+napi_value GetRestitution(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t n;
+  if (!w || !u32(env, argv[1], &n)) return nullptr;
+  void *data;
+  napi_value out = newTyped(env, napi_float32_array, n, 4, &data);
+  if (!out) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_get_restitution(w, static_cast<float *>(data)));
+  return out;
+}
+
 napi_value GetParams(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
@@ -519,6 +550,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"deviceBytes", nullptr, DeviceBytes, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"enqueueCommands", nullptr, EnqueueCommands, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setFilters", nullptr, SetFilters, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setRestitution", nullptr, SetRestitution, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"getRestitution", nullptr, GetRestitution, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };

@@ -109,6 +109,7 @@ class AvbdScene3D {
     if (Math.abs(Math.hypot(this.up[0], this.up[1], this.up[2]) - 1) > 1e-5) throw Error('up must be a unit vector');
     this.records = [];
     this.filters = new Map();
+    this.restitution = new Map();  // body -> restitution coefficient above 0
     this.joints = [];
     this.ignore = [];
     this.created = false;
@@ -146,12 +147,14 @@ class AvbdScene3D {
     if (o.velocity !== undefined && o.velocity !== null) rec.set(vector(o.velocity, 3, 'velocity'), 32);
     if (o.angularVelocity !== undefined && o.angularVelocity !== null)
       rec.set(vector(o.angularVelocity, 3, 'angularVelocity'), 36);
+    const restitution = o.restitution === undefined ? undefined : coefficient(o.restitution);
     const group = o.group === undefined ? 0xffffffff : o.group;
     const mask = o.collidesWith === undefined ? 0xffffffff : o.collidesWith;
     for (const m of [group, mask])
       if (!Number.isInteger(m) || m < 0 || m > 0xffffffff) throw Error('Collision masks must be unsigned 32-bit integers');
     const index = this.records.length;
     if (group !== 0xffffffff || mask !== 0xffffffff) this.filters.set(index, [group, mask]);
+    if (restitution) this.restitution.set(index, restitution);
     this.records.push(rec);
     return index;
   }
@@ -319,6 +322,11 @@ function unitRotation(rotation) {
   return q.map((v) => v / l);
 }
 
+function coefficient(v) {
+  if (typeof v !== 'number' || !Number.isFinite(v) || v < 0 || v > 1) throw Error('restitution must be between 0 and 1');
+  return v;
+}
+
 function mask32(v) {
   if (!Number.isInteger(v) || v < 0 || v > 0xffffffff) throw Error('Collision masks must be unsigned 32-bit integers');
   return v;
@@ -362,9 +370,9 @@ function solverCode(solver) {
 
 /** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */
 /**
- * A body handle (runtime.js:168-237 NativeBody): chainable edits of one body slot, and readState(). Triggers,
- * restitution and sleeping are not built on this backend: turning them on throws, turning them off (their only state
- * here) is accepted.
+ * A body handle (runtime.js:168-237 NativeBody): chainable edits of one body slot, and readState(). Triggers and
+ * sleeping are not built on this backend: turning them on throws, turning them off (their only state here) is
+ * accepted.
  */
 class HipBody {
   constructor(solver, index) {
@@ -384,12 +392,7 @@ class HipBody {
     if (enabled) throw Error('Trigger bodies are not available on the hip backend');
     return this;
   }
-  setRestitution(value) {
-    this.gpuIndex;
-    if (!Number.isFinite(value) || value < 0) throw Error('Restitution must be a nonnegative finite number');
-    if (value !== 0) throw Error('Restitution is not available on the hip backend');
-    return this;
-  }
+  setRestitution(value) { this.solver.setRestitution(this.gpuIndex, value); return this; }
   setCollisionGroups(group, collidesWith) {
     this.solver.setFilters([this.gpuIndex], [group], [collidesWith === undefined ? 0xffffffff : collidesWith]);
     return this;
@@ -433,6 +436,13 @@ class HipSolver3D {
     this.sleepAllowed = new Map();
     this.sceneBinding = null;
     this.scenes = list;
+    // Bodies built with a restitution coefficient (scenes.js:684-687 applies them after creation the same way)
+    const bouncy = [];
+    list.forEach((sc, k) => {
+      for (const [i, e] of Array.from(sc.restitution.entries()).sort((x, y) => x[0] - y[0]))
+        bouncy.push([this.firstBodies[k] + i, e]);
+    });
+    if (bouncy.length) this.setRestitution(bouncy.map((b) => b[0]), bouncy.map((b) => b[1]));
   }
 
   alive() { if (this.destroyed) throw Error('The solver has been destroyed'); }
@@ -611,12 +621,14 @@ class HipSolver3D {
     const properties = [];
     for (const edit of edits) {
       const index = this.bodyIndex(edit.body);
-      for (const key of ['isTrigger', 'restitution', 'allowSleep'])
+      for (const key of ['isTrigger', 'allowSleep'])
         if (edit[key] !== undefined) throw Error(`${key} edits are not available on the hip backend`);
-      if (edit.group !== undefined || edit.collidesWith !== undefined) {
+      const restitution = edit.restitution === undefined ? undefined : coefficient(edit.restitution);
+      if (edit.group !== undefined || edit.collidesWith !== undefined || restitution !== undefined) {
         if (edit.collidesWith !== undefined && edit.group === undefined)
           throw Error('A collision-mask edit requires group as well');
-        properties.push([index, mask32(edit.group), edit.collidesWith === undefined ? 0xffffffff : mask32(edit.collidesWith)]);
+        properties.push([index, edit.group === undefined ? undefined : mask32(edit.group),
+          edit.collidesWith === undefined ? 0xffffffff : mask32(edit.collidesWith), restitution]);
       }
       const point = edit.worldPoint === undefined ? undefined : components(edit.worldPoint, 3, 'worldPoint');
       if (edit.position !== undefined) {
@@ -634,9 +646,26 @@ class HipSolver3D {
       }
     }
     this.commands.extend(staged);
-    for (const [index, group, collides] of properties) this.setFilters([index], [group], [collides]);
+    for (const [index, group, collides, restitution] of properties) {
+      if (group !== undefined) this.setFilters([index], [group], [collides]);
+      if (restitution !== undefined) this.setRestitution(index, restitution);
+    }
     return this;
   }
+
+  /** setRestitution(index, value) (appGpuSolver3D.js:455-471), or many (indices, values): coefficients in [0, 1],
+   * from the next step on; while any is above 0 each step ends with the bounce passes (src/gpu/restitution.js). */
+  setRestitution(indices, values) {
+    this.alive();
+    const many = typeof indices !== 'number';
+    const idx = Uint32Array.from(many ? indices : [indices], (i) => this.liveIndex(i));
+    const val = Float32Array.from(many ? values : [values], coefficient);
+    if (idx.length !== val.length) throw Error('Each body index needs a restitution value');
+    this.flushEdits();  // commands queued before keep their place ahead of the change
+    native().setRestitution(this.handle, idx, val);
+    return this;
+  }
+  getRestitution() { this.alive(); return native().getRestitution(this.handle, this.bodyCount); }
 
   /** setFilters(indices, groups, collidesWith) (appGpuSolver3D.js:958-983). */
   setFilters(indices, groups, collidesWith) {

@@ -111,6 +111,7 @@ class AvbdScene3D:
         self._starts = []         # first body of each block
         self._count = 0
         self._filters = {}        # body -> (group, collidesWith)
+        self._restitution = {}    # body -> restitution coefficient above 0
         self._joints = []         # (record[32], info[4])
         self._ignore = []
         self.created = False
@@ -144,6 +145,11 @@ class AvbdScene3D:
             rec[32:35] = _vector(options["velocity"], 3, "velocity")
         if options.get("angularVelocity") is not None:
             rec[36:39] = _vector(options["angularVelocity"], 3, "angularVelocity")
+        restitution = options.get("restitution")
+        if restitution is not None:
+            r = float(restitution) if isinstance(restitution, (int, float)) else math.nan
+            if not (math.isfinite(r) and 0 <= r <= 1):
+                raise ValueError("restitution must be between 0 and 1")
         group = options.get("group", 0xFFFFFFFF)
         mask = options.get("collidesWith", 0xFFFFFFFF)
         for m in (group, mask):
@@ -152,6 +158,8 @@ class AvbdScene3D:
         index = self._count
         if group != 0xFFFFFFFF or mask != 0xFFFFFFFF:
             self._filters[index] = (int(group), int(mask))
+        if restitution is not None and r > 0:
+            self._restitution[index] = r
         self._starts.append(self._count)
         self._chunks.append(rec[None, :])
         self._count += 1

@@ -101,6 +101,11 @@ class HipWorld:
         self.worldCount = len(scenes)
         self._dts = [float(sc.params["dt"]) for sc in scenes]
         self._pending = _edits.CommandStaging()
+        # Bodies built with a restitution coefficient (scenes.js:684-687 applies them after creation the same way)
+        bouncy = [(int(first) + i, e) for first, sc in zip(self.world_first_body, scenes)
+                  for i, e in sorted(sc._restitution.items())]
+        if bouncy:
+            self.setRestitution([i for i, _ in bouncy], [e for _, e in bouncy])
 
     # --- lifetime ---------------------------------------------------------------------------------------------------
 
@@ -239,9 +244,33 @@ class HipWorld:
         staged = _edits.CommandStaging()
         properties = _edits.stage_edits(edits, self.bodyIndex, self._dt_of, staged)
         self._pending.extend(staged)
-        for index, group, collides in properties:
-            self.setFilters([index], [group], [collides])
+        for index, group, collides, restitution in properties:
+            if group is not None:
+                self.setFilters([index], [group], [collides])
+            if restitution is not None:
+                self.setRestitution([index], [restitution])
         return self
+
+    def setRestitution(self, indices, values):
+        """setRestitution (appGpuSolver3D.js:455-471) for one body (index, value) or many: coefficients in [0, 1],
+        from the next step on; while any is above 0 each step ends with the bounce passes (src/gpu/restitution.js)."""
+        self._alive()
+        if np.ndim(indices) == 0:
+            indices, values = [indices], [values]
+        idx = np.array([self.liveIndex(i) for i in indices], np.uint32)
+        val = np.array([_edits.coefficient(v) for v in values], np.float32)
+        if len(idx) != len(val):
+            raise ValueError("Each body index needs a restitution value")
+        self.flushEdits()  # commands queued before keep their place ahead of the change
+        check(self._lib.avbd_world_set_restitution(self._h, ptr(idx, ctypes.c_uint32), ptr(val, ctypes.c_float),
+                                                   len(idx)))
+        return self
+
+    def getRestitution(self):
+        self._alive()
+        out = np.zeros(self.bodyCount, np.float32)
+        check(self._lib.avbd_world_get_restitution(self._h, ptr(out, ctypes.c_float)))
+        return out
 
     def setFilters(self, indices, groups, collidesWith):
         """setFilters (appGpuSolver3D.js:958-983): collision group and mask per body, from the next step on."""
@@ -519,8 +548,8 @@ class HipWorld:
 
 
 class HipBody:
-    """A body handle (runtime.js:168-237 NativeBody): chainable edits of one slot and readState(). Trigger bodies,
-    restitution and sleeping are not built on this backend: turning them on raises, turning them off is accepted."""
+    """A body handle (runtime.js:168-237 NativeBody): chainable edits of one slot and readState(). Trigger bodies and
+    sleeping are not built on this backend: turning them on raises, turning them off is accepted."""
 
     def __init__(self, solver, index):
         self.solver = solver
@@ -569,11 +598,7 @@ class HipBody:
         return self
 
     def setRestitution(self, value):
-        self.gpuIndex
-        if not math.isfinite(value) or value < 0:
-            raise ValueError("Restitution must be a nonnegative finite number")
-        if value != 0:
-            raise ValueError("Restitution is not available on the hip backend")
+        self.solver.setRestitution(self.gpuIndex, value)
         return self
 
     def setSleepEnabled(self, enabled):

@@ -235,6 +235,12 @@ int avbd_world_set_params(avbd_world *world, uint32_t world_index, const AvbdPar
 /** setFilters(indices, groups, collidesWith) (appGpuSolver3D.js:958-983): collision groups and masks of n bodies. */
 int avbd_world_set_filters(avbd_world *world, const uint32_t *indices, const uint32_t *groups,
                            const uint32_t *collides_with, uint32_t n);
+/** setRestitution(index, value) (appGpuSolver3D.js:455-471; src/gpu/restitution.js): restitution coefficients in
+ * [0, 1] of n bodies, from the next step on. While any body's coefficient is above 0 each step ends with the
+ * reference's bounce passes (incoming velocities captured before the step, 12 impulse rounds after it). */
+int avbd_world_set_restitution(avbd_world *world, const uint32_t *indices, const float *values, uint32_t n);
+/** The coefficients of all bodies (world->bodies floats). */
+int avbd_world_get_restitution(avbd_world *world, float *out);
 
 /* ---- Serving loop ------------------------------------------------------------------------------------------------
  * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):

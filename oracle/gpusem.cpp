@@ -104,6 +104,7 @@ void build(World<R> &w, uint32_t nBodies, const float *bodyRecs, const uint32_t 
            const float *worldParams, uint32_t nJoints, const float *jointRecs, const int32_t *info, uint32_t nNoCollide,
            const uint32_t *noCollide, const uint32_t *filters) {
   w.bodies.resize(nBodies);
+  w.restitution.assign(nBodies, R(0));
   w.world.assign(nBodies, 0);
   for (uint32_t i = 0; i < nBodies; i++) {
     unpackBody(bodyRecs + 40 * i, w.bodies[i]);
@@ -257,9 +258,12 @@ void gs_step(void *p, int n) {
         int32_t info[4] = {h->d->joints[j].type, h->d->joints[j].a, h->d->joints[j].b, 0};
         unpackJoint(rec, info, h->f->joints[j]);
       }
+      const bool bounce = h->d->bouncing();
+      if (bounce) h->d->restitutionBefore();
       h->f->collisionAndTopology();
       copyTopology(*h->d, *h->f);
       h->d->solve();
+      if (bounce) h->d->restitutionAfter();
     } else {
       h->f->step();
     }
@@ -394,6 +398,15 @@ void gs_set_filters(void *p, const uint32_t *indices, const uint32_t *groups, co
       h->d->filters[2 * size_t(indices[k])] = groups[k];
       h->d->filters[2 * size_t(indices[k]) + 1] = masks[k];
     }
+  }
+}
+
+/** Restitution coefficients of n bodies (the bounce passes run while any is above 0). */
+void gs_set_restitution(void *p, const uint32_t *indices, const float *values, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  for (uint32_t k = 0; k < n; k++) {
+    h->f->restitution[indices[k]] = values[k];
+    if (h->d) h->d->restitution[indices[k]] = double(values[k]);
   }
 }
 

@@ -1015,10 +1015,132 @@ template <class R> class World {
     updateVelocities();
   }
 
+  // --- restitution around the step (src/gpu/restitution.js:16-88) -----------------------------------------------
+
+  std::vector<R> restitution;  // per body, in [0, 1]
+  struct BounceVel {
+    V3<R> inLin, inAng, lin, ang;
+    R linW = 0, angW = 0;
+  };
+  std::vector<BounceVel> bounceVel;
+  struct Bounce {
+    R impulse = 0, delta = 0, rebound = 0, mass = 0;
+    V3<R> ra, rb;
+  };
+
+  bool bouncing() const {
+    for (R r : restitution)
+      if (r > 0) return true;
+    return false;
+  }
+  static R inverseMassOf(const Body<R> &b) { return b.mass > 0 ? R(1) / rmax(b.mass, R(1e-20f)) : R(0); }
+  /** restitution.js:29-34: the world-frame inverse inertia applied to t. */
+  static V3<R> inverseInertiaOf(const Body<R> &b, V3<R> t) {
+    if (b.mass <= 0) return V3<R>(0, 0, 0);
+    V3<R> local = rotate(conj(b.rot), t);
+    V3<R> m = b.moment;
+    V3<R> scaled(m.x > 0 ? local.x / rmax(m.x, R(1e-20f)) : R(0), m.y > 0 ? local.y / rmax(m.y, R(1e-20f)) : R(0),
+                 m.z > 0 ? local.z / rmax(m.z, R(1e-20f)) : R(0));
+    return rotate(b.rot, scaled);
+  }
+  V3<R> pointVelocity(uint32_t i, V3<R> r, bool incoming) const {
+    const BounceVel &v = bounceVel[i];
+    return add(incoming ? v.inLin : v.lin, cross(incoming ? v.inAng : v.ang, r));
+  }
+  /** captureIncoming: every body's velocities before the step. */
+  void restitutionBefore() {
+    bounceVel.resize(n());
+    for (uint32_t i = 0; i < n(); i++) {
+      bounceVel[i].inLin = bodies[i].vel;
+      bounceVel[i].inAng = bodies[i].angVel;
+    }
+  }
+  /** captureSolved, initialize, 12 × (pairs, apply), writeBack. */
+  void restitutionAfter() {
+    const uint32_t N = uint32_t(n()), M = uint32_t(manifolds.size()), J = uint32_t(joints.size());
+    for (uint32_t i = 0; i < N; i++) {
+      bounceVel[i].lin = bodies[i].vel;
+      bounceVel[i].linW = bodies[i].prevUp;
+      bounceVel[i].ang = bodies[i].angVel;
+      bounceVel[i].angW = bodies[i].shape;
+    }
+    std::vector<Bounce> bounce(M);
+#pragma omp parallel for schedule(static)
+    for (uint32_t m = 0; m < M; m++) {
+      const Manifold<R> &mf = manifolds[m];
+      const uint32_t count = mf.count(), a = mf.a, b = mf.b;
+      const R e = rmax(restitution[a], restitution[b]);
+      if (count == 0 || e <= 0 || (mf.countw & 0x80000000u) != 0 || clashes != 0) continue;
+      const Body<R> &A = bodies[a], &B = bodies[b];
+      V3<R> ra(0, 0, 0), rb(0, 0, 0);
+      for (uint32_t c = 0; c < count; c++) {
+        ra = add(ra, rotate(A.rot, contacts[mf.first + c].rA));
+        rb = add(rb, rotate(B.rot, contacts[mf.first + c].rB));
+      }
+      const R cn = R(count);
+      ra = V3<R>(ra.x / cn, ra.y / cn, ra.z / cn);
+      rb = V3<R>(rb.x / cn, rb.y / cn, rb.z / cn);
+      const R vn = dot(mf.normal, sub(pointVelocity(a, ra, true), pointVelocity(b, rb, true)));
+      if (vn >= R(-1)) continue;
+      const V3<R> ta = cross(ra, mf.normal), tb = cross(rb, mf.normal);
+      const R k = inverseMassOf(A) + inverseMassOf(B) + dot(ta, inverseInertiaOf(A, ta)) + dot(tb, inverseInertiaOf(B, tb));
+      if (k <= 0) continue;
+      Bounce &s = bounce[m];
+      s.rebound = -e * vn;
+      s.mass = k;
+      s.ra = ra;
+      s.rb = rb;
+    }
+    for (int round = 0; round < 12; round++) {
+#pragma omp parallel for schedule(static)
+      for (uint32_t m = 0; m < M; m++) {
+        Bounce &s = bounce[m];
+        if (s.mass <= 0) continue;
+        const Manifold<R> &mf = manifolds[m];
+        const R vn = dot(mf.normal, sub(pointVelocity(mf.a, s.ra, false), pointVelocity(mf.b, s.rb, false)));
+        const uint32_t degree = rmax(1u, rmax(adjStart[mf.a + 1] - adjStart[mf.a], adjStart[mf.b + 1] - adjStart[mf.b]));
+        const R impulse = rmax(R(0), s.impulse + (s.rebound - vn) / (s.mass * R(degree)));
+        s.delta = impulse - s.impulse;
+        s.impulse = impulse;
+      }
+#pragma omp parallel for schedule(static)
+      for (uint32_t i = 0; i < N; i++) {
+        const Body<R> &B = bodies[i];
+        if (B.mass <= 0) continue;
+        V3<R> dv(0, 0, 0), dw(0, 0, 0);
+        for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++) {
+          const uint32_t id = adjList[e];
+          if (id < J) continue;
+          const Manifold<R> &mf = manifolds[id - J];
+          const Bounce &s = bounce[id - J];
+          if (s.mass <= 0) continue;
+          const bool isA = i == mf.a;
+          const V3<R> impulse = scale(scale(mf.normal, s.delta), isA ? R(1) : R(-1));
+          dv = add(dv, scale(impulse, inverseMassOf(B)));
+          dw = add(dw, inverseInertiaOf(B, cross(isA ? s.ra : s.rb, impulse)));
+        }
+        BounceVel &v = bounceVel[i];
+        v.lin = add(v.lin, dv);
+        v.linW = v.linW + R(0);
+        v.ang = add(v.ang, dw);
+        v.angW = v.angW + R(0);
+        if (round == 11) {
+          bodies[i].vel = v.lin;
+          bodies[i].prevUp = v.linW;
+          bodies[i].angVel = v.ang;
+          bodies[i].shape = v.angW;
+        }
+      }
+    }
+  }
+
   void step() {
     pivotFallbacks = 0;
+    const bool bounce = bouncing();
+    if (bounce) restitutionBefore();
     collisionAndTopology();
     solve();
+    if (bounce) restitutionAfter();
   }
 
   // --- edits between steps -----------------------------------------------------------------------------------------

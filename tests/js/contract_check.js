@@ -1,8 +1,9 @@
 'use strict';
 // The reference's Babylon-free world-API contracts (tests/backend-contract.js checkBackend and checkFloor,
 // tests/packed-poses-contract.js packedPoseContract), restated for the "hip" backend of babylon-avbd_amd/js/index.js:
-// the same calls and the same expectations, in 3D. What this backend does not build (trigger bodies, restitution,
-// sleeping, ray queries) is checked to be refused with an error, or, for switching it off, accepted.
+// the same calls and the same expectations, in 3D, and the restitution part of checkContacts. What this backend does not
+// build (trigger bodies, sleeping, ray queries) is checked to be refused with an error, or, for switching it off,
+// accepted.
 // Prints one JSON object with a record per contract; exits non-zero on the first failure.
 const path = require('path');
 const avbd = require(path.join(__dirname, '..', '..', 'babylon-avbd_amd', 'js', 'index.js'));
@@ -47,10 +48,10 @@ async function checkBackend() {
     state = await handle.readState();
     close(state.linearVelocity[0], 1);
     check(state.angularVelocity[2] < 0, 'Off-centre impulse did not rotate the body');
-    handle.setCollisionGroups(4, 4);
-    // Not built here: a trigger body, restitution and ray queries are refused, not ignored
+    handle.setCollisionGroups(4, 4).setRestitution(0.6);
+    check(world.getRestitution()[id] === Math.fround(0.6), 'Restitution was not set');
+    // Not built here: a trigger body and ray queries are refused, not ignored
     check(throws(() => handle.setTrigger(true)), 'A trigger was accepted');
-    check(throws(() => handle.setRestitution(0.6)), 'Restitution was accepted');
     check(typeof world.raycast !== 'function', 'A ray query appeared without an implementation');
     const selected = await world.readSelectedBodies([id], { posesOnly: true });
     check(selected.length === 8, 'Pose stride changed across backends');
@@ -114,6 +115,24 @@ async function checkFloor() {
     check(state.position.every(Number.isFinite), 'Nonfinite floor state');
     check(state.position[1] > 0.35 && state.position[1] < 0.7, `Floor failed: ${state.position[1]}`);
     return { backend: 'hip', dimension: 3, floorHeight: state.position[1], passed: true };
+  } finally {
+    pipeline.destroy();
+  }
+}
+
+/** checkContacts (backend-contract.js), its restitution part: a sphere with restitution 1 falling at 3 m/s onto the
+ * floor with gravity 0 moves up faster than 2 m/s after 12 steps. */
+async function checkRestitution() {
+  const pipeline = await avbd.createSolverPipeline({ backend: 'hip' });
+  const scene = new avbd.AvbdScene3D({ gravity: 0 });
+  scene.addBox([10, 1, 4], { density: 0, position: [0, -0.5, 0] });
+  const ball = scene.addSphere(0.5, { position: [0, 0.51, 0], velocity: [0, -3, 0], restitution: 1 });
+  const world = scene.createSolver(pipeline, { spatialSort: false });
+  try {
+    for (let i = 0; i < 12; i++) world.step();
+    const state = await world.body(ball).readState();
+    check(state.linearVelocity[1] > 2, `Restitution failed: ${state.linearVelocity[1]}`);
+    return { backend: 'hip', dimension: 3, reboundSpeed: state.linearVelocity[1], passed: true };
   } finally {
     pipeline.destroy();
   }
@@ -183,6 +202,7 @@ async function checkPackedPoses() {
   const out = {};
   out.backend = await checkBackend();
   out.floor = await checkFloor();
+  out.restitution = await checkRestitution();
   out.packedPoses = await checkPackedPoses();
   process.stdout.write(JSON.stringify(out) + '\n');
 })().catch((e) => {

@@ -75,6 +75,7 @@ def lib():
             "gs_set_iterations": (None, [vp, ctypes.c_int]),
             "gs_set_filters": (None, [vp, u32p, u32p, u32p, ctypes.c_uint32]),
             "gs_set_threads": (None, [ctypes.c_int]),
+            "gs_set_restitution": (None, [vp, u32p, f32p, ctypes.c_uint32]),
             "gs_get_threads": (ctypes.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -150,6 +151,13 @@ class OracleWorld:
         self._brute = brute_force
         if brute_force:
             L.gs_set_brute_force(self.h, 1)
+        self._restitution = None
+        bouncy, base = [], 0
+        for sc in scenes:  # bodies built with a restitution coefficient, as the HIP world applies them
+            bouncy += [(base + i, e) for i, e in sorted(getattr(sc, "_restitution", {}).items())]
+            base += sc.body_count
+        if bouncy:
+            self.set_restitution([i for i, _ in bouncy], [e for _, e in bouncy])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -180,6 +188,8 @@ class OracleWorld:
         steps = np.ascontiguousarray(state["steps"], np.uint32)
         L.gs_set_state(self.h, P(refs, ctypes.c_float), P(colors, cc), len(mf), P(mf, cc), len(ct), P(ct, cc),
                        P(steps, cc))
+        if getattr(self, "_restitution", None) is not None:
+            self.set_restitution(*self._restitution)
 
     def step(self, n=1):
         lib().gs_step(self.h, n)
@@ -256,6 +266,12 @@ class OracleWorld:
     def set_iterations(self, n):
         self._iters = int(n)
         lib().gs_set_iterations(self.h, int(n))
+
+    def set_restitution(self, indices, values):
+        idx = np.ascontiguousarray(indices, np.uint32)
+        val = np.ascontiguousarray(values, np.float32)
+        lib().gs_set_restitution(self.h, P(idx, ctypes.c_uint32), P(val, ctypes.c_float), len(idx))
+        self._restitution = (idx, val)
 
     def set_filters(self, indices, groups, masks):
         idx = np.ascontiguousarray(indices, np.uint32)

@@ -259,9 +259,14 @@ def test_invalid_edits_queue_nothing(avbd, op, message):
 
 
 def test_unsupported_properties_are_refused(avbd):
+    """Trigger bodies and sleeping are not built: their edits are refused; a restitution outside [0, 1] too."""
     st = _Staging(avbd)
     with pytest.raises(NotImplementedError):
-        st.world.editBodies([{"body": 1, "restitution": 0.5}])
+        st.world.editBodies([{"body": 1, "isTrigger": True}])
+    with pytest.raises(NotImplementedError):
+        st.world.editBodies([{"body": 1, "allowSleep": True}])
+    with pytest.raises(ValueError):
+        st.world.editBodies([{"body": 1, "restitution": 1.5}])
 
 
 NODE = shutil.which("node")
@@ -281,3 +286,80 @@ def test_js_facade_stages_the_same_records(avbd, tmp_path):
     assert out["error"] is None
     assert out["words"] == st.words().ravel().tolist()
     assert out["filters"] == [[[2], [4], [5]]]
+
+
+def _bounce_scene(avbd, e, speed, gravity=0.0, radius=0.5, sphere=True):
+    s = avbd.AvbdScene3D(gravity=gravity)
+    s.addBox([10, 1, 10], density=0, position=[0, -0.5, 0])
+    if sphere:
+        s.addSphere(radius, position=[0, radius + 0.01, 0], velocity=[0, -speed, 0])
+    else:
+        s.addBox([1, 1, 1], position=[0, 0.51, 0], velocity=[0, -speed, 0])
+    return s
+
+
+@pytest.mark.parametrize("e", [1.0, 0.5, 0.25])
+def test_restitution_rebounds_at_e_times_the_impact_speed(avbd, e):
+    """A sphere hitting a static floor head on (one manifold, degree 1, no spin): the first bounce round sets the
+    normal velocity to the rebound speed −e·vn exactly, so the ball leaves at e times its impact speed."""
+    from oracle_lib import OracleWorld
+    speed = 3.0
+    w = OracleWorld([_bounce_scene(avbd, e, speed)], precision=32)
+    w.set_restitution([1], [e])
+    for _ in range(12):  # the first step whose velocity points up is the bounce
+        w.step()
+        vy = w.bodies()[1, 33]
+        if vy > 0:
+            break
+    assert abs(vy - e * speed) < 1e-5 * speed, f"e={e}: left at {vy}, expected {e * speed}"
+
+
+def test_no_restitution_no_bounce_and_slow_contacts_do_not_bounce(avbd):
+    """e = 0 leaves the step untouched (bit for bit the plain step), and an approach slower than 1 m/s never bounces."""
+    from oracle_lib import OracleWorld
+    a = OracleWorld([_bounce_scene(avbd, 0, 3.0)], precision=32)
+    b = OracleWorld([_bounce_scene(avbd, 0, 3.0)], precision=32)
+    b.set_restitution([1], [0.0])
+    for _ in range(10):
+        a.step()
+        b.step()
+    np.testing.assert_array_equal(a.bodies(), b.bodies())
+    slow = OracleWorld([_bounce_scene(avbd, 1.0, 0.5)], precision=32)
+    slow.set_restitution([1], [1.0])
+    for _ in range(12):
+        slow.step()
+        assert slow.bodies()[1, 33] < 0.5, "a 0.5 m/s approach bounced"
+
+
+def test_reference_restitution_contract(avbd):
+    """backend-contract.js checkContacts: a sphere with restitution 1 at −3 m/s onto the floor, gravity 0: after 12
+    steps it moves up faster than 2 m/s. Also for a box (a four-point face manifold)."""
+    from oracle_lib import OracleWorld
+    for sphere in (True, False):
+        w = OracleWorld([_bounce_scene(avbd, 1.0, 3.0, sphere=sphere)], precision=32)
+        w.set_restitution([1], [1.0])
+        w.step(12)
+        assert w.bodies()[1, 33] > 2, f"restitution failed ({'sphere' if sphere else 'box'}): {w.bodies()[1, 33]}"
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_restitution_keeps_momentum_between_two_free_bodies(avbd, precision):
+    """Two equal free spheres meeting head on with e = 1, no gravity and no floor: the bounce passes add equal and
+    opposite velocity changes (the step without restitution, from the same state, is the reference), and they rebound
+    at about their approach speed."""
+    from oracle_lib import OracleWorld
+
+    def world(e):
+        s = avbd.AvbdScene3D(gravity=0)
+        s.addSphere(0.5, position=[-0.505, 0, 0], velocity=[3, 0, 0])
+        s.addSphere(0.5, position=[0.505, 0, 0], velocity=[-3, 0, 0])
+        w = OracleWorld([s], precision=precision)
+        w.set_restitution([0, 1], [e, e])
+        return w
+    bounced, plain = world(1.0), world(0.0)
+    for _ in range(2):  # the spheres touch in the second step
+        bounced.step()
+        plain.step()
+    change = bounced.bodies()[:, 32:35] - plain.bodies()[:, 32:35]
+    assert np.abs(change[0] + change[1]).max() < (1e-5 if precision == 32 else 1e-12)
+    assert bounced.bodies()[0, 32] < -2.9 and bounced.bodies()[1, 32] > 2.9

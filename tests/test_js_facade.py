@@ -166,6 +166,6 @@ def test_reference_api_contracts_hold_on_the_hip_backend():
     r = subprocess.run([NODE, str(script)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
-    assert out["backend"]["passed"] and out["floor"]["passed"]
+    assert out["backend"]["passed"] and out["floor"]["passed"] and out["restitution"]["reboundSpeed"] > 2
     assert 0.35 < out["floor"]["floorHeight"] < 0.7
     assert all(out["packedPoses"].values())
