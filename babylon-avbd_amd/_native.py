@@ -148,6 +148,16 @@ SIGNATURES = {
     "avbd_world_set_filters": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_uint32]),
     "avbd_world_set_restitution": (ctypes.c_int, [ctypes.c_void_p, u32p, f32p, ctypes.c_uint32]),
     "avbd_world_get_restitution": (ctypes.c_int, [ctypes.c_void_p, f32p]),
+    "avbd_world_enable_sleeping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                                  ctypes.c_int]),
+    "avbd_world_disable_sleeping": (ctypes.c_int, [ctypes.c_void_p]),
+    "avbd_world_sleeping": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "avbd_world_wake_all": (ctypes.c_int, [ctypes.c_void_p]),
+    "avbd_world_wake_bodies": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),
+    "avbd_world_set_sleep_enabled": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.POINTER(ctypes.c_uint8),
+                                                    ctypes.c_uint32]),
+    "avbd_world_sleep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, u32p]),
+    "avbd_world_read_sleep": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p]),
     "avbd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "avbd_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_server_add_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), u32p]),

@@ -497,6 +497,70 @@ int avbd_world_get_restitution(avbd_world *world, float *out) {
   });
 }
 
+int avbd_world_enable_sleeping(avbd_world *world, float speed_threshold, float time_threshold, int islands,
+                               int start_asleep) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->enableSleeping(speed_threshold, time_threshold, islands, start_asleep != 0);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_disable_sleeping(avbd_world *world) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->disableSleeping();
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_sleeping(avbd_world *world, int *islands) {
+  CHECK_WORLD(world);
+  if (islands) *islands = world->engine->sleepIslands() ? 1 : 0;
+  return world->engine->sleeping() ? 1 : 0;
+}
+
+int avbd_world_wake_all(avbd_world *world) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->wakeAll();
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_wake_bodies(avbd_world *world, const uint32_t *indices, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->wakeBodies(indices, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_sleep_enabled(avbd_world *world, const uint32_t *indices, const uint8_t *flags, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->setSleepEnabled(indices, flags, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_sleep_stats(avbd_world *world, uint32_t world_index, uint32_t out[2]) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    world->engine->sleepStats(world_index, out);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_sleep(avbd_world *world, uint32_t *states, uint32_t *globals) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->readSleep(states, globals);
+    return AVBD_OK;
+  });
+}
+
 // --- serving loop ----------------------------------------------------------------------------------------------------
 
 #define CHECK_SERVER(s) \

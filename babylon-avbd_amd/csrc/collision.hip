@@ -147,6 +147,7 @@ __global__ void k_grid_scatter(DeviceWorld w) {
  * partner appears once, so the order is total and the list is the same whatever order the grid buckets hold).
  */
 template <bool STASH> __global__ void k_pairs(DeviceWorld w) {
+  if (gatedOff(w)) return;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= w.nBodies) return;
   // Threads take bodies in cell order (small bodies) then the large ones: neighbouring threads scan the same cells
@@ -221,6 +222,7 @@ template <bool STASH> __global__ void k_pairs(DeviceWorld w) {
 }
 
 __global__ void k_pairs_total(DeviceWorld w) {
+  if (gatedOff(w)) return;
   uint32_t total = w.pairCount[w.nBodies];
   if (total > w.pairCapacity) {
     w.counters[C_OVERFLOW] |= 1u;
@@ -279,6 +281,7 @@ AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t p
  * without spheres or capsules), which keeps the kernel's code small enough to stay in the instruction cache.
  */
 template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narrowphase(DeviceWorld w) {
+  if (gatedOff(w)) return;
   uint32_t P = w.counters[C_PAIRS];
   for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
     uint2 pr = w.pairs[pslot];
@@ -429,6 +432,7 @@ AVBD_D uint32_t blockExclusive(uint32_t v, uint32_t &total, uint32_t *waveTot) {
  * degrees (cleared by k_refs_and_clear).
  */
 __global__ void __launch_bounds__(COMPACT_BLOCK) k_manifold_compact(DeviceWorld w) {
+  if (gatedOff(w)) return;
   __shared__ uint32_t waveTot[COMPACT_BLOCK / 64];
   __shared__ uint32_t mFirst, pFirst;
   const uint32_t P = w.counters[C_PAIRS];
@@ -507,7 +511,7 @@ void launch_grid(const DeviceWorld &w, hipStream_t s) {
 }
 void launch_pairs(const DeviceWorld &w, hipStream_t s) {
   k_pairs<true><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT, w.gate);
   k_pairs<false><<<blocks(w.nBodies, 128), 128, 0, s>>>(w);
   k_pairs_total<<<1, 1, 0, s>>>(w);
 }

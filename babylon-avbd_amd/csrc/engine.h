@@ -129,6 +129,28 @@ class Engine {
    * coefficient is above 0). */
   void setRestitution(const uint32_t *indices, const float *values, uint32_t n);
   void getRestitution(float *out) const;
+  /**
+   * Sleeping (off until enabled; enabling again keeps the first settings): speed threshold (linear speed + angular
+   * speed × bounding radius) and quiet time before sleeping, island mode (-1: islands when worlds average 4096 bodies
+   * or more), and start asleep (island mode: the next step puts still, grounded groups to sleep). Eligibility comes from
+   * setSleepEnabled (every body until then).
+   */
+  void enableSleeping(float speed, float delay, int islands, bool startAsleep);
+  /** Wake every body (masses back) and stop sleeping. */
+  void disableSleeping();
+  bool sleeping() const { return sleeping_; }
+  bool sleepIslands() const { return sleepIslands_; }
+  /** Wake every body now (on the stream, before the next step). */
+  void wakeAll();
+  /** Hold n bodies awake through the next step; in island mode their islands wake at its start. */
+  void wakeBodies(const uint32_t *indices, uint32_t n);
+  /** Sleep eligibility of n bodies (0 or 1); a body allowed to sleep enables sleeping; every body wakes. */
+  void setSleepEnabled(const uint32_t *indices, const uint8_t *flags, uint32_t n);
+  /** World k after the last step: out[0] bodies asleep, out[1] a wake-all request pending. */
+  void sleepStats(uint32_t world, uint32_t *out);
+  /** Every body's sleep state, 8 words each (mass and quiet as f32 bits, asleep, support, eligible, parent, wake,
+   * flags), and the 4 words per world of sleep globals. */
+  void readSleep(uint32_t *states, uint32_t *globals);
   /** Solver iterations per step (all worlds of the batch). */
   void setIterations(uint32_t n);
 
@@ -148,7 +170,10 @@ class Engine {
  private:
   void allocate();
   void allocateContacts(uint32_t pairCap);
-  void encodeStep(const DeviceWorld &w, bool rebuild);
+  void encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce = 0);
+  SleepParams sleepParams(uint32_t force) const { return SleepParams{sleepSpeed_, sleepDelay_, sleepIslands_ ? 1u : 0u, force}; }
+  /** Impacts on sleepers after the narrowphase, then the pairs and narrowphase again, gated on a body having woken. */
+  void encodeSleepImpact(const DeviceWorld &w, uint32_t force);
   void encodeCollision(const DeviceWorld &w, bool rebuild);
   void allocateBvh(const std::vector<uint32_t> &counts);
   void *dalloc(size_t bytes);
@@ -177,6 +202,10 @@ class Engine {
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
   std::vector<float> restitution_;  // host copy of the per-body coefficients
   uint32_t bouncing_ = 0;           // bodies with a coefficient above 0
+  bool sleeping_ = false, sleepIslands_ = false;
+  float sleepSpeed_ = 0.03f, sleepDelay_ = 0.5f;
+  uint32_t sleepForce_ = 0;           // force bits of the next step (2: start asleep)
+  std::vector<uint8_t> sleepEligible_;  // host copy of the per-body eligibility
   bool listedRounds_ = false;  // colouring rounds over the pending list in one block (few pending bodies lately)
   uint32_t coloringRounds_ = 0;
   uint32_t listedVotes_ = 0;

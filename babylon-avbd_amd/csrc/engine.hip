@@ -170,6 +170,10 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.restitution = dallocT<float>(nBodies_);
   w.bounceVel = dallocT<float4>(4ull * nBodies_);
   restitution_.assign(nBodies_, 0.f);
+  w.sleep = dallocT<SleepState>(nBodies_);
+  w.sleepGlobals = dallocT<uint32_t>(4ull * nWorlds_ + 1);
+  w.gate = nullptr;
+  sleepEligible_.assign(nBodies_, 1u);
   w.bodyDone = dallocT<uint32_t>(nBodies_);
   w.dualDone = dallocT<uint32_t>(nBodies_);
   w.dfPlan = dallocT<DataflowPlan>(1);
@@ -387,10 +391,27 @@ void Engine::encodeCollision(const DeviceWorld &w, bool rebuild) {
   launch_narrowphase(w, s);
 }
 
-void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
+void Engine::encodeSleepImpact(const DeviceWorld &w, uint32_t force) {
   hipStream_t s = stream_;
+  launch_sleep_impact(w, sleepParams(force), s);
+  DeviceWorld g = w;
+  g.gate = w.sleepGlobals + 4ull * nWorlds_;
+  launch_sleep_refresh_prepare(g, s);
+  if (useBvh_) {
+    launch_bvh_refit(g, bvh_, plan_, s);  // the woken leaves are dynamic again for the pair search's pruning
+    launch_bvh_pairs(g, bvh_, s);
+  } else {
+    launch_pairs(g, s);
+  }
+  launch_narrowphase(g, s);
+}
+
+void Engine::encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce) {
+  hipStream_t s = stream_;
+  if (sleeping_) launch_sleep_before(w, sleepParams(sleepForce), s);
   if (bouncing_) launch_restitution_before(w, s);
   encodeCollision(w, rebuild);
+  if (sleeping_) encodeSleepImpact(w, sleepForce);
   launch_adjacency(w, pairCap_, s);
   launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, listedRounds_, s);
   launch_warm_start(w, s);
@@ -404,6 +425,7 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild) {
   }
   launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);
+  if (sleeping_) launch_sleep_after(w, sleepParams(sleepForce), s);
 }
 
 void Engine::pollCounters() {
@@ -427,7 +449,10 @@ void Engine::step(uint32_t n) {
     DeviceWorld &w = views_[parity_];
     w.colorCap = colorCap_;
     const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
-    if (useGraph_) {
+    if (sleepForce_) {  // the start-asleep step runs once, outside the graphs
+      encodeStep(w, rebuild, sleepForce_);
+      sleepForce_ = 0;
+    } else if (useGraph_) {
       hipGraphExec_t &exec = graphs_[parity_ * 2 + (rebuild ? 1 : 0)];
       if (!exec) {
         hipGraph_t g;
@@ -496,6 +521,9 @@ PhaseTimes Engine::profileStep() {
   w.colorCap = colorCap_;
   const bool rebuild = useBvh_ && stepIndex_ % rebuildInterval_ == 0;
   HIP_CHECK(hipEventRecord(ev[0], s));
+  const uint32_t force = sleepForce_;
+  sleepForce_ = 0;
+  if (sleeping_) launch_sleep_before(w, sleepParams(force), s);
   if (bouncing_) launch_restitution_before(w, s);
   // encodeCollision, with events after the frame set-up and after the broadphase
   launch_begin_frame(w, s);
@@ -513,6 +541,7 @@ PhaseTimes Engine::profileStep() {
   }
   HIP_CHECK(hipEventRecord(ev[2 + 2 * launches + 5], s));
   launch_narrowphase(w, s);
+  if (sleeping_) encodeSleepImpact(w, force);
   HIP_CHECK(hipEventRecord(ev[1], s));
   launch_adjacency(w, pairCap_, s);
   HIP_CHECK(hipEventRecord(ev[2], s));
@@ -540,6 +569,7 @@ PhaseTimes Engine::profileStep() {
   }
   launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);  // timed with the velocities
+  if (sleeping_) launch_sleep_after(w, sleepParams(force), s);
   HIP_CHECK(hipEventRecord(ev[5], s));
   HIP_CHECK(hipEventSynchronize(ev[5]));
   stepIndex_++;
@@ -594,6 +624,10 @@ void Engine::writeBodies(uint32_t first, uint32_t count, const Body *in) {
         break;
       }
   HIP_CHECK(hipMemcpyAsync(base_.bodies + first, in, size_t(count) * sizeof(Body), hipMemcpyHostToDevice, stream_));
+  if (sleeping_) {  // the written records' masses are the awake masses (rewriteBodies re-registers them)
+    launch_sleep_init(base_, nullptr, first, count, stream_);
+    wakeAll();
+  }
   synchronize();
 }
 void Engine::readJoints(uint32_t first, uint32_t count, Joint *out) {
@@ -855,6 +889,7 @@ void Engine::applyCommands(const uint32_t *words, uint32_t n) {
   if (teleports && !marks_) marks_ = dallocT<uint32_t>(nBodies_);
   uint32_t *dev = uploadEdit(buf.data(), buf.size());
   const uint2 *devRanges = reinterpret_cast<const uint2 *>(dev + 12ull * n);
+  wakeAll();  // body commands act on awake masses (every body wakes first)
   launch_body_commands(base_.bodies, dev, devRanges, nRanges, marks_, stream_);
   if (teleports) {
     launch_invalidate_manifolds(views_[parity_].prevManifolds, base_.counters, pairCap_, marks_, stream_);
@@ -874,6 +909,7 @@ void Engine::setFilters(const uint32_t *indices, const uint32_t *groups, const u
     buf[3 * k + 1] = groups[k];
     buf[3 * k + 2] = masks[k];
   }
+  wakeAll();
   launch_set_filters(base_.filters, uploadEdit(buf.data(), buf.size()), n, stream_);
   HIP_CHECK(hipGetLastError());
 }
@@ -895,12 +931,106 @@ void Engine::setRestitution(const uint32_t *indices, const float *values, uint32
     buf[2 * k] = indices[k];
     std::memcpy(&buf[2 * k + 1], &values[k], 4);
   }
+  wakeAll();
   launch_set_restitution(base_.restitution, uploadEdit(buf.data(), buf.size()), n, stream_);
   HIP_CHECK(hipGetLastError());
   if ((before == 0) != (bouncing_ == 0)) resetGraphs();  // the step gains or loses its bounce passes
 }
 
 void Engine::getRestitution(float *out) const { std::copy(restitution_.begin(), restitution_.end(), out); }
+
+void Engine::enableSleeping(float speed, float delay, int islands, bool startAsleep) {
+  if (!(std::isfinite(speed) && speed > 0.f && std::isfinite(delay) && delay > 0.f))
+    throw std::invalid_argument("Sleep speedThreshold and timeThreshold must be positive and finite");
+  if (islands < -1 || islands > 1) throw std::invalid_argument("islands must be -1 (auto), 0 or 1");
+  if (sleeping_) return;
+  HIP_CHECK(hipSetDevice(device_));
+  sleepSpeed_ = speed;
+  sleepDelay_ = delay;
+  sleepIslands_ = islands < 0 ? nBodies_ / std::max(1u, nWorlds_) >= 4096u : islands == 1;
+  std::vector<uint32_t> packed((sleepEligible_.size() + 3) / 4, 0u);
+  std::memcpy(packed.data(), sleepEligible_.data(), sleepEligible_.size());
+  const uint8_t *eligible = reinterpret_cast<const uint8_t *>(uploadEdit(packed.data(), packed.size()));
+  launch_sleep_init(base_, eligible, 0, nBodies_, stream_);
+  launch_fill_u32(base_.sleepGlobals, 0u, 4ull * nWorlds_ + 1, stream_);
+  HIP_CHECK(hipGetLastError());
+  sleeping_ = true;
+  sleepForce_ = startAsleep && sleepIslands_ ? 2u : 0u;
+  resetGraphs();
+}
+
+void Engine::disableSleeping() {
+  if (!sleeping_) return;
+  wakeAll();
+  sleeping_ = false;
+  sleepForce_ = 0;
+  resetGraphs();
+}
+
+void Engine::wakeAll() {
+  if (!sleeping_) return;
+  HIP_CHECK(hipSetDevice(device_));
+  launch_sleep_before(base_, sleepParams(1u), stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::wakeBodies(const uint32_t *indices, uint32_t n) {
+  if (n == 0) return;
+  if (!indices) throw std::invalid_argument("indices is NULL");
+  for (uint32_t k = 0; k < n; k++)
+    if (indices[k] >= nBodies_) throw std::invalid_argument("Sleeping wake requires a live GPU body index");
+  if (!sleeping_) return;
+  HIP_CHECK(hipSetDevice(device_));
+  launch_sleep_hold(base_, uploadEdit(indices, n), n, sleepIslands_, stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::setSleepEnabled(const uint32_t *indices, const uint8_t *flags, uint32_t n) {
+  if (n == 0) return;
+  if (!indices || !flags) throw std::invalid_argument("indices or flags is NULL");
+  bool any = false;
+  for (uint32_t k = 0; k < n; k++) {
+    if (indices[k] >= nBodies_) throw std::invalid_argument("A live 3D GPU body index is required");
+    if (flags[k] > 1) throw std::invalid_argument("Sleep flag must be boolean");
+    any = any || flags[k] == 1;
+  }
+  std::vector<uint32_t> buf(2ull * n);
+  for (uint32_t k = 0; k < n; k++) {
+    sleepEligible_[indices[k]] = flags[k];
+    buf[2 * k] = indices[k];
+    buf[2 * k + 1] = flags[k];
+  }
+  if (!sleeping_) {
+    if (any) enableSleeping(sleepSpeed_, sleepDelay_, -1, false);  // takes every body's eligibility
+    return;
+  }
+  HIP_CHECK(hipSetDevice(device_));
+  launch_sleep_eligible(base_, uploadEdit(buf.data(), buf.size()), n, stream_);
+  wakeAll();
+}
+
+void Engine::sleepStats(uint32_t world, uint32_t *out) {
+  if (world >= nWorlds_) throw std::out_of_range("world index");
+  out[0] = out[1] = 0;
+  if (!sleeping_) return;
+  HIP_CHECK(hipSetDevice(device_));
+  uint32_t g[4];
+  HIP_CHECK(hipMemcpyAsync(g, base_.sleepGlobals + 4ull * world, sizeof g, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  out[0] = g[1];
+  out[1] = g[0];
+}
+
+void Engine::readSleep(uint32_t *states, uint32_t *globals) {
+  HIP_CHECK(hipSetDevice(device_));
+  if (states)
+    HIP_CHECK(hipMemcpyAsync(states, base_.sleep, size_t(nBodies_) * sizeof(SleepState), hipMemcpyDeviceToHost,
+                             stream_));
+  if (globals)
+    HIP_CHECK(hipMemcpyAsync(globals, base_.sleepGlobals, 4ull * nWorlds_ * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             stream_));
+  synchronize();
+}
 
 void Engine::setParams(uint32_t world, const ParamEdit &e) {
   if (world >= nWorlds_) throw std::invalid_argument("world index out of range");

@@ -17,6 +17,25 @@ struct DataflowPlan {
   uint32_t heavyWaves[64];
 };
 
+/** One body's sleeping state (32 B, the reference's per-body sleep record). */
+struct SleepState {
+  float mass;         // the body's mass while awake (0: static)
+  float quiet;        // seconds it has been quiet and supported without a break
+  uint32_t asleep;
+  uint32_t support;   // this step: a contact holds it from below
+  uint32_t eligible;  // it may sleep
+  uint32_t parent;    // island forest (island mode): parent <= index; each body's island root after every step
+  uint32_t wake;      // at an island root, this step: 1 hit, 2 rests on still static support, 4 busy, 8 wake requested
+  uint32_t flags;     // 1 quiet and supported at the last rest pass, 2 held awake, 4 still at start, 8 island woken
+};
+static_assert(sizeof(SleepState) == 32, "SleepState is 32 bytes");
+
+/** Sleeping parameters of a launch: thresholds, mode, and force bits (1 wake all, 2 start asleep). */
+struct SleepParams {
+  float speed, delay;
+  uint32_t islands, force;
+};
+
 struct DeviceWorld {
   // Scene
   Body *bodies;
@@ -87,7 +106,14 @@ struct DeviceWorld {
   uint4 *colorSlot;       // per bucketed body: body, first and end adjacency entry, world
   uint32_t colorGroups;
   uint32_t colorCap;
+  // Sleeping
+  SleepState *sleep;        // per body
+  uint32_t *sleepGlobals;   // per world: wake all, sleeping, woken by an impact, island hit; then bodies woken (gate)
+  const uint32_t *gate;     // null, or: the collision refresh's kernels return while *gate == 0
 };
+
+/** A gated launch (the collision refresh after sleepers woke) has nothing to do. */
+__device__ __forceinline__ bool gatedOff(const DeviceWorld &w) { return w.gate != nullptr && *w.gate == 0u; }
 
 /** The writable parameters of a world (its step counter, body range and large-body radius are kept). */
 struct ParamEdit {

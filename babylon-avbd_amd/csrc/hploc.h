@@ -44,6 +44,7 @@ struct BvhPlan {
 };
 
 struct Bvh {
+  const uint32_t *gate;  // set by the launchers from the DeviceWorld: a gated launch returns while *gate == 0
   BvhNode *nodes;      // leaves [0, N), internal nodes [N, 2N - W)
   BvhPair *pairs;      // per internal node: its children's boxes, tags and maxPos (leaves' entries unused)
   BvhQuad *quads;      // per internal node id - N: its four lifted entries (what the pair search walks)

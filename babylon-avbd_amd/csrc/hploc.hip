@@ -267,6 +267,7 @@ __global__ void k_bvh_leafmax(DeviceWorld w, Bvh t) {
 }
 /** A refit's leaves: boxes from the current poses and maxPos from the last build's ranks. */
 __global__ void k_bvh_refit_leaves(DeviceWorld w, Bvh t) {
+  if (gatedOff(w)) return;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nBodies) return;
   leafBox(w, t, i);
@@ -287,6 +288,7 @@ constexpr uint32_t LEAF = 0x80000000u;
  */
 __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup *groups, uint32_t nGroups,
                                                            uint32_t nLeaves) {
+  if (t.gate && *t.gate == 0u) return;
   __shared__ float4 nodeLo[64], nodeHi[64];         // refitted box of each node slot
   __shared__ uint32_t nodeMp[64];
   __shared__ float4 childLo[2][64], childHi[2][64];  // per node slot: its two children's boxes and maxPos
@@ -346,6 +348,7 @@ __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup
 
 /** The quads of the internal nodes [N, nodeCount) from their pair records (after a refit wrote them). */
 __global__ void k_bvh_quads(Bvh t, uint32_t nLeaves) {
+  if (t.gate && *t.gate == 0u) return;
   const uint32_t id = nLeaves + blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= t.nodeCount) return;
   const BvhPair p = t.pairs[id];
@@ -418,6 +421,7 @@ template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint
  * leaves. Threads take bodies in Morton order (neighbouring threads walk neighbouring paths).
  */
 __global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
+  if (gatedOff(w)) return;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= w.nBodies) return;
   uint32_t i = t.order[q];
@@ -445,6 +449,7 @@ __global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
  * more candidates than it kept was counted by pass 1.
  */
 __global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
+  if (gatedOff(w)) return;
   const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t i = uint32_t(g / 32), lane = uint32_t(g % 32);
   const bool live = i < w.nBodies;
@@ -472,6 +477,7 @@ __global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
  * clamped to the capacity (overflow bit 0 when it is not).
  */
 __global__ void k_bvh_write(DeviceWorld w, Bvh t) {
+  if (gatedOff(w)) return;
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q == 0) {
     uint32_t total = w.pairCount[w.nBodies];
@@ -520,7 +526,9 @@ __global__ void k_bvh_copy(uint32_t *dst, const uint32_t *src, uint32_t n) {
   if (i < n) dst[i] = src[i];
 }
 
-void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
+void launch_bvh_build(const DeviceWorld &w, const Bvh &tree, const BvhPlan &plan, hipStream_t s) {
+  Bvh t = tree;
+  t.gate = nullptr;
   const uint32_t n = w.nBodies;
   launch_fill_u32(t.parent, END, plan.nodeCount, s);
   k_bvh_leaves<<<blocks(n, 256), 256, 0, s>>>(w, t);
@@ -557,7 +565,9 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
   if (plan.nodeCount > n) k_bvh_quads<<<blocks(plan.nodeCount - n, 256), 256, 0, s>>>(t, n);
 }
 
-void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, hipStream_t s) {
+void launch_bvh_refit(const DeviceWorld &w, const Bvh &tree, const BvhPlan &plan, hipStream_t s) {
+  Bvh t = tree;
+  t.gate = w.gate;
   k_bvh_refit_leaves<<<blocks(w.nBodies, 256), 256, 0, s>>>(w, t);
   for (const BvhLevel &lev : plan.levels)
     k_bvh_refit_treelets<<<blocks(lev.groupCount, 4), 64, 0, s>>>(t, t.groups + lev.groupOffset, lev.groupCount,
@@ -566,11 +576,13 @@ void launch_bvh_refit(const DeviceWorld &w, const Bvh &t, const BvhPlan &plan, h
     k_bvh_quads<<<blocks(plan.nodeCount - w.nBodies, 256), 256, 0, s>>>(t, w.nBodies);
 }
 
-void launch_bvh_pairs(const DeviceWorld &w, const Bvh &t, hipStream_t s) {
+void launch_bvh_pairs(const DeviceWorld &w, const Bvh &tree, hipStream_t s) {
+  Bvh t = tree;
+  t.gate = w.gate;
   const size_t stackBytes = size_t(64) * t.stackDepth * sizeof(uint32_t);
   k_bvh_cand<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
   k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
-  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT);
+  launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT, w.gate);
   k_bvh_write<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
 }
 

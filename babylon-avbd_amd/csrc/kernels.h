@@ -13,7 +13,7 @@ namespace avbd {
  * stream. A wait given up (never expected) sets *abortFlag, or scratch word 2 when abortFlag is null.
  */
 void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s,
-                           uint32_t *abortFlag = nullptr);
+                           uint32_t *abortFlag = nullptr, const uint32_t *gate = nullptr);
 uint32_t scan_scratch_words(uint32_t n);
 /** Fill n words with a value (a kernel, so it can be captured in a graph). */
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s);
@@ -49,6 +49,16 @@ void launch_velocities(const DeviceWorld &w, hipStream_t s);
 /** Restitution (src/gpu/restitution.js): incoming velocities before the step; bounce rounds after it. */
 void launch_restitution_before(const DeviceWorld &w, hipStream_t s);
 void launch_restitution_after(const DeviceWorld &w, hipStream_t s);
+
+/** Sleeping: state set-up and edits (outside the step), the step's passes before it, after its narrowphase (impacts,
+ * then the collision refresh's preparation, gated on bodies having woken) and after it. */
+void launch_sleep_init(const DeviceWorld &w, const uint8_t *eligible, uint32_t first, uint32_t count, hipStream_t s);
+void launch_sleep_eligible(const DeviceWorld &w, const uint32_t *words, uint32_t n, hipStream_t s);
+void launch_sleep_hold(const DeviceWorld &w, const uint32_t *indices, uint32_t n, bool islands, hipStream_t s);
+void launch_sleep_before(const DeviceWorld &w, const SleepParams &p, hipStream_t s);
+void launch_sleep_impact(const DeviceWorld &w, const SleepParams &p, hipStream_t s);
+void launch_sleep_refresh_prepare(const DeviceWorld &w, hipStream_t s);
+void launch_sleep_after(const DeviceWorld &w, const SleepParams &p, hipStream_t s);
 
 /** Blocks of the dataflow solve that are resident at once on the current device (0 if unknown). */
 uint32_t dataflow_grid();

@@ -25,7 +25,9 @@ constexpr uint64_t SCAN_TOTAL = 1ull << 62, SCAN_PREFIX = 2ull << 62;
 constexpr uint32_t SCAN_SPIN = 1u << 20;
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_onepass(uint32_t *data, uint32_t n, uint32_t tiles,
-                                                              uint32_t *state, uint32_t *abortFlag) {
+                                                              uint32_t *state, uint32_t *abortFlag,
+                                                              const uint32_t *gate) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t waveTot[SCAN_BLOCK / 64];
   __shared__ uint32_t tileShared, prefixShared, lastShared;
   uint64_t *status = reinterpret_cast<uint64_t *>(state + 4);
@@ -120,10 +122,11 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s) {
 
 uint32_t scan_scratch_words(uint32_t n) { return 4u + 2u * ((n + SCAN_TILE - 1) / SCAN_TILE); }
 
-void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s, uint32_t *abortFlag) {
+void launch_scan_exclusive(uint32_t *data, uint32_t n, uint32_t *scratch, hipStream_t s, uint32_t *abortFlag,
+                           const uint32_t *gate) {
   if (n == 0) return;
   const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  k_scan_onepass<<<tiles, SCAN_BLOCK, 0, s>>>(data, n, tiles, scratch, abortFlag);
+  k_scan_onepass<<<tiles, SCAN_BLOCK, 0, s>>>(data, n, tiles, scratch, abortFlag, gate);
 }
 
 }  // namespace avbd

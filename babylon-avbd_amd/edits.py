@@ -13,7 +13,7 @@ import numpy as np
 
 EDIT_KEYS = ("linearVelocity", "angularVelocity", "impulse", "force", "", "angularImpulse", "torque")
 LINEAR_VELOCITY, ANGULAR_VELOCITY, IMPULSE, FORCE, TELEPORT, ANGULAR_IMPULSE, TORQUE = range(7)
-_UNSUPPORTED = ("isTrigger", "allowSleep")
+_UNSUPPORTED = ("isTrigger",)
 
 
 def components(value, count, name):
@@ -107,8 +107,8 @@ class CommandStaging:
 
 def stage_edits(edits, body_index, dt_of, commands):
     """stageEdits (runtime.js:60-175) into `commands` (cleared first): per edit, a teleport first, then the remaining
-    kinds in editKeys order. Returns the property edits (body, group, collidesWith, restitution) to apply afterwards
-    (group None: no filter change; restitution None: no coefficient change).
+    kinds in editKeys order. Returns the property edits (body, group, collidesWith, restitution, allowSleep) to apply
+    afterwards (group None: no filter change; restitution or allowSleep None: no change).
 
     body_index(body) -> live slot (raises otherwise); dt_of(slot) -> the body's world time step."""
     commands.clear()
@@ -122,11 +122,14 @@ def stage_edits(edits, body_index, dt_of, commands):
         restitution = edit.get("restitution")
         if restitution is not None:
             restitution = coefficient(restitution)
-        if group is not None or collides is not None or restitution is not None:
+        allow = edit.get("allowSleep")
+        if allow is not None and not isinstance(allow, bool):
+            raise ValueError("allowSleep must be a boolean")
+        if group is not None or collides is not None or restitution is not None or allow is not None:
             if collides is not None and group is None:
                 raise ValueError("A collision-mask edit requires group as well")
             properties.append((index, None if group is None else mask(group),
-                               mask(collides) if collides is not None else 0xFFFFFFFF, restitution))
+                               mask(collides) if collides is not None else 0xFFFFFFFF, restitution, allow))
         point = edit.get("worldPoint")
         point = None if point is None else components(point, 3, "worldPoint")
         if edit.get("position") is not None:

@@ -112,6 +112,8 @@ class AvbdScene3D:
         self._count = 0
         self._filters = {}        # body -> (group, collidesWith)
         self._restitution = {}    # body -> restitution coefficient above 0
+        self._no_sleep = set()    # bodies built with allowSleep false
+        self.sleepOptIn = False   # a body was built with allowSleep true: its world sleeps (scenes.js:233-240, 689)
         self._joints = []         # (record[32], info[4])
         self._ignore = []
         self.created = False
@@ -155,7 +157,13 @@ class AvbdScene3D:
         for m in (group, mask):
             if int(m) != m or m < 0 or m > 0xFFFFFFFF:
                 raise ValueError("Collision masks must be unsigned 32-bit integers")
+        allow = options.get("allowSleep")
+        if allow is not None and not isinstance(allow, bool):
+            raise ValueError("allowSleep must be a boolean")
         index = self._count
+        if allow is False:
+            self._no_sleep.add(index)
+        self.sleepOptIn = self.sleepOptIn or allow is True
         if group != 0xFFFFFFFF or mask != 0xFFFFFFFF:
             self._filters[index] = (int(group), int(mask))
         if restitution is not None and r > 0:

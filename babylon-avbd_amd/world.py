@@ -106,6 +106,18 @@ class HipWorld:
                   for i, e in sorted(sc._restitution.items())]
         if bouncy:
             self.setRestitution([i for i, _ in bouncy], [e for _, e in bouncy])
+        self._masses = np.concatenate([np.concatenate(s._chunks)[:, 19] if s._chunks else np.zeros(0, np.float32)
+                                       for s in scenes]).astype(np.float32)
+        # Sleeping (scenes.js:689-693): asked for in the options, or a body was built with allowSleep true
+        no_sleep = [int(first) + i for first, sc in zip(self.world_first_body, scenes) for i in sorted(sc._no_sleep)]
+        if no_sleep:
+            flags = np.zeros(len(no_sleep), np.uint8)
+            idx = np.array(no_sleep, np.uint32)
+            check(lib.avbd_world_set_sleep_enabled(self._h, ptr(idx, ctypes.c_uint32), ptr(flags, ctypes.c_uint8),
+                                                   len(idx)))
+        sleeping = options.get("sleeping")
+        if sleeping or any(s.sleepOptIn for s in scenes):
+            self.enableSleeping(sleeping if isinstance(sleeping, dict) else {})
 
     # --- lifetime ---------------------------------------------------------------------------------------------------
 
@@ -244,12 +256,103 @@ class HipWorld:
         staged = _edits.CommandStaging()
         properties = _edits.stage_edits(edits, self.bodyIndex, self._dt_of, staged)
         self._pending.extend(staged)
-        for index, group, collides, restitution in properties:
+        for index, group, collides, restitution, allow in properties:
             if group is not None:
                 self.setFilters([index], [group], [collides])
             if restitution is not None:
                 self.setRestitution([index], [restitution])
+            if allow is not None:
+                self.setSleepEnabled(index, allow)
         return self
+
+    # --- sleeping (worldSleeping.js; solver3D.js:317-343; appGpuSolver3D.js:869-889) -------------------------------
+
+    def enableSleeping(self, options=None):
+        """enableSleeping({speedThreshold, timeThreshold, islands, startAsleep}): bodies quiet and supported for
+        timeThreshold seconds sleep as fixed supports until an impact, an edit or a request wakes them. islands (default:
+        worlds of 4096 bodies or more) wakes and sleeps connected groups together instead of the whole world;
+        startAsleep (island mode) takes one step that puts still, grounded groups to sleep at once."""
+        self._alive()
+        o = dict(options or {})
+        unknown = set(o) - {"speedThreshold", "timeThreshold", "islands", "startAsleep"}
+        if unknown:
+            raise ValueError(f"Unknown sleep option: {sorted(unknown)[0]}")
+        speed, delay = o.get("speedThreshold", 0.03), o.get("timeThreshold", 0.5)
+        ok = all(isinstance(v, (int, float)) and not isinstance(v, bool) and math.isfinite(v) and v > 0
+                 for v in (speed, delay))
+        if not ok:
+            raise ValueError("Sleep speedThreshold and timeThreshold must be positive and finite")
+        if o.get("startAsleep") is not None and not isinstance(o["startAsleep"], bool):
+            raise ValueError("startAsleep must be a boolean")
+        if o.get("islands") is not None and not isinstance(o["islands"], bool):
+            raise ValueError("Sleep islands must be a boolean")
+        if self.sleeping:
+            return self
+        islands = -1 if o.get("islands") is None else int(o["islands"])
+        start = bool(o.get("startAsleep", False))
+        self.flushEdits()
+        check(self._lib.avbd_world_enable_sleeping(self._h, float(speed), float(delay), islands, int(start)))
+        if start and self.sleepIslands:
+            self.step()
+        return self
+
+    @property
+    def sleeping(self):
+        return bool(self._lib.avbd_world_sleeping(self._h, None)) if not self.destroyed else False
+
+    @property
+    def sleepIslands(self):
+        mode = ctypes.c_int(0)
+        on = self._lib.avbd_world_sleeping(self._h, ctypes.byref(mode))
+        return bool(on and mode.value)
+
+    def disableSleeping(self):
+        """Every body wakes with its mass back (poses kept) and sleeping stops."""
+        self._alive()
+        self.flushEdits()
+        check(self._lib.avbd_world_disable_sleeping(self._h))
+        return self
+
+    def wakeAll(self):
+        self._alive()
+        self.flushEdits()
+        check(self._lib.avbd_world_wake_all(self._h))
+        return self
+
+    def wakeBodies(self, indices):
+        """wakeBody for each index: held awake through the next step; in island mode their islands wake."""
+        self._alive()
+        idx = np.array([self.liveIndex(i) for i in np.atleast_1d(indices)], np.uint32)
+        self.flushEdits()
+        check(self._lib.avbd_world_wake_bodies(self._h, ptr(idx, ctypes.c_uint32), len(idx)))
+        return self
+
+    def setSleepEnabled(self, index, enabled):
+        """Sleep eligibility of one body; allowing it enables sleeping; every body wakes."""
+        self._alive()
+        if not isinstance(enabled, bool):
+            raise ValueError("Sleep flag must be boolean")
+        idx = np.array([self.liveIndex(index)], np.uint32)
+        flag = np.array([int(enabled)], np.uint8)
+        self.flushEdits()
+        check(self._lib.avbd_world_set_sleep_enabled(self._h, ptr(idx, ctypes.c_uint32), ptr(flag, ctypes.c_uint8), 1))
+        return self
+
+    def readSleepStats(self, world=0):
+        """{sleeping, wakeRequested} of one world after the last step."""
+        self._ready()
+        out = np.zeros(2, np.uint32)
+        check(self._lib.avbd_world_sleep_stats(self._h, int(world), ptr(out, ctypes.c_uint32)))
+        return {"sleeping": int(out[0]), "wakeRequested": bool(out[1])}
+
+    def readSleep(self):
+        """Every body's sleep state as an (n, 8) u32 array (mass and quiet as f32 bits, asleep, support, eligible,
+        parent, wake, flags) and the (worlds, 4) sleep globals."""
+        self._ready()
+        states = np.zeros((self.bodyCount, 8), np.uint32)
+        globals_ = np.zeros((self.worldCount, 4), np.uint32)
+        check(self._lib.avbd_world_read_sleep(self._h, ptr(states, ctypes.c_uint32), ptr(globals_, ctypes.c_uint32)))
+        return states, globals_
 
     def setRestitution(self, indices, values):
         """setRestitution (appGpuSolver3D.js:455-471) for one body (index, value) or many: coefficients in [0, 1],
@@ -340,6 +443,7 @@ class HipWorld:
         self._ready()
         records = np.ascontiguousarray(records, np.float32).reshape(-1, 40)
         check(self._lib.avbd_world_write_bodies(self._h, int(first), len(records), ptr(records, ctypes.c_float)))
+        self._masses[int(first):int(first) + len(records)] = records[:, 19]  # the awake masses from now on
 
     def readSelectedBodies(self, indices, posesOnly=False):
         """readSelectedBodies (solver3D.js:223-230): 40 or (posesOnly) 8 f32 per body."""
@@ -368,7 +472,8 @@ class HipWorld:
         c = _native.Counters()
         check(self._lib.avbd_world_read_counters(self._h, ctypes.byref(c)))
         return dict(pairs=c.pairs, contacts=c.contacts, manifolds=c.manifolds, joints=c.joints, overflow=c.overflow,
-                    clashes=c.clashes, colors=c.colors, sleeping=0, pivotFallbacks=c.pivot_fallbacks,
+                    clashes=c.clashes, colors=c.colors,
+                    sleeping=self.readSleepStats()["sleeping"] if self.sleeping else 0, pivotFallbacks=c.pivot_fallbacks,
                     solveAborts=c.solve_aborts)
 
     def readPairs(self):
@@ -507,9 +612,12 @@ class HipWorld:
 
     def readBodyState(self, index):
         """Position, rotation, linear and angular velocity of one body (queued edits applied first)."""
-        r = self.readSelectedBodies([self.liveIndex(index)])[0]
+        index = self.liveIndex(index)
+        r = self.readSelectedBodies([index])[0]
+        mass = float(self._masses[index])
         return {"position": r[0:3].tolist(), "rotation": r[4:8].tolist(), "linearVelocity": r[32:35].tolist(),
-                "angularVelocity": r[36:39].tolist()}
+                "angularVelocity": r[36:39].tolist(), "mass": mass, "effectiveMass": float(r[19]),
+                "sleeping": mass > 0 and r[19] == 0}
 
     def setVelocities(self, indices, linear=None, angular=None):
         """setVelocities(indices, {linear, angular}) (runtime.js:356-378): xyz per body, the batch checked whole."""
@@ -541,15 +649,11 @@ class HipWorld:
     def setAngularVelocities(self, indices, values):
         return self.setVelocities(indices, angular=values)
 
-    def disableSleeping(self):
-        """Sleeping is not built on this backend: bodies never sleep, which is what disabling it asks for."""
-        self._alive()
-        return self
 
 
 class HipBody:
-    """A body handle (runtime.js:168-237 NativeBody): chainable edits of one slot and readState(). Trigger bodies and
-    sleeping are not built on this backend: turning them on raises, turning them off is accepted."""
+    """A body handle (runtime.js:168-237 NativeBody): chainable edits of one slot and readState(). Trigger bodies are
+    not built on this backend: turning one on raises, turning it off is accepted."""
 
     def __init__(self, solver, index):
         self.solver = solver
@@ -602,11 +706,11 @@ class HipBody:
         return self
 
     def setSleepEnabled(self, enabled):
-        self.gpuIndex  # recorded nowhere: this backend does not put bodies to sleep
+        self.solver.setSleepEnabled(self.gpuIndex, enabled)
         return self
 
     def wakeUp(self):
-        self.gpuIndex
+        self.solver.wakeBodies([self.gpuIndex])
         return self
 
     def readState(self):

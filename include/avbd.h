@@ -242,6 +242,37 @@ int avbd_world_set_restitution(avbd_world *world, const uint32_t *indices, const
 /** The coefficients of all bodies (world->bodies floats). */
 int avbd_world_get_restitution(avbd_world *world, float *out);
 
+/* ---- Sleeping (src/gpu/worldSleeping.js, worldSleeping.wgsl, islandSleeping.wgsl, sleepContact.wgsl) ----------------
+ * A sleeping body is a fixed support: its mass (record float 19) reads 0 and its velocities 0 until it wakes, when its
+ * awake mass comes back. Body commands, filter and restitution edits and body writes wake every body first, as the
+ * reference's wakeAll does before its property and command flushes. */
+
+/** enableSleeping(options) (appGpuSolver3D.js:869-881, worldSleeping.js:17-40): speed_threshold (linear speed plus
+ * angular speed times bounding radius, world units/s; the reference's default 0.03) and time_threshold (seconds quiet
+ * and supported; default 0.5), both positive and finite; islands -1 (auto: worlds averaging 4096 bodies or more),
+ * 0 (any impact wakes the world) or 1 (connected groups sleep and wake together); start_asleep (island mode): the next
+ * step puts still groups resting on still static support to sleep. Enabling again keeps the first settings. */
+int avbd_world_enable_sleeping(avbd_world *world, float speed_threshold, float time_threshold, int islands,
+                               int start_asleep);
+/** disableSleeping() (solver3D.js:333-343): every body wakes with its mass back; sleeping stops. */
+int avbd_world_disable_sleeping(avbd_world *world);
+/** 1 if sleeping is enabled, 0 if not; *islands (if not NULL) its mode. */
+int avbd_world_sleeping(avbd_world *world, int *islands);
+/** wakeAll() (appGpuSolver3D.js:882-884): every body wakes before the next step. */
+int avbd_world_wake_all(avbd_world *world);
+/** wakeBody(index) (worldSleeping.js:226-235) for n bodies: held awake through the next step; in island mode their
+ * islands wake at its start. */
+int avbd_world_wake_bodies(avbd_world *world, const uint32_t *indices, uint32_t n);
+/** setSleepEnabled(index, enabled) (solver3D.js:317-327) for n bodies (flags 0 or 1): eligibility; a body allowed to
+ * sleep enables sleeping with the default thresholds; every body wakes. */
+int avbd_world_set_sleep_enabled(avbd_world *world, const uint32_t *indices, const uint8_t *flags, uint32_t n);
+/** readSleepStats() (appGpuSolver3D.js:885-889, worldSleeping.js:257-274) of world world_index after the last step:
+ * out[0] bodies asleep, out[1] 1 if a wake-all request is pending. */
+int avbd_world_sleep_stats(avbd_world *world, uint32_t world_index, uint32_t out[2]);
+/** Every body's sleep state, 8 words (mass and quiet time as f32 bits, asleep, support, eligible, island parent, island
+ * wake bits, flags), and 4 words per world (wake all, asleep, woken by an impact, island hit); either may be NULL. */
+int avbd_world_read_sleep(avbd_world *world, uint32_t *states, uint32_t *globals);
+
 /* ---- Serving loop ------------------------------------------------------------------------------------------------
  * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):
  * a server takes step/rollout requests for many registered worlds as they arrive, keeps world state resident in HBM

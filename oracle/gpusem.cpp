@@ -360,6 +360,7 @@ void gs_read_refs(void *p, float *out) {
 /** Body commands (12 words each, World::applyCommands), in submission order. */
 void gs_apply_commands(void *p, const uint32_t *words, uint32_t n) {
   auto *h = static_cast<Handle *>(p);
+  if (h->f->sleepOn) h->f->sleepWakeAll();  // body commands wake every body first (prepareBodyCommands)
   h->f->applyCommands(words, n);
   if (h->d) h->d->applyCommands(words, n);
 }
@@ -391,6 +392,7 @@ void gs_set_iterations(void *p, int iterations) {
 /** Collision filters (groups, collides-with) of n bodies. */
 void gs_set_filters(void *p, const uint32_t *indices, const uint32_t *groups, const uint32_t *masks, uint32_t n) {
   auto *h = static_cast<Handle *>(p);
+  if (h->f->sleepOn) h->f->sleepWakeAll();
   for (uint32_t k = 0; k < n; k++) {
     h->f->filters[2 * size_t(indices[k])] = groups[k];
     h->f->filters[2 * size_t(indices[k]) + 1] = masks[k];
@@ -404,10 +406,70 @@ void gs_set_filters(void *p, const uint32_t *indices, const uint32_t *groups, co
 /** Restitution coefficients of n bodies (the bounce passes run while any is above 0). */
 void gs_set_restitution(void *p, const uint32_t *indices, const float *values, uint32_t n) {
   auto *h = static_cast<Handle *>(p);
+  if (h->f->sleepOn) h->f->sleepWakeAll();
   for (uint32_t k = 0; k < n; k++) {
     h->f->restitution[indices[k]] = values[k];
     if (h->d) h->d->restitution[indices[k]] = double(values[k]);
   }
+}
+
+// --- sleeping (FP32 mirror only) ------------------------------------------------------------------------------------
+
+/** Enable sleeping (eligible: one byte per body, null = all); returns -1 for an FP64 handle. */
+int gs_enable_sleeping(void *p, float speed, float delay, int islands, int startAsleep, const uint8_t *eligible) {
+  auto *h = static_cast<Handle *>(p);
+  if (h->d) return -1;
+  h->f->sleepEnable(speed, delay, islands != 0, startAsleep != 0, eligible);
+  return 0;
+}
+void gs_disable_sleeping(void *p) { static_cast<Handle *>(p)->f->sleepDisable(); }
+void gs_wake_all(void *p) {
+  auto *h = static_cast<Handle *>(p);
+  if (h->f->sleepOn) h->f->sleepWakeAll();
+}
+/** wakeBody() for n bodies: held awake through the next step; in island mode their islands wake. */
+void gs_wake_bodies(void *p, const uint32_t *indices, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  if (h->f->sleepOn) h->f->sleepHold(indices, n);
+}
+/** Sleep eligibility of n bodies, then every body wakes (setSleepEnabled). */
+void gs_set_sleep_enabled(void *p, const uint32_t *indices, const uint8_t *flags, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  if (!h->f->sleepOn) return;
+  for (uint32_t k = 0; k < n; k++) h->f->sleep[indices[k]].eligible = flags[k];
+  h->f->sleepWakeAll();
+}
+/** World k's sleeping count and wake-all request after the last step. */
+void gs_sleep_stats(void *p, uint32_t k, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  out[0] = h->f->sleepOn ? h->f->sleepGlobals[4 * k + 1] : 0u;
+  out[1] = h->f->sleepOn ? h->f->sleepGlobals[4 * k] : 0u;
+}
+/** Every body's sleep state, 8 words: mass (f32 bits), quiet (f32 bits), asleep, support, eligible, parent, wake,
+ * flags. */
+void gs_read_sleep(void *p, uint32_t *out) {
+  auto *h = static_cast<Handle *>(p);
+  for (size_t i = 0; i < h->f->sleep.size(); i++) {
+    const auto &s = h->f->sleep[i];
+    uint32_t *o = out + 8 * i;
+    o[0] = floatToBits(s.mass);
+    o[1] = floatToBits(s.quiet);
+    o[2] = s.asleep; o[3] = s.support; o[4] = s.eligible; o[5] = s.parent; o[6] = s.wake; o[7] = s.flags;
+  }
+}
+
+/** Seed every body's sleep state (gs_read_sleep's layout) and every world's 4 sleep globals. */
+void gs_set_sleep(void *p, const uint32_t *states, const uint32_t *globals) {
+  auto *h = static_cast<Handle *>(p);
+  if (!h->f->sleepOn) return;
+  for (size_t i = 0; i < h->f->sleep.size(); i++) {
+    auto &s = h->f->sleep[i];
+    const uint32_t *o = states + 8 * i;
+    s.mass = bitsToFloat(o[0]);
+    s.quiet = bitsToFloat(o[1]);
+    s.asleep = o[2]; s.support = o[3]; s.eligible = o[4]; s.parent = o[5]; s.wake = o[6]; s.flags = o[7];
+  }
+  std::copy(globals, globals + h->f->sleepGlobals.size(), h->f->sleepGlobals.begin());
 }
 
 // --- unit-level probes (for tests of single functions against the upstream restatement) -----------------------------

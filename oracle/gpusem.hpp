@@ -989,6 +989,13 @@ template <class R> class World {
     updateRefs();
     findPairs();
     narrowphase();
+    // sleepers woken by an impact: their pairs with other sleepers were left out above, so collide again
+    if (sleepOn && sleepImpact()) {
+      manifolds.clear();
+      contacts.clear();
+      findPairs();
+      narrowphase();
+    }
     buildAdjacency();
     colour();
   }
@@ -1134,13 +1141,279 @@ template <class R> class World {
     }
   }
 
+  // --- sleeping (src/gpu/worldSleeping.js, worldSleeping.wgsl, islandSleeping.wgsl, sleepContact.wgsl) --------------
+  // A sleeping body is a fixed support: its mass (size.w) is 0 and its velocities are 0; its mass while awake is kept in
+  // its sleep state and restored when it wakes. Whole-world mode: a body sleeps once it has been quiet (linear speed +
+  // angular speed × bounding radius at most `speed`) and supported (gravity 0, or resting on a static, sleeping or
+  // quiet partner below it) for `delay` seconds; any impact on a sleeper wakes the world. Island mode: bodies joined by
+  // contacts and joints (links among sleepers are kept while they sleep) form islands in a union-find forest whose
+  // roots are the islands' smallest indices; an island sleeps only when no member is busy and wakes as a whole.
+
+  struct Sleep {
+    R mass = 0, quiet = 0;
+    uint32_t asleep = 0, support = 0, eligible = 1, parent = 0, wake = 0, flags = 0;
+  };
+  enum : uint32_t { SF_READY = 1, SF_HELD = 2, SF_START_QUIET = 4, SF_GATHERED = 8 };  // Sleep::flags
+  enum : uint32_t { SW_IMPACT = 1, SW_GROUNDED = 2, SW_BUSY = 4, SW_REQUEST = 8 };     // Sleep::wake (at a root)
+  std::vector<Sleep> sleep;
+  std::vector<uint32_t> sleepGlobals;  // per world: wake all, sleeping, woken by an impact, local wake signal
+  bool sleepOn = false, sleepIslands = false;
+  uint32_t sleepForce = 0;  // the next step's force bits: 1 wake all, 2 start asleep (island mode)
+  R sleepSpeed = R(0.03f), sleepDelay = R(0.5f);
+  static constexpr uint32_t SLEEP_NONE = 0xffffffffu;
+
+  /** worldSleeping.js:17-103, register(): every body's state from its current mass and eligibility. */
+  void sleepEnable(float speed, float delay, bool islands, bool startAsleep, const uint8_t *eligible) {
+    sleepOn = true;
+    sleepIslands = islands;
+    sleepSpeed = R(speed);
+    sleepDelay = R(delay);
+    sleep.assign(n(), Sleep{});
+    for (uint32_t i = 0; i < n(); i++) {
+      sleep[i].mass = bodies[i].mass;
+      sleep[i].eligible = eligible ? eligible[i] : 1u;
+      sleep[i].parent = i;
+    }
+    sleepGlobals.assign(4 * params.size(), 0u);
+    sleepForce = startAsleep && islands ? 2u : 0u;
+  }
+  /** disableSleeping(): wake every body (restoring masses), then drop the state. */
+  void sleepDisable() {
+    if (!sleepOn) return;
+    sleepWakeAll();
+    sleepOn = false;
+    sleep.clear();
+  }
+  R motion(uint32_t i) const { return length(bodies[i].vel) + length(bodies[i].angVel) * bodies[i].radius; }
+  uint32_t *globalsOf(uint32_t i) { return &sleepGlobals[4 * world[i]]; }
+  void wakeOne(uint32_t i) {
+    Sleep &s = sleep[i];
+    if (s.mass > 0) {
+      bodies[i].mass = s.mass;
+      s.asleep = 0;
+      s.quiet = 0;
+      s.flags = 0;
+    }
+  }
+  /** islandSleeping.wgsl:104-118 root() (its path halving changes no root, so it is left out here). */
+  uint32_t sleepRoot(uint32_t i) const {
+    while (sleep[i].parent != i) i = sleep[i].parent;
+    return i;
+  }
+  /** islandSleeping.wgsl:119-128: link the higher root below the lower one. */
+  void sleepConnect(uint32_t a, uint32_t b) {
+    if (a >= n() || b >= n() || sleep[a].mass <= 0 || sleep[b].mass <= 0) return;
+    const uint32_t ra = sleepRoot(a), rb = sleepRoot(b);
+    if (ra != rb) sleep[std::max(ra, rb)].parent = std::min(ra, rb);
+  }
+  /** before(): both shaders' before(), then clear(). */
+  void sleepBefore(uint32_t force) {
+    for (uint32_t i = 0; i < n(); i++) {
+      Sleep &s = sleep[i];
+      const bool all = (force & 1u) || globalsOf(i)[0] != 0;
+      if (sleepIslands) {
+        if (all) {
+          wakeOne(i);
+          s.parent = i;
+        } else if (s.flags & (SF_HELD | SF_GATHERED)) {
+          const uint32_t held = s.flags & SF_HELD;
+          wakeOne(i);
+          s.flags |= held;
+        }
+        if (!s.asleep) s.parent = i;
+        if ((force & 2u) && motion(i) <= sleepSpeed && s.eligible) s.flags |= SF_START_QUIET;
+        s.wake = 0;
+      } else if (all || (s.flags & SF_HELD)) {
+        wakeOne(i);
+      }
+      s.support = 0;
+    }
+    std::fill(sleepGlobals.begin(), sleepGlobals.end(), 0u);
+  }
+  void sleepWakeAll() { sleepBefore(1u); }
+  /** wakeBody() + requestWake/gatherWake (islandSleeping.wgsl:23-30): hold bodies awake; in island mode their
+   * islands wake at the next step's start. */
+  void sleepHold(const uint32_t *indices, uint32_t count) {
+    for (uint32_t k = 0; k < count; k++) sleep[indices[k]].flags = SF_HELD;
+    if (!sleepIslands) return;
+    for (uint32_t i = 0; i < n(); i++)
+      if (sleep[i].asleep && (sleep[i].flags & SF_HELD)) sleep[sleepRoot(i)].wake |= SW_REQUEST;
+    for (uint32_t i = 0; i < n(); i++)
+      if (sleep[sleepRoot(i)].wake & SW_REQUEST) sleep[i].flags |= SF_GATHERED;
+  }
+  void signalWake(uint32_t i) {
+    uint32_t *g = globalsOf(i);
+    if (sleepIslands) {
+      g[3] = 1;
+      sleep[sleepRoot(i)].wake |= SW_IMPACT;
+    } else {
+      g[0] = 1;
+    }
+  }
+  void sleepImpactBodies(uint32_t a, uint32_t b) {
+    if (sleep[a].asleep && !sleep[b].asleep && motion(b) > R(2) * sleepSpeed) signalWake(a);
+  }
+  /** sleepContact.wgsl:14-30 contactVelocity(): a point's velocity now, or over the step just solved (from the
+   * step-start and inertial poses, less this step's gravity for a dynamic body). */
+  V3<R> sleepPointVelocity(uint32_t i, V3<R> r, bool previous) const {
+    if (sleep[i].asleep) return V3<R>(0, 0, 0);
+    const Body<R> &B = bodies[i];
+    V3<R> v = B.vel, w = B.angVel;
+    Q4<R> q = B.rot;
+    if (previous) {
+      const WorldParams &p = P(i);
+      const R dt = R(p.dt);
+      v = V3<R>((B.inertialPos.x - B.ipos.x) / dt, (B.inertialPos.y - B.ipos.y) / dt, (B.inertialPos.z - B.ipos.z) / dt);
+      if (B.mass > 0) v = sub(v, scale(V3<R>(R(p.up[0]), R(p.up[1]), R(p.up[2])), R(p.gravity) * dt));
+      const Q4<R> a = B.inertialRot, b = B.irot;
+      const V3<R> av(a.x, a.y, a.z), bv(b.x, b.y, b.z);
+      const V3<R> t = scale(sub(add(scale(bv, -a.w), scale(av, b.w)), cross(av, bv)), R(2));
+      const R d = (((b.x * b.x + b.y * b.y) + b.z * b.z) + b.w * b.w) * dt;
+      w = V3<R>(t.x / d, t.y / d, t.z / d);
+      q = b;
+    }
+    return add(v, cross(w, rotate(q, r)));
+  }
+  void sleepContactImpact(uint32_t a, uint32_t b, V3<R> nrm, V3<R> rA, V3<R> rB, bool beforeSolve) {
+    if (!sleep[a].asleep || sleep[b].asleep) return;
+    R incoming = 0;
+    if (!beforeSolve) incoming = dot(sub(sleepPointVelocity(b, rB, true), sleepPointVelocity(a, rA, true)), nrm);
+    const R outgoing = dot(sub(sleepPointVelocity(b, rB, false), sleepPointVelocity(a, rA, false)), nrm);
+    if (rmax(incoming, outgoing) > R(2) * sleepSpeed) signalWake(a);
+  }
+  /** sleepContact.wgsl:31-40 manifoldImpact(): each point, both ways, for a manifold between a sleeper and a waker. */
+  void sleepManifoldImpact(const Manifold<R> &m, bool beforeSolve) {
+    const uint32_t a = m.a, b = m.b;
+    if (sleep[a].asleep == sleep[b].asleep) return;
+    for (uint32_t k = 0; k < m.count(); k++) {
+      const Contact<R> &c = contacts[m.first + k];
+      sleepContactImpact(a, b, m.normal, c.rA, c.rB, beforeSolve);
+      sleepContactImpact(b, a, neg(m.normal), c.rB, c.rA, beforeSolve);
+    }
+  }
+  bool sleepLiveManifold(const Manifold<R> &m) const { return m.count() != 0 && (m.countw & 0x80000000u) == 0; }
+  bool sleepLiveJoint(const Joint<R> &j) const { return j.type != 0 && j.a >= 0; }
+  /** encodeImpact(): impacts on sleepers from this step's contacts and joints; the islands (or world) they hit wake.
+   * Returns whether a body woke (the pairs are then found again). */
+  bool sleepImpact() {
+    for (const Manifold<R> &m : manifolds)
+      if (sleepLiveManifold(m)) sleepManifoldImpact(m, true);
+    for (const Joint<R> &j : joints)
+      if (sleepLiveJoint(j)) {
+        sleepImpactBodies(uint32_t(j.a), uint32_t(j.b));
+        sleepImpactBodies(uint32_t(j.b), uint32_t(j.a));
+      }
+    uint32_t woken = 0;
+    for (uint32_t i = 0; i < n(); i++) {
+      uint32_t *g = globalsOf(i);
+      if (!sleep[i].asleep || (g[3] == 0 && g[0] == 0)) continue;
+      if (g[0] != 0 || (sleepIslands && (sleep[sleepRoot(i)].wake & SW_IMPACT))) {
+        wakeOne(i);
+        g[2]++;
+        woken++;
+      }
+    }
+    return woken != 0;
+  }
+  /** support(): a's contact below with a static or sleeping partner, or one that was quiet and supported and is still
+   * quiet, supports a. */
+  void sleepSupport(uint32_t a, uint32_t b) {
+    const Sleep &s = sleep[b];
+    if (s.mass <= 0 || s.asleep || ((s.flags & SF_READY) && motion(b) <= sleepSpeed)) sleep[a].support = 1;
+  }
+  /** after(): islands (connect, flattened to their roots), contacts, joints, activity (island mode), rest. */
+  void sleepAfter(uint32_t force) {
+    const uint32_t N = uint32_t(n());
+    if (sleepIslands) {
+      for (const Manifold<R> &m : manifolds)
+        if (sleepLiveManifold(m)) sleepConnect(m.a, m.b);
+      for (const Joint<R> &j : joints)
+        if (j.type != 0 && j.a >= 0 && j.b >= 0) sleepConnect(uint32_t(j.a), uint32_t(j.b));
+      for (uint32_t i = 0; i < N; i++) sleep[i].parent = sleepRoot(i);
+    }
+    for (const Manifold<R> &m : manifolds) {
+      if (!sleepLiveManifold(m)) continue;
+      const uint32_t a = m.a, b = m.b;
+      sleepManifoldImpact(m, false);
+      const WorldParams &p = P(a);
+      const R up = dot(m.normal, V3<R>(R(p.up[0]), R(p.up[1]), R(p.up[2])));
+      if (up > R(0.3f)) sleepSupport(a, b);
+      if (up < R(-0.3f)) sleepSupport(b, a);
+      if (sleepIslands && (force & 2u)) {
+        if (sleep[b].mass <= 0 && motion(b) <= sleepSpeed && up > R(0.3f) && sleep[a].mass > 0)
+          sleep[sleepRoot(a)].wake |= SW_GROUNDED;
+        if (sleep[a].mass <= 0 && motion(a) <= sleepSpeed && up < R(-0.3f) && sleep[b].mass > 0)
+          sleep[sleepRoot(b)].wake |= SW_GROUNDED;
+      }
+    }
+    for (const Joint<R> &j : joints)
+      if (sleepLiveJoint(j)) {
+        sleepImpactBodies(uint32_t(j.a), uint32_t(j.b));
+        sleepImpactBodies(uint32_t(j.b), uint32_t(j.a));
+      }
+    if (sleepIslands)
+      for (uint32_t i = 0; i < N; i++) {
+        const Sleep &s = sleep[i];
+        if (s.mass <= 0) continue;
+        const WorldParams &p = P(i);
+        bool busy = false;
+        if (force & 2u) {
+          busy = (s.flags & SF_START_QUIET) == 0;
+        } else if (!s.asleep) {
+          busy = motion(i) > sleepSpeed;
+          if (s.eligible && !(s.flags & SF_HELD))
+            busy = busy || s.quiet + R(p.dt) < sleepDelay || (p.gravity != 0.f && !s.support);
+        }
+        if (busy) sleep[sleepRoot(i)].wake |= SW_BUSY;
+      }
+    for (uint32_t i = 0; i < N; i++) {
+      Sleep &s = sleep[i];
+      if (s.mass <= 0) continue;
+      if (!s.eligible) {
+        wakeOne(i);
+        continue;
+      }
+      uint32_t *g = globalsOf(i);
+      const uint32_t islandWake = sleepIslands ? sleep[sleepRoot(i)].wake : 0u;
+      if ((islandWake & SW_IMPACT) || g[0] != 0 || clashes != 0) {
+        wakeOne(i);
+        continue;
+      }
+      const WorldParams &p = P(i);
+      const bool startQuiet = (s.flags & SF_START_QUIET) != 0, held = (s.flags & SF_HELD) != 0;
+      const bool ready = motion(i) <= sleepSpeed && (p.gravity == 0.f || s.support != 0);
+      s.flags = ready ? SF_READY : 0u;
+      if (!s.asleep) {
+        s.quiet = ready ? s.quiet + R(p.dt) : R(0);
+        const bool now = sleepIslands ? !held && ((s.quiet >= sleepDelay && !(islandWake & SW_BUSY)) ||
+                                                  ((force & 2u) && startQuiet && (islandWake & 6u) == SW_GROUNDED))
+                                      : s.quiet >= sleepDelay;
+        if (now) {
+          s.asleep = 1;
+          Body<R> &B = bodies[i];
+          B.mass = 0;
+          B.vel = V3<R>(0, 0, 0);
+          B.prevUp = 0;
+          B.angVel = V3<R>(0, 0, 0);
+        }
+      }
+      if (s.asleep) g[1]++;
+    }
+  }
+
   void step() {
     pivotFallbacks = 0;
+    const uint32_t force = sleepForce;
+    if (sleepOn) {
+      sleepBefore(force);
+      sleepForce = 0;
+    }
     const bool bounce = bouncing();
     if (bounce) restitutionBefore();
     collisionAndTopology();
     solve();
     if (bounce) restitutionAfter();
+    if (sleepOn) sleepAfter(force);
   }
 
   // --- edits between steps -----------------------------------------------------------------------------------------

@@ -77,6 +77,15 @@ def lib():
             "gs_set_threads": (None, [ctypes.c_int]),
             "gs_set_restitution": (None, [vp, u32p, f32p, ctypes.c_uint32]),
             "gs_get_threads": (ctypes.c_int, []),
+            "gs_enable_sleeping": (ctypes.c_int, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_uint8)]),
+            "gs_disable_sleeping": (None, [vp]),
+            "gs_wake_all": (None, [vp]),
+            "gs_wake_bodies": (None, [vp, u32p, ctypes.c_uint32]),
+            "gs_set_sleep_enabled": (None, [vp, u32p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32]),
+            "gs_sleep_stats": (None, [vp, ctypes.c_uint32, u32p]),
+            "gs_read_sleep": (None, [vp, u32p]),
+            "gs_set_sleep": (None, [vp, u32p, u32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -152,6 +161,7 @@ class OracleWorld:
         if brute_force:
             L.gs_set_brute_force(self.h, 1)
         self._restitution = None
+        self._sleeping = None
         bouncy, base = [], 0
         for sc in scenes:  # bodies built with a restitution coefficient, as the HIP world applies them
             bouncy += [(base + i, e) for i, e in sorted(getattr(sc, "_restitution", {}).items())]
@@ -190,6 +200,51 @@ class OracleWorld:
                        P(steps, cc))
         if getattr(self, "_restitution", None) is not None:
             self.set_restitution(*self._restitution)
+        if getattr(self, "_sleeping", None) is not None:
+            self.enable_sleeping(**self._sleeping)
+            if "sleep" in state:
+                self.set_sleep_state(state["sleep"], state["sleep_globals"])
+
+    # --- sleeping ---------------------------------------------------------------------------------------------------
+    def enable_sleeping(self, speed=0.03, delay=0.5, islands=False, start_asleep=False, eligible=None):
+        e = None if eligible is None else np.ascontiguousarray(eligible, np.uint8)
+        rc = lib().gs_enable_sleeping(self.h, float(speed), float(delay), int(bool(islands)), int(bool(start_asleep)),
+                                      P(e, ctypes.c_uint8))
+        if rc != 0:
+            raise ValueError("the oracle sleeps FP32 worlds only")
+        self._sleeping = dict(speed=speed, delay=delay, islands=islands, start_asleep=False, eligible=eligible)
+
+    def disable_sleeping(self):
+        lib().gs_disable_sleeping(self.h)
+        self._sleeping = None
+
+    def wake_all(self):
+        lib().gs_wake_all(self.h)
+
+    def wake_bodies(self, indices):
+        idx = np.ascontiguousarray(indices, np.uint32)
+        lib().gs_wake_bodies(self.h, P(idx, ctypes.c_uint32), len(idx))
+
+    def set_sleep_enabled(self, indices, flags):
+        idx = np.ascontiguousarray(indices, np.uint32)
+        f = np.ascontiguousarray(flags, np.uint8)
+        lib().gs_set_sleep_enabled(self.h, P(idx, ctypes.c_uint32), P(f, ctypes.c_uint8), len(idx))
+
+    def sleep_stats(self, k=0):
+        out = np.zeros(2, np.uint32)
+        lib().gs_sleep_stats(self.h, int(k), P(out, ctypes.c_uint32))
+        return dict(sleeping=int(out[0]), wakeRequested=bool(out[1]))
+
+    def sleep_state(self):
+        """(n, 8) u32: mass bits, quiet bits, asleep, support, eligible, parent, wake, flags."""
+        out = np.zeros((self.n, 8), np.uint32)
+        lib().gs_read_sleep(self.h, P(out, ctypes.c_uint32))
+        return out
+
+    def set_sleep_state(self, states, globals_):
+        st = np.ascontiguousarray(states, np.uint32).reshape(-1, 8)
+        g = np.ascontiguousarray(globals_, np.uint32)
+        lib().gs_set_sleep(self.h, P(st, ctypes.c_uint32), P(g, ctypes.c_uint32))
 
     def step(self, n=1):
         lib().gs_step(self.h, n)

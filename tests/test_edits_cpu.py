@@ -259,14 +259,19 @@ def test_invalid_edits_queue_nothing(avbd, op, message):
 
 
 def test_unsupported_properties_are_refused(avbd):
-    """Trigger bodies and sleeping are not built: their edits are refused; a restitution outside [0, 1] too."""
+    """Trigger bodies are not built: their edits are refused; a restitution outside [0, 1] and a non-boolean
+    allowSleep too (runtime.js:83-84); an allowSleep edit becomes setSleepEnabled."""
     st = _Staging(avbd)
     with pytest.raises(NotImplementedError):
         st.world.editBodies([{"body": 1, "isTrigger": True}])
-    with pytest.raises(NotImplementedError):
-        st.world.editBodies([{"body": 1, "allowSleep": True}])
     with pytest.raises(ValueError):
         st.world.editBodies([{"body": 1, "restitution": 1.5}])
+    with pytest.raises(ValueError, match="allowSleep must be a boolean"):
+        st.world.editBodies([{"body": 1, "allowSleep": 1}])
+    calls = []
+    st.world.setSleepEnabled = lambda i, e: calls.append((i, e))
+    st.world.editBodies([{"body": 3, "allowSleep": False}])
+    assert calls == [(3, False)]
 
 
 NODE = shutil.which("node")
