@@ -461,6 +461,97 @@ napi_value GetRestitution(napi_env env, napi_callback_info info) {
   return out;
 }
 
+napi_value EnableSleeping(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!args(env, info, 5, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  double speed, delay;
+  int32_t islands;
+  bool start;
+  if (!w) return nullptr;
+  if (napi_get_value_double(env, argv[1], &speed) != napi_ok || napi_get_value_double(env, argv[2], &delay) != napi_ok ||
+      napi_get_value_int32(env, argv[3], &islands) != napi_ok || napi_get_value_bool(env, argv[4], &start) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "expected speed, delay, islands and startAsleep");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_enable_sleeping(w, float(speed), float(delay), islands, start ? 1 : 0));
+  return nullptr;
+}
+
+template <int (*F)(avbd_world *)>
+napi_value WorldCall(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  AVBD_OK_OR_THROW(F(w));
+  return nullptr;
+}
+
+/** {enabled, islands} of the world's sleeping. */
+napi_value SleepMode(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  if (!w) return nullptr;
+  int islands = 0;
+  const int on = avbd_world_sleeping(w, &islands);
+  napi_value out, a, b;
+  napi_create_object(env, &out);
+  napi_get_boolean(env, on == 1, &a);
+  napi_get_boolean(env, on == 1 && islands == 1, &b);
+  napi_set_named_property(env, out, "enabled", a);
+  napi_set_named_property(env, out, "islands", b);
+  return out;
+}
+
+napi_value WakeBodies(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx;
+  size_t n;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false)) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_wake_bodies(w, static_cast<const uint32_t *>(idx), uint32_t(n)));
+  return nullptr;
+}
+
+napi_value SetSleepEnabled(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx, *flags;
+  size_t n, nf;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false) ||
+      !typed(env, argv[2], napi_uint8_array, &flags, &nf, false))
+    return nullptr;
+  if (nf != n) {
+    napi_throw_range_error(env, nullptr, "Each body index needs a sleep flag");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_set_sleep_enabled(w, static_cast<const uint32_t *>(idx),
+                                                static_cast<const uint8_t *>(flags), uint32_t(n)));
+  return nullptr;
+}
+
+/** {sleeping, wakeRequested} of one world of the batch after the last step. */
+napi_value SleepStats(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t k;
+  if (!w || !u32(env, argv[1], &k)) return nullptr;
+  uint32_t s[2];
+  AVBD_OK_OR_THROW(avbd_world_sleep_stats(w, k, s));
+  napi_value out, a, b;
+  napi_create_object(env, &out);
+  napi_create_uint32(env, s[0], &a);
+  napi_get_boolean(env, s[1] != 0, &b);
+  napi_set_named_property(env, out, "sleeping", a);
+  napi_set_named_property(env, out, "wakeRequested", b);
+  return out;
+}
+
 napi_value GetParams(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
@@ -552,6 +643,14 @@ napi_value Init(napi_env env, napi_value exports) {
       {"setFilters", nullptr, SetFilters, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setRestitution", nullptr, SetRestitution, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getRestitution", nullptr, GetRestitution, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"enableSleeping", nullptr, EnableSleeping, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"disableSleeping", nullptr, WorldCall<avbd_world_disable_sleeping>, nullptr, nullptr, nullptr, napi_enumerable,
+       nullptr},
+      {"wakeAll", nullptr, WorldCall<avbd_world_wake_all>, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"sleepMode", nullptr, SleepMode, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"wakeBodies", nullptr, WakeBodies, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setSleepEnabled", nullptr, SetSleepEnabled, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"sleepStats", nullptr, SleepStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };

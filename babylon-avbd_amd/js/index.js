@@ -110,6 +110,8 @@ class AvbdScene3D {
     this.records = [];
     this.filters = new Map();
     this.restitution = new Map();  // body -> restitution coefficient above 0
+    this.noSleep = new Set();      // bodies built with allowSleep false
+    this.sleepOptIn = false;       // a body was built with allowSleep true: its world sleeps (scenes.js:233-240, 689)
     this.joints = [];
     this.ignore = [];
     this.created = false;
@@ -152,7 +154,10 @@ class AvbdScene3D {
     const mask = o.collidesWith === undefined ? 0xffffffff : o.collidesWith;
     for (const m of [group, mask])
       if (!Number.isInteger(m) || m < 0 || m > 0xffffffff) throw Error('Collision masks must be unsigned 32-bit integers');
+    if (o.allowSleep !== undefined && typeof o.allowSleep !== 'boolean') throw Error('allowSleep must be a boolean');
     const index = this.records.length;
+    if (o.allowSleep === false) this.noSleep.add(index);
+    this.sleepOptIn = this.sleepOptIn || o.allowSleep === true;
     if (group !== 0xffffffff || mask !== 0xffffffff) this.filters.set(index, [group, mask]);
     if (restitution) this.restitution.set(index, restitution);
     this.records.push(rec);
@@ -370,9 +375,8 @@ function solverCode(solver) {
 
 /** A world (or a batch of worlds) on the GPU with the shape of AvbdGpuSolver3D (scenes.d.ts:317-454). */
 /**
- * A body handle (runtime.js:168-237 NativeBody): chainable edits of one body slot, and readState(). Triggers and
- * sleeping are not built on this backend: turning them on throws, turning them off (their only state here) is
- * accepted.
+ * A body handle (runtime.js:168-237 NativeBody): chainable edits of one body slot, and readState(). Triggers are not
+ * built on this backend: turning one on throws, turning it off is accepted.
  */
 class HipBody {
   constructor(solver, index) {
@@ -397,12 +401,8 @@ class HipBody {
     this.solver.setFilters([this.gpuIndex], [group], [collidesWith === undefined ? 0xffffffff : collidesWith]);
     return this;
   }
-  setSleepEnabled(enabled) {
-    this.gpuIndex;
-    this.solver.sleepAllowed.set(this.index, !!enabled);  // recorded; this backend does not put bodies to sleep
-    return this;
-  }
-  wakeUp() { this.gpuIndex; return this; }
+  setSleepEnabled(enabled) { this.solver.setSleepEnabled(this.gpuIndex, enabled); return this; }
+  wakeUp() { this.solver.wakeBodies([this.gpuIndex]); return this; }
   readState() { return this.solver.readBodyState(this.gpuIndex); }
 }
 
@@ -433,7 +433,6 @@ class HipSolver3D {
     }
     this.commands = new CommandQueue();
     this.handles = new Map();
-    this.sleepAllowed = new Map();
     this.sceneBinding = null;
     this.scenes = list;
     // Bodies built with a restitution coefficient (scenes.js:684-687 applies them after creation the same way)
@@ -443,6 +442,69 @@ class HipSolver3D {
         bouncy.push([this.firstBodies[k] + i, e]);
     });
     if (bouncy.length) this.setRestitution(bouncy.map((b) => b[0]), bouncy.map((b) => b[1]));
+    this.masses = new Float32Array(this.bodyCount);
+    list.forEach((sc, k) => sc.records.forEach((r, i) => { this.masses[this.firstBodies[k] + i] = r[19]; }));
+    // Sleeping (scenes.js:689-693): asked for in the options, or a body was built with allowSleep true
+    const noSleep = [];
+    list.forEach((sc, k) => { for (const i of Array.from(sc.noSleep).sort((x, y) => x - y)) noSleep.push(this.firstBodies[k] + i); });
+    if (noSleep.length) native().setSleepEnabled(this.handle, Uint32Array.from(noSleep), new Uint8Array(noSleep.length));
+    const sleeping = (options || {}).sleeping;
+    if (sleeping || list.some((s) => s.sleepOptIn))
+      this.enableSleeping(sleeping && typeof sleeping === 'object' ? sleeping : {});
+  }
+
+  // --- sleeping (worldSleeping.js; solver3D.js:317-343; appGpuSolver3D.js:869-889) ---------------------------------
+
+  /** enableSleeping({speedThreshold, timeThreshold, islands, startAsleep}): bodies quiet and supported for
+   * timeThreshold seconds sleep as fixed supports until an impact, an edit or a request wakes them; islands (default:
+   * worlds of 4096 bodies or more) wakes and sleeps connected groups together; startAsleep (island mode) takes one
+   * step that puts still, grounded groups to sleep at once. */
+  enableSleeping(options) {
+    this.alive();
+    const o = options || {};
+    for (const key of Object.keys(o))
+      if (!['speedThreshold', 'timeThreshold', 'islands', 'startAsleep'].includes(key))
+        throw Error(`Unknown sleep option: ${key}`);
+    const speed = o.speedThreshold === undefined ? 0.03 : o.speedThreshold;
+    const delay = o.timeThreshold === undefined ? 0.5 : o.timeThreshold;
+    if (!Number.isFinite(speed) || speed <= 0 || !Number.isFinite(delay) || delay <= 0)
+      throw Error('Sleep speedThreshold and timeThreshold must be positive and finite');
+    if (o.startAsleep !== undefined && typeof o.startAsleep !== 'boolean') throw Error('startAsleep must be a boolean');
+    if (o.islands !== undefined && typeof o.islands !== 'boolean') throw Error('Sleep islands must be a boolean');
+    if (this.sleeping) return this;
+    this.flushEdits();
+    native().enableSleeping(this.handle, speed, delay, o.islands === undefined ? -1 : Number(o.islands),
+                            !!o.startAsleep);
+    if (o.startAsleep && this.sleepIslands) this.step();
+    return this;
+  }
+  get sleeping() { return !this.destroyed && native().sleepMode(this.handle).enabled; }
+  get sleepIslands() { return !this.destroyed && native().sleepMode(this.handle).islands; }
+  /** Every body wakes with its mass back (poses kept) and sleeping stops. */
+  disableSleeping() { this.alive(); this.flushEdits(); native().disableSleeping(this.handle); return this; }
+  wakeAll() { this.alive(); this.flushEdits(); native().wakeAll(this.handle); return this; }
+  /** wakeBody for each index: held awake through the next step; in island mode their islands wake. */
+  wakeBodies(indices) {
+    this.alive();
+    const idx = Uint32Array.from(indices, (i) => this.liveIndex(i));
+    this.flushEdits();
+    native().wakeBodies(this.handle, idx);
+    return this;
+  }
+  /** Sleep eligibility of one body; allowing it enables sleeping; every body wakes. */
+  setSleepEnabled(index, enabled) {
+    this.alive();
+    if (typeof enabled !== 'boolean') throw Error('Sleep flag must be boolean');
+    const idx = Uint32Array.of(this.liveIndex(index));
+    this.flushEdits();
+    native().setSleepEnabled(this.handle, idx, Uint8Array.of(enabled ? 1 : 0));
+    return this;
+  }
+  /** {sleeping, wakeRequested} of one world after the last step. */
+  readSleepStats(world = 0) {
+    this.alive();
+    this.flushEdits();
+    return Promise.resolve(native().sleepStats(this.handle, world));
   }
 
   alive() { if (this.destroyed) throw Error('The solver has been destroyed'); }
@@ -466,9 +528,12 @@ class HipSolver3D {
 
   /** readBodyState: position, rotation, linear and angular velocity of one body (after queued edits). */
   async readBodyState(index) {
-    const r = await this.readSelectedBodies([this.liveIndex(index)]);
+    const i = this.liveIndex(index);
+    const r = await this.readSelectedBodies([i]);
+    const mass = this.masses[i];
     return { position: Array.from(r.subarray(0, 3)), rotation: Array.from(r.subarray(4, 8)),
-             linearVelocity: Array.from(r.subarray(32, 35)), angularVelocity: Array.from(r.subarray(36, 39)) };
+             linearVelocity: Array.from(r.subarray(32, 35)), angularVelocity: Array.from(r.subarray(36, 39)),
+             mass, effectiveMass: r[19], sleeping: mass > 0 && r[19] === 0 };
   }
 
   /** setVelocities(indices, {linear, angular}) (runtime.js:356-378): xyz triples per index, either or both. */
@@ -498,9 +563,6 @@ class HipSolver3D {
   setLinearVelocities(indices, values) { return this.setVelocities(indices, { linear: values }); }
   setAngularVelocities(indices, values) { return this.setVelocities(indices, { angular: values }); }
 
-  /** Sleeping is not built on this backend: bodies never sleep, which is what disabling it asks for. */
-  disableSleeping() { this.alive(); return this; }
-  wakeAll() { this.alive(); return this; }
 
   /**
    * attachToScene(scene, {afterStep, maxSubSteps, maxFrameTime}) (runtime.js:275-311): step from the scene's
@@ -621,14 +683,16 @@ class HipSolver3D {
     const properties = [];
     for (const edit of edits) {
       const index = this.bodyIndex(edit.body);
-      for (const key of ['isTrigger', 'allowSleep'])
-        if (edit[key] !== undefined) throw Error(`${key} edits are not available on the hip backend`);
+      if (edit.isTrigger !== undefined) throw Error('isTrigger edits are not available on the hip backend');
       const restitution = edit.restitution === undefined ? undefined : coefficient(edit.restitution);
-      if (edit.group !== undefined || edit.collidesWith !== undefined || restitution !== undefined) {
+      if (edit.allowSleep !== undefined && typeof edit.allowSleep !== 'boolean')
+        throw Error('allowSleep must be a boolean');
+      if (edit.group !== undefined || edit.collidesWith !== undefined || restitution !== undefined ||
+          edit.allowSleep !== undefined) {
         if (edit.collidesWith !== undefined && edit.group === undefined)
           throw Error('A collision-mask edit requires group as well');
         properties.push([index, edit.group === undefined ? undefined : mask32(edit.group),
-          edit.collidesWith === undefined ? 0xffffffff : mask32(edit.collidesWith), restitution]);
+          edit.collidesWith === undefined ? 0xffffffff : mask32(edit.collidesWith), restitution, edit.allowSleep]);
       }
       const point = edit.worldPoint === undefined ? undefined : components(edit.worldPoint, 3, 'worldPoint');
       if (edit.position !== undefined) {
@@ -646,9 +710,10 @@ class HipSolver3D {
       }
     }
     this.commands.extend(staged);
-    for (const [index, group, collides, restitution] of properties) {
+    for (const [index, group, collides, restitution, allow] of properties) {
       if (group !== undefined) this.setFilters([index], [group], [collides]);
       if (restitution !== undefined) this.setRestitution(index, restitution);
+      if (allow !== undefined) this.setSleepEnabled(index, allow);
     }
     return this;
   }
@@ -736,7 +801,12 @@ class HipSolver3D {
   readJoints() { this.ready(); return Promise.resolve(native().readJoints(this.handle, 0, this.jointCount)); }
 
   /** {pairs, contacts, manifolds, joints, overflow, clashes, colors, sleeping} (world.js:672-692). */
-  readCounters() { this.alive(); return Promise.resolve(native().readCounters(this.handle)); }
+  readCounters() {
+    this.alive();
+    const c = native().readCounters(this.handle);
+    if (this.sleeping) c.sleeping = native().sleepStats(this.handle, 0).sleeping;
+    return Promise.resolve(c);
+  }
 
   /** The last step's broadphase pairs (a > b) as [a0, b0, a1, b1, ...] (solver.ts:1471-1475). */
   readPairs() { this.alive(); return Promise.resolve(native().readPairs(this.handle)); }

@@ -237,6 +237,36 @@ def jointed_mix(avbd, steps, warmup):
             "step_ms_p99": float(np.percentile(ms, 99)), "counters": counters, "phases_ms": prof}
 
 
+def sleeping_pile(avbd, steps, warmup):
+    """The 100k-box pile with sleeping on (island mode, the reference's default thresholds 0.03 and 0.5 s), timed over
+    the same step range as the headline run: the step cost with the sleep passes, and how many bodies rest by then."""
+    scene = avbd.configs.large_world()
+    n = scene.body_count
+    world = avbd.createSolverPipeline("hip").createSolver(scene, {"colorCap": 16, "colorRounds": 16, "graph": True,
+                                                                  "adaptive": True})
+    world.enableSleeping({"islands": True})
+    done = 0
+    while done < warmup:
+        k = min(100, warmup - done)
+        world.step(k)
+        world.synchronize()
+        done += k
+    asleep_before = world.readSleepStats()["sleeping"]
+    ms = []
+    t0 = time.perf_counter()
+    while len(ms) < steps:
+        ms.extend(world.stepTimed(min(50, steps - len(ms))).tolist())
+    elapsed = time.perf_counter() - t0
+    asleep_after = world.readSleepStats()["sleeping"]
+    print(f"# sleeping pile: {asleep_before} -> {asleep_after} asleep, {elapsed * 1e3 / steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    world.destroy()
+    return {"workload": "the 100k-box pile with sleeping on (islands, speed 0.03, 0.5 s)", "bodies": n,
+            "steps_timed": f"{warmup + 1}-{warmup + steps}", "body_steps_per_s": n * steps / elapsed,
+            "ms_per_step": elapsed * 1e3 / steps, "step_ms_p50": float(np.percentile(ms, 50)),
+            "asleep_at_start": asleep_before, "asleep_at_end": asleep_after}
+
+
 def serving(avbd, requests, rate, pool, max_active):
     """The serving workload: Poisson arrivals (`rate` per second) of rollout requests (U{1..120} steps) for a pool of
     worlds of 16..50,000 boxes (log-uniform sizes, the single large world's body distributions), admitted by the
@@ -293,6 +323,7 @@ def main():
                     help="the large world's solve: one launch per colour, or every iteration in one dataflow launch")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--no-sleeping", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -398,6 +429,7 @@ def main():
 
     world.destroy()
     mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
+    sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, args.warmup)
     served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
                                                   args.serve_active)
     batched = None
@@ -440,6 +472,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "jointed_mix": mix,
+        "sleeping_pile": sleeping,
         "serving": served,
     }
     print(json.dumps(line))

@@ -1,9 +1,9 @@
 'use strict';
 // The reference's Babylon-free world-API contracts (tests/backend-contract.js checkBackend and checkFloor,
 // tests/packed-poses-contract.js packedPoseContract), restated for the "hip" backend of babylon-avbd_amd/js/index.js:
-// the same calls and the same expectations, in 3D, and the restitution part of checkContacts. What this backend does not
-// build (trigger bodies, sleeping, ray queries) is checked to be refused with an error, or, for switching it off,
-// accepted.
+// the same calls and the same expectations, in 3D, the restitution part of checkContacts, and checkSleeping. What this
+// backend does not build (trigger bodies, ray queries) is checked to be refused with an error, or, for switching it
+// off, accepted.
 // Prints one JSON object with a record per contract; exits non-zero on the first failure.
 const path = require('path');
 const avbd = require(path.join(__dirname, '..', '..', 'babylon-avbd_amd', 'js', 'index.js'));
@@ -138,6 +138,29 @@ async function checkRestitution() {
   }
 }
 
+/** checkSleeping (backend-contract.js:441-467): a box with allowSleep true resting on a floor sleeps within 180 steps;
+ * an impulse (1, 2, 0) wakes it and it moves up faster than 1 m/s after one step. */
+async function checkSleeping() {
+  const pipeline = await avbd.createSolverPipeline({ backend: 'hip' });
+  const scene = new avbd.AvbdScene3D();
+  scene.addBox([10, 1, 10], { density: 0, position: [0, -0.5, 0] });
+  const box = scene.addBox([1, 1, 1], { position: [0, 0.5, 0], allowSleep: true });
+  const world = scene.createSolver(pipeline, { spatialSort: false });
+  try {
+    for (let i = 0; i < 180; i++) world.step();
+    const body = world.body(box);
+    const state = await body.readState();
+    check(state.sleeping, 'Supported body did not sleep');
+    body.applyImpulse([1, 2, 0]);
+    world.step();
+    const awake = await body.readState();
+    check(!awake.sleeping && awake.linearVelocity[1] > 1, 'Impulse did not wake and move the body');
+    return { backend: 'hip', dimension: 3, sleeping: true, waking: true, passed: true };
+  } finally {
+    pipeline.destroy();
+  }
+}
+
 async function packedPoseContract(g, ids) {
   const expect = check;
   const near = (actual, expected, message) => check(Math.abs(actual - expected) < 1e-5, message);
@@ -203,6 +226,7 @@ async function checkPackedPoses() {
   out.backend = await checkBackend();
   out.floor = await checkFloor();
   out.restitution = await checkRestitution();
+  out.sleeping = await checkSleeping();
   out.packedPoses = await checkPackedPoses();
   process.stdout.write(JSON.stringify(out) + '\n');
 })().catch((e) => {

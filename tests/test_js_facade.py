@@ -167,5 +167,6 @@ def test_reference_api_contracts_hold_on_the_hip_backend():
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
     assert out["backend"]["passed"] and out["floor"]["passed"] and out["restitution"]["reboundSpeed"] > 2
+    assert out["sleeping"]["sleeping"] and out["sleeping"]["waking"]
     assert 0.35 < out["floor"]["floorHeight"] < 0.7
     assert all(out["packedPoses"].values())
