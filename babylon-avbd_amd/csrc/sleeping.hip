@@ -55,7 +55,7 @@ AVBD_D uint32_t rootOf(const DeviceWorld &w, uint32_t i) {
     if (p == i) return i;
     if (p > i) break;
     const uint32_t gp = ldRelaxed(&w.sleep[p].parent);
-    atomicMin(&w.sleep[i].parent, gp);
+    if (gp != p) atomicMin(&w.sleep[i].parent, gp);  // p is not a root: halve (a no-op write is skipped)
     i = gp;
   }
   stRelaxed(globalsOf(w, start), 1u);
@@ -74,13 +74,29 @@ AVBD_D void connect(const DeviceWorld &w, uint32_t a, uint32_t b) {
   stRelaxed(globalsOf(w, a), 1u);
 }
 
+/** Or bits into an island root's wake word; skipped when they are already set (one island's members would otherwise
+ * all hit the same word). */
+AVBD_D void orWake(const DeviceWorld &w, uint32_t r, uint32_t bits) {
+  if ((ldRelaxed(&w.sleep[r].wake) & bits) != bits) atomicOr(&w.sleep[r].wake, bits);
+}
+
+/** orWake for the lanes with `want` set, called by every lane of the wave that entered: the lanes sharing the first
+ * wanting lane's root are covered by its one or. */
+AVBD_D void orWakeWave(const DeviceWorld &w, uint32_t r, uint32_t bits, bool want) {
+  const uint64_t lanes = __ballot(want);
+  if (!lanes) return;
+  const int leader = __ffsll(static_cast<unsigned long long>(lanes)) - 1;
+  const uint32_t r0 = __shfl(r, leader);
+  if (want && (r != r0 || int(__lane_id()) == leader)) orWake(w, r, bits);
+}
+
 AVBD_D void signalWake(const DeviceWorld &w, const SleepParams &p, uint32_t i) {
   uint32_t *g = globalsOf(w, i);
   if (p.islands) {
-    stRelaxed(g + 3, 1u);
+    if (!ldRelaxed(g + 3)) stRelaxed(g + 3, 1u);
     const uint32_t r = rootOf(w, i);
-    if (r != NO_ROOT) atomicOr(&w.sleep[r].wake, SW_IMPACT);
-  } else {
+    if (r != NO_ROOT) orWake(w, r, SW_IMPACT);
+  } else if (!ldRelaxed(g)) {
     stRelaxed(g, 1u);
   }
 }
@@ -181,7 +197,7 @@ __global__ void k_sleep_request(DeviceWorld w) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nBodies || !w.sleep[i].asleep || !(w.sleep[i].flags & SF_HELD)) return;
   const uint32_t r = rootOf(w, i);
-  if (r != NO_ROOT) atomicOr(&w.sleep[r].wake, SW_REQUEST);
+  if (r != NO_ROOT) orWake(w, r, SW_REQUEST);
 }
 
 /** gatherWake: the members of a requesting island are marked to wake. */
@@ -281,6 +297,58 @@ __global__ void k_sleep_refresh_prepare(DeviceWorld w) {
   }
 }
 
+/** Before the unions: each awake dynamic body (a root after the reset) hooks below its smallest dynamic partner in its
+ * adjacency list (contacts and active joints), one store to its own parent; with the pointer jumping after it most of
+ * the unions then find their two roots equal. Links go to lower indices only, so the islands and their roots are those
+ * the unions alone would give. */
+__global__ void k_sleep_hook(DeviceWorld w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies || w.sleep[i].mass <= 0.f || w.sleep[i].parent != i) return;
+  uint32_t best = i;
+  const uint32_t end = umin(w.adjStart[i + 1], w.adjCapacity);
+  for (uint32_t e = w.adjStart[i]; e < end; e++) {
+    const uint32_t key = w.adjList[e];
+    uint32_t j;
+    if (key < w.nJoints) {
+      const int4 f = w.info[key];
+      if (f.x == 0 || f.y < 0 || f.z < 0) continue;
+      j = uint32_t(f.y) == i ? uint32_t(f.z) : uint32_t(f.y);
+    } else {
+      const Manifold m = w.manifolds[key - w.nJoints];
+      if (!liveManifold(m)) continue;
+      j = m.ids.x == i ? m.ids.y : m.ids.x;
+    }
+    if (j < best && w.sleep[j].mass > 0.f) best = j;
+  }
+  if (best != i) stRelaxed(&w.sleep[i].parent, best);
+}
+
+/** A linking pass before the unions: for each contact or joint whose ends have different roots, the higher root's
+ * parent drops to the lower root (an atomic min, no retry: a link it overrides is made again by the unions, which see
+ * every contact and joint). */
+__global__ void k_sleep_link(DeviceWorld w) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a, b;
+  if (t < w.pairCapacity) {
+    if (t >= manifoldCount(w)) return;
+    const Manifold m = w.manifolds[t];
+    if (!liveManifold(m)) return;
+    a = m.ids.x;
+    b = m.ids.y;
+  } else {
+    const uint32_t j = t - w.pairCapacity;
+    if (j >= w.nJoints) return;
+    const int4 f = w.info[j];
+    if (f.x == 0 || f.y < 0 || f.z < 0) return;
+    a = uint32_t(f.y);
+    b = uint32_t(f.z);
+  }
+  if (a >= w.nBodies || b >= w.nBodies || w.sleep[a].mass <= 0.f || w.sleep[b].mass <= 0.f) return;
+  const uint32_t ra = rootOf(w, a), rb = rootOf(w, b);
+  if (ra == rb || ra == NO_ROOT || rb == NO_ROOT) return;
+  atomicMin(&w.sleep[umax(ra, rb)].parent, umin(ra, rb));
+}
+
 /** Island links of this step's contacts and joints (island mode). Items: manifold slots, then joints. */
 __global__ void k_sleep_connect(DeviceWorld w) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -321,11 +389,11 @@ __global__ void k_sleep_contacts(DeviceWorld w, SleepParams p) {
       const SleepState &sa = w.sleep[a], &sb = w.sleep[b];
       if (sb.mass <= 0.f && motion(w.bodies[b]) <= p.speed && up > 0.3f && sa.mass > 0.f) {
         const uint32_t r = rootOf(w, a);
-        if (r != NO_ROOT) atomicOr(&w.sleep[r].wake, SW_GROUNDED);
+        if (r != NO_ROOT) orWake(w, r, SW_GROUNDED);
       }
       if (sa.mass <= 0.f && motion(w.bodies[a]) <= p.speed && up < -0.3f && sb.mass > 0.f) {
         const uint32_t r = rootOf(w, b);
-        if (r != NO_ROOT) atomicOr(&w.sleep[r].wake, SW_GROUNDED);
+        if (r != NO_ROOT) orWake(w, r, SW_GROUNDED);
       }
     }
     return;
@@ -354,10 +422,8 @@ __global__ void k_sleep_activity(DeviceWorld w, SleepParams p) {
     if (s.eligible && !(s.flags & SF_HELD))
       busy = busy || s.quiet + wp.dt < p.delay || (wp.gravity != 0.f && !ldRelaxed(&s.support));
   }
-  if (busy) {
-    const uint32_t r = rootOf(w, i);
-    if (r != NO_ROOT) atomicOr(&w.sleep[r].wake, SW_BUSY);
-  }
+  const uint32_t r = busy ? rootOf(w, i) : NO_ROOT;
+  orWakeWave(w, r, SW_BUSY, r != NO_ROOT);
 }
 
 /** rest: the quiet timer, falling asleep (mass and velocities to 0), waking on impacts, overflow or clashes, and the
@@ -437,6 +503,10 @@ void launch_sleep_refresh_prepare(const DeviceWorld &w, hipStream_t s) {
 void launch_sleep_after(const DeviceWorld &w, const SleepParams &p, hipStream_t s) {
   const uint32_t items = blocks(uint64_t(w.pairCapacity) + w.nJoints, 256);
   if (p.islands) {
+    k_sleep_hook<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+    k_sleep_flatten<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
+    k_sleep_link<<<items, 256, 0, s>>>(w);
+    k_sleep_flatten<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
     k_sleep_connect<<<items, 256, 0, s>>>(w);
     k_sleep_flatten<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
   }

@@ -237,13 +237,13 @@ def jointed_mix(avbd, steps, warmup):
             "step_ms_p99": float(np.percentile(ms, 99)), "counters": counters, "phases_ms": prof}
 
 
-def sleeping_pile(avbd, steps, warmup):
+def sleeping_pile(avbd, steps, warmup, solver, graph=True):
     """The 100k-box pile with sleeping on (island mode, the reference's default thresholds 0.03 and 0.5 s), timed over
     the same step range as the headline run: the step cost with the sleep passes, and how many bodies rest by then."""
     scene = avbd.configs.large_world()
     n = scene.body_count
-    world = avbd.createSolverPipeline("hip").createSolver(scene, {"colorCap": 16, "colorRounds": 16, "graph": True,
-                                                                  "adaptive": True})
+    world = avbd.createSolverPipeline("hip").createSolver(scene, {"colorCap": 16, "colorRounds": 16, "graph": graph,
+                                                                  "adaptive": True, "solver": solver})
     world.enableSleeping({"islands": True})
     done = 0
     while done < warmup:
@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--no-sleeping", action="store_true")
+    ap.add_argument("--sleeping-only", action="store_true",
+                    help="run only the sleeping-pile record (with --no-graph: without graph replay, for profilers)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -349,6 +351,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the AVBD step here is single-GPU; "
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.sleeping_only:
+        print(json.dumps(sleeping_pile(importlib.import_module("babylon-avbd_amd"), args.steps, args.warmup,
+                                       args.solver, not args.no_graph)))
+        return
 
     import torch
 
@@ -429,7 +435,7 @@ def main():
 
     world.destroy()
     mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
-    sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, args.warmup)
+    sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, args.warmup, args.solver)
     served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
                                                   args.serve_active)
     batched = None
