@@ -63,6 +63,12 @@ class Options(ctypes.Structure):
     ]
 
 
+class QueryOptions(ctypes.Structure):
+    """AvbdQueryOptions (include/avbd.h)."""
+    _fields_ = [("ignore", ctypes.POINTER(ctypes.c_uint32)), ("ignore_count", ctypes.c_uint32),
+                ("collides_with", ctypes.c_uint32), ("membership", ctypes.c_uint32), ("use_membership", ctypes.c_int)]
+
+
 class Counters(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in
                 ("pairs", "contacts", "manifolds", "overflow", "clashes", "colors", "joints", "bodies",
@@ -158,6 +164,7 @@ SIGNATURES = {
                                                     ctypes.c_uint32]),
     "avbd_world_sleep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, u32p]),
     "avbd_world_read_sleep": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p]),
+    "avbd_world_cast": (ctypes.c_int, [ctypes.c_void_p, f32p, ctypes.c_uint32, ctypes.c_void_p, f32p]),
     "avbd_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "avbd_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "avbd_server_add_world": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), u32p]),

@@ -151,6 +151,12 @@ class Engine {
   /** Every body's sleep state, 8 words each (mass and quiet as f32 bits, asleep, support, eligible, parent, wake,
    * flags), and the 4 words per world of sleep globals. */
   void readSleep(uint32_t *states, uint32_t *globals);
+  /**
+   * Shape queries against the current poses (queued edits first, in stream order): n queries of 8 floats (origin xyz,
+   * maximum distance, direction xyz (normalised here), sweep radius); hits of 8 floats (normal xyz, distance or -1,
+   * body index bits (~0: none), 0, 0, 0).
+   */
+  void cast(const float *queries, uint32_t n, const QueryFilter &filter, float *hits);
   /** Solver iterations per step (all worlds of the batch). */
   void setIterations(uint32_t n);
 
@@ -206,6 +212,10 @@ class Engine {
   float sleepSpeed_ = 0.03f, sleepDelay_ = 0.5f;
   uint32_t sleepForce_ = 0;           // force bits of the next step (2: start asleep)
   std::vector<uint8_t> sleepEligible_;  // host copy of the per-body eligibility
+  QueryRec *queryDev_ = nullptr;        // shape query batches (allocated on first use, grown as needed)
+  uint32_t *queryBest_ = nullptr;
+  float4 *queryHits_ = nullptr;
+  uint32_t queryCap_ = 0;
   bool listedRounds_ = false;  // colouring rounds over the pending list in one block (few pending bodies lately)
   uint32_t coloringRounds_ = 0;
   uint32_t listedVotes_ = 0;

@@ -353,6 +353,9 @@ void Engine::allocateContacts(uint32_t pairCap) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   resetGraphs();
+  if (queryDev_) (void)hipFree(queryDev_);
+  if (queryBest_) (void)hipFree(queryBest_);
+  if (queryHits_) (void)hipFree(queryHits_);
   for (auto &ev : counterEvents_)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : timingEvents_) (void)hipEventDestroy(ev);
@@ -1019,6 +1022,43 @@ void Engine::sleepStats(uint32_t world, uint32_t *out) {
   synchronize();
   out[0] = g[1];
   out[1] = g[0];
+}
+
+void Engine::cast(const float *queries, uint32_t n, const QueryFilter &filter, float *hits) {
+  if (n == 0) return;
+  if (!queries || !hits) throw std::invalid_argument("queries or hits is NULL");
+  if (n > 65535) throw std::invalid_argument("Too many queries in one batch");
+  if (filter.ignoreCount > 16) throw std::invalid_argument("Queries can ignore up to 16 live body indices");
+  for (uint32_t k = 0; k < filter.ignoreCount; k++)
+    if (filter.ignore[k] >= nBodies_) throw std::invalid_argument("Queries can ignore up to 16 live body indices");
+  std::vector<QueryRec> recs(n);
+  for (uint32_t k = 0; k < n; k++) {
+    const float *q = queries + 8ull * k;
+    bool ok = true;
+    for (int c = 0; c < 8; c++) ok = ok && std::isfinite(q[c]);
+    const double len = std::sqrt(double(q[4]) * q[4] + double(q[5]) * q[5] + double(q[6]) * q[6]);
+    if (!ok || len == 0.0) throw std::invalid_argument("A query requires 3 finite components and a nonzero direction");
+    if (q[3] < 0.f || q[7] < 0.f)
+      throw std::invalid_argument("Query radius and maximum distance must be finite and nonnegative");
+    recs[k].origin = make_float4(q[0], q[1], q[2], q[3]);
+    recs[k].direction = make_float4(float(q[4] / len), float(q[5] / len), float(q[6] / len), q[7]);
+  }
+  HIP_CHECK(hipSetDevice(device_));
+  if (n > queryCap_) {
+    if (queryDev_) HIP_CHECK(hipFree(queryDev_));
+    if (queryBest_) HIP_CHECK(hipFree(queryBest_));
+    if (queryHits_) HIP_CHECK(hipFree(queryHits_));
+    queryCap_ = std::max<uint32_t>(n, 64);
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&queryDev_), size_t(queryCap_) * sizeof(QueryRec)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&queryBest_), size_t(queryCap_) * 2 * sizeof(uint32_t)));
+    HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&queryHits_), size_t(queryCap_) * 2 * sizeof(float4)));
+    launch_query_reset(queryBest_, queryCap_, stream_);
+  }
+  HIP_CHECK(hipMemcpyAsync(queryDev_, recs.data(), size_t(n) * sizeof(QueryRec), hipMemcpyHostToDevice, stream_));
+  launch_queries(base_, queryDev_, n, filter, queryBest_, queryHits_, stream_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(hits, queryHits_, size_t(n) * 2 * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  synchronize();
 }
 
 void Engine::readSleep(uint32_t *states, uint32_t *globals) {

@@ -112,6 +112,19 @@ struct DeviceWorld {
   const uint32_t *gate;     // null, or: the collision refresh's kernels return while *gate == 0
 };
 
+/** A shape query (32 B): origin xyz, maximum distance; unit direction xyz, sweep radius (0: a ray). */
+struct QueryRec {
+  float4 origin;
+  float4 direction;
+};
+
+/** Which bodies a batch of queries sees: body group & mask != 0; with membership, the body's mask & membership != 0;
+ * never the ignored ones. */
+struct QueryFilter {
+  uint32_t ignore[16];
+  uint32_t ignoreCount, mask, membership, useMembership;
+};
+
 /** A gated launch (the collision refresh after sleepers woke) has nothing to do. */
 __device__ __forceinline__ bool gatedOff(const DeviceWorld &w) { return w.gate != nullptr && *w.gate == 0u; }
 

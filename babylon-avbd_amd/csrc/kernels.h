@@ -50,6 +50,12 @@ void launch_velocities(const DeviceWorld &w, hipStream_t s);
 void launch_restitution_before(const DeviceWorld &w, hipStream_t s);
 void launch_restitution_after(const DeviceWorld &w, hipStream_t s);
 
+/** Shape queries: every (body, query) pair traced; hits = 2 float4 per query (normal, distance or -1; index bits).
+ * `best` holds 2 words per query, reset by launch_query_reset before first use and left reset by each batch. */
+void launch_query_reset(uint32_t *best, uint32_t capacity, hipStream_t s);
+void launch_queries(const DeviceWorld &w, const QueryRec *queries, uint32_t n, const QueryFilter &f, uint32_t *best,
+                    float4 *hits, hipStream_t s);
+
 /** Sleeping: state set-up and edits (outside the step), the step's passes before it, after its narrowphase (impacts,
  * then the collision refresh's preparation, gated on bodies having woken) and after it. */
 void launch_sleep_init(const DeviceWorld &w, const uint8_t *eligible, uint32_t first, uint32_t count, hipStream_t s);

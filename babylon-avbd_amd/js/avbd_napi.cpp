@@ -552,6 +552,33 @@ napi_value SleepStats(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/** cast(world, queries: Float32Array (8 per query), ignore: Uint32Array, collidesWith, membership (-1: none)) ->
+ * Float32Array hits (8 per query). */
+napi_value Cast(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!args(env, info, 5, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *q, *ign;
+  size_t nq, ni;
+  uint32_t mask;
+  int64_t membership;
+  if (!w || !typed(env, argv[1], napi_float32_array, &q, &nq, false) ||
+      !typed(env, argv[2], napi_uint32_array, &ign, &ni, false) || !u32(env, argv[3], &mask))
+    return nullptr;
+  if (napi_get_value_int64(env, argv[4], &membership) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "expected a membership mask or -1");
+    return nullptr;
+  }
+  const uint32_t n = uint32_t(nq / 8);
+  void *data;
+  napi_value out = newTyped(env, napi_float32_array, size_t(n) * 8, 4, &data);
+  if (!out) return nullptr;
+  AvbdQueryOptions o{static_cast<const uint32_t *>(ign), uint32_t(ni), mask,
+                     membership < 0 ? 0u : uint32_t(membership), membership < 0 ? 0 : 1};
+  AVBD_OK_OR_THROW(avbd_world_cast(w, static_cast<const float *>(q), n, &o, static_cast<float *>(data)));
+  return out;
+}
+
 napi_value GetParams(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
@@ -651,6 +678,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"wakeBodies", nullptr, WakeBodies, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setSleepEnabled", nullptr, SetSleepEnabled, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"sleepStats", nullptr, SleepStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cast", nullptr, Cast, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };

@@ -500,6 +500,63 @@ class HipSolver3D {
     native().setSleepEnabled(this.handle, idx, Uint8Array.of(enabled ? 1 : 0));
     return this;
   }
+  // --- shape queries (appGpuSolver3D.js:306-332; shapeQueries.js:99-258) --------------------------------------------
+
+  async raycast(origin, direction, options) {
+    const o = options || {};
+    const [hit] = await this.raycastAll([{ origin, direction, maxDistance: o.maxDistance }], o);
+    return hit;
+  }
+  /** Rays {origin, direction, maxDistance} against every body's current pose: per ray null or
+   * {index, distance, normal, center, point}. Options: ignore, collidesWith, membership, includeTriggers, maxDistance. */
+  raycastAll(rays, options) { return this.cast(rays.map((r) => Object.assign({}, r, { radius: 0 })), options); }
+  async sphereCast(origin, radius, direction, options) {
+    const o = options || {};
+    const [hit] = await this.sphereCastAll([{ origin, radius, direction, maxDistance: o.maxDistance }], o);
+    return hit;
+  }
+  sphereCastAll(casts, options) { return this.cast(casts, options); }
+  cast(queries, options) {
+    this.alive();
+    const o = options || {};
+    const ignore = o.ignore === undefined ? [] : Array.from(o.ignore);
+    const mask = o.collidesWith === undefined ? 0xffffffff : o.collidesWith;
+    if (!Number.isInteger(mask) || mask < 0 || mask > 0xffffffff) throw Error('Query mask must be an unsigned 32-bit integer');
+    const include = o.includeTriggers === undefined ? true : o.includeTriggers;
+    if (typeof include !== 'boolean') throw Error('includeTriggers must be boolean');
+    if (o.membership !== undefined && (!Number.isInteger(o.membership) || o.membership < 0 || o.membership > 0xffffffff))
+      throw Error('Query membership must be an unsigned 32-bit integer');
+    if (ignore.length > 16 || ignore.some((i) => !Number.isInteger(i) || i < 0 || i >= this.bodyCount))
+      throw Error('Queries can ignore up to 16 live body indices');
+    if (queries.length > 65535) throw Error('Too many queries in one batch');
+    const checked = queries.map((q) => {
+      const origin = Array.from(q.origin), direction = Array.from(q.direction);
+      const radius = q.radius === undefined ? 0 : q.radius;
+      const maxDistance = q.maxDistance !== undefined ? q.maxDistance : o.maxDistance !== undefined ? o.maxDistance : 3e38;
+      if (origin.length !== 3 || direction.length !== 3 || ![...origin, ...direction].every(Number.isFinite) ||
+          Math.hypot(...direction) === 0)
+        throw Error('A query requires 3 finite components and a nonzero direction');
+      if (!Number.isFinite(radius) || radius < 0 || !Number.isFinite(maxDistance) || maxDistance < 0)
+        throw Error('Query radius and maximum distance must be finite and nonnegative');
+      const length = Math.hypot(...direction);
+      return { origin, direction: direction.map((x) => x / length), radius, maxDistance };
+    });
+    if (!checked.length) return Promise.resolve([]);
+    this.flushEdits();
+    const data = new Float32Array(checked.length * 8);
+    checked.forEach((q, i) => data.set([...q.origin, q.maxDistance, ...q.direction, q.radius], i * 8));
+    const hits = native().cast(this.handle, data, Uint32Array.from(ignore), mask >>> 0,
+                               o.membership === undefined ? -1 : o.membership);
+    const index = new Uint32Array(hits.buffer);
+    return Promise.resolve(checked.map((q, i) => {
+      const distance = hits[i * 8 + 3];
+      if (distance < 0) return null;
+      const normal = Array.from(hits.subarray(i * 8, i * 8 + 3));
+      const center = q.origin.map((x, k) => x + distance * q.direction[k]);
+      return { index: index[i * 8 + 4], distance, normal, center, point: center.map((x, k) => x - q.radius * normal[k]) };
+    }));
+  }
+
   /** {sleeping, wakeRequested} of one world after the last step. */
   readSleepStats(world = 0) {
     this.alive();

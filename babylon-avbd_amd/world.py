@@ -345,6 +345,89 @@ class HipWorld:
         check(self._lib.avbd_world_sleep_stats(self._h, int(world), ptr(out, ctypes.c_uint32)))
         return {"sleeping": int(out[0]), "wakeRequested": bool(out[1])}
 
+    # --- shape queries (appGpuSolver3D.js:306-332; shapeQueries.js:99-258) ------------------------------------------
+
+    def raycast(self, origin, direction, options=None):
+        """The nearest hit of one ray, or None (raycastAll of one ray)."""
+        o = dict(options or {})
+        return self.raycastAll([{"origin": origin, "direction": direction, "maxDistance": o.get("maxDistance")}], o)[0]
+
+    def raycastAll(self, rays, options=None):
+        """Rays {origin, direction, maxDistance?} against every body's current pose: per ray None or
+        {index, distance, normal, center, point}. Options: ignore (up to 16 bodies), collidesWith (a body is seen when
+        its group & collidesWith != 0), membership (also its collides-with mask & membership != 0), includeTriggers,
+        maxDistance (the batch default)."""
+        return self._cast([dict(r, radius=0.0) for r in rays], options)
+
+    def sphereCast(self, origin, radius, direction, options=None):
+        o = dict(options or {})
+        return self.sphereCastAll([{"origin": origin, "radius": radius, "direction": direction,
+                                    "maxDistance": o.get("maxDistance")}], o)[0]
+
+    def sphereCastAll(self, casts, options=None):
+        """Spheres {origin, radius, direction, maxDistance?} swept along their directions; results as raycastAll's,
+        `point` being the touching point on the body (center − radius · normal)."""
+        return self._cast(casts, options)
+
+    def _cast(self, queries, options):
+        self._ready()
+        o = dict(options or {})
+        ignore = list(o.get("ignore", []))
+        mask = o.get("collidesWith", 0xFFFFFFFF)
+        membership = o.get("membership")
+        include = o.get("includeTriggers", True)
+        if isinstance(mask, bool) or not isinstance(mask, (int, np.integer)) or not 0 <= mask <= 0xFFFFFFFF:
+            raise ValueError("Query mask must be an unsigned 32-bit integer")
+        if not isinstance(include, bool):
+            raise ValueError("includeTriggers must be boolean")
+        if membership is not None and (isinstance(membership, bool) or not isinstance(membership, (int, np.integer))
+                                       or not 0 <= membership <= 0xFFFFFFFF):
+            raise ValueError("Query membership must be an unsigned 32-bit integer")
+        if len(ignore) > 16 or any(isinstance(i, bool) or not isinstance(i, (int, np.integer)) or not
+                                   0 <= i < self.bodyCount for i in ignore):
+            raise ValueError("Queries can ignore up to 16 live body indices")
+        if len(queries) > 65535:
+            raise ValueError("Too many queries in one batch")
+        batch_max = o.get("maxDistance")
+        rows, checked = [], []
+        for q in queries:
+            origin = [float(x) for x in q["origin"]]
+            direction = [float(x) for x in q["direction"]]
+            radius = q.get("radius", 0.0)
+            radius = 0.0 if radius is None else float(radius)
+            md = q.get("maxDistance")
+            md = (batch_max if batch_max is not None else 3e38) if md is None else md
+            md = float(md)
+            if len(origin) != 3 or len(direction) != 3 or not all(math.isfinite(x) for x in origin + direction) \
+                    or math.hypot(*direction) == 0:
+                raise ValueError("A query requires 3 finite components and a nonzero direction")
+            if not (math.isfinite(radius) and radius >= 0 and math.isfinite(md) and md >= 0):
+                raise ValueError("Query radius and maximum distance must be finite and nonnegative")
+            length = math.hypot(*direction)
+            unit = [x / length for x in direction]
+            rows.append(origin + [md] + unit + [radius])
+            checked.append((origin, unit, radius))
+        if not rows:
+            return []
+        data = np.ascontiguousarray(rows, np.float32)
+        hits = np.zeros((len(rows), 8), np.float32)
+        ign = np.ascontiguousarray(ignore, np.uint32)
+        opts = _native.QueryOptions(ign.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if len(ign) else None,
+                                    len(ign), int(mask), int(membership or 0), int(membership is not None))
+        check(self._lib.avbd_world_cast(self._h, ptr(data, ctypes.c_float), len(rows), ctypes.byref(opts),
+                                        ptr(hits, ctypes.c_float)))
+        out = []
+        for (origin, unit, radius), h in zip(checked, hits):
+            if h[3] < 0:
+                out.append(None)
+                continue
+            d = float(h[3])
+            normal = h[0:3].astype(float).tolist()
+            center = [origin[k] + d * unit[k] for k in range(3)]
+            out.append({"index": int(h[4:5].view(np.uint32)[0]), "distance": d, "normal": normal, "center": center,
+                        "point": [center[k] - radius * normal[k] for k in range(3)]})
+        return out
+
     def readSleep(self):
         """Every body's sleep state as an (n, 8) u32 array (mass and quiet as f32 bits, asleep, support, eligible,
         parent, wake, flags) and the (worlds, 4) sleep globals."""

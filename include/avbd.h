@@ -273,6 +273,24 @@ int avbd_world_sleep_stats(avbd_world *world, uint32_t world_index, uint32_t out
  * wake bits, flags), and 4 words per world (wake all, asleep, woken by an impact, island hit); either may be NULL. */
 int avbd_world_read_sleep(avbd_world *world, uint32_t *states, uint32_t *globals);
 
+/* ---- Shape queries (src/gpu/shapeQueries.js, shapeQueries.wgsl) --------------------------------------------------- */
+
+typedef struct AvbdQueryOptions {
+  const uint32_t *ignore; /* up to 16 body indices left out, or NULL */
+  uint32_t ignore_count;
+  uint32_t collides_with; /* bodies whose collision group & collides_with is 0 are left out (0xFFFFFFFF: none) */
+  uint32_t membership;    /* with use_membership: bodies whose collides-with mask & membership is 0 are left out */
+  int use_membership;
+} AvbdQueryOptions;
+
+/** raycastAll / sphereCastAll (appGpuSolver3D.js:306-332, shapeQueries.js:99-258): n queries (at most 65,535) of 8
+ * floats — origin xyz, maximum distance, direction xyz (nonzero; normalised here), sweep radius (0: a ray) — against
+ * every body's current pose; each query's nearest hit (smallest body index on a tie) into 8 floats of hits: world
+ * normal xyz, distance (-1: no hit), body index (u32 bits, 0xFFFFFFFF: none), 0, 0, 0. A query starting inside or
+ * touching a shape hits it at distance 0 with normal -direction. options may be NULL (no filtering). */
+int avbd_world_cast(avbd_world *world, const float *queries, uint32_t n, const AvbdQueryOptions *options,
+                    float *hits);
+
 /* ---- Serving loop ------------------------------------------------------------------------------------------------
  * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):
  * a server takes step/rollout requests for many registered worlds as they arrive, keeps world state resident in HBM

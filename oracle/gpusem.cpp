@@ -4,6 +4,7 @@
 // records, seed its persistent state from a state read back from the HIP library, step it in FP32 (the mirror) or
 // FP64 (with the FP32 mirror's contacts and colours, to state the FP32 error of the solve), and read everything back in
 // the same record layouts.
+#include "queries.hpp"
 #include <cstring>
 #include <memory>
 
@@ -470,6 +471,52 @@ void gs_set_sleep(void *p, const uint32_t *states, const uint32_t *globals) {
     s.asleep = o[2]; s.support = o[3]; s.eligible = o[4]; s.parent = o[5]; s.wake = o[6]; s.flags = o[7];
   }
   std::copy(globals, globals + h->f->sleepGlobals.size(), h->f->sleepGlobals.begin());
+}
+
+// --- shape queries ---------------------------------------------------------------------------------------------------
+
+/**
+ * raycastAll / sphereCastAll over n body records (40 floats) with their filters (group, collides-with): queries of 8
+ * floats (origin xyz, max distance, unit direction xyz, radius); hits of 8 floats (normal xyz, distance or -1, index
+ * bits, 0, 0, 0). A body is left out when group & mask is 0, when (with useMembership) membership is 0 or its
+ * collides-with & membership is 0, or when it is in `ignore`.
+ */
+void gs_cast(uint32_t nBodies, const float *bodyRecs, const uint32_t *filters, const float *queries, uint32_t n,
+             const uint32_t *ignore, uint32_t nIgnore, uint32_t mask, uint32_t membership, int useMembership,
+             float *hits) {
+  for (uint32_t q = 0; q < n; q++) {
+    const float *query = queries + 8ull * q;
+    float bestT = INFINITY;
+    uint32_t bestI = 0xffffffffu;
+    oracle_queries::Hit best{-1.f, {0, 0, 0}};
+    for (uint32_t i = 0; i < nBodies; i++) {
+      if ((filters[2 * i] & mask) == 0u) continue;
+      if (useMembership && (membership == 0u || (filters[2 * i + 1] & membership) == 0u)) continue;
+      bool skip = false;
+      for (uint32_t k = 0; k < nIgnore; k++) skip = skip || ignore[k] == i;
+      if (skip) continue;
+      const oracle_queries::Hit h = oracle_queries::trace(bodyRecs + 40ull * i, query);
+      if (h.t < 0.f) continue;
+      const float t = h.t > 0.f ? h.t : 0.f;
+      if (t < bestT) {  // bodies in index order: a tie keeps the smaller index
+        bestT = t;
+        bestI = i;
+        best = h;
+      }
+    }
+    float *o = hits + 8ull * q;
+    for (int k = 0; k < 8; k++) o[k] = 0.f;
+    uint32_t idx = bestI;
+    std::memcpy(o + 4, &idx, 4);
+    if (bestI == 0xffffffffu) {
+      o[3] = -1.f;
+      continue;
+    }
+    o[0] = best.n.x;
+    o[1] = best.n.y;
+    o[2] = best.n.z;
+    o[3] = bestT;
+  }
 }
 
 // --- unit-level probes (for tests of single functions against the upstream restatement) -----------------------------

@@ -2,8 +2,8 @@
 // The reference's Babylon-free world-API contracts (tests/backend-contract.js checkBackend and checkFloor,
 // tests/packed-poses-contract.js packedPoseContract), restated for the "hip" backend of babylon-avbd_amd/js/index.js:
 // the same calls and the same expectations, in 3D, the restitution part of checkContacts, and checkSleeping. What this
-// backend does not build (trigger bodies, ray queries) is checked to be refused with an error, or, for switching it
-// off, accepted.
+// backend does not build (trigger bodies) is checked to be refused with an error, or, for switching it off, accepted;
+// a ray query finds the teleported box.
 // Prints one JSON object with a record per contract; exits non-zero on the first failure.
 const path = require('path');
 const avbd = require(path.join(__dirname, '..', '..', 'babylon-avbd_amd', 'js', 'index.js'));
@@ -50,9 +50,10 @@ async function checkBackend() {
     check(state.angularVelocity[2] < 0, 'Off-centre impulse did not rotate the body');
     handle.setCollisionGroups(4, 4).setRestitution(0.6);
     check(world.getRestitution()[id] === Math.fround(0.6), 'Restitution was not set');
-    // Not built here: a trigger body and ray queries are refused, not ignored
+    // Not built here: a trigger body is refused, not ignored
     check(throws(() => handle.setTrigger(true)), 'A trigger was accepted');
-    check(typeof world.raycast !== 'function', 'A ray query appeared without an implementation');
+    const ray = await world.raycast([2, 10, 4], [0, -1, 0]);
+    check(ray && ray.index === id && Math.abs(ray.distance - 6.5) < 1e-5, 'Ray query missed the body');
     const selected = await world.readSelectedBodies([id], { posesOnly: true });
     check(selected.length === 8, 'Pose stride changed across backends');
     handle.setTrigger(false).setSleepEnabled(true).setSleepEnabled(false);
