@@ -1,7 +1,7 @@
 """The reference's Babylon-free world-API contracts on the Python facade of the hip backend: checkBackend and
 checkFloor of reference tests/backend-contract.js and packedPoseContract of tests/packed-poses-contract.js, the same
-calls and expectations in 3D. What this backend does not build (trigger bodies, sleeping, ray queries) must be refused,
-or accepted when it is switched off."""
+calls and expectations in 3D. What this backend does not build (trigger bodies) must be refused, or accepted when it
+is switched off; a ray query finds the teleported box."""
 import math
 
 import numpy as np
@@ -38,7 +38,8 @@ def test_backend_contract(avbd):
         assert world.getRestitution()[i] == np.float32(0.6)
         with pytest.raises(ValueError):
             handle.setTrigger(True)
-        assert not hasattr(world, "raycast")
+        ray = world.raycast([2, 10, 4], [0, -1, 0])  # the teleported unit box, top face at y = 3.5
+        assert ray is not None and ray["index"] == i and abs(ray["distance"] - 6.5) < 1e-5
         assert world.readSelectedBodies([i], posesOnly=True).size == 8
         handle.setTrigger(False).setSleepEnabled(True).setSleepEnabled(False)
         world.disableSleeping()
