@@ -581,6 +581,42 @@ int avbd_world_cast(avbd_world *world, const float *queries, uint32_t n, const A
   });
 }
 
+int avbd_world_watch_contacts(avbd_world *world, uint32_t max_pairs, uint32_t max_events) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    world->engine->watchContacts(max_pairs, max_events);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_watched(avbd_world *world, const uint32_t *indices, const uint8_t *flags, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (n && (!indices || !flags)) throw std::invalid_argument("indices or flags is NULL");
+    world->engine->setWatched(indices, flags, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_reset_contact_history(avbd_world *world, const uint32_t *indices, uint32_t n) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    if (n && !indices) throw std::invalid_argument("indices is NULL");
+    world->engine->resetContactHistory(indices, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_read_contact_events(avbd_world *world, uint32_t *out, uint32_t cap, uint32_t *count, uint32_t *written,
+                                   uint32_t *full) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    const uint32_t n = world->engine->readContactEvents(out, cap, written, full);
+    if (count) *count = n;
+    return AVBD_OK;
+  });
+}
+
 // --- serving loop ----------------------------------------------------------------------------------------------------
 
 #define CHECK_SERVER(s) \

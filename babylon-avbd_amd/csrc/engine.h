@@ -157,6 +157,17 @@ class Engine {
    * body index bits (~0: none), 0, 0, 0).
    */
   void cast(const float *queries, uint32_t n, const QueryFilter &filter, float *hits);
+  /** Contact events (ContactWatch): storage for maxPairs touching pairs per step and maxEvents unread events, made on
+   * the first call (every body watched); later calls keep it. */
+  void watchContacts(uint32_t maxPairs, uint32_t maxEvents);
+  bool watchingContacts() const { return cw_.table != nullptr; }
+  /** Which bodies report their contacts (flags 0 or 1). */
+  void setWatched(const uint32_t *indices, const uint8_t *flags, uint32_t n);
+  /** New lifetimes for n bodies (sorted, unique): their pairs begin again and do not end. */
+  void resetContactHistory(const uint32_t *indices, uint32_t n);
+  /** Unread events (12 words each: kind, a, b, step, point xyz, impulse, normal xyz, 0) into out (room for cap);
+   * returns how many were copied; *written = all events since the last read, *full = a set overflowed. Clears the log. */
+  uint32_t readContactEvents(uint32_t *out, uint32_t cap, uint32_t *written, uint32_t *full);
   /** Solver iterations per step (all worlds of the batch). */
   void setIterations(uint32_t n);
 
@@ -177,6 +188,7 @@ class Engine {
   void allocate();
   void allocateContacts(uint32_t pairCap);
   void encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce = 0);
+  void uploadWatch();
   SleepParams sleepParams(uint32_t force) const { return SleepParams{sleepSpeed_, sleepDelay_, sleepIslands_ ? 1u : 0u, force}; }
   /** Impacts on sleepers after the narrowphase, then the pairs and narrowphase again, gated on a body having woken. */
   void encodeSleepImpact(const DeviceWorld &w, uint32_t force);
@@ -216,6 +228,8 @@ class Engine {
   uint32_t *queryBest_ = nullptr;
   float4 *queryHits_ = nullptr;
   uint32_t queryCap_ = 0;
+  ContactWatchDev cw_{};
+  std::vector<uint32_t> watchWords_;
   bool listedRounds_ = false;  // colouring rounds over the pending list in one block (few pending bodies lately)
   uint32_t coloringRounds_ = 0;
   uint32_t listedVotes_ = 0;

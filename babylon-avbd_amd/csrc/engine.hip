@@ -356,6 +356,9 @@ Engine::~Engine() {
   if (queryDev_) (void)hipFree(queryDev_);
   if (queryBest_) (void)hipFree(queryBest_);
   if (queryHits_) (void)hipFree(queryHits_);
+  for (void *p : {static_cast<void *>(cw_.table), static_cast<void *>(cw_.events), static_cast<void *>(cw_.watch),
+                  static_cast<void *>(cw_.previousWatch)})
+    if (p) (void)hipFree(p);
   for (auto &ev : counterEvents_)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : timingEvents_) (void)hipEventDestroy(ev);
@@ -429,6 +432,7 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce)
   launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);
   if (sleeping_) launch_sleep_after(w, sleepParams(sleepForce), s);
+  if (cw_.table) launch_contact_events(w, cw_, &w == &views_[1] ? 1u : 0u, s);
 }
 
 void Engine::pollCounters() {
@@ -573,6 +577,7 @@ PhaseTimes Engine::profileStep() {
   launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);  // timed with the velocities
   if (sleeping_) launch_sleep_after(w, sleepParams(force), s);
+  if (cw_.table) launch_contact_events(w, cw_, &w == &views_[1] ? 1u : 0u, s);
   HIP_CHECK(hipEventRecord(ev[5], s));
   HIP_CHECK(hipEventSynchronize(ev[5]));
   stepIndex_++;
@@ -1059,6 +1064,70 @@ void Engine::cast(const float *queries, uint32_t n, const QueryFilter &filter, f
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(hits, queryHits_, size_t(n) * 2 * sizeof(float4), hipMemcpyDeviceToHost, stream_));
   synchronize();
+}
+
+void Engine::watchContacts(uint32_t maxPairs, uint32_t maxEvents) {
+  if (maxPairs < 1 || maxEvents < 1) throw std::invalid_argument("Contact capacities must be positive integers");
+  if (cw_.table) return;
+  HIP_CHECK(hipSetDevice(device_));
+  uint32_t slots = 64;
+  while (slots < 2ull * maxPairs) slots *= 2;
+  cw_.mask = slots - 1;
+  cw_.capacity = maxEvents;
+  const size_t tableWords = CW_HEAD + 4ull * slots;
+  HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&cw_.table), tableWords * 4));
+  HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&cw_.events), size_t(maxEvents) * 3 * sizeof(float4)));
+  HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&cw_.watch), size_t(nBodies_) * 4 + 4));
+  HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&cw_.previousWatch), size_t(nBodies_) * 4 + 4));
+  launch_fill_u32(cw_.table, 0u, tableWords, stream_);
+  watchWords_.assign(nBodies_, 1u);
+  uploadWatch();
+  HIP_CHECK(hipMemcpyAsync(cw_.previousWatch, cw_.watch, size_t(nBodies_) * 4, hipMemcpyDeviceToDevice, stream_));
+  synchronize();
+  resetGraphs();  // the step gains its event passes
+}
+
+void Engine::uploadWatch() {
+  HIP_CHECK(hipMemcpyAsync(cw_.watch, watchWords_.data(), size_t(nBodies_) * 4, hipMemcpyHostToDevice, stream_));
+  synchronize();
+}
+
+void Engine::setWatched(const uint32_t *indices, const uint8_t *flags, uint32_t n) {
+  if (!cw_.table) watchContacts(8192, 4096);
+  for (uint32_t k = 0; k < n; k++)
+    if (indices[k] >= nBodies_ || flags[k] > 1) throw std::invalid_argument("Watched indices must be live body indices");
+  for (uint32_t k = 0; k < n; k++) watchWords_[indices[k]] = (watchWords_[indices[k]] & ~1u) | flags[k];
+  HIP_CHECK(hipSetDevice(device_));
+  uploadWatch();
+}
+
+void Engine::resetContactHistory(const uint32_t *indices, uint32_t n) {
+  int64_t previous = -1;
+  for (uint32_t k = 0; k < n; k++) {
+    if (indices[k] >= nBodies_ || int64_t(indices[k]) <= previous)
+      throw std::invalid_argument("Contact reset slots must be unique and ascending");
+    previous = indices[k];
+  }
+  if (!cw_.table) watchContacts(8192, 4096);
+  for (uint32_t k = 0; k < n; k++) watchWords_[indices[k]] += 2u;
+  HIP_CHECK(hipSetDevice(device_));
+  uploadWatch();
+}
+
+uint32_t Engine::readContactEvents(uint32_t *out, uint32_t cap, uint32_t *written, uint32_t *full) {
+  if (!cw_.table) watchContacts(8192, 4096);
+  HIP_CHECK(hipSetDevice(device_));
+  uint32_t head[2];
+  HIP_CHECK(hipMemcpyAsync(head, cw_.table, sizeof head, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  const uint32_t n = std::min(std::min(head[0], cw_.capacity), cap);
+  if (n && out)
+    HIP_CHECK(hipMemcpyAsync(out, cw_.events, size_t(n) * 3 * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  launch_fill_u32(cw_.table, 0u, 2, stream_);
+  synchronize();
+  if (written) *written = head[0];
+  if (full) *full = head[1];
+  return n;
 }
 
 void Engine::readSleep(uint32_t *states, uint32_t *globals) {

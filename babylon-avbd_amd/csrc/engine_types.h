@@ -125,6 +125,18 @@ struct QueryFilter {
   uint32_t ignoreCount, mask, membership, useMembership;
 };
 
+/** Contact event storage: head words (events written, a set overflowed, 2 spare), then two hash sets of mask + 1 slots
+ * (2 words: a + 1, b); events of 3 float4 (kind, a, b, step bits; point, impulse; normal); watch words per body (bit
+ * 0: watched, above it: lifetime epoch) now and at the last step. */
+constexpr uint32_t CW_HEAD = 4, CW_BEGIN = 1, CW_END = 2;
+struct ContactWatchDev {
+  uint32_t *table;
+  float4 *events;
+  uint32_t *watch;
+  uint32_t *previousWatch;
+  uint32_t mask, capacity;
+};
+
 /** A gated launch (the collision refresh after sleepers woke) has nothing to do. */
 __device__ __forceinline__ bool gatedOff(const DeviceWorld &w) { return w.gate != nullptr && *w.gate == 0u; }
 

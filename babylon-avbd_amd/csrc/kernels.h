@@ -56,6 +56,9 @@ void launch_query_reset(uint32_t *best, uint32_t capacity, hipStream_t s);
 void launch_queries(const DeviceWorld &w, const QueryRec *queries, uint32_t n, const QueryFilter &f, uint32_t *best,
                     float4 *hits, hipStream_t s);
 
+/** Contact events of the step just taken (after its solve); `now` alternates with the step parity. */
+void launch_contact_events(const DeviceWorld &w, const ContactWatchDev &c, uint32_t now, hipStream_t s);
+
 /** Sleeping: state set-up and edits (outside the step), the step's passes before it, after its narrowphase (impacts,
  * then the collision refresh's preparation, gated on bodies having woken) and after it. */
 void launch_sleep_init(const DeviceWorld &w, const uint8_t *eligible, uint32_t first, uint32_t count, hipStream_t s);

@@ -428,6 +428,56 @@ class HipWorld:
                         "point": [center[k] - radius * normal[k] for k in range(3)]})
         return out
 
+    # --- contact events (solver3D.js:252-311; contacts.ts ContactWatch) --------------------------------------------
+
+    def watchContacts(self, maxPairs=8192, maxEvents=4096, indices=None):
+        """Bounded contact-event storage, made on the first call (every body watched); indices: only these report."""
+        self._alive()
+        for n in (maxPairs, maxEvents):
+            if isinstance(n, bool) or not isinstance(n, (int, np.integer)) or n < 1:
+                raise ValueError("Contact capacities must be positive integers")
+        if indices is not None and not isinstance(indices, (list, tuple)):
+            raise ValueError("Watched indices must be an array")
+        idx = [self.liveIndex(i) for i in (indices or [])]
+        self.flushEdits()
+        check(self._lib.avbd_world_watch_contacts(self._h, int(maxPairs), int(maxEvents)))
+        self._event_capacity = getattr(self, "_event_capacity", int(maxEvents))
+        if indices is not None:
+            selected = set(idx)
+            all_ = np.arange(self.bodyCount, dtype=np.uint32)
+            flags = np.array([int(i in selected) for i in range(self.bodyCount)], np.uint8)
+            check(self._lib.avbd_world_set_watched(self._h, ptr(all_, ctypes.c_uint32), ptr(flags, ctypes.c_uint8),
+                                                   self.bodyCount))
+        return self
+
+    def readContactEvents(self):
+        """{events: [{type 'begin'|'end', a, b, step, point, impulse, normal, isTrigger}], dropped, full}: the events
+        since the last read, in step order, clearing them."""
+        self._ready()
+        if not hasattr(self, "_event_capacity"):
+            self.watchContacts()
+        cap = self._event_capacity
+        out = np.zeros((cap, 12), np.uint32)
+        count, written, full = (np.zeros(1, np.uint32) for _ in range(3))
+        check(self._lib.avbd_world_read_contact_events(self._h, ptr(out, ctypes.c_uint32), cap,
+                                                       ptr(count, ctypes.c_uint32), ptr(written, ctypes.c_uint32),
+                                                       ptr(full, ctypes.c_uint32)))
+        rows = out[:int(count[0])]
+        f = rows.view(np.float32)
+        events = [{"type": "begin" if r[0] == 1 else "end", "kind": int(r[0]), "a": int(r[1]), "b": int(r[2]),
+                   "step": int(r[3]), "point": f[k, 4:7].tolist(), "impulse": float(f[k, 7]),
+                   "normal": f[k, 8:11].tolist(), "isTrigger": False} for k, r in enumerate(rows)]
+        events.sort(key=lambda e: e["step"])
+        return {"events": events, "dropped": int(written[0] - count[0]), "full": bool(full[0])}
+
+    def resetContactHistory(self, indices):
+        """New lifetimes for sorted unique slots: their pairs begin again and do not end."""
+        self._alive()
+        idx = np.ascontiguousarray(indices, np.uint32)
+        self.flushEdits()
+        check(self._lib.avbd_world_reset_contact_history(self._h, ptr(idx, ctypes.c_uint32), len(idx)))
+        return self
+
     def readSleep(self):
         """Every body's sleep state as an (n, 8) u32 array (mass and quiet as f32 bits, asleep, support, eligible,
         parent, wake, flags) and the (worlds, 4) sleep globals."""

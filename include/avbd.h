@@ -291,6 +291,22 @@ typedef struct AvbdQueryOptions {
 int avbd_world_cast(avbd_world *world, const float *queries, uint32_t n, const AvbdQueryOptions *options,
                     float *hits);
 
+/* ---- Contact events (src/vendor/three-avbd-lib/contacts.ts ContactWatch; src/native/solver3D.js:252-311) -------- */
+
+/** watchContacts({maxPairs, maxEvents}): event storage, made once (every body watched); after each step a pair with a
+ * watched body that starts touching logs a begin event and one that stops an end event. */
+int avbd_world_watch_contacts(avbd_world *world, uint32_t max_pairs, uint32_t max_events);
+/** watchContacts({indices}): which bodies report their contacts (flags 0 or 1). */
+int avbd_world_set_watched(avbd_world *world, const uint32_t *indices, const uint8_t *flags, uint32_t n);
+/** resetContactHistory(indices) (sorted, unique): new lifetimes; their pairs begin again and do not end. */
+int avbd_world_reset_contact_history(avbd_world *world, const uint32_t *indices, uint32_t n);
+/** readContactEvents(): up to cap unread events, 12 words each (kind 1 begin / 2 end, a, b, step, point xyz (f32,
+ * mean of the points on a), impulse (f32, |sum of normal multipliers| x dt), normal xyz (f32, b to a), 0), into out;
+ * *count = copied, *written = events since the last read (above capacity some were dropped), *full = a pair set
+ * overflowed. Clears the log. */
+int avbd_world_read_contact_events(avbd_world *world, uint32_t *out, uint32_t cap, uint32_t *count, uint32_t *written,
+                                   uint32_t *full);
+
 /* ---- Serving loop ------------------------------------------------------------------------------------------------
  * No counterpart in the reference, whose worlds are stepped one submission at a time by their owner (pipeline.js:49-79):
  * a server takes step/rollout requests for many registered worlds as they arrive, keeps world state resident in HBM
