@@ -579,6 +579,69 @@ napi_value Cast(napi_env env, napi_callback_info info) {
   return out;
 }
 
+napi_value WatchContacts(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t pairs, events;
+  if (!w || !u32(env, argv[1], &pairs) || !u32(env, argv[2], &events)) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_watch_contacts(w, pairs, events));
+  return nullptr;
+}
+
+napi_value SetWatched(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!args(env, info, 3, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx, *flags;
+  size_t n, nf;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false) ||
+      !typed(env, argv[2], napi_uint8_array, &flags, &nf, false))
+    return nullptr;
+  if (nf != n) {
+    napi_throw_range_error(env, nullptr, "Each watched index needs a flag");
+    return nullptr;
+  }
+  AVBD_OK_OR_THROW(avbd_world_set_watched(w, static_cast<const uint32_t *>(idx), static_cast<const uint8_t *>(flags),
+                                          uint32_t(n)));
+  return nullptr;
+}
+
+napi_value ResetContactHistory(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  void *idx;
+  size_t n;
+  if (!w || !typed(env, argv[1], napi_uint32_array, &idx, &n, false)) return nullptr;
+  AVBD_OK_OR_THROW(avbd_world_reset_contact_history(w, static_cast<const uint32_t *>(idx), uint32_t(n)));
+  return nullptr;
+}
+
+/** readContactEvents(world, capacity) -> {words: Uint32Array (12 per event), written, full}. */
+napi_value ReadContactEvents(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  avbd_world *w = worldOf(env, argv[0]);
+  uint32_t cap;
+  if (!w || !u32(env, argv[1], &cap)) return nullptr;
+  std::vector<uint32_t> buf(size_t(cap) * 12);
+  uint32_t count = 0, written = 0, full = 0;
+  AVBD_OK_OR_THROW(avbd_world_read_contact_events(w, buf.data(), cap, &count, &written, &full));
+  void *data;
+  napi_value words = newTyped(env, napi_uint32_array, size_t(count) * 12, 4, &data);
+  if (!words) return nullptr;
+  std::memcpy(data, buf.data(), size_t(count) * 48);
+  napi_value out, a, b;
+  napi_create_object(env, &out);
+  napi_create_uint32(env, written, &a);
+  napi_get_boolean(env, full != 0, &b);
+  napi_set_named_property(env, out, "words", words);
+  napi_set_named_property(env, out, "written", a);
+  napi_set_named_property(env, out, "full", b);
+  return out;
+}
+
 napi_value GetParams(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!args(env, info, 2, argv)) return nullptr;
@@ -679,6 +742,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"setSleepEnabled", nullptr, SetSleepEnabled, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"sleepStats", nullptr, SleepStats, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cast", nullptr, Cast, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"watchContacts", nullptr, WatchContacts, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"setWatched", nullptr, SetWatched, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"resetContactHistory", nullptr, ResetContactHistory, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"readContactEvents", nullptr, ReadContactEvents, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };

@@ -557,6 +557,53 @@ class HipSolver3D {
     }));
   }
 
+  // --- contact events (solver3D.js:252-311; contacts.ts ContactWatch) -------------------------------------------
+
+  /** Bounded contact-event storage, made on the first call (every body watched); indices: only these report. */
+  watchContacts(options) {
+    this.alive();
+    const o = options || {};
+    const maxPairs = o.maxPairs === undefined ? 8192 : o.maxPairs, maxEvents = o.maxEvents === undefined ? 4096 : o.maxEvents;
+    for (const n of [maxPairs, maxEvents])
+      if (!Number.isInteger(n) || n < 1) throw Error('Contact capacities must be positive integers');
+    if (o.indices !== undefined && !Array.isArray(o.indices)) throw Error('Watched indices must be an array');
+    const indices = o.indices === undefined ? undefined : o.indices.map((i) => this.liveIndex(i));
+    this.flushEdits();
+    native().watchContacts(this.handle, maxPairs, maxEvents);
+    if (this.eventCapacity === undefined) this.eventCapacity = maxEvents;
+    if (indices !== undefined) {
+      const selected = new Set(indices);
+      const all = Uint32Array.from({ length: this.bodyCount }, (_, i) => i);
+      native().setWatched(this.handle, all, Uint8Array.from(all, (i) => (selected.has(i) ? 1 : 0)));
+    }
+    return this;
+  }
+  /** {events: [{type, a, b, step, point, impulse, normal, isTrigger}], dropped, full} since the last read. */
+  readContactEvents() {
+    this.alive();
+    this.flushEdits();
+    if (this.eventCapacity === undefined) this.watchContacts();
+    const r = native().readContactEvents(this.handle, this.eventCapacity);
+    const f = new Float32Array(r.words.buffer, r.words.byteOffset, r.words.length);
+    const events = [];
+    for (let e = 0; e < r.words.length / 12; e++) {
+      const o = e * 12;
+      events.push({ type: r.words[o] === 1 ? 'begin' : 'end', a: r.words[o + 1], b: r.words[o + 2], step: r.words[o + 3],
+                    point: Array.from(f.subarray(o + 4, o + 7)), impulse: f[o + 7],
+                    normal: Array.from(f.subarray(o + 8, o + 11)), isTrigger: false });
+    }
+    events.sort((x, y) => x.step - y.step);
+    return Promise.resolve({ events, dropped: r.written - events.length, full: r.full });
+  }
+  /** New lifetimes for sorted unique slots: their pairs begin again and do not end. */
+  resetContactHistory(indices) {
+    this.alive();
+    if (!(indices instanceof Uint32Array)) throw Error('Contact resets require Uint32 GPU slots');
+    this.flushEdits();
+    native().resetContactHistory(this.handle, indices);
+    return this;
+  }
+
   /** {sleeping, wakeRequested} of one world after the last step. */
   readSleepStats(world = 0) {
     this.alive();

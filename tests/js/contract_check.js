@@ -52,6 +52,9 @@ async function checkBackend() {
     check(world.getRestitution()[id] === Math.fround(0.6), 'Restitution was not set');
     // Not built here: a trigger body is refused, not ignored
     check(throws(() => handle.setTrigger(true)), 'A trigger was accepted');
+    world.watchContacts({ maxPairs: 64, maxEvents: 64 });
+    const noEvents = await world.readContactEvents();
+    check(noEvents.events.length === 0 && noEvents.dropped === 0 && !noEvents.full, 'Events before any step');
     const ray = await world.raycast([2, 10, 4], [0, -1, 0]);
     check(ray && ray.index === id && Math.abs(ray.distance - 6.5) < 1e-5, 'Ray query missed the body');
     const selected = await world.readSelectedBodies([id], { posesOnly: true });
