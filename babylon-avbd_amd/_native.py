@@ -146,6 +146,8 @@ SIGNATURES = {
     "avbd_world_write_state": (ctypes.c_int, [ctypes.c_void_p, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
     "avbd_world_set_coloring": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "avbd_world_adapt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "avbd_world_get_coloring": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),
     "avbd_world_device_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
     "avbd_world_enqueue_commands": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),

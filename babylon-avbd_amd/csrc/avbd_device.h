@@ -54,6 +54,7 @@ enum Counter : uint32_t {
   C_PIVOT64 = 9,  // bodies solved again in FP64 this step (an FP32 pivot took its floor)
   C_DF_ABORT = 10,  // a bounded wait gave up (dataflow solve, prefix scan; never expected: the step is then not valid)
   C_PENDING = 11,   // bodies the colouring's mark pass left pending (listed in colorPending)
+  C_PAIR_DEMAND = 12,  // pairs the broadphase found, before clamping to the capacity (C_PAIRS is the clamped count)
   COUNTER_WORDS = 16
 };
 

@@ -392,6 +392,16 @@ int avbd_world_set_coloring(avbd_world *world, uint32_t color_cap, uint32_t colo
   });
 }
 
+int avbd_world_get_coloring(avbd_world *world, uint32_t *color_cap, uint32_t *color_rounds, uint32_t *pair_capacity) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (color_cap) *color_cap = world->engine->colorCap();
+    if (color_rounds) *color_rounds = world->engine->colorRounds();
+    if (pair_capacity) *pair_capacity = world->engine->pairCapacity();
+    return AVBD_OK;
+  });
+}
+
 int avbd_world_adapt(avbd_world *world, int *changed) {
   return guarded([&] {
     CHECK_WORLD(world);

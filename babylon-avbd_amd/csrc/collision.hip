@@ -224,6 +224,7 @@ template <bool STASH> __global__ void k_pairs(DeviceWorld w) {
 __global__ void k_pairs_total(DeviceWorld w) {
   if (gatedOff(w)) return;
   uint32_t total = w.pairCount[w.nBodies];
+  w.counters[C_PAIR_DEMAND] = total;
   if (total > w.pairCapacity) {
     w.counters[C_OVERFLOW] |= 1u;
     total = w.pairCapacity;

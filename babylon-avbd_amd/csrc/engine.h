@@ -58,7 +58,8 @@ struct SavedState {
 };
 
 struct Counters {
-  uint32_t pairs, contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
+  uint32_t pairs;  // pairs the broadphase found (above the capacity when it overflowed), as the reference's counter
+  uint32_t contacts, manifolds, overflow, clashes, colors, prevManifolds, pivot64;
   uint32_t solveAborts;  // dataflow solve waits given up in the last step (0 unless something is wrong)
   uint32_t pending = 0;  // bodies the colouring's mark pass left pending
 };
@@ -174,6 +175,7 @@ class Engine {
   void setColorCap(uint32_t cap);
   void setColorRounds(uint32_t rounds);
   uint32_t colorCap() const { return colorCap_; }
+  uint32_t colorRounds() const { return colorRounds_; }
   uint32_t pairCapacity() const { return pairCap_; }
   bool usesBvh() const { return useBvh_; }
   /** Grow contact storage / colour cap from the last counters (the reference's adapt); returns true if it changed. */

@@ -443,7 +443,7 @@ void Engine::pollCounters() {
     if (hipEventQuery(counterEvents_[slot]) != hipSuccess) continue;
     counterPending_[slot] = false;
     const uint32_t *c = hostCounters_ + slot * COUNTER_WORDS;
-    Counters cc{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
+    Counters cc{c[C_PAIR_DEMAND], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
                 c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
     adaptFrom(cc);
   }
@@ -655,7 +655,7 @@ Counters Engine::readCounters() {
   uint32_t c[COUNTER_WORDS];
   HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
   synchronize();
-  return Counters{c[C_PAIRS], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
+  return Counters{c[C_PAIR_DEMAND], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
                   c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
 }
 
@@ -1225,10 +1225,11 @@ bool Engine::adaptFrom(const Counters &c) {
     resetGraphs();
     changed = true;
   }
-  // Storage: grow to twice the demand when full beyond 80% or overflowed (solver.ts:1512-1518)
+  // Storage: grow to twice the demand (the pairs found, also past the capacity) when full beyond 80% or overflowed,
+  // and by at least a quarter (solver.ts:1512-1518)
   bool overflow = (c.overflow & 7u) != 0;
   if (overflow || c.pairs > 0.8 * pairCap_) {
-    uint32_t want = uint32_t(std::max<double>(2.0 * std::max(c.pairs, pairCap_), 1.25 * pairCap_));
+    uint32_t want = uint32_t(std::max<double>(2.0 * c.pairs, 1.25 * pairCap_));
     allocateContacts(want);
     changed = true;
   }

@@ -481,6 +481,7 @@ __global__ void k_bvh_write(DeviceWorld w, Bvh t) {
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q == 0) {
     uint32_t total = w.pairCount[w.nBodies];
+    w.counters[C_PAIR_DEMAND] = total;
     if (total > w.pairCapacity) {
       atomicOr(&w.counters[C_OVERFLOW], 1u);
       total = w.pairCapacity;

@@ -3,8 +3,8 @@
 // floor), the dual/penalty ramp over joints and contact pairs, and the BDF1 velocities.
 // Semantics: reference/three-avbd/src/avbd3d/gpu/wgsl-solve.ts (addRow 53-59, addJoint 90-140, evalContact 171-206,
 // warmStartJoints 210-226, warmStartBodies 259-300, primal/accumulate/finishBody 314-470, dual 474-535,
-// updateVelocities 537-549). A step whose contact storage overflowed is held (every kernel here leaves the world
-// untouched), as the reference's protectSolverStep does (src/gpu/gpuSolverKernels.js:72-103).
+// updateVelocities 537-549). A step whose contact storage overflowed or whose colouring clashed is held (every kernel
+// here leaves the world untouched), as the reference's protectSolverStep does (src/gpu/gpuSolverKernels.js:72-103).
 #include "engine_types.h"
 #include "kernels.h"
 #include "stepsetup.h"
@@ -265,10 +265,11 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
 }
 
 /**
- * Whether the dynamic body (colour ca, index a) finishes its pair with the body (colour cb, index b) in the dataflow
- * solve: the later colour runs the pair's dual right after its primal, ties (a clash) go to the higher index.
+ * Whether the dynamic body of colour ca finishes its pair with a dynamic body of colour cb in the dataflow solve: the
+ * later colour runs the pair's dual right after its primal. Partners never share a colour in a solved step (a step
+ * whose colouring clashed is held), so exactly one side finishes each pair.
  */
-AVBD_D bool finishesPair(uint32_t ca, uint32_t a, uint32_t cb, uint32_t b) { return ca > cb || (ca == cb && a > b); }
+AVBD_D bool finishesPair(uint32_t ca, uint32_t cb) { return ca > cb; }
 
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
 template <bool DF>
@@ -287,7 +288,7 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
       if (pc < df.color)
         waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
       else if (df.it != 0u)
-        waitFor(w, finishesPair(pc, head.x, df.color, i) ? &w.dualDone[head.x] : &w.bodyDone[head.x],
+        waitFor(w, finishesPair(pc, df.color) ? &w.dualDone[head.x] : &w.bodyDone[head.x],
                 df.base + df.it);
     }
   }
@@ -536,8 +537,8 @@ template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, cons
 }
 
 /**
- * The dataflow solve's dual for the pairs body i finishes (its partner is static, of an earlier colour, or of its
- * colour with a lower index), right after its primal: each such contact point is evaluated at the body's new pose
+ * The dataflow solve's dual for the pairs body i finishes (its partner is static or of an earlier colour), right after
+ * its primal: each such contact point is evaluated at the body's new pose
  * (pos, rot, held by every lane of the group) and the partner's pose of this iteration (an earlier colour's was waited
  * for by the primal), and updated as the dual pass does. The items are spread over the group's lanes as in
  * accumulate. Then the body's dual stamp.
@@ -560,7 +561,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
       head = w.adjEntry[2ull * e];
       tail = w.adjEntry[2ull * e + 1];
       if (!(head.z & ENTRY_JOINT) &&
-          (!(head.z & ENTRY_DYNAMIC) || finishesPair(df.color, i, w.colorA[head.x], head.x)))
+          (!(head.z & ENTRY_DYNAMIC) || finishesPair(df.color, w.colorA[head.x])))
         items = head.z & 15u;
     }
     uint32_t incl = items;
@@ -585,8 +586,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
                             __uint_as_float(__shfl(tail.z, owner, W)));
       const uint32_t pt = t - __shfl(excl, owner, W);
       if (t < total) {
-        // A partner of this colour (a clash: the colouring ran out of colours) runs beside this body and is read as
-        // it stands, as the primal reads it; waiting for it could wait for a unit queued behind this one
+        // The partner is static or of an earlier colour: its pose of this iteration was waited for by the primal
         const bool isA = (h.z & ENTRY_IS_A) != 0;
         const PairBody other = pairBody<true>(w, h.x);
         const uint32_t c = h.y + pt;
@@ -638,12 +638,12 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
  * contacts does not hold back the seven bodies sharing its wave.
  */
 __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint32_t color) {
-  // The hold flag and the colour's ranges are read together (neither waits for the other)
-  const uint32_t overflow = w.counters[C_OVERFLOW];
+  // The hold flags and the colour's ranges are read together (neither waits for the other)
+  const uint32_t overflow = w.counters[C_OVERFLOW], clashes = w.counters[C_CLASHES];
   const uint32_t heavyStart = w.colorStart[2 * color];
   const uint32_t lightStart = w.colorStart[2 * color + 1];
   const uint32_t lightEnd = w.colorStart[2 * color + 2];
-  if (overflow != 0u) return;
+  if ((overflow | clashes) != 0u) return;
   constexpr uint32_t heavyPerWave = WAVE / HEAVY_LANES, lightPerWave = WAVE / LANES;
   const uint32_t heavyWaves = (lightStart - heavyStart + heavyPerWave - 1) / heavyPerWave;
   const uint32_t waves = heavyWaves + (lightEnd - lightStart + lightPerWave - 1) / lightPerWave;

@@ -5,8 +5,12 @@
 
 namespace avbd {
 
-/** A step whose contact storage overflowed is held: the solve leaves the world untouched. */
-AVBD_D bool held(const DeviceWorld &w) { return w.counters[C_OVERFLOW] != 0u; }
+/**
+ * A step is held (the solve leaves bodies and joints untouched) when its contact storage overflowed or its colouring
+ * clashed (a body left pending after the rounds may share a colour with a neighbour), as protectSolverStep's
+ * solverStepValid() (src/gpu/gpuSolverKernels.js:72-103). So a solved step always has a proper colouring.
+ */
+AVBD_D bool held(const DeviceWorld &w) { return (w.counters[C_OVERFLOW] | w.counters[C_CLASHES]) != 0u; }
 
 /**
  * warmStartBodies (wgsl-solve.ts:259-300) for body i: the step-start pose, the inertial pose, and the initial guess

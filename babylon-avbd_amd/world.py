@@ -701,6 +701,13 @@ class HipWorld:
         self._alive()
         check(self._lib.avbd_world_set_coloring(self._h, int(colorCap), int(colorRounds)))
 
+    def getColoring(self):
+        """{colorCap, colorRounds, pairCapacity} in use (as setColoring and adapt() leave them)."""
+        self._alive()
+        cap, rounds, pairs = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._lib.avbd_world_get_coloring(self._h, ctypes.byref(cap), ctypes.byref(rounds), ctypes.byref(pairs)))
+        return dict(colorCap=cap.value, colorRounds=rounds.value, pairCapacity=pairs.value)
+
     def adapt(self, counters=None):
         """Grow storage and the colour cap from the last step's counters (solver.ts:1484-1519)."""
         self._alive()

@@ -173,6 +173,9 @@ int avbd_world_write_state(avbd_world *world, const float *refs, const uint32_t 
 
 /** Colour cap and Jones–Plassmann rounds (GpuSolver3D.colorCap / colorRounds). */
 int avbd_world_set_coloring(avbd_world *world, uint32_t color_cap, uint32_t color_rounds);
+/** The colour cap, Jones–Plassmann rounds and pair capacity in use (GpuSolver3D.colorCap / colorRounds /
+ * contactCapacity, solver.ts:384-386, as adapt() leaves them); any pointer may be NULL. */
+int avbd_world_get_coloring(avbd_world *world, uint32_t *color_cap, uint32_t *color_rounds, uint32_t *pair_capacity);
 /** adapt(counters) (solver.ts:1484-1519): grow storage and the colour cap from the last step's counters. */
 int avbd_world_adapt(avbd_world *world, int *changed);
 

@@ -180,6 +180,7 @@ void copyTopology(World<double> &d, const World<float> &f) {
   d.colorStart = f.colorStart;
   d.colorBodies = f.colorBodies;
   d.clashes = f.clashes;
+  d.overflow = f.overflow;
   d.numColors = f.numColors;
 }
 
@@ -215,6 +216,23 @@ void *gs_create(int precision, uint32_t nBodies, const float *bodyRecs, const ui
 }
 
 void gs_destroy(void *p) { delete static_cast<Handle *>(p); }
+
+/** Colour cap and Jones–Plassmann rounds from the next step on (the GPU world's setColoring / adapt). */
+void gs_set_coloring(void *p, uint32_t colorCap, uint32_t colorRounds) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->colorCap = colorCap;
+  h->f->colorRounds = colorRounds;
+  if (h->d) {
+    h->d->colorCap = colorCap;
+    h->d->colorRounds = colorRounds;
+  }
+}
+
+/** Pair capacity (0: unbounded): a step with more broadphase pairs overflows and is held, as on the GPU. */
+void gs_set_pair_capacity(void *p, uint32_t capacity) { static_cast<Handle *>(p)->f->pairCapacity = capacity; }
+
+/** 1 if the last step overflowed its pair capacity. */
+uint32_t gs_overflow(void *p) { return static_cast<Handle *>(p)->f->overflow; }
 
 void gs_set_brute_force(void *p, int on) {
   auto *h = static_cast<Handle *>(p);

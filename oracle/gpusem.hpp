@@ -164,6 +164,8 @@ template <class R> class World {
   std::vector<uint32_t> colorStart, colorBodies;
   uint32_t clashes = 0, numColors = 0;
   uint32_t pivotFallbacks = 0;  // bodies solved again in FP64 in the last step
+  // Pair storage as the GPU world sizes it (0: unbounded); a step with more pairs overflows and is held
+  uint32_t pairCapacity = 0, overflow = 0;
   // Global schedule
   uint32_t colorCap = 64, colorRounds = 16;
   int iterations = 10;
@@ -996,18 +998,23 @@ template <class R> class World {
       findPairs();
       narrowphase();
     }
+    overflow = (pairCapacity != 0 && pairs.size() > pairCapacity) ? 1u : 0u;
     buildAdjacency();
     colour();
   }
 
-  /** Warm starts, iterations × (primal per colour, dual), velocities. */
+  /**
+   * Warm starts, iterations × (primal per colour, dual), velocities. A step whose colouring clashed (bodies still
+   * pending after the rounds) or whose pairs overflowed the pair capacity is held: bodies and joints stay as they are
+   * (protectSolverStep's solverStepValid, src/gpu/gpuSolverKernels.js:72-103).
+   */
   void solve() {
+    if (clashes != 0 || overflow != 0) return;
     warmStartJoints();
     warmStartBodies();
     for (int it = 0; it < iterations; it++) {
       for (uint32_t c = 0; c < colorCap; c++)
-        // bodies still pending after the rounds (clashes) may share a colour with a neighbour: then all run in order
-#pragma omp parallel for schedule(dynamic, 64) if (clashes == 0)
+#pragma omp parallel for schedule(dynamic, 64)
         for (uint32_t s = colorStart[c]; s < colorStart[c + 1]; s++) {  // a colour's bodies are independent
           uint32_t i = colorBodies[s];
           R alpha = R(P(i).alpha);
@@ -1077,7 +1084,7 @@ template <class R> class World {
       const Manifold<R> &mf = manifolds[m];
       const uint32_t count = mf.count(), a = mf.a, b = mf.b;
       const R e = rmax(restitution[a], restitution[b]);
-      if (count == 0 || e <= 0 || (mf.countw & 0x80000000u) != 0 || clashes != 0) continue;
+      if (count == 0 || e <= 0 || (mf.countw & 0x80000000u) != 0 || clashes != 0 || overflow != 0) continue;
       const Body<R> &A = bodies[a], &B = bodies[b];
       V3<R> ra(0, 0, 0), rb(0, 0, 0);
       for (uint32_t c = 0; c < count; c++) {
@@ -1375,7 +1382,7 @@ template <class R> class World {
       }
       uint32_t *g = globalsOf(i);
       const uint32_t islandWake = sleepIslands ? sleep[sleepRoot(i)].wake : 0u;
-      if ((islandWake & SW_IMPACT) || g[0] != 0 || clashes != 0) {
+      if ((islandWake & SW_IMPACT) || g[0] != 0 || clashes != 0 || overflow != 0) {
         wakeOne(i);
         continue;
       }

@@ -53,6 +53,9 @@ def lib():
                                i32p, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
             "gs_destroy": (None, [vp]),
             "gs_set_brute_force": (None, [vp, ctypes.c_int]),
+            "gs_set_coloring": (None, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+            "gs_set_pair_capacity": (None, [vp, ctypes.c_uint32]),
+            "gs_overflow": (ctypes.c_uint32, [vp]),
             "gs_set_state": (None, [vp, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
             "gs_step": (None, [vp, ctypes.c_int]),
             "gs_counts": (None, [vp, u32p]),
@@ -176,6 +179,7 @@ class OracleWorld:
         self._brute = brute_force
         if brute_force:
             L.gs_set_brute_force(self.h, 1)
+        self._pair_capacity = 0
         self._restitution = None
         self._sleeping = None
         bouncy, base = [], 0
@@ -207,6 +211,8 @@ class OracleWorld:
         L.gs_destroy(old)
         if self._brute:
             L.gs_set_brute_force(self.h, 1)
+        if self._pair_capacity:
+            L.gs_set_pair_capacity(self.h, self._pair_capacity)
         refs = np.ascontiguousarray(state["refs"], np.float32)
         colors = np.ascontiguousarray(state["colors"], np.uint32)
         mf = np.ascontiguousarray(state["manifolds"], np.uint32)
@@ -265,11 +271,22 @@ class OracleWorld:
     def step(self, n=1):
         lib().gs_step(self.h, n)
 
+    def set_coloring(self, color_cap, color_rounds):
+        """Colour cap and rounds from the next step on (also after a later seed())."""
+        self._cap, self._rounds = int(color_cap), int(color_rounds)
+        lib().gs_set_coloring(self.h, self._cap, self._rounds)
+
+    def set_pair_capacity(self, capacity):
+        """Pair storage like the GPU world's (0: unbounded): a step with more pairs overflows and is held."""
+        self._pair_capacity = int(capacity)
+        lib().gs_set_pair_capacity(self.h, self._pair_capacity)
+
     def counts(self):
         c = np.zeros(8, np.uint32)
-        lib().gs_counts(self.h, P(c, ctypes.c_uint32))
+        L = lib()
+        L.gs_counts(self.h, P(c, ctypes.c_uint32))
         return dict(pairs=int(c[0]), manifolds=int(c[1]), contacts=int(c[2]), clashes=int(c[3]), colors=int(c[4]),
-                    pivotFallbacks=int(c[7]))
+                    pivotFallbacks=int(c[7]), overflow=int(L.gs_overflow(self.h)))
 
     def bodies(self):
         out = np.zeros((self.n, 40))

@@ -382,3 +382,33 @@ def test_colour_parallel_variant_equals_in_order(avbd, cap):
     np.testing.assert_array_equal(b1, b4)
     np.testing.assert_array_equal(j1, j4)
     np.testing.assert_array_equal(c1, c4)
+
+
+def _low_pile():
+    import importlib
+    avbd = importlib.import_module("babylon-avbd_amd")
+    return avbd.configs.box_pile(counts=(4, 2, 4), centre=(0.0, 1.2, 0.0), seed=5)
+
+
+@pytest.mark.parametrize("failure", ["clashes", "overflow"])
+def test_failed_step_is_held(failure):
+    """protectSolverStep (src/gpu/gpuSolverKernels.js:72-103): a step whose colouring clashes (one colour for touching
+    bodies) or whose pairs exceed the pair capacity leaves every body as it was; the same step with room solves."""
+    free = OracleWorld([_low_pile()], precision=32, color_cap=16, color_rounds=16)
+    tight = OracleWorld([_low_pile()], precision=32, color_cap=1 if failure == "clashes" else 16, color_rounds=2)
+    if failure == "overflow":
+        tight.set_pair_capacity(3)
+    for w in (free, tight):
+        w.step(6)  # the pile lands: contacts between the boxes and with the ground
+    before = tight.bodies().copy()
+    free.seed(dict(bodies=before.astype(np.float32), joints=tight.joints().astype(np.float32), refs=tight.refs(),
+                   colors=tight.colors(), manifolds=tight.manifolds(), contacts=tight.contact_words(),
+                   steps=np.array([6], np.uint32)))
+    tight.step()
+    free.step()
+    c = tight.counts()
+    assert c[failure] > 0
+    assert free.counts()["clashes"] == 0 and free.counts()["overflow"] == 0
+    held = tight.bodies()
+    assert np.array_equal(held[:, np.r_[0:8, 32:40]], before[:, np.r_[0:8, 32:40]])
+    assert not np.array_equal(free.bodies()[:, 0:8], before[:, 0:8])
