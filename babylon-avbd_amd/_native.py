@@ -167,6 +167,7 @@ SIGNATURES = {
     "avbd_world_sleep_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, u32p]),
     "avbd_world_read_sleep": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p]),
     "avbd_world_cast": (ctypes.c_int, [ctypes.c_void_p, f32p, ctypes.c_uint32, ctypes.c_void_p, f32p]),
+    "avbd_world_cast_noalloc": (ctypes.c_int, [ctypes.c_void_p, f32p, ctypes.c_uint32, ctypes.c_void_p, f32p]),
     "avbd_world_watch_contacts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "avbd_world_set_watched": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32]),
     "avbd_world_reset_contact_history": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),

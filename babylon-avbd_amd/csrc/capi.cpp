@@ -591,6 +591,26 @@ int avbd_world_cast(avbd_world *world, const float *queries, uint32_t n, const A
   });
 }
 
+int avbd_world_cast_noalloc(avbd_world *world, const float *queries, uint32_t n, const AvbdQueryOptions *options,
+                            float *hits) {
+  CHECK_WORLD(world);
+  return guarded([&] {
+    avbd::QueryFilter f{};
+    f.mask = 0xffffffffu;
+    if (options) {
+      if (options->ignore_count > 16 || (options->ignore_count && !options->ignore))
+        throw std::invalid_argument("Queries can ignore up to 16 live body indices");
+      for (uint32_t k = 0; k < options->ignore_count; k++) f.ignore[k] = options->ignore[k];
+      f.ignoreCount = options->ignore_count;
+      f.mask = options->collides_with;
+      f.membership = options->membership;
+      f.useMembership = options->use_membership ? 1u : 0u;
+    }
+    world->engine->castNoAlloc(queries, n, f, hits);
+    return AVBD_OK;
+  });
+}
+
 int avbd_world_watch_contacts(avbd_world *world, uint32_t max_pairs, uint32_t max_events) {
   CHECK_WORLD(world);
   return guarded([&] {

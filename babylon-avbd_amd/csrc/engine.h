@@ -158,6 +158,9 @@ class Engine {
    * body index bits (~0: none), 0, 0, 0).
    */
   void cast(const float *queries, uint32_t n, const QueryFilter &filter, float *hits);
+  /** cast() without allocating: the queries' records, best words and hits staging live in a fixed device array shared
+   * by every call on the device (at most NOALLOC_MAX_QUERIES queries per call; overlapping calls on one device race). */
+  void castNoAlloc(const float *queries, uint32_t n, const QueryFilter &filter, float *hits);
   /** Contact events (ContactWatch): storage for maxPairs touching pairs per step and maxEvents unread events, made on
    * the first call (every body watched); later calls keep it. */
   void watchContacts(uint32_t maxPairs, uint32_t maxEvents);

@@ -1066,6 +1066,49 @@ void Engine::cast(const float *queries, uint32_t n, const QueryFilter &filter, f
   synchronize();
 }
 
+void Engine::castNoAlloc(const float *queries, uint32_t n, const QueryFilter &filter, float *hits) {
+  if (n == 0) return;
+  if (!queries || !hits) throw std::invalid_argument("queries or hits is NULL");
+  if (n > NOALLOC_MAX_QUERIES) throw std::invalid_argument("castNoAlloc takes at most 16384 queries per call");
+  if (filter.ignoreCount > 16) throw std::invalid_argument("Queries can ignore up to 16 live body indices");
+  for (uint32_t k = 0; k < filter.ignoreCount; k++)
+    if (filter.ignore[k] >= nBodies_) throw std::invalid_argument("Queries can ignore up to 16 live body indices");
+  for (uint32_t k = 0; k < n; k++) {  // every query is checked before anything runs
+    const float *q = queries + 8ull * k;
+    bool ok = true;
+    for (int c = 0; c < 8; c++) ok = ok && std::isfinite(q[c]);
+    const double len = std::sqrt(double(q[4]) * q[4] + double(q[5]) * q[5] + double(q[6]) * q[6]);
+    if (!ok || len == 0.0) throw std::invalid_argument("A query requires 3 finite components and a nonzero direction");
+    if (q[3] < 0.f || q[7] < 0.f)
+      throw std::invalid_argument("Query radius and maximum distance must be finite and nonnegative");
+  }
+  HIP_CHECK(hipSetDevice(device_));
+  uint8_t *base = static_cast<uint8_t *>(query_scratch(device_));
+  QueryRec *recs = reinterpret_cast<QueryRec *>(base);
+  uint32_t *best = reinterpret_cast<uint32_t *>(base + size_t(NOALLOC_MAX_QUERIES) * sizeof(QueryRec));
+  float4 *out = reinterpret_cast<float4 *>(base + size_t(NOALLOC_MAX_QUERIES) * (sizeof(QueryRec) + 2 * sizeof(uint32_t)));
+  // The best words start cleared in every call (the array keeps whatever the last call on this device left)
+  launch_query_reset(best, n, stream_);
+  // Records normalised as cast() does, through a fixed host chunk: each chunk's copy completes before it is refilled
+  constexpr uint32_t CHUNK = 256;
+  QueryRec chunk[CHUNK];
+  for (uint32_t k0 = 0; k0 < n; k0 += CHUNK) {
+    const uint32_t m = std::min(CHUNK, n - k0);
+    for (uint32_t k = 0; k < m; k++) {
+      const float *q = queries + 8ull * (k0 + k);
+      const double len = std::sqrt(double(q[4]) * q[4] + double(q[5]) * q[5] + double(q[6]) * q[6]);
+      chunk[k].origin = make_float4(q[0], q[1], q[2], q[3]);
+      chunk[k].direction = make_float4(float(q[4] / len), float(q[5] / len), float(q[6] / len), q[7]);
+    }
+    HIP_CHECK(hipMemcpyAsync(recs + k0, chunk, size_t(m) * sizeof(QueryRec), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  launch_queries(base_, recs, n, filter, best, out, stream_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(hits, out, size_t(n) * 2 * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
 void Engine::watchContacts(uint32_t maxPairs, uint32_t maxEvents) {
   if (maxPairs < 1 || maxEvents < 1) throw std::invalid_argument("Contact capacities must be positive integers");
   if (cw_.table) return;

@@ -53,6 +53,13 @@ void launch_restitution_after(const DeviceWorld &w, hipStream_t s);
 /** Shape queries: every (body, query) pair traced; hits = 2 float4 per query (normal, distance or -1; index bits).
  * `best` holds 2 words per query, reset by launch_query_reset before first use and left reset by each batch. */
 void launch_query_reset(uint32_t *best, uint32_t capacity, hipStream_t s);
+/** Queries per castNoAlloc call: their records (32 B), best words (8 B) and hits (32 B) live in a file-scope device
+ * array of queries.hip (NOALLOC_QUERY_BYTES per query), one copy per device, shared by every call on that device. */
+constexpr uint32_t NOALLOC_MAX_QUERIES = 16384;
+constexpr uint32_t NOALLOC_QUERY_BYTES = sizeof(QueryRec) + 2 * sizeof(uint32_t) + 2 * sizeof(float4);
+/** The device address of that array on `device` (looked up once per device, with the device made current); throws
+ * std::runtime_error when the lookup fails. */
+void *query_scratch(int device);
 void launch_queries(const DeviceWorld &w, const QueryRec *queries, uint32_t n, const QueryFilter &f, uint32_t *best,
                     float4 *hits, hipStream_t s);
 

@@ -4,6 +4,10 @@
 // body index at that distance; a third traces the winner again for its normal. Boxes are tested as rounded boxes
 // (offset faces and edge capsules, so a sphere cast never hits a corner of an inflated bounding box), spheres and
 // capsules analytically. A query that starts inside or touching a shape hits it at distance 0 with normal −direction.
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "kernels.h"
 
 namespace avbd {
@@ -234,6 +238,31 @@ __global__ void k_query_reset(uint32_t *best, uint32_t n) {
   if (q >= n) return;
   best[2 * q] = 0x7f800000u;
   best[2 * q + 1] = 0xffffffffu;
+}
+
+// The scratch of castNoAlloc: NOALLOC_MAX_QUERIES query records, then their best words, then their hits. Each
+// device holds its own copy at its own address until the process ends; host code reaches it only through the address
+// query_scratch() looked up for that device (the array's name is not a device address in host code).
+static __device__ __align__(256) uint8_t scratch[size_t(NOALLOC_MAX_QUERIES) * NOALLOC_QUERY_BYTES];
+
+void *query_scratch(int device) {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex mu;
+  static void *address[MAX_DEVICES] = {};
+  if (device < 0 || device >= MAX_DEVICES) throw std::runtime_error("device ordinal out of range for query scratch");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!address[device]) {
+    int current = 0;
+    if (hipGetDevice(&current) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+    if (current != device && hipSetDevice(device) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
+    void *ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) (void)hipSetDevice(current);
+    if (err != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(err));
+    address[device] = ptr;
+  }
+  return address[device];
 }
 
 void launch_query_reset(uint32_t *best, uint32_t capacity, hipStream_t s) {

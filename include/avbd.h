@@ -294,6 +294,15 @@ typedef struct AvbdQueryOptions {
 int avbd_world_cast(avbd_world *world, const float *queries, uint32_t n, const AvbdQueryOptions *options,
                     float *hits);
 
+/** avbd_world_cast without allocating device memory: no counterpart in the reference; for callers that cannot allocate
+ * (tight serving loops, code that must not call hipMalloc). The queries' records, nearest-hit words and hits staging
+ * live in one fixed device array per GPU (AVBD_NOALLOC_MAX_QUERIES queries, 1.125 MiB), held until the process ends
+ * and shared by every call on that GPU: calls on one GPU that overlap (from several threads) race. n above the
+ * maximum is an AVBD_ERR_INVALID error. Same arguments and results as avbd_world_cast. */
+#define AVBD_NOALLOC_MAX_QUERIES 16384u
+int avbd_world_cast_noalloc(avbd_world *world, const float *queries, uint32_t n, const AvbdQueryOptions *options,
+                            float *hits);
+
 /* ---- Contact events (src/vendor/three-avbd-lib/contacts.ts ContactWatch; src/native/solver3D.js:252-311) -------- */
 
 /** watchContacts({maxPairs, maxEvents}): event storage, made once (every body watched); after each step a pair with a
