@@ -21,18 +21,33 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
+SETTLE_STEPS = 300  # the 100k-box pile lands and settles in these steps; every timed window starts after them
+
+
 def traffic_profile(kernel):
-    return os.path.join("profiles", f"{kernel}_100k_boxes_traffic.json")
+    return os.path.join("profiles", f"{kernel}_100k_boxes_settled_counters.json")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the 100k-box world, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    measurement (calibrated against a copy kernel of known size), or None."""
+def measured_counters(kernel):
+    """The committed rocprofv3 counters of `kernel` on the settled 100k-box world (the state every timed window of
+    this bench starts from): HBM bytes per launch from FETCH_SIZE/WRITE_SIZE calibrated against a copy kernel of known
+    size, and the wave, VALU and wait counters; None when absent."""
     try:
         with open(os.path.join(ROOT, traffic_profile(kernel))) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
+            return json.load(f)
+    except (OSError, ValueError):
         return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def primal_bytes_per_iteration(world):
@@ -129,7 +144,7 @@ def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
     steps_1, dt_1 = timed(1)
     what = (f"of the same {n}-body world from the GPU's state (the timed window's start), FP32 restatement of the "
             f"step (grid broadphase, colour-ordered Gauss-Seidel)")
-    return dict(value=n * steps_n / dt_n, unit="body-steps/s", cores=cores, kind="port",
+    return dict(value=n * steps_n / dt_n, unit="body-steps/s", cores=cores, kind="port", cpu=cpu_model(),
                 sample=f"steps {first_step + 1}-{first_step + steps_n} {what}, {cores} OpenMP threads "
                        f"(colour-parallel), {dt_n:.1f} s",
                 single_thread=dict(value=n * steps_1 / dt_1, unit="body-steps/s", cores=1,
@@ -308,7 +323,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed steps after the pile has settled (the first 300 steps always run before them)")
+    ap.add_argument("--settle", type=int, default=SETTLE_STEPS,
+                    help="steps run before the warm-up so the timed window is the settled pile (default 300)")
     ap.add_argument("--workload", default="large", choices=["large", "pile1000"])
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -352,7 +370,7 @@ def main():
               "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
     if args.sleeping_only:
-        print(json.dumps(sleeping_pile(importlib.import_module("babylon-avbd_amd"), args.steps, args.warmup,
+        print(json.dumps(sleeping_pile(importlib.import_module("babylon-avbd_amd"), args.steps, args.settle + args.warmup,
                                        args.solver, not args.no_graph)))
         return
 
@@ -374,7 +392,9 @@ def main():
                 "bvh": {"rebuildInterval": args.bvh_rebuild}})
     print(f"# world: {n_bodies} bodies, {world.deviceBytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
 
-    # Warm-up: the pile falls and lands; the colour/capacity policy adapts from counters read back asynchronously
+    # Settling: the pile falls and lands (steps 1..settle, always run, whatever the warm-up count); the colour and
+    # capacity policy adapts from counters read back asynchronously. Then the warm-up steps, then the timed window.
+    settle_rate = None
     if args.state and os.path.exists(args.state):
         saved = dict(np.load(args.state))
         world.writeState(saved)
@@ -382,14 +402,25 @@ def main():
         print(f"# state loaded from {args.state}", file=sys.stderr, flush=True)
     else:
         done = 0
-        while done < args.warmup:
-            k = min(50, args.warmup - done)
+        t_settle = time.perf_counter()
+        while done < args.settle:
+            k = min(50, args.settle - done)
             world.step(k)
             world.synchronize()
             done += k
-            print(f"# warm-up {done}: {world.readCounters()}", file=sys.stderr, flush=True)
+            print(f"# settling {done}: {world.readCounters()}", file=sys.stderr, flush=True)
+        if args.settle:
+            settle_rate = n_bodies * args.settle / (time.perf_counter() - t_settle)
         if args.state:
             np.savez(args.state, colorsUsed=world.readCounters()["colors"], **world.readState())
+    done = 0
+    while done < args.warmup:
+        k = min(50, args.warmup - done)
+        world.step(k)
+        world.synchronize()
+        done += k
+        print(f"# warm-up {args.settle + done}: {world.readCounters()}", file=sys.stderr, flush=True)
+    first = args.settle + args.warmup  # steps taken before the timed window
     counters = world.readCounters()
     state0 = None if args.no_cpu else world.readState()
 
@@ -427,15 +458,27 @@ def main():
         launch_ms = prof["primal"] / max(prof["primalLaunches"], 1)
         bytes_note = {"primal_bytes_per_iteration": bytes_iter}
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    # HBM traffic and the VALU issue floor of the same kernel in the same (settled) state, from the committed counters
+    traffic, traffic_note, valu = None, "not measured for this workload or window", None
+    measured = measured_counters(kernel) if (args.workload == "large" and args.settle == SETTLE_STEPS
+                                             and args.solver == "dataflow") else None
+    if measured:
+        traffic = measured["hbm_bytes_per_launch"]
+        traffic_note = (f"{traffic_profile(kernel)}: rocprofv3 2 x FETCH_SIZE + WRITE_SIZE per launch (copy-kernel "
+                        f"calibrated) on the settled pile, {measured['window']}")
+        v = measured["valu"]
+        valu = {"bound": "valu-issue", "insts_per_launch": v["insts_per_launch"], "floor_ms": v["floor_ms"],
+                "frac": v["floor_ms"] / launch_ms, "wait_fraction": v["wait_fraction"],
+                "waves_per_simd": v["waves_per_simd"], "source": traffic_profile(kernel)}
     print(f"# profile: {json.dumps(prof)}; {kernel} bytes/launch {launch_bytes} {parts}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene_fn, state0, args.warmup, args.cpu_steps, args.cpu_budget, 64)
+        cpu = cpu_baseline(scene_fn, state0, first, args.cpu_steps, args.cpu_budget, 64)
 
     world.destroy()
     mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
-    sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, args.warmup, args.solver)
+    sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, first, args.solver)
     served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
                                                   args.serve_active)
     batched = None
@@ -457,7 +500,11 @@ def main():
         "data": "synthetic (seeded)",
         "config": {"workload": workload, "bodies": n_bodies, "iterations": iters, "dt": scene.params["dt"],
                    "broadphase": {"auto": "auto (H-PLOC BVH: one world of >= 10,000 bodies)", "grid": "uniform grid",
-                                  "hploc": "H-PLOC BVH"}[args.broadphase], "steps_timed": f"{args.warmup + 1}-{args.warmup + args.steps}",
+                                  "hploc": "H-PLOC BVH"}[args.broadphase],
+                   "steps_timed": f"{first + 1}-{first + args.steps}",
+                   "window": f"the settled pile: steps 1-{args.settle} settle and {args.warmup} warm-up steps run "
+                             "untimed first",
+                   "settling_body_steps_per_s": settle_rate,
                    "counters": counters},
         "roofline": {
             "bound": "hbm",
@@ -465,14 +512,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(kernel) if args.workload == "large" else None,
-            "traffic_source": f"{traffic_profile(kernel)} (rocprofv3 2 x FETCH_SIZE + WRITE_SIZE per launch, "
-                              "copy-kernel calibrated)",
+            "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": kernel_desc,
             "bytes_per_launch": launch_bytes,
             **bytes_note,
             "launches_per_iteration": launches,
             "avg_launch_ms": launch_ms,
+            "valu": valu,
         },
         "phases_ms": prof,
         "cpu_baseline": cpu,
