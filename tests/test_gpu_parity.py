@@ -77,6 +77,53 @@ def compare_step(world, oracle, exact=True):
     return dict(pairs=len(pg), manifolds=len(mg), features=features)
 
 
+def _sorted_manifolds(mf, ct):
+    """Manifolds sorted by (a, b) and, in that order, each one's contact words (vectorised, for large worlds)."""
+    mf = np.asarray(mf, np.uint32).reshape(-1, 8)
+    ct = np.asarray(ct, np.uint32).reshape(-1, 16)
+    key = (mf[:, 0].astype(np.uint64) << np.uint64(32)) | mf[:, 1].astype(np.uint64)
+    order = np.argsort(key, kind="stable")
+    mf = mf[order]
+    counts = (mf[:, 3] & 15).astype(np.int64)
+    firsts = mf[:, 2].astype(np.int64)
+    total = int(counts.sum())
+    within = np.arange(total) - np.repeat(np.cumsum(counts) - counts, counts)
+    return key[order], mf, ct[np.repeat(firsts, counts) + within]
+
+
+def compare_step_arrays(world, oracle):
+    """compare_step for worlds of any size: the same checks, vectorised (pairs, manifold pairs, count/generation words,
+    normals and frictions, feature keys and colours bit-exact; poses, velocities, λ and penalties within 1e-5)."""
+    def pair_keys(p):
+        p = np.asarray(p, np.uint32).reshape(-1, 2)
+        return np.sort((p[:, 0].astype(np.uint64) << np.uint64(32)) | p[:, 1].astype(np.uint64))
+    pg, pc = pair_keys(world.readPairs()), pair_keys(oracle.pairs())
+    assert np.array_equal(pg, pc), f"pair sets differ: {len(pg)} vs {len(pc)} pairs"
+    mf, ct = world.readContactRecords()
+    kg, mg, cg = _sorted_manifolds(mf, ct)
+    kc, mc, cc = _sorted_manifolds(oracle.manifolds(), oracle.contact_words())
+    assert np.array_equal(kg, kc), f"manifold pair sets differ: {len(kg)} vs {len(kc)}"
+    bad = np.flatnonzero(mg[:, 3] != mc[:, 3])
+    assert not len(bad), f"{len(bad)} manifolds differ in count/generation, first {mg[bad[0], :2]}"
+    bad = np.flatnonzero((mg[:, 4:8] != mc[:, 4:8]).any(1))
+    assert not len(bad), f"{len(bad)} manifolds differ in normal/friction bits, first {mg[bad[0], :2]}"
+    assert np.array_equal(cg[:, 3], cc[:, 3]), f"{int((cg[:, 3] != cc[:, 3]).sum())} feature keys differ"
+    colg, colc = world.readColors(), oracle.colors()
+    assert np.array_equal(colg, colc), f"colours differ at {np.flatnonzero(colg != colc)[:8]}"
+    bg = world.readBodies().astype(np.float64)
+    bc = oracle.bodies()
+    _close(bg[:, 0:3], bc[:, 0:3], 1.0, "positions")
+    _close(bg[:, 4:8], bc[:, 4:8], 1.0, "rotations")
+    _close(bg[:, 32:35], bc[:, 32:35], 1.0, "velocities")
+    _close(bg[:, 36:39], bc[:, 36:39], 1.0, "angular velocities")
+    fg, fc = cg.view(np.float32).astype(np.float64), cc.view(np.float32).astype(np.float64)
+    _close(fg[:, 12:15], fc[:, 12:15], 1.0, "lambda")
+    _close(fg[:, 8:11], fc[:, 8:11], 1.0, "penalty")
+    assert world.readCounters()["pivotFallbacks"] == oracle.counts()["pivotFallbacks"], "FP64 pivot fallbacks differ"
+    exact = bool(np.array_equal(bg[:, 0:8].astype(np.float32), bc[:, 0:8].astype(np.float32)))
+    return dict(pairs=len(pg), manifolds=len(kg), contacts=len(cg), exact_poses=exact)
+
+
 def _world(avbd, scene, cap=32, rounds=16, **opts):
     return avbd.createSolverPipeline("hip").createSolver(scene, dict(colorCap=cap, colorRounds=rounds, **opts))
 
