@@ -472,6 +472,11 @@ def main():
                 "waves_per_simd": v["waves_per_simd"], "source": traffic_profile(kernel)}
     print(f"# profile: {json.dumps(prof)}; {kernel} bytes/launch {launch_bytes} {parts}", file=sys.stderr, flush=True)
 
+    # Long-horizon statistics of the world where the run ends (rest heights, energy, penetration)
+    mf_end, ct_end = world.readContactRecords()
+    statistics = dict(step=first + args.steps + args.profile_steps,
+                      **avbd.stats.world_statistics(world.readBodies(), mf_end, ct_end, scene.up,
+                                                    scene.params["gravity"]))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene_fn, state0, first, args.cpu_steps, args.cpu_budget, 64)
@@ -505,7 +510,7 @@ def main():
                    "window": f"the settled pile: steps 1-{args.settle} settle and {args.warmup} warm-up steps run "
                              "untimed first",
                    "settling_body_steps_per_s": settle_rate,
-                   "counters": counters},
+                   "counters": counters, "statistics": statistics},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

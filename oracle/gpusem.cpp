@@ -228,6 +228,20 @@ void gs_set_coloring(void *p, uint32_t colorCap, uint32_t colorRounds) {
   }
 }
 
+/**
+ * The first part of a step only, for checks of the warm start: stage 1 runs collision and topology (the contacts as
+ * the narrowphase warm-started them, before any solve), stage 2 also the joint and body warm starts (each body's
+ * initial guess). FP32 worlds only.
+ */
+void gs_partial_step(void *p, int stage) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->collisionAndTopology();
+  if (stage >= 2) {
+    h->f->warmStartJoints();
+    h->f->warmStartBodies();
+  }
+}
+
 /** Pair capacity (0: unbounded): a step with more broadphase pairs overflows and is held, as on the GPU. */
 void gs_set_pair_capacity(void *p, uint32_t capacity) { static_cast<Handle *>(p)->f->pairCapacity = capacity; }
 

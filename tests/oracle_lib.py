@@ -56,6 +56,7 @@ def lib():
             "gs_set_coloring": (None, [vp, ctypes.c_uint32, ctypes.c_uint32]),
             "gs_set_pair_capacity": (None, [vp, ctypes.c_uint32]),
             "gs_overflow": (ctypes.c_uint32, [vp]),
+            "gs_partial_step": (None, [vp, ctypes.c_int]),
             "gs_set_state": (None, [vp, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
             "gs_step": (None, [vp, ctypes.c_int]),
             "gs_counts": (None, [vp, u32p]),
@@ -270,6 +271,10 @@ class OracleWorld:
 
     def step(self, n=1):
         lib().gs_step(self.h, n)
+
+    def partial_step(self, stage):
+        """Stage 1: collision and topology only (contacts as warm-started); stage 2: also the warm starts."""
+        lib().gs_partial_step(self.h, int(stage))
 
     def set_coloring(self, color_cap, color_rounds):
         """Colour cap and rounds from the next step on (also after a later seed())."""
