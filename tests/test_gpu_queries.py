@@ -49,12 +49,17 @@ def test_casts_match_the_oracle(avbd, radius):
     filters[idx, 0], filters[idx, 1] = 2, 4
     world.setFilters(idx.tolist(), [2] * len(idx), [4] * len(idx))
     q = queries(2000, radius, 5 + int(radius * 10))
+    # The oracle traces the directions the boundary traces: avbd_world_cast normalises each direction again (in
+    # double precision), which moves about one direction in 2,000 by one ulp even though q is already normalised
+    qo = q.copy()
+    d64 = q[:, 4:7].astype(np.float64)
+    qo[:, 4:7] = d64 / np.sqrt(d64[:, 0] * d64[:, 0] + d64[:, 1] * d64[:, 1] + d64[:, 2] * d64[:, 2])[:, None]
     bodies = world.readBodies()
     import ctypes
     from importlib import import_module
     native = import_module("babylon-avbd_amd._native")
     for opts in (dict(), dict(mask=1), dict(mask=0xFFFFFFFF, membership=4), dict(ignore=[1, 2, 3, 40])):
-        want = cast(bodies, filters, q, ignore=opts.get("ignore", ()), mask=opts.get("mask", 0xFFFFFFFF),
+        want = cast(bodies, filters, qo, ignore=opts.get("ignore", ()), mask=opts.get("mask", 0xFFFFFFFF),
                     membership=opts.get("membership"))
         got = np.zeros_like(want)
         ign = np.ascontiguousarray(opts.get("ignore", []), np.uint32)
@@ -63,16 +68,15 @@ def test_casts_match_the_oracle(avbd, radius):
                                 int(opts.get("membership") is not None))
         native.check(world._lib.avbd_world_cast(world._h, native.ptr(q, ctypes.c_float), len(q), ctypes.byref(o),
                                                 native.ptr(got, ctypes.c_float)))
-        # the hit body (index) must be the same; distance and normal within 1e-6 (relative for the distance), and
-        # nearly every hit the same bits (rare one-ulp differences in a normal have been seen, about 1 in 8,000)
+        # every hit the same bits: body index, distance and normal
         gi, wi = got[:, 4].view(np.uint32), want[:, 4].view(np.uint32)
         assert np.array_equal(gi, wi), f"{opts}: hit bodies differ at {np.nonzero(gi != wi)[0][:5].tolist()}"
         hit = want[:, 3] >= 0
         assert np.array_equal(got[:, 3] >= 0, hit)
         assert np.allclose(got[hit, 3], want[hit, 3], rtol=1e-6, atol=0)
         assert np.abs(got[hit, 0:3] - want[hit, 0:3]).max(initial=0) <= 1e-6
-        same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=1).mean()
-        assert same >= 0.999, f"{opts}: only {same:.4f} of the hits are bit-identical"
+        same = (got.view(np.uint32) == want.view(np.uint32)).all(axis=1)
+        assert same.all(), f"{opts}: {int((~same).sum())} hits differ in bits, first at {np.flatnonzero(~same)[:5]}"
         assert (want[:, 3] >= 0).sum() > 500
     world.destroy()
 
