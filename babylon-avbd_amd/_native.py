@@ -146,6 +146,15 @@ SIGNATURES = {
     "avbd_world_write_state": (ctypes.c_int, [ctypes.c_void_p, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
     "avbd_world_set_coloring": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "avbd_world_adapt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "avbd_world_release_joints": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),
+    "avbd_world_disable_constraint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "avbd_world_set_world_anchor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, f32p]),
+    "avbd_world_set_motor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, f32p, f32p]),
+    "avbd_world_set_spring_material": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
+                                                      ctypes.c_float]),
+    "avbd_world_set_angular_constraint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, f32p, f32p,
+                                                         ctypes.c_float, ctypes.c_float]),
+    "avbd_world_joint_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]),
     "avbd_world_get_coloring": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),

@@ -30,7 +30,9 @@ constexpr uint32_t STICK_BIT = 0x80000000u;
 constexpr uint32_t SPHERE_FEATURE = 3u << 24;
 constexpr uint32_t CAPSULE_FEATURE = 4u << 24;
 constexpr uint32_t MAX_COLORS = 64, NO_COLOR = 255, PENDING = 256;
-constexpr int32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2;
+// Constraint types of the joint records' info word: a released slot, a ball joint (+ angle lock), a spring, and the
+// two angular constraints about a hinge axis (a motor driving the twist speed, a limit keeping the twist in a range)
+constexpr int32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, T_MOTOR = 3, T_LIMIT = 4;
 constexpr float SHAPE_BOX = 0.f, SHAPE_SPHERE = 1.f, SHAPE_SAIL = 2.f, SHAPE_CAPSULE = -1.f;
 constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
                    FLAG_MASS_PENALTY = 16;
@@ -74,15 +76,24 @@ struct Body {
 };
 static_assert(sizeof(Body) == 160, "Body record is 160 bytes");
 
+/**
+ * A constraint record (layout.ts:27-36). Ball joint: penalties, multipliers, C0 and anchors as named; the rest
+ * orientation of an angle-locked joint (B relative to A when it was made) is the quaternion (c0Lin.w, c0Ang.w, rA.w,
+ * rB.w) (gpuSolverKernels.js:105-130). Spring: stiffness in penLin.w, rest length in rA.w, break strain in lamLin.w and
+ * break force in lamAng.w (0: unbreakable; springMaterial.js). Motor / limit (angularConstraints.js): row penalties
+ * penLin.xy (penLin.w 3e38 while enabled), multipliers lamLin.xy, hinge axes rA.xyz (in A) and rB.xyz (in B),
+ * reference tangents c0Lin.xyz (in A) and c0Ang.xyz (in B), warm-start values c0Lin.w / c0Ang.w, and rA.w / rB.w the
+ * motor's speed and torque bound or the limit's lower and upper angle.
+ */
 struct Joint {
   float4 penLin;  // xyz, w: linear stiffness (>= HARD: hard); spring: its stiffness
   float4 penAng;  // xyz, w: angular stiffness
-  float4 lamLin;  // xyz, w: fracture threshold (negative: breaks on pull too)
-  float4 lamAng;  // xyz, w: torque arm
-  float4 c0Lin;
-  float4 c0Ang;
-  float4 rA;      // xyz (world point when A is the world), w: spring rest length
-  float4 rB;
+  float4 lamLin;  // xyz, w: fracture threshold (negative: breaks on pull too); spring: break strain
+  float4 lamAng;  // xyz, w: torque arm; spring: break force
+  float4 c0Lin;   // xyz, w: rest orientation x
+  float4 c0Ang;   // xyz, w: rest orientation y
+  float4 rA;      // xyz (world point when A is the world), w: spring rest length / rest orientation z
+  float4 rB;      // xyz, w: rest orientation w
 };
 static_assert(sizeof(Joint) == 128, "Joint record is 128 bytes");
 
@@ -171,6 +182,41 @@ AVBD_HD f3 qrotate(float4 q, f3 v) {
   return f3{v.x + tx * q.w + (q.y * tz - q.z * ty), v.y + ty * q.w + (q.z * tx - q.x * tz),
             v.z + tz * q.w + (q.x * ty - q.y * tx)};
 }
+
+/**
+ * atan(t) for t in [0, 1] in FP32 arithmetic only: above tan 15° the identity atan t = 30° + atan((√3 t − 1)/(√3 + t))
+ * brings the argument into |u| ≤ tan 15° ≈ 0.268, where the odd series u − u³/3 + … − u¹¹/11 (Horner) is within
+ * 3e-9 of atan u. The same operations in the same order run on the CPU, so both sides round alike.
+ */
+AVBD_HD float atanUnit(float t) {
+  const float TAN15 = 0.267949192f, SQRT3 = 1.73205081f, PI6 = 0.523598776f;
+  float offset = 0.f, u = t;
+  if (t > TAN15) {
+    u = (t * SQRT3 - 1.f) / (SQRT3 + t);
+    offset = PI6;
+  }
+  const float u2 = u * u;
+  float p = -1.f / 11.f;
+  p = p * u2 + 1.f / 9.f;
+  p = p * u2 - 1.f / 7.f;
+  p = p * u2 + 1.f / 5.f;
+  p = p * u2 - 1.f / 3.f;
+  p = p * u2 + 1.f;
+  return offset + u * p;
+}
+constexpr float PI_F = 3.14159265f, HALF_PI_F = 1.57079633f, TWO_PI_F = 6.28318531f;
+/** atan2(y, x) in (−π, π] from atanUnit by octant (0 at the origin). */
+AVBD_HD float atan2r(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  if (ax == 0.f && ay == 0.f) return 0.f;
+  const bool steep = ay > ax;
+  float a = atanUnit(steep ? ax / ay : ay / ax);
+  if (steep) a = HALF_PI_F - a;
+  if (x < 0.f) a = PI_F - a;
+  return y < 0.f ? -a : a;
+}
+/** An angle wrapped into [−π, π) (the reference's atan2(sin x, cos x)). */
+AVBD_HD float wrapAngle(float x) { return x - TWO_PI_F * floorf((x + PI_F) / TWO_PI_F); }
 
 /** Contact basis rows (n, t1, t2) (layout.ts:236-242). */
 struct Basis {

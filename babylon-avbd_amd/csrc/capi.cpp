@@ -402,6 +402,68 @@ int avbd_world_get_coloring(avbd_world *world, uint32_t *color_cap, uint32_t *co
   });
 }
 
+int avbd_world_release_joints(avbd_world *world, const uint32_t *slots, uint32_t n) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->releaseJoints(slots, n);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_disable_constraint(avbd_world *world, uint32_t slot) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->disableConstraint(slot);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_world_anchor(avbd_world *world, uint32_t slot, const float *point) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->setWorldAnchor(slot, point);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_motor(avbd_world *world, uint32_t slot, const float *speed, const float *max_torque) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->setMotor(slot, speed, max_torque);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_spring_material(avbd_world *world, uint32_t slot, float break_strain, float break_force) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->setSpringMaterial(slot, break_strain, break_force);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_set_angular_constraint(avbd_world *world, uint32_t slot, int32_t type, const float *axis_a,
+                                      const float *axis_b, float lo, float hi) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    world->engine->setAngularConstraint(slot, type, axis_a, axis_b, lo, hi);
+    return AVBD_OK;
+  });
+}
+
+int avbd_world_joint_info(avbd_world *world, uint32_t slot, int32_t *out) {
+  return guarded([&] {
+    CHECK_WORLD(world);
+    if (!out) throw std::invalid_argument("out is NULL");
+    const int4 f = world->engine->jointInfo(slot);
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+    out[3] = f.w;
+    return AVBD_OK;
+  });
+}
+
 int avbd_world_adapt(avbd_world *world, int *changed) {
   return guarded([&] {
     CHECK_WORLD(world);

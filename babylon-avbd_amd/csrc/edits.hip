@@ -105,7 +105,70 @@ __global__ void k_set_params(WorldParams *p, ParamEdit e) {
   p->flags = e.flags;
 }
 
+/** Constraint record edits: (slot, word, value bits) triples, word 0..31 a word of the record, 32..34 the info
+ * word's type, body A and body B. The host leaves at most one triple per (slot, word) in a launch. */
+__global__ void k_edit_joints(Joint *joints, int4 *info, const uint32_t *words, uint32_t n) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t slot = words[3 * e], word = words[3 * e + 1], value = words[3 * e + 2];
+  if (word < 32) reinterpret_cast<uint32_t *>(&joints[slot])[word] = value;
+  else reinterpret_cast<int32_t *>(&info[slot])[word - 32] = int32_t(value);
+}
+
+/** The quaternion a ⊗ b with the reference capture's grouping (a.w b.xyz + b.w a.xyz + a.xyz × b.xyz, ...). */
+AVBD_D float4 capMul(float4 a, float4 b) {
+  return make_float4((a.w * b.x + b.w * a.x) + (a.y * b.z - a.z * b.y), (a.w * b.y + b.w * a.y) + (a.z * b.x - a.x * b.z),
+                     (a.w * b.z + b.w * a.z) + (a.x * b.y - a.y * b.x), a.w * b.w - ((a.x * b.x + a.y * b.y) + a.z * b.z));
+}
+AVBD_D f3 perpendicular(f3 v) { return fabsf(v.x) > fabsf(v.z) ? mk3(-v.y, v.x, 0.f) : mk3(0.f, -v.z, v.y); }
+
+/**
+ * Frames of new constraints from the live poses: an angle-locked joint's rest orientation of B relative to A
+ * (constraintFrames.wgsl:11-19), a motor's or limit's reference tangents about its axes (angularFrames.wgsl).
+ */
+__global__ void k_capture_frames(const Body *bodies, Joint *joints, const int4 *info, const uint32_t *slots,
+                                 uint32_t n) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t j = slots[e];
+  const int4 f = info[j];
+  Joint k = joints[j];
+  const float4 qa = f.y >= 0 ? bodies[f.y].rot : make_float4(0.f, 0.f, 0.f, 1.f);
+  const float4 qb = bodies[f.z].rot;
+  if (f.x == T_JOINT) {
+    const float4 r = capMul(qconj(qa), qb);
+    const float l = sqrtf(qlengthSq(r));
+    k.c0Lin.w = r.x / l;
+    k.c0Ang.w = r.y / l;
+    k.rA.w = r.z / l;
+    k.rB.w = r.w / l;
+  } else if (f.x == T_MOTOR || f.x == T_LIMIT) {
+    const f3 tangent = normalize(perpendicular(xyz(k.rA)));
+    const f3 world = qrotate(qa, tangent);
+    const f3 axisB = normalize(xyz(k.rB));
+    f3 tangentB = qrotate(qconj(qb), world);
+    tangentB = sub(tangentB, scale(axisB, dot(tangentB, axisB)));
+    if (length(tangentB) < 1.0e-6f) tangentB = perpendicular(axisB);
+    k.c0Lin = mk4(tangent, 0.f);
+    k.c0Ang = mk4(normalize(tangentB), 0.f);
+  } else {
+    return;
+  }
+  joints[j] = k;
+}
+
 }  // namespace
+
+void launch_edit_joints(Joint *joints, int4 *info, const uint32_t *words, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  k_edit_joints<<<(n + 255) / 256, 256, 0, s>>>(joints, info, words, n);
+}
+
+void launch_capture_frames(const Body *bodies, Joint *joints, const int4 *info, const uint32_t *slots, uint32_t n,
+                           hipStream_t s) {
+  if (n == 0) return;
+  k_capture_frames<<<(n + 63) / 64, 64, 0, s>>>(bodies, joints, info, slots, n);
+}
 
 void launch_body_commands(Body *bodies, const uint32_t *commands, const uint2 *ranges, uint32_t n, uint32_t *marks,
                           hipStream_t s) {

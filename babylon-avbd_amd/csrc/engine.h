@@ -172,6 +172,19 @@ class Engine {
   /** Unread events (12 words each: kind, a, b, step, point xyz, impulse, normal xyz, 0) into out (room for cap);
    * returns how many were copied; *written = all events since the last read, *full = a set overflowed. Clears the log. */
   uint32_t readContactEvents(uint32_t *out, uint32_t cap, uint32_t *written, uint32_t *full);
+  /** Constraint edits (applied on the stream after the steps enqueued before): release slots (zeroed, type none: the
+   * bodies collide again; released or unknown slots are ignored), disable one (penalties, stiffnesses and multipliers
+   * zeroed), move a world joint's anchor, change a motor's speed and/or torque bound, set a spring's break strain and
+   * force (Infinity: unbreakable), make a slot a motor (type 3: lo speed, hi torque bound) or limit (type 4: lo, hi
+   * angles) about body-local axes, its frames captured from the current poses. Sleeping bodies involved wake. */
+  void releaseJoints(const uint32_t *slots, uint32_t n);
+  void disableConstraint(uint32_t slot);
+  void setWorldAnchor(uint32_t slot, const float *point);
+  void setMotor(uint32_t slot, const float *speed, const float *maxTorque);
+  void setSpringMaterial(uint32_t slot, float breakStrain, float breakForce);
+  void setAngularConstraint(uint32_t slot, int32_t type, const float *axisA, const float *axisB, float lo, float hi);
+  /** Type, body A (-1: world), body B of a constraint slot (batch indices). */
+  int4 jointInfo(uint32_t slot) const;
   /** Solver iterations per step (all worlds of the batch). */
   void setIterations(uint32_t n);
 
@@ -191,6 +204,10 @@ class Engine {
 
  private:
   void allocate();
+  void captureFrames(const std::vector<uint32_t> &slots);
+  /** (slot, word, value bits) triples, at most one per (slot, word); words 32..34 set the info word. */
+  void editJointWords(const std::vector<uint32_t> &triples);
+  void wakeConstraint(uint32_t slot);
   void allocateContacts(uint32_t pairCap);
   void encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce = 0);
   void uploadWatch();
@@ -223,6 +240,7 @@ class Engine {
   Manifold *manifoldBuf_[2] = {nullptr, nullptr};
   Contact *contactBuf_[2] = {nullptr, nullptr};
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
+  std::vector<int4> infoHost_;      // host copy of the constraint info words (type, a, b, 0)
   std::vector<float> restitution_;  // host copy of the per-body coefficients
   uint32_t bouncing_ = 0;           // bodies with a coefficient above 0
   bool sleeping_ = false, sleepIslands_ = false;

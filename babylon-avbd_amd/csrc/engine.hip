@@ -98,7 +98,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
       int4 f = hw.info[j];
       if (f.z < 0 || uint32_t(f.z) >= n || f.y >= int(n) || f.y == f.z)
         throw std::invalid_argument("joint endpoints must be distinct bodies of the world");
-      if (f.x != T_JOINT && f.x != T_SPRING && f.x != T_NONE) throw std::invalid_argument("unknown joint type");
+      if (f.x < T_NONE || f.x > T_LIMIT) throw std::invalid_argument("unknown joint type");
       if (f.x == T_SPRING && f.y < 0) throw std::invalid_argument("A spring requires two bodies");
       int a = f.y < 0 ? -1 : int(base) + f.y;
       int b = int(base) + f.z;
@@ -219,6 +219,14 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   up(w.filters, filters.data(), filters.size() * sizeof(uint2));
   up(w.joints, joints.data(), joints.size() * sizeof(Joint));
   up(w.info, info.data(), info.size() * sizeof(int4));
+  infoHost_ = info;
+  // Frames of the constraints from the starting poses: rest orientations of angle-locked joints, reference tangents
+  // of motors and limits (scenes.js:666-673, appGpuSolver3D.js:500-526)
+  std::vector<uint32_t> frameSlots;
+  for (uint32_t j = 0; j < nJoints_; j++)
+    if ((info[j].x == T_JOINT && joints[j].penAng.w > 0.f) || info[j].x == T_MOTOR || info[j].x == T_LIMIT)
+      frameSlots.push_back(j);
+  captureFrames(frameSlots);
   std::vector<uint32_t> nc;
   for (auto &t : noCollide) {
     nc.push_back(std::get<0>(t));
@@ -1194,6 +1202,145 @@ void Engine::setParams(uint32_t world, const ParamEdit &e) {
   HIP_CHECK(hipSetDevice(device_));
   launch_set_params(base_.worlds + world, e, stream_);
   HIP_CHECK(hipGetLastError());
+}
+
+// --- constraint edits between steps (appGpuSolver3D.js:500-563, 810-837, 922-957; solver.ts:1138-1204) ----------
+
+void Engine::captureFrames(const std::vector<uint32_t> &slots) {
+  if (slots.empty()) return;
+  HIP_CHECK(hipSetDevice(device_));
+  const uint32_t *dev = uploadEdit(slots.data(), slots.size());
+  launch_capture_frames(base_.bodies, base_.joints, base_.info, dev, uint32_t(slots.size()), stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Engine::editJointWords(const std::vector<uint32_t> &triples) {
+  if (triples.empty()) return;
+  HIP_CHECK(hipSetDevice(device_));
+  const uint32_t *dev = uploadEdit(triples.data(), triples.size());
+  launch_edit_joints(base_.joints, base_.info, dev, uint32_t(triples.size() / 3), stream_);
+  HIP_CHECK(hipGetLastError());
+}
+
+int4 Engine::jointInfo(uint32_t slot) const {
+  if (slot >= nJoints_) throw std::out_of_range("no such constraint slot");
+  return infoHost_[slot];
+}
+
+void Engine::wakeConstraint(uint32_t slot) {
+  if (!sleeping_) return;
+  const int4 f = infoHost_[slot];
+  uint32_t bodies[2];
+  uint32_t n = 0;
+  if (f.y >= 0) bodies[n++] = uint32_t(f.y);
+  if (f.z >= 0) bodies[n++] = uint32_t(f.z);
+  if (n) wakeBodies(bodies, n);
+}
+
+void Engine::releaseJoints(const uint32_t *slots, uint32_t n) {
+  if (n && !slots) throw std::invalid_argument("slots is NULL");
+  std::vector<uint32_t> order;
+  for (uint32_t k = 0; k < n; k++)
+    if (slots[k] < nJoints_ && infoHost_[slots[k]].x != T_NONE) order.push_back(slots[k]);
+  std::sort(order.begin(), order.end());
+  order.erase(std::unique(order.begin(), order.end()), order.end());
+  if (order.empty()) return;
+  for (uint32_t slot : order) wakeConstraint(slot);
+  // The whole record zeroed and the slot's type none: no rows, no adjacency entry, and its bodies collide again
+  std::vector<uint32_t> t;
+  t.reserve(order.size() * 35 * 3);
+  for (uint32_t slot : order) {
+    for (uint32_t word = 0; word < 32; word++) t.insert(t.end(), {slot, word, 0u});
+    t.insert(t.end(), {slot, 32u, uint32_t(T_NONE), slot, 33u, 0xffffffffu, slot, 34u, 0xffffffffu});
+    infoHost_[slot] = make_int4(T_NONE, -1, -1, 0);
+  }
+  editJointWords(t);
+}
+
+void Engine::disableConstraint(uint32_t slot) {
+  if (slot >= nJoints_) throw std::out_of_range("no such constraint slot");
+  wakeConstraint(slot);
+  // Penalties and stiffnesses (8 words) and both multipliers' xyz to zero: the constraint stops acting and its
+  // bodies collide again; the thresholds in lamLin.w / lamAng.w stay (solver.ts:1200-1204)
+  std::vector<uint32_t> t;
+  for (uint32_t word : {0u, 1u, 2u, 3u, 4u, 5u, 6u, 7u, 8u, 9u, 10u, 12u, 13u, 14u}) t.insert(t.end(), {slot, word, 0u});
+  editJointWords(t);
+}
+
+static uint32_t fbitsHost(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+void Engine::setWorldAnchor(uint32_t slot, const float *point) {
+  if (slot >= nJoints_ || infoHost_[slot].y != -1 || infoHost_[slot].x == T_NONE || !point ||
+      !(std::isfinite(point[0]) && std::isfinite(point[1]) && std::isfinite(point[2])))
+    throw std::invalid_argument("A world joint and finite xyz anchor are required");
+  if (sleeping_) {
+    const uint32_t b = uint32_t(infoHost_[slot].z);
+    wakeBodies(&b, 1);
+  }
+  editJointWords({slot, 24u, fbitsHost(point[0]), slot, 25u, fbitsHost(point[1]), slot, 26u, fbitsHost(point[2])});
+}
+
+void Engine::setMotor(uint32_t slot, const float *speed, const float *maxTorque) {
+  if (slot >= nJoints_ || infoHost_[slot].x != T_MOTOR) throw std::invalid_argument("Motor needs an active motor slot");
+  if (speed && !std::isfinite(*speed)) throw std::invalid_argument("Motor speed must be finite");
+  if (maxTorque && !(std::isfinite(*maxTorque) && *maxTorque >= 0.f))
+    throw std::invalid_argument("maxTorque must be nonnegative");
+  std::vector<uint32_t> t;
+  if (speed) t.insert(t.end(), {slot, 27u, fbitsHost(*speed)});
+  if (maxTorque) t.insert(t.end(), {slot, 31u, fbitsHost(*maxTorque)});
+  if (sleeping_) wakeAll();
+  editJointWords(t);
+}
+
+void Engine::setSpringMaterial(uint32_t slot, float breakStrain, float breakForce) {
+  if (slot >= nJoints_ || infoHost_[slot].x != T_SPRING)
+    throw std::invalid_argument("Spring material requires an active spring slot");
+  for (float v : {breakStrain, breakForce})
+    if (!(v == INFINITY || (std::isfinite(v) && v > 0.f)))
+      throw std::invalid_argument("breakStrain and breakForce must be positive or Infinity");
+  if (sleeping_) wakeAll();
+  editJointWords({slot, 11u, fbitsHost(std::isinf(breakStrain) ? 0.f : breakStrain), slot, 15u,
+                  fbitsHost(std::isinf(breakForce) ? 0.f : breakForce)});
+}
+
+void Engine::setAngularConstraint(uint32_t slot, int32_t type, const float *axisA, const float *axisB, float lo,
+                                  float hi) {
+  if (slot >= nJoints_ || infoHost_[slot].x == T_NONE) throw std::invalid_argument("Angular constraint needs a live slot");
+  if (type != T_MOTOR && type != T_LIMIT) throw std::invalid_argument("Invalid angular constraint type");
+  float axes[6];
+  for (int s = 0; s < 2; s++) {
+    const float *v = s ? axisB : axisA;
+    if (!v) throw std::invalid_argument("Angular axis requires three finite, nonzero components");
+    const double l = std::sqrt(double(v[0]) * v[0] + double(v[1]) * v[1] + double(v[2]) * v[2]);
+    if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])) || !(l >= 1e-8))
+      throw std::invalid_argument("Angular axis requires three finite, nonzero components");
+    for (int c = 0; c < 3; c++) axes[3 * s + c] = float(v[c] / l);
+  }
+  if (type == T_MOTOR && !(std::isfinite(lo) && std::isfinite(hi) && hi >= 0.f))
+    throw std::invalid_argument("Motor speed must be finite and maxTorque nonnegative");
+  if (type == T_LIMIT && !(std::isfinite(lo) && std::isfinite(hi) && lo >= -3.14159265358979 && hi <= 3.14159265358979 && lo <= hi))
+    throw std::invalid_argument("Hinge angles must be ordered within -PI..PI radians");
+  // penLin = (1000, 1000, 0, 3e38), axes in rA / rB, speed or lower angle in rA.w, torque bound or upper angle in rB.w
+  float rec[32] = {0};
+  rec[0] = rec[1] = 1000.f;
+  rec[3] = BIG;
+  for (int c = 0; c < 3; c++) {
+    rec[24 + c] = axes[c];
+    rec[28 + c] = axes[3 + c];
+  }
+  rec[27] = lo;
+  rec[31] = hi;
+  std::vector<uint32_t> t;
+  for (uint32_t word = 0; word < 32; word++) t.insert(t.end(), {slot, word, fbitsHost(rec[word])});
+  t.insert(t.end(), {slot, 32u, uint32_t(type)});
+  infoHost_[slot].x = type;
+  editJointWords(t);
+  captureFrames({slot});
+  if (sleeping_) wakeAll();
 }
 
 void Engine::setIterations(uint32_t n) {

@@ -98,5 +98,11 @@ void launch_set_filters(uint2 *filters, const uint32_t *words, uint32_t n, hipSt
 /** Restitution coefficients of n bodies: words = (body, f32 bits) pairs. */
 void launch_set_restitution(float *restitution, const uint32_t *words, uint32_t n, hipStream_t s);
 void launch_set_params(WorldParams *p, const ParamEdit &e, hipStream_t s);
+/** Constraint record edits: n (slot, word, value bits) triples (word 32..34: info type, body A, body B). */
+void launch_edit_joints(Joint *joints, int4 *info, const uint32_t *words, uint32_t n, hipStream_t s);
+/** Rest orientations of angle-locked joints and reference tangents of motors and limits, from the current poses,
+ * for the n constraint slots listed. */
+void launch_capture_frames(const Body *bodies, Joint *joints, const int4 *info, const uint32_t *slots, uint32_t n,
+                           hipStream_t s);
 
 }  // namespace avbd

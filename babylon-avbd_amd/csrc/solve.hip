@@ -96,8 +96,89 @@ AVBD_D float4 rotA(const DeviceWorld &w, int a) {
 AVBD_D f3 jointLinC(const DeviceWorld &w, const Joint &k, int a, int b) {
   return sub(anchorA(w, k, a), add(qrotate(w.bodies[b].rot, xyz(k.rB)), xyz(w.bodies[b].pos)));
 }
+/**
+ * The orientation a joint's angle rows hold B at: A's rotation times the rest orientation captured when the joint was
+ * made ((c0Lin.w, c0Ang.w, rA.w, rB.w)), or A's rotation alone for a record without one (withJointRestFrames,
+ * src/gpu/gpuSolverKernels.js:105-130).
+ */
+AVBD_D float4 restTarget(const DeviceWorld &w, const Joint &k, int a) {
+  const float4 t = rotA(w, a);
+  const float4 r = make_float4(k.c0Lin.w, k.c0Ang.w, k.rA.w, k.rB.w);
+  if (!(qlengthSq(r) > 0.5f)) return t;
+  return make_float4((t.w * r.x + r.w * t.x) + (t.y * r.z - t.z * r.y), (t.w * r.y + r.w * t.y) + (t.z * r.x - t.x * r.z),
+                     (t.w * r.z + r.w * t.z) + (t.x * r.y - t.y * r.x), t.w * r.w - ((t.x * r.x + t.y * r.y) + t.z * r.z));
+}
 AVBD_D f3 jointAngC(const DeviceWorld &w, const Joint &k, int a, int b) {
-  return scale(qsub(rotA(w, a), w.bodies[b].rot), k.lamAng.w);
+  return scale(qsub(restTarget(w, k, a), w.bodies[b].rot), k.lamAng.w);
+}
+
+// --- angular constraints: a motor or a limit about a hinge axis (src/gpu/angularConstraints.js:16-83) --------------
+
+struct HingeFrame {
+  f3 axis, ta, tb;  // A's axis, A's reference tangent, B's reference tangent, in world space
+};
+AVBD_D HingeFrame hingeFrame(const DeviceWorld &w, const Joint &k, int a, int b) {
+  const float4 qa = rotA(w, a);
+  return HingeFrame{qrotate(qa, xyz(k.rA)), qrotate(qa, xyz(k.c0Lin)), qrotate(w.bodies[b].rot, xyz(k.c0Ang))};
+}
+/** The twist of B about A's axis: the signed angle from A's tangent to B's. */
+AVBD_D float hingeAngle(const HingeFrame &f) { return atan2r(dot(f.axis, cross(f.ta, f.tb)), dot(f.ta, f.tb)); }
+/** Rows: a motor's one row (the twist this step beyond speed × dt, wrapped), a limit's lower and upper row. */
+AVBD_D float2 hingeRows(const Joint &k, int32_t type, float angle, float alpha, float dt) {
+  if (type == T_MOTOR) return make_float2(wrapAngle(angle - k.c0Lin.w) - k.rA.w * dt, 0.f);
+  return make_float2((angle - k.rA.w) - alpha * k.c0Lin.w, (angle - k.rB.w) - alpha * k.c0Ang.w);
+}
+/** Forces: a motor's clamped to ± its torque bound, a limit's one-sided (lower pushes up, upper pushes down). */
+AVBD_D float2 hingeForce(const Joint &k, int32_t type, float2 C) {
+  const float fx = k.penLin.x * C.x + k.lamLin.x, fy = k.penLin.y * C.y + k.lamLin.y;
+  if (type == T_MOTOR) return make_float2(wclamp(fx, -k.rB.w, k.rB.w), 0.f);
+  return make_float2(rmin(0.f, fx), rmax(0.f, fy));
+}
+AVBD_D void warmHinge(const DeviceWorld &w, uint32_t j, int4 f) {
+  Joint k = w.joints[j];
+  if (k.penLin.w == 0.f) return;
+  const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
+  const float angle = hingeAngle(hingeFrame(w, k, f.y, f.z));
+  k.c0Lin.w = f.x == T_MOTOR ? angle : rmin(0.f, angle - k.rA.w);
+  k.c0Ang.w = rmax(0.f, angle - k.rB.w);
+  const float decay = p.alpha * p.gamma;
+  k.lamLin = mk4(scale(xyz(k.lamLin), decay), k.lamLin.w);
+  k.penLin = make_float4(wclamp(k.penLin.x * p.gamma, PENALTY_MIN, PENALTY_MAX),
+                         wclamp(k.penLin.y * p.gamma, PENALTY_MIN, PENALTY_MAX),
+                         wclamp(k.penLin.z * p.gamma, PENALTY_MIN, PENALTY_MAX), k.penLin.w);
+  w.joints[j] = k;
+}
+AVBD_D void dualHinge(const DeviceWorld &w, uint32_t j, int4 f) {
+  Joint k = w.joints[j];
+  if (k.penLin.w == 0.f) return;
+  const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
+  const float2 C = hingeRows(k, f.x, hingeAngle(hingeFrame(w, k, f.y, f.z)), p.alpha, p.dt);
+  const float2 F = hingeForce(k, f.x, C);
+  k.lamLin = make_float4(F.x, F.y, 0.f, k.lamLin.w);
+  const float vx = f.x == T_MOTOR ? fabsf(C.x) : rmax(0.f, -C.x), vy = f.x == T_MOTOR ? fabsf(C.y) : rmax(0.f, C.y);
+  k.penLin.x = rmin(k.penLin.x + vx * p.betaAng, PENALTY_MAX);
+  k.penLin.y = rmin(k.penLin.y + vy * p.betaAng, PENALTY_MAX);
+  w.joints[j] = k;
+}
+
+/** Tensile failure of a spring (withSpringFracture, src/gpu/springMaterial.js:39-68): stretched past its break strain
+ * (of the rest length) or pulling past its break force, it stops acting, keeping both thresholds. */
+AVBD_D void springFracture(const DeviceWorld &w, uint32_t j, int4 f) {
+  Joint k = w.joints[j];
+  const float strainLimit = k.lamLin.w, forceLimit = k.lamAng.w;
+  if (!(k.penLin.w > 0.f && (strainLimit > 0.f || forceLimit > 0.f))) return;
+  const f3 pb = add(qrotate(w.bodies[f.z].rot, xyz(k.rB)), xyz(w.bodies[f.z].pos));
+  const float lengthNow = length(sub(anchorA(w, k, f.y), pb));
+  const float extension = rmax(0.f, lengthNow - k.rA.w);
+  const float strain = extension / rmax(k.rA.w, 1.0e-6f);
+  const float tension = k.penLin.w * extension;
+  if ((strainLimit > 0.f && k.rA.w > 1.0e-6f && strain > strainLimit) || (forceLimit > 0.f && tension > forceLimit)) {
+    k.penLin = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.penAng = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.lamLin = make_float4(0.f, 0.f, 0.f, strainLimit);
+    k.lamAng = make_float4(0.f, 0.f, 0.f, forceLimit);
+    w.joints[j] = k;
+  }
 }
 AVBD_D f3 jointLinRows(const DeviceWorld &w, const Joint &k, int a, int b, float alpha) {
   return sub(jointLinC(w, k, a, b), k.penLin.w >= HARD ? scale(xyz(k.c0Lin), alpha) : mk3(0.f, 0.f, 0.f));
@@ -115,11 +196,16 @@ __global__ void k_warm_joints(DeviceWorld w) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= w.nJoints || held(w)) return;
   int4 f = w.info[j];
+  if (f.x == T_MOTOR || f.x == T_LIMIT) {
+    warmHinge(w, j, f);
+    return;
+  }
   if (f.x != T_JOINT) return;
   Joint k = w.joints[j];
   const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
-  k.c0Lin = mk4(jointLinC(w, k, f.y, f.z), 0.f);
-  k.c0Ang = mk4(jointAngC(w, k, f.y, f.z), 0.f);
+  // The w words keep the rest orientation
+  k.c0Lin = mk4(jointLinC(w, k, f.y, f.z), k.c0Lin.w);
+  k.c0Ang = mk4(jointAngC(w, k, f.y, f.z), k.c0Ang.w);
   float decay = p.alpha * p.gamma;
   k.lamLin = mk4(scale(xyz(k.lamLin), decay), k.lamLin.w);
   k.lamAng = mk4(scale(xyz(k.lamAng), decay), k.lamAng.w);
@@ -178,6 +264,16 @@ AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, ui
   const Joint k = w.joints[j];
   bool isA = int(i) == a;
   float sg = isA ? 1.f : -1.f;
+  if (f.x == T_MOTOR || f.x == T_LIMIT) {  // (addAngular, angularConstraints.js:36-44)
+    if (k.penLin.w == 0.f || (f.x == T_MOTOR && k.rB.w == 0.f)) return;
+    const HingeFrame hf = hingeFrame(w, k, a, b);
+    const f3 axis = scale(hf.axis, isA ? -1.f : 1.f);
+    const float2 C = hingeRows(k, f.x, hingeAngle(hf), alpha, w.worlds[w.bodyWorld[b]].dt);
+    const float2 F = hingeForce(k, f.x, C);
+    if (f.x == T_MOTOR || F.x != 0.f) addRow(acc, mk3(0.f, 0.f, 0.f), axis, k.penLin.x, F.x);
+    if (f.x == T_LIMIT && F.y != 0.f) addRow(acc, mk3(0.f, 0.f, 0.f), axis, k.penLin.y, F.y);
+    return;
+  }
   if (f.x == T_SPRING) {
     f3 rAW = qrotate(w.bodies[a].rot, xyz(k.rA));
     f3 rBW = qrotate(w.bodies[b].rot, xyz(k.rB));
@@ -662,6 +758,14 @@ __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint3
 
 AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
   int4 f = w.info[j];
+  if (f.x == T_SPRING) {
+    springFracture(w, j, f);
+    return;
+  }
+  if (f.x == T_MOTOR || f.x == T_LIMIT) {
+    dualHinge(w, j, f);
+    return;
+  }
   if (f.x != T_JOINT) return;
   int a = f.y, b = f.z;
   const WorldParams &p = w.worlds[w.bodyWorld[b]];

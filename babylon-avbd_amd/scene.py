@@ -15,6 +15,8 @@ from . import _native
 BIG = 3.0e38
 T_JOINT = 1
 T_SPRING = 2
+T_MOTOR = 3
+T_LIMIT = 4
 SHAPE_BOX = 0.0
 SHAPE_SPHERE = 1.0
 SHAPE_CAPSULE = -1.0
@@ -90,6 +92,43 @@ def body_record(position, rotation, size, mass, moment, radius, friction, shape,
     r[36:39] = angular_velocity
     r[39] = shape
     return r
+
+
+def _unit(v):
+    a = _vector(v, 3, "axis")
+    n = math.sqrt(sum(x * x for x in a))
+    if not (n >= 1e-8):
+        raise ValueError("Angular axis requires three finite, nonzero components")
+    return [x / n for x in a]
+
+
+def _break_limits(breakStrain, breakForce):
+    """springMaterial (src/gpu/springMaterial.js:4-15): positive or Infinity; stored 0 for Infinity."""
+    out = []
+    for name, v in (("breakStrain", breakStrain), ("breakForce", breakForce)):
+        if v != math.inf and not (isinstance(v, (int, float)) and math.isfinite(v) and v > 0):
+            raise ValueError(f"{name} must be positive or Infinity")
+        out.append(0.0 if v == math.inf else float(v))
+    return out
+
+
+def _angular_record(kind, axisA, axisB, speed=1.0, maxTorque=100.0, minAngle=-math.pi, maxAngle=math.pi):
+    """The record setAngularConstraint writes (appGpuSolver3D.js:500-517; angularConstraintOptions,
+    angularConstraints.js:84-120): row penalties 1000, stiffness 3e38, unit axes, speed/torque bound or angles."""
+    if not (isinstance(speed, (int, float)) and math.isfinite(speed) and isinstance(maxTorque, (int, float))
+            and math.isfinite(maxTorque) and maxTorque >= 0):
+        raise ValueError("Motor speed must be finite and maxTorque nonnegative")
+    if not (math.isfinite(minAngle) and math.isfinite(maxAngle) and minAngle >= -math.pi and maxAngle <= math.pi
+            and minAngle <= maxAngle):
+        raise ValueError("Hinge angles must be ordered within -PI..PI radians")
+    rec = np.zeros(32, np.float32)
+    rec[0] = rec[1] = 1000.0
+    rec[3] = 3e38
+    rec[24:27] = _unit(axisA)
+    rec[28:31] = _unit(axisB)
+    rec[27] = speed if kind == "motor" else minAngle
+    rec[31] = maxTorque if kind == "motor" else maxAngle
+    return rec
 
 
 class AvbdScene3D:
@@ -293,8 +332,10 @@ class AvbdScene3D:
         self._joints.append((rec, (T_JOINT, -1 if a is None else int(a), int(b), 0)))
         return len(self._joints) - 1
 
-    def addSpring(self, a, b, anchorA=(0, 0, 0), anchorB=(0, 0, 0), stiffness=1000.0, rest=-1.0):
-        """Spring between body-local anchors; rest -1 takes their initial separation (scenes.js:546-575)."""
+    def addSpring(self, a, b, anchorA=(0, 0, 0), anchorB=(0, 0, 0), stiffness=1000.0, rest=-1.0,
+                  breakStrain=math.inf, breakForce=math.inf):
+        """Spring between body-local anchors; rest -1 takes their initial separation. A finite breakStrain (extension
+        over the rest length) or breakForce (tension) lets it tear for good (scenes.js:546-575; springMaterial.js)."""
         self.assertEditable()
         self._check(a, b)
         if a is None:
@@ -310,13 +351,48 @@ class AvbdScene3D:
             pb = np.add(_rotate([float(x) for x in rb[4:8]], rB), rb[0:3].astype(np.float64))
             d = pa - pb
             rest = math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+        strain, force = _break_limits(breakStrain, breakForce)
         rec = np.zeros(32, np.float32)
         rec[3] = float(stiffness)
+        rec[11] = strain
+        rec[15] = force
         rec[24:27] = rA
         rec[27] = rest
         rec[28:31] = rB
         self._joints.append((rec, (T_SPRING, int(a), int(b), 0)))
         return len(self._joints) - 1
+
+    def addMotor(self, a, b, axisA=(0, 1, 0), axisB=(0, 1, 0), speed=1.0, maxTorque=100.0):
+        """An angular motor driving B's twist about the hinge axis (body-local axisA in A, axisB in B) at `speed`
+        rad/s with at most `maxTorque` (scenes.js:572-582; angularConstraints.js). a = None: the world."""
+        self.assertEditable()
+        self._check(a, b)
+        rec = _angular_record("motor", axisA, axisB, speed=speed, maxTorque=maxTorque)
+        self._joints.append((rec, (T_MOTOR, -1 if a is None else int(a), int(b), 0)))
+        return len(self._joints) - 1
+
+    def addHinge(self, a, b, anchorA=(0, 0, 0), anchorB=(0, 0, 0), axisA=(0, 1, 0), axisB=(0, 1, 0), span=1.0,
+                 minAngle=None, maxAngle=None, motor=None):
+        """Two ball joints `span` apart along the axis, plus an angle limit when minAngle or maxAngle is given and a
+        motor when `motor` (speed, maxTorque) is (scenes.js:587-630). Returns {joints, limit, motor} slots."""
+        self.assertEditable()
+        self._check(a, b)
+        _angular_record("limit", axisA, axisB, minAngle=-math.pi if minAngle is None else minAngle,
+                        maxAngle=math.pi if maxAngle is None else maxAngle)  # validates
+        span = _positive(span, "span")
+        ua, ub = _unit(axisA), _unit(axisB)
+        aa, bb = _vector(anchorA, 3, "anchorA"), _vector(anchorB, 3, "anchorB")
+        joints = [self.addJoint(a, b, [aa[i] + sign * span * 0.5 * ua[i] for i in range(3)],
+                                [bb[i] + sign * span * 0.5 * ub[i] for i in range(3)]) for sign in (-1, 1)]
+        limit = motor_slot = None
+        if minAngle is not None or maxAngle is not None:
+            rec = _angular_record("limit", axisA, axisB, minAngle=-math.pi if minAngle is None else minAngle,
+                                  maxAngle=math.pi if maxAngle is None else maxAngle)
+            self._joints.append((rec, (T_LIMIT, -1 if a is None else int(a), int(b), 0)))
+            limit = len(self._joints) - 1
+        if motor:
+            motor_slot = self.addMotor(a, b, axisA, axisB, **motor)
+        return {"joints": joints, "limit": limit, "motor": motor_slot}
 
     def ignoreCollision(self, a, b):
         """The pair never collides (ref/forces.ts IgnoreCollision)."""

@@ -701,6 +701,55 @@ class HipWorld:
         self._alive()
         check(self._lib.avbd_world_set_coloring(self._h, int(colorCap), int(colorRounds)))
 
+    # --- constraint edits (appGpuSolver3D.js:500-563, 810-837, 922-957; solver.ts:1138-1204) -----------------------
+
+    def releaseJoints(self, slots):
+        """Release constraint slots: they stop acting and their bodies collide again."""
+        self._alive()
+        s = np.ascontiguousarray(list(slots), np.uint32)
+        check(self._lib.avbd_world_release_joints(self._h, ptr(s, ctypes.c_uint32) if len(s) else None, len(s)))
+
+    def disableConstraint(self, slot):
+        self._alive()
+        check(self._lib.avbd_world_disable_constraint(self._h, int(slot)))
+
+    def setWorldAnchor(self, slot, point):
+        self._alive()
+        p = np.ascontiguousarray(point, np.float32)
+        if p.shape != (3,):
+            raise ValueError("A world joint and finite xyz anchor are required")
+        check(self._lib.avbd_world_set_world_anchor(self._h, int(slot), ptr(p, ctypes.c_float)))
+
+    def setMotor(self, slot, speed=None, maxTorque=None):
+        self._alive()
+        sp = None if speed is None else np.array([speed], np.float32)
+        mt = None if maxTorque is None else np.array([maxTorque], np.float32)
+        check(self._lib.avbd_world_set_motor(self._h, int(slot), ptr(sp, ctypes.c_float), ptr(mt, ctypes.c_float)))
+
+    def setSpringMaterial(self, slot, breakStrain=math.inf, breakForce=math.inf):
+        self._alive()
+        check(self._lib.avbd_world_set_spring_material(self._h, int(slot), float(breakStrain), float(breakForce)))
+
+    def setAngularConstraint(self, slot, type, axisA=(0, 1, 0), axisB=(0, 1, 0), speed=1.0, maxTorque=100.0,
+                             minAngle=-math.pi, maxAngle=math.pi):
+        """The slot becomes a motor ("motor": speed, maxTorque) or a limit ("limit": minAngle, maxAngle)."""
+        self._alive()
+        if type not in ("motor", "limit"):
+            raise ValueError("Invalid angular constraint type")
+        a = np.ascontiguousarray(axisA, np.float32)
+        b = np.ascontiguousarray(axisB, np.float32)
+        lo, hi = (speed, maxTorque) if type == "motor" else (minAngle, maxAngle)
+        check(self._lib.avbd_world_set_angular_constraint(self._h, int(slot), 3 if type == "motor" else 4,
+                                                          ptr(a, ctypes.c_float), ptr(b, ctypes.c_float), float(lo),
+                                                          float(hi)))
+
+    def jointInfo(self, slot):
+        """(type, body A or -1, body B, 0) of a constraint slot."""
+        self._alive()
+        out = (ctypes.c_int32 * 4)()
+        check(self._lib.avbd_world_joint_info(self._h, int(slot), out))
+        return tuple(out)
+
     def getColoring(self):
         """{colorCap, colorRounds, pairCapacity} in use (as setColoring and adapt() leave them)."""
         self._alive()

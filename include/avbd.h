@@ -245,6 +245,40 @@ int avbd_world_set_restitution(avbd_world *world, const uint32_t *indices, const
 /** The coefficients of all bodies (world->bodies floats). */
 int avbd_world_get_restitution(avbd_world *world, float *out);
 
+/* ---- Constraint edits (src/gpu/appGpuSolver3D.js:500-563, 810-837, 922-957; reference/three-avbd/src/avbd3d/gpu/
+ * solver.ts:1138-1204). Constraint types of joint_info[0]: 0 released, 1 ball joint (+ angle lock), 2 spring, 3 motor,
+ * 4 limit. A scene may pass motors and limits as records of their own (the layout setAngularConstraint writes: penLin
+ * (1000, 1000, 0, 3e38), axes in rA.xyz / rB.xyz, speed or lower angle in rA.w, torque bound or upper angle in rB.w);
+ * the world captures their reference tangents, and the rest orientation of every ball joint with a nonzero angular
+ * stiffness, from the starting poses. Edits apply on the world's stream after the steps enqueued before them. */
+#define AVBD_JOINT_NONE 0
+#define AVBD_JOINT_BALL 1
+#define AVBD_JOINT_SPRING 2
+#define AVBD_JOINT_MOTOR 3
+#define AVBD_JOINT_LIMIT 4
+
+/** releaseJoints(slots) (solver.ts:1138-1162): each live slot's record zeroed and its type 0, so its bodies collide
+ * again; released or out-of-range slots are ignored. Sleeping bodies of the constraints wake. */
+int avbd_world_release_joints(avbd_world *world, const uint32_t *slots, uint32_t n);
+/** disableConstraint(slot) (solver.ts:1200-1204): penalties, stiffnesses and multipliers zeroed (fracture and torque
+ * arm kept); the constraint stops acting and its bodies collide again. */
+int avbd_world_disable_constraint(avbd_world *world, uint32_t slot);
+/** setWorldAnchor(slot, point) (appGpuSolver3D.js:928-941): the world point of a joint whose first body is the world. */
+int avbd_world_set_world_anchor(avbd_world *world, uint32_t slot, const float *point);
+/** setMotor(slot, {speed, maxTorque}) (appGpuSolver3D.js:547-563): either pointer may be NULL (kept). */
+int avbd_world_set_motor(avbd_world *world, uint32_t slot, const float *speed, const float *max_torque);
+/** setSpringMaterial(slot, {breakStrain, breakForce}) (appGpuSolver3D.js:810-837; springMaterial.js): positive, or
+ * INFINITY for unbreakable. A spring stretched past break_strain × its rest length, or pulling with more than
+ * break_force, stops acting for good. */
+int avbd_world_set_spring_material(avbd_world *world, uint32_t slot, float break_strain, float break_force);
+/** setAngularConstraint(slot, options) (appGpuSolver3D.js:500-526; angularConstraints.js:84-120): the slot becomes a
+ * motor (type 3: lo = speed, hi = torque bound >= 0) or a limit (type 4: lo <= hi within -pi..pi) about the body-local
+ * axes (normalised here), its reference tangents captured from the current poses. */
+int avbd_world_set_angular_constraint(avbd_world *world, uint32_t slot, int32_t type, const float *axis_a,
+                                      const float *axis_b, float lo, float hi);
+/** A constraint's info word: type, body A (-1: world), body B, 0. */
+int avbd_world_joint_info(avbd_world *world, uint32_t slot, int32_t *out);
+
 /* ---- Sleeping (src/gpu/worldSleeping.js, worldSleeping.wgsl, islandSleeping.wgsl, sleepContact.wgsl) ----------------
  * A sleeping body is a fixed support: its mass (record float 19) reads 0 and its velocities 0 until it wakes, when its
  * awake mass comes back. Body commands, filter and restitution edits and body writes wake every body first, as the

@@ -58,22 +58,31 @@ template <class R, class T> void packBody(const Body<R> &b, T *o) {
   for (int k = 0; k < 40; k++) o[k] = T(v[k]);
   o[27] = sizeof(T) == 4 ? T(bitsToFloat(b.stamp)) : T(b.stamp);
 }
+/** Word `word` (0..31) of a constraint record. */
+template <class R> R *jointWord(Joint<R> &k, uint32_t word) {
+  V3<R> *v[8] = {&k.penLin, &k.penAng, &k.lamLin, &k.lamAng, &k.c0Lin, &k.c0Ang, &k.rA, &k.rB};
+  R *w[8] = {&k.stiffLin, &k.stiffAng, &k.frac, &k.torqueArm, &k.c0LinW, &k.c0AngW, &k.rest, &k.rBW};
+  const uint32_t q = word / 4, c = word % 4;
+  if (c == 3) return w[q];
+  return c == 0 ? &v[q]->x : (c == 1 ? &v[q]->y : &v[q]->z);
+}
+
 template <class R> void unpackJoint(const float *s, const int32_t *info, Joint<R> &k) {
   k.penLin = V3<R>(s[0], s[1], s[2]); k.stiffLin = s[3];
   k.penAng = V3<R>(s[4], s[5], s[6]); k.stiffAng = s[7];
   k.lamLin = V3<R>(s[8], s[9], s[10]); k.frac = s[11];
   k.lamAng = V3<R>(s[12], s[13], s[14]); k.torqueArm = s[15];
-  k.c0Lin = V3<R>(s[16], s[17], s[18]);
-  k.c0Ang = V3<R>(s[20], s[21], s[22]);
+  k.c0Lin = V3<R>(s[16], s[17], s[18]); k.c0LinW = s[19];
+  k.c0Ang = V3<R>(s[20], s[21], s[22]); k.c0AngW = s[23];
   k.rA = V3<R>(s[24], s[25], s[26]); k.rest = s[27];
-  k.rB = V3<R>(s[28], s[29], s[30]);
+  k.rB = V3<R>(s[28], s[29], s[30]); k.rBW = s[31];
   k.type = info[0]; k.a = info[1]; k.b = info[2];
 }
 template <class R, class T> void packJoint(const Joint<R> &k, T *o) {
   const R v[32] = {k.penLin.x, k.penLin.y, k.penLin.z, k.stiffLin, k.penAng.x, k.penAng.y, k.penAng.z, k.stiffAng,
                    k.lamLin.x, k.lamLin.y, k.lamLin.z, k.frac, k.lamAng.x, k.lamAng.y, k.lamAng.z, k.torqueArm,
-                   k.c0Lin.x, k.c0Lin.y, k.c0Lin.z, 0, k.c0Ang.x, k.c0Ang.y, k.c0Ang.z, 0,
-                   k.rA.x, k.rA.y, k.rA.z, k.rest, k.rB.x, k.rB.y, k.rB.z, 0};
+                   k.c0Lin.x, k.c0Lin.y, k.c0Lin.z, k.c0LinW, k.c0Ang.x, k.c0Ang.y, k.c0Ang.z, k.c0AngW,
+                   k.rA.x, k.rA.y, k.rA.z, k.rest, k.rB.x, k.rB.y, k.rB.z, k.rBW};
   for (int i = 0; i < 32; i++) o[i] = T(v[i]);
 }
 template <class R> void unpackManifold(const uint32_t *w, Manifold<R> &m) {
@@ -216,6 +225,38 @@ void *gs_create(int precision, uint32_t nBodies, const float *bodyRecs, const ui
 }
 
 void gs_destroy(void *p) { delete static_cast<Handle *>(p); }
+
+/** The frames constraints take at creation (rest orientations of angle-locked joints, motors' and limits' reference
+ * tangents) from the current poses: for all such slots (n == 0) or the listed ones. */
+void gs_capture_frames(void *p, const uint32_t *slots, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  if (n == 0) {
+    h->f->captureFrames();
+    if (h->d) h->d->captureFrames();
+    return;
+  }
+  for (uint32_t e = 0; e < n; e++) {
+    h->f->captureFrame(h->f->joints[slots[e]]);
+    if (h->d) h->d->captureFrame(h->d->joints[slots[e]]);
+  }
+}
+
+/** Constraint record edits: n (slot, word, value bits) triples; words 32..34 set the type, body A and body B. */
+void gs_edit_joints(void *p, const uint32_t *triples, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  auto apply = [&](auto &w) {
+    for (uint32_t e = 0; e < n; e++) {
+      const uint32_t slot = triples[3 * e], word = triples[3 * e + 1], value = triples[3 * e + 2];
+      auto &k = w.joints[slot];
+      if (word == 32) k.type = int32_t(value);
+      else if (word == 33) k.a = int32_t(value);
+      else if (word == 34) k.b = int32_t(value);
+      else *jointWord(k, word) = bitsToFloat(value);
+    }
+  };
+  apply(*h->f);
+  if (h->d) apply(*h->d);
+}
 
 /** Colour cap and Jones–Plassmann rounds from the next step on (the GPU world's setColoring / adapt). */
 void gs_set_coloring(void *p, uint32_t colorCap, uint32_t colorRounds) {

@@ -31,7 +31,7 @@ namespace oracle {
 namespace gpu {
 
 constexpr uint32_t NO_COLOR = 255, PENDING = 256, MAX_COLORS = 64, STICK_BIT = 0x80000000u;
-constexpr uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2;
+constexpr uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, T_MOTOR = 3, T_LIMIT = 4;
 constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
                    FLAG_MASS_PENALTY = 16;
 constexpr float SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_CAPSULE = -1;
@@ -63,14 +63,18 @@ template <class R> struct Body {
   V3<R> angVel; R shape = 0;
 };
 
+/** The 32-word constraint record (layout.ts:27-36). The w words of c0Lin, c0Ang, rA and rB carry a ball joint's rest
+ * orientation (x, y, z, w); a spring keeps its rest length in rA.w and its break strain / force in frac / torqueArm;
+ * a motor or limit keeps its warm-start values in c0LinW / c0AngW and speed, torque bound or angles in rest / rBW. */
 template <class R> struct Joint {
   V3<R> penLin; R stiffLin = 0;
   V3<R> penAng; R stiffAng = 0;
   V3<R> lamLin; R frac = 0;
   V3<R> lamAng; R torqueArm = 0;
-  V3<R> c0Lin, c0Ang;
+  V3<R> c0Lin; R c0LinW = 0;
+  V3<R> c0Ang; R c0AngW = 0;
   V3<R> rA; R rest = 0;
-  V3<R> rB;
+  V3<R> rB; R rBW = 0;
   int32_t type = 0, a = -1, b = -1;
 };
 
@@ -652,9 +656,112 @@ template <class R> class World {
     const Body<R> &B = bodies[size_t(k.b)];
     return sub(anchorA(k), add(rotate(B.rot, k.rB), B.pos));
   }
+  /** withJointRestFrames (src/gpu/gpuSolverKernels.js:105-130): B is held at A's rotation times the captured rest
+   * orientation (A's rotation alone for a record without one). */
+  Q4<R> restTarget(const Joint<R> &k) const {
+    const Q4<R> t = k.a < 0 ? Q4<R>(0, 0, 0, 1) : bodies[size_t(k.a)].rot;
+    const Q4<R> r(k.c0LinW, k.c0AngW, k.rest, k.rBW);
+    if (!(qlengthSq(r) > R(0.5))) return t;
+    return Q4<R>((t.w * r.x + r.w * t.x) + (t.y * r.z - t.z * r.y), (t.w * r.y + r.w * t.y) + (t.z * r.x - t.x * r.z),
+                 (t.w * r.z + r.w * t.z) + (t.x * r.y - t.y * r.x), t.w * r.w - ((t.x * r.x + t.y * r.y) + t.z * r.z));
+  }
   V3<R> jointAngC(const Joint<R> &k) const {
-    Q4<R> ra = k.a < 0 ? Q4<R>(0, 0, 0, 1) : bodies[size_t(k.a)].rot;
-    return scale(qsub(ra, bodies[size_t(k.b)].rot), k.torqueArm);
+    return scale(qsub(restTarget(k), bodies[size_t(k.b)].rot), k.torqueArm);
+  }
+
+  // Hinge motors and limits (src/gpu/angularConstraints.js:16-83)
+  struct HingeFrame {
+    V3<R> axis, ta, tb;
+  };
+  HingeFrame hingeFrame(const Joint<R> &k) const {
+    const Q4<R> qa = k.a < 0 ? Q4<R>(0, 0, 0, 1) : bodies[size_t(k.a)].rot;
+    return HingeFrame{rotate(qa, k.rA), rotate(qa, k.c0Lin), rotate(bodies[size_t(k.b)].rot, k.c0Ang)};
+  }
+  static R hingeAngle(const HingeFrame &f) { return atan2R(dot(f.axis, cross(f.ta, f.tb)), dot(f.ta, f.tb)); }
+  static std::pair<R, R> hingeRows(const Joint<R> &k, R angle, R alpha, R dt) {
+    if (k.type == int32_t(T_MOTOR)) return {wrapAngleR(angle - k.c0LinW) - k.rest * dt, R(0)};
+    return {(angle - k.rest) - alpha * k.c0LinW, (angle - k.rBW) - alpha * k.c0AngW};
+  }
+  static std::pair<R, R> hingeForce(const Joint<R> &k, std::pair<R, R> C) {
+    const R fx = k.penLin.x * C.first + k.lamLin.x, fy = k.penLin.y * C.second + k.lamLin.y;
+    if (k.type == int32_t(T_MOTOR)) return {rmin(rmax(fx, -k.rBW), k.rBW), R(0)};
+    return {rmin(R(0), fx), rmax(R(0), fy)};
+  }
+  void warmHinge(Joint<R> &k) const {
+    if (k.stiffLin == 0) return;
+    const WorldParams &p = P(uint32_t(k.b));
+    const R angle = hingeAngle(hingeFrame(k));
+    k.c0LinW = k.type == int32_t(T_MOTOR) ? angle : rmin(R(0), angle - k.rest);
+    k.c0AngW = rmax(R(0), angle - k.rBW);
+    k.lamLin = scale(k.lamLin, R(p.alpha) * R(p.gamma));
+    const R g = R(p.gamma);
+    k.penLin = V3<R>(rclampW(k.penLin.x * g), rclampW(k.penLin.y * g), rclampW(k.penLin.z * g));
+  }
+  void dualHinge(Joint<R> &k, R alpha) const {
+    if (k.stiffLin == 0) return;
+    const WorldParams &p = P(uint32_t(k.b));
+    const std::pair<R, R> C = hingeRows(k, hingeAngle(hingeFrame(k)), alpha, R(p.dt));
+    const std::pair<R, R> F = hingeForce(k, C);
+    k.lamLin = V3<R>(F.first, F.second, R(0));
+    const bool motor = k.type == int32_t(T_MOTOR);
+    const R vx = motor ? std::fabs(C.first) : rmax(R(0), -C.first), vy = motor ? std::fabs(C.second) : rmax(R(0), C.second);
+    k.penLin.x = rmin(k.penLin.x + vx * R(p.betaAng), R(1e10f));
+    k.penLin.y = rmin(k.penLin.y + vy * R(p.betaAng), R(1e10f));
+  }
+  /** withSpringFracture (src/gpu/springMaterial.js:39-68): past its break strain or force a spring stops acting. */
+  void springFracture(Joint<R> &k) const {
+    const R strainLimit = k.frac, forceLimit = k.torqueArm;
+    if (!(k.stiffLin > 0 && (strainLimit > 0 || forceLimit > 0))) return;
+    const Body<R> &B = bodies[size_t(k.b)];
+    const R lengthNow = length(sub(anchorA(k), add(rotate(B.rot, k.rB), B.pos)));
+    const R extension = rmax(R(0), lengthNow - k.rest);
+    const R strain = extension / rmax(k.rest, R(1.0e-6f));
+    const R tension = k.stiffLin * extension;
+    if ((strainLimit > 0 && k.rest > R(1.0e-6f) && strain > strainLimit) || (forceLimit > 0 && tension > forceLimit)) {
+      k.penLin = V3<R>();
+      k.stiffLin = 0;
+      k.penAng = V3<R>();
+      k.stiffAng = 0;
+      k.lamLin = V3<R>();
+      k.lamAng = V3<R>();
+    }
+  }
+
+  /** The frames a new constraint takes from the current poses (constraintFrames.wgsl:11-19, angularFrames.wgsl): a
+   * ball joint with angular stiffness its rest orientation, a motor or limit its reference tangents. */
+  void captureFrame(Joint<R> &k) const {
+    const Q4<R> qa = k.a < 0 ? Q4<R>(0, 0, 0, 1) : bodies[size_t(k.a)].rot;
+    const Q4<R> qb = bodies[size_t(k.b)].rot;
+    auto perpendicular = [](V3<R> v) {
+      return std::fabs(v.x) > std::fabs(v.z) ? V3<R>(-v.y, v.x, 0) : V3<R>(0, -v.z, v.y);
+    };
+    if (k.type == int32_t(T_JOINT)) {
+      const Q4<R> a = conj(qa), b = qb;
+      const Q4<R> r((a.w * b.x + b.w * a.x) + (a.y * b.z - a.z * b.y), (a.w * b.y + b.w * a.y) + (a.z * b.x - a.x * b.z),
+                    (a.w * b.z + b.w * a.z) + (a.x * b.y - a.y * b.x), a.w * b.w - ((a.x * b.x + a.y * b.y) + a.z * b.z));
+      const R l = std::sqrt(qlengthSq(r));
+      k.c0LinW = r.x / l;
+      k.c0AngW = r.y / l;
+      k.rest = r.z / l;
+      k.rBW = r.w / l;
+    } else if (k.type == int32_t(T_MOTOR) || k.type == int32_t(T_LIMIT)) {
+      const V3<R> tangent = normalize(perpendicular(k.rA));
+      const V3<R> world = rotate(qa, tangent);
+      const V3<R> axisB = normalize(k.rB);
+      V3<R> tangentB = rotate(conj(qb), world);
+      tangentB = sub(tangentB, scale(axisB, dot(tangentB, axisB)));
+      if (length(tangentB) < R(1.0e-6f)) tangentB = perpendicular(axisB);
+      k.c0Lin = tangent;
+      k.c0LinW = 0;
+      k.c0Ang = normalize(tangentB);
+      k.c0AngW = 0;
+    }
+  }
+  /** Frames of every constraint that takes them at creation: ball joints with angular stiffness, motors, limits. */
+  void captureFrames() {
+    for (auto &k : joints)
+      if ((k.type == int32_t(T_JOINT) && k.stiffAng > 0) || k.type == int32_t(T_MOTOR) || k.type == int32_t(T_LIMIT))
+        captureFrame(k);
   }
   static bool hard(R stiffness) { return stiffness >= R(1e30f); }
 
@@ -662,6 +769,10 @@ template <class R> class World {
 #pragma omp parallel for schedule(static)
     for (size_t jj = 0; jj < joints.size(); jj++) {
       Joint<R> &k = joints[jj];
+      if (k.type == int32_t(T_MOTOR) || k.type == int32_t(T_LIMIT)) {
+        warmHinge(k);
+        continue;
+      }
       if (k.type != int32_t(T_JOINT)) continue;
       const WorldParams &p = P(uint32_t(k.b));
       k.c0Lin = jointLinC(k);
@@ -783,6 +894,16 @@ template <class R> class World {
   void addJoint(Acc &acc, const Joint<R> &k, R alpha, uint32_t i) const {
     bool isA = int64_t(i) == k.a;
     R sg = isA ? R(1) : R(-1);
+    if (k.type == int32_t(T_MOTOR) || k.type == int32_t(T_LIMIT)) {  // addAngular (angularConstraints.js:36-44)
+      if (k.stiffLin == 0 || (k.type == int32_t(T_MOTOR) && k.rBW == 0)) return;
+      const HingeFrame hf = hingeFrame(k);
+      const V3<R> axis = scale(hf.axis, isA ? R(-1) : R(1));
+      const std::pair<R, R> C = hingeRows(k, hingeAngle(hf), alpha, R(P(uint32_t(k.b)).dt));
+      const std::pair<R, R> F = hingeForce(k, C);
+      if (k.type == int32_t(T_MOTOR) || F.first != 0) addRow(acc, V3<R>(0, 0, 0), axis, k.penLin.x, F.first);
+      if (k.type == int32_t(T_LIMIT) && F.second != 0) addRow(acc, V3<R>(0, 0, 0), axis, k.penLin.y, F.second);
+      return;
+    }
     if (k.type == int32_t(T_SPRING)) {
       const Body<R> &A = bodies[size_t(k.a)], &B = bodies[size_t(k.b)];
       V3<R> rAW = rotate(A.rot, k.rA), rBW = rotate(B.rot, k.rB);
@@ -914,6 +1035,14 @@ template <class R> class World {
   }
 
   void dualJoint(Joint<R> &k, R alpha) {
+    if (k.type == int32_t(T_SPRING)) {
+      springFracture(k);
+      return;
+    }
+    if (k.type == int32_t(T_MOTOR) || k.type == int32_t(T_LIMIT)) {
+      dualHinge(k, alpha);
+      return;
+    }
     if (k.type != int32_t(T_JOINT)) return;
     const WorldParams &p = P(uint32_t(k.b));
     if (dot(k.penLin, k.penLin) > 0) {
@@ -1022,7 +1151,8 @@ template <class R> class World {
           finishBody(i, acc);
         }
 #pragma omp parallel for schedule(static)
-      for (size_t j = 0; j < joints.size(); j++) dualJoint(joints[j], R(P(uint32_t(joints[j].b)).alpha));
+      for (size_t j = 0; j < joints.size(); j++)
+        if (joints[j].type != int32_t(T_NONE)) dualJoint(joints[j], R(P(uint32_t(joints[j].b)).alpha));
 #pragma omp parallel for schedule(dynamic, 256)
       for (uint32_t m = 0; m < manifolds.size(); m++) dualManifold(m, R(P(manifolds[m].a).alpha));
     }

@@ -234,6 +234,41 @@ inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3
   xLin.x = z1 - L21 * xLin.y - L31 * xLin.z - L41 * xAng.x - L51 * xAng.y - L61 * xAng.z;
 }
 
+/**
+ * atan2 and the wrap of an angle into [−π, π) for the hinge rows (angularConstraints.js:21-28 uses WGSL atan2, sin and
+ * cos). The FP32 forms use FP32 arithmetic only, as the GPU step does: atan(t) on [0, 1] by the reduction
+ * atan t = 30° + atan((√3 t − 1)/(√3 + t)) above tan 15° and the odd Taylor series to u¹¹ (error below 3e-9), octants
+ * by symmetry; the wrap as x − 2π ⌊(x + π) / 2π⌋ (the value atan2(sin x, cos x) takes). The FP64 forms are the library's.
+ */
+inline float atanUnitF(float t) {
+  const float TAN15 = 0.267949192f, SQRT3 = 1.73205081f, PI6 = 0.523598776f;
+  float offset = 0.f, u = t;
+  if (t > TAN15) {
+    u = (t * SQRT3 - 1.f) / (SQRT3 + t);
+    offset = PI6;
+  }
+  const float u2 = u * u;
+  float p = -1.f / 11.f;
+  p = p * u2 + 1.f / 9.f;
+  p = p * u2 - 1.f / 7.f;
+  p = p * u2 + 1.f / 5.f;
+  p = p * u2 - 1.f / 3.f;
+  p = p * u2 + 1.f;
+  return offset + u * p;
+}
+inline float atan2R(float y, float x) {
+  const float ax = std::fabs(x), ay = std::fabs(y);
+  if (ax == 0.f && ay == 0.f) return 0.f;
+  const bool steep = ay > ax;
+  float a = atanUnitF(steep ? ax / ay : ay / ax);
+  if (steep) a = 1.57079633f - a;
+  if (x < 0.f) a = 3.14159265f - a;
+  return y < 0.f ? -a : a;
+}
+inline double atan2R(double y, double x) { return std::atan2(y, x); }
+inline float wrapAngleR(float x) { return x - 6.28318531f * std::floor((x + 3.14159265f) / 6.28318531f); }
+inline double wrapAngleR(double x) { return x - 2 * M_PI * std::floor((x + M_PI) / (2 * M_PI)); }
+
 /** lowbias32 integer hash (reference/three-avbd/src/avbd2d/gpu/layout.ts:106-115). */
 inline uint32_t hash32(uint32_t x) {
   x ^= x >> 16;

@@ -57,6 +57,8 @@ def lib():
             "gs_set_pair_capacity": (None, [vp, ctypes.c_uint32]),
             "gs_overflow": (ctypes.c_uint32, [vp]),
             "gs_partial_step": (None, [vp, ctypes.c_int]),
+            "gs_capture_frames": (None, [vp, u32p, ctypes.c_uint32]),
+            "gs_edit_joints": (None, [vp, u32p, ctypes.c_uint32]),
             "gs_set_state": (None, [vp, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
             "gs_step": (None, [vp, ctypes.c_int]),
             "gs_counts": (None, [vp, u32p]),
@@ -181,6 +183,7 @@ class OracleWorld:
         if brute_force:
             L.gs_set_brute_force(self.h, 1)
         self._pair_capacity = 0
+        L.gs_capture_frames(self.h, None, 0)  # as a new GPU world does from its starting poses
         self._restitution = None
         self._sleeping = None
         bouncy, base = [], 0
@@ -271,6 +274,52 @@ class OracleWorld:
 
     def step(self, n=1):
         lib().gs_step(self.h, n)
+
+    # --- constraint edits, as the GPU world applies them (the same record words) ----------------------------------
+    def _edit(self, triples):
+        t = np.ascontiguousarray(np.asarray(triples, np.uint32).reshape(-1, 3))
+        if len(t):
+            lib().gs_edit_joints(self.h, P(t, ctypes.c_uint32), len(t))
+
+    def release_joints(self, slots):
+        t = []
+        for s in sorted(set(int(x) for x in slots)):
+            if s < self.nj and self.info[s, 0] != 0:
+                t += [(s, w, 0) for w in range(32)] + [(s, 32, 0), (s, 33, 0xFFFFFFFF), (s, 34, 0xFFFFFFFF)]
+                self.info[s] = (0, -1, -1, 0)  # (the current types: a later seed() keeps them)
+        self._edit(t)
+
+    def disable_constraint(self, slot):
+        self._edit([(slot, w, 0) for w in (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14)])
+
+    def set_world_anchor(self, slot, point):
+        p = np.asarray(point, np.float32).view(np.uint32)
+        self._edit([(slot, 24 + k, int(p[k])) for k in range(3)])
+
+    def set_motor(self, slot, speed=None, max_torque=None):
+        t = []
+        if speed is not None:
+            t.append((slot, 27, int(np.float32(speed).view(np.uint32))))
+        if max_torque is not None:
+            t.append((slot, 31, int(np.float32(max_torque).view(np.uint32))))
+        self._edit(t)
+
+    def set_spring_material(self, slot, break_strain, break_force):
+        v = [0.0 if np.isinf(x) else x for x in (break_strain, break_force)]
+        self._edit([(slot, 11, int(np.float32(v[0]).view(np.uint32))), (slot, 15, int(np.float32(v[1]).view(np.uint32)))])
+
+    def set_angular_constraint(self, slot, kind, axis_a, axis_b, lo, hi):
+        rec = np.zeros(32, np.float32)
+        rec[0] = rec[1] = 1000.0
+        rec[3] = 3e38
+        for k, axis in ((24, axis_a), (28, axis_b)):
+            a = np.asarray(axis, np.float32).astype(np.float64)
+            rec[k:k + 3] = a / np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+        rec[27], rec[31] = lo, hi
+        self._edit([(slot, w, int(rec.view(np.uint32)[w])) for w in range(32)] + [(slot, 32, int(kind))])
+        self.info[slot, 0] = int(kind)
+        s = np.array([slot], np.uint32)
+        lib().gs_capture_frames(self.h, P(s, ctypes.c_uint32), 1)
 
     def partial_step(self, stage):
         """Stage 1: collision and topology only (contacts as warm-started); stage 2: also the warm starts."""
