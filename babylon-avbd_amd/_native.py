@@ -20,6 +20,7 @@ FLAG_REUSE_CONTACTS = 4
 FLAG_START_AT_REST = 8
 FLAG_MASS_PENALTY = 16
 FLAGS_DEFAULT = 31
+FLAG_CAPSULE_STEP_START_ARMS = 32  # this build's option, off by default (the reference turns capsule lever arms)
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 i32p = ctypes.POINTER(ctypes.c_int32)

@@ -35,7 +35,8 @@ constexpr uint32_t MAX_COLORS = 64, NO_COLOR = 255, PENDING = 256;
 constexpr int32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, T_MOTOR = 3, T_LIMIT = 4;
 constexpr float SHAPE_BOX = 0.f, SHAPE_SPHERE = 1.f, SHAPE_SAIL = 2.f, SHAPE_CAPSULE = -1.f;
 constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
-                   FLAG_MASS_PENALTY = 16;
+                   FLAG_MASS_PENALTY = 16,
+                   FLAG_CAPSULE_STEP_START_ARMS = 32;  // this build's option: capsule lever arms at x⁻ like spheres
 constexpr int MAX_CONTACTS = 8;
 constexpr int MAX_POLY = 8;
 constexpr float SAT_AXIS_EPSILON = 1.0e-6f;

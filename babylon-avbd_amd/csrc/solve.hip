@@ -335,12 +335,14 @@ AVBD_D float rowC(float c0, float keep, f3 l, const Eval &e, const PairBody &A, 
 
 /** evalContact (wgsl-solve.ts:171-206). */
 AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, const PairBody &A, const PairBody &B,
-                        float alpha) {
+                        float alpha, uint32_t flags) {
   uint32_t key = fbits(k.rA.w);
-  // Rounded shapes (the sphere key, wgsl-solve.ts:172-181, and the capsule keys 4<<24 | j) take their lever arms at the
-  // step-start rotation: a rolling sphere or capsule would otherwise see a false separation in its normal row
+  // Sphere contacts take their lever arms at the step-start rotation (wgsl-solve.ts:172-181): a rolling sphere would
+  // otherwise see a false separation in its normal row. Capsule contacts (keys 4<<24 | j) do so only with
+  // FLAG_CAPSULE_STEP_START_ARMS; by default they use the current rotation, as the reference does
   const uint32_t feature = key & ~STICK_BIT;
-  const bool rounded = feature == SPHERE_FEATURE || (feature >> 24) == (CAPSULE_FEATURE >> 24);
+  const bool rounded = feature == SPHERE_FEATURE ||
+                       ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (feature >> 24) == (CAPSULE_FEATURE >> 24));
   Eval e;
   e.rAW = qrotate(rounded ? A.irot : A.rot, xyz(k.rA));
   e.rBW = qrotate(rounded ? B.irot : B.rot, xyz(k.rB));
@@ -370,7 +372,7 @@ AVBD_D bool finishesPair(uint32_t ca, uint32_t cb) { return ca > cb; }
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
 template <bool DF>
 AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
-                    uint32_t p, float alpha, const DfCtx &df) {
+                    uint32_t p, float alpha, uint32_t flags, const DfCtx &df) {
   if (head.z & ENTRY_JOINT) {
     addJoint(w, acc, head.y, alpha, i);  // the dataflow solve runs only for batches without joints
     return;
@@ -400,7 +402,7 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
     k.lam = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_LAM));
   }
-  Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha);
+  Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha, flags);
   f3 r = isA ? ev.rAW : ev.rBW;
   addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
   addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
@@ -417,7 +419,7 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
  */
 template <uint32_t W, bool DF>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
-                       float dt, Acc &acc, const DfCtx &df) {
+                       float dt, uint32_t flags, Acc &acc, const DfCtx &df) {
   const Body &B = w.bodies[i];
   if constexpr (DF) {
     if (df.it) waitFor(w, &w.dualDone[i], df.base + df.it);  // the body's own last iteration and its duals
@@ -469,7 +471,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
                                  __shfl(head.w, owner, W));
       const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W), 0u);
       const uint32_t p = t - __shfl(excl, owner, W);
-      if (t < total) addItem<DF>(w, acc, i, self, h, tl, p, alpha, df);
+      if (t < total) addItem<DF>(w, acc, i, self, h, tl, p, alpha, flags, df);
     }
   }
 }
@@ -690,7 +692,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
         kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
         kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
         const Eval ev = evalContact(kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
-                                    isA ? other : self, p.alpha);
+                                    isA ? other : self, p.alpha, p.flags);
         dualUpdate<true>(w, c, kc, ev, p);
       }
     }
@@ -710,7 +712,7 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
   Acc acc;
   if (live) {
     const WorldParams &p = w.worlds[slot.w];
-    accumulate<W, DF>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, acc, df);
+    accumulate<W, DF>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df);
   } else {
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
@@ -807,7 +809,7 @@ AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
   const Contact kc = w.contacts[c];
-  dualUpdate<false>(w, c, kc, evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha), p);
+  dualUpdate<false>(w, c, kc, evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */

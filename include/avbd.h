@@ -46,6 +46,10 @@ extern "C" {
 #define AVBD_FLAG_START_AT_REST 8u
 #define AVBD_FLAG_MASS_PENALTY 16u
 #define AVBD_FLAGS_DEFAULT 31u
+/* No counterpart in the reference (off by default): capsule contacts take their lever arms at the step-start rotation,
+ * as sphere contacts do (wgsl-solve.ts:172-181); the reference turns capsule lever arms with the body, which lets a
+ * capsule rolling hard on its side sink into what it rests on. */
+#define AVBD_FLAG_CAPSULE_STEP_START_ARMS 32u
 
 typedef struct avbd_world avbd_world;
 

@@ -33,7 +33,8 @@ namespace gpu {
 constexpr uint32_t NO_COLOR = 255, PENDING = 256, MAX_COLORS = 64, STICK_BIT = 0x80000000u;
 constexpr uint32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, T_MOTOR = 3, T_LIMIT = 4;
 constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
-                   FLAG_MASS_PENALTY = 16;
+                   FLAG_MASS_PENALTY = 16,
+                   FLAG_CAPSULE_STEP_START_ARMS = 32;  // not the reference's: capsule lever arms at x⁻ too
 constexpr float SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_CAPSULE = -1;
 
 /** Reference record layouts (reference/three-avbd/src/avbd3d/gpu/layout.ts:8-55, 127-169). */
@@ -96,14 +97,14 @@ struct RefPose { float pos[4] = {0, 0, 0, 0}; float rot[4] = {0, 0, 0, 0}; };
 template <class R> inline R toR(float x) { return R(x); }
 
 /**
- * Contact keys of rounded shapes: the sphere key (wgsl-solve.ts:172-181 evaluates its lever arms at the step-start
- * rotation, the Taylor point) and the capsule keys 4<<24 | j, which take the same rule here: a capsule rolling on its
- * side has the sphere's problem (a lever arm turned with the body puts a false separation into the normal row, and the
- * capsule sinks while gaining speed).
+ * Contacts whose lever arms are taken at the step-start rotation (the Taylor point): the sphere key, as
+ * wgsl-solve.ts:172-181 does (a lever arm turned with a rolling sphere puts a false separation into the normal row);
+ * the capsule keys 4<<24 | j only with FLAG_CAPSULE_STEP_START_ARMS (an option of this build: the reference evaluates
+ * capsule contacts at the current rotation, and a capsule rolling on its side then has the sphere's problem).
  */
-inline bool roundedKey(uint32_t key) {
+inline bool stepStartArms(uint32_t key, uint32_t flags) {
   uint32_t k = key & ~STICK_BIT;
-  return k == SPHERE_FEATURE || (k >> 24) == (CAPSULE_FEATURE >> 24);
+  return k == SPHERE_FEATURE || ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (k >> 24) == (CAPSULE_FEATURE >> 24));
 }
 
 /**
@@ -855,8 +856,7 @@ template <class R> class World {
   Eval evalContact(const Contact<R> &k, const M3<R> &basis, R friction, const PairBody &A, const PairBody &B,
                    R alpha) const {
     Q4<R> rotA = A.rot, rotB = B.rot;
-    // Rounded-shape contacts (sphere, and here also capsule keys) take the lever arms at the step-start rotation
-    if (roundedKey(k.key)) {
+    if (stepStartArms(k.key, P(A.index).flags)) {
       rotA = bodies[A.index].irot;
       rotB = bodies[B.index].irot;
     }

@@ -299,9 +299,16 @@ def test_jointed_mix_steps_stay_in_parity(avbd):
     world.destroy()
 
 
-def test_jointed_mix_trajectory_bit_exact(avbd):
-    world = _world(avbd, _small_mix(avbd))
-    oracle = OracleWorld([_small_mix(avbd)], precision=32, color_cap=32, color_rounds=16)
+@pytest.mark.parametrize("capsule_arms", [False, True], ids=["reference-capsule-arms", "step-start-capsule-arms"])
+def test_jointed_mix_trajectory_bit_exact(avbd, capsule_arms):
+    """300 steps of the small mix, with the reference's capsule lever arms and with AVBD_FLAG_CAPSULE_STEP_START_ARMS."""
+    def scene():
+        m = _small_mix(avbd)
+        if capsule_arms:
+            m.params["flags"] |= avbd._native.FLAG_CAPSULE_STEP_START_ARMS
+        return m
+    world = _world(avbd, scene())
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
     world.step(300)
     oracle.step(300)
     assert np.array_equal(world.readBodies()[:, 0:8], oracle.bodies()[:, 0:8].astype(np.float32))
