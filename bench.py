@@ -113,42 +113,64 @@ def host_threads():
 
 
 def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
-    """The CPU restatement (FP32) of the same step, started from the GPU's state at the start of the timed window and
-    run for a bounded sample (whole steps until `budget_s` seconds or `max_steps`): once on every host core (the
-    colour-parallel OpenMP variant: bodies of a colour, pairs and manifolds in parallel, the same results) and once
-    on one thread. The all-cores figure is the baseline."""
+    """Two CPU figures for the same world, each started from the GPU's state at the start of the timed window and run
+    for a bounded sample (whole steps until `budget_s` seconds or `max_steps`):
+
+    * the baseline: the FP64 restatement of the reference's own CPU path (CpuSolver3D.step, cpuSolver3D.ts:144-280,
+      with cpuBroadphase3D's hash grid and the face-biased box SAT), on every host core in its colour-parallel form
+      (greedy body colours, a colour's Newton steps and every force's dual in parallel), and on one thread as the
+      reference runs it (sequential Gauss-Seidel, newest body first); it starts from the GPU's poses and velocities with
+      no contact history, as the reference's solver does for a world it is handed;
+    * `fp32_port`: the FP32 restatement of this library's own step (the same pairs, colours and order as the HIP
+      path, bit-identical to it), seeded with the GPU's whole state, on every host core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # the checker, timed here as the CPU baseline
 
     scene = scene_fn()
     n = scene.body_count
-
-    def timed(threads):
-        oracle_lib.set_threads(threads)
-        try:
-            oracle = oracle_lib.OracleWorld([scene], precision=32, color_cap=colour_cap, color_rounds=16)
-            oracle.seed(state)
-            steps = 0
-            t0 = time.perf_counter()
-            while steps < max_steps:
-                oracle.step(1)
-                steps += 1
-                if time.perf_counter() - t0 > budget_s:
-                    break
-            return steps, time.perf_counter() - t0
-        finally:
-            oracle_lib.set_threads(1)
-
     cores = host_threads()
-    steps_n, dt_n = timed(cores)
-    steps_1, dt_1 = timed(1)
-    what = (f"of the same {n}-body world from the GPU's state (the timed window's start), FP32 restatement of the "
-            f"step (grid broadphase, colour-ordered Gauss-Seidel)")
-    return dict(value=n * steps_n / dt_n, unit="body-steps/s", cores=cores, kind="port", cpu=cpu_model(),
-                sample=f"steps {first_step + 1}-{first_step + steps_n} {what}, {cores} OpenMP threads "
-                       f"(colour-parallel), {dt_n:.1f} s",
-                single_thread=dict(value=n * steps_1 / dt_1, unit="body-steps/s", cores=1,
-                                   sample=f"steps {first_step + 1}-{first_step + steps_1}, one thread, {dt_1:.1f} s"))
+
+    def run(world_step):
+        steps = 0
+        t0 = time.perf_counter()
+        while steps < max_steps:
+            world_step()
+            steps += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return steps, time.perf_counter() - t0
+
+    def reference_path(threads):
+        world = oracle_lib.CpuPathWorld(scene, threads=threads, bodies=state["bodies"])
+        steps, dt = run(world.step)
+        manifolds = world.counts()["manifolds"]
+        del world
+        return steps, dt, manifolds
+
+    steps_n, dt_n, mf_n = reference_path(cores)
+    steps_1, dt_1, mf_1 = reference_path(1)
+
+    oracle_lib.set_threads(cores)
+    try:
+        oracle = oracle_lib.OracleWorld([scene], precision=32, color_cap=colour_cap, color_rounds=16)
+        oracle.seed(state)
+        steps_p, dt_p = run(lambda: oracle.step(1))
+        del oracle
+    finally:
+        oracle_lib.set_threads(1)
+    span = lambda k: f"steps {first_step + 1}-{first_step + k}"
+    return dict(
+        value=n * steps_n / dt_n, unit="body-steps/s", cores=cores, kind="port", cpu=cpu_model(),
+        sample=(f"{span(steps_n)} of the same {n}-body world from the GPU's poses and velocities: FP64 restatement of "
+                f"the reference's CPU path (CpuSolver3D.step with cpuBroadphase3D and the face-biased SAT), "
+                f"colour-parallel on {cores} OpenMP threads, {dt_n:.1f} s, {mf_n} manifolds at the end"),
+        single_thread=dict(value=n * steps_1 / dt_1, unit="body-steps/s", cores=1,
+                           sample=f"{span(steps_1)}, the same restatement on one thread (sequential Gauss-Seidel, "
+                                  f"newest body first, as the reference), {dt_1:.1f} s, {mf_1} manifolds at the end"),
+        fp32_port=dict(value=n * steps_p / dt_p, unit="body-steps/s", cores=cores,
+                       sample=f"{span(steps_p)}, FP32 restatement of this library's step (same pairs, colours and "
+                              f"order as the HIP path), seeded with the GPU's state, {cores} OpenMP threads, "
+                              f"{dt_p:.1f} s"))
 
 
 def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):

@@ -307,6 +307,107 @@ double ref_world_joint_error(void *h, int b, const double *target, const double 
   V pB = transform(body->positionLin, body->positionAng, V(local[0], local[1], local[2]));
   return length(sub(V(target[0], target[1], target[2]), pB));
 }
+/**
+ * The package's CPU path (CpuSolver3D.step) over packed scene records: bodies 40 floats each (boxes only: a sphere or
+ * capsule makes it return NULL), joints 32 floats + info (type 1 joint / 2 spring, a or -1, b); params 16 floats
+ * (dt, gravity, betaLin, betaAng, gamma, alpha, up xyz, flags bits: 2 face bias, 16 mass penalty). threads > 1: the
+ * colour-parallel variant.
+ */
+void *cpu_create(uint32_t nBodies, const float *bodyRecs, uint32_t nJoints, const float *jointRecs,
+                 const int32_t *info, const float *params, int iterations, int threads) {
+  auto s = std::make_unique<Solver>();
+  s->package = true;
+  s->dt = params[0];
+  s->gravity = params[1];
+  s->betaLin = params[2];
+  s->betaAng = params[3];
+  s->gamma = params[4];
+  s->alpha = params[5];
+  s->up = V(params[6], params[7], params[8]);
+  uint32_t flags;
+  std::memcpy(&flags, params + 9, 4);
+  s->faceBias = (flags & 2u) != 0;
+  s->massPenalty = (flags & 16u) != 0;
+  s->iterations = iterations;
+  s->threads = threads < 1 ? 1 : threads;
+  for (uint32_t i = 0; i < nBodies; i++) {
+    const float *r = bodyRecs + 40ull * i;
+    if (r[39] != 0.f) return nullptr;  // boxes only
+    Rigid *b = s->addBody(V(r[16], r[17], r[18]), 0.0, r[3], V(r[0], r[1], r[2]), V(r[32], r[33], r[34]));
+    b->positionAng = Q(r[4], r[5], r[6], r[7]);
+    b->mass = r[19];
+    b->moment = V(r[20], r[21], r[22]);
+    b->radius = r[23];
+    b->velocityAng = V(r[36], r[37], r[38]);
+  }
+  auto fin = [](float x) { return x >= 1e30f ? INF : double(x); };
+  for (uint32_t j = 0; j < nJoints; j++) {
+    const float *k = jointRecs + 32ull * j;
+    const int32_t *f = info + 4ull * j;
+    Rigid *a = f[1] >= 0 ? s->bodies[size_t(f[1])] : nullptr;
+    Rigid *b = s->bodies[size_t(f[2])];
+    if (f[0] == 1)
+      s->addForce<Joint>(a, b, V(k[24], k[25], k[26]), V(k[28], k[29], k[30]), fin(k[3]), fin(k[7]), fin(k[11]));
+    else if (f[0] == 2)
+      s->addForce<Spring>(a, b, V(k[24], k[25], k[26]), V(k[28], k[29], k[30]), double(k[3]), double(k[27]));
+    else
+      return nullptr;
+  }
+  return s.release();
+}
+void cpu_destroy(void *h) { delete static_cast<Solver *>(h); }
+void cpu_step(void *h, int n) {
+  auto *s = static_cast<Solver *>(h);
+  for (int i = 0; i < n; i++) s->step();
+}
+void cpu_set_threads(void *h, int threads) { static_cast<Solver *>(h)->threads = threads < 1 ? 1 : threads; }
+/** Bodies as 40 values each in the record layout (pos, friction, rot, size, mass, moment, radius, vel, angVel). */
+void cpu_read_bodies(void *h, double *out) {
+  auto *s = static_cast<Solver *>(h);
+  for (size_t i = 0; i < s->bodies.size(); i++) {
+    const Rigid *b = s->bodies[i];
+    double *o = out + 40 * i;
+    for (int k = 0; k < 40; k++) o[k] = 0;
+    o[0] = b->positionLin.x; o[1] = b->positionLin.y; o[2] = b->positionLin.z; o[3] = b->friction;
+    o[4] = b->positionAng.x; o[5] = b->positionAng.y; o[6] = b->positionAng.z; o[7] = b->positionAng.w;
+    o[16] = b->size.x; o[17] = b->size.y; o[18] = b->size.z; o[19] = b->mass;
+    o[20] = b->moment.x; o[21] = b->moment.y; o[22] = b->moment.z; o[23] = b->radius;
+    o[32] = b->velocityLin.x; o[33] = b->velocityLin.y; o[34] = b->velocityLin.z;
+    o[36] = b->velocityAng.x; o[37] = b->velocityAng.y; o[38] = b->velocityAng.z;
+  }
+}
+/** Live forces by kind: out[0] manifolds, out[1] joints, out[2] springs; returns the contact points. */
+uint32_t cpu_counts(void *h, uint32_t *out) {
+  auto *s = static_cast<Solver *>(h);
+  uint32_t points = 0;
+  out[0] = out[1] = out[2] = 0;
+  for (const Force *f : s->forces) {
+    const int k = f->kind();
+    if (k < 3) out[k]++;
+    if (k == 0) points += uint32_t(static_cast<const Manifold *>(f)->contacts.size());
+  }
+  return points;
+}
+/** The broadphase alone (manifolds created for the new pairs, not initialised), for checks of its pair set. */
+void cpu_broadphase(void *h) { static_cast<Solver *>(h)->packageBroadphase(); }
+/** The manifold pairs (a > b as body indices), for checks of the broadphase. */
+uint32_t cpu_read_pairs(void *h, uint32_t *out) {
+  auto *s = static_cast<Solver *>(h);
+  std::unordered_map<const Rigid *, uint32_t> index;
+  for (uint32_t i = 0; i < s->bodies.size(); i++) index[s->bodies[i]] = i;
+  uint32_t n = 0;
+  for (const Force *f : s->forces)
+    if (f->kind() == 0) {
+      if (out) {
+        const uint32_t a = index[f->bodyA], b = index[f->bodyB];
+        out[2 * n] = std::max(a, b);
+        out[2 * n + 1] = std::min(a, b);
+      }
+      n++;
+    }
+  return n;
+}
+
 /** Mass properties of a box of the given size and density (body.ts:38-47): mass, moment xyz, radius. */
 void ref_box_mass(const double *size, double density, double *out) {
   Solver s;

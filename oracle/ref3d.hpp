@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <limits>
 #include <memory>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "collide.hpp"
@@ -87,6 +89,15 @@ struct Contact {
 struct Solver {
   R dt = 1.0 / 60.0, gravity = -10, alpha = 0.99, betaLin = 10000, betaAng = 100, gamma = 0.999;
   int iterations = 10;
+  // The package's CPU path (src/native/cpu/cpuSolver3D.ts:144-280, cpuPhysics3D.js:67-147, 304-363): gravity along
+  // `up`, the contact mass-penalty floor, the face-biased box SAT and the hash-grid broadphase. Off: the upstream demo.
+  bool package = false;
+  V up = V(0, 0, 1);
+  bool massPenalty = true, faceBias = true;
+  // The colour-parallel variant of the package step (threads > 1): the same equations, each body's Newton step taken
+  // with the bodies of its greedy colour in parallel (Gauss–Seidel by colour instead of by index), forces initialised
+  // and duals updated in parallel. threads == 1: the reference's sequential order.
+  int threads = 1;
   std::vector<std::unique_ptr<Rigid>> bodyStore;
   std::vector<Rigid *> bodies;
   std::vector<Force *> forces;                    // live, creation order
@@ -122,6 +133,9 @@ struct Solver {
     return raw;
   }
   void step();
+  void packageBroadphase();
+  void packageStep();
+  std::vector<uint32_t> colors;  // the colour-parallel variant's body colours of the current step
 };
 
 /** manifold.ts:67-218. */
@@ -138,7 +152,7 @@ struct Manifold : Force {
     Box<R> bA = makeBox(A->positionLin, A->positionAng, A->size);
     Box<R> bB = makeBox(B->positionLin, B->positionAng, B->size);
     Sat<R> sat;
-    Found<R> found = collideBoxes(bA, bB, false, 16, sat);
+    Found<R> found = collideBoxes(bA, bB, solver->package && solver->faceBias, 16, sat);
     if (found.count) basis = orthonormal(neg(sat.n));
     std::vector<Contact> fresh;
     for (int i = 0; i < found.count; i++) {
@@ -173,6 +187,10 @@ struct Manifold : Force {
       for (int i = 0; i < 3; i++) {
         c.lambda[i] = c.lambda[i] * solver->alpha * solver->gamma;
         c.penalty[i] = rclamp(c.penalty[i] * solver->gamma, PENALTY_MIN, PENALTY_MAX);
+      }
+      if (solver->package && solver->massPenalty) {  // CpuManifold3D.initialize (cpuPhysics3D.js:130-139)
+        const R mass = std::min(A->mass > 0 ? A->mass : INF, B->mass > 0 ? B->mass : INF);
+        c.penalty.x = std::max(c.penalty.x, rclamp(mass / (solver->dt * solver->dt), PENALTY_MIN, PENALTY_MAX));
       }
     }
     return !contacts.empty();
@@ -411,8 +429,12 @@ struct IgnoreCollision : Force {
   void updateDual(R) override {}
 };
 
-/** solver.ts:134-219. */
+/** solver.ts:134-219 (the package's CPU step when `package` is set). */
 inline void Solver::step() {
+  if (package) {
+    packageStep();
+    return;
+  }
   // Broadphase: O(n²) bounding spheres, newest first
   for (int i = int(bodies.size()) - 1; i >= 0; i--) {
     Rigid *A = bodies[i];
@@ -470,6 +492,207 @@ inline void Solver::step() {
     for (Force *f : forces) f->updateDual(alpha);
   }
 
+  for (Rigid *body : bodies) {
+    body->prevVelocityLin = body->velocityLin;
+    if (body->mass > 0) {
+      body->velocityLin = divs(sub(body->positionLin, body->initialLin), dt);
+      body->velocityAng = divs(qsub(body->positionAng, body->initialAng), dt);
+    }
+  }
+}
+
+/**
+ * cpuBroadphase3D (cpuPhysics3D.js:304-363): a hash grid of cell = max(0.1, 2 × the median bounding radius of the first
+ * 1,024 dynamic bodies); each body, in index order, is binned into every cell its bounding sphere's box touches and
+ * paired with the bodies already there (a body touching more than 128 cells is paired with every body instead); a pair
+ * (higher, lower index), not both static and not seen before, whose bounding spheres overlap and that no force links
+ * yet, gets a new manifold.
+ */
+inline void Solver::packageBroadphase() {
+  const size_t n = bodies.size();
+  std::vector<R> radii;
+  for (Rigid *b : bodies)
+    if (b->mass > 0 && radii.size() < 1024) radii.push_back(b->radius);
+  std::sort(radii.begin(), radii.end());
+  const R cell = std::max(R(0.1), (radii.empty() ? R(1) : radii[radii.size() / 2]) * 2);
+  // Cell lists in insertion order (the reference's Map of arrays keyed by "x,y,z") as open-addressing slots whose
+  // entries chain head → tail; the seen-pair set as an open-addressing table of (hi · n + lo) + 1
+  size_t slots = 1024;
+  while (slots < 8 * n + 1024) slots *= 2;
+  struct Cell {
+    int64_t x, y, z;
+    int32_t head = -1, tail = -1;
+  };
+  std::vector<Cell> cells(slots);
+  std::vector<char> used(slots, 0);
+  std::vector<int32_t> entryBody, entryNext;
+  auto hashCell = [&](int64_t x, int64_t y, int64_t z) {
+    uint64_t h = uint64_t(x) * 0x9E3779B97F4A7C15ull ^ uint64_t(y) * 0xC2B2AE3D27D4EB4Full ^ uint64_t(z) * 0x165667B19E3779F9ull;
+    return size_t(h ^ (h >> 29)) & (slots - 1);
+  };
+  size_t seenSlots = 1024;
+  while (seenSlots < 32 * n + 1024) seenSlots *= 2;
+  std::vector<uint64_t> seen(seenSlots, 0);
+  size_t seenCount = 0;
+  auto firstTime = [&](uint64_t key) {
+    if (2 * (seenCount + 1) > seenSlots) {  // grow (rehash) past half full
+      std::vector<uint64_t> old;
+      old.swap(seen);
+      seenSlots *= 2;
+      seen.assign(seenSlots, 0);
+      for (uint64_t k : old)
+        if (k) {
+          size_t h = size_t((k * 0x9E3779B97F4A7C15ull) >> 20) & (seenSlots - 1);
+          while (seen[h]) h = (h + 1) & (seenSlots - 1);
+          seen[h] = k;
+        }
+    }
+    const uint64_t k = key + 1;
+    size_t h = size_t((k * 0x9E3779B97F4A7C15ull) >> 20) & (seenSlots - 1);
+    while (seen[h]) {
+      if (seen[h] == k) return false;
+      h = (h + 1) & (seenSlots - 1);
+    }
+    seen[h] = k;
+    seenCount++;
+    return true;
+  };
+  std::vector<uint32_t> large;
+  auto pair = [&](uint32_t a, uint32_t b) {
+    if (a == b || (bodies[a]->mass <= 0 && bodies[b]->mass <= 0)) return;
+    const uint32_t hi = std::max(a, b), lo = std::min(a, b);
+    if (!firstTime(uint64_t(hi) * n + lo)) return;
+    Rigid *A = bodies[hi], *B = bodies[lo];
+    const R r = A->radius + B->radius;
+    const V d = sub(A->positionLin, B->positionLin);
+    if (dot(d, d) <= r * r && !A->constrainedTo(B)) addForce<Manifold>(A, B);
+  };
+  for (uint32_t i = 0; i < n; i++) {
+    const Rigid *b = bodies[i];
+    int64_t lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+      lo[k] = int64_t(std::floor((b->positionLin[k] - b->radius) / cell));
+      hi[k] = int64_t(std::floor((b->positionLin[k] + b->radius) / cell));
+    }
+    if ((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) > 128) {
+      large.push_back(i);
+      continue;
+    }
+    for (int64_t x = lo[0]; x <= hi[0]; x++)
+      for (int64_t y = lo[1]; y <= hi[1]; y++)
+        for (int64_t z = lo[2]; z <= hi[2]; z++) {
+          size_t h = hashCell(x, y, z);
+          while (used[h] && !(cells[h].x == x && cells[h].y == y && cells[h].z == z)) h = (h + 1) & (slots - 1);
+          Cell &c = cells[h];
+          if (!used[h]) {
+            used[h] = 1;
+            c.x = x;
+            c.y = y;
+            c.z = z;
+          }
+          for (int32_t e = c.head; e >= 0; e = entryNext[size_t(e)]) pair(i, uint32_t(entryBody[size_t(e)]));
+          const int32_t e = int32_t(entryBody.size());
+          entryBody.push_back(int32_t(i));
+          entryNext.push_back(-1);
+          if (c.tail >= 0) entryNext[size_t(c.tail)] = e;
+          else c.head = e;
+          c.tail = e;
+        }
+  }
+  for (uint32_t i : large)
+    for (uint32_t j = 0; j < n; j++) pair(i, j);
+}
+
+/**
+ * CpuSolver3D.step (cpuSolver3D.ts:144-280) for boxes, joints and springs: broadphase, force initialisation (dropping
+ * the inactive), body warm start with gravity along `up` and the adaptive weight, `iterations` × (each dynamic body's
+ * Newton step, newest first, its forces in reverse order; every force's dual), BDF1 velocities. The colour-parallel
+ * variant takes the Newton steps colour by colour (greedy colouring of the bodies linked by forces, in index order),
+ * a colour's bodies in parallel.
+ */
+inline void Solver::packageStep() {
+  packageBroadphase();
+  std::vector<char> alive(forces.size());
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads) if (threads > 1)
+  for (size_t k = 0; k < forces.size(); k++) alive[k] = forces[k]->initialize() ? 1 : 0;
+  std::vector<Force *> kept;
+  for (size_t k = 0; k < forces.size(); k++) {
+    if (alive[k]) kept.push_back(forces[k]);
+    else forces[k]->unlinkFromBodies();
+  }
+  forces = kept;
+
+  const R g = gravity;
+  for (int bi = int(bodies.size()) - 1; bi >= 0; bi--) {
+    Rigid *body = bodies[size_t(bi)];
+    body->inertialLin = addScaled(body->positionLin, body->velocityLin, dt);
+    if (body->mass > 0) body->inertialLin = addScaled(body->inertialLin, up, g * dt * dt);
+    body->inertialAng = qadd(body->positionAng, scale(body->velocityAng, dt));
+    const V accel = divs(sub(body->velocityLin, body->prevVelocityLin), dt);
+    const R accelExt = dot(accel, up) * rsign(g);
+    R accelWeight = rclamp(accelExt / std::fabs(g), 0.0, 1.0);
+    if (!std::isfinite(accelWeight)) accelWeight = 0;
+    body->initialLin = body->positionLin;
+    body->initialAng = body->positionAng;
+    if (body->mass > 0) {
+      body->positionLin = addScaled(body->positionLin, body->velocityLin, dt);
+      body->positionLin = addScaled(body->positionLin, up, g * accelWeight * dt * dt);
+      body->positionAng = qadd(body->positionAng, scale(body->velocityAng, dt));
+    }
+  }
+
+  auto newton = [&](Rigid *body) {
+    BodySystem sys;
+    const R m = body->mass;
+    const V I = body->moment;
+    sys.lhsLin = diag(m / (dt * dt), m / (dt * dt), m / (dt * dt));
+    sys.lhsAng = diag(I.x / (dt * dt), I.y / (dt * dt), I.z / (dt * dt));
+    sys.rhsLin = mulv(sys.lhsLin, sub(body->positionLin, body->inertialLin));
+    sys.rhsAng = mulv(sys.lhsAng, qsub(body->positionAng, body->inertialAng));
+    for (int f = int(body->forces.size()) - 1; f >= 0; f--) body->forces[size_t(f)]->updatePrimal(body, alpha, sys);
+    V dxLin, dxAng;
+    solve6(sys.lhsLin, sys.lhsAng, sys.lhsCross, neg(sys.rhsLin), neg(sys.rhsAng), dxLin, dxAng);
+    body->positionLin = addScaled(body->positionLin, dxLin, 1.0);
+    body->positionAng = qadd(body->positionAng, dxAng);
+  };
+  std::vector<std::vector<Rigid *>> byColor;
+  if (threads > 1) {  // greedy colours in index order over the bodies the live forces link
+    std::unordered_map<const Rigid *, uint32_t> index;
+    for (uint32_t i = 0; i < bodies.size(); i++) index[bodies[i]] = i;
+    colors.assign(bodies.size(), ~0u);
+    for (uint32_t i = 0; i < bodies.size(); i++) {
+      Rigid *b = bodies[i];
+      if (b->mass <= 0) continue;
+      std::vector<char> used;
+      for (const Force *f : b->forces) {
+        const Rigid *o = f->bodyA == b ? f->bodyB : f->bodyA;
+        if (!o || o->mass <= 0) continue;
+        const uint32_t c = colors[index[o]];
+        if (c == ~0u) continue;
+        if (used.size() <= c) used.resize(c + 1, 0);
+        used[c] = 1;
+      }
+      uint32_t c = 0;
+      while (c < used.size() && used[c]) c++;
+      colors[i] = c;
+      if (byColor.size() <= c) byColor.resize(c + 1);
+      byColor[c].push_back(b);
+    }
+  }
+  for (int it = 0; it < iterations; it++) {
+    if (threads > 1) {
+      for (auto &group : byColor) {
+#pragma omp parallel for schedule(dynamic, 32) num_threads(threads)
+        for (size_t k = 0; k < group.size(); k++) newton(group[k]);
+      }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+      for (size_t k = 0; k < forces.size(); k++) forces[k]->updateDual(alpha);
+    } else {
+      for (int bi = int(bodies.size()) - 1; bi >= 0; bi--)
+        if (bodies[size_t(bi)]->mass > 0) newton(bodies[size_t(bi)]);
+      for (Force *f : forces) f->updateDual(alpha);
+    }
+  }
   for (Rigid *body : bodies) {
     body->prevVelocityLin = body->velocityLin;
     if (body->mass > 0) {

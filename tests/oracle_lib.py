@@ -57,6 +57,14 @@ def lib():
             "gs_set_pair_capacity": (None, [vp, ctypes.c_uint32]),
             "gs_overflow": (ctypes.c_uint32, [vp]),
             "gs_partial_step": (None, [vp, ctypes.c_int]),
+            "cpu_create": (vp, [ctypes.c_uint32, f32p, ctypes.c_uint32, f32p, i32p, f32p, ctypes.c_int, ctypes.c_int]),
+            "cpu_destroy": (None, [vp]),
+            "cpu_step": (None, [vp, ctypes.c_int]),
+            "cpu_set_threads": (None, [vp, ctypes.c_int]),
+            "cpu_read_bodies": (None, [vp, f64p]),
+            "cpu_counts": (ctypes.c_uint32, [vp, u32p]),
+            "cpu_read_pairs": (ctypes.c_uint32, [vp, u32p]),
+            "cpu_broadphase": (None, [vp]),
             "gs_capture_frames": (None, [vp, u32p, ctypes.c_uint32]),
             "gs_edit_joints": (None, [vp, u32p, ctypes.c_uint32]),
             "gs_set_state": (None, [vp, f32p, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p]),
@@ -422,3 +430,57 @@ class OracleWorld:
         self.filters[idx, 0] = g
         self.filters[idx, 1] = m
         lib().gs_set_filters(self.h, P(idx, ctypes.c_uint32), P(g, ctypes.c_uint32), P(m, ctypes.c_uint32), len(idx))
+
+
+class CpuPathWorld:
+    """The FP64 restatement of the reference's CPU path (CpuSolver3D.step over the package's builder records: grid
+    broadphase, face-biased box SAT, mass-penalty floor, sequential Gauss–Seidel newest first); threads > 1 runs its
+    colour-parallel variant. Boxes, joints and springs only."""
+
+    def __init__(self, scene, threads=1, bodies=None):
+        L = lib()
+        b, j, i, _, _ = scene.packed()
+        if bodies is not None:
+            b = np.ascontiguousarray(bodies, np.float32)
+        self.bodies0 = np.ascontiguousarray(b, np.float32)
+        self.joints0 = np.ascontiguousarray(j, np.float32).reshape(-1, 32)
+        self.info = np.ascontiguousarray(i, np.int32).reshape(-1, 4)
+        self.params = np.ascontiguousarray(scene.world_params(), np.float32)
+        self.n = len(self.bodies0)
+        self.h = L.cpu_create(self.n, P(self.bodies0, ctypes.c_float), len(self.joints0), P(self.joints0, ctypes.c_float),
+                              P(self.info, ctypes.c_int32), P(self.params, ctypes.c_float),
+                              int(scene.params["iterations"]), int(threads))
+        if not self.h:
+            raise ValueError("the CPU path restatement takes boxes, joints and springs only")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cpu_destroy(self.h)
+            self.h = None
+
+    def step(self, n=1):
+        lib().cpu_step(self.h, int(n))
+
+    def set_threads(self, n):
+        lib().cpu_set_threads(self.h, int(n))
+
+    def bodies(self):
+        out = np.zeros((self.n, 40))
+        lib().cpu_read_bodies(self.h, P(out, ctypes.c_double))
+        return out
+
+    def counts(self):
+        c = np.zeros(3, np.uint32)
+        points = lib().cpu_counts(self.h, P(c, ctypes.c_uint32))
+        return dict(manifolds=int(c[0]), joints=int(c[1]), springs=int(c[2]), contacts=int(points))
+
+    def broadphase(self):
+        """The broadphase alone: manifolds for the new pairs (not yet initialised)."""
+        lib().cpu_broadphase(self.h)
+
+    def pairs(self):
+        L = lib()
+        n = L.cpu_read_pairs(self.h, None)
+        out = np.zeros((n, 2), np.uint32)
+        L.cpu_read_pairs(self.h, P(out, ctypes.c_uint32))
+        return out
