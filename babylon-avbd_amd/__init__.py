@@ -14,7 +14,10 @@ package is the host-side mirror of src/native/{pipeline,scenes,solver3D}.js.
 from ._native import AvbdError, load as load_library, LIB_PATH  # noqa: F401
 from .scene import AvbdScene3D, box_mass, capsule_mass  # noqa: F401
 from .world import HipBody, HipWorld, SolverPipeline, createSolverPipeline  # noqa: F401
+from .hull import HullShape, MAX_HULL_VERTICES, convexHull, hullFromTriangles  # noqa: F401
+from .terrain import cookHeightfield, cookStaticMesh, createTerrainBodies  # noqa: F401
 from . import configs, serve, stats, tune  # noqa: F401
 
 __all__ = ["AvbdScene3D", "HipBody", "HipWorld", "SolverPipeline", "createSolverPipeline", "AvbdError", "configs",
-           "load_library"]
+           "load_library", "HullShape", "MAX_HULL_VERTICES", "convexHull", "hullFromTriangles", "cookHeightfield",
+           "cookStaticMesh", "createTerrainBodies"]

@@ -46,7 +46,28 @@ class SceneDesc(ctypes.Structure):
         ("beta_ang", ctypes.c_float),
         ("iterations", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("hull_words", ctypes.c_uint32),
+        ("hulls", u32p),
     ]
+
+
+class HullInfo(ctypes.Structure):
+    _fields_ = [
+        ("vertex_count", ctypes.c_uint32),
+        ("face_count", ctypes.c_uint32),
+        ("edge_count", ctypes.c_uint32),
+        ("words", ctypes.c_uint32),
+        ("volume", ctypes.c_double),
+        ("moments", ctypes.c_double * 3),
+        ("size", ctypes.c_double * 3),
+        ("radius", ctypes.c_double),
+        ("center", ctypes.c_double * 3),
+        ("rotation", ctypes.c_double * 4),
+    ]
+
+
+HULL_MAX_WORDS = 320
+SHAPE_HULL = 3.0
 
 
 class Options(ctypes.Structure):
@@ -122,6 +143,10 @@ SIGNATURES = {
     "avbd_version": (ctypes.c_char_p, []),
     "avbd_device_count": (ctypes.c_int, []),
     "avbd_world_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+    "avbd_hull_from_points": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                             ctypes.c_uint32, ctypes.POINTER(HullInfo)]),
+    "avbd_hull_from_triangles": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                                ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.POINTER(HullInfo)]),
     "avbd_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.c_uint32, ctypes.POINTER(Options),
                                          ctypes.POINTER(ctypes.c_void_p)]),
     "avbd_world_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libavbd_hip.so")
-SOURCES = ["collision.hip", "hploc.hip", "topology.hip", "solve.hip", "scan.hip", "edits.hip", "restitution.hip", "sleeping.hip", "queries.hip", "events.hip", "engine.hip", "capi.cpp", "server.cpp"]
+SOURCES = ["collision.hip", "hploc.hip", "topology.hip", "solve.hip", "scan.hip", "edits.hip", "restitution.hip", "sleeping.hip", "queries.hip", "events.hip", "engine.hip", "capi.cpp", "server.cpp", "hull_cook.cpp"]
 ARCH = os.environ.get("AVBD_OFFLOAD_ARCH", "gfx950")
 
 

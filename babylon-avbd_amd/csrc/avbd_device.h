@@ -34,6 +34,8 @@ constexpr uint32_t MAX_COLORS = 64, NO_COLOR = 255, PENDING = 256;
 // two angular constraints about a hinge axis (a motor driving the twist speed, a limit keeping the twist in a range)
 constexpr int32_t T_NONE = 0, T_JOINT = 1, T_SPRING = 2, T_MOTOR = 3, T_LIMIT = 4;
 constexpr float SHAPE_BOX = 0.f, SHAPE_SPHERE = 1.f, SHAPE_SAIL = 2.f, SHAPE_CAPSULE = -1.f;
+/** Hulls: shape code SHAPE_HULL + the word of the body's hull record header (layout.ts:68). */
+constexpr float SHAPE_HULL = 3.f;
 constexpr uint32_t FLAG_MATCH_NEAREST = 1, FLAG_FACE_BIAS = 2, FLAG_REUSE_CONTACTS = 4, FLAG_START_AT_REST = 8,
                    FLAG_MASS_PENALTY = 16,
                    FLAG_CAPSULE_STEP_START_ARMS = 32;  // this build's option: capsule lever arms at x⁻ like spheres
@@ -58,6 +60,7 @@ enum Counter : uint32_t {
   C_DF_ABORT = 10,  // a bounded wait gave up (dataflow solve, prefix scan; never expected: the step is then not valid)
   C_PENDING = 11,   // bodies the colouring's mark pass left pending (listed in colorPending)
   C_PAIR_DEMAND = 12,  // pairs the broadphase found, before clamping to the capacity (C_PAIRS is the clamped count)
+  C_HULL_PAIRS = 13,   // pair slots with a hull this step (listed in hullPairs for the hull narrowphase)
   COUNTER_WORDS = 16
 };
 

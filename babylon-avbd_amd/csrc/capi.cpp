@@ -9,6 +9,7 @@
 
 #include "../../include/avbd.h"
 #include "engine.h"
+#include "hull_cook.h"
 #include "server.h"
 
 using namespace avbd;
@@ -65,6 +66,9 @@ HostWorld toHost(const AvbdSceneDesc &s) {
     std::memcpy(w.info.data(), s.joint_info, size_t(s.joint_count) * sizeof(int4));
   }
   for (uint32_t k = 0; k < s.ignore_count; k++) w.ignorePairs.emplace_back(s.ignore_pairs[2 * k], s.ignore_pairs[2 * k + 1]);
+  if (s.hull_words && !s.hulls) throw std::invalid_argument("hulls is NULL");
+  w.hulls.resize(s.hull_words);
+  if (s.hull_words) std::memcpy(w.hulls.data(), s.hulls, size_t(s.hull_words) * sizeof(uint4));
   if (s.filters) {
     w.filters.resize(s.body_count);
     for (uint32_t i = 0; i < s.body_count; i++) w.filters[i] = make_uint2(s.filters[2 * i], s.filters[2 * i + 1]);
@@ -713,6 +717,62 @@ int avbd_world_read_contact_events(avbd_world *world, uint32_t *out, uint32_t ca
 
 #define CHECK_SERVER(s) \
   if (!(s) || !(s)->server) return fail(AVBD_ERR_DESTROYED, "the server has been destroyed")
+
+// --- convex hull cooking (host only) -----------------------------------------------------------------------------------
+
+static int cookedHull(bool ok, const std::string &err, const avbd::hullcook::Shape &shape, uint32_t at, uint32_t *record,
+                      uint32_t capacity, AvbdHullInfo *info) {
+  if (!ok) throw std::invalid_argument(err);
+  const size_t words = avbd::hullcook::recordWords(shape);
+  if (uint64_t(at) + words >= (1ull << 24)) throw std::invalid_argument("hull buffer beyond 2^24 words");
+  if (record) {
+    if (capacity < words) throw std::invalid_argument("record capacity too small for the hull");
+    avbd::hullcook::pack(shape, at, record);
+  }
+  if (info) {
+    *info = AvbdHullInfo{};
+    info->vertex_count = uint32_t(shape.vertices.size() / 3);
+    info->face_count = uint32_t(shape.faces.size());
+    info->edge_count = uint32_t(shape.edges.size());
+    info->words = uint32_t(words);
+    info->volume = shape.volume;
+    info->radius = shape.radius;
+    for (int k = 0; k < 3; k++) {
+      info->moments[k] = shape.moments[k];
+      info->size[k] = shape.size[k];
+      info->center[k] = shape.center[k];
+    }
+    for (int k = 0; k < 4; k++) info->rotation[k] = shape.rotation[k];
+  }
+  return AVBD_OK;
+}
+
+int avbd_hull_from_points(const double *xyz, uint32_t n_points, uint32_t at, uint32_t *record, uint32_t capacity_words,
+                          AvbdHullInfo *info) {
+  return guarded([&] {
+    if (n_points && !xyz) throw std::invalid_argument("xyz is NULL");
+    for (uint64_t k = 0; k < 3ull * n_points; k++)
+      if (!std::isfinite(xyz[k])) throw std::invalid_argument("Hull points must contain finite xyz triples");
+    avbd::hullcook::Shape shape;
+    std::string err;
+    const bool ok = avbd::hullcook::fromPoints(xyz, n_points, shape, err);
+    return cookedHull(ok, ok ? err : "A convex hull requires finite points spanning a volume", shape, at, record,
+                      capacity_words, info);
+  });
+}
+
+int avbd_hull_from_triangles(const double *xyz, uint32_t n_points, const uint32_t *indices, uint32_t n_indices,
+                             uint32_t at, uint32_t *record, uint32_t capacity_words, AvbdHullInfo *info) {
+  return guarded([&] {
+    if ((n_points && !xyz) || (n_indices && !indices)) throw std::invalid_argument("xyz or indices is NULL");
+    for (uint64_t k = 0; k < 3ull * n_points; k++)
+      if (!std::isfinite(xyz[k])) throw std::invalid_argument("positions must be finite xyz triples");
+    avbd::hullcook::Shape shape;
+    std::string err;
+    const bool ok = avbd::hullcook::fromTriangles(xyz, n_points, indices, n_indices, shape, err);
+    return cookedHull(ok, err, shape, at, record, capacity_words, info);
+  });
+}
 
 int avbd_server_create(const AvbdServerOptions *options, avbd_server **out) {
   return guarded([&] {

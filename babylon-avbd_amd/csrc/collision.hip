@@ -10,6 +10,7 @@
 
 #include "engine_types.h"
 #include "kernels.h"
+#include "hullnarrow.h"
 #include "narrowphase.h"
 #include "pairtest.h"
 
@@ -34,6 +35,7 @@ __global__ void k_begin_frame(DeviceWorld w) {
     c[C_PIVOT64] = 0;
     c[C_DF_ABORT] = 0;
     c[C_PENDING] = 0;
+    c[C_HULL_PAIRS] = 0;
   }
 }
 
@@ -278,10 +280,91 @@ AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t p
 }
 
 /**
- * Contacts of every pair slot (wgsl-collision.ts:617-808). kRounded = false compiles the box/box path only (a batch
- * without spheres or capsules), which keeps the kernel's code small enough to stay in the instruction cache.
+ * A pair slot's manifold and contact points from the narrowphase's points (wgsl-collision.ts:734-807): the manifold
+ * word (count | generation step), the normal (B towards A, -satN) and friction; each point's body-local anchors, warm
+ * start from last step's point with the same feature (or the nearest anchor with FLAG_MATCH_NEAREST; static friction
+ * keeps the old anchors except for rounded shapes), its C0 rows and the scaled multipliers and penalties.
  */
-template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narrowphase(DeviceWorld w) {
+AVBD_D void writePairContacts(const DeviceWorld &w, const WorldParams &p, uint32_t pslot, uint32_t a, uint32_t b,
+                              const Body &BA, const Body &BB, const Box &A, const Box &B, const Found &found, f3 satN,
+                              bool anySphere) {
+  uint32_t base = pslot * MAX_CONTACTS;
+  if (found.count == 0) {
+    w.mfSlots[pslot].ids = make_uint4(a, b, base, 0u);
+    return;
+  }
+  f3 n = neg(satN);
+  w.mfSlots[pslot] = Manifold{make_uint4(a, b, base, uint32_t(found.count) | (p.step << 4)),
+                              mk4(n, sqrtf(BA.pos.w * BB.pos.w))};
+  float4 qA = BA.rot, qB = BB.rot;
+  Basis basis = orthonormal(n);
+  uint32_t prevFirst = 0, prevCount = 0;
+  int pm = hashFind(w, a, b);
+  if (pm >= 0) {
+    prevFirst = w.prevManifolds[pm].ids.z;
+    prevCount = w.prevManifolds[pm].ids.w & 15u;
+  }
+  bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
+  float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
+  float pmin = contactPenaltyMin(w, p, a, b);
+  // Unrolled over the slots, so the found points are read at constant indices and stay in registers
+#pragma unroll
+  for (int i = 0; i < MAX_CONTACTS; i++) {
+    if (i >= found.count) break;
+    const f3 xA = found.xA[i], xB = found.xB[i];
+    const uint32_t feature = found.feature[i];
+    f3 rA = qrotate(qconj(qA), sub(xA, A.c));
+    f3 rB = qrotate(qconj(qB), sub(xB, B.c));
+    f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);
+    uint32_t stick = 0;
+    int j = -1;
+    for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
+      if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == feature) {
+        j = int(c);
+        break;
+      }
+    if (j < 0 && matchNearest) {
+      float bestDist = NEAREST_FRACTION * minSide;
+      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
+        float dist = length(sub(xyz(w.prevContacts[c].rA), rA));
+        if (dist <= bestDist) {
+          bestDist = dist;
+          j = int(c);
+        }
+      }
+    }
+    if (j >= 0) {
+      Contact pc = w.prevContacts[j];
+      pen = xyz(pc.pen);
+      lam = xyz(pc.lam);
+      stick = fbits(pc.rA.w) & STICK_BIT;
+      if (stick != 0 && !anySphere) {
+        rA = xyz(pc.rA);
+        rB = xyz(pc.rB);
+      }
+    }
+    f3 d = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
+    float c0x = dot(basis.n, d) + COLLISION_MARGIN;
+    float c0y = dot(basis.t1, d);
+    float c0z = dot(basis.t2, d);
+    lam = scale(scale(lam, p.alpha), p.gamma);
+    pen = clampPen(pen, p.gamma);
+    pen.x = rmax(pen.x, pmin);
+    Contact k;
+    k.rA = mk4(rA, bitsf(feature | stick));
+    k.rB = mk4(rB, c0x);
+    k.pen = mk4(pen, c0y);
+    k.lam = mk4(lam, c0z);
+    w.contacts[base + i] = k;
+  }
+}
+
+/**
+ * Contacts of every pair slot (wgsl-collision.ts:617-808). kRounded = false compiles the box/box path only (a batch
+ * without spheres or capsules), which keeps the kernel's code small enough to stay in the instruction cache. With
+ * kHulls, a pair with a hull on either side is listed for k_narrowphase_hull instead.
+ */
+template <bool kRounded, bool kHulls> __global__ void __launch_bounds__(NARROW_BLOCK) k_narrowphase(DeviceWorld w) {
   if (gatedOff(w)) return;
   uint32_t P = w.counters[C_PAIRS];
   for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
@@ -291,6 +374,10 @@ template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narro
     if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(w, p, pslot, a, b)) continue;
     const Body &BA = w.bodies[a];
     const Body &BB = w.bodies[b];
+    if (kHulls && (isHullShape(BA.angVel.w) || isHullShape(BB.angVel.w))) {
+      w.hullPairs[atomicAdd(&w.counters[C_HULL_PAIRS], 1u)] = pslot;  // at most one entry per pair slot
+      continue;
+    }
     Box A = makeBox(xyz(BA.pos), BA.rot, xyz(BA.size));
     Box B = makeBox(xyz(BB.pos), BB.rot, xyz(BB.size));
     Sat sat;
@@ -327,76 +414,54 @@ template <bool kRounded> __global__ void __launch_bounds__(NARROW_BLOCK) k_narro
     } else {
       found = collideBoxes(A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
     }
-    uint32_t base = pslot * MAX_CONTACTS;
-    if (found.count == 0) {
-      w.mfSlots[pslot].ids = make_uint4(a, b, base, 0u);
-      continue;
+    writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB);
+  }
+}
+
+/**
+ * Contacts of the pair slots with a hull (the hull branch of wgsl-collision.ts:697-711 and capsuleCollision.js:17-29),
+ * one wavefront per pair: capsule against hull (capsulePoly, the capsule as the first argument), sphere against hull
+ * (sphereHull), hull or box against hull (collidePoly); then the same manifold and warm start as every pair.
+ */
+__global__ void __launch_bounds__(HULL_BLOCK) k_narrowphase_hull(DeviceWorld w) {
+  if (gatedOff(w)) return;
+  __shared__ HullScratch sc;
+  const uint32_t count = w.counters[C_HULL_PAIRS];
+  for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+    const uint32_t pslot = w.hullPairs[k];
+    const uint2 pr = w.pairs[pslot];
+    const uint32_t a = pr.x, b = pr.y;
+    const WorldParams &p = w.worlds[w.bodyWorld[a]];
+    const Body &BA = w.bodies[a];
+    const Body &BB = w.bodies[b];
+    const Box A = makeBox(xyz(BA.pos), BA.rot, xyz(BA.size));
+    const Box B = makeBox(xyz(BB.pos), BB.rot, xyz(BB.size));
+    Sat sat;
+    sat.n = mk3(0.f, 0.f, 0.f);
+    Found found;
+    found.count = 0;
+    const bool sphereA = BA.angVel.w == SHAPE_SPHERE, sphereB = BB.angVel.w == SHAPE_SPHERE;
+    const bool capsuleA = BA.angVel.w == SHAPE_CAPSULE, capsuleB = BB.angVel.w == SHAPE_CAPSULE;
+    if (capsuleA) {
+      found = capsulePoly(w, capsuleOf(BA), polyOf(w, b), sat);
+    } else if (capsuleB) {
+      found = capsulePoly(w, capsuleOf(BB), polyOf(w, a), sat);
+      sat.n = neg(sat.n);
+      swapFoundSides(found);
+    } else if (sphereA) {
+      const SphereBoxHit h = sphereHull(w, A.c, A.h.x, polyOf(w, b));
+      sat.n = h.n;
+      if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    } else if (sphereB) {
+      const SphereBoxHit h = sphereHull(w, B.c, B.h.x, polyOf(w, a));
+      sat.n = neg(h.n);
+      if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+    } else {
+      found = collidePoly(w, polyOf(w, a), polyOf(w, b), (p.flags & FLAG_FACE_BIAS) != 0, sat, sc);
     }
-    f3 n = neg(sat.n);
-    w.mfSlots[pslot] = Manifold{make_uint4(a, b, base, uint32_t(found.count) | (p.step << 4)),
-                                mk4(n, sqrtf(BA.pos.w * BB.pos.w))};
-    float4 qA = BA.rot, qB = BB.rot;
-    Basis basis = orthonormal(n);
-    bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
-    uint32_t prevFirst = 0, prevCount = 0;
-    int pm = hashFind(w, a, b);
-    if (pm >= 0) {
-      prevFirst = w.prevManifolds[pm].ids.z;
-      prevCount = w.prevManifolds[pm].ids.w & 15u;
-    }
-    bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
-    float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
-    float pmin = contactPenaltyMin(w, p, a, b);
-    // Unrolled over the slots, so the found points are read at constant indices and stay in registers
-#pragma unroll
-    for (int i = 0; i < MAX_CONTACTS; i++) {
-      if (i >= found.count) break;
-      const f3 xA = found.xA[i], xB = found.xB[i];
-      const uint32_t feature = found.feature[i];
-      f3 rA = qrotate(qconj(qA), sub(xA, A.c));
-      f3 rB = qrotate(qconj(qB), sub(xB, B.c));
-      f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);
-      uint32_t stick = 0;
-      int j = -1;
-      for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
-        if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == feature) {
-          j = int(c);
-          break;
-        }
-      if (j < 0 && matchNearest) {
-        float bestDist = NEAREST_FRACTION * minSide;
-        for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
-          float dist = length(sub(xyz(w.prevContacts[c].rA), rA));
-          if (dist <= bestDist) {
-            bestDist = dist;
-            j = int(c);
-          }
-        }
-      }
-      if (j >= 0) {
-        Contact pc = w.prevContacts[j];
-        pen = xyz(pc.pen);
-        lam = xyz(pc.lam);
-        stick = fbits(pc.rA.w) & STICK_BIT;
-        if (stick != 0 && !anySphere) {
-          rA = xyz(pc.rA);
-          rB = xyz(pc.rB);
-        }
-      }
-      f3 d = sub(add(qrotate(qA, rA), A.c), add(qrotate(qB, rB), B.c));
-      float c0x = dot(basis.n, d) + COLLISION_MARGIN;
-      float c0y = dot(basis.t1, d);
-      float c0z = dot(basis.t2, d);
-      lam = scale(scale(lam, p.alpha), p.gamma);
-      pen = clampPen(pen, p.gamma);
-      pen.x = rmax(pen.x, pmin);
-      Contact k;
-      k.rA = mk4(rA, bitsf(feature | stick));
-      k.rB = mk4(rB, c0x);
-      k.pen = mk4(pen, c0y);
-      k.lam = mk4(lam, c0z);
-      w.contacts[base + i] = k;
-    }
+    if (laneId() == 0)
+      writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB);
+    __syncthreads();  // the scratch is free for the next pair
   }
 }
 
@@ -518,13 +583,18 @@ void launch_pairs(const DeviceWorld &w, hipStream_t s) {
 }
 void launch_pairs_total(const DeviceWorld &w, hipStream_t s) { k_pairs_total<<<1, 1, 0, s>>>(w); }
 void launch_narrowphase(const DeviceWorld &w, hipStream_t s) {
-  if (w.roundedShapes)
-    k_narrowphase<true><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
-  else
-    k_narrowphase<false><<<blocks(w.pairCapacity, NARROW_BLOCK, 8192), NARROW_BLOCK, 0, s>>>(w);
+  const uint32_t grid = blocks(w.pairCapacity, NARROW_BLOCK, 8192);
+  if (w.hullShapes) {
+    k_narrowphase<true, true><<<grid, NARROW_BLOCK, 0, s>>>(w);
+    k_narrowphase_hull<<<2048, HULL_BLOCK, 0, s>>>(w);
+  } else if (w.roundedShapes) {
+    k_narrowphase<true, false><<<grid, NARROW_BLOCK, 0, s>>>(w);
+  } else {
+    k_narrowphase<false, false><<<grid, NARROW_BLOCK, 0, s>>>(w);
+  }
   // Ranges of at most 4096 slots, so a range's manifold and point counts each fit 16 bits of the packed scan
-  const uint32_t grid = std::max<uint32_t>(COMPACT_GRID, (w.pairCapacity + 4095u) / 4096u);
-  k_manifold_compact<<<grid, COMPACT_BLOCK, 0, s>>>(w);
+  const uint32_t compactGrid = std::max<uint32_t>(COMPACT_GRID, (w.pairCapacity + 4095u) / 4096u);
+  k_manifold_compact<<<compactGrid, COMPACT_BLOCK, 0, s>>>(w);
 }
 
 }  // namespace avbd

@@ -19,6 +19,7 @@ struct HostWorld {
   std::vector<int4> info;                       // type, a, b (world-local; a = -1: world)
   std::vector<std::pair<uint32_t, uint32_t>> ignorePairs;  // world-local body pairs that never collide
   std::vector<uint2> filters;                   // empty: collide with everything
+  std::vector<uint4> hulls;                     // hull records (headers at the words body shape codes name)
   WorldParams params;
   uint32_t iterations = 10;
 };
@@ -241,6 +242,8 @@ class Engine {
   Contact *contactBuf_[2] = {nullptr, nullptr};
   hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};  // parity × (refit, rebuild)
   std::vector<int4> infoHost_;      // host copy of the constraint info words (type, a, b, 0)
+  std::vector<uint4> hullsHost_;    // every world's hull records, concatenated (headers rebased)
+  std::vector<char> hullHeader_;    // per word of hullsHost_: a checked record's header starts there
   std::vector<float> restitution_;  // host copy of the per-body coefficients
   uint32_t bouncing_ = 0;           // bodies with a coefficient above 0
   bool sleeping_ = false, sleepIslands_ = false;

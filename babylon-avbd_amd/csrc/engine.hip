@@ -21,6 +21,45 @@ namespace avbd {
       throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(err_) + " at " #x);             \
   } while (0)
 
+/** The header word a hull shape code names, checked to be a whole number inside a buffer of `words` words. */
+static uint32_t hullHeaderOf(float code, uint32_t words) {
+  const float h = code - SHAPE_HULL;
+  if (!(h >= 0.f) || h != std::floor(h) || !(h + 1.f < float(words)))
+    throw std::invalid_argument("a hull body's shape code names no hull record of its world");
+  return uint32_t(h);
+}
+
+/**
+ * A hull record's every offset and index inside its buffer, with the narrowphase's limits (shapes.ts:59-61): at most
+ * 32 vertices, 255 faces and 255 edges, faces of 3 to 32 vertices.
+ */
+static void checkHullRecord(const std::vector<uint4> &buf, uint32_t h) {
+  const uint64_t words = buf.size();
+  auto bad = [](const char *what) { throw std::invalid_argument(std::string("hull record: ") + what); };
+  if (uint64_t(h) + 2 > words) bad("header outside the buffer");
+  const uint4 h0 = buf[h], h1 = buf[h + 1];
+  const uint32_t vs = h0.x, V = h0.y, fs = h0.z, F = h0.w, es = h1.x, E = h1.y, is = h1.z;
+  if (V < 4 || V > 32 || F < 4 || F > 255 || E < 6 || E > 255) bad("vertex, face or edge count out of range");
+  if (uint64_t(vs) + V > words || uint64_t(fs) + 2ull * F > words || uint64_t(es) + E > words || is >= words)
+    bad("a section lies outside the buffer");
+  for (uint32_t f = 0; f < F; f++) {
+    const uint4 face = buf[fs + 2ull * f + 1];
+    if (face.y < 3 || face.y > 32) bad("a face has fewer than 3 or more than 32 vertices");
+    const uint64_t last = uint64_t(is) * 4 + face.x + face.y;
+    if (last > words * 4) bad("a face's vertex indices lie outside the buffer");
+    for (uint32_t j = 0; j < face.y; j++) {
+      const uint64_t k = uint64_t(face.x) + j;
+      const uint4 u = buf[is + k / 4];
+      const uint32_t v = k % 4 == 0 ? u.x : (k % 4 == 1 ? u.y : (k % 4 == 2 ? u.z : u.w));
+      if (v >= V) bad("a face names a vertex the hull does not have");
+    }
+  }
+  for (uint32_t e = 0; e < E; e++) {
+    const uint4 edge = buf[es + e];
+    if (edge.x >= V || edge.y >= V || edge.z >= F || edge.w >= F) bad("an edge names a vertex or face out of range");
+  }
+}
+
 static uint32_t pow2AtLeast(uint64_t n) {
   uint64_t p = 2;
   while (p < n) p <<= 1;
@@ -84,6 +123,9 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   double cellSize = 1e-3;
   for (uint32_t k = 0; k < nWorlds_; k++) {
     const HostWorld &hw = worlds[k];
+    const uint32_t hullBase = uint32_t(hullsHost_.size());
+    if (uint64_t(hullBase) + hw.hulls.size() >= (1ull << 24) - 4)
+      throw std::invalid_argument("hull buffers beyond 2^24 words (shape codes are exact f32 integers)");
     if (hw.iterations != iterations_) throw std::invalid_argument("every world of a batch takes the same iterations");
     const uint32_t base = uint32_t(bodies.size());
     const uint32_t jbase = uint32_t(joints.size());
@@ -92,6 +134,33 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
     firstJoint_.push_back(jbase);
     bodies.insert(bodies.end(), hw.bodies.begin(), hw.bodies.end());
     bodyWorld.insert(bodyWorld.end(), n, k);
+    // Hull records: each one a body names is checked, then the world's buffer is appended with its offsets rebased
+    std::vector<char> header(hw.hulls.size(), 0);
+    for (uint32_t i = 0; i < n; i++) {
+      const float code = hw.bodies[i].angVel.w;
+      if (!(code >= SHAPE_HULL)) continue;
+      const uint32_t h = hullHeaderOf(code, uint32_t(hw.hulls.size()));
+      if (!header[h]) checkHullRecord(hw.hulls, h);
+      header[h] = 1;
+      bodies[base + i].angVel.w = SHAPE_HULL + float(hullBase + h);
+    }
+    for (size_t q = 0; q + 1 < hw.hulls.size(); q++)
+      if (header[q] && header[q + 1]) throw std::invalid_argument("hull record: two headers overlap");
+    for (size_t q = 0; q < hw.hulls.size(); q++) {
+      uint4 u = hw.hulls[q];
+      if (header[q]) {
+        u.x += hullBase;
+        u.z += hullBase;
+      }
+      hullsHost_.push_back(u);
+      hullHeader_.push_back(header[q]);
+    }
+    for (size_t q = 0; q < hw.hulls.size(); q++)
+      if (header[q]) {
+        uint4 &h1 = hullsHost_[hullBase + q + 1];
+        h1.x += hullBase;
+        h1.z += hullBase;
+      }
     if (!hw.filters.empty() && hw.filters.size() != n) throw std::invalid_argument("filters: one pair per body");
     for (uint32_t i = 0; i < n; i++) filters.push_back(hw.filters.empty() ? make_uint2(~0u, ~0u) : hw.filters[i]);
     for (size_t j = 0; j < hw.joints.size(); j++) {
@@ -161,8 +230,13 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.nWorlds = nWorlds_;
   w.noCollideCount = noCollideCount_;
   w.roundedShapes = 0;
-  for (const Body &b : bodies)
+  w.hullShapes = 0;
+  for (const Body &b : bodies) {
     if (b.angVel.w == SHAPE_SPHERE || b.angVel.w == SHAPE_CAPSULE) w.roundedShapes = 1;
+    if (b.angVel.w >= SHAPE_HULL) w.hullShapes = 1;
+  }
+  w.hulls = dallocT<uint4>(std::max<size_t>(hullsHost_.size(), 1));
+  w.hullPairs = nullptr;  // with the pair storage
   w.bodies = dallocT<Body>(nBodies_);
   w.bodyWorld = dallocT<uint32_t>(nBodies_);
   w.worlds = dallocT<WorldParams>(nWorlds_);
@@ -214,6 +288,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
     if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
   };
   up(w.bodies, bodies.data(), bodies.size() * sizeof(Body));
+  up(const_cast<uint4 *>(w.hulls), hullsHost_.data(), hullsHost_.size() * sizeof(uint4));
   up(w.bodyWorld, bodyWorld.data(), bodyWorld.size() * 4);
   up(w.worlds, params.data(), params.size() * sizeof(WorldParams));
   up(w.filters, filters.data(), filters.size() * sizeof(uint2));
@@ -321,6 +396,7 @@ void Engine::allocateContacts(uint32_t pairCap) {
   w.pairCapacity = pairCap_;
   w.pairs = static_cast<uint2 *>(alloc(size_t(pairCap_) * sizeof(uint2)));
   w.mfSlots = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
+  w.hullPairs = static_cast<uint32_t *>(alloc(w.hullShapes ? size_t(pairCap_) * sizeof(uint32_t) : 16));
   w.pointMap = static_cast<uint32_t *>(alloc(size_t(pairCap_) * MAX_CONTACTS * sizeof(uint32_t)));
   for (int k = 0; k < 2; k++) {
     manifoldBuf_[k] = static_cast<Manifold *>(alloc(size_t(pairCap_) * sizeof(Manifold)));
@@ -631,6 +707,13 @@ void Engine::readBodies(uint32_t first, uint32_t count, Body *out) {
 }
 void Engine::writeBodies(uint32_t first, uint32_t count, const Body *in) {
   if (size_t(first) + count > nBodies_) throw std::out_of_range("body range");
+  // A hull body's shape code must name a record this engine holds (the narrowphase reads it unchecked)
+  for (uint32_t k = 0; k < count; k++) {
+    const float code = in[k].angVel.w;
+    if (!(code >= SHAPE_HULL)) continue;
+    const uint32_t h = hullHeaderOf(code, uint32_t(hullsHost_.size()));
+    if (!hullHeader_[h]) throw std::invalid_argument("a hull body's shape code names no hull record of this world");
+  }
   if (!base_.roundedShapes)
     for (uint32_t k = 0; k < count; k++)
       if (in[k].angVel.w == SHAPE_SPHERE || in[k].angVel.w == SHAPE_CAPSULE) {

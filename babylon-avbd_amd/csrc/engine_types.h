@@ -49,6 +49,9 @@ struct DeviceWorld {
   uint32_t *ncStart;     // nBodies + 1: body hi's triples are noCollide[ncStart[hi] .. ncStart[hi + 1])
   uint32_t nBodies, nJoints, nWorlds, noCollideCount;
   uint32_t roundedShapes;  // any sphere or capsule in the batch (else the narrowphase runs its box-only variant)
+  uint32_t hullShapes;     // any hull in the batch: pairs with a hull go to the hull narrowphase
+  const uint4 *hulls;      // hull records (4 u32 per word; a hull body's shape code is 3 + its header word)
+  uint32_t *hullPairs;     // pair slots with a hull, listed by the narrowphase (C_HULL_PAIRS of them)
   uint32_t *counters;    // COUNTER_WORDS
   uint32_t *scanScratch; // block sums of the multi-level prefix scan
 

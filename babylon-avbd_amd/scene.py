@@ -20,6 +20,7 @@ T_LIMIT = 4
 SHAPE_BOX = 0.0
 SHAPE_SPHERE = 1.0
 SHAPE_CAPSULE = -1.0
+SHAPE_HULL = 3.0  # + the word of the body's hull record in the scene's hull buffer
 
 
 def _finite(x):
@@ -94,6 +95,10 @@ def body_record(position, rotation, size, mass, moment, radius, friction, shape,
     return r
 
 
+_HEIGHTFIELD_KEYS = {"heights", "columns", "rows", "spacing", "origin", "upAxis", "baseHeight", "holes"}
+_MESH_KEYS = {"positions", "indices", "thickness"}
+
+
 def _unit(v):
     a = _vector(v, 3, "axis")
     n = math.sqrt(sum(x * x for x in a))
@@ -154,6 +159,8 @@ class AvbdScene3D:
         self._no_sleep = set()    # bodies built with allowSleep false
         self.sleepOptIn = False   # a body was built with allowSleep true: its world sleeps (scenes.js:233-240, 689)
         self._joints = []         # (record[32], info[4])
+        self._hulls = {}          # id(HullShape) -> (shape, header word): one record per shape, shared by its bodies
+        self._hull_words = 0
         self._ignore = []
         self.created = False
 
@@ -252,6 +259,69 @@ class AvbdScene3D:
         m, moment, bound = capsule_mass(r, h, density)
         return self._add(body_record(pos, (0, 0, 0, 1), (2 * r, h, 2 * r), m, moment, bound, friction, SHAPE_CAPSULE),
                          options)
+
+    def addHull(self, points, **options):
+        """A convex hull of xyz points, its principal frame at `position` (its centre of mass), as scenes.js:160-179:
+        the points' hull (at most 32 vertices) of the given density; `rotation` turns the principal frame."""
+        from .hull import convexHull
+        self.assertEditable()
+        flat = np.asarray(points, np.float64).reshape(-1)
+        if flat.size % 3 or not np.all(np.isfinite(flat)):
+            raise ValueError("Hull points must contain finite xyz triples")
+        shape = convexHull(flat)
+        if shape is None:
+            raise ValueError("A convex hull requires finite points spanning a volume")
+        return self.addHullShape(shape, **options)
+
+    def addHullShape(self, shape, **options):
+        """A body of a cooked HullShape (shapes.ts:50-70 `hull`): size, mass, moments and radius from the shape."""
+        self.assertEditable()
+        if shape.vertices.shape[0] > 32 or len(shape.faces) > 255 or len(shape.edges) > 255:
+            raise ValueError(f"hull: {shape.vertices.shape[0]} vertices, {len(shape.faces)} faces, "
+                             f"{len(shape.edges)} edges (at most 32 vertices, 255 faces and edges)")
+        density = _positive(options.get("density", 1), "density", True)
+        friction = _positive(options.get("friction", 0.6), "friction", True)
+        pos = _vector(options.get("position", (0, 0, 0)), 3, "position")
+        entry = self._hulls.get(id(shape))
+        if entry is None:
+            entry = (shape, self._hull_words)
+            self._hulls[id(shape)] = entry
+            self._hull_words += shape.words
+        at = entry[1]
+        m = shape.volume * density if density > 0 else 0.0
+        moment = [x * max(density, 0.0) for x in shape.moments]
+        rec = body_record(pos, (0, 0, 0, 1), shape.size, m, moment, shape.radius, friction, SHAPE_HULL + at)
+        if rec[39] != np.float32(SHAPE_HULL + at):
+            raise ValueError("hull buffer beyond 2^24 words")
+        return self._add(rec, options)
+
+    def hull_buffer(self):
+        """The scene's hull records, concatenated ((words, 4) uint32; empty without hulls)."""
+        if not self._hulls:
+            return np.zeros((0, 4), np.uint32)
+        out = np.zeros((self._hull_words, 4), np.uint32)
+        for shape, at in self._hulls.values():
+            out[at:at + shape.words] = shape.record(at)
+        return out
+
+    def addHeightfield(self, **options):
+        """Static solid ground from a height grid (scenes.js:196-198; terrain.js cookHeightfield)."""
+        from .terrain import cookHeightfield
+        return self.addTerrain(cookHeightfield(**{k: v for k, v in options.items() if k in _HEIGHTFIELD_KEYS}),
+                               **{k: v for k, v in options.items() if k not in _HEIGHTFIELD_KEYS})
+
+    def addStaticMesh(self, **options):
+        """Static triangle surfaces as thin solid prisms (scenes.js:199-201; terrain.js cookStaticMesh)."""
+        from .terrain import cookStaticMesh
+        return self.addTerrain(cookStaticMesh(**{k: v for k, v in options.items() if k in _MESH_KEYS}),
+                               **{k: v for k, v in options.items() if k not in _MESH_KEYS})
+
+    def addTerrain(self, cooked, **options):
+        """Register cooked terrain pieces as static hull bodies (scenes.js:203-231; terrain.js createTerrainBodies)."""
+        from .terrain import createTerrainBodies
+        self.assertEditable()
+        bodies = createTerrainBodies(self, cooked, options)
+        return dict(bodies=bodies, triangleCount=cooked["triangleCount"], kind=cooked["kind"])
 
     def addBoxes(self, sizes, positions, rotations=None, density=1.0, friction=0.6, velocities=None,
                  angularVelocities=None):

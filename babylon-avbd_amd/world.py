@@ -35,6 +35,11 @@ def _scene_desc(scene, keep):
     d.beta_ang = p["betaAng"]
     d.iterations = p["iterations"]
     d.flags = p["flags"]
+    hulls = scene.hull_buffer()
+    if len(hulls):
+        keep.append(hulls)
+        d.hull_words = len(hulls)
+        d.hulls = ptr(hulls, ctypes.c_uint32)
     return d
 
 

@@ -71,6 +71,10 @@ typedef struct AvbdSceneDesc {
   float alpha, gamma, beta_lin, beta_ang;
   uint32_t iterations;
   uint32_t flags;               /* AVBD_FLAG_* */
+  uint32_t hull_words;          /* size of the hull buffer in 16-byte words (0: no hulls) */
+  const uint32_t *hulls;        /* the hull buffer (4 u32 per word; layout of avbd_hull_from_points), or NULL; a hull
+                                   body's shape code (angVel.w) is AVBD_SHAPE_HULL + its record's header word
+                                   (solver.ts:170-197, 813-817; wgsl-hull.ts:13-19) */
 } AvbdSceneDesc;
 
 /** Solver options (GpuSolverOptions, solver.ts:127-151, plus this library's). Zero-initialised = defaults. */
@@ -364,6 +368,38 @@ int avbd_world_read_contact_events(avbd_world *world, uint32_t *out, uint32_t ca
  * requested), and runs admitted requests concurrently, one HIP stream per world, launching the next step of every
  * running request each turn (continuous batching at step granularity). Requests of one world run in submission order,
  * so a world's result does not depend on what else is served. Served worlds do not adapt their colour cap. */
+
+/*
+ * Convex hulls (reference/three-avbd/src/avbd3d/hull.ts: convexHull :60-66, hullFromTriangles :208-212; the body builder
+ * shapes.ts:50-70; the record solver.ts:170-197). Cooking runs on the host and touches no device. The record, in
+ * 16-byte words from the start of the buffer it is placed in (header at word `at`):
+ *   at:   vertex start, vertex count, face start, face count
+ *   at+1: edge start, edge count, index start, 0
+ *   vertices (x, y, z, 0; f32 bits), faces (normal.xyz, plane offset as f32 bits; first index, index count, 0, 0),
+ *   vertex indices (4 per word), edges (vertex a < vertex b, face with a→b, face with b→a),
+ * all in the hull's principal frame centred on its centre of mass. A body of this hull has shape code
+ * AVBD_SHAPE_HULL + at, size = `size`, mass = volume · density, moments = `moments` · density, radius = `radius`.
+ * Errors (AVBD_ERR_INVALID): the points span no volume; the triangles do not close up; capacity_words too small.
+ */
+#define AVBD_SHAPE_HULL 3.0f
+#define AVBD_HULL_MAX_VERTICES 32u
+#define AVBD_HULL_MAX_WORDS 320u
+typedef struct AvbdHullInfo {
+  uint32_t vertex_count, face_count, edge_count;
+  uint32_t words;      /* record size in 16-byte words */
+  double volume;       /* per unit density */
+  double moments[3];   /* principal moments per unit density */
+  double size[3];      /* full widths along the principal axes */
+  double radius;       /* farthest vertex from the centre of mass */
+  double center[3];    /* the principal frame in the input points' frame: centre of mass and rotation (x, y, z, w) */
+  double rotation[4];
+} AvbdHullInfo;
+/** The convex hull of n_points xyz points (at most 32 vertices kept); record may be NULL (info only). */
+int avbd_hull_from_points(const double *xyz, uint32_t n_points, uint32_t at, uint32_t *record, uint32_t capacity_words,
+                          AvbdHullInfo *info);
+/** A hull from a closed, outward-wound triangulation of a convex point set. */
+int avbd_hull_from_triangles(const double *xyz, uint32_t n_points, const uint32_t *indices, uint32_t n_indices,
+                             uint32_t at, uint32_t *record, uint32_t capacity_words, AvbdHullInfo *info);
 
 typedef struct avbd_server avbd_server;
 

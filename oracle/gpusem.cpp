@@ -653,6 +653,64 @@ int gs_probe_pair_contacts(int precision, const float *recA, const float *recB, 
   return precision == 64 ? run(0.0) : run(0.0f);
 }
 
+/** The hull buffer (4 u32 per word) every hull body's shape code indexes. */
+void gs_set_hulls(void *p, const uint32_t *words, uint32_t n) {
+  auto *h = static_cast<Handle *>(p);
+  h->f->hulls.w.assign(words, words + 4ull * n);
+  if (h->d) h->d->hulls.w.assign(words, words + 4ull * n);
+}
+
+/**
+ * Contacts of one pair of body records with a hull buffer (any shapes), as gs_probe_pair_contacts; for two boxes or
+ * hulls also the three separating-axis queries of collidePoly in `queries` (face of A: sep, index; face of B: sep,
+ * index; best edge pair: sep, valid, A edge, B edge), each -3.4e38 where the query did not run.
+ */
+int gs_probe_hull_pair(int precision, const float *recA, const float *recB, const uint32_t *words, uint32_t nWords,
+                       int faceBias, uint32_t *features, double *points, double *normal, double *queries) {
+  HullBuffer H;
+  H.w.assign(words, words + 4ull * nWords);
+  auto run = [&](auto zero) {
+    using R = decltype(zero);
+    Body<R> BA, BB;
+    unpackBody(recA, BA);
+    unpackBody(recB, BB);
+    Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
+    Sat<R> sat;
+    Found<R> f;
+    for (int k = 0; k < 8; k++) queries[k] = -3.4e38;
+    const bool polyPair = BA.shape != R(SHAPE_SPHERE) && BB.shape != R(SHAPE_SPHERE) && BA.shape != R(SHAPE_CAPSULE) &&
+                          BB.shape != R(SHAPE_CAPSULE) && (BA.shape >= R(SHAPE_HULL_F) || BB.shape >= R(SHAPE_HULL_F));
+    if (polyPair) {
+      PolyQueries<R> q;
+      q.fa.sep = q.fb.sep = R(-3.4e38f);
+      q.fa.face = q.fb.face = 0;
+      f = collidePoly(H, polyOf(H, BA.pos, BA.rot, BA.size, BA.shape), polyOf(H, BB.pos, BB.rot, BB.size, BB.shape),
+                      faceBias != 0, sat, &q);
+      queries[0] = double(q.fa.sep);
+      queries[1] = q.fa.face;
+      queries[2] = double(q.fb.sep);
+      queries[3] = q.fb.face;
+      queries[4] = double(q.eq.sep);
+      queries[5] = q.eq.valid ? 1 : 0;
+      queries[6] = q.eq.ea;
+      queries[7] = q.eq.eb;
+    } else {
+      f = shapeContacts(BA, BB, A, B, faceBias != 0, sat, &H);
+    }
+    for (int i = 0; i < f.count; i++) {
+      features[i] = f.feature[i];
+      double v[6] = {double(f.xA[i].x), double(f.xA[i].y), double(f.xA[i].z),
+                     double(f.xB[i].x), double(f.xB[i].y), double(f.xB[i].z)};
+      std::memcpy(points + 6 * i, v, sizeof v);
+    }
+    normal[0] = -double(sat.n.x);
+    normal[1] = -double(sat.n.y);
+    normal[2] = -double(sat.n.z);
+    return f.count;
+  };
+  return precision == 64 ? run(0.0) : run(0.0f);
+}
+
 /** LDLᵀ 6x6 solve (lower triangle of the symmetric 6x6 `a`, row-major 36) with an optional pivot floor. */
 void gs_probe_solve6(const double *a, const double *b, double pivotFloor, double *x) {
   M3<double> lin, ang, crs;

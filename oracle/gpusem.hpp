@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "collide.hpp"
+#include "hull.hpp"
 
 namespace oracle {
 namespace gpu {
@@ -108,30 +109,50 @@ inline bool stepStartArms(uint32_t key, uint32_t flags) {
 }
 
 /**
- * Contact points of one pair by shape (wgsl-collision.ts:684-726 with the capsule splice of capsuleCollision.js:17-29):
- * capsules first, then spheres, then box/box SAT. sat.n points from A towards B.
+ * Contact points of one pair by shape (wgsl-collision.ts:684-726 with the hull branch :697-711 and the capsule splice
+ * of capsuleCollision.js:17-29): capsules first (against a hull: capsulePoly), then two spheres, then pairs with a
+ * hull (sphereHull or collidePoly, when a hull buffer is given), then sphere/box, then box/box SAT. sat.n points from
+ * A towards B.
  */
 template <class R>
 inline Found<R> shapeContacts(const Body<R> &BA, const Body<R> &BB, const Box<R> &A, const Box<R> &B, bool faceBias,
-                              Sat<R> &sat) {
+                              Sat<R> &sat, const HullBuffer *H = nullptr) {
   Found<R> found;
   bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
   bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
+  const bool hullA = H && BA.shape >= R(SHAPE_HULL_F), hullB = H && BB.shape >= R(SHAPE_HULL_F);
+  auto poly = [&](const Body<R> &b) { return polyOf(*H, b.pos, b.rot, b.size, b.shape); };
   if (capsuleA || capsuleB) {
     // capsuleCollision.js:17-29: the capsule is the first argument; a capsule B flips the result back
     if (capsuleA && capsuleB) {
       found = capsuleCapsule(capsuleOf(BA.pos, BA.rot, BA.size), capsuleOf(BB.pos, BB.rot, BB.size), sat);
     } else if (capsuleA) {
       Capsule<R> C = capsuleOf(BA.pos, BA.rot, BA.size);
-      found = sphereB ? capsuleSphere(C, B.c, B.h.x, sat) : capsuleBox(C, B, BB.rot, sat);
+      if (sphereB) found = capsuleSphere(C, B.c, B.h.x, sat);
+      else if (hullB) found = capsulePoly(*H, C, poly(BB), sat);
+      else found = capsuleBox(C, B, BB.rot, sat);
     } else {
       Capsule<R> C = capsuleOf(BB.pos, BB.rot, BB.size);
-      found = sphereA ? capsuleSphere(C, A.c, A.h.x, sat) : capsuleBox(C, A, BA.rot, sat);
+      if (sphereA) found = capsuleSphere(C, A.c, A.h.x, sat);
+      else if (hullA) found = capsulePoly(*H, C, poly(BA), sat);
+      else found = capsuleBox(C, A, BA.rot, sat);
       sat.n = neg(sat.n);
       for (int c = 0; c < found.count; c++) std::swap(found.xA[c], found.xB[c]);
     }
   } else if (sphereA && sphereB) {
     found = sphereSphere(A, B, sat);
+  } else if (hullA || hullB) {
+    if (sphereA) {
+      SphereBoxHit<R> h = sphereHull(*H, A.c, A.h.x, poly(BB));
+      sat.n = h.n;
+      if (h.hit) addFound(found, h.onSphere, h.onBox, SPHERE_FEATURE);
+    } else if (sphereB) {
+      SphereBoxHit<R> h = sphereHull(*H, B.c, B.h.x, poly(BA));
+      sat.n = neg(h.n);
+      if (h.hit) addFound(found, h.onBox, h.onSphere, SPHERE_FEATURE);
+    } else {
+      found = collidePoly(*H, poly(BA), poly(BB), faceBias, sat);
+    }
   } else if (sphereA || sphereB) {
     if (sphereA) {
       SphereBoxHit<R> h = sphereBox(A, B, BB.rot);
@@ -157,6 +178,7 @@ template <class R> class World {
   std::vector<WorldParams> params;    // per world
   std::vector<Joint<R>> joints;
   std::vector<uint32_t> filters;      // 2 per body: groups, collides-with
+  HullBuffer hulls;                   // hull records (shape codes 3 + header word)
   std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> noCollide;  // (hi, lo, joint slot or ~0), sorted
   // Persistent state
   std::vector<RefPose> refs;
@@ -413,7 +435,7 @@ template <class R> class World {
     Box<R> A = makeBox(BA.pos, BA.rot, BA.size), B = makeBox(BB.pos, BB.rot, BB.size);
     Sat<R> sat;
     Found<R> found;
-    found = shapeContacts(BA, BB, A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
+    found = shapeContacts(BA, BB, A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat, &hulls);
     bool sphereA = BA.shape == R(SHAPE_SPHERE), sphereB = BB.shape == R(SHAPE_SPHERE);
     bool capsuleA = BA.shape == R(SHAPE_CAPSULE), capsuleB = BB.shape == R(SHAPE_CAPSULE);
     if (found.count == 0) return;

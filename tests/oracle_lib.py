@@ -55,6 +55,10 @@ def lib():
             "gs_set_brute_force": (None, [vp, ctypes.c_int]),
             "gs_set_coloring": (None, [vp, ctypes.c_uint32, ctypes.c_uint32]),
             "gs_set_pair_capacity": (None, [vp, ctypes.c_uint32]),
+            "gs_set_hulls": (None, [vp, u32p, ctypes.c_uint32]),
+            "gs_probe_hull_pair": (ctypes.c_int, [ctypes.c_int, f32p, f32p, u32p, ctypes.c_uint32, ctypes.c_int, u32p,
+                                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                  ctypes.POINTER(ctypes.c_double)]),
             "gs_overflow": (ctypes.c_uint32, [vp]),
             "gs_partial_step": (None, [vp, ctypes.c_int]),
             "cpu_create": (vp, [ctypes.c_uint32, f32p, ctypes.c_uint32, f32p, i32p, f32p, ctypes.c_int, ctypes.c_int]),
@@ -149,6 +153,39 @@ def no_collide(info, ignore):
     return np.array(out, np.uint32).reshape(-1, 3)
 
 
+def rebase_hulls(bodies, hulls, base):
+    """A scene's body records and hull buffer moved to word `base` of a batch's buffer, as the engine concatenates
+    them: hull shape codes and the four offsets of each named record's header grow by `base`."""
+    b = np.array(bodies, np.float32, copy=True)
+    hb = np.array(hulls, np.uint32, copy=True).reshape(-1, 4)
+    codes = b[:, 39].astype(np.float64)
+    named = sorted({int(c - 3) for c in codes if c >= 3})
+    if base:
+        b[codes >= 3, 39] = (codes[codes >= 3] + base).astype(np.float32)
+        for h in named:
+            hb[h, 0] += base
+            hb[h, 2] += base
+            hb[h + 1, 0] += base
+            hb[h + 1, 2] += base
+    return b, hb
+
+
+def probe_hull_pair(recA, recB, hulls, face_bias=True, precision=32):
+    """Contacts of one pair (any shapes, hulls from `hulls`): (features, points (k, 6), normal B->A, queries (8,):
+    A-face sep, face, B-face sep, face, edge sep, valid, A edge, B edge; -3.4e38 where not run)."""
+    a = np.ascontiguousarray(recA, np.float32)
+    b = np.ascontiguousarray(recB, np.float32)
+    h = np.ascontiguousarray(hulls, np.uint32).reshape(-1, 4)
+    feats = np.zeros(8, np.uint32)
+    pts = np.zeros((8, 6))
+    normal = np.zeros(3)
+    q = np.zeros(8)
+    n = lib().gs_probe_hull_pair(precision, P(a, ctypes.c_float), P(b, ctypes.c_float), P(h, ctypes.c_uint32), len(h),
+                                 int(face_bias), P(feats, ctypes.c_uint32), P(pts, ctypes.c_double),
+                                 P(normal, ctypes.c_double), P(q, ctypes.c_double))
+    return feats[:n], pts[:n], normal, q
+
+
 class OracleWorld:
     """The GPU-semantics oracle over packed records (several scenes = a batch)."""
 
@@ -156,8 +193,12 @@ class OracleWorld:
         L = lib()
         bodies, joints, info, ignore, filters, params, world = [], [], [], [], [], [], []
         base = 0
+        hulls, hull_base = [], 0
         for k, s in enumerate(scenes):
             b, j, i, g, f = s.packed()
+            b, hb = rebase_hulls(b, s.hull_buffer(), hull_base)
+            hulls.append(hb)
+            hull_base += len(hb)
             bodies.append(b)
             joints.append(j)
             i = i.copy()
@@ -186,6 +227,8 @@ class OracleWorld:
                              P(self.info, ctypes.c_int32), len(self.nc), P(self.nc, ctypes.c_uint32),
                              P(self.filters, ctypes.c_uint32), color_cap, color_rounds,
                              int(scenes[0].params["iterations"]))
+        self.hulls = np.ascontiguousarray(np.concatenate(hulls) if hull_base else np.zeros((0, 4), np.uint32))
+        self._set_hulls()
         self._cap, self._rounds, self._iters = color_cap, color_rounds, int(scenes[0].params["iterations"])
         self._brute = brute_force
         if brute_force:
@@ -206,6 +249,10 @@ class OracleWorld:
             lib().gs_destroy(self.h)
             self.h = None
 
+    def _set_hulls(self):
+        if len(self.hulls):
+            lib().gs_set_hulls(self.h, P(self.hulls, ctypes.c_uint32), len(self.hulls))
+
     def seed(self, state):
         """Start from a state read back from the HIP world (bodies, joints, refs, colours, contacts, steps)."""
         L = lib()
@@ -221,6 +268,7 @@ class OracleWorld:
                              P(self.info, ctypes.c_int32), len(self.nc), P(self.nc, cc), P(self.filters, cc),
                              self._cap, self._rounds, self._iters)
         L.gs_destroy(old)
+        self._set_hulls()
         if self._brute:
             L.gs_set_brute_force(self.h, 1)
         if self._pair_capacity:
