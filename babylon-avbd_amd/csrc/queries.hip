@@ -89,27 +89,67 @@ AVBD_D float4 facePlane(uint32_t f, f3 h) {
 __constant__ uint8_t FACE_CORNERS[24] = {1, 3, 7, 5, 0, 2, 6, 4, 2, 3, 7, 6, 0, 1, 5, 4, 4, 5, 7, 6, 0, 1, 3, 2};
 __constant__ uint8_t EDGE_CORNERS[24] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 2, 1, 3, 4, 6, 5, 7, 0, 4, 1, 5, 2, 6, 3, 7};
 
+/**
+ * The convex shape a query traces in the body's frame: a box of half extents h, or a hull record (its header word;
+ * shapeQueries.wgsl:34-46 vertex, plane, faceCount, faceVertex and the edge list).
+ */
+struct QShape {
+  f3 h;
+  const uint4 *H;
+  uint32_t header;
+  bool hull;
+};
+AVBD_D f3 qVertex(const QShape &S, uint32_t v) {
+  if (!S.hull) return corner(v, S.h);
+  const uint4 u = S.H[S.H[S.header].x + v];
+  return mk3(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z));
+}
+AVBD_D float4 qPlane(const QShape &S, uint32_t f) {
+  if (!S.hull) return facePlane(f, S.h);
+  const uint4 u = S.H[S.H[S.header].z + 2u * f];
+  return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+}
+AVBD_D uint32_t qFaceCount(const QShape &S, uint32_t f) { return S.hull ? S.H[S.H[S.header].z + 2u * f + 1u].y : 4u; }
+AVBD_D uint32_t qFaceVertex(const QShape &S, uint32_t f, uint32_t j) {
+  if (!S.hull) return FACE_CORNERS[f * 4 + j];
+  const uint32_t index = S.H[S.H[S.header].z + 2u * f + 1u].x + j;
+  const uint4 u = S.H[S.H[S.header + 1u].z + index / 4u];
+  const uint32_t k = index % 4u;
+  return k == 0 ? u.x : (k == 1 ? u.y : (k == 2 ? u.z : u.w));
+}
+AVBD_D uint2 qEdge(const QShape &S, uint32_t e) {
+  if (!S.hull) return make_uint2(EDGE_CORNERS[2 * e], EDGE_CORNERS[2 * e + 1]);
+  const uint4 u = S.H[S.H[S.header + 1u].x + e];
+  return make_uint2(u.x, u.y);
+}
+
 /** p (on face f's plane) lies inside the face, with a 1e-6 tolerance. */
-AVBD_D bool onFace(f3 p, uint32_t f, f3 h) {
-  const f3 normal = xyz(facePlane(f, h));
+AVBD_D bool onFace(f3 p, uint32_t f, const QShape &S) {
+  const f3 normal = xyz(qPlane(S, f));
+  const uint32_t count = qFaceCount(S, f);
   f3 center = mk3(0.f, 0.f, 0.f);
-  for (uint32_t j = 0; j < 4; j++) center = add(center, corner(FACE_CORNERS[f * 4 + j], h));
-  center = divs(center, 4.f);
-  for (uint32_t j = 0; j < 4; j++) {
-    const f3 a = corner(FACE_CORNERS[f * 4 + j], h), b = corner(FACE_CORNERS[f * 4 + (j + 1) % 4], h);
+  for (uint32_t j = 0; j < count; j++) center = add(center, qVertex(S, qFaceVertex(S, f, j)));
+  center = divs(center, float(count));
+  for (uint32_t j = 0; j < count; j++) {
+    const f3 a = qVertex(S, qFaceVertex(S, f, j)), b = qVertex(S, qFaceVertex(S, f, (j + 1) % count));
     const f3 side = cross(sub(b, a), normal);
     if (dot(side, sub(p, a)) * (dot(side, sub(center, a)) >= 0.f ? 1.f : -1.f) < -1e-6f) return false;
   }
   return true;
 }
 
-/** A ray (r = 0: slab test) or a sphere of radius r swept along d against the box of half extents h. */
-AVBD_D Hit roundedBox(f3 o, f3 d, f3 h, float r) {
+/**
+ * A ray (r = 0: the faces' slab test) or a sphere of radius r swept along d against a box or hull (roundedPoly,
+ * shapeQueries.wgsl:54-76): the offset faces, the edges' capsules (whose end spheres cover the corners), and an
+ * overlap at the start (centre inside or within r of the surface) as a hit at 0.
+ */
+AVBD_D Hit roundedPoly(f3 o, f3 d, const QShape &S, float r) {
+  const uint32_t nf = S.hull ? S.H[S.header].w : 6u, ne = S.hull ? S.H[S.header + 1u].y : 12u;
   if (r == 0.f) {
     float enter = -3.4e38f, exit = 3.4e38f;
     f3 normal = neg(d);
-    for (uint32_t f = 0; f < 6; f++) {
-      const float4 pl = facePlane(f, h);
+    for (uint32_t f = 0; f < nf; f++) {
+      const float4 pl = qPlane(S, f);
       const float denom = dot(xyz(pl), d), gap = pl.w - dot(xyz(pl), o);
       if (fabsf(denom) < 1e-12f) {
         if (gap < 0.f) return miss();
@@ -133,21 +173,22 @@ AVBD_D Hit roundedBox(f3 o, f3 d, f3 h, float r) {
   bool inside = true;
   float distance = 3.4e38f;
   Hit hit = miss();
-  for (uint32_t f = 0; f < 6; f++) {
-    const float4 pl = facePlane(f, h);
+  for (uint32_t f = 0; f < nf; f++) {
+    const float4 pl = qPlane(S, f);
     const f3 n = xyz(pl);
     const float signed_ = dot(n, o) - pl.w;
     if (signed_ > 0.f) inside = false;
-    if (onFace(sub(o, scale(n, signed_)), f, h)) distance = rmin(distance, signed_ * signed_);
+    if (onFace(sub(o, scale(n, signed_)), f, S)) distance = rmin(distance, signed_ * signed_);
     const float denom = dot(n, d);
     if (denom < -1e-12f) {
       const float t = ((pl.w + r) - dot(n, o)) / denom;
-      if (t >= 0.f && (hit.t < 0.f || t < hit.t) && onFace(sub(add(o, scale(d, t)), scale(n, r)), f, h))
+      if (t >= 0.f && (hit.t < 0.f || t < hit.t) && onFace(sub(add(o, scale(d, t)), scale(n, r)), f, S))
         hit = Hit{t, n};
     }
   }
-  for (uint32_t e = 0; e < 12; e++) {
-    const f3 a = corner(EDGE_CORNERS[2 * e], h), b = corner(EDGE_CORNERS[2 * e + 1], h);
+  for (uint32_t e = 0; e < ne; e++) {
+    const uint2 edge = qEdge(S, e);
+    const f3 a = qVertex(S, edge.x), b = qVertex(S, edge.y);
     const f3 q = sub(o, closest(a, b, o));
     distance = rmin(distance, dot(q, q));
     const Hit c = rayCapsule(o, d, a, b, r);
@@ -157,8 +198,8 @@ AVBD_D Hit roundedBox(f3 o, f3 d, f3 h, float r) {
   return hit;
 }
 
-/** Query q against body i in the body's frame; the normal comes back in world space. */
-AVBD_D Hit trace(const Body &B, const QueryRec &q) {
+/** Query q against body B in the body's frame; the normal comes back in world space. */
+AVBD_D Hit trace(const Body &B, const QueryRec &q, const uint4 *hulls) {
   const float4 rot = B.rot;
   const float4 inv = make_float4(-rot.x, -rot.y, -rot.z, rot.w);
   const f3 o = turn(inv, sub(xyz(q.origin), xyz(B.pos)));
@@ -177,7 +218,9 @@ AVBD_D Hit trace(const Body &B, const QueryRec &q) {
     const f3 delta = sub(o, closest(a, b, o));
     hit = dot(delta, delta) <= radius * radius ? Hit{0.f, neg(d)} : rayCapsule(o, d, a, b, radius);
   } else {
-    hit = roundedBox(o, d, scale(size, 0.5f), r);
+    const bool isHull = shape >= SHAPE_HULL;
+    const QShape S{scale(size, 0.5f), hulls, isHull ? uint32_t(rmax(0.f, shape - SHAPE_HULL) + 0.5f) : 0u, isHull};
+    hit = roundedPoly(o, d, S, r);
   }
   if (hit.t > q.origin.w) return miss();
   hit.n = turn(rot, hit.n);
@@ -201,7 +244,7 @@ AVBD_D bool ignored(const DeviceWorld &w, uint32_t i, const QueryFilter &f) {
 __global__ void k_query_nearest(DeviceWorld w, const QueryRec *queries, QueryFilter f, uint32_t *best) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
   if (i >= w.nBodies || ignored(w, i, f)) return;
-  const Hit h = trace(w.bodies[i], queries[q]);
+  const Hit h = trace(w.bodies[i], queries[q], w.hulls);
   if (h.t >= 0.f) atomicMin(&best[2 * q], __float_as_uint(nonneg(h.t)));
 }
 
@@ -209,7 +252,7 @@ __global__ void k_query_nearest(DeviceWorld w, const QueryRec *queries, QueryFil
 __global__ void k_query_owner(DeviceWorld w, const QueryRec *queries, QueryFilter f, uint32_t *best) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
   if (i >= w.nBodies || ignored(w, i, f)) return;
-  const Hit h = trace(w.bodies[i], queries[q]);
+  const Hit h = trace(w.bodies[i], queries[q], w.hulls);
   if (h.t >= 0.f && __float_as_uint(nonneg(h.t)) == __hip_atomic_load(&best[2 * q], __ATOMIC_RELAXED,
                                                                           __HIP_MEMORY_SCOPE_AGENT))
     atomicMin(&best[2 * q + 1], i);
@@ -228,7 +271,7 @@ __global__ void k_query_finish(DeviceWorld w, const QueryRec *queries, uint32_t 
     hits[2 * q + 1] = make_float4(__uint_as_float(0xffffffffu), 0.f, 0.f, 0.f);
     return;
   }
-  const Hit h = trace(w.bodies[index], queries[q]);
+  const Hit h = trace(w.bodies[index], queries[q], w.hulls);
   hits[2 * q] = mk4(h.n, nonneg(h.t));
   hits[2 * q + 1] = make_float4(__uint_as_float(index), 0.f, 0.f, 0.f);
 }

@@ -556,7 +556,7 @@ void gs_set_sleep(void *p, const uint32_t *states, const uint32_t *globals) {
  */
 void gs_cast(uint32_t nBodies, const float *bodyRecs, const uint32_t *filters, const float *queries, uint32_t n,
              const uint32_t *ignore, uint32_t nIgnore, uint32_t mask, uint32_t membership, int useMembership,
-             float *hits) {
+             float *hits, const uint32_t *hullWords) {
   for (uint32_t q = 0; q < n; q++) {
     const float *query = queries + 8ull * q;
     float bestT = INFINITY;
@@ -568,7 +568,7 @@ void gs_cast(uint32_t nBodies, const float *bodyRecs, const uint32_t *filters, c
       bool skip = false;
       for (uint32_t k = 0; k < nIgnore; k++) skip = skip || ignore[k] == i;
       if (skip) continue;
-      const oracle_queries::Hit h = oracle_queries::trace(bodyRecs + 40ull * i, query);
+      const oracle_queries::Hit h = oracle_queries::trace(bodyRecs + 40ull * i, query, hullWords);
       if (h.t < 0.f) continue;
       const float t = h.t > 0.f ? h.t : 0.f;
       if (t < bestT) {  // bodies in index order: a tie keeps the smaller index

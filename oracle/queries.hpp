@@ -6,6 +6,7 @@
 // Pinned in tests by an independent signed-distance ray march (sphere, capsule and box distance functions) and by
 // closed forms (a ray at a sphere's centre, at an axis-aligned box face, a sphere cast at a face).
 #pragma once
+#include <cstring>
 #include <cmath>
 #include <cstdint>
 
@@ -87,25 +88,70 @@ inline V faceNormal(unsigned f) {
   const unsigned k = f / 2;
   return mk(k == 0 ? s : 0.f, k == 1 ? s : 0.f, k == 2 ? s : 0.f);
 }
-inline bool insideFace(V p, unsigned f, V h) {
-  const V n = faceNormal(f);
+/** A box of half extents h, or a hull record at word `header` of the hull buffer (4 u32 per word). */
+struct Shape {
+  V h;
+  const uint32_t *H = nullptr;
+  unsigned header = 0;
+  bool hull = false;
+  unsigned u(unsigned word, int c) const { return H[4ull * word + c]; }
+  float f(unsigned word, int c) const {
+    float x;
+    std::memcpy(&x, &H[4ull * word + c], 4);
+    return x;
+  }
+};
+inline V vertexOf(const Shape &S, unsigned v) {
+  if (!S.hull) return boxCorner(v, S.h);
+  const unsigned w = S.u(S.header, 0) + v;
+  return mk(S.f(w, 0), S.f(w, 1), S.f(w, 2));
+}
+inline V planeNormal(const Shape &S, unsigned f) {
+  if (!S.hull) return faceNormal(f);
+  const unsigned w = S.u(S.header, 2) + 2 * f;
+  return mk(S.f(w, 0), S.f(w, 1), S.f(w, 2));
+}
+inline float planeOffset(const Shape &S, unsigned f) {
+  return S.hull ? S.f(S.u(S.header, 2) + 2 * f, 3) : axis(S.h, int(f / 2));
+}
+inline unsigned faceCount(const Shape &S, unsigned f) { return S.hull ? S.u(S.u(S.header, 2) + 2 * f + 1, 1) : 4u; }
+inline unsigned faceVertex(const Shape &S, unsigned f, unsigned j) {
+  if (!S.hull) return kFace[4 * f + j];
+  const unsigned index = S.u(S.u(S.header, 2) + 2 * f + 1, 0) + j;
+  return S.u(S.u(S.header + 1, 2) + index / 4, int(index % 4));
+}
+inline void edgeOf(const Shape &S, unsigned e, unsigned &a, unsigned &b) {
+  if (!S.hull) {
+    a = kEdge[2 * e];
+    b = kEdge[2 * e + 1];
+    return;
+  }
+  const unsigned w = S.u(S.header + 1, 0) + e;
+  a = S.u(w, 0);
+  b = S.u(w, 1);
+}
+inline bool insideFace(V p, unsigned f, const Shape &S) {
+  const V n = planeNormal(S, f);
+  const unsigned count = faceCount(S, f);
   V c = mk(0, 0, 0);
-  for (unsigned j = 0; j < 4; j++) c = add(c, boxCorner(kFace[4 * f + j], h));
-  c = quo(c, 4.f);
-  for (unsigned j = 0; j < 4; j++) {
-    const V a = boxCorner(kFace[4 * f + j], h), b = boxCorner(kFace[4 * f + (j + 1) % 4], h);
+  for (unsigned j = 0; j < count; j++) c = add(c, vertexOf(S, faceVertex(S, f, j)));
+  c = quo(c, float(count));
+  for (unsigned j = 0; j < count; j++) {
+    const V a = vertexOf(S, faceVertex(S, f, j)), b = vertexOf(S, faceVertex(S, f, (j + 1) % count));
     const V side = cross(sub(b, a), n);
     if (dot(side, sub(p, a)) * (dot(side, sub(c, a)) >= 0.f ? 1.f : -1.f) < -1e-6f) return false;
   }
   return true;
 }
-inline Hit box(V o, V d, V h, float r) {
+/** roundedPoly (shapeQueries.wgsl:54-76) for a box or a hull. */
+inline Hit poly(V o, V d, const Shape &S, float r) {
+  const unsigned nf = S.hull ? S.u(S.header, 3) : 6u, ne = S.hull ? S.u(S.header + 1, 1) : 12u;
   if (r == 0.f) {
     float enter = -3.4e38f, exit = 3.4e38f;
     V normal = neg(d);
-    for (unsigned f = 0; f < 6; f++) {
-      const V n = faceNormal(f);
-      const float w = axis(h, int(f / 2));
+    for (unsigned f = 0; f < nf; f++) {
+      const V n = planeNormal(S, f);
+      const float w = planeOffset(S, f);
       const float denom = dot(n, d), gap = w - dot(n, o);
       if (std::fabs(denom) < 1e-12f) {
         if (gap < 0.f) return none();
@@ -129,21 +175,23 @@ inline Hit box(V o, V d, V h, float r) {
   bool inside = true;
   float distance = 3.4e38f;
   Hit hit = none();
-  for (unsigned f = 0; f < 6; f++) {
-    const V n = faceNormal(f);
-    const float w = axis(h, int(f / 2));
+  for (unsigned f = 0; f < nf; f++) {
+    const V n = planeNormal(S, f);
+    const float w = planeOffset(S, f);
     const float s = dot(n, o) - w;
     if (s > 0.f) inside = false;
-    if (insideFace(sub(o, mul(n, s)), f, h)) distance = fminr(distance, s * s);
+    if (insideFace(sub(o, mul(n, s)), f, S)) distance = fminr(distance, s * s);
     const float denom = dot(n, d);
     if (denom < -1e-12f) {
       const float t = ((w + r) - dot(n, o)) / denom;
-      if (t >= 0.f && (hit.t < 0.f || t < hit.t) && insideFace(sub(add(o, mul(d, t)), mul(n, r)), f, h))
+      if (t >= 0.f && (hit.t < 0.f || t < hit.t) && insideFace(sub(add(o, mul(d, t)), mul(n, r)), f, S))
         hit = Hit{t, n};
     }
   }
-  for (unsigned e = 0; e < 12; e++) {
-    const V a = boxCorner(kEdge[2 * e], h), b = boxCorner(kEdge[2 * e + 1], h);
+  for (unsigned e = 0; e < ne; e++) {
+    unsigned ia, ib;
+    edgeOf(S, e, ia, ib);
+    const V a = vertexOf(S, ia), b = vertexOf(S, ib);
     const V q = sub(o, closest(a, b, o));
     distance = fminr(distance, dot(q, q));
     const Hit c = capsule(o, d, a, b, r);
@@ -153,8 +201,9 @@ inline Hit box(V o, V d, V h, float r) {
   return hit;
 }
 
-/** One query (origin, max distance, unit direction, radius) against one body record (40 floats). */
-inline Hit trace(const float *rec, const float *q) {
+/** One query (origin, max distance, unit direction, radius) against one body record (40 floats); hull bodies read
+ * their record from the hull buffer H. */
+inline Hit trace(const float *rec, const float *q, const uint32_t *H = nullptr) {
   const float rot[4] = {rec[4], rec[5], rec[6], rec[7]};
   const float inv[4] = {-rec[4], -rec[5], -rec[6], rec[7]};
   const V o = turn(inv, sub(mk(q[0], q[1], q[2]), mk(rec[0], rec[1], rec[2])));
@@ -173,7 +222,14 @@ inline Hit trace(const float *rec, const float *q) {
     const V delta = sub(o, closest(a, b, o));
     hit = dot(delta, delta) <= radius * radius ? Hit{0.f, neg(d)} : capsule(o, d, a, b, radius);
   } else {
-    hit = box(o, d, mul(size, 0.5f), r);
+    Shape S;
+    S.h = mul(size, 0.5f);
+    if (shape >= 3.f && H) {
+      S.hull = true;
+      S.H = H;
+      S.header = unsigned(fmaxr(0.f, shape - 3.f) + 0.5f);
+    }
+    hit = poly(o, d, S, r);
   }
   if (hit.t > q[3]) return none();
   hit.n = turn(rot, hit.n);

@@ -105,7 +105,7 @@ def lib():
             "gs_read_sleep": (None, [vp, u32p]),
             "gs_set_sleep": (None, [vp, u32p, u32p]),
             "gs_cast": (None, [ctypes.c_uint32, f32p, u32p, f32p, ctypes.c_uint32, u32p, ctypes.c_uint32,
-                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, f32p]),
+                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, f32p, u32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -115,17 +115,19 @@ def lib():
     return _lib
 
 
-def cast(bodies, filters, queries, ignore=(), mask=0xFFFFFFFF, membership=None):
+def cast(bodies, filters, queries, ignore=(), mask=0xFFFFFFFF, membership=None, hulls=None):
     """The oracle's shape queries: body records (n, 40), filters (n, 2) u32 (group, collides-with), queries (m, 8)
-    (origin, max distance, unit direction, radius) -> hits (m, 8): normal, distance or -1, index bits, 0, 0, 0."""
+    (origin, max distance, unit direction, radius), the hull buffer hull bodies name -> hits (m, 8): normal, distance
+    or -1, index bits, 0, 0, 0."""
     b = np.ascontiguousarray(bodies, np.float32).reshape(-1, 40)
     f = np.ascontiguousarray(filters, np.uint32).reshape(-1, 2)
     q = np.ascontiguousarray(queries, np.float32).reshape(-1, 8)
     ig = np.ascontiguousarray(list(ignore) or [0], np.uint32)
     hits = np.zeros((len(q), 8), np.float32)
+    hb = None if hulls is None or not len(hulls) else np.ascontiguousarray(hulls, np.uint32)
     lib().gs_cast(len(b), P(b, ctypes.c_float), P(f, ctypes.c_uint32), P(q, ctypes.c_float), len(q),
                   P(ig, ctypes.c_uint32), len(ignore), int(mask), int(membership or 0), int(membership is not None),
-                  P(hits, ctypes.c_float))
+                  P(hits, ctypes.c_float), P(hb, ctypes.c_uint32))
     return hits
 
 

@@ -1,6 +1,6 @@
 """Shape queries (src/gpu/shapeQueries.js) on the GPU against the oracle: every hit's normal, distance and body index
-bit for bit, for rays and sphere casts into a pile of boxes, spheres and capsules, with group masks, membership and an
-ignore list; and the facade's result records (raycast, raycastAll, sphereCast) with the reference's validation."""
+bit for bit, for rays and sphere casts into a pile of boxes, spheres and capsules (and one with convex hulls), with
+group masks, membership and an ignore list; and the facade's result records (raycast, raycastAll, sphereCast) with the reference's validation."""
 import numpy as np
 import pytest
 
@@ -9,13 +9,16 @@ from oracle_lib import cast
 pytestmark = pytest.mark.gpu
 
 
-def pile(avbd):
+def pile(avbd, hulls=False):
     rng = np.random.default_rng(11)
     s = avbd.AvbdScene3D(gravity=-9.81)
     s.addBox([30, 1, 30], density=0, position=[0, -0.5, 0])
+    shapes = [avbd.convexHull(rng.normal(size=(20, 3)) * rng.uniform(0.2, 0.5, 3)) for _ in range(3)] if hulls else []
     for k in range(300):
         pos = [float(rng.uniform(-6, 6)), float(0.6 + 0.3 * k / 10), float(rng.uniform(-6, 6))]
-        if k % 3 == 0:
+        if hulls and k % 4 == 3:
+            s.addHullShape(shapes[k % 3], position=pos, rotation=list(rng.normal(size=4)))
+        elif k % 3 == 0:
             s.addSphere(float(rng.uniform(0.25, 0.6)), position=pos)
         elif k % 3 == 1:
             s.addCapsule(float(rng.uniform(0.15, 0.35)), float(rng.uniform(0.8, 1.6)), position=pos,
@@ -39,9 +42,11 @@ def queries(n, radius, seed):
     return q
 
 
+@pytest.mark.parametrize("hulls", [False, True], ids=["boxes-spheres-capsules", "with-hulls"])
 @pytest.mark.parametrize("radius", [0.0, 0.2])
-def test_casts_match_the_oracle(avbd, radius):
-    world = pile(avbd).createSolver(avbd.createSolverPipeline("hip"))
+def test_casts_match_the_oracle(avbd, radius, hulls):
+    scene = pile(avbd, hulls)
+    world = scene.createSolver(avbd.createSolverPipeline("hip"))
     world.step(60)
     n = world.bodyCount
     filters = np.full((n, 2), 0xFFFFFFFF, np.uint32)
@@ -60,7 +65,7 @@ def test_casts_match_the_oracle(avbd, radius):
     native = import_module("babylon-avbd_amd._native")
     for opts in (dict(), dict(mask=1), dict(mask=0xFFFFFFFF, membership=4), dict(ignore=[1, 2, 3, 40])):
         want = cast(bodies, filters, qo, ignore=opts.get("ignore", ()), mask=opts.get("mask", 0xFFFFFFFF),
-                    membership=opts.get("membership"))
+                    membership=opts.get("membership"), hulls=scene.hull_buffer())
         got = np.zeros_like(want)
         ign = np.ascontiguousarray(opts.get("ignore", []), np.uint32)
         o = native.QueryOptions(ign.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if len(ign) else None, len(ign),

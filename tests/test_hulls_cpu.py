@@ -441,3 +441,76 @@ def test_hulls_and_terrain_rest_in_the_oracle():
     assert 0.25 - 0.06 < b[caps, 1] < 0.25, b[caps, 1]
     assert abs(b[box, 1] - 0.29) < 0.01, b[box, 1]
     assert np.abs(b[[cube, ball, caps, box], 32:35]).max() < 0.05
+
+
+# --- shape queries against hulls (the oracle's roundedPoly for hull records) ---------------------------------------------
+
+def _cast_scene(build):
+    from oracle_lib import cast
+    s = avbd.AvbdScene3D(gravity=0)
+    build(s)
+    b, *_ = s.packed()
+    filters = np.full((len(b), 2), 0xFFFFFFFF, np.uint32)
+    return lambda q: cast(b, filters, np.asarray(q, np.float32).reshape(-1, 8), hulls=s.hull_buffer())
+
+
+def test_sphere_casts_round_a_hull_corner():
+    # tests/gpu-package-features.js:193-221: a 2 m hull cube, a 0.5 sphere swept along +x
+    run = _cast_scene(lambda s: s.addHull(box_points(2, 2, 2), density=0))
+    hit = run([-3, 0, 0, 1e3, 1, 0, 0, 0.5])[0]
+    assert abs(hit[3] - 1.5) < 1e-3 and np.allclose(hit[0:3], [-1, 0, 0])
+    assert run([-3, 1.4, 1.4, 1e3, 1, 0, 0, 0.5])[0][3] < 0, "the hull's corner is rounded, not its bounding box"
+
+
+@pytest.mark.parametrize("radius", [0.0, 0.3])
+def test_queries_of_a_box_hull_equal_the_box(radius):
+    rng = np.random.default_rng(41)
+    hull_run = _cast_scene(lambda s: s.addHull(box_points(1.4, 0.8, 2.0), density=0))
+    box_run = _cast_scene(lambda s: s.addBox([1.4, 0.8, 2.0], density=0))
+    o = rng.uniform(-3, 3, (400, 3))
+    d = rng.uniform(-1, 1, (400, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q = np.concatenate([o, np.full((400, 1), 1e3), d, np.full((400, 1), radius)], 1)
+    h, b = hull_run(q), box_run(q)
+    assert np.array_equal(h[:, 3] >= 0, b[:, 3] >= 0)
+    hit = b[:, 3] >= 0
+    assert 100 < hit.sum() < 400
+    # (an edge's swept capsule is solved from its other end when the hull lists it the other way round: the FP32
+    # quadratic then rounds differently, by up to a few 1e-6 in distance and 1e-5 in normal)
+    assert np.allclose(h[hit, 3], b[hit, 3], atol=1e-5)
+    assert np.allclose(h[hit, 0:3], b[hit, 0:3], atol=3e-5)
+
+
+def test_rays_into_a_hull_match_a_slab_clip():
+    rng = np.random.default_rng(43)
+    shape = avbd.convexHull(rng.normal(size=(25, 3)) * [0.8, 0.5, 0.6])
+    q0 = _random_quat(rng)
+    run = _cast_scene(lambda s: s.addHullShape(shape, density=0, position=[0.3, -0.2, 0.1], rotation=q0))
+    s = avbd.AvbdScene3D()
+    s.addHullShape(shape, density=0, position=[0.3, -0.2, 0.1], rotation=q0)
+    rec = s.packed()[0][0]
+    _, planes, _ = _world_geometry(shape, rec)
+    o = rng.uniform(-3, 3, (300, 3))
+    d = rng.normal(size=(300, 3)) * 0.3 - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hits = run(np.concatenate([o, np.full((300, 1), 1e3), d, np.zeros((300, 1))], 1))
+    agree = 0
+    for k in range(300):
+        enter, leave = -np.inf, np.inf
+        for n, w in planes:  # the ray's entry and exit through every face plane (Cyrus–Beck), in float64
+            dn, gap = n @ d[k], w - n @ o[k]
+            if abs(dn) < 1e-12:
+                if gap < 0:
+                    enter, leave = 1, 0
+            elif dn < 0:
+                enter = max(enter, gap / dn)
+            else:
+                leave = min(leave, gap / dn)
+        want = None if enter > leave or leave < 0 else max(enter, 0.0)
+        got = hits[k, 3]
+        if want is None:
+            assert got < 0 or leave - enter > -1e-5, k
+        else:
+            assert got >= 0 and abs(got - want) < 1e-5, (k, got, want)
+            agree += 1
+    assert agree > 50
