@@ -180,6 +180,13 @@ napi_value CreateWorld(napi_env env, napi_callback_info info) {
     d.beta_ang = float(numberProp(env, s, "betaAng", 100));
     d.iterations = uint32_t(numberProp(env, s, "iterations", 10));
     d.flags = uint32_t(numberProp(env, s, "flags", AVBD_FLAGS_DEFAULT));
+    if (!typed(env, prop(env, s, "hulls"), napi_uint32_array, &p, &len, true)) return nullptr;
+    if (len % 4) {
+      napi_throw_range_error(env, nullptr, "hulls needs 4 words per record word");
+      return nullptr;
+    }
+    d.hull_words = uint32_t(len / 4);
+    d.hulls = static_cast<const uint32_t *>(p);
   }
   AvbdOptions o;
   std::memset(&o, 0, sizeof o);
@@ -704,6 +711,76 @@ napi_value SetParams(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+napi_value setNumber(napi_env env, napi_value obj, const char *name, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, name, x);
+  return obj;
+}
+napi_value setNumbers(napi_env env, napi_value obj, const char *name, const double *v, uint32_t n) {
+  napi_value a;
+  napi_create_array_with_length(env, n, &a);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value x;
+    napi_create_double(env, v[k], &x);
+    napi_set_element(env, a, k, x);
+  }
+  napi_set_named_property(env, obj, name, a);
+  return obj;
+}
+
+/** A cooked hull as {record: Uint32Array (header at word 0), volume, moments, size, radius, center, rotation}, or null
+ * when the input makes no hull (the reference's convexHull / hullFromTriangles return null). */
+napi_value hullResult(napi_env env, int code, const std::vector<uint32_t> &record, const AvbdHullInfo &info) {
+  napi_value out;
+  if (code != AVBD_OK) {
+    napi_get_null(env, &out);
+    return out;
+  }
+  void *data;
+  napi_value rec = newTyped(env, napi_uint32_array, 4ull * info.words, 4, &data);
+  std::memcpy(data, record.data(), 16ull * info.words);
+  napi_create_object(env, &out);
+  napi_set_named_property(env, out, "record", rec);
+  setNumber(env, out, "volume", info.volume);
+  setNumbers(env, out, "moments", info.moments, 3);
+  setNumbers(env, out, "size", info.size, 3);
+  setNumber(env, out, "radius", info.radius);
+  setNumbers(env, out, "center", info.center, 3);
+  setNumbers(env, out, "rotation", info.rotation, 4);
+  return out;
+}
+
+/** hullFromPoints(Float64Array xyz) (avbd_hull_from_points). */
+napi_value HullFromPoints(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!args(env, info, 1, argv)) return nullptr;
+  void *p;
+  size_t len;
+  if (!typed(env, argv[0], napi_float64_array, &p, &len, false)) return nullptr;
+  std::vector<uint32_t> record(4ull * AVBD_HULL_MAX_WORDS);
+  AvbdHullInfo hi;
+  const int code = avbd_hull_from_points(static_cast<const double *>(p), uint32_t(len / 3), 0, record.data(),
+                                         AVBD_HULL_MAX_WORDS, &hi);
+  return hullResult(env, code, record, hi);
+}
+
+/** hullFromTriangles(Float64Array xyz, Uint32Array indices) (avbd_hull_from_triangles). */
+napi_value HullFromTriangles(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!args(env, info, 2, argv)) return nullptr;
+  void *p, *q;
+  size_t len, ni;
+  if (!typed(env, argv[0], napi_float64_array, &p, &len, false)) return nullptr;
+  if (!typed(env, argv[1], napi_uint32_array, &q, &ni, false)) return nullptr;
+  std::vector<uint32_t> record(4ull * AVBD_HULL_MAX_WORDS);
+  AvbdHullInfo hi;
+  const int code = avbd_hull_from_triangles(static_cast<const double *>(p), uint32_t(len / 3),
+                                            static_cast<const uint32_t *>(q), uint32_t(ni), 0, record.data(),
+                                            AVBD_HULL_MAX_WORDS, &hi);
+  return hullResult(env, code, record, hi);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   const napi_property_descriptor props[] = {
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
@@ -748,6 +825,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"readContactEvents", nullptr, ReadContactEvents, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"getParams", nullptr, GetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setParams", nullptr, SetParams, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"hullFromPoints", nullptr, HullFromPoints, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"hullFromTriangles", nullptr, HullFromTriangles, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
   if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return nullptr;
   return exports;

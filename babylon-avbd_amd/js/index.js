@@ -20,7 +20,9 @@ const T_SPRING = 2;
 const SHAPE_BOX = 0;
 const SHAPE_SPHERE = 1;
 const SHAPE_CAPSULE = -1;
+const SHAPE_HULL = 3; // + the word of the body's hull record in the scene's hull buffer
 const FLAGS_DEFAULT = 31;
+const MAX_HULL_VERTICES = 32;
 
 function finite(x) {
   if (x === Infinity) return BIG;
@@ -87,6 +89,160 @@ function bodyRecord(position, size, mass, moment, radius, friction, shape) {
   return r;
 }
 
+/**
+ * A cooked convex hull (reference/three-avbd/src/avbd3d/hull.ts HullShape): the packed record the narrowphase reads
+ * (header at word 0; `record(at)` rebases it), mass properties per unit density and the principal frame, with the
+ * vertices, faces and edges decoded from the record.
+ */
+class HullShape {
+  constructor(cooked) {
+    this.words = cooked.record.length / 4;
+    this.data = cooked.record;
+    Object.assign(this, { volume: cooked.volume, moments: cooked.moments, size: cooked.size, radius: cooked.radius,
+      center: cooked.center, rotation: cooked.rotation });
+    const u = this.data, f = new Float32Array(u.buffer, u.byteOffset, u.length);
+    const [vs, nv, fs, nf] = [u[0], u[1], u[2], u[3]];
+    const [es, ne, is] = [u[4], u[5], u[6]];
+    this.vertices = new Float32Array(3 * nv);
+    for (let v = 0; v < nv; v++) this.vertices.set(f.subarray(4 * (vs + v), 4 * (vs + v) + 3), 3 * v);
+    this.faces = [];
+    for (let k = 0; k < nf; k++) {
+      const w = 4 * (fs + 2 * k), first = u[w + 4], count = u[w + 5];
+      this.faces.push({ normal: Array.from(f.subarray(w, w + 3)), d: f[w + 3],
+        verts: Array.from(u.subarray(4 * is + first, 4 * is + first + count)) });
+    }
+    this.edges = [];
+    for (let k = 0; k < ne; k++) this.edges.push(Array.from(u.subarray(4 * (es + k), 4 * (es + k) + 4)));
+  }
+
+  record(at) {
+    const r = Uint32Array.from(this.data);
+    for (const w of [0, 2, 4, 6]) r[w] += at;
+    return r;
+  }
+}
+
+function flatPoints(points, name) {
+  const a = Array.from(points, Number);
+  if (a.length % 3 || !a.every(Number.isFinite)) throw Error(`${name} must contain finite xyz triples`);
+  return Float64Array.from(a);
+}
+
+/** The convex hull of xyz points (at most MAX_HULL_VERTICES vertices), or null when they span no volume (hull.ts:60). */
+function convexHull(points) {
+  const cooked = native().hullFromPoints(flatPoints(points, 'Hull points'));
+  return cooked ? new HullShape(cooked) : null;
+}
+
+/** A hull from a closed outward triangulation of a convex point set, or null (hull.ts:208). */
+function hullFromTriangles(positions, indices) {
+  const idx = Array.from(indices, Number);
+  const n = positions.length / 3;
+  if (idx.length % 3 || !idx.every((i) => Number.isInteger(i) && i >= 0 && i < n)) return null;
+  const cooked = native().hullFromTriangles(flatPoints(positions, 'positions'), Uint32Array.from(idx));
+  return cooked ? new HullShape(cooked) : null;
+}
+
+// --- terrain (src/native/terrain.js): every triangle a static prism hull ---------------------------------------------
+
+// The closed prism over top corners 0, 1, 2 and their base copies 3, 4, 5, and the same with each triangle turned
+const PRISM_FACES = [0, 1, 2, 3, 5, 4, 0, 3, 4, 0, 4, 1, 1, 4, 5, 1, 5, 2, 2, 5, 3, 2, 3, 0];
+const REVERSED_PRISM_FACES = PRISM_FACES.map((v, i) => PRISM_FACES[i - (i % 3) + [0, 2, 1][i % 3]]);
+
+function finiteList(values, length, name) {
+  const a = Array.from(values || [], Number);
+  if (a.length !== length || !a.every(Number.isFinite)) throw Error(`${name} requires ${length} finite values`);
+  return a;
+}
+
+function terrainPiece(points) {
+  const center = [0, 1, 2].map((k) => points.reduce((sum, p) => sum + p[k], 0) / points.length);
+  const [a, b, c, back] = points;
+  const u = b.map((v, k) => v - a[k]), v = c.map((x, k) => x - a[k]);
+  const outward = (u[1] * v[2] - u[2] * v[1]) * (a[0] - back[0]) + (u[2] * v[0] - u[0] * v[2]) * (a[1] - back[1]) +
+    (u[0] * v[1] - u[1] * v[0]) * (a[2] - back[2]);
+  const shape = hullFromTriangles([].concat(...points.map((p) => p.map((x, k) => x - center[k]))),
+    outward > 0 ? PRISM_FACES : REVERSED_PRISM_FACES);
+  if (!shape) throw Error('Terrain contains a degenerate collision piece');
+  return { shape, position: shape.center.map((x, k) => x + center[k]), rotation: shape.rotation };
+}
+
+/** Static solid ground; heights are row-major, each cell keeps both triangles (terrain.js:62-134). */
+function cookHeightfield(options) {
+  const o = options || {};
+  const { columns, rows } = o;
+  for (const [name, value] of [['columns', columns], ['rows', rows]])
+    if (!Number.isInteger(value) || value < 2) throw Error(`${name} must be an integer >= 2`);
+  const values = finiteList(o.heights, columns * rows, 'heights');
+  const offset = finiteList(o.origin || [0, 0, 0], 3, 'origin');
+  const upAxis = o.upAxis === undefined ? 1 : o.upAxis;
+  if (![0, 1, 2].includes(upAxis)) throw Error('upAxis must be 0, 1 or 2');
+  const spacing = o.spacing === undefined ? 1 : o.spacing;
+  const step = typeof spacing === 'number' ? [spacing, spacing] : finiteList(spacing, 2, 'spacing');
+  step.forEach((v) => positive(v, 'spacing'));
+  const base = o.baseHeight === undefined ? Math.min(...values) - 2 : o.baseHeight;
+  if (!Number.isFinite(base) || values.some((h) => h <= base)) throw Error('baseHeight must be below every terrain height');
+  const cells = (columns - 1) * (rows - 1);
+  if (o.holes && o.holes.length !== cells) throw Error('holes requires one value per terrain cell');
+  const axes = [0, 1, 2].filter((k) => k !== upAxis), pieces = [];
+  const point = (x, z) => {
+    const p = offset.slice();
+    p[axes[0]] += x * step[0];
+    p[axes[1]] += z * step[1];
+    p[upAxis] += values[z * columns + x];
+    return p;
+  };
+  for (let z = 0; z < rows - 1; z++)
+    for (let x = 0; x < columns - 1; x++) {
+      if (o.holes && o.holes[z * (columns - 1) + x]) continue;
+      const a = point(x, z), b = point(x + 1, z), c = point(x, z + 1), d = point(x + 1, z + 1);
+      for (const top of [[a, b, c], [b, d, c]])
+        pieces.push(terrainPiece(top.concat(top.map((p) => { const q = p.slice(); q[upAxis] = offset[upAxis] + base; return q; }))));
+    }
+  return { kind: 'heightfield', pieces, triangleCount: pieces.length };
+}
+
+/** Triangle surfaces, each backed `thickness` against its winding normal (terrain.js:137-176). */
+function cookStaticMesh(options) {
+  const o = options || {};
+  if (!o.positions || o.positions.length % 3) throw Error('positions requires xyz triples');
+  const p = finiteList(o.positions, o.positions.length, 'positions'), count = p.length / 3;
+  if (o.indices === undefined && count % 3) throw Error('Nonindexed positions require three vertices per triangle');
+  const ids = o.indices === undefined ? Array.from({ length: count }, (_, i) => i) : Array.from(o.indices);
+  if (!ids.length || ids.length % 3 || !ids.every((i) => Number.isInteger(i) && i >= 0 && i < count))
+    throw Error('indices requires valid triangle vertex indices');
+  const thickness = o.thickness === undefined ? 0.1 : o.thickness;
+  positive(thickness, 'thickness');
+  const pieces = [];
+  for (let i = 0; i < ids.length; i += 3) {
+    const top = ids.slice(i, i + 3).map((id) => p.slice(3 * id, 3 * id + 3)), [a, b, c] = top;
+    const u = b.map((v, k) => v - a[k]), v = c.map((x, k) => x - a[k]);
+    const n = [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]];
+    const length = Math.hypot(...n);
+    if (length === 0) continue; // imported meshes often carry empty seam triangles
+    pieces.push(terrainPiece(top.concat(top.map((q) => q.map((x, k) => x - (n[k] * thickness) / length)))));
+  }
+  if (!pieces.length) throw Error('Static mesh requires at least one nondegenerate triangle');
+  return { kind: 'static-mesh', pieces, triangleCount: pieces.length };
+}
+
+/** Static hull bodies for cooked pieces (terrain.js:179-218). */
+function createTerrainBodies(scene, cooked, options) {
+  const o = options || {};
+  if (o.mass !== undefined && o.mass !== 0) throw Error('Terrain is static and requires zero mass');
+  const friction = o.friction === undefined ? 0.6 : o.friction, restitution = o.restitution === undefined ? 0 : o.restitution;
+  if (!Number.isFinite(friction) || friction < 0) throw Error('friction must be nonnegative and finite');
+  if (!Number.isFinite(restitution) || restitution < 0 || restitution > 1) throw Error('restitution must be between 0 and 1');
+  for (const name of ['group', 'collidesWith']) {
+    const value = o[name] === undefined ? 0xffffffff : o[name];
+    if (!Number.isInteger(value) || value < 0 || value > 0xffffffff) throw Error(`${name} must be an unsigned 32-bit integer`);
+  }
+  if (!cooked || !cooked.pieces || !cooked.pieces.length) throw Error('Terrain requires collision pieces');
+  return cooked.pieces.map((p) => scene.addHullShape(p.shape, { density: 0, friction, position: p.position,
+    rotation: p.rotation, allowSleep: false, restitution, group: o.group === undefined ? 0xffffffff : o.group,
+    collidesWith: o.collidesWith === undefined ? 0xffffffff : o.collidesWith }));
+}
+
 /** Portable 3D scene builder (scenes.js:107-273): Y up, gravity -10, dt 1/60, 10 iterations by default. */
 class AvbdScene3D {
   constructor(options) {
@@ -114,6 +270,8 @@ class AvbdScene3D {
     this.sleepOptIn = false;       // a body was built with allowSleep true: its world sleeps (scenes.js:233-240, 689)
     this.joints = [];
     this.ignore = [];
+    this.hulls = new Map();  // HullShape -> header word in the scene's hull buffer (one record per shape)
+    this.hullWords = 0;
     this.created = false;
   }
 
@@ -202,6 +360,53 @@ class AvbdScene3D {
     return this.add(bodyRecord(pos, [2 * r, h, 2 * r], m, moment, bound, friction, SHAPE_CAPSULE), o);
   }
 
+  /** A convex hull of xyz points, its principal frame at `position` (its centre of mass) (scenes.js:160-179). */
+  addHull(points, options) {
+    this.assertEditable();
+    const shape = convexHull(flatPoints(points, 'Hull points'));
+    if (!shape) throw Error('A convex hull requires finite points spanning a volume');
+    return this.addHullShape(shape, options);
+  }
+
+  /** A body of a cooked HullShape (shapes.ts:50-70 `hull`). */
+  addHullShape(shape, options) {
+    const o = options || {};
+    this.assertEditable();
+    if (shape.vertices.length / 3 > MAX_HULL_VERTICES || shape.faces.length > 255 || shape.edges.length > 255)
+      throw Error(`hull: ${shape.vertices.length / 3} vertices, ${shape.faces.length} faces, ${shape.edges.length} edges (at most ${MAX_HULL_VERTICES} vertices, 255 faces and edges)`);
+    const density = positive(o.density === undefined ? 1 : o.density, 'density', true);
+    const friction = positive(o.friction === undefined ? 0.6 : o.friction, 'friction', true);
+    const pos = vector(o.position || [0, 0, 0], 3, 'position');
+    if (!this.hulls.has(shape)) {
+      this.hulls.set(shape, this.hullWords);
+      this.hullWords += shape.words;
+    }
+    const at = this.hulls.get(shape);
+    const m = density > 0 ? shape.volume * density : 0;
+    const moment = shape.moments.map((x) => x * Math.max(density, 0));
+    if (Math.fround(SHAPE_HULL + at) !== SHAPE_HULL + at) throw Error('hull buffer beyond 2^24 words');
+    return this.add(bodyRecord(pos, shape.size, m, moment, shape.radius, friction, SHAPE_HULL + at), o);
+  }
+
+  /** The scene's hull records, concatenated (4 u32 per word), or null without hulls. */
+  hullBuffer() {
+    if (!this.hulls.size) return null;
+    const out = new Uint32Array(4 * this.hullWords);
+    for (const [shape, at] of this.hulls) out.set(shape.record(at), 4 * at);
+    return out;
+  }
+
+  addHeightfield(options) { return this.addTerrain(cookHeightfield(options), options); }
+
+  addStaticMesh(options) { return this.addTerrain(cookStaticMesh(options), options); }
+
+  /** Register cooked terrain pieces as static hull bodies (scenes.js:203-231). */
+  addTerrain(cooked, options) {
+    this.assertEditable();
+    const bodies = createTerrainBodies(this, cooked, options);
+    return { bodies, triangleCount: cooked.triangleCount, kind: cooked.kind };
+  }
+
   check(a, b) {
     const n = this.records.length;
     if (b === null || b === undefined || a === b || !(b >= 0 && b < n) || (a !== null && !(a >= 0 && a < n)))
@@ -284,7 +489,7 @@ class AvbdScene3D {
       filters = new Uint32Array(2 * n).fill(0xffffffff);
       for (const [i, [g, m]] of this.filters) { filters[2 * i] = g; filters[2 * i + 1] = m; }
     }
-    return Object.assign({ bodies, joints, jointInfo, ignore, filters, up: this.up }, this.params);
+    return Object.assign({ bodies, joints, jointInfo, ignore, filters, hulls: this.hullBuffer(), up: this.up }, this.params);
   }
 }
 
@@ -995,4 +1200,5 @@ async function createSolverPipeline(options) {
   };
 }
 
-module.exports = { createSolverPipeline, AvbdScene3D, HipSolver3D, CommandQueue, native, FLAGS_DEFAULT };
+module.exports = { createSolverPipeline, AvbdScene3D, HipSolver3D, CommandQueue, native, FLAGS_DEFAULT, HullShape,
+  MAX_HULL_VERTICES, convexHull, hullFromTriangles, cookHeightfield, cookStaticMesh, createTerrainBodies };

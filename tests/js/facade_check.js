@@ -15,6 +15,9 @@ function build(spec) {
     else if (kind === 'joint') s.addJoint(a[0], a[1], a[2]);
     else if (kind === 'spring') s.addSpring(a[0], a[1], a[2]);
     else if (kind === 'ignore') s.ignoreCollision(a[0], a[1]);
+    else if (kind === 'hull') s.addHull(a[0], a[1]);
+    else if (kind === 'heightfield') s.addHeightfield(a[0]);
+    else if (kind === 'staticMesh') s.addStaticMesh(a[0]);
     else throw Error(`unknown op ${kind}`);
   }
   return s;
@@ -56,7 +59,8 @@ async function main() {
     console.log(JSON.stringify(specs.map((spec) => {
       const p = build(spec).packed();
       return { bodies: bits(p.bodies), joints: bits(p.joints), jointInfo: Array.from(p.jointInfo),
-        ignore: Array.from(p.ignore), filters: p.filters ? Array.from(p.filters) : null };
+        ignore: Array.from(p.ignore), filters: p.filters ? Array.from(p.filters) : null,
+        hulls: p.hulls ? Array.from(p.hulls) : null };
     })));
   } else if (mode === 'errors') {
     const out = {};

@@ -49,6 +49,12 @@ def build_py(avbd, spec):
             s.addSpring(a[0], a[1], **(a[2] if len(a) > 2 else {}))
         elif kind == "ignore":
             s.ignoreCollision(a[0], a[1])
+        elif kind == "hull":
+            s.addHull(a[0], **(a[1] if len(a) > 1 else {}))
+        elif kind == "heightfield":
+            s.addHeightfield(**a[0])
+        elif kind == "staticMesh":
+            s.addStaticMesh(**a[0])
     return s
 
 
@@ -68,8 +74,16 @@ def specs_mixed():
     ops.append(["spring", 4, 5, {"anchorA": [0.2, 0, 0], "stiffness": 300.0}])
     ops.append(["spring", 5, 6, {"rest": 1.25}])
     ops.append(["ignore", 7, 8])
+    cloud = [float(x) for x in rng.normal(size=60)]
+    terrain = [["heightfield", {"heights": [0.1 * (k % 5) for k in range(20)], "columns": 5, "rows": 4,
+                                "spacing": [1.5, 2.0], "origin": [-3, 0, -3], "friction": 0.8, "group": 3}],
+               ["staticMesh", {"positions": [0, 0, 0, 4, 0, 0, 0, 1, 4, 4, 1, 4], "indices": [0, 2, 1, 1, 2, 3],
+                               "thickness": 0.2}],
+               ["hull", cloud, {"position": [1, 3, 0], "rotation": [0.2, 0.1, 0.3, 0.9], "density": 2.0}],
+               ["hull", [0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], {"position": [-1, 2, 1], "mass": 3.0}]]
     return [{"params": {"gravity": -9.81, "iterations": 12, "timeStep": 1 / 120}, "ops": ops},
-            {"ops": [["box", [1, 1, 1], {"position": [0, 0.5 * k, 0]}] for k in range(5)]}]
+            {"ops": [["box", [1, 1, 1], {"position": [0, 0.5 * k, 0]}] for k in range(5)]},
+            {"params": {"gravity": -9.81}, "ops": terrain}]
 
 
 def test_addon_exports_the_bound_entry_points(tmp_path):
@@ -89,6 +103,8 @@ def test_js_and_python_builders_pack_the_same_bytes(avbd, tmp_path):
         assert got["jointInfo"] == info.ravel().tolist()
         assert got["ignore"] == ignore.ravel().tolist()
         assert got["filters"] == (None if filters is None else filters.ravel().tolist())
+        hulls = build_py(avbd, spec).hull_buffer()
+        assert got["hulls"] == (hulls.ravel().tolist() if len(hulls) else None)
 
 
 def test_errors_follow_the_reference(tmp_path):
