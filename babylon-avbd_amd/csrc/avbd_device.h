@@ -167,12 +167,17 @@ AVBD_HD float4 qmul(float4 a, float4 b) {
 }
 AVBD_HD float4 qconj(float4 q) { return make_float4(-q.x, -q.y, -q.z, q.w); }
 AVBD_HD float qlengthSq(float4 q) { return (q.x * q.x + q.y * q.y + q.z * q.z) + q.w * q.w; }
-AVBD_HD f3 qsub(float4 a, float4 b) {
+/** b⁻¹ = conj(b) / |b|² (componentwise divisions, as qsub forms it). */
+AVBD_HD float4 qinv(float4 b) {
   float l = qlengthSq(b);
-  float4 bi = make_float4(-b.x / l, -b.y / l, -b.z / l, b.w / l);
+  return make_float4(-b.x / l, -b.y / l, -b.z / l, b.w / l);
+}
+/** qsub with b⁻¹ already formed: the same bits as qsub(a, b). */
+AVBD_HD f3 qsubInv(float4 a, float4 bi) {
   float4 s = qmul(a, bi);
   return f3{s.x * 2.f, s.y * 2.f, s.z * 2.f};
 }
+AVBD_HD f3 qsub(float4 a, float4 b) { return qsubInv(a, qinv(b)); }
 AVBD_HD float4 qadd(float4 a, f3 v) {
   float4 s = qmul(make_float4(v.x, v.y, v.z, 0.f), a);
   float4 q = make_float4(a.x + s.x * 0.5f, a.y + s.y * 0.5f, a.z + s.z * 0.5f, a.w + s.w * 0.5f);
@@ -222,13 +227,14 @@ AVBD_HD float atan2r(float y, float x) {
 /** An angle wrapped into [−π, π) (the reference's atan2(sin x, cos x)). */
 AVBD_HD float wrapAngle(float x) { return x - TWO_PI_F * floorf((x + PI_F) / TWO_PI_F); }
 
-/** Contact basis rows (n, t1, t2) (layout.ts:236-242). */
+/** Contact basis rows (n, t1, t2) (layout.ts:236-242); t1 is scaled by the reciprocal of its length (one division
+ * instead of three; the CPU restatement forms it the same way). */
 struct Basis {
   f3 n, t1, t2;
 };
 AVBD_HD Basis orthonormal(f3 n) {
   f3 t1 = fabsf(n.x) > fabsf(n.z) ? f3{-n.y, n.x, 0.f} : f3{0.f, -n.z, n.y};
-  t1 = normalize(t1);
+  t1 = scale(t1, 1.f / length(t1));
   return Basis{n, t1, cross(n, t1)};
 }
 

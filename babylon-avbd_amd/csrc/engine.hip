@@ -241,6 +241,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.bodyWorld = dallocT<uint32_t>(nBodies_);
   w.worlds = dallocT<WorldParams>(nWorlds_);
   w.refs = dallocT<float4>(2ull * nBodies_);
+  w.irotInv = dallocT<float4>(nBodies_);
   w.restitution = dallocT<float>(nBodies_);
   w.bounceVel = dallocT<float4>(4ull * nBodies_);
   restitution_.assign(nBodies_, 0.f);

@@ -42,6 +42,7 @@ struct DeviceWorld {
   uint32_t *bodyWorld;   // world of each body
   WorldParams *worlds;
   float4 *refs;          // 2 per body: reference pos, reference rot (contact reuse)
+  float4 *irotInv;       // per body: the inverse of its step-start rotation, formed by the warm start
   uint2 *filters;        // groups, collides-with
   Joint *joints;
   int4 *info;            // type, a (-1 = world), b, 0

@@ -258,6 +258,53 @@ AVBD_D void addRow(Acc &s, f3 l, f3 a, float k, float f) {
   s.rAng = mk3(fmaf(a.x, f, s.rAng.x), fmaf(a.y, f, s.rAng.y), fmaf(a.z, f, s.rAng.z));
 }
 
+/**
+ * The three rows of one contact point (normal and tangents b_q of the basis, signed by the body's side sg, lever arm r,
+ * penalties k, forces f) added at once: with M = Σ k_q b_q b_qᵀ (the sign cancels) and g = sg Σ f_q b_q, the linear
+ * block takes M, the cross block [r]× M, the angular block [r]× M [r]×ᵀ, the right-hand side (g, r × g). The same
+ * system as three addRow calls, in about half the arithmetic (the CPU restatement forms it the same way).
+ */
+AVBD_D void addContactRows(Acc &s, const Basis &b, float sg, f3 r, f3 k, f3 f) {
+  const f3 kn = scale(b.n, k.x), k1 = scale(b.t1, k.y), k2 = scale(b.t2, k.z);
+  const float m00 = fmaf(k2.x, b.t2.x, fmaf(k1.x, b.t1.x, kn.x * b.n.x));
+  const float m10 = fmaf(k2.y, b.t2.x, fmaf(k1.y, b.t1.x, kn.y * b.n.x));
+  const float m11 = fmaf(k2.y, b.t2.y, fmaf(k1.y, b.t1.y, kn.y * b.n.y));
+  const float m20 = fmaf(k2.z, b.t2.x, fmaf(k1.z, b.t1.x, kn.z * b.n.x));
+  const float m21 = fmaf(k2.z, b.t2.y, fmaf(k1.z, b.t1.y, kn.z * b.n.y));
+  const float m22 = fmaf(k2.z, b.t2.z, fmaf(k1.z, b.t1.z, kn.z * b.n.z));
+  // C = [r]× M (row: angular, column: linear)
+  const float c00 = fmaf(r.y, m20, -(r.z * m10)), c01 = fmaf(r.y, m21, -(r.z * m11)), c02 = fmaf(r.y, m22, -(r.z * m21));
+  const float c10 = fmaf(r.z, m00, -(r.x * m20)), c11 = fmaf(r.z, m10, -(r.x * m21)), c12 = fmaf(r.z, m20, -(r.x * m22));
+  const float c20 = fmaf(r.x, m10, -(r.y * m00)), c21 = fmaf(r.x, m11, -(r.y * m10)), c22 = fmaf(r.x, m21, -(r.y * m20));
+  // Angular block C [r]×ᵀ, lower triangle
+  s.a00 += fmaf(r.y, c02, -(r.z * c01));
+  s.a10 += fmaf(r.y, c12, -(r.z * c11));
+  s.a11 += fmaf(r.z, c10, -(r.x * c12));
+  s.a20 += fmaf(r.y, c22, -(r.z * c21));
+  s.a21 += fmaf(r.z, c20, -(r.x * c22));
+  s.a22 += fmaf(r.x, c21, -(r.y * c20));
+  s.l00 += m00;
+  s.l10 += m10;
+  s.l11 += m11;
+  s.l20 += m20;
+  s.l21 += m21;
+  s.l22 += m22;
+  s.c[0] += c00;
+  s.c[1] += c01;
+  s.c[2] += c02;
+  s.c[3] += c10;
+  s.c[4] += c11;
+  s.c[5] += c12;
+  s.c[6] += c20;
+  s.c[7] += c21;
+  s.c[8] += c22;
+  const f3 g = scale(mk3(fmaf(f.z, b.t2.x, fmaf(f.y, b.t1.x, f.x * b.n.x)), fmaf(f.z, b.t2.y, fmaf(f.y, b.t1.y, f.x * b.n.y)),
+                         fmaf(f.z, b.t2.z, fmaf(f.y, b.t1.z, f.x * b.n.z))),
+                     sg);
+  s.rLin = add(s.rLin, g);
+  s.rAng = add(s.rAng, mk3(fmaf(r.y, g.z, -(r.z * g.y)), fmaf(r.z, g.x, -(r.x * g.z)), fmaf(r.x, g.y, -(r.y * g.x))));
+}
+
 AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, uint32_t i) {
   int4 f = w.info[j];
   int a = f.y, b = f.z;
@@ -308,19 +355,21 @@ AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, ui
   }
 }
 
-/** What a contact needs of one body: rotation, x- rotation, displacement since x- (wgsl-solve.ts:145-157). */
+/** What a contact needs of one body: rotation, displacement since x- (wgsl-solve.ts:145-157), its index. */
 struct PairBody {
-  float4 rot, irot;
+  float4 rot;
   f3 dLin, dAng;
+  uint32_t index;
 };
+/** The angular displacement qsub(rot, x- rotation) from the warm start's inverse of the x- rotation (same bits). */
 template <bool DF = false> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
   const Body &B = w.bodies[i];
   PairBody pb;
   const float4 pos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
   pb.rot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
-  pb.irot = B.irot;
   pb.dLin = sub(xyz(pos), xyz(B.ipos));
-  pb.dAng = qsub(pb.rot, pb.irot);
+  pb.dAng = qsubInv(pb.rot, w.irotInv[i]);
+  pb.index = i;
   return pb;
 }
 
@@ -329,13 +378,14 @@ struct Eval {
   float frictionScale, bounds;
 };
 
-AVBD_D float rowC(float c0, float keep, f3 l, const Eval &e, const PairBody &A, const PairBody &B) {
-  return c0 * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
-}
-
-/** evalContact (wgsl-solve.ts:171-206). */
-AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, const PairBody &A, const PairBody &B,
-                        float alpha, uint32_t flags) {
+/**
+ * evalContact (wgsl-solve.ts:171-206). The rows' C = C0 (1 - α) + J Δx take the displacement once as the relative
+ * motion of the two anchors, Δ = (Δx_A − Δx_B) + Δθ_A × r_A − Δθ_B × r_B, and each row as C0 (1 − α) + l · Δ (the same
+ * value as the reference's per-row sum l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B, rounded differently; the
+ * CPU restatement forms it the same way).
+ */
+AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction, const PairBody &A,
+                        const PairBody &B, float alpha, uint32_t flags) {
   uint32_t key = fbits(k.rA.w);
   // Sphere contacts take their lever arms at the step-start rotation (wgsl-solve.ts:172-181): a rolling sphere would
   // otherwise see a false separation in its normal row. Capsule contacts (keys 4<<24 | j) do so only with
@@ -344,11 +394,12 @@ AVBD_D Eval evalContact(const Contact &k, const Basis &basis, float friction, co
   const bool rounded = feature == SPHERE_FEATURE ||
                        ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (feature >> 24) == (CAPSULE_FEATURE >> 24));
   Eval e;
-  e.rAW = qrotate(rounded ? A.irot : A.rot, xyz(k.rA));
-  e.rBW = qrotate(rounded ? B.irot : B.rot, xyz(k.rB));
+  e.rAW = qrotate(rounded ? w.bodies[A.index].irot : A.rot, xyz(k.rA));
+  e.rBW = qrotate(rounded ? w.bodies[B.index].irot : B.rot, xyz(k.rB));
   float keep = 1.f - alpha;
-  e.C = mk3(rowC(k.rB.w, keep, basis.n, e, A, B), rowC(k.pen.w, keep, basis.t1, e, A, B),
-            rowC(k.lam.w, keep, basis.t2, e, A, B));
+  const f3 delta = sub(add(sub(A.dLin, B.dLin), cross(A.dAng, e.rAW)), cross(B.dAng, e.rBW));
+  e.C = mk3(k.rB.w * keep + dot(basis.n, delta), k.pen.w * keep + dot(basis.t1, delta),
+            k.lam.w * keep + dot(basis.t2, delta));
   f3 F = add(mul3(xyz(k.pen), e.C), xyz(k.lam));
   F.x = rmin(F.x, 0.f);
   e.bounds = fabsf(F.x) * friction;
@@ -395,18 +446,13 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
   const PairBody PA = isA ? self : other;
   const PairBody PB = isA ? other : self;
   Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
-  float sg = isA ? 1.f : -1.f;
-  f3 l0 = scale(basis.n, sg), l1 = scale(basis.t1, sg), l2 = scale(basis.t2, sg);
   Contact k = w.contacts[head.y + p];
   if constexpr (DF) {
     k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
     k.lam = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_LAM));
   }
-  Eval ev = evalContact(k, basis, __uint_as_float(head.w), PA, PB, alpha, flags);
-  f3 r = isA ? ev.rAW : ev.rBW;
-  addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-  addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-  addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+  Eval ev = evalContact(w, k, basis, __uint_as_float(head.w), PA, PB, alpha, flags);
+  addContactRows(acc, basis, isA ? 1.f : -1.f, isA ? ev.rAW : ev.rBW, xyz(k.pen), ev.F);
 }
 
 /**
@@ -648,9 +694,9 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
   const Body &B = w.bodies[i];
   PairBody self;
   self.rot = rot;
-  self.irot = B.irot;
   self.dLin = sub(xyz(pos), xyz(B.ipos));
-  self.dAng = qsub(rot, self.irot);
+  self.dAng = qsubInv(rot, w.irotInv[i]);
+  self.index = i;
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
     uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
@@ -691,7 +737,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
         Contact kc = w.contacts[c];
         kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
         kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
-        const Eval ev = evalContact(kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
+        const Eval ev = evalContact(w, kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
                                     isA ? other : self, p.alpha, p.flags);
         dualUpdate<true>(w, c, kc, ev, p);
       }
@@ -809,7 +855,7 @@ AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
   const Contact kc = w.contacts[c];
-  dualUpdate<false>(w, c, kc, evalContact(kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
+  dualUpdate<false>(w, c, kc, evalContact(w, kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */

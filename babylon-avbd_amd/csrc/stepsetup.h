@@ -38,6 +38,7 @@ AVBD_D void warmBody(const DeviceWorld &w, uint32_t i) {
   bool atRest = (p.flags & FLAG_START_AT_REST) && slow;
   B.ipos = pos;
   B.irot = rot;
+  w.irotInv[i] = qinv(rot);  // every contact row of the step turns displacements by it
   if (dyn && !atRest) {
     B.pos = mk4(add(add(xyz(pos), scale(xyz(vel), dt)), scale(up, g * (wgt * dt * dt))), pos.w);
     B.rot = qadd(rot, scale(angVel, dt));

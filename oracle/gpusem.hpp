@@ -108,6 +108,19 @@ inline bool stepStartArms(uint32_t key, uint32_t flags) {
   return k == SPHERE_FEATURE || ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (k >> 24) == (CAPSULE_FEATURE >> 24));
 }
 
+/** The GPU step's contact basis: rows (n, t1, t2) as orthonormal (layout.ts:236-242), with t1 scaled by the reciprocal
+ * of its length (the HIP orthonormal's form). */
+template <class R> inline M3<R> contactBasis(V3<R> n) {
+  V3<R> t1 = std::fabs(n.x) > std::fabs(n.z) ? V3<R>(-n.y, n.x, 0) : V3<R>(0, -n.z, n.y);
+  t1 = scale(t1, R(1) / length(t1));
+  V3<R> t2 = cross(n, t1);
+  M3<R> o;
+  o.m[0] = n.x; o.m[1] = n.y; o.m[2] = n.z;
+  o.m[3] = t1.x; o.m[4] = t1.y; o.m[5] = t1.z;
+  o.m[6] = t2.x; o.m[7] = t2.y; o.m[8] = t2.z;
+  return o;
+}
+
 /**
  * Contact points of one pair by shape (wgsl-collision.ts:684-726 with the hull branch :697-711 and the capsule splice
  * of capsuleCollision.js:17-29): capsules first (against a hull: capsulePoly), then two spheres, then pairs with a
@@ -406,7 +419,7 @@ template <class R> class World {
     o.m = prev;
     o.count = int(count);
     const Body<R> &A = bodies[a], &B = bodies[b];
-    M3<R> basis = orthonormal(prev.normal);
+    M3<R> basis = contactBasis(prev.normal);
     R alpha = R(p.alpha), gamma = R(p.gamma);
     R pmin = contactPenaltyMin(a, b);
     for (uint32_t i = 0; i < count; i++) {
@@ -447,7 +460,7 @@ template <class R> class World {
     mf.normal = neg(sat.n);
     mf.friction = std::sqrt(BA.friction * BB.friction);
     o.count = found.count;
-    M3<R> basis = orthonormal(mf.normal);
+    M3<R> basis = contactBasis(mf.normal);
     bool anySphere = sphereA || sphereB || capsuleA || capsuleB;
     uint32_t prevFirst = 0, prevCount = 0;
     int64_t at = previousManifold(a, b);
@@ -887,10 +900,10 @@ template <class R> class World {
     e.rBW = rotate(rotB, k.rB);
     V3<R> n = row(basis, 0), t1 = row(basis, 1), t2 = row(basis, 2);
     R keep = R(1) - alpha;
-    auto rowC = [&](R c0, V3<R> l) {
-      return c0 * keep + dot(l, A.dLin) - dot(l, B.dLin) + dot(cross(e.rAW, l), A.dAng) + dot(cross(e.rBW, neg(l)), B.dAng);
-    };
-    e.C = V3<R>(rowC(k.c0x, n), rowC(k.c0y, t1), rowC(k.c0z, t2));
+    // The anchors' relative displacement once, Δ = (Δx_A − Δx_B) + Δθ_A × r_A − Δθ_B × r_B, each row C0 (1 − α) + l·Δ
+    // (the reference's per-row sum l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B, rounded as the HIP step does)
+    const V3<R> delta = sub(add(sub(A.dLin, B.dLin), cross(A.dAng, e.rAW)), cross(B.dAng, e.rBW));
+    e.C = V3<R>(k.c0x * keep + dot(n, delta), k.c0y * keep + dot(t1, delta), k.c0z * keep + dot(t2, delta));
     V3<R> F = add(mulv3(k.pen, e.C), k.lam);
     F.x = rmin(F.x, R(0));
     e.bounds = std::fabs(F.x) * friction;
@@ -983,15 +996,50 @@ template <class R> class World {
     const Manifold<R> &mf = manifolds[id - J];
     bool isA = i == mf.a;
     PairBody A = pairBody(mf.a), B = pairBody(mf.b);
-    M3<R> basis = orthonormal(mf.normal);
-    R sg = isA ? R(1) : R(-1);
-    V3<R> l0 = scale(row(basis, 0), sg), l1 = scale(row(basis, 1), sg), l2 = scale(row(basis, 2), sg);
+    M3<R> basis = contactBasis(mf.normal);
     const Contact<R> &k = contacts[mf.first + p];
     Eval ev = evalContact(k, basis, mf.friction, A, B, alpha);
-    V3<R> r = isA ? ev.rAW : ev.rBW;
-    addRow(acc, l0, cross(r, l0), k.pen.x, ev.F.x);
-    addRow(acc, l1, cross(r, l1), k.pen.y, ev.F.y);
-    addRow(acc, l2, cross(r, l2), k.pen.z, ev.F.z);
+    addContactRows(acc, basis, isA ? R(1) : R(-1), isA ? ev.rAW : ev.rBW, k.pen, ev.F);
+  }
+
+  /**
+   * The three rows of one contact point at once (the HIP addContactRows): M = Σ k_q b_q b_qᵀ, g = sg Σ f_q b_q; the
+   * linear block takes M, the cross block [r]× M, the angular block [r]× M [r]×ᵀ (lower triangle; the upper one is
+   * not read), the right-hand side (g, r × g), each entry with the same fused multiply-adds.
+   */
+  static void addContactRows(Acc &acc, const M3<R> &basis, R sg, V3<R> r, V3<R> k, V3<R> f) {
+    const V3<R> n = row(basis, 0), t1 = row(basis, 1), t2 = row(basis, 2);
+    const V3<R> kn = scale(n, k.x), k1 = scale(t1, k.y), k2 = scale(t2, k.z);
+    using std::fma;
+    const R m00 = fma(k2.x, t2.x, fma(k1.x, t1.x, kn.x * n.x));
+    const R m10 = fma(k2.y, t2.x, fma(k1.y, t1.x, kn.y * n.x));
+    const R m11 = fma(k2.y, t2.y, fma(k1.y, t1.y, kn.y * n.y));
+    const R m20 = fma(k2.z, t2.x, fma(k1.z, t1.x, kn.z * n.x));
+    const R m21 = fma(k2.z, t2.y, fma(k1.z, t1.y, kn.z * n.y));
+    const R m22 = fma(k2.z, t2.z, fma(k1.z, t1.z, kn.z * n.z));
+    const R c00 = fma(r.y, m20, -(r.z * m10)), c01 = fma(r.y, m21, -(r.z * m11)), c02 = fma(r.y, m22, -(r.z * m21));
+    const R c10 = fma(r.z, m00, -(r.x * m20)), c11 = fma(r.z, m10, -(r.x * m21)), c12 = fma(r.z, m20, -(r.x * m22));
+    const R c20 = fma(r.x, m10, -(r.y * m00)), c21 = fma(r.x, m11, -(r.y * m10)), c22 = fma(r.x, m21, -(r.y * m20));
+    R *a = acc.ang.m, *l = acc.lin.m, *c = acc.cross.m;
+    a[0] = a[0] + fma(r.y, c02, -(r.z * c01));
+    a[3] = a[3] + fma(r.y, c12, -(r.z * c11));
+    a[4] = a[4] + fma(r.z, c10, -(r.x * c12));
+    a[6] = a[6] + fma(r.y, c22, -(r.z * c21));
+    a[7] = a[7] + fma(r.z, c20, -(r.x * c22));
+    a[8] = a[8] + fma(r.x, c21, -(r.y * c20));
+    l[0] = l[0] + m00;
+    l[3] = l[3] + m10;
+    l[4] = l[4] + m11;
+    l[6] = l[6] + m20;
+    l[7] = l[7] + m21;
+    l[8] = l[8] + m22;
+    const R cs[9] = {c00, c01, c02, c10, c11, c12, c20, c21, c22};
+    for (int q = 0; q < 9; q++) c[q] = c[q] + cs[q];
+    const V3<R> g = scale(V3<R>(fma(f.z, t2.x, fma(f.y, t1.x, f.x * n.x)), fma(f.z, t2.y, fma(f.y, t1.y, f.x * n.y)),
+                                fma(f.z, t2.z, fma(f.y, t1.z, f.x * n.z))),
+                          sg);
+    acc.rLin = add(acc.rLin, g);
+    acc.rAng = add(acc.rAng, V3<R>(fma(r.y, g.z, -(r.z * g.y)), fma(r.z, g.x, -(r.x * g.z)), fma(r.x, g.y, -(r.y * g.x))));
   }
 
   static void mergeInto(Acc &a, const Acc &o) {
@@ -1098,7 +1146,7 @@ template <class R> class World {
     const Manifold<R> &mf = manifolds[m];
     const WorldParams &p = P(mf.a);
     PairBody A = pairBody(mf.a), B = pairBody(mf.b);
-    M3<R> basis = orthonormal(mf.normal);
+    M3<R> basis = contactBasis(mf.normal);
     for (uint32_t c = mf.first; c < mf.first + mf.count(); c++) {
       Contact<R> &k = contacts[c];
       Eval e = evalContact(k, basis, mf.friction, A, B, alpha);

@@ -8,7 +8,8 @@ takes the FP32 mirror's pairs, contacts and colours and solves in double precisi
   so this is the FP32 rounding of the step itself.
 * Long horizon: the full runs (300 steps of the 1,000-box pile, 300 steps of a 2,000-box world with the large
   world's distributions, 240 steps of the small mix) on the GPU in FP32 and in the FP64 restatement, separately (the
-  trajectories are chaotic and part within a few steps), compared by rest-height, energy and penetration statistics.
+  trajectories are chaotic and part within a few steps), compared by rest-height, energy and penetration statistics;
+  the mean rest height is allowed the spread that FP64 runs from starts moved by 1 µm show.
 """
 import json
 import os
@@ -115,6 +116,23 @@ def _stats(avbd, world_or_oracle, scene, gpu):
     return avbd.stats.world_statistics(bodies, mf, ct, scene.up, p["gravity"])
 
 
+def _perturbed_fp64_heights(avbd, scene, steps, runs=3, size=1e-6):
+    """Mean centre heights of FP64 runs whose dynamic bodies start moved by up to `size` m (seeded): how far the
+    trajectory's own chaos spreads the statistic."""
+    out = []
+    for seed in range(runs):
+        sc = scene()
+        o = OracleWorld([sc], precision=64, color_cap=OPTS["colorCap"], color_rounds=16)
+        b = o.bodies().astype(np.float32)
+        dyn = b[:, 19] > 0
+        b[dyn, 0:3] += np.random.default_rng(100 + seed).uniform(-size, size, (int(dyn.sum()), 3)).astype(np.float32)
+        o.seed(dict(bodies=b, joints=o.joints().astype(np.float32), refs=o.refs(), colors=o.colors(),
+                    manifolds=o.manifolds(), contacts=o.contact_words(), steps=np.array([0], np.uint32)))
+        o.step(steps)
+        out.append(_stats(avbd, o, sc, False)["centre_height"]["mean"])
+    return out
+
+
 def test_long_horizon_statistics_against_fp64(avbd, oracle_threads):
     report = {}
     cases = [("1,000-box pile, 300 steps", avbd.configs.pile_1000, 300),
@@ -130,12 +148,16 @@ def test_long_horizon_statistics_against_fp64(avbd, oracle_threads):
         fp64 = OracleWorld([scene()], precision=64, color_cap=OPTS["colorCap"], color_rounds=16)
         fp64.step(steps)
         cpu = _stats(avbd, fp64, scene(), False)
-        report[name] = {"fp32_gpu": gpu, "fp64_cpu": cpu}
+        spread = max(abs(h - cpu["centre_height"]["mean"]) for h in _perturbed_fp64_heights(avbd, scene, steps))
+        report[name] = {"fp32_gpu": gpu, "fp64_cpu": cpu, "fp64_mean_height_spread_1um": spread}
     print("LONG_HORIZON " + json.dumps(report), flush=True)
     for name, r in report.items():
         g, c = r["fp32_gpu"], r["fp64_cpu"]
         assert abs(g["fallen"] - c["fallen"]) <= 0.01 * g["bodies"] + 2, name
-        assert abs(g["centre_height"]["mean"] - c["centre_height"]["mean"]) <= 0.05 * max(1.0, c["centre_height"]["mean"]), name
+        # within 5 %, or within twice what moving the FP64 run's bodies by 1 µm at the start does (the mix's capsules
+        # roll off the net or stay on it, a chaotic outcome that no precision fixes)
+        bound = max(0.05 * max(1.0, c["centre_height"]["mean"]), 2 * r["fp64_mean_height_spread_1um"])
+        assert abs(g["centre_height"]["mean"] - c["centre_height"]["mean"]) <= bound, name
         assert abs(g["potential_energy"] - c["potential_energy"]) <= 0.05 * abs(c["potential_energy"]) + 1.0, name
         # (the minimum lowest point belongs to a body sliding off the ground's edge or to a hard landing; the
         # first percentile is the resting population's)

@@ -282,7 +282,9 @@ def test_batch_matches_the_oracle_batch(avbd):
 
 
 def _small_mix(avbd):
-    return avbd.configs.jointed_mix(n=6, spheres=30, capsules=20, area=3.0, capsule_grid=(4, 1, 5))
+    s = avbd.configs.jointed_mix(n=6, spheres=30, capsules=20, area=3.0, capsule_grid=(4, 1, 5))
+    s.addSphere(0.15, position=[2.6, 0.6, -2.6])  # a loose ball, so the run always has sphere contacts
+    return s
 
 
 def test_jointed_mix_steps_stay_in_parity(avbd):
