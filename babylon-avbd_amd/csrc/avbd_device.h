@@ -232,11 +232,18 @@ AVBD_HD float wrapAngle(float x) { return x - TWO_PI_F * floorf((x + PI_F) / TWO
 struct Basis {
   f3 n, t1, t2;
 };
-AVBD_HD Basis orthonormal(f3 n) {
+/** The reciprocal of the unscaled first tangent's length (what orthonormal scales it by). */
+AVBD_HD float tangentScale(f3 n) {
+  const f3 t1 = fabsf(n.x) > fabsf(n.z) ? f3{-n.y, n.x, 0.f} : f3{0.f, -n.z, n.y};
+  return 1.f / length(t1);
+}
+/** The basis from n and its tangentScale (formed once per manifold and step by the adjacency pass). */
+AVBD_HD Basis orthonormalScaled(f3 n, float inv) {
   f3 t1 = fabsf(n.x) > fabsf(n.z) ? f3{-n.y, n.x, 0.f} : f3{0.f, -n.z, n.y};
-  t1 = scale(t1, 1.f / length(t1));
+  t1 = scale(t1, inv);
   return Basis{n, t1, cross(n, t1)};
 }
+AVBD_HD Basis orthonormal(f3 n) { return orthonormalScaled(n, tangentScale(n)); }
 
 /** lowbias32 (avbd2d/gpu/layout.ts:106-115). */
 AVBD_HD uint32_t hash32(uint32_t x) {

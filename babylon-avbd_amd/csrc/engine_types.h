@@ -167,7 +167,8 @@ constexpr uint32_t COLOR_FLAG_WORDS = 64;  // colorFlags: rounds past 61 always 
  * Adjacency entry record, written once per step after the per-body sort (2 × uint4 = 32 B):
  *   [0].x partner body (ENTRY_NONE for a world anchor), [0].y manifold: first contact / joint: joint id,
  *   [0].z flags: contact count (bits 0-3), ENTRY_IS_A, ENTRY_JOINT, ENTRY_DYNAMIC (partner is dynamic),
- *   [0].w manifold friction (f32 bits); [1].xyz manifold normal (f32 bits), [1].w constraint id.
+ *   [0].w manifold friction (f32 bits); [1].xyz manifold normal (f32 bits), [1].w the normal's tangentScale (f32
+ *   bits, formed once per step: every contact row of the pair takes its basis from it).
  * Reading it replaces the chain adjacency id → manifold/joint record → partner body in the colouring and the primal.
  */
 constexpr uint32_t ENTRY_NONE = 0xffffffffu;

@@ -445,7 +445,8 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
   const PairBody other = pairBody<DF>(w, head.x);
   const PairBody PA = isA ? self : other;
   const PairBody PB = isA ? other : self;
-  Basis basis = orthonormal(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)));
+  const Basis basis = orthonormalScaled(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)),
+                                        __uint_as_float(tail.w));
   Contact k = w.contacts[head.y + p];
   if constexpr (DF) {
     k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
@@ -515,7 +516,8 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       }
       const uint4 h = make_uint4(__shfl(head.x, owner, W), __shfl(head.y, owner, W), __shfl(head.z, owner, W),
                                  __shfl(head.w, owner, W));
-      const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W), 0u);
+      const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W),
+                                  __shfl(tail.w, owner, W));
       const uint32_t p = t - __shfl(excl, owner, W);
       if (t < total) addItem<DF>(w, acc, i, self, h, tl, p, alpha, flags, df);
     }
@@ -728,6 +730,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
                                  __shfl(head.w, owner, W));
       const f3 normal = mk3(__uint_as_float(__shfl(tail.x, owner, W)), __uint_as_float(__shfl(tail.y, owner, W)),
                             __uint_as_float(__shfl(tail.z, owner, W)));
+      const float tscale = __uint_as_float(__shfl(tail.w, owner, W));
       const uint32_t pt = t - __shfl(excl, owner, W);
       if (t < total) {
         // The partner is static or of an earlier colour: its pose of this iteration was waited for by the primal
@@ -737,8 +740,8 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
         Contact kc = w.contacts[c];
         kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
         kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
-        const Eval ev = evalContact(w, kc, orthonormal(normal), __uint_as_float(h.w), isA ? self : other,
-                                    isA ? other : self, p.alpha, p.flags);
+        const Eval ev = evalContact(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w),
+                                    isA ? self : other, isA ? other : self, p.alpha, p.flags);
         dualUpdate<true>(w, c, kc, ev, p);
       }
     }

@@ -60,7 +60,7 @@ AVBD_D void writeEntry(const DeviceWorld &w, uint32_t b, uint32_t e, uint32_t id
     int other = f.y == int(b) ? f.z : f.y;
     head = make_uint4(other < 0 ? ENTRY_NONE : uint32_t(other), id,
                       ENTRY_JOINT | (dynamicBody(w, other) ? ENTRY_DYNAMIC : 0u), 0u);
-    tail = make_uint4(0u, 0u, 0u, id);
+    tail = make_uint4(0u, 0u, 0u, 0u);
   } else {
     const Manifold mf = w.manifolds[id - w.nJoints];
     bool isA = mf.ids.x == b;
@@ -68,7 +68,8 @@ AVBD_D void writeEntry(const DeviceWorld &w, uint32_t b, uint32_t e, uint32_t id
     head = make_uint4(other, mf.ids.z,
                       (mf.ids.w & 15u) | (isA ? ENTRY_IS_A : 0u) | (dynamicBody(w, int(other)) ? ENTRY_DYNAMIC : 0u),
                       __float_as_uint(mf.geo.w));
-    tail = make_uint4(__float_as_uint(mf.geo.x), __float_as_uint(mf.geo.y), __float_as_uint(mf.geo.z), id);
+    tail = make_uint4(__float_as_uint(mf.geo.x), __float_as_uint(mf.geo.y), __float_as_uint(mf.geo.z),
+                      __float_as_uint(tangentScale(xyz(mf.geo))));
   }
   w.adjEntry[2ull * e] = head;
   w.adjEntry[2ull * e + 1] = tail;
