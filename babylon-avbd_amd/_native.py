@@ -82,6 +82,7 @@ class Options(ctypes.Structure):
         ("bvh_rebuild_interval", ctypes.c_uint32),
         ("solver", ctypes.c_uint32),
         ("coloring_rounds", ctypes.c_uint32),
+        ("world_block", ctypes.c_uint32),
     ]
 
 
@@ -185,6 +186,7 @@ SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "avbd_world_profile_step": (ctypes.c_int, [ctypes.c_void_p, f32p]),
     "avbd_world_device_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "avbd_world_solver_kind": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "avbd_world_enqueue_commands": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),
     "avbd_world_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),
     "avbd_world_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Params)]),

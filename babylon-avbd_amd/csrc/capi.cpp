@@ -101,8 +101,13 @@ EngineOptions toOptions(const AvbdOptions *o) {
     if (o->broadphase > 2) throw std::invalid_argument("broadphase must be 0 (auto), 1 (grid) or 2 (hploc)");
     e.broadphase = o->broadphase;
     if (o->bvh_rebuild_interval) e.rebuildInterval = o->bvh_rebuild_interval;
-    if (o->solver > 2) throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches) or 2 (dataflow)");
+    if (o->solver > 3)
+      throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches), 2 (dataflow) or 3 (workgroup per world)");
     e.solver = o->solver;
+    if (o->world_block != 0 && o->world_block != 128 && o->world_block != 256 && o->world_block != 512 &&
+        o->world_block != 1024)
+      throw std::invalid_argument("world_block must be 0, 128, 256, 512 or 1024");
+    e.worldBlock = o->world_block;
     if (o->coloring_rounds > 2)
       throw std::invalid_argument("coloring_rounds must be 0 (auto), 1 (one launch per round) or 2 (one block)");
     e.coloringRounds = o->coloring_rounds;
@@ -493,6 +498,13 @@ int avbd_world_device_bytes(const avbd_world *world, uint64_t *out) {
   if (!world || !world->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed");
   if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
   *out = world->engine->deviceBytes();
+  return AVBD_OK;
+}
+
+int avbd_world_solver_kind(const avbd_world *world, uint32_t *out) {
+  if (!world || !world->engine) return fail(AVBD_ERR_DESTROYED, "the world has been destroyed");
+  if (!out) return fail(AVBD_ERR_INVALID, "out is NULL");
+  *out = world->engine->solverKind();
   return AVBD_OK;
 }
 

@@ -217,12 +217,21 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   if (opt.broadphase == 2) useBvh_ = true;
   else if (opt.broadphase == 1) useBvh_ = false;
   else useBvh_ = nWorlds_ == 1 && nBodies_ >= 10000;
-  if (opt.solver > 2) throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches) or 2 (dataflow)");
+  if (opt.solver > 3)
+    throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches), 2 (dataflow) or 3 (workgroup per world)");
+  if (opt.worldBlock != 0 && opt.worldBlock != 128 && opt.worldBlock != 256 && opt.worldBlock != 512 &&
+      opt.worldBlock != 1024)
+    throw std::invalid_argument("world_block must be 0, 128, 256, 512 or 1024");
   if (opt.solver == 2 && nJoints_ == 0) {
     dfGrid_ = dataflow_grid();
     useDataflow_ = dfGrid_ > 0;
   }
   wantDataflow_ = useDataflow_;
+  // Auto: a batch of worlds of at most WORLDS_AUTO_MAX bodies each is solved one workgroup per world
+  uint32_t largestWorld = 0;
+  for (const HostWorld &hw : worlds) largestWorld = std::max<uint32_t>(largestWorld, uint32_t(hw.bodies.size()));
+  useWorlds_ = opt.solver == 3 || (opt.solver == 0 && nWorlds_ > 1 && largestWorld <= WORLDS_AUTO_MAX);
+  worldBlock_ = opt.worldBlock ? opt.worldBlock : 256;
 
   DeviceWorld &w = base_;
   w.nBodies = nBodies_;
@@ -284,6 +293,22 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.colorSlot = dallocT<uint4>(nBodies_);
   w.colorGroups = colorGroups_;
   w.colorCap = colorCap_;
+  if (useWorlds_) {
+    // Worlds in launch order: most bodies first (ties by index), so the longest solves start first
+    std::vector<uint32_t> order(nWorlds_);
+    for (uint32_t k = 0; k < nWorlds_; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return worlds[a].bodies.size() > worlds[b].bodies.size(); });
+    worldOrder_ = dallocT<uint32_t>(nWorlds_);
+    worldKeyStart_ = dallocT<uint32_t>((uint64_t(nWorlds_) + 1) * COLOR_KEYS);
+    HIP_CHECK(hipMemcpyAsync(worldOrder_, order.data(), order.size() * 4, hipMemcpyHostToDevice, stream_));
+    if (nJoints_) {
+      std::vector<uint32_t> js(firstJoint_);
+      js.push_back(nJoints_);
+      worldJointStart_ = dallocT<uint32_t>(js.size());
+      HIP_CHECK(hipMemcpyAsync(worldJointStart_, js.data(), js.size() * 4, hipMemcpyHostToDevice, stream_));
+    }
+  }
 
   auto up = [&](void *dst, const void *src, size_t bytes) {
     if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
@@ -506,15 +531,18 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce)
   launch_adjacency(w, pairCap_, s);
   launch_coloring(w, colorRounds_, useDataflow_ ? iterations_ : 0u, listedRounds_, s);
   launch_warm_start(w, s);
-  if (useDataflow_) {
+  if (useWorlds_) {
+    launch_solve_worlds(w, worldOrder_, worldKeyStart_, worldJointStart_, iterations_, worldBlock_, s);  // + velocities
+  } else if (useDataflow_) {
     launch_dataflow(w, dfGrid_, s);
+    launch_velocities(w, s);
   } else {
     for (uint32_t it = 0; it < iterations_; it++) {
       for (uint32_t c = 0; c < colorCap_; c++) launch_primal(w, c, maxDynamicBodies_, s);
       launch_dual(w, pairCap_, s);
     }
+    launch_velocities(w, s);
   }
-  launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);
   if (sleeping_) launch_sleep_after(w, sleepParams(sleepForce), s);
   if (cw_.table) launch_contact_events(w, cw_, &w == &views_[1] ? 1u : 0u, s);
@@ -642,7 +670,12 @@ PhaseTimes Engine::profileStep() {
   launch_warm_start(w, s);
   HIP_CHECK(hipEventRecord(ev[4], s));
   uint32_t k = 6;
-  if (useDataflow_) {
+  if (useWorlds_) {
+    // One launch for every iteration and the velocities: its time is reported as the primal's
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+    launch_solve_worlds(w, worldOrder_, worldKeyStart_, worldJointStart_, iterations_, worldBlock_, s);
+    HIP_CHECK(hipEventRecord(ev[k++], s));
+  } else if (useDataflow_) {
     // One launch for every iteration: its time is reported as the primal's, the dual's is in it
     HIP_CHECK(hipEventRecord(ev[k++], s));
     launch_dataflow(w, dfGrid_, s);
@@ -659,7 +692,7 @@ PhaseTimes Engine::profileStep() {
       HIP_CHECK(hipEventRecord(ev[k++], s));
     }
   }
-  launch_velocities(w, s);
+  if (!useWorlds_) launch_velocities(w, s);
   if (bouncing_) launch_restitution_after(w, s);  // timed with the velocities
   if (sleeping_) launch_sleep_after(w, sleepParams(force), s);
   if (cw_.table) launch_contact_events(w, cw_, &w == &views_[1] ? 1u : 0u, s);
@@ -682,7 +715,7 @@ PhaseTimes Engine::profileStep() {
   p.coloring = ms(2, 3);
   p.warmStart = ms(3, 4);
   k = 6;
-  if (useDataflow_) {
+  if (useDataflow_ || useWorlds_) {
     p.primal = ms(6, 7);
     p.velocities = ms(7, 5);
     p.primalLaunches = 1;

@@ -84,6 +84,15 @@ uint32_t dataflow_grid();
  */
 void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s);
 
+/**
+ * A batch's solve with one workgroup of `block` threads (128, 256, 512 or 1024) per world: every iteration's colours,
+ * the contact duals (each fused into the pair's later colour), the joint duals and the velocities, after a
+ * launch_coloring. `order` lists the worlds in launch order, `keyStart` holds (worlds + 1) × COLOR_KEYS words (written
+ * here), `jointStart` (worlds + 1 words: each world's first joint slot) is null when the batch has no joint.
+ */
+void launch_solve_worlds(const DeviceWorld &w, const uint32_t *order, uint32_t *keyStart, const uint32_t *jointStart,
+                         uint32_t iterations, uint32_t block, hipStream_t s);
+
 // Edits between steps
 /** Body commands (3 uint4 each) grouped by body: ranges[r] = (first command, count), each body's in submission
  * order; teleported bodies get marks[body] = 1. */

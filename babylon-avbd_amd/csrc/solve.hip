@@ -22,6 +22,13 @@ struct DfCtx {
   uint32_t it, color, base;
 };
 
+/**
+ * How a primal pass runs: one launch per colour (plain memory; the dual is a pass of its own), the dataflow solve
+ * (device-scope memory and stamps; each pair's dual fused into its later colour's primal), or one workgroup per world
+ * (plain memory, duals fused the same way, the world's colours separated by workgroup barriers).
+ */
+enum : int { SOLVE_COLORS = 0, SOLVE_DATAFLOW = 1, SOLVE_WORLDS = 2 };
+
 AVBD_D uint32_t ldAgent(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 AVBD_D void stAgent(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 // Poses and contact multipliers move as whole 16-byte records: raw buffer loads and stores with the device-scope
@@ -362,7 +369,8 @@ struct PairBody {
   uint32_t index;
 };
 /** The angular displacement qsub(rot, x- rotation) from the warm start's inverse of the x- rotation (same bits). */
-template <bool DF = false> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
+template <int M = SOLVE_COLORS> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
+  constexpr bool DF = M == SOLVE_DATAFLOW;
   const Body &B = w.bodies[i];
   PairBody pb;
   const float4 pos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
@@ -421,13 +429,14 @@ AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &bas
 AVBD_D bool finishesPair(uint32_t ca, uint32_t cb) { return ca > cb; }
 
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
-template <bool DF>
+template <int M>
 AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
                     uint32_t p, float alpha, uint32_t flags, const DfCtx &df) {
   if (head.z & ENTRY_JOINT) {
     addJoint(w, acc, head.y, alpha, i);  // the dataflow solve runs only for batches without joints
     return;
   }
+  constexpr bool DF = M == SOLVE_DATAFLOW;
   if constexpr (DF) {
     // The partner's pose: this iteration's if its colour comes first, else the last iteration's. The point's
     // multipliers come from the last iteration's dual, which the pair's finisher ran right after its primal: this
@@ -442,7 +451,7 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     }
   }
   const bool isA = (head.z & ENTRY_IS_A) != 0;
-  const PairBody other = pairBody<DF>(w, head.x);
+  const PairBody other = pairBody<M>(w, head.x);
   const PairBody PA = isA ? self : other;
   const PairBody PB = isA ? other : self;
   const Basis basis = orthonormalScaled(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)),
@@ -464,9 +473,10 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
  * group's prefix sums and takes the record from the lane that loaded it. A manifold with many points is thus spread
  * over several lanes. The partial systems are then summed by a fixed tree, so the result does not depend on scheduling.
  */
-template <uint32_t W, bool DF>
+template <uint32_t W, int M>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
                        float dt, uint32_t flags, Acc &acc, const DfCtx &df) {
+  constexpr bool DF = M == SOLVE_DATAFLOW;
   const Body &B = w.bodies[i];
   if constexpr (DF) {
     if (df.it) waitFor(w, &w.dualDone[i], df.base + df.it);  // the body's own last iteration and its duals
@@ -489,7 +499,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
     acc.rAng = mul3(I, qsub(brot, B.inertialRot));
   }
   if (first >= end) return;  // the same for every lane of the group
-  const PairBody self = pairBody<DF>(w, i);
+  const PairBody self = pairBody<M>(w, i);
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
     uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
@@ -519,7 +529,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W),
                                   __shfl(tail.w, owner, W));
       const uint32_t p = t - __shfl(excl, owner, W);
-      if (t < total) addItem<DF>(w, acc, i, self, h, tl, p, alpha, flags, df);
+      if (t < total) addItem<M>(w, acc, i, self, h, tl, p, alpha, flags, df);
     }
   }
 }
@@ -626,7 +636,7 @@ AVBD_D void ldlt6Double(const float (&A)[21], const float (&b)[6], float (&x)[6]
  * first; a near-singular system (a pivot at its floor, as stiff joint chains give) is solved again in FP64 from the
  * same FP32 entries, with the same floor.
  */
-template <bool DF>
+template <int M>
 AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df, float4 *newPos = nullptr,
                        float4 *newRot = nullptr) {
   const float A[21] = {s.l00, s.l10, s.l11, s.l20, s.l21, s.l22, s.c[0], s.c[1], s.c[2], s.a00, s.c[3],
@@ -640,7 +650,7 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfC
     atomicAdd(&w.counters[C_PIVOT64], 1u);
   }
   Body &B = w.bodies[i];
-  if constexpr (DF) {
+  if constexpr (M == SOLVE_DATAFLOW) {
     const float4 pos = ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS));
     *newPos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
     *newRot = qadd(ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)), mk3(-x[3], -x[4], -x[5]));
@@ -650,8 +660,14 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfC
     stAgent(&w.bodyDone[i], df.base + df.it + 1u);
   } else {
     float4 pos = B.pos;
-    B.pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
-    B.rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
+    pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
+    const float4 rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
+    B.pos = pos;
+    B.rot = rot;
+    if constexpr (M == SOLVE_WORLDS) {
+      *newPos = pos;
+      *newRot = rot;
+    }
   }
 }
 
@@ -659,8 +675,8 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfC
  * The dual update of contact slot c (dual, wgsl-solve.ts:474-535) from its record kc and its evaluation at this
  * iteration's poses: the penalty ramp where the row is active, λ = the clamped force, and the stick flag.
  */
-template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, const Contact &kc, const Eval &e,
-                                          const WorldParams &p) {
+template <int M> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, const Contact &kc, const Eval &e,
+                                         const WorldParams &p) {
   f3 pen = xyz(kc.pen);
   if (e.F.x < 0.f) pen.x = rmin(pen.x + p.betaLin * fabsf(e.C.x), PENALTY_MAX);
   uint32_t key = fbits(kc.rA.w);
@@ -671,7 +687,7 @@ template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, cons
     bool stick = sqrtf(e.C.y * e.C.y + e.C.z * e.C.z) < STICK_THRESH;
     key = (key & ~STICK_BIT) | (stick ? STICK_BIT : 0u);
   }
-  if constexpr (DF) {
+  if constexpr (M == SOLVE_DATAFLOW) {
     if (ramp) __builtin_amdgcn_raw_buffer_store_b32(key, contactRsrc(w), contactOff(c, OFF_KEY), 0, BUF_SC1);
     stDev4(contactRsrc(w), contactOff(c, OFF_PEN), mk4(pen, kc.pen.w));
     stDev4(contactRsrc(w), contactOff(c, OFF_LAM), mk4(e.F, kc.lam.w));
@@ -689,8 +705,9 @@ template <bool DF> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, cons
  * for by the primal), and updated as the dual pass does. The items are spread over the group's lanes as in
  * accumulate. Then the body's dual stamp.
  */
-template <uint32_t W>
+template <uint32_t W, int M>
 AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 pos, float4 rot, const DfCtx &df) {
+  constexpr bool DF = M == SOLVE_DATAFLOW;
   const uint32_t i = slot.x, first = slot.y, end = slot.z;
   const WorldParams &p = w.worlds[slot.w];
   const Body &B = w.bodies[i];
@@ -735,47 +752,51 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
       if (t < total) {
         // The partner is static or of an earlier colour: its pose of this iteration was waited for by the primal
         const bool isA = (h.z & ENTRY_IS_A) != 0;
-        const PairBody other = pairBody<true>(w, h.x);
+        const PairBody other = pairBody<M>(w, h.x);
         const uint32_t c = h.y + pt;
         Contact kc = w.contacts[c];
-        kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
-        kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
+        if constexpr (DF) {
+          kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
+          kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
+        }
         const Eval ev = evalContact(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w),
                                     isA ? self : other, isA ? other : self, p.alpha, p.flags);
-        dualUpdate<true>(w, c, kc, ev, p);
+        dualUpdate<M>(w, c, kc, ev, p);
       }
     }
   }
-  storesDone();
-  if (lane == 0) stAgent(&w.dualDone[i], df.base + df.it + 1u);
+  if constexpr (DF) {
+    storesDone();
+    if (lane == 0) stAgent(&w.dualDone[i], df.base + df.it + 1u);
+  }
 }
 
 constexpr uint32_t PRIMAL_BLOCK = 256;
 constexpr uint32_t WAVE = DF_WAVE;
 
 /** Bucket position k (of a colour half ending at `end`) solved by a group of W lanes; `lane` is the lane in the group. */
-template <uint32_t W, bool DF = false>
+template <uint32_t W, int M = SOLVE_COLORS>
 AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t lane, const DfCtx &df = DfCtx{}) {
   const bool live = k < end;
   const uint4 slot = live ? w.colorSlot[k] : make_uint4(0u, 0u, 0u, 0u);
   Acc acc;
   if (live) {
     const WorldParams &p = w.worlds[slot.w];
-    accumulate<W, DF>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df);
+    accumulate<W, M>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df);
   } else {
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
   }
   mergeTree<W>(acc);
-  if constexpr (DF) {
+  if constexpr (M != SOLVE_COLORS) {
     float4 pos = make_float4(0.f, 0.f, 0.f, 0.f), rot = pos;
-    if (live && lane == 0) finishBody<true>(w, slot.x, acc, df, &pos, &rot);
+    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df, &pos, &rot);
     // The new pose from the group's lane 0 to all of its lanes
     pos = make_float4(__shfl(pos.x, 0, W), __shfl(pos.y, 0, W), __shfl(pos.z, 0, W), __shfl(pos.w, 0, W));
     rot = make_float4(__shfl(rot.x, 0, W), __shfl(rot.y, 0, W), __shfl(rot.z, 0, W), __shfl(rot.w, 0, W));
-    if (live) fusedDual<W>(w, slot, lane, pos, rot, df);
+    if (live) fusedDual<W, M>(w, slot, lane, pos, rot, df);
   } else {
-    if (live && lane == 0) finishBody<false>(w, slot.x, acc, df);
+    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df);
   }
 }
 
@@ -855,10 +876,10 @@ AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   const Manifold mf = w.manifolds[m];
   const uint32_t c = mf.ids.z + k;
   const WorldParams &p = w.worlds[w.bodyWorld[mf.ids.x]];
-  PairBody PA = pairBody(w, mf.ids.x), PB = pairBody(w, mf.ids.y);
+  PairBody PA = pairBody<SOLVE_COLORS>(w, mf.ids.x), PB = pairBody<SOLVE_COLORS>(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
   const Contact kc = w.contacts[c];
-  dualUpdate<false>(w, c, kc, evalContact(w, kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
+  dualUpdate<SOLVE_COLORS>(w, c, kc, evalContact(w, kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */
@@ -876,9 +897,8 @@ __global__ void k_dual(DeviceWorld w) {
   }
 }
 
-__global__ void k_velocities(DeviceWorld w) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nBodies || held(w)) return;
+/** BDF1 velocities of body i from its step-start pose (updateVelocities, wgsl-solve.ts:537-549). */
+AVBD_D void bodyVelocity(const DeviceWorld &w, uint32_t i) {
   const WorldParams &p = w.worlds[w.bodyWorld[i]];
   Body &B = w.bodies[i];
   f3 up = mk3(p.up[0], p.up[1], p.up[2]);
@@ -889,6 +909,12 @@ __global__ void k_velocities(DeviceWorld w) {
   } else {
     B.vel.w = prevUp;
   }
+}
+
+__global__ void k_velocities(DeviceWorld w) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nBodies || held(w)) return;
+  bodyVelocity(w, i);
 }
 
 // --- dataflow solve: every iteration's colours and duals in one launch -------------------------------------------------
@@ -922,11 +948,78 @@ __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld 
                    lightEnd = w.colorStart[2 * c + 2];
     const uint32_t hw = plan.heavyWaves[c];
     if (v < hw)
-      primalGroup<HEAVY_LANES, true>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
+      primalGroup<HEAVY_LANES, SOLVE_DATAFLOW>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
                                      lane % HEAVY_LANES, df);
     else
-      primalGroup<LANES, true>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
+      primalGroup<LANES, SOLVE_DATAFLOW>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
   }
+}
+
+// --- one workgroup per world: a batch's every iteration, colour and dual in one launch ------------------------------
+
+/**
+ * Where each world's bodies sit in each colour bucket: keyStart[world × COLOR_KEYS + key] is the first position of
+ * bucket `key` (colour key / 2, heavy or light half) holding a body of `world` or of a later world. A bucket lists its
+ * bodies in index order (the scatter ranks them by index within a chunk, chunks in order) and a world owns a contiguous
+ * index range, so a world's bodies of a bucket form one run, found by a binary search on the slots' world words.
+ */
+__global__ void k_world_ranges(DeviceWorld w, uint32_t *keyStart) {
+  const uint32_t keys = 2u * w.colorCap;
+  const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= uint64_t(w.nWorlds + 1u) * keys) return;
+  const uint32_t world = uint32_t(id / keys), key = uint32_t(id % keys);
+  uint32_t lo = w.colorStart[key], hi = w.colorStart[key + 1];
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (w.colorSlot[mid].w < world) lo = mid + 1;
+    else hi = mid;
+  }
+  keyStart[size_t(world) * COLOR_KEYS + key] = lo;
+}
+
+/**
+ * The solve of a batch of small worlds, one workgroup per world (order[blockIdx] names it; the host lists the largest
+ * worlds first so they start first). The workgroup runs the world's iterations; in each, its colours in order, a
+ * colour's heavy bodies two to a wave and light ones eight to a wave as k_primal does, each body running the dual of
+ * the pairs it finishes right after its primal (as in the dataflow solve), then a workgroup barrier, which also makes
+ * the colour's poses and multipliers visible to the next; colours without a body of the world are skipped. The joints'
+ * duals follow the iteration's last colour. At the end, the world's velocities. The same arithmetic in the same order
+ * as the colour launches and the dual pass, so the same bits; nothing crosses a workgroup, so there are no waits.
+ */
+template <uint32_t BS>
+__global__ void __launch_bounds__(BS, 4)
+    k_solve_worlds(DeviceWorld w, const uint32_t *order, const uint32_t *keyStart, const uint32_t *jointStart,
+                   uint32_t iterations) {
+  if (held(w)) return;
+  const uint32_t world = order[blockIdx.x];
+  const uint32_t *lo = keyStart + size_t(world) * COLOR_KEYS, *hi = lo + COLOR_KEYS;
+  const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+  constexpr uint32_t waves = BS / WAVE, heavyPerWave = WAVE / HEAVY_LANES, lightPerWave = WAVE / LANES;
+  const uint32_t firstJoint = jointStart ? jointStart[world] : 0u, endJoint = jointStart ? jointStart[world + 1] : 0u;
+  DfCtx ctx{0u, 0u, 0u};
+  for (uint32_t it = 0; it < iterations; it++) {
+    for (uint32_t c = 0; c < w.colorCap; c++) {
+      const uint32_t hs = lo[2 * c], he = hi[2 * c], ls = lo[2 * c + 1], le = hi[2 * c + 1];
+      if (hs == he && ls == le) continue;  // the same for the whole workgroup
+      ctx.color = c;
+      const uint32_t hw = (he - hs + heavyPerWave - 1) / heavyPerWave;
+      const uint32_t units = hw + (le - ls + lightPerWave - 1) / lightPerWave;
+      for (uint32_t u = wave; u < units; u += waves) {  // uniform across a wave
+        if (u < hw)
+          primalGroup<HEAVY_LANES, SOLVE_WORLDS>(w, hs + u * heavyPerWave + lane / HEAVY_LANES, he, lane % HEAVY_LANES,
+                                                 ctx);
+        else
+          primalGroup<LANES, SOLVE_WORLDS>(w, ls + (u - hw) * lightPerWave + lane / LANES, le, lane % LANES, ctx);
+      }
+      __syncthreads();
+    }
+    if (endJoint > firstJoint) {
+      for (uint32_t j = firstJoint + threadIdx.x; j < endJoint; j += BS) dualJoint(w, j);
+      __syncthreads();
+    }
+  }
+  const WorldParams &p = w.worlds[world];
+  for (uint32_t i = p.firstBody + threadIdx.x; i < p.firstBody + p.bodyCount; i += BS) bodyVelocity(w, i);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -960,6 +1053,16 @@ uint32_t dataflow_grid() {
 }
 void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s) {
   k_solve_dataflow<<<grid, PRIMAL_BLOCK, 0, s>>>(w);
+}
+void launch_solve_worlds(const DeviceWorld &w, const uint32_t *order, uint32_t *keyStart, const uint32_t *jointStart,
+                         uint32_t iterations, uint32_t block, hipStream_t s) {
+  k_world_ranges<<<blocks(uint64_t(w.nWorlds + 1u) * 2u * w.colorCap, 256), 256, 0, s>>>(w, keyStart);
+  switch (block) {
+    case 128: k_solve_worlds<128><<<w.nWorlds, 128, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    case 512: k_solve_worlds<512><<<w.nWorlds, 512, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    case 1024: k_solve_worlds<1024><<<w.nWorlds, 1024, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    default: k_solve_worlds<256><<<w.nWorlds, 256, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+  }
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {
   k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);

@@ -131,8 +131,9 @@ napi_value DeviceCount(napi_env env, napi_callback_info) {
  * createWorld(scenes, options): scenes = [{bodies: Float32Array, joints: Float32Array, jointInfo: Int32Array,
  * ignore: Uint32Array, filters: Uint32Array | null, dt, gravity, up: [3], alpha, gamma, betaLin, betaAng, iterations,
  * flags}]; options = {colorCap, colorRounds, pairCapacity, graph, device, adaptive, broadphase (0 auto, 1 grid,
- * 2 hploc), bvhRebuildInterval, solver (0 auto, 1 colour launches, 2 dataflow), coloringRounds (0 auto, 1 launches,
- * 2 listed)}. Returns the world handle (avbd_batch_create).
+ * 2 hploc), bvhRebuildInterval, solver (0 auto, 1 colour launches, 2 dataflow, 3 workgroup per world), coloringRounds
+ * (0 auto, 1 launches, 2 listed), worldBlock (threads per world's workgroup: 128, 256, 512, 1024; 0: 256)}. Returns
+ * the world handle (avbd_batch_create).
  */
 napi_value CreateWorld(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -200,6 +201,7 @@ napi_value CreateWorld(napi_env env, napi_callback_info info) {
   o.bvh_rebuild_interval = uint32_t(numberProp(env, argv[1], "bvhRebuildInterval", 0));
   o.solver = uint32_t(numberProp(env, argv[1], "solver", 0));
   o.coloring_rounds = uint32_t(numberProp(env, argv[1], "coloringRounds", 0));
+  o.world_block = uint32_t(numberProp(env, argv[1], "worldBlock", 0));
   avbd_world *w = nullptr;
   AVBD_OK_OR_THROW(avbd_batch_create(descs.data(), n, &o, &w));
   napi_value handle;

@@ -508,6 +508,7 @@ function nativeOptions(options) {
     broadphase: code,
     bvhRebuildInterval: (o.bvh && o.bvh.rebuildInterval) || 0,
     solver: solverCode(o.solver),
+    worldBlock: o.worldBlock || 0,
   };
 }
 
@@ -573,7 +574,7 @@ class CommandQueue {
 }
 
 function solverCode(solver) {
-  const code = { auto: 0, colors: 1, dataflow: 2 }[solver === undefined ? 'auto' : solver];
+  const code = { auto: 0, colors: 1, dataflow: 2, worlds: 3 }[solver === undefined ? 'auto' : solver];
   if (code === undefined) throw Error(`Unknown solver: ${solver}`);
   return code;
 }

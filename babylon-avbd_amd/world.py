@@ -58,9 +58,13 @@ def _options(options):
     o.broadphase = {"auto": 0, "grid": 1, "hploc": 2}[bp]
     o.bvh_rebuild_interval = int((options.get("bvh") or {}).get("rebuildInterval", 0))
     solver = options.get("solver", "auto")
-    if solver not in ("auto", "colors", "dataflow"):
+    if solver not in ("auto", "colors", "dataflow", "worlds"):
         raise ValueError(f"Unknown solver: {solver}")
-    o.solver = {"auto": 0, "colors": 1, "dataflow": 2}[solver]
+    o.solver = {"auto": 0, "colors": 1, "dataflow": 2, "worlds": 3}[solver]
+    block = int(options.get("worldBlock", 0))
+    if block not in (0, 128, 256, 512, 1024):
+        raise ValueError(f"worldBlock must be 128, 256, 512 or 1024, not {block}")
+    o.world_block = block
     rounds = options.get("coloringRounds", "auto")
     if rounds not in ("auto", "launches", "listed"):
         raise ValueError(f"Unknown coloringRounds: {rounds}")
@@ -781,6 +785,12 @@ class HipWorld:
     def deviceBytes(self):
         v = ctypes.c_uint64()
         check(self._lib.avbd_world_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def solverKind(self):
+        """Which solve the steps run: 1 colour launches, 2 dataflow, 3 one workgroup per world."""
+        v = ctypes.c_uint32()
+        check(self._lib.avbd_world_solver_kind(self._h, ctypes.byref(v)))
         return v.value
 
     def gpuIndex(self, i):

@@ -88,12 +88,16 @@ typedef struct AvbdOptions {
   uint32_t broadphase;      /* 0 auto (H-PLOC for one world of >= 10,000 bodies), 1 uniform grid, 2 H-PLOC BVH
                                (AppGpuSolver3D broadphase "auto" | "grid" | "hploc", appGpuSolver3D.js:333-402) */
   uint32_t bvh_rebuild_interval; /* H-PLOC rebuild period in steps, refit in between (0: 64; gpuHploc.js:62-80) */
-  uint32_t solver;          /* 0 auto (colour launches), 1 one launch per colour and a dual launch per iteration,
-                               2 dataflow: every iteration's colours and duals in one launch, each body waiting only
-                               for its neighbours (same result; batches without joints; needs the device to itself) */
+  uint32_t solver;          /* 0 auto (a batch of worlds of at most 256 bodies each: 3; otherwise 1), 1 one launch
+                               per colour and a dual launch per iteration, 2 dataflow: every iteration's colours and
+                               duals in one launch, each body waiting only for its neighbours (batches without
+                               joints; needs the device to itself), 3 one workgroup per world: each world's
+                               iterations, colours, duals and velocities in one workgroup, every world of the batch in
+                               one launch (the same result from each) */
   uint32_t coloring_rounds; /* 0 auto (one block over the pending bodies while few are left pending, with `adaptive`),
                                1 one launch per Jones–Plassmann round, 2 always one block over the pending bodies
                                (the same colours either way) */
+  uint32_t world_block;     /* threads per world's workgroup with solver 3: 128, 256, 512 or 1024 (0: 256) */
 } AvbdOptions;
 
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */
@@ -198,6 +202,10 @@ int avbd_world_profile_step(avbd_world *world, float *out);
 
 /** Device bytes held by the world. */
 int avbd_world_device_bytes(const avbd_world *world, uint64_t *out);
+
+/** Which solve the world's steps run (this library's choice of kernels; the reference has one): 1 colour launches,
+ * 2 dataflow, 3 one workgroup per world. */
+int avbd_world_solver_kind(const avbd_world *world, uint32_t *out);
 
 /* ---- Edits between steps ---------------------------------------------------------------------------------------- */
 

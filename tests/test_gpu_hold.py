@@ -2,7 +2,7 @@
 
 A step whose pair storage overflows or whose colouring clashes is held: poses, velocities and joints keep their bits,
 the counters report the failure, and adapt() recovers (the reference's "H-PLOC overflow/clashes holds motion and
-recovers", tests/gpu-hploc.js:234-286, on the grid, H-PLOC and dataflow paths). Then the adapt path against the oracle:
+recovers", tests/gpu-hploc.js:234-286, on the grid, H-PLOC, dataflow and workgroup-per-world paths). Then the adapt path against the oracle:
 a pile stepped with a synchronous adapt() after every step, starting from a colour cap, round count and pair capacity
 too small for it, so that it clashes, overflows, grows the cap, grows storage and shrinks the cap again (recolouring the
 bodies above the new cap); every step is compared with the oracle run at the same cap, rounds and capacity from the
@@ -39,6 +39,7 @@ PATHS = [
     ("grid", dict(broadphase="grid"), True),
     ("hploc", dict(broadphase="hploc"), True),
     ("dataflow", dict(broadphase="hploc", solver="dataflow"), False),  # the dataflow solve takes worlds without joints
+    ("worlds", dict(broadphase="grid", solver="worlds"), True),
 ]
 
 
