@@ -265,6 +265,28 @@ AVBD_D void addRow(Acc &s, f3 l, f3 a, float k, float f) {
   s.rAng = mk3(fmaf(a.x, f, s.rAng.x), fmaf(a.y, f, s.rAng.y), fmaf(a.z, f, s.rAng.z));
 }
 
+// --- the contact rows' arithmetic as fused multiply-adds -----------------------------------------------------------
+// Every sum of products below is one chain of fused multiply-adds (one rounding per step, about half the VALU
+// instructions of separate multiplies and adds); the CPU restatement forms each with std::fma in the same order.
+
+AVBD_D float dotFused(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+AVBD_D f3 crossFused(f3 a, f3 b) {
+  return mk3(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
+/** q v q* for a unit quaternion (layout.ts qrotate): v + 2 (w u + q⃗ × u) with u = q⃗ × v. */
+AVBD_D f3 rotFused(float4 q, f3 v) {
+  const f3 qv = xyz(q);
+  const f3 u = crossFused(qv, v), t = crossFused(qv, u);
+  return mk3(fmaf(2.f, fmaf(q.w, u.x, t.x), v.x), fmaf(2.f, fmaf(q.w, u.y, t.y), v.y),
+             fmaf(2.f, fmaf(q.w, u.z, t.z), v.z));
+}
+/** qsub(a, b) = 2 (a b⁻¹)⃗ (layout.ts) from bi = b⁻¹, each component of the product one chain. */
+AVBD_D f3 dispFused(float4 a, float4 bi) {
+  return mk3(2.f * fmaf(a.w, bi.x, fmaf(a.x, bi.w, fmaf(a.y, bi.z, -(a.z * bi.y)))),
+             2.f * fmaf(a.w, bi.y, fmaf(a.y, bi.w, fmaf(a.z, bi.x, -(a.x * bi.z)))),
+             2.f * fmaf(a.w, bi.z, fmaf(a.z, bi.w, fmaf(a.x, bi.y, -(a.y * bi.x)))));
+}
+
 /**
  * The three rows of one contact point (normal and tangents b_q of the basis, signed by the body's side sg, lever arm r,
  * penalties k, forces f) added at once: with M = Σ k_q b_q b_qᵀ (the sign cancels) and g = sg Σ f_q b_q, the linear
@@ -283,13 +305,13 @@ AVBD_D void addContactRows(Acc &s, const Basis &b, float sg, f3 r, f3 k, f3 f) {
   const float c00 = fmaf(r.y, m20, -(r.z * m10)), c01 = fmaf(r.y, m21, -(r.z * m11)), c02 = fmaf(r.y, m22, -(r.z * m21));
   const float c10 = fmaf(r.z, m00, -(r.x * m20)), c11 = fmaf(r.z, m10, -(r.x * m21)), c12 = fmaf(r.z, m20, -(r.x * m22));
   const float c20 = fmaf(r.x, m10, -(r.y * m00)), c21 = fmaf(r.x, m11, -(r.y * m10)), c22 = fmaf(r.x, m21, -(r.y * m20));
-  // Angular block C [r]×ᵀ, lower triangle
-  s.a00 += fmaf(r.y, c02, -(r.z * c01));
-  s.a10 += fmaf(r.y, c12, -(r.z * c11));
-  s.a11 += fmaf(r.z, c10, -(r.x * c12));
-  s.a20 += fmaf(r.y, c22, -(r.z * c21));
-  s.a21 += fmaf(r.z, c20, -(r.x * c22));
-  s.a22 += fmaf(r.x, c21, -(r.y * c20));
+  // Angular block C [r]×ᵀ, lower triangle, accumulated by the same chains
+  s.a00 = fmaf(r.y, c02, fmaf(-r.z, c01, s.a00));
+  s.a10 = fmaf(r.y, c12, fmaf(-r.z, c11, s.a10));
+  s.a11 = fmaf(r.z, c10, fmaf(-r.x, c12, s.a11));
+  s.a20 = fmaf(r.y, c22, fmaf(-r.z, c21, s.a20));
+  s.a21 = fmaf(r.z, c20, fmaf(-r.x, c22, s.a21));
+  s.a22 = fmaf(r.x, c21, fmaf(-r.y, c20, s.a22));
   s.l00 += m00;
   s.l10 += m10;
   s.l11 += m11;
@@ -309,7 +331,8 @@ AVBD_D void addContactRows(Acc &s, const Basis &b, float sg, f3 r, f3 k, f3 f) {
                          fmaf(f.z, b.t2.z, fmaf(f.y, b.t1.z, f.x * b.n.z))),
                      sg);
   s.rLin = add(s.rLin, g);
-  s.rAng = add(s.rAng, mk3(fmaf(r.y, g.z, -(r.z * g.y)), fmaf(r.z, g.x, -(r.x * g.z)), fmaf(r.x, g.y, -(r.y * g.x))));
+  s.rAng = mk3(fmaf(r.y, g.z, fmaf(-r.z, g.y, s.rAng.x)), fmaf(r.z, g.x, fmaf(-r.x, g.z, s.rAng.y)),
+               fmaf(r.x, g.y, fmaf(-r.y, g.x, s.rAng.z)));
 }
 
 AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, uint32_t i) {
@@ -368,7 +391,7 @@ struct PairBody {
   f3 dLin, dAng;
   uint32_t index;
 };
-/** The angular displacement qsub(rot, x- rotation) from the warm start's inverse of the x- rotation (same bits). */
+/** The angular displacement qsub(rot, x- rotation), from the warm start's inverse of the x- rotation. */
 template <int M = SOLVE_COLORS> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
   constexpr bool DF = M == SOLVE_DATAFLOW;
   const Body &B = w.bodies[i];
@@ -376,24 +399,27 @@ template <int M = SOLVE_COLORS> AVBD_D PairBody pairBody(const DeviceWorld &w, u
   const float4 pos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
   pb.rot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
   pb.dLin = sub(xyz(pos), xyz(B.ipos));
-  pb.dAng = qsubInv(pb.rot, w.irotInv[i]);
+  pb.dAng = dispFused(pb.rot, w.irotInv[i]);
   pb.index = i;
   return pb;
 }
 
 struct Eval {
-  f3 rAW, rBW, C, F;
+  f3 rSelf, C, F;  // the evaluating body's lever arm, the rows' C and forces
   float frictionScale, bounds;
 };
 
 /**
- * evalContact (wgsl-solve.ts:171-206). The rows' C = C0 (1 - α) + J Δx take the displacement once as the relative
- * motion of the two anchors, Δ = (Δx_A − Δx_B) + Δθ_A × r_A − Δθ_B × r_B, and each row as C0 (1 − α) + l · Δ (the same
- * value as the reference's per-row sum l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B, rounded differently; the
- * CPU restatement forms it the same way).
+ * evalContact (wgsl-solve.ts:171-206) from either body's side: `self` is the body whose rows are wanted (selfIsA: it is
+ * the contact's body A), `other` its partner. The rows' C = C0 (1 − α) + J Δx take the displacement once as the
+ * relative motion of the two anchors, D = (Δx_self − Δx_other) + (Δθ_self × r_self − Δθ_other × r_other), each row as
+ * C0 (1 − α) + sg (l · D) with sg = +1 for A. Exchanging the bodies negates D exactly (a − b = −(b − a) in IEEE
+ * rounding), so both sides compute the same C and forces bit for bit, as the dual does from A's side. (The reference
+ * sums l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B per row: the same value, rounded differently; the CPU
+ * restatement forms it as here.)
  */
-AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction, const PairBody &A,
-                        const PairBody &B, float alpha, uint32_t flags) {
+AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction,
+                        const PairBody &self, const PairBody &other, bool selfIsA, float alpha, uint32_t flags) {
   uint32_t key = fbits(k.rA.w);
   // Sphere contacts take their lever arms at the step-start rotation (wgsl-solve.ts:172-181): a rolling sphere would
   // otherwise see a false separation in its normal row. Capsule contacts (keys 4<<24 | j) do so only with
@@ -402,16 +428,16 @@ AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &bas
   const bool rounded = feature == SPHERE_FEATURE ||
                        ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (feature >> 24) == (CAPSULE_FEATURE >> 24));
   Eval e;
-  e.rAW = qrotate(rounded ? w.bodies[A.index].irot : A.rot, xyz(k.rA));
-  e.rBW = qrotate(rounded ? w.bodies[B.index].irot : B.rot, xyz(k.rB));
-  float keep = 1.f - alpha;
-  const f3 delta = sub(add(sub(A.dLin, B.dLin), cross(A.dAng, e.rAW)), cross(B.dAng, e.rBW));
-  e.C = mk3(k.rB.w * keep + dot(basis.n, delta), k.pen.w * keep + dot(basis.t1, delta),
-            k.lam.w * keep + dot(basis.t2, delta));
-  f3 F = add(mul3(xyz(k.pen), e.C), xyz(k.lam));
+  e.rSelf = rotFused(rounded ? w.bodies[self.index].irot : self.rot, selfIsA ? xyz(k.rA) : xyz(k.rB));
+  const f3 rOther = rotFused(rounded ? w.bodies[other.index].irot : other.rot, selfIsA ? xyz(k.rB) : xyz(k.rA));
+  const f3 D = add(sub(self.dLin, other.dLin), sub(crossFused(self.dAng, e.rSelf), crossFused(other.dAng, rOther)));
+  const float sg = selfIsA ? 1.f : -1.f, keep = 1.f - alpha;
+  e.C = mk3(fmaf(k.rB.w, keep, sg * dotFused(basis.n, D)), fmaf(k.pen.w, keep, sg * dotFused(basis.t1, D)),
+            fmaf(k.lam.w, keep, sg * dotFused(basis.t2, D)));
+  f3 F = mk3(fmaf(k.pen.x, e.C.x, k.lam.x), fmaf(k.pen.y, e.C.y, k.lam.y), fmaf(k.pen.z, e.C.z, k.lam.z));
   F.x = rmin(F.x, 0.f);
   e.bounds = fabsf(F.x) * friction;
-  e.frictionScale = sqrtf(F.y * F.y + F.z * F.z);
+  e.frictionScale = sqrtf(fmaf(F.y, F.y, F.z * F.z));
   if (e.frictionScale > e.bounds && e.frictionScale > 0.f) {
     float s = e.bounds / e.frictionScale;
     F.y = F.y * s;
@@ -452,8 +478,6 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
   }
   const bool isA = (head.z & ENTRY_IS_A) != 0;
   const PairBody other = pairBody<M>(w, head.x);
-  const PairBody PA = isA ? self : other;
-  const PairBody PB = isA ? other : self;
   const Basis basis = orthonormalScaled(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)),
                                         __uint_as_float(tail.w));
   Contact k = w.contacts[head.y + p];
@@ -461,8 +485,8 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
     k.lam = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_LAM));
   }
-  Eval ev = evalContact(w, k, basis, __uint_as_float(head.w), PA, PB, alpha, flags);
-  addContactRows(acc, basis, isA ? 1.f : -1.f, isA ? ev.rAW : ev.rBW, xyz(k.pen), ev.F);
+  const Eval ev = evalContact(w, k, basis, __uint_as_float(head.w), self, other, isA, alpha, flags);
+  addContactRows(acc, basis, isA ? 1.f : -1.f, ev.rSelf, xyz(k.pen), ev.F);
 }
 
 /**
@@ -561,61 +585,54 @@ template <uint32_t W> AVBD_D void mergeTree(Acc &acc) {
   mergeFromAbove<W, 1>(acc);
 }
 
+template <class T> AVBD_D T fmaT(T a, T b, T c);
+template <> AVBD_D float fmaT<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <> AVBD_D double fmaT<double>(double a, double b, double c) { return fma(a, b, c); }
+
 /**
- * LDLᵀ of the symmetric 6x6 system (lower triangle `A`, row-major by rows 1..6) with each pivot floored at
- * PIVOT_FLOOR of its diagonal (wgsl-solve.ts:416-470), then x = A⁻¹ rhs. Each pivot is inverted once and the column
- * scaled by the reciprocal (6 correctly rounded divisions instead of 21; the CPU restatement does the same). `floored` reports whether any pivot took its
+ * LDLᵀ of the symmetric 6x6 system (lower triangle `A`, row-major by rows 1..6) with each pivot after the first floored
+ * at PIVOT_FLOOR of its diagonal (wgsl-solve.ts:416-470), then x = A⁻¹ rhs. Each pivot is inverted once and its column
+ * scaled by the reciprocal (6 correctly rounded divisions instead of 21), and every sum of products (D_k = A_kk −
+ * Σ_j L_kj² D_j, L_ik = (A_ik − Σ_j L_ij L_kj D_j) / D_k, the forward and back substitutions) is one chain of fused
+ * multiply-adds in increasing j (the CPU restatement does the same). `floored` reports whether any pivot took its
  * floor (the unfloored pivot was not above it).
  */
 template <class T> AVBD_D void ldlt6(const T (&A)[21], const T (&b)[6], T (&x)[6], bool &floored) {
   const T fl = T(PIVOT_FLOOR);
-  auto pivot = [&](T d, T a) {
-    T f = a * fl;
-    floored = floored || !(d > f);
-    return d > f ? d : f;
-  };
-  const T A11 = A[0];
-  const T A21 = A[1], A22 = A[2];
-  const T A31 = A[3], A32 = A[4], A33 = A[5];
-  const T A41 = A[6], A42 = A[7], A43 = A[8], A44 = A[9];
-  const T A51 = A[10], A52 = A[11], A53 = A[12], A54 = A[13], A55 = A[14];
-  const T A61 = A[15], A62 = A[16], A63 = A[17], A64 = A[18], A65 = A[19], A66 = A[20];
-  const T D1 = A11;
-  const T r1 = T(1) / D1;
-  const T L21 = A21 * r1, L31 = A31 * r1, L41 = A41 * r1, L51 = A51 * r1, L61 = A61 * r1;
-  const T D2 = pivot(A22 - L21 * L21 * D1, A22);
-  const T r2 = T(1) / D2;
-  const T L32 = (A32 - L21 * L31 * D1) * r2;
-  const T L42 = (A42 - L21 * L41 * D1) * r2;
-  const T L52 = (A52 - L21 * L51 * D1) * r2;
-  const T L62 = (A62 - L21 * L61 * D1) * r2;
-  const T D3 = pivot(A33 - (L31 * L31 * D1 + L32 * L32 * D2), A33);
-  const T r3 = T(1) / D3;
-  const T L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) * r3;
-  const T L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) * r3;
-  const T L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) * r3;
-  const T D4 = pivot(A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3), A44);
-  const T r4 = T(1) / D4;
-  const T L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) * r4;
-  const T L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) * r4;
-  const T D5 = pivot(A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4), A55);
-  const T r5 = T(1) / D5;
-  const T L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) * r5;
-  const T D6 = pivot(A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5), A66);
-  const T r6 = T(1) / D6;
-  const T y1 = b[0];
-  const T y2 = b[1] - L21 * y1;
-  const T y3 = b[2] - L31 * y1 - L32 * y2;
-  const T y4 = b[3] - L41 * y1 - L42 * y2 - L43 * y3;
-  const T y5 = b[4] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
-  const T y6 = b[5] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
-  const T z1 = y1 * r1, z2 = y2 * r2, z3 = y3 * r3, z4 = y4 * r4, z5 = y5 * r5, z6 = y6 * r6;
-  x[5] = z6;
-  x[4] = z5 - L65 * x[5];
-  x[3] = z4 - L54 * x[4] - L64 * x[5];
-  x[2] = z3 - L43 * x[3] - L53 * x[4] - L63 * x[5];
-  x[1] = z2 - L32 * x[2] - L42 * x[3] - L52 * x[4] - L62 * x[5];
-  x[0] = z1 - L21 * x[1] - L31 * x[2] - L41 * x[3] - L51 * x[4] - L61 * x[5];
+  T L[6][6], D[6], r[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    T d = A[k * (k + 1) / 2 + k];
+#pragma unroll
+    for (int j = 0; j < k; j++) d = fmaT<T>(-(L[k][j] * L[k][j]), D[j], d);
+    if (k > 0) {
+      const T f = A[k * (k + 1) / 2 + k] * fl;
+      floored = floored || !(d > f);
+      d = d > f ? d : f;
+    }
+    D[k] = d;
+    r[k] = T(1) / d;
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+      T a = A[i * (i + 1) / 2 + k];
+#pragma unroll
+      for (int j = 0; j < k; j++) a = fmaT<T>(-(L[i][j] * L[k][j]), D[j], a);
+      L[i][k] = a * r[k];
+    }
+  }
+  T y[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    y[k] = b[k];
+#pragma unroll
+    for (int j = 0; j < k; j++) y[k] = fmaT<T>(-L[k][j], y[j], y[k]);
+  }
+#pragma unroll
+  for (int k = 5; k >= 0; k--) {
+    x[k] = y[k] * r[k];
+#pragma unroll
+    for (int j = k + 1; j < 6; j++) x[k] = fmaT<T>(-L[j][k], x[j], x[k]);
+  }
 }
 
 /** The FP64 solve of a system whose FP32 factorisation hit a pivot floor. */
@@ -714,7 +731,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
   PairBody self;
   self.rot = rot;
   self.dLin = sub(xyz(pos), xyz(B.ipos));
-  self.dAng = qsubInv(rot, w.irotInv[i]);
+  self.dAng = dispFused(rot, w.irotInv[i]);
   self.index = i;
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
@@ -759,8 +776,8 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
           kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
           kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
         }
-        const Eval ev = evalContact(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w),
-                                    isA ? self : other, isA ? other : self, p.alpha, p.flags);
+        const Eval ev = evalContact(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w), self, other, isA,
+                                    p.alpha, p.flags);
         dualUpdate<M>(w, c, kc, ev, p);
       }
     }
@@ -879,7 +896,7 @@ AVBD_D void dualPoint(const DeviceWorld &w, uint32_t m, uint32_t k) {
   PairBody PA = pairBody<SOLVE_COLORS>(w, mf.ids.x), PB = pairBody<SOLVE_COLORS>(w, mf.ids.y);
   Basis basis = orthonormal(xyz(mf.geo));
   const Contact kc = w.contacts[c];
-  dualUpdate<SOLVE_COLORS>(w, c, kc, evalContact(w, kc, basis, mf.geo.w, PA, PB, p.alpha, p.flags), p);
+  dualUpdate<SOLVE_COLORS>(w, c, kc, evalContact(w, kc, basis, mf.geo.w, PA, PB, true, p.alpha, p.flags), p);
 }
 
 /** Joints first (one thread each), then one thread per contact point of the dense point list. */

@@ -875,6 +875,26 @@ template <class R> class World {
     acc.rAng = V3<R>(std::fma(a.x, f, acc.rAng.x), std::fma(a.y, f, acc.rAng.y), std::fma(a.z, f, acc.rAng.z));
   }
 
+  // The contact rows' arithmetic as the HIP solve forms it: every sum of products one chain of fused multiply-adds
+  // (solve.hip dotFused, crossFused, rotFused, dispFused).
+  static R dotF(V3<R> a, V3<R> b) { return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z)); }
+  static V3<R> crossF(V3<R> a, V3<R> b) {
+    return V3<R>(std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)), std::fma(a.x, b.y, -(a.y * b.x)));
+  }
+  /** q v q* = v + 2 (w u + q⃗ × u), u = q⃗ × v (the quaternion rotation of layout.ts). */
+  static V3<R> rotF(Q4<R> q, V3<R> v) {
+    const V3<R> qv = q.xyz(), u = crossF(qv, v), t = crossF(qv, u);
+    return V3<R>(std::fma(R(2), std::fma(q.w, u.x, t.x), v.x), std::fma(R(2), std::fma(q.w, u.y, t.y), v.y),
+                 std::fma(R(2), std::fma(q.w, u.z, t.z), v.z));
+  }
+  /** qsub(a, b) = 2 (a b⁻¹)⃗ from bi = b⁻¹ (qinverse), each component of the product one chain. */
+  static V3<R> dispF(Q4<R> a, Q4<R> bi) {
+    using std::fma;
+    return V3<R>(R(2) * fma(a.w, bi.x, fma(a.x, bi.w, fma(a.y, bi.z, -(a.z * bi.y)))),
+                 R(2) * fma(a.w, bi.y, fma(a.y, bi.w, fma(a.z, bi.x, -(a.x * bi.z)))),
+                 R(2) * fma(a.w, bi.z, fma(a.z, bi.w, fma(a.x, bi.y, -(a.y * bi.x)))));
+  }
+
   struct PairBody {
     uint32_t index;
     Q4<R> rot;
@@ -882,7 +902,7 @@ template <class R> class World {
   };
   PairBody pairBody(uint32_t i) const {
     const Body<R> &b = bodies[i];
-    return {i, b.rot, sub(b.pos, b.ipos), qsub(b.rot, b.irot)};
+    return {i, b.rot, sub(b.pos, b.ipos), dispF(b.rot, qinverse(b.irot))};
   }
   struct Eval {
     V3<R> rAW, rBW, C, F;
@@ -896,18 +916,19 @@ template <class R> class World {
       rotB = bodies[B.index].irot;
     }
     Eval e;
-    e.rAW = rotate(rotA, k.rA);
-    e.rBW = rotate(rotB, k.rB);
+    e.rAW = rotF(rotA, k.rA);
+    e.rBW = rotF(rotB, k.rB);
     V3<R> n = row(basis, 0), t1 = row(basis, 1), t2 = row(basis, 2);
     R keep = R(1) - alpha;
-    // The anchors' relative displacement once, Δ = (Δx_A − Δx_B) + Δθ_A × r_A − Δθ_B × r_B, each row C0 (1 − α) + l·Δ
-    // (the reference's per-row sum l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B, rounded as the HIP step does)
-    const V3<R> delta = sub(add(sub(A.dLin, B.dLin), cross(A.dAng, e.rAW)), cross(B.dAng, e.rBW));
-    e.C = V3<R>(k.c0x * keep + dot(n, delta), k.c0y * keep + dot(t1, delta), k.c0z * keep + dot(t2, delta));
-    V3<R> F = add(mulv3(k.pen, e.C), k.lam);
+    // The anchors' relative displacement once, D = (Δx_A − Δx_B) + (Δθ_A × r_A − Δθ_B × r_B), each row
+    // C0 (1 − α) + l·D (the reference's per-row sum l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B, rounded as the
+    // HIP step does; exchanging A and B negates D exactly, so the HIP step's rows from B's side have these bits)
+    const V3<R> D = add(sub(A.dLin, B.dLin), sub(crossF(A.dAng, e.rAW), crossF(B.dAng, e.rBW)));
+    e.C = V3<R>(std::fma(k.c0x, keep, dotF(n, D)), std::fma(k.c0y, keep, dotF(t1, D)), std::fma(k.c0z, keep, dotF(t2, D)));
+    V3<R> F(std::fma(k.pen.x, e.C.x, k.lam.x), std::fma(k.pen.y, e.C.y, k.lam.y), std::fma(k.pen.z, e.C.z, k.lam.z));
     F.x = rmin(F.x, R(0));
     e.bounds = std::fabs(F.x) * friction;
-    e.frictionScale = std::sqrt(F.y * F.y + F.z * F.z);
+    e.frictionScale = std::sqrt(std::fma(F.y, F.y, F.z * F.z));
     if (e.frictionScale > e.bounds && e.frictionScale > 0) {
       R s = e.bounds / e.frictionScale;
       F.y = F.y * s;
@@ -1021,12 +1042,12 @@ template <class R> class World {
     const R c10 = fma(r.z, m00, -(r.x * m20)), c11 = fma(r.z, m10, -(r.x * m21)), c12 = fma(r.z, m20, -(r.x * m22));
     const R c20 = fma(r.x, m10, -(r.y * m00)), c21 = fma(r.x, m11, -(r.y * m10)), c22 = fma(r.x, m21, -(r.y * m20));
     R *a = acc.ang.m, *l = acc.lin.m, *c = acc.cross.m;
-    a[0] = a[0] + fma(r.y, c02, -(r.z * c01));
-    a[3] = a[3] + fma(r.y, c12, -(r.z * c11));
-    a[4] = a[4] + fma(r.z, c10, -(r.x * c12));
-    a[6] = a[6] + fma(r.y, c22, -(r.z * c21));
-    a[7] = a[7] + fma(r.z, c20, -(r.x * c22));
-    a[8] = a[8] + fma(r.x, c21, -(r.y * c20));
+    a[0] = fma(r.y, c02, fma(-r.z, c01, a[0]));
+    a[3] = fma(r.y, c12, fma(-r.z, c11, a[3]));
+    a[4] = fma(r.z, c10, fma(-r.x, c12, a[4]));
+    a[6] = fma(r.y, c22, fma(-r.z, c21, a[6]));
+    a[7] = fma(r.z, c20, fma(-r.x, c22, a[7]));
+    a[8] = fma(r.x, c21, fma(-r.y, c20, a[8]));
     l[0] = l[0] + m00;
     l[3] = l[3] + m10;
     l[4] = l[4] + m11;
@@ -1039,7 +1060,8 @@ template <class R> class World {
                                 fma(f.z, t2.z, fma(f.y, t1.z, f.x * n.z))),
                           sg);
     acc.rLin = add(acc.rLin, g);
-    acc.rAng = add(acc.rAng, V3<R>(fma(r.y, g.z, -(r.z * g.y)), fma(r.z, g.x, -(r.x * g.z)), fma(r.x, g.y, -(r.y * g.x))));
+    acc.rAng = V3<R>(fma(r.y, g.z, fma(-r.z, g.y, acc.rAng.x)), fma(r.z, g.x, fma(-r.x, g.z, acc.rAng.y)),
+                     fma(r.x, g.y, fma(-r.y, g.x, acc.rAng.z)));
   }
 
   static void mergeInto(Acc &a, const Acc &o) {
@@ -1077,6 +1099,49 @@ template <class R> class World {
   }
 
   /**
+   * The HIP primal's LDLᵀ solve of the 6x6 system [lin crossᵀ; cross ang] x = b (solve.hip ldlt6): pivots after the
+   * first floored at pivotFloor × their diagonal (wgsl-solve.ts:416-470), each pivot inverted once, and every sum of
+   * products (D_k = A_kk − Σ_j L_kj² D_j, L_ik = (A_ik − Σ_j L_ij L_kj D_j) / D_k, both substitutions) one chain of
+   * fused multiply-adds in increasing j.
+   */
+  static void ldltFused(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3<R> bLin, V3<R> bAng, V3<R> &xLin,
+                        V3<R> &xAng, R pivotFloor, bool *floored) {
+    // Lower triangle row by row: rows 1-3 from lin, rows 4-6 from cross (columns 1-3) and ang (columns 4-6)
+    const R A[21] = {aLin.m[0], aLin.m[3], aLin.m[4], aLin.m[6], aLin.m[7], aLin.m[8],
+                     aCross.m[0], aCross.m[1], aCross.m[2], aAng.m[0],
+                     aCross.m[3], aCross.m[4], aCross.m[5], aAng.m[3], aAng.m[4],
+                     aCross.m[6], aCross.m[7], aCross.m[8], aAng.m[6], aAng.m[7], aAng.m[8]};
+    const R b[6] = {bLin.x, bLin.y, bLin.z, bAng.x, bAng.y, bAng.z};
+    R L[6][6] = {}, Dg[6], rc[6], y[6], x[6];
+    for (int k = 0; k < 6; k++) {
+      R d = A[k * (k + 1) / 2 + k];
+      for (int j = 0; j < k; j++) d = std::fma(-(L[k][j] * L[k][j]), Dg[j], d);
+      if (k > 0) {
+        const R f = A[k * (k + 1) / 2 + k] * pivotFloor;
+        if (floored && !(d > f)) *floored = true;
+        d = d > f ? d : f;
+      }
+      Dg[k] = d;
+      rc[k] = R(1) / d;
+      for (int i = k + 1; i < 6; i++) {
+        R a = A[i * (i + 1) / 2 + k];
+        for (int j = 0; j < k; j++) a = std::fma(-(L[i][j] * L[k][j]), Dg[j], a);
+        L[i][k] = a * rc[k];
+      }
+    }
+    for (int k = 0; k < 6; k++) {
+      y[k] = b[k];
+      for (int j = 0; j < k; j++) y[k] = std::fma(-L[k][j], y[j], y[k]);
+    }
+    for (int k = 5; k >= 0; k--) {
+      x[k] = y[k] * rc[k];
+      for (int j = k + 1; j < 6; j++) x[k] = std::fma(-L[j][k], x[j], x[k]);
+    }
+    xLin = V3<R>(x[0], x[1], x[2]);
+    xAng = V3<R>(x[3], x[4], x[5]);
+  }
+
+  /**
    * finishBody (wgsl-solve.ts:416-470) with the FP64 pivot fallback: when an FP32 pivot takes its floor the system is
    * solved again in FP64 from the same FP32 entries (same floor), and the result rounded to FP32.
    */
@@ -1084,7 +1149,7 @@ template <class R> class World {
     Body<R> &b = bodies[i];
     V3<R> xl, xa;
     bool floored = false;
-    solve6(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f), &floored, true);
+    ldltFused(acc.lin, acc.ang, acc.cross, acc.rLin, acc.rAng, xl, xa, R(1e-5f), &floored);
     if (floored && sizeof(R) == sizeof(float)) {
       auto up = [](const M3<R> &m) {
         M3<double> d;
@@ -1093,8 +1158,8 @@ template <class R> class World {
       };
       auto upv = [](V3<R> v) { return V3<double>(double(v.x), double(v.y), double(v.z)); };
       V3<double> dl, da;
-      solve6(up(acc.lin), up(acc.ang), up(acc.cross), upv(acc.rLin), upv(acc.rAng), dl, da, double(1e-5f), nullptr,
-             true);
+      World<double>::ldltFused(up(acc.lin), up(acc.ang), up(acc.cross), upv(acc.rLin), upv(acc.rAng), dl, da,
+                               double(1e-5f), nullptr);
       xl = V3<R>(R(dl.x), R(dl.y), R(dl.z));
       xa = V3<R>(R(da.x), R(da.y), R(da.z));
 #pragma omp atomic
