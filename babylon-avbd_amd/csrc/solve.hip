@@ -63,21 +63,6 @@ AVBD_D void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 /** Polls of one wait before it gives up (about 65 ms); a wait given up sets C_DF_ABORT, which ends every other wait. */
 constexpr uint32_t DF_SPIN = 1u << 16;
 
-/** Wait until *s >= ts and *r >= tr, both polled in one round trip. Every wait ends like waitFor's. */
-AVBD_D void waitFor2(const DeviceWorld &w, const uint32_t *s, uint32_t ts, const uint32_t *r, uint32_t tr) {
-  for (uint32_t spin = 0;; spin++) {
-    const uint32_t a = ldAgent(s), b = ldAgent(r);
-    if (a >= ts && b >= tr) break;
-    if (spin >= DF_SPIN) {
-      atomicOr(&w.counters[C_DF_ABORT], 1u);
-      break;
-    }
-    if ((spin & 31u) == 31u && ldAgent(&w.counters[C_DF_ABORT]) != 0u) break;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  asm volatile("" ::: "memory");
-}
-
 /** Wait until *s >= target. Every wait ends: at DF_SPIN polls, or at once when an earlier wait gave up. */
 AVBD_D void waitFor(const DeviceWorld &w, const uint32_t *s, uint32_t target) {
   for (uint32_t spin = 0; ldAgent(s) < target; spin++) {
@@ -463,19 +448,6 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     return;
   }
   constexpr bool DF = M == SOLVE_DATAFLOW;
-  if constexpr (DF) {
-    // The partner's pose: this iteration's if its colour comes first, else the last iteration's. The point's
-    // multipliers come from the last iteration's dual, which the pair's finisher ran right after its primal: this
-    // body (its own stamp was waited for) or the partner, whose dual stamp then covers both.
-    if (head.z & ENTRY_DYNAMIC) {
-      const uint32_t pc = w.colorA[head.x];
-      if (pc < df.color)
-        waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
-      else if (df.it != 0u)
-        waitFor(w, finishesPair(pc, df.color) ? &w.dualDone[head.x] : &w.bodyDone[head.x],
-                df.base + df.it);
-    }
-  }
   const bool isA = (head.z & ENTRY_IS_A) != 0;
   const PairBody other = pairBody<M>(w, head.x);
   const Basis basis = orthonormalScaled(mk3(__uint_as_float(tail.x), __uint_as_float(tail.y), __uint_as_float(tail.z)),
@@ -530,6 +502,19 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
     if (e < end) {
       head = w.adjEntry[2ull * e];
       tail = w.adjEntry[2ull * e + 1];
+    }
+    if constexpr (DF) {
+      // Each lane waits for its entry's partner before any item is evaluated: its pose of this iteration if its colour
+      // comes first, else of the last one. The entry's multipliers come from the last iteration's dual, which the
+      // pair's finisher ran right after its primal: this body (its own stamp was waited for) or the partner, whose
+      // dual stamp then covers both.
+      if (e < end && (head.z & ENTRY_DYNAMIC) && !(head.z & ENTRY_JOINT)) {
+        const uint32_t pc = w.colorA[head.x];
+        if (pc < df.color)
+          waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
+        else if (df.it != 0u)
+          waitFor(w, finishesPair(pc, df.color) ? &w.dualDone[head.x] : &w.bodyDone[head.x], df.base + df.it);
+      }
     }
     const uint32_t items = e < end ? ((head.z & ENTRY_JOINT) ? 1u : (head.z & 15u)) : 0u;
     uint32_t incl = items;
