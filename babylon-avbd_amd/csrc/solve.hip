@@ -376,17 +376,21 @@ struct PairBody {
   f3 dLin, dAng;
   uint32_t index;
 };
-/** The angular displacement qsub(rot, x- rotation), from the warm start's inverse of the x- rotation. */
-template <int M = SOLVE_COLORS> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
-  constexpr bool DF = M == SOLVE_DATAFLOW;
-  const Body &B = w.bodies[i];
+/** Body i at pose (pos, rot): the angular displacement qsub(rot, x- rotation) from the warm start's inverse of the x-
+ * rotation. */
+AVBD_D PairBody pairBodyAt(const DeviceWorld &w, uint32_t i, float4 pos, float4 rot) {
   PairBody pb;
-  const float4 pos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
-  pb.rot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
-  pb.dLin = sub(xyz(pos), xyz(B.ipos));
-  pb.dAng = dispFused(pb.rot, w.irotInv[i]);
+  pb.rot = rot;
+  pb.dLin = sub(xyz(pos), xyz(w.bodies[i].ipos));
+  pb.dAng = dispFused(rot, w.irotInv[i]);
   pb.index = i;
   return pb;
+}
+template <int M = SOLVE_COLORS> AVBD_D PairBody pairBody(const DeviceWorld &w, uint32_t i) {
+  if constexpr (M == SOLVE_DATAFLOW)
+    return pairBodyAt(w, i, ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)), ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)));
+  else
+    return pairBodyAt(w, i, w.bodies[i].pos, w.bodies[i].rot);
 }
 
 struct Eval {
@@ -471,14 +475,15 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
  */
 template <uint32_t W, int M>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
-                       float dt, uint32_t flags, Acc &acc, const DfCtx &df) {
+                       float dt, uint32_t flags, Acc &acc, const DfCtx &df, float4 &bpos, float4 &brot) {
   constexpr bool DF = M == SOLVE_DATAFLOW;
   const Body &B = w.bodies[i];
   if constexpr (DF) {
     if (df.it) waitFor(w, &w.dualDone[i], df.base + df.it);  // the body's own last iteration and its duals
   }
-  const float4 bpos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
-  const float4 brot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
+  // The body's pose, read once: its rows, and the update finishBody makes
+  bpos = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS)) : B.pos;
+  brot = DF ? ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)) : B.rot;
   acc.l00 = acc.l10 = acc.l11 = acc.l20 = acc.l21 = acc.l22 = 0.f;
   acc.a00 = acc.a10 = acc.a11 = acc.a20 = acc.a21 = acc.a22 = 0.f;
 #pragma unroll
@@ -495,7 +500,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
     acc.rAng = mul3(I, qsub(brot, B.inertialRot));
   }
   if (first >= end) return;  // the same for every lane of the group
-  const PairBody self = pairBody<M>(w, i);
+  const PairBody self = pairBodyAt(w, i, bpos, brot);
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
     uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
@@ -639,8 +644,8 @@ AVBD_D void ldlt6Double(const float (&A)[21], const float (&b)[6], float (&x)[6]
  * same FP32 entries, with the same floor.
  */
 template <int M>
-AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df, float4 *newPos = nullptr,
-                       float4 *newRot = nullptr) {
+AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfCtx &df, float4 pos, float4 rot,
+                       float4 *newPos = nullptr, float4 *newRot = nullptr) {
   const float A[21] = {s.l00, s.l10, s.l11, s.l20, s.l21, s.l22, s.c[0], s.c[1], s.c[2], s.a00, s.c[3],
                        s.c[4], s.c[5], s.a10, s.a11, s.c[6], s.c[7], s.c[8], s.a20, s.a21, s.a22};
   const float b[6] = {s.rLin.x, s.rLin.y, s.rLin.z, s.rAng.x, s.rAng.y, s.rAng.z};
@@ -653,17 +658,15 @@ AVBD_D void finishBody(const DeviceWorld &w, uint32_t i, const Acc &s, const DfC
   }
   Body &B = w.bodies[i];
   if constexpr (M == SOLVE_DATAFLOW) {
-    const float4 pos = ldDev4(bodyRsrc(w), bodyOff(i, OFF_POS));
     *newPos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
-    *newRot = qadd(ldDev4(bodyRsrc(w), bodyOff(i, OFF_ROT)), mk3(-x[3], -x[4], -x[5]));
+    *newRot = qadd(rot, mk3(-x[3], -x[4], -x[5]));
     stDev4(bodyRsrc(w), bodyOff(i, OFF_POS), *newPos);
     stDev4(bodyRsrc(w), bodyOff(i, OFF_ROT), *newRot);
     storesDone();
     stAgent(&w.bodyDone[i], df.base + df.it + 1u);
   } else {
-    float4 pos = B.pos;
     pos = make_float4(pos.x - x[0], pos.y - x[1], pos.z - x[2], pos.w);
-    const float4 rot = qadd(B.rot, mk3(-x[3], -x[4], -x[5]));
+    rot = qadd(rot, mk3(-x[3], -x[4], -x[5]));
     B.pos = pos;
     B.rot = rot;
     if constexpr (M == SOLVE_WORLDS) {
@@ -712,12 +715,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
   constexpr bool DF = M == SOLVE_DATAFLOW;
   const uint32_t i = slot.x, first = slot.y, end = slot.z;
   const WorldParams &p = w.worlds[slot.w];
-  const Body &B = w.bodies[i];
-  PairBody self;
-  self.rot = rot;
-  self.dLin = sub(xyz(pos), xyz(B.ipos));
-  self.dAng = dispFused(rot, w.irotInv[i]);
-  self.index = i;
+  const PairBody self = pairBodyAt(w, i, pos, rot);
   for (uint32_t c0 = first; c0 < end; c0 += W) {
     const uint32_t e = c0 + lane;
     uint4 head = make_uint4(0u, 0u, 0u, 0u), tail = make_uint4(0u, 0u, 0u, 0u);
@@ -782,9 +780,10 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
   const bool live = k < end;
   const uint4 slot = live ? w.colorSlot[k] : make_uint4(0u, 0u, 0u, 0u);
   Acc acc;
+  float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f), brot = bpos;
   if (live) {
     const WorldParams &p = w.worlds[slot.w];
-    accumulate<W, M>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df);
+    accumulate<W, M>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df, bpos, brot);
   } else {
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
@@ -792,13 +791,13 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
   mergeTree<W>(acc);
   if constexpr (M != SOLVE_COLORS) {
     float4 pos = make_float4(0.f, 0.f, 0.f, 0.f), rot = pos;
-    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df, &pos, &rot);
+    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df, bpos, brot, &pos, &rot);
     // The new pose from the group's lane 0 to all of its lanes
     pos = make_float4(__shfl(pos.x, 0, W), __shfl(pos.y, 0, W), __shfl(pos.z, 0, W), __shfl(pos.w, 0, W));
     rot = make_float4(__shfl(rot.x, 0, W), __shfl(rot.y, 0, W), __shfl(rot.z, 0, W), __shfl(rot.w, 0, W));
     if (live) fusedDual<W, M>(w, slot, lane, pos, rot, df);
   } else {
-    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df);
+    if (live && lane == 0) finishBody<M>(w, slot.x, acc, df, bpos, brot);
   }
 }
 
