@@ -173,22 +173,24 @@ def cpu_baseline(scene_fn, state, first_step, max_steps, budget_s, colour_cap):
                               f"{dt_p:.1f} s"))
 
 
-def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
+def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0, whole_steps=100):
     """The batched-worlds workload: 4,096 box piles (body counts log-uniform in [8, 2048], friction U(0.2, 0.9)), one
     launch per phase per batch from captured hipGraphs. The autotuner (babylon-avbd_amd/tune.py) first times short
-    runs of the worlds split into several batches stepped a few at a time (each on its own stream) and keeps the
-    fastest split whose per-step latency (device time of one batch's step, HIP events) meets the 5 ms target at p99;
-    the timed window then runs that split. Beside it, one world per launch: the first `sample` worlds of the
-    same draw, each its own world (own stream and graph), stepped round-robin over the same step window."""
+    runs of the worlds split into several batches stepped a few at a time (each on its own stream), each with a solve
+    kernel (colour launches, dataflow, one workgroup per world at two sizes), and keeps the fastest whose per-step
+    latency (device time of one batch's step, HIP events) meets the 5 ms target at p99; the timed window then runs that
+    choice. Beside it: the whole set stepped as one batch (every world in one launch per phase), its per-step latency
+    p50/p99 and the HBM floor of its solve; and one world per launch: the first `sample` worlds of the same draw, each
+    its own world (own stream and graph), stepped round-robin over the same step window."""
     opts = {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True}
     pipe = avbd.createSolverPipeline("hip")
     log = lambda m: print("# " + m, file=sys.stderr, flush=True)  # noqa: E731
-    (groups, concurrency), table = avbd.tune.batch_groups(pipe, avbd.configs.batched_piles, opts,
-                                                          target_p99_ms=target_p99_ms, log=log)
+    (groups, concurrency, solve), table = avbd.tune.batch_groups(pipe, avbd.configs.batched_piles, opts,
+                                                                 target_p99_ms=target_p99_ms, log=log)
     scenes = avbd.configs.batched_piles()
     bodies = sum(s.body_count for s in scenes)
-    worlds = [pipe.createBatch(part, opts) for part in avbd.tune.split_balanced(scenes, groups)]
-    log(f"batch: {len(scenes)} worlds, {bodies} bodies in {len(worlds)} batches, "
+    worlds = [pipe.createBatch(part, dict(opts, **solve)) for part in avbd.tune.split_balanced(scenes, groups)]
+    log(f"batch: {len(scenes)} worlds, {bodies} bodies in {len(worlds)} batches ({avbd.tune.describe(solve)}), "
         f"{sum(w.deviceBytes() for w in worlds) / 2**30:.1f} GiB on device")
     avbd.tune.run_groups(worlds, warmup, concurrency)
     ms, elapsed = [], 0.0
@@ -202,10 +204,27 @@ def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
         log(f"batch step {warmup + done}: {len(worlds)} batches, {concurrency} at a time, last step {m[-1]:.2f} ms")
     counters = [w.readCounters() for w in worlds]
     prof = worlds[0].profileStep()
+    kind = worlds[0].solverKind()
     for w in worlds:
         w.destroy()
     ms = np.asarray(ms)
     agg = bodies * steps / elapsed
+
+    # The whole set as one batch: every world's step lasts as long as the set's
+    whole = pipe.createBatch(avbd.configs.batched_piles(), dict(opts, solver="colors"))
+    whole_kind = whole.solverKind()
+    avbd.tune.run_groups([whole], warmup, 1)
+    wms, welapsed = avbd.tune.run_groups([whole], whole_steps, 1)
+    wcount = whole.readCounters()
+    wprof = whole.profileStep()
+    whole.destroy()
+    wms = np.asarray(wms)
+    # HBM floor of its solve: per iteration each contact point's 64-B record read by both bodies' primal and by the
+    # dual, and its 28 B of multipliers and key written; each dynamic body's 160-B record
+    iters = 10
+    floor_bytes = iters * (wcount["contacts"] * (3 * 64 + 28) + bodies * 160)
+    log(f"whole set: step p50 {np.percentile(wms, 50):.2f} ms, p99 {np.percentile(wms, 99):.2f} ms, "
+        f"solve HBM floor {floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3:.2f} ms")
 
     solos = [pipe.createSolver(s, opts) for s in avbd.configs.batched_piles(count=sample)]
     solo_bodies = sum(w.bodyCount for w in solos)
@@ -225,18 +244,32 @@ def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0):
         w.synchronize()
     solo_rate = solo_bodies * steps / (time.perf_counter() - t0)
     pipe.destroy()
-    single = next(r for r in table if r[0] == 1)
     return {
         "workload": "4,096 box piles, body counts log-uniform in [8, 2048], friction U(0.2, 0.9), 10 iterations, "
                     "one launch per phase per batch (hipGraph)",
-        "worlds": len(scenes), "bodies": bodies, "batches": len(worlds), "concurrent_batches": concurrency,
+        "worlds": len(scenes), "bodies": bodies, "batches": groups, "concurrent_batches": concurrency,
+        "solve": avbd.tune.describe(solve), "solver_kind": kind,
         "steps_timed": f"{warmup + 1}-{warmup + steps}",
         "body_steps_per_s": agg, "ms_per_round": elapsed * 1e3 / steps,
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
         "step_ms_max": float(ms.max()), "p99_target_ms": target_p99_ms,
-        "tuning": [{"batches": g, "concurrent": c, "body_steps_per_s": r, "step_ms_p50": p50, "step_ms_p99": p99,
-                    "steps": "61-80"} for g, c, r, p50, p99 in table],
-        "single_batch_probe": {"body_steps_per_s": single[2], "step_ms_p99": single[4]},
+        "latency_unit": "one batch's step (device time, HIP events); a world's step lasts as long as its batch's",
+        "tuning": [{"batches": g, "concurrent": c, "solve": avbd.tune.describe(sv), "solver_kind": kd,
+                    "body_steps_per_s": r, "step_ms_p50": p50, "step_ms_p99": p99, "steps": "61-80"}
+                   for g, c, r, p50, p99, sv, kd in table],
+        "whole_set": {
+            "what": "all 4,096 worlds in one batch (one launch per phase; colour launches, solver kind "
+                    f"{whole_kind}), steps {warmup + 1}-{warmup + whole_steps}",
+            "body_steps_per_s": bodies * whole_steps / welapsed,
+            "step_ms_p50": float(np.percentile(wms, 50)), "step_ms_p99": float(np.percentile(wms, 99)),
+            "counters": wcount, "phases_ms": wprof,
+            "solve_hbm_floor_ms": floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3,
+            "gap": "the whole set's step cannot meet 5 ms on one MI355X: its solve alone moves at least "
+                   f"{floor_bytes / 1e9:.1f} GB per step (10 iterations over {wcount['contacts']:,} contact points, "
+                   "each record read by both bodies' primal and the dual and its multipliers written, plus every "
+                   f"body's record), {floor_bytes / (HBM_PEAK_GBS * 1e9) * 1e3:.1f} ms at the 8 TB/s peak, before "
+                   "collision, adjacency and colouring",
+        },
         "counters_first_batch": counters[0], "phases_ms_first_batch": prof,
         "one_world_per_launch": {"worlds": sample, "bodies": solo_bodies, "body_steps_per_s": solo_rate,
                                  "sample": f"worlds 0-{sample - 1} of the same draw, each its own world, "
