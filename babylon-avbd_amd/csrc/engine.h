@@ -225,6 +225,8 @@ class Engine {
   template <class T> T *dallocT(size_t count) { return static_cast<T *>(dalloc(count * sizeof(T))); }
   void resetGraphs();
   void pollCounters();
+  /** After a step whose dataflow solve gave up a wait: colour launches from the next step on. */
+  void noteAborts(const Counters &c);
   /** Copy words to a device buffer on the stream (through one of two pinned staging buffers); returns the buffer. */
   uint32_t *uploadEdit(const uint32_t *words, size_t n);
 

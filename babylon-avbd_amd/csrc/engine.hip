@@ -4,6 +4,7 @@
 // iterations × (primal per colour, dual) → velocities; its capacity policy is GpuSolver3D.adapt (solver.ts:1484-1519).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <tuple>
@@ -261,6 +262,9 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   w.bodyDone = dallocT<uint32_t>(nBodies_);
   w.dualDone = dallocT<uint32_t>(nBodies_);
   w.dfPlan = dallocT<DataflowPlan>(1);
+  // Polls of a dataflow wait before it gives up (about 65 ms); AVBD_DATAFLOW_SPIN lowers it to force the give-up path
+  w.dfSpin = 1u << 16;
+  if (const char *spin = std::getenv("AVBD_DATAFLOW_SPIN")) w.dfSpin = uint32_t(std::strtoul(spin, nullptr, 10));
   w.filters = dallocT<uint2>(nBodies_);
   w.joints = dallocT<Joint>(nJoints_);
   w.info = dallocT<int4>(nJoints_);
@@ -780,8 +784,20 @@ Counters Engine::readCounters() {
   uint32_t c[COUNTER_WORDS];
   HIP_CHECK(hipMemcpyAsync(c, base_.counters, sizeof c, hipMemcpyDeviceToHost, stream_));
   synchronize();
-  return Counters{c[C_PAIR_DEMAND], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
-                  c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
+  Counters out{c[C_PAIR_DEMAND], c[C_CONTACTS], c[C_MANIFOLDS], c[C_OVERFLOW], c[C_CLASHES], c[C_NUM_COLORS],
+               c[C_PREV_MANIFOLDS], c[C_PIVOT64], c[C_DF_ABORT], c[C_PENDING]};
+  noteAborts(out);
+  return out;
+}
+
+void Engine::noteAborts(const Counters &c) {
+  // A dataflow wait gave up (its workgroups were not all resident for 65 ms: another kernel held the device): that
+  // step's result is not valid (the counter says so) and every later step runs the colour launches instead
+  if (c.solveAborts && useDataflow_) {
+    useDataflow_ = false;
+    wantDataflow_ = false;
+    resetGraphs();
+  }
 }
 
 std::vector<uint2> Engine::readPairs() {
@@ -1489,6 +1505,10 @@ bool Engine::adapt() { return adaptFrom(readCounters()); }
 
 bool Engine::adaptFrom(const Counters &c) {
   bool changed = false;
+  if (c.solveAborts && useDataflow_) {
+    noteAborts(c);
+    changed = true;
+  }
   // Colour cap and rounds (solver.ts:1495-1511): grow at once when a spare runs out or the colouring clashed; shrink
   // to the colours in use + 3 after three quiet reads
   const uint32_t SPARE = 3;

@@ -87,6 +87,7 @@ struct DeviceWorld {
   uint32_t *bodyDone;     // per body: stamp of its last primal update (dataflow solve)
   uint32_t *dualDone;     // per body: stamp of its last dual of the pairs it finishes (dataflow solve)
   DataflowPlan *dfPlan;
+  uint32_t dfSpin;        // polls of a dataflow wait before it gives up (0: give up at the first unmet poll)
   Contact *prevContacts;
   uint32_t *table;        // last step's manifolds by pair hash (value = index + 1)
   uint32_t hashMask;

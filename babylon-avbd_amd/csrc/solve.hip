@@ -60,13 +60,11 @@ constexpr uint32_t OFF_POS = 0, OFF_ROT = 16, OFF_KEY = 12, OFF_PEN = 32, OFF_LA
 /** This wave's earlier stores are complete before any later one is issued. */
 AVBD_D void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-/** Polls of one wait before it gives up (about 65 ms); a wait given up sets C_DF_ABORT, which ends every other wait. */
-constexpr uint32_t DF_SPIN = 1u << 16;
-
-/** Wait until *s >= target. Every wait ends: at DF_SPIN polls, or at once when an earlier wait gave up. */
+/** Wait until *s >= target. Every wait ends: at w.dfSpin polls (about 65 ms by default), or at once when an earlier
+ * wait gave up; a wait given up sets C_DF_ABORT, which ends every other wait. */
 AVBD_D void waitFor(const DeviceWorld &w, const uint32_t *s, uint32_t target) {
   for (uint32_t spin = 0; ldAgent(s) < target; spin++) {
-    if (spin >= DF_SPIN) {
+    if (spin >= w.dfSpin) {
       atomicOr(&w.counters[C_DF_ABORT], 1u);
       break;
     }

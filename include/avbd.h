@@ -105,7 +105,9 @@ typedef struct AvbdCounters {
   uint32_t pairs, contacts, manifolds, overflow, clashes, colors, joints, bodies;
   uint32_t pivot_fallbacks; /* bodies whose 6x6 solve was redone in FP64 in the last step (a pivot hit its floor) */
   uint32_t solve_aborts;    /* waits given up in the last step, by the dataflow solve or a prefix scan (nonzero: that
-                               step is not valid; never expected) */
+                               step is not valid; never expected with the device to itself). A world whose dataflow
+                               solve gave up runs colour launches from its next step on, once this counter has been
+                               read (avbd_world_read_counters, avbd_world_adapt or the adaptive readback) */
 } AvbdCounters;
 
 /** Thread-local message of the last failed call ("" if none). */

@@ -100,3 +100,30 @@ def test_colouring_rounds_listed_and_launched_agree(avbd, mode):
         assert np.array_equal(got[:, 0:8], want[:, 0:8]), mode  # poses, bit for bit
         assert np.array_equal(got[:, 32:39], want[:, 32:39]), mode  # velocities
     world.destroy()
+
+
+def test_a_wait_given_up_switches_to_colour_launches(avbd, monkeypatch):
+    """A dataflow wait that gives up (forced from the host: the world is made with a spin limit of 0, so the first
+    unmet poll gives up) is reported by solveAborts; reading the counters switches the world to colour launches, whose
+    next steps match the oracle from the GPU's state."""
+    scene = avbd.configs.pile_1000
+    landed = _world(avbd, scene())
+    landed.step(60)
+    state = landed.readState()
+    landed.destroy()
+    monkeypatch.setenv("AVBD_DATAFLOW_SPIN", "0")
+    world = _world(avbd, scene())
+    monkeypatch.delenv("AVBD_DATAFLOW_SPIN")
+    assert world.solverKind() == 2
+    world.writeState(state)
+    world.step()
+    assert world.readCounters()["solveAborts"] > 0, "the forced give-up is reported"
+    assert world.solverKind() == 1, "the world falls back to colour launches"
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    for _ in range(5):
+        oracle.seed(world.readState())
+        world.step()
+        oracle.step()
+        compare_step(world, oracle)
+        assert world.readCounters()["solveAborts"] == 0
+    world.destroy()
