@@ -174,5 +174,7 @@ constexpr uint32_t COLOR_FLAG_WORDS = 64;  // colorFlags: rounds past 61 always 
  */
 constexpr uint32_t ENTRY_NONE = 0xffffffffu;
 constexpr uint32_t ENTRY_IS_A = 1u << 4, ENTRY_JOINT = 1u << 5, ENTRY_DYNAMIC = 1u << 6;
+/** [0].z bits 8-15: a dynamic partner's colour, written after the colouring for the dataflow solve. */
+constexpr uint32_t ENTRY_COLOR_SHIFT = 8;
 
 }  // namespace avbd

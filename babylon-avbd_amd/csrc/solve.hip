@@ -512,7 +512,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       // pair's finisher ran right after its primal: this body (its own stamp was waited for) or the partner, whose
       // dual stamp then covers both.
       if (e < end && (head.z & ENTRY_DYNAMIC) && !(head.z & ENTRY_JOINT)) {
-        const uint32_t pc = w.colorA[head.x];
+        const uint32_t pc = head.z >> ENTRY_COLOR_SHIFT;  // the partner's colour (annotated after the colouring)
         if (pc < df.color)
           waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
         else if (df.it != 0u)
@@ -722,7 +722,8 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
       head = w.adjEntry[2ull * e];
       tail = w.adjEntry[2ull * e + 1];
       if (!(head.z & ENTRY_JOINT) &&
-          (!(head.z & ENTRY_DYNAMIC) || finishesPair(df.color, w.colorA[head.x])))
+          (!(head.z & ENTRY_DYNAMIC) ||
+           finishesPair(df.color, DF ? head.z >> ENTRY_COLOR_SHIFT : w.colorA[head.x])))
         items = head.z & 15u;
     }
     uint32_t incl = items;
