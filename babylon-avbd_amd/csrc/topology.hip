@@ -339,28 +339,28 @@ __global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w, uint
 }
 
 // 6. Scatter: chunk offset for the body's bucket plus its rank; with `warm`, also every body's warm start (the
-// scatter reads no pose, and no joint has to be warm-started first). With `annotate` (the dataflow solve), each
-// dynamic partner's colour goes into bits 8-15 of the body's entries' flag words (ENTRY_COLOR_SHIFT), so the solve
-// reads it with the entry.
-template <bool warm>
-__global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w, uint32_t annotate) {
+// scatter reads no pose, and no joint has to be warm-started first)
+template <bool warm> __global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
   uint32_t b = blockIdx.x * COLOR_WG + threadIdx.x;
   if (b >= w.nBodies) return;
   if (dynamicBody(w, int(b))) {
     uint32_t col = w.colorA[b] & 0xffu;
     uint32_t slot = w.colorHist[bucketKey(w, b, col) * w.colorGroups + blockIdx.x] + w.colorRank[b];
     w.colorBodies[slot] = b;
-    const uint32_t first = w.adjStart[b], end = umin(w.adjStart[b + 1], w.adjCapacity);
-    w.colorSlot[slot] = make_uint4(b, first, end, w.bodyWorld[b]);
-    if (annotate)
-      for (uint32_t e = first; e < end; e++) {
-        uint32_t *flags = &w.adjEntry[2ull * e].z;
-        const uint32_t f = *flags;
-        if ((f & ENTRY_DYNAMIC) && !(f & ENTRY_JOINT))
-          *flags = (f & ((1u << ENTRY_COLOR_SHIFT) - 1u)) | ((w.colorA[w.adjEntry[2ull * e].x] & 0xffu) << ENTRY_COLOR_SHIFT);
-      }
+    w.colorSlot[slot] = make_uint4(b, w.adjStart[b], umin(w.adjStart[b + 1], w.adjCapacity), w.bodyWorld[b]);
   }
   if (warm && !held(w)) warmBody(w, b);
+}
+
+// 7. For the dataflow solve: each dynamic partner's colour into bits 8-15 of its entry's flag word (ENTRY_COLOR_SHIFT),
+// so the solve reads it with the entry; one thread per entry
+__global__ void k_entry_colors(DeviceWorld w) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= umin(w.adjStart[w.nBodies], w.adjCapacity)) return;
+  const uint4 head = w.adjEntry[2ull * e];
+  if ((head.z & ENTRY_DYNAMIC) && !(head.z & ENTRY_JOINT))
+    w.adjEntry[2ull * e].z = (head.z & ((1u << ENTRY_COLOR_SHIFT) - 1u)) |
+                             ((w.colorA[head.x] & 0xffu) << ENTRY_COLOR_SHIFT);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -397,9 +397,9 @@ void launch_coloring(const DeviceWorld &w, uint32_t rounds, uint32_t planIterati
   launch_scan_exclusive(w.colorHist, COLOR_KEYS * w.colorGroups + 1, w.scanScratch, s,
                         w.counters + C_DF_ABORT);
   k_color_starts<<<1, COLOR_KEYS, 0, s>>>(w, planIterations);
-  const uint32_t annotate = planIterations > 0 ? 1u : 0u;
-  if (w.nJoints) k_color_scatter<false><<<w.colorGroups, COLOR_WG, 0, s>>>(w, annotate);
-  else k_color_scatter<true><<<w.colorGroups, COLOR_WG, 0, s>>>(w, annotate);
+  if (w.nJoints) k_color_scatter<false><<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+  else k_color_scatter<true><<<w.colorGroups, COLOR_WG, 0, s>>>(w);
+  if (planIterations) k_entry_colors<<<blocks(w.adjCapacity, 256), 256, 0, s>>>(w);
 }
 
 }  // namespace avbd
