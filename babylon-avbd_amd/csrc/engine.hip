@@ -223,7 +223,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   if (opt.worldBlock != 0 && opt.worldBlock != 128 && opt.worldBlock != 256 && opt.worldBlock != 512 &&
       opt.worldBlock != 1024)
     throw std::invalid_argument("world_block must be 0, 128, 256, 512 or 1024");
-  if (opt.solver == 2 && nJoints_ == 0) {
+  if (opt.solver == 2) {
     dfGrid_ = dataflow_grid();
     useDataflow_ = dfGrid_ > 0;
   }
@@ -392,8 +392,9 @@ void Engine::allocateBvh(const std::vector<uint32_t> &counts) {
 }
 
 void Engine::allocateContacts(uint32_t pairCap) {
-  // The dataflow solve addresses bodies and contact slots with 32-bit buffer offsets
-  const uint64_t span = std::max<uint64_t>(uint64_t(nBodies_) * sizeof(Body),
+  // The dataflow solve addresses bodies, joints and contact slots with 32-bit buffer offsets
+  const uint64_t span = std::max<uint64_t>(std::max<uint64_t>(uint64_t(nBodies_) * sizeof(Body),
+                                                              uint64_t(nJoints_) * sizeof(Joint)),
                                            uint64_t(pairCap) * MAX_CONTACTS * sizeof(Contact));
   useDataflow_ = wantDataflow_ && span < (1ull << 32);
   // Keep last step's manifolds and contacts (the next step's warm-start source) across a resize
@@ -538,7 +539,7 @@ void Engine::encodeStep(const DeviceWorld &w, bool rebuild, uint32_t sleepForce)
   if (useWorlds_) {
     launch_solve_worlds(w, worldOrder_, worldKeyStart_, worldJointStart_, iterations_, worldBlock_, s);  // + velocities
   } else if (useDataflow_) {
-    launch_dataflow(w, dfGrid_, s);
+    launch_dataflow(w, dfGrid_, iterations_, s);
     launch_velocities(w, s);
   } else {
     for (uint32_t it = 0; it < iterations_; it++) {
@@ -682,7 +683,7 @@ PhaseTimes Engine::profileStep() {
   } else if (useDataflow_) {
     // One launch for every iteration: its time is reported as the primal's, the dual's is in it
     HIP_CHECK(hipEventRecord(ev[k++], s));
-    launch_dataflow(w, dfGrid_, s);
+    launch_dataflow(w, dfGrid_, iterations_, s);
     HIP_CHECK(hipEventRecord(ev[k++], s));
   } else {
     for (uint32_t it = 0; it < iterations_; it++) {

@@ -80,9 +80,9 @@ void launch_sleep_after(const DeviceWorld &w, const SleepParams &p, hipStream_t 
 uint32_t dataflow_grid();
 /**
  * Every iteration's primal and dual in one launch of `grid` blocks (at most dataflow_grid()), after a launch_coloring
- * that wrote the work list.
+ * that wrote the work list; then the duals of joints without a dynamic body.
  */
-void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s);
+void launch_dataflow(const DeviceWorld &w, uint32_t grid, uint32_t iterations, hipStream_t s);
 
 /**
  * A batch's solve with one workgroup of `block` threads (128, 256, 512 or 1024) per world: every iteration's colours,

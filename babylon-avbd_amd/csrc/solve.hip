@@ -56,6 +56,12 @@ AVBD_D void stDev4(__amdgpu_buffer_rsrc_t r, uint32_t byteOffset, float4 f) {
 AVBD_D uint32_t bodyOff(uint32_t i, uint32_t field) { return i * uint32_t(sizeof(Body)) + field; }
 AVBD_D uint32_t contactOff(uint32_t c, uint32_t field) { return c * uint32_t(sizeof(Contact)) + field; }
 constexpr uint32_t OFF_POS = 0, OFF_ROT = 16, OFF_KEY = 12, OFF_PEN = 32, OFF_LAM = 48;
+AVBD_D __amdgpu_buffer_rsrc_t jointRsrc(const DeviceWorld &w) {
+  return __builtin_amdgcn_make_buffer_rsrc(static_cast<void *>(w.joints), 0, int(w.nJoints * uint32_t(sizeof(Joint))),
+                                           BUF_WORD3);
+}
+/** The part of a joint record the duals write: penLin, penAng, lamLin, lamAng (its first 64 bytes). */
+constexpr uint32_t JOINT_MUTABLE_RECORDS = 4;
 
 /** This wave's earlier stores are complete before any later one is issued. */
 AVBD_D void storesDone() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -76,40 +82,59 @@ AVBD_D void waitFor(const DeviceWorld &w, const uint32_t *s, uint32_t target) {
 
 // --- joints ------------------------------------------------------------------------------------------------------------
 
-AVBD_D f3 anchorA(const DeviceWorld &w, const Joint &k, int a) {
-  if (a < 0) return xyz(k.rA);
-  return add(qrotate(w.bodies[a].rot, xyz(k.rA)), xyz(w.bodies[a].pos));
+/**
+ * The two bodies of a constraint at the poses it is evaluated at. A may be the world: then rA is a world point and A's
+ * rotation the identity. The colour launches read the poses from the records; the dataflow solve hands in the body's
+ * own pose and reads its partner's with device-scope loads.
+ */
+struct JointPoses {
+  float4 posA, rotA, posB, rotB;
+  bool world;
+};
+AVBD_D JointPoses jointPoses(const DeviceWorld &w, int a, int b) {
+  JointPoses P;
+  P.world = a < 0;
+  P.posA = P.world ? make_float4(0.f, 0.f, 0.f, 0.f) : w.bodies[a].pos;
+  P.rotA = P.world ? make_float4(0.f, 0.f, 0.f, 1.f) : w.bodies[a].rot;
+  P.posB = w.bodies[b].pos;
+  P.rotB = w.bodies[b].rot;
+  return P;
 }
-AVBD_D float4 rotA(const DeviceWorld &w, int a) {
-  return a < 0 ? make_float4(0.f, 0.f, 0.f, 1.f) : w.bodies[a].rot;
+
+/** A joint's record as the solve reads it: in the dataflow solve the part the duals write with device-scope loads. */
+template <int M> AVBD_D Joint loadJoint(const DeviceWorld &w, uint32_t j);
+template <int M> AVBD_D void storeJoint(const DeviceWorld &w, uint32_t j, const Joint &k);
+/** The poses a joint of body i is evaluated at: i's own as given, its partner's read (device scope in the dataflow). */
+template <int M> AVBD_D JointPoses jointPosesFor(const DeviceWorld &w, int4 f, uint32_t i, float4 pos, float4 rot);
+
+AVBD_D f3 anchorA(const Joint &k, const JointPoses &P) {
+  if (P.world) return xyz(k.rA);
+  return add(qrotate(P.rotA, xyz(k.rA)), xyz(P.posA));
 }
-AVBD_D f3 jointLinC(const DeviceWorld &w, const Joint &k, int a, int b) {
-  return sub(anchorA(w, k, a), add(qrotate(w.bodies[b].rot, xyz(k.rB)), xyz(w.bodies[b].pos)));
+AVBD_D f3 jointLinC(const Joint &k, const JointPoses &P) {
+  return sub(anchorA(k, P), add(qrotate(P.rotB, xyz(k.rB)), xyz(P.posB)));
 }
 /**
  * The orientation a joint's angle rows hold B at: A's rotation times the rest orientation captured when the joint was
  * made ((c0Lin.w, c0Ang.w, rA.w, rB.w)), or A's rotation alone for a record without one (withJointRestFrames,
  * src/gpu/gpuSolverKernels.js:105-130).
  */
-AVBD_D float4 restTarget(const DeviceWorld &w, const Joint &k, int a) {
-  const float4 t = rotA(w, a);
+AVBD_D float4 restTarget(const Joint &k, const JointPoses &P) {
+  const float4 t = P.rotA;
   const float4 r = make_float4(k.c0Lin.w, k.c0Ang.w, k.rA.w, k.rB.w);
   if (!(qlengthSq(r) > 0.5f)) return t;
   return make_float4((t.w * r.x + r.w * t.x) + (t.y * r.z - t.z * r.y), (t.w * r.y + r.w * t.y) + (t.z * r.x - t.x * r.z),
                      (t.w * r.z + r.w * t.z) + (t.x * r.y - t.y * r.x), t.w * r.w - ((t.x * r.x + t.y * r.y) + t.z * r.z));
 }
-AVBD_D f3 jointAngC(const DeviceWorld &w, const Joint &k, int a, int b) {
-  return scale(qsub(restTarget(w, k, a), w.bodies[b].rot), k.lamAng.w);
-}
+AVBD_D f3 jointAngC(const Joint &k, const JointPoses &P) { return scale(qsub(restTarget(k, P), P.rotB), k.lamAng.w); }
 
 // --- angular constraints: a motor or a limit about a hinge axis (src/gpu/angularConstraints.js:16-83) --------------
 
 struct HingeFrame {
   f3 axis, ta, tb;  // A's axis, A's reference tangent, B's reference tangent, in world space
 };
-AVBD_D HingeFrame hingeFrame(const DeviceWorld &w, const Joint &k, int a, int b) {
-  const float4 qa = rotA(w, a);
-  return HingeFrame{qrotate(qa, xyz(k.rA)), qrotate(qa, xyz(k.c0Lin)), qrotate(w.bodies[b].rot, xyz(k.c0Ang))};
+AVBD_D HingeFrame hingeFrame(const Joint &k, const JointPoses &P) {
+  return HingeFrame{qrotate(P.rotA, xyz(k.rA)), qrotate(P.rotA, xyz(k.c0Lin)), qrotate(P.rotB, xyz(k.c0Ang))};
 }
 /** The twist of B about A's axis: the signed angle from A's tangent to B's. */
 AVBD_D float hingeAngle(const HingeFrame &f) { return atan2r(dot(f.axis, cross(f.ta, f.tb)), dot(f.ta, f.tb)); }
@@ -128,7 +153,7 @@ AVBD_D void warmHinge(const DeviceWorld &w, uint32_t j, int4 f) {
   Joint k = w.joints[j];
   if (k.penLin.w == 0.f) return;
   const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
-  const float angle = hingeAngle(hingeFrame(w, k, f.y, f.z));
+  const float angle = hingeAngle(hingeFrame(k, jointPoses(w, f.y, f.z)));
   k.c0Lin.w = f.x == T_MOTOR ? angle : rmin(0.f, angle - k.rA.w);
   k.c0Ang.w = rmax(0.f, angle - k.rB.w);
   const float decay = p.alpha * p.gamma;
@@ -138,27 +163,25 @@ AVBD_D void warmHinge(const DeviceWorld &w, uint32_t j, int4 f) {
                          wclamp(k.penLin.z * p.gamma, PENALTY_MIN, PENALTY_MAX), k.penLin.w);
   w.joints[j] = k;
 }
-AVBD_D void dualHinge(const DeviceWorld &w, uint32_t j, int4 f) {
-  Joint k = w.joints[j];
-  if (k.penLin.w == 0.f) return;
-  const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
-  const float2 C = hingeRows(k, f.x, hingeAngle(hingeFrame(w, k, f.y, f.z)), p.alpha, p.dt);
+/** The dual of a motor or limit on its record; false when it has nothing to do. */
+AVBD_D bool dualHinge(const WorldParams &p, Joint &k, int4 f, const JointPoses &P) {
+  if (k.penLin.w == 0.f) return false;
+  const float2 C = hingeRows(k, f.x, hingeAngle(hingeFrame(k, P)), p.alpha, p.dt);
   const float2 F = hingeForce(k, f.x, C);
   k.lamLin = make_float4(F.x, F.y, 0.f, k.lamLin.w);
   const float vx = f.x == T_MOTOR ? fabsf(C.x) : rmax(0.f, -C.x), vy = f.x == T_MOTOR ? fabsf(C.y) : rmax(0.f, C.y);
   k.penLin.x = rmin(k.penLin.x + vx * p.betaAng, PENALTY_MAX);
   k.penLin.y = rmin(k.penLin.y + vy * p.betaAng, PENALTY_MAX);
-  w.joints[j] = k;
+  return true;
 }
 
 /** Tensile failure of a spring (withSpringFracture, src/gpu/springMaterial.js:39-68): stretched past its break strain
- * (of the rest length) or pulling past its break force, it stops acting, keeping both thresholds. */
-AVBD_D void springFracture(const DeviceWorld &w, uint32_t j, int4 f) {
-  Joint k = w.joints[j];
+ * (of the rest length) or pulling past its break force, it stops acting, keeping both thresholds. True when it broke. */
+AVBD_D bool springFracture(Joint &k, const JointPoses &P) {
   const float strainLimit = k.lamLin.w, forceLimit = k.lamAng.w;
-  if (!(k.penLin.w > 0.f && (strainLimit > 0.f || forceLimit > 0.f))) return;
-  const f3 pb = add(qrotate(w.bodies[f.z].rot, xyz(k.rB)), xyz(w.bodies[f.z].pos));
-  const float lengthNow = length(sub(anchorA(w, k, f.y), pb));
+  if (!(k.penLin.w > 0.f && (strainLimit > 0.f || forceLimit > 0.f))) return false;
+  const f3 pb = add(qrotate(P.rotB, xyz(k.rB)), xyz(P.posB));
+  const float lengthNow = length(sub(anchorA(k, P), pb));
   const float extension = rmax(0.f, lengthNow - k.rA.w);
   const float strain = extension / rmax(k.rA.w, 1.0e-6f);
   const float tension = k.penLin.w * extension;
@@ -167,14 +190,15 @@ AVBD_D void springFracture(const DeviceWorld &w, uint32_t j, int4 f) {
     k.penAng = make_float4(0.f, 0.f, 0.f, 0.f);
     k.lamLin = make_float4(0.f, 0.f, 0.f, strainLimit);
     k.lamAng = make_float4(0.f, 0.f, 0.f, forceLimit);
-    w.joints[j] = k;
+    return true;
   }
+  return false;
 }
-AVBD_D f3 jointLinRows(const DeviceWorld &w, const Joint &k, int a, int b, float alpha) {
-  return sub(jointLinC(w, k, a, b), k.penLin.w >= HARD ? scale(xyz(k.c0Lin), alpha) : mk3(0.f, 0.f, 0.f));
+AVBD_D f3 jointLinRows(const Joint &k, const JointPoses &P, float alpha) {
+  return sub(jointLinC(k, P), k.penLin.w >= HARD ? scale(xyz(k.c0Lin), alpha) : mk3(0.f, 0.f, 0.f));
 }
-AVBD_D f3 jointAngRows(const DeviceWorld &w, const Joint &k, int a, int b, float alpha) {
-  return sub(jointAngC(w, k, a, b), k.penAng.w >= HARD ? scale(xyz(k.c0Ang), alpha) : mk3(0.f, 0.f, 0.f));
+AVBD_D f3 jointAngRows(const Joint &k, const JointPoses &P, float alpha) {
+  return sub(jointAngC(k, P), k.penAng.w >= HARD ? scale(xyz(k.c0Ang), alpha) : mk3(0.f, 0.f, 0.f));
 }
 AVBD_D f3 penClamp(f3 pen, float gamma, float stiffness) {
   return mk3(rmin(wclamp(pen.x * gamma, PENALTY_MIN, PENALTY_MAX), stiffness),
@@ -194,8 +218,9 @@ __global__ void k_warm_joints(DeviceWorld w) {
   Joint k = w.joints[j];
   const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
   // The w words keep the rest orientation
-  k.c0Lin = mk4(jointLinC(w, k, f.y, f.z), k.c0Lin.w);
-  k.c0Ang = mk4(jointAngC(w, k, f.y, f.z), k.c0Ang.w);
+  const JointPoses P = jointPoses(w, f.y, f.z);
+  k.c0Lin = mk4(jointLinC(k, P), k.c0Lin.w);
+  k.c0Ang = mk4(jointAngC(k, P), k.c0Ang.w);
   float decay = p.alpha * p.gamma;
   k.lamLin = mk4(scale(xyz(k.lamLin), decay), k.lamLin.w);
   k.lamAng = mk4(scale(xyz(k.lamAng), decay), k.lamAng.w);
@@ -208,6 +233,49 @@ __global__ void k_warm_bodies(DeviceWorld w) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nBodies || held(w)) return;
   warmBody(w, i);
+}
+
+template <int M> AVBD_D Joint loadJoint(const DeviceWorld &w, uint32_t j) {
+  Joint k = w.joints[j];
+  if constexpr (M == SOLVE_DATAFLOW) {
+    float4 *mut = &k.penLin;
+#pragma unroll
+    for (uint32_t q = 0; q < JOINT_MUTABLE_RECORDS; q++) mut[q] = ldDev4(jointRsrc(w), j * uint32_t(sizeof(Joint)) + 16u * q);
+  }
+  return k;
+}
+template <int M> AVBD_D void storeJoint(const DeviceWorld &w, uint32_t j, const Joint &k) {
+  if constexpr (M == SOLVE_DATAFLOW) {
+    const float4 *mut = &k.penLin;
+#pragma unroll
+    for (uint32_t q = 0; q < JOINT_MUTABLE_RECORDS; q++) stDev4(jointRsrc(w), j * uint32_t(sizeof(Joint)) + 16u * q, mut[q]);
+  } else {
+    w.joints[j] = k;
+  }
+}
+template <int M> AVBD_D JointPoses jointPosesFor(const DeviceWorld &w, int4 f, uint32_t i, float4 pos, float4 rot) {
+  auto pose = [&](int x, float4 &p, float4 &q) {
+    if (uint32_t(x) == i) {
+      p = pos;
+      q = rot;
+    } else if constexpr (M == SOLVE_DATAFLOW) {
+      p = ldDev4(bodyRsrc(w), bodyOff(uint32_t(x), OFF_POS));
+      q = ldDev4(bodyRsrc(w), bodyOff(uint32_t(x), OFF_ROT));
+    } else {
+      p = w.bodies[x].pos;
+      q = w.bodies[x].rot;
+    }
+  };
+  JointPoses P;
+  P.world = f.y < 0;
+  if (P.world) {
+    P.posA = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.rotA = make_float4(0.f, 0.f, 0.f, 1.f);
+  } else {
+    pose(f.y, P.posA, P.rotA);
+  }
+  pose(f.z, P.posB, P.rotB);
+  return P;
 }
 
 // --- primal --------------------------------------------------------------------------------------------------------
@@ -318,15 +386,16 @@ AVBD_D void addContactRows(Acc &s, const Basis &b, float sg, f3 r, f3 k, f3 f) {
                fmaf(r.x, g.y, fmaf(-r.y, g.x, s.rAng.z)));
 }
 
-AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, uint32_t i) {
-  int4 f = w.info[j];
-  int a = f.y, b = f.z;
-  const Joint k = w.joints[j];
+/** A joint's rows for body i (addJoint, wgsl-solve.ts:90-140; addAngular, angularConstraints.js:36-44), from its record
+ * and its bodies' poses. */
+AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, int4 f, const Joint &k, const JointPoses &P, float alpha,
+                     uint32_t i) {
+  const int a = f.y, b = f.z;
   bool isA = int(i) == a;
   float sg = isA ? 1.f : -1.f;
-  if (f.x == T_MOTOR || f.x == T_LIMIT) {  // (addAngular, angularConstraints.js:36-44)
+  if (f.x == T_MOTOR || f.x == T_LIMIT) {
     if (k.penLin.w == 0.f || (f.x == T_MOTOR && k.rB.w == 0.f)) return;
-    const HingeFrame hf = hingeFrame(w, k, a, b);
+    const HingeFrame hf = hingeFrame(k, P);
     const f3 axis = scale(hf.axis, isA ? -1.f : 1.f);
     const float2 C = hingeRows(k, f.x, hingeAngle(hf), alpha, w.worlds[w.bodyWorld[b]].dt);
     const float2 F = hingeForce(k, f.x, C);
@@ -335,9 +404,9 @@ AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, ui
     return;
   }
   if (f.x == T_SPRING) {
-    f3 rAW = qrotate(w.bodies[a].rot, xyz(k.rA));
-    f3 rBW = qrotate(w.bodies[b].rot, xyz(k.rB));
-    f3 d = sub(add(rAW, xyz(w.bodies[a].pos)), add(rBW, xyz(w.bodies[b].pos)));
+    f3 rAW = qrotate(P.rotA, xyz(k.rA));
+    f3 rBW = qrotate(P.rotB, xyz(k.rB));
+    f3 d = sub(add(rAW, xyz(P.posA)), add(rBW, xyz(P.posB)));
     float len = length(d);
     if (len <= 1.0e-6f) return;
     f3 n = divs(d, len);
@@ -348,8 +417,8 @@ AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, ui
   }
   f3 penLin = xyz(k.penLin), penAng = xyz(k.penAng);
   if (dot(penLin, penLin) > 0.f) {
-    f3 F = add(mul3(penLin, jointLinRows(w, k, a, b, alpha)), xyz(k.lamLin));
-    f3 r = isA ? qrotate(w.bodies[a].rot, xyz(k.rA)) : neg(qrotate(w.bodies[b].rot, xyz(k.rB)));
+    f3 F = add(mul3(penLin, jointLinRows(k, P, alpha)), xyz(k.lamLin));
+    f3 r = isA ? qrotate(P.rotA, xyz(k.rA)) : neg(qrotate(P.rotB, xyz(k.rB)));
     f3 v = neg(r);
     addRow(acc, mk3(sg, 0.f, 0.f), mk3(0.f, -v.z, v.y), penLin.x, F.x);
     addRow(acc, mk3(0.f, sg, 0.f), mk3(v.z, 0.f, -v.x), penLin.y, F.y);
@@ -360,12 +429,50 @@ AVBD_D void addJoint(const DeviceWorld &w, Acc &acc, uint32_t j, float alpha, ui
     acc.a22 += length(sub(scale(r, F.z), mk3(0.f, 0.f, rf)));
   }
   if (dot(penAng, penAng) > 0.f) {
-    f3 F = add(mul3(penAng, jointAngRows(w, k, a, b, alpha)), xyz(k.lamAng));
+    f3 F = add(mul3(penAng, jointAngRows(k, P, alpha)), xyz(k.lamAng));
     float s = sg * k.lamAng.w;
     addRow(acc, mk3(0.f, 0.f, 0.f), mk3(s, 0.f, 0.f), penAng.x, F.x);
     addRow(acc, mk3(0.f, 0.f, 0.f), mk3(0.f, s, 0.f), penAng.y, F.y);
     addRow(acc, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, s), penAng.z, F.z);
   }
+}
+
+/**
+ * The dual of a joint on its record (dual, wgsl-solve.ts:474-535 for joints; a spring's fracture; a motor's or limit's
+ * ramp) at its bodies' poses; true when the record changed.
+ */
+AVBD_D bool dualJointRecord(const DeviceWorld &w, int4 f, Joint &k, const JointPoses &P) {
+  if (f.x == T_SPRING) return springFracture(k, P);
+  const WorldParams &p = w.worlds[w.bodyWorld[f.z]];
+  if (f.x == T_MOTOR || f.x == T_LIMIT) return dualHinge(p, k, f, P);
+  if (f.x != T_JOINT) return false;
+  float alpha = p.alpha;
+  f3 penLin = xyz(k.penLin), penAng = xyz(k.penAng);
+  if (dot(penLin, penLin) > 0.f) {
+    f3 C = jointLinRows(k, P, alpha);
+    if (k.penLin.w >= HARD) k.lamLin = mk4(add(mul3(penLin, C), xyz(k.lamLin)), k.lamLin.w);
+    float cap = rmin(k.penLin.w, PENALTY_MAX);
+    f3 q = add(penLin, scale(vabs(C), p.betaLin));
+    k.penLin = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penLin.w);
+  }
+  if (dot(penAng, penAng) > 0.f) {
+    f3 C = jointAngRows(k, P, alpha);
+    if (k.penAng.w >= HARD) k.lamAng = mk4(add(mul3(penAng, C), xyz(k.lamAng)), k.lamAng.w);
+    float cap = rmin(k.penAng.w, PENALTY_MAX);
+    f3 q = add(penAng, scale(vabs(C), p.betaAng));
+    k.penAng = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penAng.w);
+  }
+  float frac = k.lamLin.w;
+  float limit = fabsf(frac);
+  f3 ll = xyz(k.lamLin), la = xyz(k.lamAng);
+  bool linear = frac < 0.f && dot(ll, ll) > limit * limit;
+  if (limit < BIG && (dot(la, la) > limit * limit || linear)) {
+    k.penLin = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.penAng = make_float4(0.f, 0.f, 0.f, 0.f);
+    k.lamLin = make_float4(0.f, 0.f, 0.f, frac);
+    k.lamAng = make_float4(0.f, 0.f, 0.f, k.lamAng.w);
+  }
+  return true;
 }
 
 /** What a contact needs of one body: rotation, displacement since x- (wgsl-solve.ts:145-157), its index. */
@@ -442,12 +549,16 @@ AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &bas
 AVBD_D bool finishesPair(uint32_t ca, uint32_t cb) { return ca > cb; }
 
 /** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
-template <int M>
-AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, uint4 head, uint4 tail,
-                    uint32_t p, float alpha, uint32_t flags, const DfCtx &df) {
-  if (head.z & ENTRY_JOINT) {
-    addJoint(w, acc, head.y, alpha, i);  // the dataflow solve runs only for batches without joints
-    return;
+template <int M, bool J>
+AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, float4 selfPos, uint4 head,
+                    uint4 tail, uint32_t p, float alpha, uint32_t flags, const DfCtx &df) {
+  if constexpr (J) {  // J: the batch has joints (their code is compiled out otherwise)
+    if (head.z & ENTRY_JOINT) {
+      const uint32_t j = head.y;
+      const int4 f = w.info[j];
+      addJoint(w, acc, f, loadJoint<M>(w, j), jointPosesFor<M>(w, f, i, selfPos, self.rot), alpha, i);
+      return;
+    }
   }
   constexpr bool DF = M == SOLVE_DATAFLOW;
   const bool isA = (head.z & ENTRY_IS_A) != 0;
@@ -471,7 +582,7 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
  * group's prefix sums and takes the record from the lane that loaded it. A manifold with many points is thus spread
  * over several lanes. The partial systems are then summed by a fixed tree, so the result does not depend on scheduling.
  */
-template <uint32_t W, int M>
+template <uint32_t W, int M, bool J>
 AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_t end, uint32_t lane, float alpha,
                        float dt, uint32_t flags, Acc &acc, const DfCtx &df, float4 &bpos, float4 &brot) {
   constexpr bool DF = M == SOLVE_DATAFLOW;
@@ -511,7 +622,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       // comes first, else of the last one. The entry's multipliers come from the last iteration's dual, which the
       // pair's finisher ran right after its primal: this body (its own stamp was waited for) or the partner, whose
       // dual stamp then covers both.
-      if (e < end && (head.z & ENTRY_DYNAMIC) && !(head.z & ENTRY_JOINT)) {
+      if (e < end && (head.z & ENTRY_DYNAMIC)) {
         const uint32_t pc = head.z >> ENTRY_COLOR_SHIFT;  // the partner's colour (annotated after the colouring)
         if (pc < df.color)
           waitFor(w, &w.bodyDone[head.x], df.base + df.it + 1u);
@@ -541,7 +652,7 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W),
                                   __shfl(tail.w, owner, W));
       const uint32_t p = t - __shfl(excl, owner, W);
-      if (t < total) addItem<M>(w, acc, i, self, h, tl, p, alpha, flags, df);
+      if (t < total) addItem<M, J>(w, acc, i, self, bpos, h, tl, p, alpha, flags, df);
     }
   }
 }
@@ -708,7 +819,7 @@ template <int M> AVBD_D void dualUpdate(const DeviceWorld &w, uint32_t c, const 
  * for by the primal), and updated as the dual pass does. The items are spread over the group's lanes as in
  * accumulate. Then the body's dual stamp.
  */
-template <uint32_t W, int M>
+template <uint32_t W, int M, bool J>
 AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 pos, float4 rot, const DfCtx &df) {
   constexpr bool DF = M == SOLVE_DATAFLOW;
   const uint32_t i = slot.x, first = slot.y, end = slot.z;
@@ -721,10 +832,11 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
     if (e < end) {
       head = w.adjEntry[2ull * e];
       tail = w.adjEntry[2ull * e + 1];
-      if (!(head.z & ENTRY_JOINT) &&
-          (!(head.z & ENTRY_DYNAMIC) ||
-           finishesPair(df.color, DF ? head.z >> ENTRY_COLOR_SHIFT : w.colorA[head.x])))
-        items = head.z & 15u;
+      const bool finishes = !(head.z & ENTRY_DYNAMIC) ||
+                            finishesPair(df.color, DF ? head.z >> ENTRY_COLOR_SHIFT : w.colorA[head.x]);
+      // A joint's dual is one item here in the dataflow solve (the workgroup-per-world solve runs the joints' duals
+      // after each iteration's colours)
+      if (finishes) items = (head.z & ENTRY_JOINT) ? (DF && J ? 1u : 0u) : (head.z & 15u);
     }
     uint32_t incl = items;
 #pragma unroll
@@ -748,7 +860,14 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
                             __uint_as_float(__shfl(tail.z, owner, W)));
       const float tscale = __uint_as_float(__shfl(tail.w, owner, W));
       const uint32_t pt = t - __shfl(excl, owner, W);
-      if (t < total) {
+      if (J && t < total && (h.z & ENTRY_JOINT)) {
+        if constexpr (J) {
+          const uint32_t j = h.y;
+          const int4 f = w.info[j];
+          Joint k = loadJoint<M>(w, j);
+          if (dualJointRecord(w, f, k, jointPosesFor<M>(w, f, i, pos, rot))) storeJoint<M>(w, j, k);
+        }
+      } else if (t < total) {
         // The partner is static or of an earlier colour: its pose of this iteration was waited for by the primal
         const bool isA = (h.z & ENTRY_IS_A) != 0;
         const PairBody other = pairBody<M>(w, h.x);
@@ -774,7 +893,7 @@ constexpr uint32_t PRIMAL_BLOCK = 256;
 constexpr uint32_t WAVE = DF_WAVE;
 
 /** Bucket position k (of a colour half ending at `end`) solved by a group of W lanes; `lane` is the lane in the group. */
-template <uint32_t W, int M = SOLVE_COLORS>
+template <uint32_t W, int M = SOLVE_COLORS, bool J = true>
 AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t lane, const DfCtx &df = DfCtx{}) {
   const bool live = k < end;
   const uint4 slot = live ? w.colorSlot[k] : make_uint4(0u, 0u, 0u, 0u);
@@ -782,7 +901,7 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
   float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f), brot = bpos;
   if (live) {
     const WorldParams &p = w.worlds[slot.w];
-    accumulate<W, M>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df, bpos, brot);
+    accumulate<W, M, J>(w, slot.x, slot.y, slot.z, lane, p.alpha, p.dt, p.flags, acc, df, bpos, brot);
   } else {
     float *a = reinterpret_cast<float *>(&acc);
     for (int q = 0; q < int(sizeof(Acc) / sizeof(float)); q++) a[q] = 0.f;
@@ -794,7 +913,7 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
     // The new pose from the group's lane 0 to all of its lanes
     pos = make_float4(__shfl(pos.x, 0, W), __shfl(pos.y, 0, W), __shfl(pos.z, 0, W), __shfl(pos.w, 0, W));
     rot = make_float4(__shfl(rot.x, 0, W), __shfl(rot.y, 0, W), __shfl(rot.z, 0, W), __shfl(rot.w, 0, W));
-    if (live) fusedDual<W, M>(w, slot, lane, pos, rot, df);
+    if (live) fusedDual<W, M, J>(w, slot, lane, pos, rot, df);
   } else {
     if (live && lane == 0) finishBody<M>(w, slot.x, acc, df, bpos, brot);
   }
@@ -805,7 +924,7 @@ AVBD_D void primalGroup(const DeviceWorld &w, uint32_t k, uint32_t end, uint32_t
  * two to a wave (HEAVY_LANES lanes each), then its light bodies, eight to a wave (LANES lanes each), so a body with many
  * contacts does not hold back the seven bodies sharing its wave.
  */
-__global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint32_t color) {
+template <bool J> __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint32_t color) {
   // The hold flags and the colour's ranges are read together (neither waits for the other)
   const uint32_t overflow = w.counters[C_OVERFLOW], clashes = w.counters[C_CLASHES];
   const uint32_t heavyStart = w.colorStart[2 * color];
@@ -820,55 +939,21 @@ __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_primal(DeviceWorld w, uint3
   // u is uniform across a wave, so every lane of a group takes the same turns (the shuffles need whole groups)
   for (uint32_t u = blockIdx.x * wavesPerBlock + threadIdx.x / WAVE; u < waves; u += gridDim.x * wavesPerBlock) {
     if (u < heavyWaves)
-      primalGroup<HEAVY_LANES>(w, heavyStart + u * heavyPerWave + lane / HEAVY_LANES, lightStart, lane % HEAVY_LANES);
+      primalGroup<HEAVY_LANES, SOLVE_COLORS, J>(w, heavyStart + u * heavyPerWave + lane / HEAVY_LANES, lightStart,
+                                                lane % HEAVY_LANES);
     else
-      primalGroup<LANES>(w, lightStart + (u - heavyWaves) * lightPerWave + lane / LANES, lightEnd, lane % LANES);
+      primalGroup<LANES, SOLVE_COLORS, J>(w, lightStart + (u - heavyWaves) * lightPerWave + lane / LANES, lightEnd,
+                                          lane % LANES);
   }
 }
 
 // --- dual --------------------------------------------------------------------------------------------------------------
 
+/** Joint j's dual in the dual pass (plain memory, the poses in the records). */
 AVBD_D void dualJoint(const DeviceWorld &w, uint32_t j) {
-  int4 f = w.info[j];
-  if (f.x == T_SPRING) {
-    springFracture(w, j, f);
-    return;
-  }
-  if (f.x == T_MOTOR || f.x == T_LIMIT) {
-    dualHinge(w, j, f);
-    return;
-  }
-  if (f.x != T_JOINT) return;
-  int a = f.y, b = f.z;
-  const WorldParams &p = w.worlds[w.bodyWorld[b]];
-  float alpha = p.alpha;
+  const int4 f = w.info[j];
   Joint k = w.joints[j];
-  f3 penLin = xyz(k.penLin), penAng = xyz(k.penAng);
-  if (dot(penLin, penLin) > 0.f) {
-    f3 C = jointLinRows(w, k, a, b, alpha);
-    if (k.penLin.w >= HARD) k.lamLin = mk4(add(mul3(penLin, C), xyz(k.lamLin)), k.lamLin.w);
-    float cap = rmin(k.penLin.w, PENALTY_MAX);
-    f3 q = add(penLin, scale(vabs(C), p.betaLin));
-    k.penLin = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penLin.w);
-  }
-  if (dot(penAng, penAng) > 0.f) {
-    f3 C = jointAngRows(w, k, a, b, alpha);
-    if (k.penAng.w >= HARD) k.lamAng = mk4(add(mul3(penAng, C), xyz(k.lamAng)), k.lamAng.w);
-    float cap = rmin(k.penAng.w, PENALTY_MAX);
-    f3 q = add(penAng, scale(vabs(C), p.betaAng));
-    k.penAng = make_float4(rmin(q.x, cap), rmin(q.y, cap), rmin(q.z, cap), k.penAng.w);
-  }
-  float frac = k.lamLin.w;
-  float limit = fabsf(frac);
-  f3 ll = xyz(k.lamLin), la = xyz(k.lamAng);
-  bool linear = frac < 0.f && dot(ll, ll) > limit * limit;
-  if (limit < BIG && (dot(la, la) > limit * limit || linear)) {
-    k.penLin = make_float4(0.f, 0.f, 0.f, 0.f);
-    k.penAng = make_float4(0.f, 0.f, 0.f, 0.f);
-    k.lamLin = make_float4(0.f, 0.f, 0.f, frac);
-    k.lamAng = make_float4(0.f, 0.f, 0.f, k.lamAng.w);
-  }
-  w.joints[j] = k;
+  if (dualJointRecord(w, f, k, jointPoses(w, f.y, f.z))) w.joints[j] = k;
 }
 
 /** Point k of manifold m: every point is a thread of its own (the points are independent of each other). */
@@ -929,7 +1014,7 @@ __global__ void k_velocities(DeviceWorld w) {
  * exactly what the colour-by-colour launches and the dual pass do. Every wave of the grid is resident at once (the
  * grid is sized by the occupancy), so the unit everything else waits for always runs; and every wait is bounded.
  */
-__global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld w) {
+template <bool J> __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld w) {
   if (held(w)) return;
   const DataflowPlan &plan = *w.dfPlan;
   const uint32_t lane = threadIdx.x % WAVE;
@@ -948,10 +1033,11 @@ __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld 
                    lightEnd = w.colorStart[2 * c + 2];
     const uint32_t hw = plan.heavyWaves[c];
     if (v < hw)
-      primalGroup<HEAVY_LANES, SOLVE_DATAFLOW>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES, lightStart,
-                                     lane % HEAVY_LANES, df);
+      primalGroup<HEAVY_LANES, SOLVE_DATAFLOW, J>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES,
+                                                  lightStart, lane % HEAVY_LANES, df);
     else
-      primalGroup<LANES, SOLVE_DATAFLOW>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd, lane % LANES, df);
+      primalGroup<LANES, SOLVE_DATAFLOW, J>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd,
+                                            lane % LANES, df);
   }
 }
 
@@ -986,7 +1072,7 @@ __global__ void k_world_ranges(DeviceWorld w, uint32_t *keyStart) {
  * duals follow the iteration's last colour. At the end, the world's velocities. The same arithmetic in the same order
  * as the colour launches and the dual pass, so the same bits; nothing crosses a workgroup, so there are no waits.
  */
-template <uint32_t BS>
+template <uint32_t BS, bool J>
 __global__ void __launch_bounds__(BS, 4)
     k_solve_worlds(DeviceWorld w, const uint32_t *order, const uint32_t *keyStart, const uint32_t *jointStart,
                    uint32_t iterations) {
@@ -1006,10 +1092,10 @@ __global__ void __launch_bounds__(BS, 4)
       const uint32_t units = hw + (le - ls + lightPerWave - 1) / lightPerWave;
       for (uint32_t u = wave; u < units; u += waves) {  // uniform across a wave
         if (u < hw)
-          primalGroup<HEAVY_LANES, SOLVE_WORLDS>(w, hs + u * heavyPerWave + lane / HEAVY_LANES, he, lane % HEAVY_LANES,
-                                                 ctx);
+          primalGroup<HEAVY_LANES, SOLVE_WORLDS, J>(w, hs + u * heavyPerWave + lane / HEAVY_LANES, he,
+                                                    lane % HEAVY_LANES, ctx);
         else
-          primalGroup<LANES, SOLVE_WORLDS>(w, ls + (u - hw) * lightPerWave + lane / LANES, le, lane % LANES, ctx);
+          primalGroup<LANES, SOLVE_WORLDS, J>(w, ls + (u - hw) * lightPerWave + lane / LANES, le, lane % LANES, ctx);
       }
       __syncthreads();
     }
@@ -1020,6 +1106,20 @@ __global__ void __launch_bounds__(BS, 4)
   }
   const WorldParams &p = w.worlds[world];
   for (uint32_t i = p.firstBody + threadIdx.x; i < p.firstBody + p.bodyCount; i += BS) bodyVelocity(w, i);
+}
+
+/**
+ * For the dataflow solve: the iterations' duals of the joints no dynamic body lists (both ends static or asleep, or the
+ * world and a static body). Their bodies do not move during the solve and nothing else reads them, so their
+ * `iterations` duals run back to back, as the dual pass would run them once per iteration.
+ */
+__global__ void k_dual_static_joints(DeviceWorld w, uint32_t iterations) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= w.nJoints || held(w)) return;
+  const int4 f = w.info[j];
+  const bool dynA = f.y >= 0 && w.bodies[f.y].size.w > 0.f, dynB = w.bodies[f.z].size.w > 0.f;
+  if (dynA || dynB) return;
+  for (uint32_t it = 0; it < iterations; it++) dualJoint(w, j);
 }
 
 // --- launch wrappers ---------------------------------------------------------------------------------------------------
@@ -1037,7 +1137,9 @@ void launch_warm_start(const DeviceWorld &w, hipStream_t s) {
   k_warm_bodies<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);
 }
 void launch_primal(const DeviceWorld &w, uint32_t color, uint32_t maxColorBodies, hipStream_t s) {
-  k_primal<<<blocks(uint64_t(maxColorBodies) * LANES, PRIMAL_BLOCK, 4096), PRIMAL_BLOCK, 0, s>>>(w, color);
+  const uint32_t grid = blocks(uint64_t(maxColorBodies) * LANES, PRIMAL_BLOCK, 4096);
+  if (w.nJoints) k_primal<true><<<grid, PRIMAL_BLOCK, 0, s>>>(w, color);
+  else k_primal<false><<<grid, PRIMAL_BLOCK, 0, s>>>(w, color);
 }
 void launch_dual(const DeviceWorld &w, uint32_t manifoldCapacity, hipStream_t s) {
   k_dual<<<blocks(uint64_t(w.nJoints) + uint64_t(manifoldCapacity) * 2, 256, 16384), 256, 0, s>>>(w);
@@ -1046,23 +1148,31 @@ uint32_t dataflow_grid() {
   int device = 0, perCu = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void *>(k_solve_dataflow),
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void *>(k_solve_dataflow<true>),
                                                    PRIMAL_BLOCK, 0) != hipSuccess)
     return 0;
   return uint32_t(perCu) * uint32_t(prop.multiProcessorCount);
 }
-void launch_dataflow(const DeviceWorld &w, uint32_t grid, hipStream_t s) {
-  k_solve_dataflow<<<grid, PRIMAL_BLOCK, 0, s>>>(w);
+void launch_dataflow(const DeviceWorld &w, uint32_t grid, uint32_t iterations, hipStream_t s) {
+  if (w.nJoints) k_solve_dataflow<true><<<grid, PRIMAL_BLOCK, 0, s>>>(w);
+  else k_solve_dataflow<false><<<grid, PRIMAL_BLOCK, 0, s>>>(w);
+  if (w.nJoints) k_dual_static_joints<<<blocks(w.nJoints, 256), 256, 0, s>>>(w, iterations);
+}
+template <bool J>
+static void launch_worlds(const DeviceWorld &w, const uint32_t *order, const uint32_t *keyStart,
+                          const uint32_t *jointStart, uint32_t iterations, uint32_t block, hipStream_t s) {
+  switch (block) {
+    case 128: k_solve_worlds<128, J><<<w.nWorlds, 128, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    case 512: k_solve_worlds<512, J><<<w.nWorlds, 512, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    case 1024: k_solve_worlds<1024, J><<<w.nWorlds, 1024, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+    default: k_solve_worlds<256, J><<<w.nWorlds, 256, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
+  }
 }
 void launch_solve_worlds(const DeviceWorld &w, const uint32_t *order, uint32_t *keyStart, const uint32_t *jointStart,
                          uint32_t iterations, uint32_t block, hipStream_t s) {
   k_world_ranges<<<blocks(uint64_t(w.nWorlds + 1u) * 2u * w.colorCap, 256), 256, 0, s>>>(w, keyStart);
-  switch (block) {
-    case 128: k_solve_worlds<128><<<w.nWorlds, 128, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
-    case 512: k_solve_worlds<512><<<w.nWorlds, 512, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
-    case 1024: k_solve_worlds<1024><<<w.nWorlds, 1024, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
-    default: k_solve_worlds<256><<<w.nWorlds, 256, 0, s>>>(w, order, keyStart, jointStart, iterations); break;
-  }
+  if (jointStart) launch_worlds<true>(w, order, keyStart, jointStart, iterations, block, s);
+  else launch_worlds<false>(w, order, keyStart, jointStart, iterations, block, s);
 }
 void launch_velocities(const DeviceWorld &w, hipStream_t s) {
   k_velocities<<<blocks(w.nBodies, 256), 256, 0, s>>>(w);

@@ -358,7 +358,7 @@ __global__ void k_entry_colors(DeviceWorld w) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= umin(w.adjStart[w.nBodies], w.adjCapacity)) return;
   const uint4 head = w.adjEntry[2ull * e];
-  if ((head.z & ENTRY_DYNAMIC) && !(head.z & ENTRY_JOINT))
+  if (head.z & ENTRY_DYNAMIC)
     w.adjEntry[2ull * e].z = (head.z & ((1u << ENTRY_COLOR_SHIFT) - 1u)) |
                              ((w.colorA[head.x] & 0xffu) << ENTRY_COLOR_SHIFT);
 }

@@ -278,13 +278,13 @@ def batched_worlds(avbd, steps, warmup, sample, target_p99_ms=5.0, whole_steps=1
     }
 
 
-def jointed_mix(avbd, steps, warmup):
+def jointed_mix(avbd, steps, warmup, solver="dataflow"):
     """The jointed + contact mix: a 64 x 64 net of 0.2 m boxes on hard ball joints dropped onto 2,000 static spheres
-    and 20,000 free capsules; dt 1/120, 20 iterations, one world (H-PLOC broadphase)."""
+    and 20,000 free capsules; dt 1/120, 20 iterations, one world (H-PLOC broadphase, the dataflow solve)."""
     scene = avbd.configs.jointed_mix()
     n = scene.body_count
     world = avbd.createSolverPipeline("hip").createSolver(
-        scene, {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True})
+        scene, {"colorCap": 16, "colorRounds": 16, "graph": True, "adaptive": True, "solver": solver})
     done = 0
     while done < warmup:
         k = min(50, warmup - done)
@@ -304,7 +304,8 @@ def jointed_mix(avbd, steps, warmup):
                         "ground, dt 1/120, 20 iterations", "bodies": n, "joints": len(scene._joints),
             "steps_timed": f"{warmup + 1}-{warmup + steps}", "body_steps_per_s": n * steps / elapsed,
             "ms_per_step": elapsed * 1e3 / steps, "step_ms_p50": float(np.percentile(ms, 50)),
-            "step_ms_p99": float(np.percentile(ms, 99)), "counters": counters, "phases_ms": prof}
+            "step_ms_p99": float(np.percentile(ms, 99)), "solver": solver, "counters": counters,
+            "phases_ms": prof}
 
 
 def sleeping_pile(avbd, steps, warmup, solver, graph=True):
@@ -537,7 +538,7 @@ def main():
         cpu = cpu_baseline(scene_fn, state0, first, args.cpu_steps, args.cpu_budget, 64)
 
     world.destroy()
-    mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup)
+    mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup, args.solver)
     sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, first, args.solver)
     served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
                                                   args.serve_active)

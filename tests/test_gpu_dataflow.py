@@ -74,13 +74,48 @@ def test_batch_worlds_match_their_solo_runs(avbd):
     batch.destroy()
 
 
-def test_joints_fall_back_to_colour_launches(avbd):
+def test_jointed_mix_same_bits_as_colour_launches(avbd):
+    """Ball joints, spheres and capsules: the dataflow solve (joint duals fused into the unit that finishes each joint)
+    steps to the same bits as the colour launches and the dual pass, bodies and joint records."""
     mix = lambda: avbd.configs.jointed_mix(n=6, spheres=30, capsules=20, area=3.0, capsule_grid=(4, 1, 5))  # noqa: E731
     a = _world(avbd, mix(), solver="dataflow")
     b = _world(avbd, mix(), solver="colors")
-    a.step(30)
-    b.step(30)
-    assert np.array_equal(a.readBodies(), b.readBodies())
+    assert a.solverKind() == 2
+    for _ in range(5):
+        a.step(30)
+        b.step(30)
+        assert np.array_equal(a.readBodies().view(np.uint32), b.readBodies().view(np.uint32))
+        assert np.array_equal(a.readJoints().view(np.uint32), b.readJoints().view(np.uint32))
+        assert a.readCounters()["solveAborts"] == 0
+    a.destroy()
+    b.destroy()
+
+
+def test_pivot_fallback_matches_colour_launches(avbd):
+    """A chain of 8 g boxes on hard joints warm-started at penalty 1e8: the FP64 redo happens for the same bodies in
+    the dataflow solve as in the colour launches, with the same bits."""
+    def chain():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        for i in range(6):
+            s.addBox([0.2, 0.2, 0.2], position=[0.2 * i + 0.1, 5, 0])
+        s.addJoint(None, 0, anchorA=[0, 5, 0], anchorB=[-0.1, 0, 0])
+        for i in range(5):
+            s.addJoint(i, i + 1, anchorA=[0.1, 0, 0], anchorB=[-0.1, 0, 0])
+        for rec, _ in s._joints:
+            rec[0:3] = 1e8
+        return s
+    a = _world(avbd, chain(), solver="dataflow")
+    b = _world(avbd, chain(), solver="colors")
+    total = 0
+    for _ in range(10):
+        a.step()
+        b.step()
+        fa, fb = a.readCounters()["pivotFallbacks"], b.readCounters()["pivotFallbacks"]
+        assert fa == fb
+        total += fa
+        assert np.array_equal(a.readBodies().view(np.uint32), b.readBodies().view(np.uint32))
+        assert np.array_equal(a.readJoints().view(np.uint32), b.readJoints().view(np.uint32))
+    assert total > 0
     a.destroy()
     b.destroy()
 

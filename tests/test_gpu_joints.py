@@ -3,7 +3,9 @@ with their rest frames, hinges (two ball joints, a limit, a motor), world motors
 and the edits between steps (setMotor, setWorldAnchor, disableConstraint, releaseJoints, setSpringMaterial,
 setAngularConstraint). The frames each world captures at creation equal the oracle's bit for bit; then every step,
 from the GPU's state on both sides, matches (pairs, keys, colours bit-exact; poses, velocities, contact and joint
-multipliers within 1e-5), and a run without reseeding stays bit-identical."""
+multipliers within 1e-5), and a run without reseeding stays bit-identical — with the colour launches, the dataflow
+solve (joint duals fused into the finishing body's unit; joints without a dynamic body dualled after it) and one
+workgroup per world."""
 import math
 
 import numpy as np
@@ -56,6 +58,8 @@ def scene(avbd):
     a = s.addBox([0.6, 0.6, 0.6], position=[1.5, 0.31, 3], rotation=_quat([0, 1, 0], 0.2))
     b = s.addBox([0.6, 0.6, 0.6], position=[1.5, 0.92, 3], rotation=_quat([1, 0, 0], 0.3))
     slots["weld"] = s.addJoint(a, b, [0, 0.3, 0], [0, -0.3, 0], angularStiffness=math.inf)
+    # a joint between two static bodies (no dynamic body lists it: its duals still run every iteration)
+    slots["static_pair"] = s.addJoint(post, frame, [0, -0.2, 0], [0, 1.0, 0], linearStiffness=500)
     for x in np.linspace(-5, 5, 6):  # a few loose boxes to land on things
         s.addBox([0.5, 0.5, 0.5], position=[float(x), 3.0, 1.0])
     return s, slots
@@ -101,9 +105,15 @@ def test_frames_captured_at_creation_match_the_oracle(avbd):
     world.destroy()
 
 
-def test_constraint_steps_and_edits_match_the_oracle(avbd):
+SOLVERS = ["colors", "dataflow", "worlds"]
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_constraint_steps_and_edits_match_the_oracle(avbd, solver):
     s, slots = scene(avbd)
-    world = avbd.createSolverPipeline("hip").createSolver(s, OPTS)
+    world = avbd.createSolverPipeline("hip").createSolver(s, dict(OPTS, solver=solver))
+    assert world.solverKind() == SOLVERS.index(solver) + 1
+    print(f"constraints vs oracle: {solver}", flush=True)
     oracle = OracleWorld([scene(avbd)[0]], precision=32, color_cap=32, color_rounds=16)
     for step in range(90):
         _edit(world, oracle, step, slots)
@@ -120,9 +130,11 @@ def test_constraint_steps_and_edits_match_the_oracle(avbd):
     world.destroy()
 
 
-def test_constraint_trajectory_bit_exact(avbd):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_constraint_trajectory_bit_exact(avbd, solver):
     s, slots = scene(avbd)
-    world = avbd.createSolverPipeline("hip").createSolver(s, OPTS)
+    world = avbd.createSolverPipeline("hip").createSolver(s, dict(OPTS, solver=solver))
+    print(f"constraint trajectory: {solver}", flush=True)
     oracle = OracleWorld([scene(avbd)[0]], precision=32, color_cap=32, color_rounds=16)
     for step in range(90):
         _edit(world, oracle, step, slots)
@@ -131,7 +143,9 @@ def test_constraint_trajectory_bit_exact(avbd):
     got = world.readBodies()
     want = oracle.bodies().astype(np.float32)
     assert np.array_equal(got[:, 0:8], want[:, 0:8]), f"max pose difference {np.abs(got[:, :8] - want[:, :8]).max()}"
-    assert np.array_equal(world.readJoints().view(np.uint32), oracle.joints().astype(np.float32).view(np.uint32))
+    joints = world.readJoints()
+    assert np.array_equal(joints.view(np.uint32), oracle.joints().astype(np.float32).view(np.uint32))
+    assert joints[slots["static_pair"], 0] == 500.0, "the static pair's duals ramped its penalty to its stiffness"
     world.destroy()
 
 
