@@ -83,6 +83,7 @@ class Options(ctypes.Structure):
         ("solver", ctypes.c_uint32),
         ("coloring_rounds", ctypes.c_uint32),
         ("world_block", ctypes.c_uint32),
+        ("dataflow_share", ctypes.c_uint32),
     ]
 
 

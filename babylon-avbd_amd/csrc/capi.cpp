@@ -108,6 +108,8 @@ EngineOptions toOptions(const AvbdOptions *o) {
         o->world_block != 1024)
       throw std::invalid_argument("world_block must be 0, 128, 256, 512 or 1024");
     e.worldBlock = o->world_block;
+    if (o->dataflow_share > 1024) throw std::invalid_argument("dataflow_share must be at most 1024");
+    e.dataflowShare = o->dataflow_share;
     if (o->coloring_rounds > 2)
       throw std::invalid_argument("coloring_rounds must be 0 (auto), 1 (one launch per round) or 2 (one block)");
     e.coloringRounds = o->coloring_rounds;

@@ -37,6 +37,8 @@ struct EngineOptions {
                                 // 1 one launch per colour, 2 dataflow (one launch for all iterations; batches without
                                 // joints, one engine on the device at a time), 3 one workgroup per world
   uint32_t worldBlock = 0;      // threads per world's workgroup with solver 3 (128, 256, 512, 1024; 0: 256)
+  uint32_t dataflowShare = 0;   // solver 2: the dataflow grid takes 1/share of the device's resident workgroups (0, 1:
+                                // all), so `share` engines' solves can run at once without waiting on one another
   uint32_t coloringRounds = 0;  // 0 auto (listed while few bodies are left pending, with adaptive), 1 one launch per
                                 // round, 2 always listed: one block over the pending bodies (same colours)
 };

@@ -225,6 +225,9 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
     throw std::invalid_argument("world_block must be 0, 128, 256, 512 or 1024");
   if (opt.solver == 2) {
     dfGrid_ = dataflow_grid();
+    // A share of the device: engines whose solves run together (each on its own stream) must not hold more
+    // workgroups than fit at once, or each would wait for its own workgroups that cannot start
+    if (opt.dataflowShare > 1) dfGrid_ = std::max<uint32_t>(1, dfGrid_ / opt.dataflowShare);
     useDataflow_ = dfGrid_ > 0;
   }
   wantDataflow_ = useDataflow_;

@@ -132,7 +132,8 @@ napi_value DeviceCount(napi_env env, napi_callback_info) {
  * ignore: Uint32Array, filters: Uint32Array | null, dt, gravity, up: [3], alpha, gamma, betaLin, betaAng, iterations,
  * flags}]; options = {colorCap, colorRounds, pairCapacity, graph, device, adaptive, broadphase (0 auto, 1 grid,
  * 2 hploc), bvhRebuildInterval, solver (0 auto, 1 colour launches, 2 dataflow, 3 workgroup per world), coloringRounds
- * (0 auto, 1 launches, 2 listed), worldBlock (threads per world's workgroup: 128, 256, 512, 1024; 0: 256)}. Returns
+ * (0 auto, 1 launches, 2 listed), worldBlock (threads per world's workgroup: 128, 256, 512, 1024; 0: 256), dataflowShare (the dataflow grid's share
+ * of the device: 1/dataflowShare; 0: all)}. Returns
  * the world handle (avbd_batch_create).
  */
 napi_value CreateWorld(napi_env env, napi_callback_info info) {
@@ -202,6 +203,7 @@ napi_value CreateWorld(napi_env env, napi_callback_info info) {
   o.solver = uint32_t(numberProp(env, argv[1], "solver", 0));
   o.coloring_rounds = uint32_t(numberProp(env, argv[1], "coloringRounds", 0));
   o.world_block = uint32_t(numberProp(env, argv[1], "worldBlock", 0));
+  o.dataflow_share = uint32_t(numberProp(env, argv[1], "dataflowShare", 0));
   avbd_world *w = nullptr;
   AVBD_OK_OR_THROW(avbd_batch_create(descs.data(), n, &o, &w));
   napi_value handle;

@@ -509,6 +509,7 @@ function nativeOptions(options) {
     bvhRebuildInterval: (o.bvh && o.bvh.rebuildInterval) || 0,
     solver: solverCode(o.solver),
     worldBlock: o.worldBlock || 0,
+    dataflowShare: o.dataflowShare || 0,
   };
 }
 

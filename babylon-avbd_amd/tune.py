@@ -4,9 +4,9 @@ once, and which solve kernel each batch runs.
 One batch steps every world in one launch per phase: the most work per launch, but a world's step then lasts as long
 as the whole set's. Splitting the worlds into G batches stepped one after another keeps the aggregate rate close
 (every batch still fills the device with its share) while each world's step lasts about 1/G as long. The solve
-kernel is the other axis: one launch per colour (many small launches), the dataflow solve (one persistent launch;
-it needs the device to itself, so its batches run one at a time) or one workgroup per world (one launch, each world
-on one compute unit, with the workgroup size a further choice). The tuner times a few steps of each candidate from
+kernel is the other axis: one launch per colour (many small launches), the dataflow solve (one persistent launch whose
+workgroups must all fit on the device at once, so batches stepped together each take a share of it) or one workgroup
+per world (one launch, each world on one compute unit, with the workgroup size a further choice). The tuner times a few steps of each candidate from
 the same starting state and keeps the fastest whose per-step latency (device time of one batch's step, HIP events)
 meets the target at the 99th percentile.
 """
@@ -40,13 +40,16 @@ def run_groups(worlds, steps, concurrency=1):
 
 
 #: Candidate (batches, concurrency, solve options) for the batched workload: colour launches at several splits,
-#: the dataflow solve one batch at a time, one workgroup per world at two workgroup sizes.
+#: the dataflow solve one batch at a time or several at once (each grid a matching share of the device), one workgroup
+#: per world at two workgroup sizes.
 CANDIDATES = (
     (16, 2, {"solver": "colors"}),
     (32, 4, {"solver": "colors"}),
     (64, 8, {"solver": "colors"}),
     (8, 1, {"solver": "dataflow"}),
     (16, 1, {"solver": "dataflow"}),
+    (16, 2, {"solver": "dataflow", "dataflowShare": 2}),
+    (32, 4, {"solver": "dataflow", "dataflowShare": 4}),
     (32, 4, {"solver": "worlds", "worldBlock": 256}),
     (32, 4, {"solver": "worlds", "worldBlock": 1024}),
 )
@@ -55,7 +58,11 @@ CANDIDATES = (
 def describe(solve):
     """A short name for a candidate's solve options."""
     name = solve.get("solver", "auto")
-    return f"{name}/{solve['worldBlock']}" if "worldBlock" in solve else name
+    if "worldBlock" in solve:
+        name += f"/{solve['worldBlock']}"
+    if "dataflowShare" in solve:
+        name += f" (1/{solve['dataflowShare']} of the device each)"
+    return name
 
 
 def batch_groups(pipeline, make_scenes, options, target_p99_ms=5.0, candidates=CANDIDATES, warmup=60, probe_steps=20,

@@ -65,6 +65,10 @@ def _options(options):
     if block not in (0, 128, 256, 512, 1024):
         raise ValueError(f"worldBlock must be 128, 256, 512 or 1024, not {block}")
     o.world_block = block
+    share = int(options.get("dataflowShare", 0))
+    if not 0 <= share <= 1024:
+        raise ValueError(f"dataflowShare must be between 0 and 1024, not {share}")
+    o.dataflow_share = share
     rounds = options.get("coloringRounds", "auto")
     if rounds not in ("auto", "launches", "listed"):
         raise ValueError(f"Unknown coloringRounds: {rounds}")

@@ -98,6 +98,9 @@ typedef struct AvbdOptions {
                                1 one launch per Jones–Plassmann round, 2 always one block over the pending bodies
                                (the same colours either way) */
   uint32_t world_block;     /* threads per world's workgroup with solver 3: 128, 256, 512 or 1024 (0: 256) */
+  uint32_t dataflow_share;  /* solver 2: the solve's grid takes 1/dataflow_share of the device's resident workgroups
+                               (0 or 1: all of them). Worlds whose dataflow solves run at the same time on their own
+                               streams need shares that add up to at most the device */
 } AvbdOptions;
 
 /** readCounters() (solver.ts:1429-1432; world.js:672-692). overflow bits: 1 pairs, 2 contacts, 4 manifolds. */
