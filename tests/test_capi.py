@@ -87,3 +87,18 @@ def test_mass_properties_match_the_reference(avbd):
     # a capsule of zero segment is a sphere: I = 2/5 m r²
     m, (ir, ia, _), _ = avbd.capsule_mass(0.5, 1.0, 1.0)
     assert abs(ia - 0.4 * m * 0.25) < 1e-12 and abs(ir - ia) < 1e-12
+
+
+def test_solver_options_are_checked(avbd):
+    """The solve kernel options the facade passes through the C ABI (solver, worldBlock, dataflowShare)."""
+    opts = avbd.world._options
+    assert opts({"solver": "worlds", "worldBlock": 512}).solver == 3
+    assert opts({"solver": "worlds", "worldBlock": 512}).world_block == 512
+    assert opts({"solver": "dataflow", "dataflowShare": 4}).dataflow_share == 4
+    assert opts({}).solver == 0 and opts({}).world_block == 0 and opts({}).dataflow_share == 0
+    with pytest.raises(ValueError, match="solver"):
+        opts({"solver": "bogus"})
+    with pytest.raises(ValueError, match="worldBlock"):
+        opts({"worldBlock": 300})
+    with pytest.raises(ValueError, match="dataflowShare"):
+        opts({"dataflowShare": -1})
