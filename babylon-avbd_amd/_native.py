@@ -109,6 +109,10 @@ class ServerOptions(ctypes.Structure):
         ("color_rounds", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
         ("broadphase", ctypes.c_uint32),
+        ("small_world_bodies", ctypes.c_uint32),
+        ("small_world_block", ctypes.c_uint32),
+        ("large_world_bodies", ctypes.c_uint32),
+        ("large_world_share", ctypes.c_uint32),
     ]
 
 
@@ -118,7 +122,7 @@ class ServerStats(ctypes.Structure):
                [(k, ctypes.c_double) for k in
                 ("wall_seconds", "step_ms_p50", "step_ms_p99", "step_ms_max", "request_ms_p50", "request_ms_p99")] + \
                [("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
-                ("resident_worlds", ctypes.c_uint32), ("worlds", ctypes.c_uint32)]
+                ("resident_worlds", ctypes.c_uint32), ("worlds", ctypes.c_uint32), ("solve_aborts", ctypes.c_uint64)]
 
 
 class Params(ctypes.Structure):

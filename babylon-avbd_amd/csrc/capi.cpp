@@ -804,6 +804,14 @@ int avbd_server_create(const AvbdServerOptions *options, avbd_server **out) {
       if (options->color_rounds) o.engine.colorRounds = options->color_rounds;
       if (options->broadphase > 2) throw std::invalid_argument("broadphase must be 0 (auto), 1 (grid) or 2 (hploc)");
       o.engine.broadphase = options->broadphase;
+      o.smallWorldBodies = options->small_world_bodies;
+      const uint32_t b = options->small_world_block;
+      if (b != 0 && b != 128 && b != 256 && b != 512 && b != 1024)
+        throw std::invalid_argument("small_world_block must be 0, 128, 256, 512 or 1024");
+      if (b) o.smallWorldBlock = b;
+      o.largeWorldBodies = options->large_world_bodies;
+      if (options->large_world_share > 1024) throw std::invalid_argument("large_world_share must be at most 1024");
+      if (options->large_world_share) o.largeWorldShare = options->large_world_share;
     }
     if (o.engine.colorCap > 64) throw std::invalid_argument("color_cap must be at most 64");
     auto s = std::make_unique<avbd_server>();
@@ -900,6 +908,7 @@ int avbd_server_stats(avbd_server *server, AvbdServerStats *out) {
     out->peak_resident_bytes = s.peakResidentBytes;
     out->resident_worlds = s.residentWorlds;
     out->worlds = s.worlds;
+    out->solve_aborts = s.solveAborts;
     return AVBD_OK;
   });
 }

@@ -176,7 +176,19 @@ void Server::makeResident(World &w) {
     one[0].bodies = w.saved.bodies;
     one[0].joints = w.saved.joints;
   }
-  w.engine = std::make_unique<Engine>(one, opt_.engine);
+  // A small world's iterations in one workgroup: one launch instead of one per colour and iteration, so the device's
+  // few hardware queues carry more worlds' work at once (the same results either way)
+  EngineOptions eo = opt_.engine;
+  if (w.scene.bodies.size() <= opt_.smallWorldBodies) {
+    eo.solver = 3;
+    eo.worldBlock = opt_.smallWorldBlock;
+  } else if (opt_.largeWorldBodies && w.scene.bodies.size() >= opt_.largeWorldBodies) {
+    // A large world's iterations in one persistent launch holding a share of the device's workgroups: as many of
+    // them fit at once as the shares allow, so solves running together never wait for workgroups that cannot start
+    eo.solver = 2;
+    eo.dataflowShare = opt_.largeWorldShare;
+  }
+  w.engine = std::make_unique<Engine>(one, eo);
   if (w.hasSaved) {
     w.engine->restore(w.saved);
     w.saved = SavedState{};
@@ -283,10 +295,18 @@ void Server::loop() {
         }
         if (r->retired == r->steps) finished.push_back(r);
       }
+      // A finished request's world reports whether a dataflow wait gave up in its steps (reading the counters also
+      // switches that world to colour launches)
+      uint64_t aborts = 0;
+      for (Request *r : finished) {
+        Engine &e = *worlds_[r->world]->engine;
+        if (e.solverKind() == 2) aborts += e.readCounters().solveAborts;
+      }
       if (!stepTimes.empty() || !finished.empty()) {
         const double t = now();
         std::lock_guard<std::mutex> lk(mu_);
         stepMs_.insert(stepMs_.end(), stepTimes.begin(), stepTimes.end());
+        counters_.solveAborts += aborts;
         for (Request *r : finished) {
           World &w = *worlds_[r->world];
           r->done = true;

@@ -26,11 +26,16 @@ struct ServerOptions {
   uint32_t lookahead = 2;                  // steps in flight per active request
   int device = 0;
   EngineOptions engine;                    // per-world options (not adaptive: a run must not depend on timing)
+  uint32_t smallWorldBodies = 0;           // worlds of at most this many bodies: one workgroup solves each step
+  uint32_t smallWorldBlock = 1024;         // its threads
+  uint32_t largeWorldBodies = 0;           // worlds of at least this many bodies (0: none): the dataflow solve, ...
+  uint32_t largeWorldShare = 4;            // ... each grid this share of the device
 };
 
 struct ServerStats {
   uint64_t submitted = 0, completed = 0, admissions = 0, evictions = 0, restores = 0;
   uint64_t steps = 0, bodySteps = 0;
+  uint64_t solveAborts = 0;                // dataflow waits given up (read as each request finishes)
   double wallSeconds = 0;                  // since the first submission (or the last reset) until the last completion
   float stepMsP50 = 0, stepMsP99 = 0, stepMsMax = 0;
   double requestMsP50 = 0, requestMsP99 = 0;

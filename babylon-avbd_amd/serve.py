@@ -21,7 +21,7 @@ from .world import _scene_desc
 
 class Server:
     def __init__(self, hbmBudget=200 << 30, maxActive=32, lookahead=2, device=0, colorCap=16, colorRounds=16,
-                 broadphase="auto"):
+                 broadphase="auto", smallWorldBodies=0, smallWorldBlock=1024, largeWorldBodies=0, largeWorldShare=4):
         lib = _native.load()
         o = _native.ServerOptions()
         o.hbm_budget_bytes = int(hbmBudget)
@@ -34,6 +34,12 @@ class Server:
         if broadphase not in ("auto", "grid", "hploc"):
             raise ValueError(f"Unknown broadphase: {broadphase}")
         o.broadphase = {"auto": 0, "grid": 1, "hploc": 2}[broadphase]
+        if smallWorldBlock not in (128, 256, 512, 1024):
+            raise ValueError(f"smallWorldBlock must be 128, 256, 512 or 1024, not {smallWorldBlock}")
+        o.small_world_bodies = int(smallWorldBodies)
+        o.small_world_block = int(smallWorldBlock)
+        o.large_world_bodies = int(largeWorldBodies)
+        o.large_world_share = int(largeWorldShare)
         h = ctypes.c_void_p()
         check(lib.avbd_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._lib, self._h, self.destroyed = lib, h, False

@@ -422,6 +422,12 @@ typedef struct AvbdServerOptions {
   uint32_t lookahead;        /* steps in flight per running request (0: 2) */
   int32_t device;
   uint32_t color_cap, color_rounds, use_graph, broadphase; /* per world, as in AvbdOptions (0: defaults, graph on) */
+  uint32_t small_world_bodies; /* worlds of at most this many bodies are solved by one workgroup (solver 3, workgroups
+                                  of small_world_block threads); 0: every world by colour launches */
+  uint32_t small_world_block;  /* 128, 256, 512 or 1024 (0: 1024) */
+  uint32_t large_world_bodies; /* worlds of at least this many bodies run the dataflow solve (solver 2), each grid
+                                  1/large_world_share of the device; 0: none */
+  uint32_t large_world_share;  /* at least the number of such solves that can run at once (0: 4, the hardware queues) */
 } AvbdServerOptions;
 
 typedef struct AvbdServerStats {
@@ -431,6 +437,8 @@ typedef struct AvbdServerStats {
   double request_ms_p50, request_ms_p99;  /* submission to completion */
   uint64_t resident_bytes, peak_resident_bytes;
   uint32_t resident_worlds, worlds;
+  uint64_t solve_aborts; /* dataflow waits given up in served steps (read when each request finishes; a world that gave
+                            one up runs colour launches from then on) */
 } AvbdServerStats;
 
 int avbd_server_create(const AvbdServerOptions *options, avbd_server **out);
