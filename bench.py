@@ -338,14 +338,14 @@ def sleeping_pile(avbd, steps, warmup, solver, graph=True):
             "asleep_at_start": asleep_before, "asleep_at_end": asleep_after}
 
 
-def serving(avbd, requests, rate, pool, max_active, small_world=0, large_world=0):
+def serving(avbd, requests, rate, pool, max_active, small_world=0, large_world=0, lookahead=2):
     """The serving workload: Poisson arrivals (`rate` per second) of rollout requests (U{1..120} steps) for a pool of
     worlds of 16..50,000 boxes (log-uniform sizes, the single large world's body distributions), admitted by the
     serving loop against a 200 GB HBM budget and run concurrently (one stream per world). Worlds of at most
     `small_world` bodies are solved by one workgroup each (one launch per step's solve)."""
     sizes = avbd.configs.serving_pool(pool)
-    server = avbd.serve.Server(hbmBudget=200 * 10**9, maxActive=max_active, smallWorldBodies=small_world,
-                               largeWorldBodies=large_world)
+    server = avbd.serve.Server(hbmBudget=200 * 10**9, maxActive=max_active, lookahead=lookahead,
+                               smallWorldBodies=small_world, largeWorldBodies=large_world)
     ids = [server.addWorld(avbd.configs.boxes_world(int(n), avbd.configs.SEED + 100 + k)) for k, n in enumerate(sizes)]
     warm = [server.submit(w, 1) for w in ids]  # admission and graph capture of every world
     for r in warm:
@@ -374,7 +374,7 @@ def serving(avbd, requests, rate, pool, max_active, small_world=0, large_world=0
             "step_ms_p50": st["step_ms_p50"], "step_ms_p99": st["step_ms_p99"], "step_ms_max": st["step_ms_max"],
             "request_ms_p50": st["request_ms_p50"], "request_ms_p99": st["request_ms_p99"],
             "evictions": st["evictions"], "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
-            "max_active": max_active, "small_world_bodies": small_world, "large_world_bodies": large_world,
+            "max_active": max_active, "lookahead": lookahead, "small_world_bodies": small_world, "large_world_bodies": large_world,
             "solve_aborts": st["solve_aborts"]}
 
 
@@ -408,6 +408,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-pool", type=int, default=64)
     ap.add_argument("--serve-active", type=int, default=32)
+    ap.add_argument("--serve-lookahead", type=int, default=2, help="steps in flight per running request")
     ap.add_argument("--serve-small-world", type=int, default=512,
                     help="served worlds of at most this many bodies are solved one workgroup each")
     ap.add_argument("--serve-large-world", type=int, default=4096,
@@ -548,7 +549,8 @@ def main():
     mix = None if args.no_mix else jointed_mix(avbd, args.mix_steps, args.mix_warmup, args.solver)
     sleeping = None if args.no_sleeping else sleeping_pile(avbd, args.steps, first, args.solver)
     served = None if args.no_serving else serving(avbd, args.serve_requests, args.serve_rate, args.serve_pool,
-                                                  args.serve_active, args.serve_small_world, args.serve_large_world)
+                                                  args.serve_active, args.serve_small_world, args.serve_large_world,
+                                                  args.serve_lookahead)
     batched = None
     if not args.no_batched:
         batched = batched_worlds(avbd, args.batch_steps, args.batch_warmup, args.batch_sample)
