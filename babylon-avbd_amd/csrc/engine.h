@@ -34,8 +34,9 @@ struct EngineOptions {
   uint32_t broadphase = 0;      // 0 auto (H-PLOC for one world of >= 10,000 bodies, else grid), 1 grid, 2 H-PLOC
   uint32_t rebuildInterval = 64;  // H-PLOC: rebuild every this many steps, refit in between
   uint32_t solver = 0;          // 0 auto (batches of small worlds: one workgroup per world; else colour launches),
-                                // 1 one launch per colour, 2 dataflow (one launch for all iterations; batches without
-                                // joints, one engine on the device at a time), 3 one workgroup per world
+                                // 1 one launch per colour, 2 dataflow (one launch for all iterations; its grid must
+                                // fit the device beside the other dataflow solves running: dataflowShare), 3 one
+                                // workgroup per world
   uint32_t worldBlock = 0;      // threads per world's workgroup with solver 3 (128, 256, 512, 1024; 0: 256)
   uint32_t dataflowShare = 0;   // solver 2: the dataflow grid takes 1/share of the device's resident workgroups (0, 1:
                                 // all), so `share` engines' solves can run at once without waiting on one another

@@ -90,8 +90,8 @@ typedef struct AvbdOptions {
   uint32_t bvh_rebuild_interval; /* H-PLOC rebuild period in steps, refit in between (0: 64; gpuHploc.js:62-80) */
   uint32_t solver;          /* 0 auto (a batch of worlds of at most 256 bodies each: 3; otherwise 1), 1 one launch
                                per colour and a dual launch per iteration, 2 dataflow: every iteration's colours and
-                               duals in one launch, each body waiting only for its neighbours (batches without
-                               joints; needs the device to itself), 3 one workgroup per world: each world's
+                               duals in one launch, each body waiting only for its neighbours (its workgroups must
+                               all fit on the device at once: see dataflow_share), 3 one workgroup per world: each world's
                                iterations, colours, duals and velocities in one workgroup, every world of the batch in
                                one launch (the same result from each) */
   uint32_t coloring_rounds; /* 0 auto (one block over the pending bodies while few are left pending, with `adaptive`),
