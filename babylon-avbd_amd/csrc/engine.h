@@ -31,7 +31,7 @@ struct EngineOptions {
   bool useGraph = false;
   bool adaptive = false;        // adapt colour cap/rounds and storage from counters read back asynchronously
   int device = 0;
-  uint32_t broadphase = 0;      // 0 auto (H-PLOC for one world of >= 10,000 bodies, else grid), 1 grid, 2 H-PLOC
+  uint32_t broadphase = 0;      // 0 auto (H-PLOC for a world or batch of >= 10,000 bodies, else grid), 1 grid, 2 H-PLOC
   uint32_t rebuildInterval = 64;  // H-PLOC: rebuild every this many steps, refit in between
   uint32_t solver = 0;          // 0 auto (batches of small worlds: one workgroup per world; else colour launches),
                                 // 1 one launch per colour, 2 dataflow (one launch for all iterations; its grid must

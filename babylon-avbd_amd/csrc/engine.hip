@@ -217,7 +217,7 @@ Engine::Engine(const std::vector<HostWorld> &worlds, const EngineOptions &opt) {
   listedRounds_ = coloringRounds_ == 2;
   if (opt.broadphase == 2) useBvh_ = true;
   else if (opt.broadphase == 1) useBvh_ = false;
-  else useBvh_ = nWorlds_ == 1 && nBodies_ >= 10000;
+  else useBvh_ = nBodies_ >= 10000;  // one world or a batch (each world its own tree) of 10,000 bodies or more
   if (opt.solver > 3)
     throw std::invalid_argument("solver must be 0 (auto), 1 (colour launches), 2 (dataflow) or 3 (workgroup per world)");
   if (opt.worldBlock != 0 && opt.worldBlock != 128 && opt.worldBlock != 256 && opt.worldBlock != 512 &&

@@ -569,7 +569,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded)",
         "config": {"workload": workload, "bodies": n_bodies, "iterations": iters, "dt": scene.params["dt"],
-                   "broadphase": {"auto": "auto (H-PLOC BVH: one world of >= 10,000 bodies)", "grid": "uniform grid",
+                   "broadphase": {"auto": "auto (H-PLOC BVH: a world or batch of >= 10,000 bodies)", "grid": "uniform grid",
                                   "hploc": "H-PLOC BVH"}[args.broadphase],
                    "steps_timed": f"{first + 1}-{first + args.steps}",
                    "window": f"the settled pile: steps 1-{args.settle} settle and {args.warmup} warm-up steps run "

@@ -85,7 +85,7 @@ typedef struct AvbdOptions {
   uint32_t use_graph;       /* replay each step from a captured hipGraph */
   int32_t device;           /* HIP device ordinal */
   uint32_t adaptive;        /* adapt colour cap/rounds and storage from counters read back without waiting */
-  uint32_t broadphase;      /* 0 auto (H-PLOC for one world of >= 10,000 bodies), 1 uniform grid, 2 H-PLOC BVH
+  uint32_t broadphase;      /* 0 auto (H-PLOC for a world or batch of >= 10,000 bodies), 1 uniform grid, 2 H-PLOC BVH
                                (AppGpuSolver3D broadphase "auto" | "grid" | "hploc", appGpuSolver3D.js:333-402) */
   uint32_t bvh_rebuild_interval; /* H-PLOC rebuild period in steps, refit in between (0: 64; gpuHploc.js:62-80) */
   uint32_t solver;          /* 0 auto (a batch of worlds of at most 256 bodies each: 3; otherwise 1), 1 one launch

@@ -170,3 +170,20 @@ def test_pivot_fallback_counts_match(avbd):
     assert total > 0
     a.destroy()
     b.destroy()
+
+
+def test_batch_hploc_equals_grid(avbd):
+    """A batch stepped with the H-PLOC broadphase (a tree per world; auto for batches of 10,000 bodies or more) finds
+    the grid's pairs every step and steps to the same bits."""
+    scenes = lambda: _uneven_piles(avbd, count=24, seed=9)  # noqa: E731
+    a = _batch(avbd, scenes(), "colors", broadphase="grid")
+    b = _batch(avbd, scenes(), "colors", broadphase="hploc", bvh={"rebuildInterval": 7})
+    for _ in range(40):
+        a.step()
+        b.step()
+        pa = {tuple(p) for p in a.readPairs().tolist()}
+        pb = {tuple(p) for p in b.readPairs().tolist()}
+        assert pa == pb
+    assert np.array_equal(a.readBodies().view(np.uint32), b.readBodies().view(np.uint32))
+    a.destroy()
+    b.destroy()
