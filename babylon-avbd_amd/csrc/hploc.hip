@@ -416,64 +416,114 @@ template <class F> AVBD_D void walkTree(const DeviceWorld &w, const Bvh &t, uint
   }
 }
 
+/** Lanes per query in the candidate pass: one per entry of a quad. */
+constexpr uint32_t WALK_LANES = 4;
+
+/** The votes of this lane's group (lanes 4g..4g+3 of the wavefront) as bits 0..3, lane 4g + k at bit k. */
+AVBD_D uint32_t groupVote(bool b) {
+  return uint32_t(__ballot(b) >> (__lane_id() & ~(WALK_LANES - 1u))) & ((1u << WALK_LANES) - 1u);
+}
+
+/**
+ * walkTree with a quad's four entries on four lanes (the query's group: lanes 4g..4g+3 of the wavefront): each lane
+ * loads its own entry (32 bytes; the group reads the quad in one 128-byte access) and tests it, so a wavefront walks
+ * 16 trees and four times as many walks are in flight as with a lane per query. The group takes the same path as
+ * walkTree: leaf entries visited in entry order, the first internal one descended into, the others pushed in entry
+ * order onto the group's stack (in LDS, entry d at stack[d * groups + group]). visit(leaf, j, rank, count) is called
+ * on all four lanes for each quad: leaf says whether this lane's entry is a leaf hit, j its body, rank the number of
+ * leaf hits before it in the quad, count the quad's leaf hits (the same on every lane); the callback may use ballots.
+ */
+template <class F>
+AVBD_D void walkTreeLanes(const DeviceWorld &w, const Bvh &t, uint32_t i, uint32_t q, F &&visit) {
+  const BvhNode self = t.nodes[i];
+  const f3 qlo = xyz(self.lo), qhi = xyz(self.hi);
+  const uint32_t root = t.roots[w.bodyWorld[i]];
+  const uint32_t N = w.nBodies;
+  if (root < N) return;
+  const uint32_t sub = threadIdx.x % WALK_LANES, first = __lane_id() - sub, below = (1u << sub) - 1u;
+  const uint32_t groups = blockDim.x / WALK_LANES;
+  uint32_t *stack = bvhStack + threadIdx.x / WALK_LANES;
+  uint32_t at = root, sp = 0;
+  // Each internal node is entered at most once; the bound only guards against a malformed tree
+  for (uint32_t guard = 0; guard < t.nodeCount; guard++) {
+    const BvhQuad *quad = t.quads + (at - N);
+    const float4 lo = quad->lo[sub], hi = quad->hi[sub];
+    const uint32_t mp = __float_as_uint(hi.w), tag = __float_as_uint(lo.w);
+    const bool hit = !(hi.x < qlo.x || hi.y < qlo.y || hi.z < qlo.z || lo.x > qhi.x || lo.y > qhi.y || lo.z > qhi.z) &&
+                     ((mp & MAXPOS_STATIC) || (mp & ~MAXPOS_STATIC) > q);
+    const bool leaf = hit && (tag & LEAF) != 0, inner = hit && (tag & LEAF) == 0;
+    const uint32_t leafBits = groupVote(leaf), innerBits = groupVote(inner);
+    visit(leaf, tag & ~LEAF, uint32_t(__popc(leafBits & below)), uint32_t(__popc(leafBits)));
+    if (innerBits != 0u) {
+      const uint32_t head = uint32_t(__builtin_ctz(innerBits));
+      // Internal hits after the first: pushed in entry order, as many as the stack holds
+      if (inner && sub != head) {
+        const uint32_t d = sp + uint32_t(__popc(innerBits & below)) - 1u;
+        if (d < t.stackDepth) stack[d * groups] = tag;
+      }
+      sp = umin(sp + uint32_t(__popc(innerBits)) - 1u, t.stackDepth);
+      at = uint32_t(__shfl(int(tag), int(first + head)));
+    } else {
+      if (sp == 0) break;
+      __builtin_amdgcn_wave_barrier();  // the pushes above (other lanes of the group) before this read
+      at = stack[(--sp) * groups];
+    }
+  }
+}
+
 /**
  * Pass 1: each dynamic body's query walks the tree with the box test only and keeps its first BVH_CAND candidate
- * leaves. Threads take bodies in Morton order (neighbouring threads walk neighbouring paths).
+ * leaves, four lanes per query (walkTreeLanes); then the same four lanes run the pair predicate on the kept
+ * candidates, four at a time, writing the accepted mask (bit k of word r: candidate 32 r + k passed) and the pair
+ * count. A query with more candidates than it keeps walks again, testing as it goes, and only counts. Groups take
+ * bodies in Morton order (neighbouring groups walk neighbouring paths).
  */
 __global__ void k_bvh_cand(DeviceWorld w, Bvh t) {
   if (gatedOff(w)) return;
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) / WALK_LANES;
   if (q >= w.nBodies) return;
-  uint32_t i = t.order[q];
+  const uint32_t sub = threadIdx.x % WALK_LANES;
+  const uint32_t i = t.order[q];
+  uint32_t *stash = t.candStash + size_t(i) * BVH_CAND;
   uint32_t n = 0;
   if (w.bodies[i].size.w > 0.f) {
-    uint32_t *out = t.candStash + size_t(i) * BVH_CAND;
-    walkTree(w, t, i, q, [&](uint32_t j) {
-      if (n < BVH_CAND) out[n] = j;
-      n++;
+    walkTreeLanes(w, t, i, q, [&](bool leaf, uint32_t j, uint32_t rank, uint32_t count) {
+      if (leaf && n + rank < BVH_CAND) stash[n + rank] = j;
+      n += count;
     });
   }
-  t.candCount[i] = n;
-  if (n > BVH_CAND) {  // more candidates than kept: walk again, testing as it goes, and count the pairs
-    uint32_t pairs = 0;
-    walkTree(w, t, i, q, [&](uint32_t j) {
-      if (acceptPair(w, umax(i, j), umin(i, j))) pairs++;
+  uint32_t pairs = 0;
+  if (n <= BVH_CAND) {
+    // The group's stash writes (any of its lanes) before its reads
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    uint32_t word = 0;
+    for (uint32_t base = 0; base < n; base += WALK_LANES) {
+      const uint32_t slot = base + sub;
+      bool accept = false;
+      if (slot < n) {
+        const uint32_t j = stash[slot];
+        accept = acceptPair(w, umax(i, j), umin(i, j));
+      }
+      const uint32_t bits = groupVote(accept);
+      word |= bits << (base % 32u);
+      pairs += uint32_t(__popc(bits));
+      const bool full = base % 32u == 32u - WALK_LANES;
+      if (sub == 0 && (full || base + WALK_LANES >= n)) t.candMask[size_t(i) * BVH_CAND_WORDS + base / 32u] = word;
+      if (full) word = 0;
+    }
+  } else {
+    walkTreeLanes(w, t, i, q, [&](bool leaf, uint32_t j, uint32_t, uint32_t) {
+      pairs += uint32_t(__popc(groupVote(leaf && acceptPair(w, umax(i, j), umin(i, j)))));
     });
+  }
+  if (sub == 0) {
+    t.candCount[i] = n;
     w.pairCount[i] = pairs;
   }
 }
 
 /**
- * Pass 2: the pair predicate on every kept candidate, 32 lanes per body (two bodies a wave), 32 candidates a round;
- * each round's ballot is a word of the body's accepted mask and the mask's population its pair count. A body with
- * more candidates than it kept was counted by pass 1.
- */
-__global__ void k_bvh_cand_test(DeviceWorld w, Bvh t) {
-  if (gatedOff(w)) return;
-  const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint32_t i = uint32_t(g / 32), lane = uint32_t(g % 32);
-  const bool live = i < w.nBodies;
-  const uint32_t count = live ? t.candCount[i] : 0u;
-  const uint32_t kept = count <= BVH_CAND ? count : 0u;
-  // Rounds are the same for both bodies of a wave (the ballot needs every lane): the larger of the two
-  const uint32_t rounds = (umax(kept, __shfl_xor(kept, 32)) + 31) / 32;
-  uint32_t pairs = 0;
-  for (uint32_t r = 0; r < rounds; r++) {
-    const uint32_t slot = r * 32 + lane;
-    bool accept = false;
-    if (live && slot < kept) {
-      uint32_t j = t.candStash[size_t(i) * BVH_CAND + slot];
-      accept = acceptPair(w, umax(i, j), umin(i, j));
-    }
-    const uint32_t bits = uint32_t(__ballot(accept) >> (threadIdx.x % 64 / 32 * 32));
-    if (live && lane == 0 && r < BVH_CAND_WORDS) t.candMask[size_t(i) * BVH_CAND_WORDS + r] = bits;
-    pairs += uint32_t(__popc(bits));
-  }
-  if (live && lane == 0 && count <= BVH_CAND) w.pairCount[i] = pairs;
-}
-
-/**
- * Pass 3 (after the scan of the counts): write each body's pairs (a > b) in candidate order; and the pair count,
+ * Pass 2 (after the scan of the counts): write each body's pairs (a > b) in candidate order; and the pair count,
  * clamped to the capacity (overflow bit 0 when it is not).
  */
 __global__ void k_bvh_write(DeviceWorld w, Bvh t) {
@@ -581,8 +631,7 @@ void launch_bvh_pairs(const DeviceWorld &w, const Bvh &tree, hipStream_t s) {
   Bvh t = tree;
   t.gate = w.gate;
   const size_t stackBytes = size_t(64) * t.stackDepth * sizeof(uint32_t);
-  k_bvh_cand<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
-  k_bvh_cand_test<<<blocks(uint64_t(w.nBodies) * 32, 256), 256, 0, s>>>(w, t);
+  k_bvh_cand<<<blocks(uint64_t(w.nBodies) * WALK_LANES, 256), 256, stackBytes, s>>>(w, t);
   launch_scan_exclusive(w.pairCount, w.nBodies + 1, w.scanScratch, s, w.counters + C_DF_ABORT, w.gate);
   k_bvh_write<<<blocks(w.nBodies, 64), 64, stackBytes, s>>>(w, t);
 }

@@ -244,6 +244,32 @@ def test_hploc_teleport_outside_bounds(avbd):
     tree.destroy()
 
 
+def test_hploc_query_with_more_candidates_than_kept(avbd):
+    """A 12 m plate resting on a 24 x 24 carpet of small boxes: its query finds several hundred leaves, more than the
+    128 candidates a query keeps, so the candidate pass counts its pairs in a second walk and the write pass walks
+    again; pair sets still equal the grid's step by step, and the trajectories are the same bits."""
+    def scene():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        s.addBox([30, 1, 30], density=0, position=[0, -0.5, 0])
+        for x in range(24):
+            for z in range(24):
+                s.addBox([0.4, 0.4, 0.4], position=[-5.75 + 0.5 * x, 0.2, -5.75 + 0.5 * z])
+        s.addBox([12, 0.4, 12], position=[0, 0.595, 0])
+        return s
+    grid = _world(avbd, scene(), broadphase="grid")
+    tree = _world(avbd, scene(), broadphase="hploc", bvh={"rebuildInterval": 5})
+    plate = 24 * 24 + 1
+    for s in range(12):
+        grid.step(1)
+        tree.step(1)
+        pairs = _pair_set(tree.readPairs())
+        assert pairs == _pair_set(grid.readPairs()), f"step {s}"
+        assert sum(1 for a, b in pairs if plate in (a, b)) > 128
+    assert np.array_equal(grid.readBodies().view(np.uint32), tree.readBodies().view(np.uint32))
+    grid.destroy()
+    tree.destroy()
+
+
 # --- batches: a world steps to the same bits alone and inside a batch ------------------------------------------------
 
 
