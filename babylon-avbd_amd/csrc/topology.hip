@@ -154,24 +154,45 @@ AVBD_D int neighbour(const DeviceWorld &w, uint32_t e) {
   return (head.z & ENTRY_DYNAMIC) ? int(head.x) : -1;
 }
 
-AVBD_D uint32_t smallestFree(const DeviceWorld &w, uint32_t b, const uint32_t *src, bool skipPending) {
-  uint32_t lo = 0, hi = 0;
-  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1]; e++) {
-    int j = neighbour(w, e);
-    if (j < 0) continue;
-    uint32_t s = src[j];
-    if (skipPending && (s & PENDING)) continue;
-    uint32_t col = s & 0xffu;
-    if (col == NO_COLOR) continue;
-    if (col < 32) lo |= 1u << col;
-    else hi |= 1u << (col - 32);
+/** Neighbours read per batch by the colouring passes. */
+constexpr uint32_t NB_BATCH = 8;
+
+/**
+ * Adjacency entries [e0, min(e0 + NB_BATCH, end)) of body b: the dynamic neighbour of each (-1 for a static partner or
+ * a slot past the list) and its state in `src`. Every slot loads (a slot past the list re-reads the list's last entry,
+ * a slot without a neighbour reads b's own state), so a batch's loads are in flight together rather than one
+ * dependent pair per entry.
+ */
+struct NeighbourBatch {
+  int j[NB_BATCH];
+  uint32_t s[NB_BATCH];
+};
+AVBD_D NeighbourBatch neighbourBatch(const DeviceWorld &w, uint32_t b, uint32_t e0, uint32_t end, const uint32_t *src) {
+  NeighbourBatch n;
+#pragma unroll
+  for (uint32_t q = 0; q < NB_BATCH; q++) {
+    const int j = neighbour(w, umin(e0 + q, end - 1u));
+    n.j[q] = e0 + q < end ? j : -1;
   }
+#pragma unroll
+  for (uint32_t q = 0; q < NB_BATCH; q++) n.s[q] = src[n.j[q] >= 0 ? uint32_t(n.j[q]) : b];
+  return n;
+}
+
+/** Colour bit c of a 64-colour set (lo: colours 0-31, hi: 32-63). */
+AVBD_D void addColour(uint32_t &lo, uint32_t &hi, uint32_t c) {
+  if (c < 32) lo |= 1u << c;
+  else hi |= 1u << (c - 32);
+}
+/** The lowest colour missing from the set; MAX_COLORS - 1 when all 64 are taken. */
+AVBD_D uint32_t lowestMissing(uint32_t lo, uint32_t hi) {
   if (~lo != 0u) return uint32_t(__builtin_ctz(~lo));
   if (~hi != 0u) return 32u + uint32_t(__builtin_ctz(~hi));
   return MAX_COLORS - 1;
 }
 
-// 1. Compaction: a body whose priority beats all neighbours may drop to a smaller free colour (A → B)
+// Step 1 (state A to state B), lowering: a coloured body that outranks every dynamic neighbour moves down to the
+// lowest colour none of them holds, if that is below its own
 __global__ void k_color_compact(DeviceWorld w) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < COLOR_FLAG_WORDS) w.colorFlags[b] = 0u;  // the rounds' flags start clear (current state: colorA)
@@ -185,17 +206,25 @@ __global__ void k_color_compact(DeviceWorld w) {
   if (col >= w.colorCap) col = NO_COLOR;
   if (col != NO_COLOR) {
     bool isMax = true;
-    const uint32_t first = firstOf(w, b);
-    for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
-      int j = neighbour(w, e);
-      if (j >= 0 && !beats(b, uint32_t(j), first)) isMax = false;
+    uint32_t lo = 0, hi = 0;
+    const uint32_t first = firstOf(w, b), end = w.adjStart[b + 1];
+    for (uint32_t e0 = w.adjStart[b]; e0 < end && isMax; e0 += NB_BATCH) {
+      const NeighbourBatch n = neighbourBatch(w, b, e0, end, w.colorA);
+#pragma unroll
+      for (uint32_t q = 0; q < NB_BATCH; q++) {
+        if (n.j[q] < 0) continue;
+        if (!beats(b, uint32_t(n.j[q]), first)) isMax = false;
+        const uint32_t c = n.s[q] & 0xffu;
+        if (c != NO_COLOR) addColour(lo, hi, c);
+      }
     }
-    if (isMax) col = umin(col, smallestFree(w, b, w.colorA, false));
+    if (isMax) col = umin(col, lowestMissing(lo, hi));
   }
   w.colorB[b] = col;
 }
 
-// 2. Pending: no colour, or the same colour as a higher-priority neighbour (B → A)
+// Step 2 (state B to state A), conflicts: a body is left to the rounds (pending, and listed) when it has no colour or
+// shares its colour with a neighbour that outranks it
 __global__ void k_color_mark(DeviceWorld w) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= w.nBodies) return;
@@ -205,10 +234,12 @@ __global__ void k_color_mark(DeviceWorld w) {
     return;
   }
   bool pending = col == NO_COLOR;
-  const uint32_t first = firstOf(w, b);
-  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && !pending; e++) {
-    int j = neighbour(w, e);
-    if (j >= 0 && (w.colorB[j] & 0xffu) == col && beats(uint32_t(j), b, first)) pending = true;
+  const uint32_t first = firstOf(w, b), end = w.adjStart[b + 1];
+  for (uint32_t e0 = w.adjStart[b]; e0 < end && !pending; e0 += NB_BATCH) {
+    const NeighbourBatch n = neighbourBatch(w, b, e0, end, w.colorB);
+#pragma unroll
+    for (uint32_t q = 0; q < NB_BATCH; q++)
+      if (n.j[q] >= 0 && (n.s[q] & 0xffu) == col && beats(uint32_t(n.j[q]), b, first)) pending = true;
   }
   w.colorA[b] = col | (pending ? PENDING : 0u);
   if (pending) {
@@ -217,22 +248,33 @@ __global__ void k_color_mark(DeviceWorld w) {
   }
 }
 
-// 3. Jones–Plassmann round r: a pending body beating all pending neighbours takes the smallest colour its settled
-// neighbours leave free; none left below the cap: the last colour, still pending (a counted clash). A round whose
-// input holds no pending body would copy it unchanged, and so would every later round: it returns at once, and the
-// count pass reads the state from the buffer the last working round wrote (colorFlags[0]).
+// Step 3, the rounds (Jones–Plassmann, reading one state buffer and writing the other): a pending body that outranks
+// each pending neighbour settles on the lowest colour its settled neighbours leave free. With every colour below the
+// cap taken it takes the cap's last colour and stays pending, which is counted as a clash. Once a round's input has no
+// pending body, that round and all later ones would only copy their input: they return at once, and the count pass
+// reads the state from the buffer the last working round wrote (colorFlags[0]).
 /** Body b's state after one round, from the round's input state src. */
 AVBD_D uint32_t roundState(const DeviceWorld &w, const uint32_t *src, uint32_t b) {
   uint32_t s = src[b];
   if (!(s & PENDING)) return s;
   bool isMax = true;
-  const uint32_t first = firstOf(w, b);
-  for (uint32_t e = w.adjStart[b]; e < w.adjStart[b + 1] && isMax; e++) {
-    int j = neighbour(w, e);
-    if (j >= 0 && (src[j] & PENDING) && !beats(b, uint32_t(j), first)) isMax = false;
+  uint32_t lo = 0, hi = 0;  // colours of the settled neighbours
+  const uint32_t first = firstOf(w, b), end = w.adjStart[b + 1];
+  for (uint32_t e0 = w.adjStart[b]; e0 < end && isMax; e0 += NB_BATCH) {
+    const NeighbourBatch n = neighbourBatch(w, b, e0, end, src);
+#pragma unroll
+    for (uint32_t q = 0; q < NB_BATCH; q++) {
+      if (n.j[q] < 0) continue;
+      if (n.s[q] & PENDING) {
+        if (!beats(b, uint32_t(n.j[q]), first)) isMax = false;
+      } else {
+        const uint32_t c = n.s[q] & 0xffu;
+        if (c != NO_COLOR) addColour(lo, hi, c);
+      }
+    }
   }
   if (!isMax) return s;
-  uint32_t fr = smallestFree(w, b, src, true);
+  const uint32_t fr = lowestMissing(lo, hi);
   return fr >= w.colorCap ? ((w.colorCap - 1) | PENDING) : fr;
 }
 
@@ -276,7 +318,8 @@ __global__ void __launch_bounds__(1024) k_color_rounds_listed(DeviceWorld w, uin
   if (threadIdx.x == 0) w.colorFlags[0] = current;
 }
 
-// 4. Per-chunk histogram and each body's rank within its chunk's colour (keeps index order inside a bucket)
+// Step 4, counting: each block of COLOR_WG bodies counts its bodies per bucket and ranks each body among the block's
+// bodies of its bucket, so a bucket lists its bodies in index order
 /** Bucket key of a coloured body: its colour's heavy or light half (engine_types.h, COLOR_KEYS). */
 AVBD_D uint32_t bucketKey(const DeviceWorld &w, uint32_t b, uint32_t col) {
   return 2u * col + (w.adjStart[b + 1] - w.adjStart[b] > LANES ? 0u : 1u);
@@ -314,9 +357,9 @@ __global__ void __launch_bounds__(COLOR_WG) k_color_count(DeviceWorld w) {
   if (blockIdx.x == 0 && lid == 0) w.colorHist[COLOR_KEYS * w.colorGroups] = 0;
 }
 
-// 5. Bucket k starts at its first chunk's offset
-// and the colour count: one more than the highest colour with a body (either of its buckets non-empty); with
-// planIterations > 0 also the dataflow solve's work list
+// Step 5, after the scan of the counts: where each bucket begins (the scanned count of its first block), how many
+// colours are in use (one past the highest colour with a body in either of its buckets) and, when planIterations > 0,
+// the dataflow solve's work list
 __global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w, uint32_t planIterations) {
   __shared__ uint32_t used;
   __shared__ uint32_t starts[COLOR_KEYS + 1];
@@ -338,8 +381,8 @@ __global__ void __launch_bounds__(COLOR_KEYS) k_color_starts(DeviceWorld w, uint
   }
 }
 
-// 6. Scatter: chunk offset for the body's bucket plus its rank; with `warm`, also every body's warm start (the
-// scatter reads no pose, and no joint has to be warm-started first)
+// Step 6, placing: a body's slot is its block's scanned offset in its bucket plus its rank; with `warm`, also every
+// body's warm start (placing reads no pose, and no joint has to be warm-started first)
 template <bool warm> __global__ void __launch_bounds__(COLOR_WG) k_color_scatter(DeviceWorld w) {
   uint32_t b = blockIdx.x * COLOR_WG + threadIdx.x;
   if (b >= w.nBodies) return;
@@ -352,7 +395,7 @@ template <bool warm> __global__ void __launch_bounds__(COLOR_WG) k_color_scatter
   if (warm && !held(w)) warmBody(w, b);
 }
 
-// 7. For the dataflow solve: each dynamic partner's colour into bits 8-15 of its entry's flag word (ENTRY_COLOR_SHIFT),
+// Step 7, for the dataflow solve: each dynamic partner's colour into bits 8-15 of its entry's flag word (ENTRY_COLOR_SHIFT),
 // so the solve reads it with the entry; one thread per entry
 __global__ void k_entry_colors(DeviceWorld w) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
