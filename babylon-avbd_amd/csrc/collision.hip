@@ -89,10 +89,20 @@ __global__ void k_hash_insert(DeviceWorld w) {
   }
 }
 
-AVBD_D int hashFind(const DeviceWorld &w, uint32_t a, uint32_t b) {
-  uint32_t h = pairHash(a, b) & w.hashMask;
+/** The first slot of pair (a, b)'s probe sequence and the value it holds: read ahead of the collision test, so the
+ * load's latency is hidden behind it. */
+struct HashProbe {
+  uint32_t h, v;
+};
+AVBD_D HashProbe hashProbe(const DeviceWorld &w, uint32_t a, uint32_t b) {
+  const uint32_t h = pairHash(a, b) & w.hashMask;
+  return HashProbe{h, __atomic_load_n(&w.table[h], __ATOMIC_RELAXED)};
+}
+/** The previous manifold of pair (a, b), or -1, probing on from the first slot `pr` already read. */
+AVBD_D int hashFind(const DeviceWorld &w, uint32_t a, uint32_t b, HashProbe pr) {
+  uint32_t h = pr.h, v = pr.v;
   for (uint32_t probe = 0; probe <= w.hashMask; probe++) {
-    uint32_t v = __atomic_load_n(&w.table[h], __ATOMIC_RELAXED);
+    if (probe != 0) v = __atomic_load_n(&w.table[h], __ATOMIC_RELAXED);
     if (v == 0) return -1;
     uint4 ids = w.prevManifolds[v - 1].ids;
     if (ids.x == a && ids.y == b) return int(v - 1);
@@ -100,6 +110,7 @@ AVBD_D int hashFind(const DeviceWorld &w, uint32_t a, uint32_t b) {
   }
   return -1;
 }
+AVBD_D int hashFind(const DeviceWorld &w, uint32_t a, uint32_t b) { return hashFind(w, a, b, hashProbe(w, a, b)); }
 
 // --- broadphase ------------------------------------------------------------------------------------------------------
 
@@ -287,7 +298,7 @@ AVBD_D bool reuseContacts(const DeviceWorld &w, const WorldParams &p, uint32_t p
  */
 AVBD_D void writePairContacts(const DeviceWorld &w, const WorldParams &p, uint32_t pslot, uint32_t a, uint32_t b,
                               const Body &BA, const Body &BB, const Box &A, const Box &B, const Found &found, f3 satN,
-                              bool anySphere) {
+                              bool anySphere, HashProbe probe) {
   uint32_t base = pslot * MAX_CONTACTS;
   if (found.count == 0) {
     w.mfSlots[pslot].ids = make_uint4(a, b, base, 0u);
@@ -299,11 +310,16 @@ AVBD_D void writePairContacts(const DeviceWorld &w, const WorldParams &p, uint32
   float4 qA = BA.rot, qB = BB.rot;
   Basis basis = orthonormal(n);
   uint32_t prevFirst = 0, prevCount = 0;
-  int pm = hashFind(w, a, b);
+  int pm = hashFind(w, a, b, probe);
   if (pm >= 0) {
     prevFirst = w.prevManifolds[pm].ids.z;
     prevCount = w.prevManifolds[pm].ids.w & 15u;
   }
+  // The previous points' feature keys, read together once: each new point is matched against them in registers
+  uint32_t prevKey[MAX_CONTACTS];
+#pragma unroll
+  for (uint32_t c = 0; c < uint32_t(MAX_CONTACTS); c++)
+    prevKey[c] = c < prevCount ? fbits(w.prevContacts[prevFirst + c].rA.w) & ~STICK_BIT : 0u;
   bool matchNearest = (p.flags & FLAG_MATCH_NEAREST) != 0;
   float minSide = rmin(rmin(rmin(A.h.x, A.h.y), A.h.z), rmin(rmin(B.h.x, B.h.y), B.h.z)) * 2.f;
   float pmin = contactPenaltyMin(w, p, a, b);
@@ -317,12 +333,10 @@ AVBD_D void writePairContacts(const DeviceWorld &w, const WorldParams &p, uint32
     f3 rB = qrotate(qconj(qB), sub(xB, B.c));
     f3 pen = mk3(0.f, 0.f, 0.f), lam = mk3(0.f, 0.f, 0.f);
     uint32_t stick = 0;
-    int j = -1;
-    for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++)
-      if ((fbits(w.prevContacts[c].rA.w) & ~STICK_BIT) == feature) {
-        j = int(c);
-        break;
-      }
+    int j = -1;  // the first previous point with the same key
+#pragma unroll
+    for (int c = MAX_CONTACTS - 1; c >= 0; c--)
+      if (uint32_t(c) < prevCount && prevKey[c] == feature) j = int(prevFirst) + c;
     if (j < 0 && matchNearest) {
       float bestDist = NEAREST_FRACTION * minSide;
       for (uint32_t c = prevFirst; c < prevFirst + prevCount; c++) {
@@ -370,6 +384,7 @@ template <bool kRounded, bool kHulls> __global__ void __launch_bounds__(NARROW_B
   for (uint32_t pslot = blockIdx.x * blockDim.x + threadIdx.x; pslot < P; pslot += gridDim.x * blockDim.x) {
     uint2 pr = w.pairs[pslot];
     uint32_t a = pr.x, b = pr.y;
+    const HashProbe probe = hashProbe(w, a, b);
     const WorldParams &p = w.worlds[w.bodyWorld[a]];
     if ((p.flags & FLAG_REUSE_CONTACTS) && reuseContacts(w, p, pslot, a, b)) continue;
     const Body &BA = w.bodies[a];
@@ -414,7 +429,8 @@ template <bool kRounded, bool kHulls> __global__ void __launch_bounds__(NARROW_B
     } else {
       found = collideBoxes(A, B, (p.flags & FLAG_FACE_BIAS) != 0, sat);
     }
-    writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB);
+    writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB,
+                      probe);
   }
 }
 
@@ -460,7 +476,8 @@ __global__ void __launch_bounds__(HULL_BLOCK) k_narrowphase_hull(DeviceWorld w) 
       found = collidePoly(w, polyOf(w, a), polyOf(w, b), (p.flags & FLAG_FACE_BIAS) != 0, sat, sc);
     }
     if (laneId() == 0)
-      writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB);
+      writePairContacts(w, p, pslot, a, b, BA, BB, A, B, found, sat.n, sphereA || sphereB || capsuleA || capsuleB,
+                        hashProbe(w, a, b));
     __syncthreads();  // the scratch is free for the next pair
   }
 }
