@@ -17,10 +17,23 @@ namespace avbd {
 // device-scope (sc1) accesses, which are coherent across the XCDs' L2 caches; a writer waits for its stores to complete
 // before it publishes a stamp, and a reader reads only after it has seen the stamp.
 
-/** The dataflow solve's position in the step: iteration, colour of the running primal, the step's stamp base. */
+/**
+ * The dataflow solve's position in the step: iteration, colour of the running primal, the step's stamp base; and the
+ * running group's item stash in LDS (slot t of the group at stash[f * DF_WAVE + t] for word f; null outside the
+ * dataflow solve).
+ */
 struct DfCtx {
   uint32_t it, color, base;
+  float *stash = nullptr;
 };
+
+/**
+ * Words of a stashed contact item: the contact record (16) and the partner's share of the anchors' relative motion
+ * (6). The primal stashes each item of a body's first round (its first W items); the body's fused dual then reads them
+ * there instead of from memory: a pair's finisher is the only writer of its multipliers and its partner does not move
+ * before the finisher's dual stamp, so the stashed values are the ones memory still holds.
+ */
+constexpr uint32_t STASH_WORDS = 22;
 
 /**
  * How a primal pass runs: one launch per colour (plain memory; the dual is a pass of its own), the dataflow solve
@@ -503,28 +516,30 @@ struct Eval {
   float frictionScale, bounds;
 };
 
-/**
- * evalContact (wgsl-solve.ts:171-206) from either body's side: `self` is the body whose rows are wanted (selfIsA: it is
- * the contact's body A), `other` its partner. The rows' C = C0 (1 − α) + J Δx take the displacement once as the
- * relative motion of the two anchors, D = (Δx_self − Δx_other) + (Δθ_self × r_self − Δθ_other × r_other), each row as
- * C0 (1 − α) + sg (l · D) with sg = +1 for A. Exchanging the bodies negates D exactly (a − b = −(b − a) in IEEE
- * rounding), so both sides compute the same C and forces bit for bit, as the dual does from A's side. (The reference
- * sums l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B per row: the same value, rounded differently; the CPU
- * restatement forms it as here.)
- */
-AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction,
-                        const PairBody &self, const PairBody &other, bool selfIsA, float alpha, uint32_t flags) {
-  uint32_t key = fbits(k.rA.w);
-  // Sphere contacts take their lever arms at the step-start rotation (wgsl-solve.ts:172-181): a rolling sphere would
-  // otherwise see a false separation in its normal row. Capsule contacts (keys 4<<24 | j) do so only with
-  // FLAG_CAPSULE_STEP_START_ARMS; by default they use the current rotation, as the reference does
-  const uint32_t feature = key & ~STICK_BIT;
-  const bool rounded = feature == SPHERE_FEATURE ||
-                       ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (feature >> 24) == (CAPSULE_FEATURE >> 24));
+/** Whether contact k's lever arms are taken at the step-start rotations (sphere contacts; capsule contacts with
+ * FLAG_CAPSULE_STEP_START_ARMS) rather than the current ones. */
+AVBD_D bool stepStartArms(const Contact &k, uint32_t flags) {
+  const uint32_t feature = fbits(k.rA.w) & ~STICK_BIT;
+  return feature == SPHERE_FEATURE ||
+         ((flags & FLAG_CAPSULE_STEP_START_ARMS) && (feature >> 24) == (CAPSULE_FEATURE >> 24));
+}
+
+/** The partner's share of the anchors' relative motion D: its displacement, and its Δθ × r at its anchor. */
+struct PartnerTerms {
+  f3 lin, ang;
+};
+AVBD_D PartnerTerms partnerTerms(const DeviceWorld &w, const Contact &k, const PairBody &other, bool selfIsA,
+                                 bool stepStart) {
+  const f3 rOther = rotFused(stepStart ? w.bodies[other.index].irot : other.rot, selfIsA ? xyz(k.rB) : xyz(k.rA));
+  return PartnerTerms{other.dLin, crossFused(other.dAng, rOther)};
+}
+
+/** evalContact with the partner's share of D already formed (partnerTerms). */
+AVBD_D Eval evalContactTerms(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction,
+                             const PairBody &self, const PartnerTerms &o, bool selfIsA, bool stepStart, float alpha) {
   Eval e;
-  e.rSelf = rotFused(rounded ? w.bodies[self.index].irot : self.rot, selfIsA ? xyz(k.rA) : xyz(k.rB));
-  const f3 rOther = rotFused(rounded ? w.bodies[other.index].irot : other.rot, selfIsA ? xyz(k.rB) : xyz(k.rA));
-  const f3 D = add(sub(self.dLin, other.dLin), sub(crossFused(self.dAng, e.rSelf), crossFused(other.dAng, rOther)));
+  e.rSelf = rotFused(stepStart ? w.bodies[self.index].irot : self.rot, selfIsA ? xyz(k.rA) : xyz(k.rB));
+  const f3 D = add(sub(self.dLin, o.lin), sub(crossFused(self.dAng, e.rSelf), o.ang));
   const float sg = selfIsA ? 1.f : -1.f, keep = 1.f - alpha;
   e.C = mk3(fmaf(k.rB.w, keep, sg * dotFused(basis.n, D)), fmaf(k.pen.w, keep, sg * dotFused(basis.t1, D)),
             fmaf(k.lam.w, keep, sg * dotFused(basis.t2, D)));
@@ -542,16 +557,61 @@ AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &bas
 }
 
 /**
+ * evalContact (wgsl-solve.ts:171-206) from either body's side: `self` is the body whose rows are wanted (selfIsA: it is
+ * the contact's body A), `other` its partner. The rows' C = C0 (1 − α) + J Δx take the displacement once as the
+ * relative motion of the two anchors, D = (Δx_self − Δx_other) + (Δθ_self × r_self − Δθ_other × r_other), each row as
+ * C0 (1 − α) + sg (l · D) with sg = +1 for A. Exchanging the bodies negates D exactly (a − b = −(b − a) in IEEE
+ * rounding), so both sides compute the same C and forces bit for bit, as the dual does from A's side. (The reference
+ * sums l·Δx_A − l·Δx_B + (r_A × l)·Δθ_A − (r_B × l)·Δθ_B per row: the same value, rounded differently; the CPU
+ * restatement forms it as here.)
+ * Sphere contacts take their lever arms at the step-start rotation (wgsl-solve.ts:172-181): a rolling sphere would
+ * otherwise see a false separation in its normal row. Capsule contacts (keys 4<<24 | j) do so only with
+ * FLAG_CAPSULE_STEP_START_ARMS; by default they use the current rotation, as the reference does.
+ */
+AVBD_D Eval evalContact(const DeviceWorld &w, const Contact &k, const Basis &basis, float friction,
+                        const PairBody &self, const PairBody &other, bool selfIsA, float alpha, uint32_t flags) {
+  const bool stepStart = stepStartArms(k, flags);
+  return evalContactTerms(w, k, basis, friction, self, partnerTerms(w, k, other, selfIsA, stepStart), selfIsA,
+                          stepStart, alpha);
+}
+
+/**
  * Whether the dynamic body of colour ca finishes its pair with a dynamic body of colour cb in the dataflow solve: the
  * later colour runs the pair's dual right after its primal. Partners never share a colour in a solved step (a step
  * whose colouring clashed is held), so exactly one side finishes each pair.
  */
 AVBD_D bool finishesPair(uint32_t ca, uint32_t cb) { return ca > cb; }
 
-/** One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. */
+/** Stash slot value for an item that is not stashed. */
+constexpr uint32_t NO_STASH = ~0u;
+
+/** Store / load a stashed contact item (slot t of the group; word f at stash[f * DF_WAVE + t]). */
+AVBD_D void stashItem(float *stash, uint32_t t, const Contact &k, const PartnerTerms &o) {
+  const float v[STASH_WORDS] = {k.rA.x,  k.rA.y,  k.rA.z,  k.rA.w,  k.rB.x,  k.rB.y,  k.rB.z,  k.rB.w,
+                                k.pen.x, k.pen.y, k.pen.z, k.pen.w, k.lam.x, k.lam.y, k.lam.z, k.lam.w,
+                                o.lin.x, o.lin.y, o.lin.z, o.ang.x, o.ang.y, o.ang.z};
+#pragma unroll
+  for (uint32_t f = 0; f < STASH_WORDS; f++) stash[f * DF_WAVE + t] = v[f];
+}
+AVBD_D void unstashItem(const float *stash, uint32_t t, Contact &k, PartnerTerms &o) {
+  float v[STASH_WORDS];
+#pragma unroll
+  for (uint32_t f = 0; f < STASH_WORDS; f++) v[f] = stash[f * DF_WAVE + t];
+  k.rA = make_float4(v[0], v[1], v[2], v[3]);
+  k.rB = make_float4(v[4], v[5], v[6], v[7]);
+  k.pen = make_float4(v[8], v[9], v[10], v[11]);
+  k.lam = make_float4(v[12], v[13], v[14], v[15]);
+  o.lin = mk3(v[16], v[17], v[18]);
+  o.ang = mk3(v[19], v[20], v[21]);
+}
+
+/**
+ * One work item of body i: a joint's rows, or the three rows of contact point p of a manifold entry. In the dataflow
+ * solve a contact item with a stash slot (not NO_STASH) is also stashed there for the body's fused dual.
+ */
 template <int M, bool J>
 AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &self, float4 selfPos, uint4 head,
-                    uint4 tail, uint32_t p, float alpha, uint32_t flags, const DfCtx &df) {
+                    uint4 tail, uint32_t p, float alpha, uint32_t flags, const DfCtx &df, uint32_t slot) {
   if constexpr (J) {  // J: the batch has joints (their code is compiled out otherwise)
     if (head.z & ENTRY_JOINT) {
       const uint32_t j = head.y;
@@ -570,7 +630,12 @@ AVBD_D void addItem(const DeviceWorld &w, Acc &acc, uint32_t i, const PairBody &
     k.pen = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_PEN));
     k.lam = ldDev4(contactRsrc(w), contactOff(head.y + p, OFF_LAM));
   }
-  const Eval ev = evalContact(w, k, basis, __uint_as_float(head.w), self, other, isA, alpha, flags);
+  const bool stepStart = stepStartArms(k, flags);
+  const PartnerTerms o = partnerTerms(w, k, other, isA, stepStart);
+  if constexpr (DF) {
+    if (slot != NO_STASH) stashItem(df.stash, slot, k, o);
+  }
+  const Eval ev = evalContactTerms(w, k, basis, __uint_as_float(head.w), self, o, isA, stepStart, alpha);
   addContactRows(acc, basis, isA ? 1.f : -1.f, ev.rSelf, xyz(k.pen), ev.F);
 }
 
@@ -652,7 +717,9 @@ AVBD_D void accumulate(const DeviceWorld &w, uint32_t i, uint32_t first, uint32_
       const uint4 tl = make_uint4(__shfl(tail.x, owner, W), __shfl(tail.y, owner, W), __shfl(tail.z, owner, W),
                                   __shfl(tail.w, owner, W));
       const uint32_t p = t - __shfl(excl, owner, W);
-      if (t < total) addItem<M, J>(w, acc, i, self, bpos, h, tl, p, alpha, flags, df);
+      // The first round of the first chunk is stashed for the fused dual (slot t of the group)
+      const uint32_t slot = DF && c0 == first && r0 == 0 ? t : NO_STASH;
+      if (t < total) addItem<M, J>(w, acc, i, self, bpos, h, tl, p, alpha, flags, df, slot);
     }
   }
 }
@@ -846,6 +913,20 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
     }
     const uint32_t excl = incl - items;
     const uint32_t total = __shfl(incl, W - 1, W);
+    // In the first chunk, each entry's first item number in the primal's numbering (every item, finished here or
+    // not): items the primal took in its first round are in the stash
+    const bool stashed = DF && c0 == first;  // the same for every lane of the group
+    uint32_t exclAll = 0;
+    if (stashed) {
+      const uint32_t all = e < end ? ((head.z & ENTRY_JOINT) ? 1u : (head.z & 15u)) : 0u;
+      uint32_t inclAll = all;
+#pragma unroll
+      for (uint32_t s = 1; s < W; s <<= 1) {
+        uint32_t o = __shfl_up(inclAll, s, W);
+        if (lane >= s) inclAll += o;
+      }
+      exclAll = inclAll - all;
+    }
     for (uint32_t r0 = 0; r0 < total; r0 += W) {  // the same turns for every lane of the group
       const uint32_t t = r0 + lane;
       uint32_t owner = 0;
@@ -860,6 +941,7 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
                             __uint_as_float(__shfl(tail.z, owner, W)));
       const float tscale = __uint_as_float(__shfl(tail.w, owner, W));
       const uint32_t pt = t - __shfl(excl, owner, W);
+      const uint32_t tp = stashed ? __shfl(exclAll, owner, W) + pt : W;  // the item's slot in the stash if < W
       if (J && t < total && (h.z & ENTRY_JOINT)) {
         if constexpr (J) {
           const uint32_t j = h.y;
@@ -870,15 +952,21 @@ AVBD_D void fusedDual(const DeviceWorld &w, uint4 slot, uint32_t lane, float4 po
       } else if (t < total) {
         // The partner is static or of an earlier colour: its pose of this iteration was waited for by the primal
         const bool isA = (h.z & ENTRY_IS_A) != 0;
-        const PairBody other = pairBody<M>(w, h.x);
         const uint32_t c = h.y + pt;
-        Contact kc = w.contacts[c];
-        if constexpr (DF) {
-          kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
-          kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
+        Contact kc;
+        PartnerTerms o;
+        if (DF && tp < W) {
+          unstashItem(df.stash, tp, kc, o);
+        } else {
+          kc = w.contacts[c];
+          if constexpr (DF) {
+            kc.pen = ldDev4(contactRsrc(w), contactOff(c, OFF_PEN));
+            kc.lam = ldDev4(contactRsrc(w), contactOff(c, OFF_LAM));
+          }
+          o = partnerTerms(w, kc, pairBody<M>(w, h.x), isA, stepStartArms(kc, p.flags));
         }
-        const Eval ev = evalContact(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w), self, other, isA,
-                                    p.alpha, p.flags);
+        const Eval ev = evalContactTerms(w, kc, orthonormalScaled(normal, tscale), __uint_as_float(h.w), self, o, isA,
+                                         stepStartArms(kc, p.flags), p.alpha);
         dualUpdate<M>(w, c, kc, ev, p);
       }
     }
@@ -1015,11 +1103,13 @@ __global__ void k_velocities(DeviceWorld w) {
  * grid is sized by the occupancy), so the unit everything else waits for always runs; and every wait is bounded.
  */
 template <bool J> __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dataflow(DeviceWorld w) {
+  __shared__ float stash[PRIMAL_BLOCK / WAVE][STASH_WORDS * WAVE];  // each wave's contact items, primal to dual
   if (held(w)) return;
   const DataflowPlan &plan = *w.dfPlan;
   const uint32_t lane = threadIdx.x % WAVE;
   const uint32_t waves = gridDim.x * (PRIMAL_BLOCK / WAVE);
   const uint32_t total = plan.total, perIter = plan.unitsPerIter;
+  float *const waveStash = stash[threadIdx.x / WAVE];
   DfCtx df;
   df.base = plan.base;
   for (uint32_t u = blockIdx.x * (PRIMAL_BLOCK / WAVE) + threadIdx.x / WAVE; u < total; u += waves) {
@@ -1032,12 +1122,15 @@ template <bool J> __global__ void __launch_bounds__(PRIMAL_BLOCK, 4) k_solve_dat
     const uint32_t heavyStart = w.colorStart[2 * c], lightStart = w.colorStart[2 * c + 1],
                    lightEnd = w.colorStart[2 * c + 2];
     const uint32_t hw = plan.heavyWaves[c];
-    if (v < hw)
+    if (v < hw) {
+      df.stash = waveStash + (lane - lane % HEAVY_LANES);
       primalGroup<HEAVY_LANES, SOLVE_DATAFLOW, J>(w, heavyStart + v * (WAVE / HEAVY_LANES) + lane / HEAVY_LANES,
                                                   lightStart, lane % HEAVY_LANES, df);
-    else
+    } else {
+      df.stash = waveStash + (lane - lane % LANES);
       primalGroup<LANES, SOLVE_DATAFLOW, J>(w, lightStart + (v - hw) * (WAVE / LANES) + lane / LANES, lightEnd,
                                             lane % LANES, df);
+    }
   }
 }
 
