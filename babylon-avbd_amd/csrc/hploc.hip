@@ -70,7 +70,17 @@ __global__ void k_bvh_bounds_init(Bvh t, uint32_t nWorlds) {
   }
 }
 
-__global__ void k_bvh_bounds(DeviceWorld w, Bvh t) {
+/** Threads per block of the bounds pass. */
+constexpr uint32_t BOUNDS_BLOCK = 1024;
+
+/**
+ * Per-world bounds of the body centres (order-preserving bits, atomic min/max). A wavefront whose bodies share one
+ * world reduces in registers, and a block whose wavefronts all share one then reduces in LDS: one block of a large
+ * world makes 6 atomics instead of 6 per wavefront (the atomics on a world's six words serialise). Other wavefronts
+ * make theirs per wavefront or per body.
+ */
+__global__ void __launch_bounds__(BOUNDS_BLOCK) k_bvh_bounds(DeviceWorld w, Bvh t) {
+  __shared__ uint32_t waveWorld[BOUNDS_BLOCK / 64], waveLo[3][BOUNDS_BLOCK / 64], waveHi[3][BOUNDS_BLOCK / 64];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = i < w.nBodies;
   uint32_t k = live ? w.bodyWorld[i] : 0xffffffffu;
@@ -81,24 +91,50 @@ __global__ void k_bvh_bounds(DeviceWorld w, Bvh t) {
     lo[0] = lo[1] = lo[2] = 0xffffffffu;
     hi[0] = hi[1] = hi[2] = 0u;
   }
-  // A wavefront whose bodies share one world reduces in registers and makes 6 atomics instead of 6 per body
+  const uint32_t wave = threadIdx.x / 64, waves = blockDim.x / 64;
   uint32_t k0 = __shfl(k, 0, 64);
-  bool uniform = __all(k == k0 || !live);
+  const bool uniform = __all(k == k0 || !live);
   if (uniform) {
     for (int d = 32; d > 0; d >>= 1)
       for (int a = 0; a < 3; a++) {
         lo[a] = umin(lo[a], __shfl_xor(lo[a], d, 64));
         hi[a] = umax(hi[a], __shfl_xor(hi[a], d, 64));
       }
-    if ((threadIdx.x & 63) == 0 && k0 != 0xffffffffu)
-      for (int a = 0; a < 3; a++) {
-        atomicMin(&t.bounds[6 * k0 + a], lo[a]);
-        atomicMax(&t.bounds[6 * k0 + 3 + a], hi[a]);
-      }
   } else if (live) {
     for (int a = 0; a < 3; a++) {
       atomicMin(&t.bounds[6 * k + a], lo[a]);
       atomicMax(&t.bounds[6 * k + 3 + a], hi[a]);
+    }
+  }
+  // Wavefront results to LDS (a mixed wavefront, or one without a live body, has world ~0: it is done)
+  if ((threadIdx.x & 63) == 0) {
+    waveWorld[wave] = uniform ? k0 : 0xffffffffu;
+    for (int a = 0; a < 3; a++) {
+      waveLo[a][wave] = lo[a];
+      waveHi[a][wave] = hi[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x >= waves) return;
+  const uint32_t wk = waveWorld[threadIdx.x];
+  // The block's first lanes, one per wavefront: do all uniform wavefronts share one world?
+  bool same = true;
+  for (uint32_t v = 0; v < waves; v++) same = same && waveWorld[v] == waveWorld[0];
+  if (same) {
+    if (threadIdx.x != 0 || wk == 0xffffffffu) return;
+    for (int a = 0; a < 3; a++) {
+      uint32_t l = waveLo[a][0], h = waveHi[a][0];
+      for (uint32_t v = 1; v < waves; v++) {
+        l = umin(l, waveLo[a][v]);
+        h = umax(h, waveHi[a][v]);
+      }
+      atomicMin(&t.bounds[6 * wk + a], l);
+      atomicMax(&t.bounds[6 * wk + 3 + a], h);
+    }
+  } else if (wk != 0xffffffffu) {
+    for (int a = 0; a < 3; a++) {
+      atomicMin(&t.bounds[6 * wk + a], waveLo[a][threadIdx.x]);
+      atomicMax(&t.bounds[6 * wk + 3 + a], waveHi[a][threadIdx.x]);
     }
   }
 }
@@ -584,7 +620,7 @@ void launch_bvh_build(const DeviceWorld &w, const Bvh &tree, const BvhPlan &plan
   launch_fill_u32(t.parent, END, plan.nodeCount, s);
   k_bvh_leaves<<<blocks(n, 256), 256, 0, s>>>(w, t);
   k_bvh_bounds_init<<<blocks(w.nWorlds, 256), 256, 0, s>>>(t, w.nWorlds);
-  k_bvh_bounds<<<blocks(n, 256), 256, 0, s>>>(w, t);
+  k_bvh_bounds<<<blocks(n, BOUNDS_BLOCK), BOUNDS_BLOCK, 0, s>>>(w, t);
   k_bvh_morton<<<blocks(n, 256), 256, 0, s>>>(w, t);
   // Stable LSD radix sort of (key, body)
   const uint32_t tiles = blocks(n, SORT_TILE);
