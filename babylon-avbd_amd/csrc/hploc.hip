@@ -319,8 +319,8 @@ constexpr uint32_t LEAF = 0x80000000u;
 
 /**
  * Refit one level, a 16-lane group per treelet (four a block): the lanes read their nodes' child ids and the boxes and
- * maxPos of children outside the treelet in parallel; one lane then refits the nodes in creation order (children
- * before parents) in LDS; the lanes write the nodes, their maxPos and their pair records.
+ * maxPos of children outside the treelet in parallel; the lanes then refit the nodes in LDS in rounds (a node once its
+ * children are); the lanes write the nodes, their maxPos and their pair records.
  */
 __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup *groups, uint32_t nGroups,
                                                            uint32_t nLeaves) {
@@ -329,6 +329,7 @@ __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup
   __shared__ uint32_t nodeMp[64];
   __shared__ float4 childLo[2][64], childHi[2][64];  // per node slot: its two children's boxes and maxPos
   __shared__ uint32_t childMp[2][64], childId[2][64];
+  __shared__ uint32_t refitted[64];                  // per node slot: refitted in an earlier round
   const uint32_t lane = threadIdx.x % WIDTH, g = threadIdx.x / WIDTH, slot = threadIdx.x;
   const uint32_t gi = blockIdx.x * 4 + g;
   const bool groupLive = gi < nGroups;
@@ -349,24 +350,42 @@ __global__ void __launch_bounds__(64) k_bvh_refit_treelets(Bvh t, const BvhGroup
       }
     }
   }
+  // Rounds, a lane per node: a node whose children are refitted (outside children always are) takes their boxes and
+  // is refitted; the treelet's depth in rounds instead of its node count in sequence (min and max are exact, so the
+  // order changes no bit)
+  const uint32_t base = g * WIDTH;
+  bool done = !live;
+  refitted[slot] = 0u;
   __syncthreads();
-  if (groupLive && lane == 0) {
-    const uint32_t base = g * WIDTH;
-    for (uint32_t j = 0; j < nodes; j++) {
-      const uint32_t sj = base + j;
+  for (uint32_t round = 0; round < WIDTH; round++) {
+    bool ready = !done;
+    uint32_t in[2] = {0u, 0u};
+    if (!done) {
 #pragma unroll
       for (int k = 0; k < 2; k++) {
-        const uint32_t in = childId[k][sj] - grp.nodeBase;
-        if (in < nodes) {  // an earlier node of the treelet
-          childLo[k][sj] = nodeLo[base + in];
-          childHi[k][sj] = nodeHi[base + in];
-          childMp[k][sj] = nodeMp[base + in];
+        in[k] = childId[k][slot] - grp.nodeBase;
+        if (in[k] < nodes && !refitted[base + in[k]]) ready = false;
+      }
+    }
+    if (ready) {
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (in[k] < nodes) {  // a node of the treelet (refitted in an earlier round)
+          childLo[k][slot] = nodeLo[base + in[k]];
+          childHi[k][slot] = nodeHi[base + in[k]];
+          childMp[k][slot] = nodeMp[base + in[k]];
         }
       }
-      nodeLo[sj] = mk4(fmin3(xyz(childLo[0][sj]), xyz(childLo[1][sj])), 0.f);
-      nodeHi[sj] = mk4(fmax3(xyz(childHi[0][sj]), xyz(childHi[1][sj])), 0.f);
-      nodeMp[sj] = combineMaxPos(childMp[0][sj], childMp[1][sj]);
     }
+    __syncthreads();  // every read of this round before any write
+    if (ready) {
+      nodeLo[slot] = mk4(fmin3(xyz(childLo[0][slot]), xyz(childLo[1][slot])), 0.f);
+      nodeHi[slot] = mk4(fmax3(xyz(childHi[0][slot]), xyz(childHi[1][slot])), 0.f);
+      nodeMp[slot] = combineMaxPos(childMp[0][slot], childMp[1][slot]);
+      refitted[slot] = 1u;
+      done = true;
+    }
+    if (!__syncthreads_or(!done)) break;  // also orders this round's writes before the next round's reads
   }
   __syncthreads();
   if (live) {
