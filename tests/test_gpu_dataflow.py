@@ -60,6 +60,36 @@ def test_same_bits_as_the_colour_launches(avbd, graph):
     b.destroy()
 
 
+def test_heavy_body_same_bits_as_the_colour_launches(avbd):
+    """A 12 m plate resting on a 24 x 24 carpet of small boxes: the plate's adjacency list runs to hundreds of entries
+    (a heavy body, many chunks and item rounds, its fused dual reading its first round from the stash and the rest from
+    memory), each carpet box has the plate and a few neighbours. 40 steps bit-identical to the colour launches, and the
+    oracle's step from the GPU's state."""
+    def scene():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        s.addBox([30, 1, 30], density=0, position=[0, -0.5, 0])
+        for x in range(24):
+            for z in range(24):
+                s.addBox([0.4, 0.4, 0.4], position=[-5.75 + 0.5 * x, 0.2, -5.75 + 0.5 * z])
+        s.addBox([12, 0.4, 12], position=[0, 0.595, 0])
+        return s
+    a = _world(avbd, scene(), solver="colors")
+    b = _world(avbd, scene(), solver="dataflow")
+    assert b.solverKind() == 2
+    for _ in range(4):
+        a.step(10)
+        b.step(10)
+        assert np.array_equal(a.readBodies().view(np.uint32), b.readBodies().view(np.uint32))
+        assert b.readCounters()["solveAborts"] == 0
+    oracle = OracleWorld([scene()], precision=32, color_cap=32, color_rounds=16)
+    oracle.seed(b.readState())
+    b.step()
+    oracle.step()
+    compare_step(b, oracle)
+    a.destroy()
+    b.destroy()
+
+
 def test_batch_worlds_match_their_solo_runs(avbd):
     scenes = [avbd.configs.box_pile(counts=(4, 4, 4), seed=s) for s in (1, 2, 3)]
     batch = avbd.createSolverPipeline("hip").createBatch(scenes, dict(colorCap=32, colorRounds=16, solver="dataflow"))
