@@ -44,6 +44,27 @@ def test_single_world_same_bits_as_the_colour_launches(avbd, block):
     b.destroy()
 
 
+def test_heavy_body_same_bits_as_the_colour_launches(avbd):
+    """A 12 m plate on a 24 x 24 carpet of small boxes (the plate a heavy body with hundreds of entries) in one
+    workgroup: 40 steps bit-identical to the colour launches."""
+    def scene():
+        s = avbd.AvbdScene3D(gravity=-9.81)
+        s.addBox([30, 1, 30], density=0, position=[0, -0.5, 0])
+        for x in range(24):
+            for z in range(24):
+                s.addBox([0.4, 0.4, 0.4], position=[-5.75 + 0.5 * x, 0.2, -5.75 + 0.5 * z])
+        s.addBox([12, 0.4, 12], position=[0, 0.595, 0])
+        return s
+    a = _solver(avbd, scene(), "colors")
+    b = _solver(avbd, scene(), "worlds", worldBlock=1024)
+    for _ in range(4):
+        a.step(10)
+        b.step(10)
+        assert np.array_equal(a.readBodies().view(np.uint32), b.readBodies().view(np.uint32))
+    a.destroy()
+    b.destroy()
+
+
 def test_single_world_steps_match_the_oracle(avbd):
     scene = avbd.configs.pile_1000
     world = _solver(avbd, scene(), "worlds")
