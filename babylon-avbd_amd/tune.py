@@ -39,17 +39,18 @@ def run_groups(worlds, steps, concurrency=1):
     return ms, time.perf_counter() - t0
 
 
-#: Candidate (batches, concurrency, solve options) for the batched workload: colour launches at several splits,
-#: the dataflow solve one batch at a time or several at once (each grid a matching share of the device), one workgroup
-#: per world at two workgroup sizes.
+#: Candidate (batches, concurrency, solve options) for the batched workload: colour launches at several splits and
+#: concurrencies (batches a multiple of the concurrency, so every round is full), the dataflow solve one batch at a
+#: time or two at once (each grid half the device), one workgroup per world at two workgroup sizes.
 CANDIDATES = (
     (16, 2, {"solver": "colors"}),
+    (24, 3, {"solver": "colors"}),
+    (24, 4, {"solver": "colors"}),
+    (28, 4, {"solver": "colors"}),
     (32, 4, {"solver": "colors"}),
-    (64, 8, {"solver": "colors"}),
     (8, 1, {"solver": "dataflow"}),
     (16, 1, {"solver": "dataflow"}),
     (16, 2, {"solver": "dataflow", "dataflowShare": 2}),
-    (32, 4, {"solver": "dataflow", "dataflowShare": 4}),
     (32, 4, {"solver": "worlds", "worldBlock": 256}),
     (32, 4, {"solver": "worlds", "worldBlock": 1024}),
 )
